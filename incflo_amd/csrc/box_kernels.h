@@ -1,0 +1,267 @@
+// box_kernels.h -- kernels that move data between the caller's boxed fields
+// (AMReX Array4 layout, boxes back to back in one allocation) and the
+// solver's canonical level arrays, driven by a device box-offset table and a
+// tile table (one 256-cell chunk of one box per workgroup), so that all boxes
+// of uneven sizes are handled by a single launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nodal_kernels.h"
+
+namespace mgx {
+
+struct BoxInfo {
+    int lo[3];
+    int len[3];      // valid cells per direction
+    int rlo[3];      // iteration region lo / extent for "grown" cell loops
+    int rlen[3];
+    long pc[4];      // prefix sums over earlier boxes for cell-field offsets
+    long pn[4];      // ... for nodal-field offsets
+};
+
+struct Tile {
+    int box;
+    int start;
+};
+
+enum { REG_CELL = 0, REG_CELL_GROWN = 1, REG_NODE = 2 };
+
+__device__ __forceinline__ long field_offset(const BoxInfo& b, int ng, int ncomp, bool nodal) {
+    const long* P = nodal ? b.pn : b.pc;
+    const long g = ng;
+    return (long)ncomp * (P[0] + 2 * g * P[1] + 4 * g * g * P[2] + 8 * g * g * g * P[3]);
+}
+
+// index of (i,j,k,comp) inside box b of a field with ghost width ng
+__device__ __forceinline__ long box_index(const BoxInfo& b, int ng, bool nodal, int i, int j, int k,
+                                          int comp) {
+    const int e = nodal ? 1 : 0;
+    const long D0 = b.len[0] + e + 2 * ng, D1 = b.len[1] + e + 2 * ng, D2 = b.len[2] + e + 2 * ng;
+    return (i - b.lo[0] + ng) + D0 * ((j - b.lo[1] + ng) + D1 * ((k - b.lo[2] + ng) + D2 * comp));
+}
+
+// Decode the (box, i, j, k) handled by this thread; false when past the region.
+__device__ __forceinline__ bool tile_point(const BoxInfo* boxes, const Tile* tiles, int region,
+                                           int& bidx, int& i, int& j, int& k) {
+    const Tile t = tiles[blockIdx.x];
+    bidx = t.box;
+    const BoxInfo& b = boxes[bidx];
+    int lo[3], len[3];
+    for (int d = 0; d < 3; ++d) {
+        if (region == REG_CELL_GROWN) { lo[d] = b.rlo[d]; len[d] = b.rlen[d]; }
+        else { lo[d] = b.lo[d]; len[d] = b.len[d] + (region == REG_NODE ? 1 : 0); }
+    }
+    const long lin = (long)t.start + threadIdx.x;
+    const long vol = (long)len[0] * len[1] * len[2];
+    if (lin >= vol) return false;
+    i = lo[0] + (int)(lin % len[0]);
+    j = lo[1] + (int)((lin / len[0]) % len[1]);
+    k = lo[2] + (int)(lin / ((long)len[0] * len[1]));
+    return true;
+}
+
+struct GatherVelArgs {
+    KGeom g;
+    const BoxInfo* boxes;
+    const Tile* tiles;
+    const double* vel; int vel_ng;
+    const double* vel_old; int vold_ng;
+    const double* gp; int gp_ng;
+    const double* rho; int rho_ng;
+    double const_rho;  // > 0: sigma = scale / const_rho
+    double scale;      // scaling_factor s
+    int sigma_from_rho;
+    double* ug;        // canonical ghosted cell velocity (3 comps)
+    double* sig;       // canonical ghosted sigma (level 0)
+    int ghost_rule;    // 0: take domain-face ghosts from vel; 1: 0 / inflow table
+    double inflow[18];
+    int bclo[3], bchi[3];
+};
+
+// u* = u (+ (s/rho) gp) (- u_old) on valid cells; sigma = s/rho; ghost cells
+// outside non-periodic faces from the caller or the inflow rule.
+static __global__ void __launch_bounds__(256) gather_vel_kernel(GatherVelArgs a) {
+    int bi, i, j, k;
+    if (!tile_point(a.boxes, a.tiles, REG_CELL_GROWN, bi, i, j, k)) return;
+    const BoxInfo& b = a.boxes[bi];
+    const KGeom& g = a.g;
+    const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
+    const bool inside = i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2];
+    const long boff_v = field_offset(b, a.vel_ng, 3, false);
+    if (inside) {
+        double s_rho = 0.0, sigma = 0.0;
+        if (a.rho) {
+            const double r = a.rho[field_offset(b, a.rho_ng, 1, false) +
+                                   box_index(b, a.rho_ng, false, i, j, k, 0)];
+            s_rho = a.scale / r;
+        } else {
+            s_rho = a.scale / a.const_rho;
+        }
+        sigma = (a.const_rho > 0.0) ? a.scale / a.const_rho : s_rho;
+        for (int c = 0; c < 3; ++c) {
+            double u = a.vel[boff_v + box_index(b, a.vel_ng, false, i, j, k, c)];
+            if (a.gp) u += s_rho * a.gp[field_offset(b, a.gp_ng, 3, false) +
+                                        box_index(b, a.gp_ng, false, i, j, k, c)];
+            if (a.vel_old) u -= a.vel_old[field_offset(b, a.vold_ng, 3, false) +
+                                          box_index(b, a.vold_ng, false, i, j, k, c)];
+            write_cell_images(a.ug + c * ug3, g, i, j, k, u);
+        }
+        if (a.sigma_from_rho) write_cell_images(a.sig, g, i, j, k, sigma);
+    } else {
+        // ghost cell outside at least one non-periodic face
+        int nonper = 0, face = -1;
+        const int c3[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d)
+            if (c3[d] < 0 || c3[d] >= g.n[d]) { ++nonper; face = 2 * d + (c3[d] < 0 ? 0 : 1); }
+        for (int c = 0; c < 3; ++c) {
+            double u;
+            if (!a.ghost_rule) {
+                u = a.vel[boff_v + box_index(b, a.vel_ng, false, i, j, k, c)];
+            } else if (nonper == 1) {
+                const int d = face >> 1;
+                const int bc = (face & 1) ? a.bchi[d] : a.bclo[d];
+                u = (bc == MGX_BC_INFLOW) ? a.inflow[face * 3 + c] : 0.0;
+            } else {
+                u = 0.0;
+            }
+            // periodic images in the other (periodic) directions
+            int xs[2] = {i + 1, -1}, ys[2] = {j + 1, -1}, zs[2] = {k + 1, -1};
+            if (g.per[0]) xs[1] = (i == 0) ? g.n[0] + 1 : (i == g.n[0] - 1 ? 0 : -1);
+            if (g.per[1]) ys[1] = (j == 0) ? g.n[1] + 1 : (j == g.n[1] - 1 ? 0 : -1);
+            if (g.per[2]) zs[1] = (k == 0) ? g.n[2] + 1 : (k == g.n[2] - 1 ? 0 : -1);
+            for (int q = 0; q < 8; ++q) {
+                const int X = xs[q & 1], Y = ys[(q >> 1) & 1], Z = zs[(q >> 2) & 1];
+                if (X < 0 || Y < 0 || Z < 0) continue;
+                a.ug[c * ug3 + X + (long)g.sg[0] * (Y + (long)g.sg[1] * Z)] = u;
+            }
+        }
+    }
+}
+
+// (D u)_n = sum_d 1/(4 h_d) [ sum_{+d cells} u_d - sum_{-d cells} u_d ]
+static __global__ void __launch_bounds__(256) nodal_rhs_kernel(KGeom g, const double* __restrict__ ug,
+                                                        double* __restrict__ rhs, double fx,
+                                                        double fy, double fz) {
+    const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int i = id % g.nn[0];
+    const int j = (id / g.nn[0]) % g.nn[1];
+    const int k = id / ((long)g.nn[0] * g.nn[1]);
+    const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int aa = q & 1, bb = (q >> 1) & 1, cc = (q >> 2) & 1;
+        const long gi = (i + aa) + (long)g.sg[0] * ((j + bb) + (long)g.sg[1] * (k + cc));
+        const double ux = ug[gi], uy = ug[ug3 + gi], uz = ug[2 * ug3 + gi];
+        sx += aa ? ux : -ux;
+        sy += bb ? uy : -uy;
+        sz += cc ? uz : -uz;
+    }
+    rhs[id] = sx * fx + sy * fy + sz * fz;
+}
+
+// sigma from a boxed cell field
+static __global__ void __launch_bounds__(256) gather_sigma_kernel(KGeom g, const BoxInfo* boxes,
+                                                           const Tile* tiles, const double* sigma,
+                                                           int ng, double* sig) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_CELL, bi, i, j, k)) return;
+    const BoxInfo& b = boxes[bi];
+    write_cell_images(sig, g, i, j, k,
+                      sigma[field_offset(b, ng, 1, false) + box_index(b, ng, false, i, j, k, 0)]);
+}
+
+__device__ __forceinline__ void cell_gradient(const KGeom& g, const double* phi, int i, int j,
+                                              int k, const double f[3], double gr[3]) {
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int aa = q & 1, bb = (q >> 1) & 1, cc = (q >> 2) & 1;
+        int ii = i + aa, jj = j + bb, kk = k + cc;
+        if (ii >= g.nn[0]) ii -= g.nn[0];
+        if (jj >= g.nn[1]) jj -= g.nn[1];
+        if (kk >= g.nn[2]) kk -= g.nn[2];
+        const double v = phi[ii + (long)g.nn[0] * (jj + (long)g.nn[1] * kk)];
+        gx += aa ? v : -v;
+        gy += bb ? v : -v;
+        gz += cc ? v : -v;
+    }
+    gr[0] = gx * f[0];
+    gr[1] = gy * f[1];
+    gr[2] = gz * f[2];
+}
+
+struct UpdateArgs {
+    KGeom g;
+    const BoxInfo* boxes;
+    const Tile* tiles;
+    const double* phi;     // canonical nodes
+    const double* ug;      // canonical u* (NULL: read vel instead)
+    const double* sig;     // canonical ghosted sigma
+    double f[3];           // 1/(4 h_d)
+    double* vel; int vel_ng;          // u_new written here (NULL: skip)
+    const double* vel_old; int vold_ng;
+    double* gp; int gp_ng; int gp_add;   // G(phi) written / added (NULL: skip)
+};
+
+// u_new = u* - sigma G(phi) (+ u_old); gp = G(phi) or gp += G(phi)
+static __global__ void __launch_bounds__(256) update_cells_kernel(UpdateArgs a) {
+    int bi, i, j, k;
+    if (!tile_point(a.boxes, a.tiles, REG_CELL, bi, i, j, k)) return;
+    const BoxInfo& b = a.boxes[bi];
+    const KGeom& g = a.g;
+    double gr[3];
+    cell_gradient(g, a.phi, i, j, k, a.f, gr);
+    if (a.vel) {
+        const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
+        const long gi = (i + 1) + (long)g.sg[0] * ((j + 1) + (long)g.sg[1] * (k + 1));
+        const double sigma = a.sig[gi];
+        const long off = field_offset(b, a.vel_ng, 3, false);
+        for (int c = 0; c < 3; ++c) {
+            const long vi = off + box_index(b, a.vel_ng, false, i, j, k, c);
+            double u = (a.ug ? a.ug[c * ug3 + gi] : a.vel[vi]) - sigma * gr[c];
+            if (a.vel_old) u += a.vel_old[field_offset(b, a.vold_ng, 3, false) +
+                                          box_index(b, a.vold_ng, false, i, j, k, c)];
+            a.vel[vi] = u;
+        }
+    }
+    if (a.gp) {
+        const long off = field_offset(b, a.gp_ng, 3, false);
+        for (int c = 0; c < 3; ++c) {
+            const long gi = off + box_index(b, a.gp_ng, false, i, j, k, c);
+            a.gp[gi] = a.gp_add ? a.gp[gi] + gr[c] : gr[c];
+        }
+    }
+}
+
+// boxed nodal field <- canonical nodes (periodic node n == node 0)
+static __global__ void __launch_bounds__(256) scatter_nodes_kernel(KGeom g, const BoxInfo* boxes,
+                                                            const Tile* tiles, const double* phi,
+                                                            double* out, int ng, int add) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_NODE, bi, i, j, k)) return;
+    const BoxInfo& b = boxes[bi];
+    const int ii = (i >= g.nn[0]) ? i - g.nn[0] : i;
+    const int jj = (j >= g.nn[1]) ? j - g.nn[1] : j;
+    const int kk = (k >= g.nn[2]) ? k - g.nn[2] : k;
+    const double v = phi[ii + (long)g.nn[0] * (jj + (long)g.nn[1] * kk)];
+    const long o = field_offset(b, ng, 1, true) + box_index(b, ng, true, i, j, k, 0);
+    out[o] = add ? out[o] + v : v;
+}
+
+// canonical nodes <- boxed nodal field
+static __global__ void __launch_bounds__(256) gather_nodes_kernel(KGeom g, const BoxInfo* boxes,
+                                                           const Tile* tiles, const double* in,
+                                                           int ng, double* phi) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_NODE, bi, i, j, k)) return;
+    if (i >= g.nn[0] || j >= g.nn[1] || k >= g.nn[2]) return;  // periodic duplicate
+    const BoxInfo& b = boxes[bi];
+    phi[i + (long)g.nn[0] * (j + (long)g.nn[1] * k)] =
+        in[field_offset(b, ng, 1, true) + box_index(b, ng, true, i, j, k, 0)];
+}
+
+}  // namespace mgx

@@ -1,0 +1,493 @@
+// mgx_api.hip -- the C ABI (include/mgx.h): context, device-memory plumbing,
+// and the nodal projection entry points.
+#include <cstring>
+#include <memory>
+
+#include "mgx_ctx.h"
+#include "nodal_mg.h"
+
+using namespace mgx;
+
+namespace {
+
+__global__ void fill_sigma_const_kernel(KGeom g, double v, double* sig) {
+    const long n = (long)g.n[0] * g.n[1] * g.n[2];
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int i = id % g.n[0];
+    const int j = (id / g.n[0]) % g.n[1];
+    const int k = id / ((long)g.n[0] * g.n[1]);
+    write_cell_images(sig, g, i, j, k, v);
+}
+
+}  // namespace
+
+struct mgx_nodal_s {
+    mgx_ctx_s* ctx = nullptr;
+    int n_cell[3] = {0, 0, 0};
+    double prob_lo[3] = {0, 0, 0}, prob_hi[3] = {1, 1, 1};
+    int periodic[3] = {0, 0, 0};
+    std::vector<int> boxes;
+    int bclo[3] = {0, 0, 0}, bchi[3] = {0, 0, 0};
+    bool bc_set = false;
+    MGOpts opts;
+    bool dirty = true;
+    NodalGeom g{};
+    NodalSolver solver;
+    BoxLayout layout;
+    DevBuf ug, rhs;
+    bool have_phi = false;
+
+    void ensure_setup() {
+        if (!dirty) return;
+        for (int d = 0; d < 3; ++d)
+            MGX_REQUIRE(periodic[d] || bc_set, MGX_ERR_STATE,
+                        "nodal projector: setDomainBC is required for non-periodic domains");
+        double dx[3];
+        for (int d = 0; d < 3; ++d) dx[d] = (prob_hi[d] - prob_lo[d]) / n_cell[d];
+        g = make_nodal_geom(n_cell, periodic, bclo, bchi, dx);
+        solver.setup(g, opts);
+        layout.build(boxes.data(), (int)boxes.size() / 6, g);
+        ug.alloc(3 * g.nsig() * sizeof(double));
+        MGX_HIP(hipMemset(ug.p, 0, 3 * g.nsig() * sizeof(double)));
+        rhs.alloc(g.nnode() * sizeof(double));
+        have_phi = false;
+        dirty = false;
+    }
+    hipStream_t stream() const { return ctx->stream; }
+    void compute_rhs() {
+        const KGeom kg = kgeom(g);
+        nodal_rhs_kernel<<<grid1d(g.nnode()), 256, 0, stream()>>>(
+            kg, ug.as<double>(), rhs.as<double>(), 0.25 / g.dx[0], 0.25 / g.dx[1], 0.25 / g.dx[2]);
+    }
+    void zero_phi() {
+        MGX_HIP(hipMemsetAsync(solver.phi, 0, g.nnode() * sizeof(double), stream()));
+    }
+    UpdateArgs update_args() {
+        UpdateArgs u{};
+        u.g = kgeom(g);
+        u.boxes = layout.dev_boxes();
+        u.tiles = layout.tiles(REG_CELL);
+        u.phi = solver.phi;
+        u.sig = solver.sigma0();
+        for (int d = 0; d < 3; ++d) u.f[d] = 0.25 / g.dx[d];
+        return u;
+    }
+};
+
+static void check_mode(int mode) {
+    MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED || mode == MGX_MODE_LEARNED,
+                MGX_ERR_ARG, "unknown solve mode " + std::to_string(mode));
+}
+
+static void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
+    MGX_REQUIRE(o->maxiter >= 1 && o->bottom_maxiter >= 1, MGX_ERR_ARG, "maxiter must be >= 1");
+    MGX_REQUIRE(o->nu1 >= 0 && o->nu2 >= 0 && o->nu1 + o->nu2 >= 1, MGX_ERR_ARG,
+                "smoothing sweeps must be non-negative with nu1+nu2 >= 1");
+    MGX_REQUIRE(o->omega > 0.0 && o->omega < 2.0, MGX_ERR_ARG, "omega must lie in (0,2)");
+    m.verbose = o->verbose;
+    m.maxiter = o->maxiter;
+    m.bottom_maxiter = o->bottom_maxiter;
+    m.bottom_rtol = o->bottom_rtol;
+    m.max_coarsening_level = o->max_coarsening_level < 0 ? 0 : o->max_coarsening_level;
+    m.nu1 = o->nu1;
+    m.nu2 = o->nu2;
+    m.omega = o->omega;
+    m.stall_iters = o->stall_iters;
+}
+
+extern "C" {
+
+MGX_API const char* mgx_version(void) { return "mgx 0.1 (gfx950)"; }
+
+MGX_API void mgx_default_mlmg_opts(mgx_mlmg_opts* o) {
+    if (!o) return;
+    MGOpts m;
+    o->verbose = 0;
+    o->bottom_verbose = 0;
+    o->maxiter = m.maxiter;
+    o->bottom_maxiter = m.bottom_maxiter;
+    o->bottom_rtol = m.bottom_rtol;
+    o->bottom_atol = -1.0;
+    o->max_coarsening_level = m.max_coarsening_level;
+    o->nu1 = m.nu1;
+    o->nu2 = m.nu2;
+    o->omega = m.omega;
+    o->stall_iters = m.stall_iters;
+}
+
+MGX_API int mgx_ctx_create(int device, void* stream, mgx_ctx* out) {
+    if (!out) return MGX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return MGX_ERR_HIP;
+    auto* c = new mgx_ctx_s();
+    c->device = device;
+    c->stream = (hipStream_t)stream;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return MGX_ERR_HIP; }
+    *out = c;
+    return MGX_OK;
+}
+
+MGX_API int mgx_ctx_destroy(mgx_ctx ctx) {
+    if (!ctx) return MGX_OK;
+    ctx->make_current();
+    if (ctx->events_made)
+        for (auto& e : ctx->events) (void)hipEventDestroy(e);
+    delete ctx;
+    return MGX_OK;
+}
+
+MGX_API const char* mgx_ctx_last_error(mgx_ctx ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+MGX_API int mgx_ctx_synchronize(mgx_ctx ctx) {
+    return guarded(ctx, [&] { MGX_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+MGX_API int mgx_dev_malloc(mgx_ctx ctx, unsigned long long bytes, void** out) {
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(out != nullptr, MGX_ERR_ARG, "null output pointer");
+        MGX_HIP(hipMalloc(out, bytes ? bytes : 1));
+    });
+}
+
+MGX_API int mgx_dev_free(mgx_ctx ctx, void* p) {
+    return guarded(ctx, [&] { MGX_HIP(hipFree(p)); });
+}
+
+MGX_API int mgx_memcpy_h2d(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes) {
+    return guarded(ctx, [&] {
+        MGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        MGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+MGX_API int mgx_memcpy_d2h(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes) {
+    return guarded(ctx, [&] {
+        MGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        MGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+MGX_API int mgx_memcpy_d2d(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes) {
+    return guarded(ctx, [&] {
+        MGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+MGX_API int mgx_memset_d(mgx_ctx ctx, void* dst, int value, unsigned long long bytes) {
+    return guarded(ctx, [&] { MGX_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream)); });
+}
+
+MGX_API int mgx_ctx_record(mgx_ctx ctx, int slot) {
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(slot >= 0 && slot < 16, MGX_ERR_ARG, "event slot out of range");
+        if (!ctx->events_made) {
+            for (auto& e : ctx->events) MGX_HIP(hipEventCreate(&e));
+            ctx->events_made = true;
+        }
+        MGX_HIP(hipEventRecord(ctx->events[slot], ctx->stream));
+    });
+}
+
+MGX_API int mgx_ctx_elapsed_ms(mgx_ctx ctx, int a, int b, double* ms) {
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(ctx->events_made && a >= 0 && a < 16 && b >= 0 && b < 16 && ms, MGX_ERR_ARG,
+                    "bad event slots");
+        MGX_HIP(hipEventSynchronize(ctx->events[b]));
+        float f = 0.f;
+        MGX_HIP(hipEventElapsedTime(&f, ctx->events[a], ctx->events[b]));
+        *ms = f;
+    });
+}
+
+// ---- nodal ---------------------------------------------------------------
+
+MGX_API int mgx_nodal_create(mgx_ctx ctx, const mgx_level_desc* lv, mgx_nodal* out) {
+    if (!ctx || !out) return MGX_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(lv != nullptr && lv->boxes != nullptr && lv->nboxes > 0, MGX_ERR_ARG,
+                    "level description needs a box list");
+        auto h = std::make_unique<mgx_nodal_s>();
+        h->ctx = ctx;
+        for (int d = 0; d < 3; ++d) {
+            MGX_REQUIRE(lv->n_cell[d] >= 1, MGX_ERR_ARG, "n_cell must be positive");
+            MGX_REQUIRE(lv->prob_hi[d] > lv->prob_lo[d], MGX_ERR_ARG, "prob_hi must exceed prob_lo");
+            h->n_cell[d] = lv->n_cell[d];
+            h->prob_lo[d] = lv->prob_lo[d];
+            h->prob_hi[d] = lv->prob_hi[d];
+            h->periodic[d] = lv->is_periodic[d] ? 1 : 0;
+            h->bclo[d] = h->bchi[d] = MGX_BC_PERIODIC;
+        }
+        h->boxes.assign(lv->boxes, lv->boxes + 6 * lv->nboxes);
+        // validate the box list now (errors at construction, like BoxArray checks)
+        {
+            double dx[3];
+            for (int d = 0; d < 3; ++d) dx[d] = (h->prob_hi[d] - h->prob_lo[d]) / h->n_cell[d];
+            NodalGeom g = make_nodal_geom(h->n_cell, h->periodic, h->bclo, h->bchi, dx);
+            BoxLayout probe;
+            probe.build(h->boxes.data(), lv->nboxes, g);
+        }
+        *out = h.release();
+    });
+}
+
+MGX_API int mgx_nodal_destroy(mgx_nodal h) {
+    if (!h) return MGX_OK;
+    h->ctx->make_current();
+    delete h;
+    return MGX_OK;
+}
+
+MGX_API int mgx_nodal_set_domain_bc(mgx_nodal h, const int lo[3], const int hi[3]) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(lo && hi, MGX_ERR_ARG, "null bc arrays");
+        for (int d = 0; d < 3; ++d) {
+            for (int side = 0; side < 2; ++side) {
+                const int bc = side ? hi[d] : lo[d];
+                MGX_REQUIRE(bc >= MGX_BC_PERIODIC && bc <= MGX_BC_ROBIN, MGX_ERR_ARG,
+                            "nodal projector: undefined BC type");
+                MGX_REQUIRE((bc == MGX_BC_PERIODIC) == (h->periodic[d] != 0), MGX_ERR_ARG,
+                            "nodal projector: periodic BC must match the geometry's periodicity");
+                MGX_REQUIRE(bc != MGX_BC_ROBIN, MGX_ERR_UNSUPPORTED,
+                            "nodal projector: Robin is not a nodal-projection BC");
+            }
+            h->bclo[d] = lo[d];
+            h->bchi[d] = hi[d];
+        }
+        h->bc_set = true;
+        h->dirty = true;
+    });
+}
+
+MGX_API int mgx_nodal_set_options(mgx_nodal h, const mgx_mlmg_opts* o) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(o != nullptr, MGX_ERR_ARG, "null options");
+        MGOpts m;
+        to_opts(o, m);
+        const bool relevel = m.max_coarsening_level != h->opts.max_coarsening_level;
+        h->opts = m;
+        if (relevel) h->dirty = true;
+        else h->solver.opts = m;
+    });
+}
+
+MGX_API int mgx_nodal_set_timing(mgx_nodal h, int on) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        h->solver.fine_timer.on = on != 0;
+        h->solver.fine_timer.reset();
+    });
+}
+
+MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long long* count) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(which == 0 && total_ms && count, MGX_ERR_ARG, "bad timing query");
+        *total_ms = h->solver.fine_timer.total_ms;
+        *count = h->solver.fine_timer.count;
+    });
+}
+
+MGX_API int mgx_nodal_set_sigma(mgx_nodal h, const double* sigma, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(sigma != nullptr && ng >= 0, MGX_ERR_ARG, "bad sigma field");
+        h->ensure_setup();
+        gather_sigma_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
+            kgeom(h->g), h->layout.dev_boxes(), h->layout.tiles(REG_CELL), sigma, ng,
+            h->solver.sigma0());
+        h->solver.sigma_changed();
+    });
+}
+
+MGX_API int mgx_nodal_set_const_sigma(mgx_nodal h, double v) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(v > 0.0, MGX_ERR_ARG, "sigma must be positive");
+        h->ensure_setup();
+        const long nc = (long)h->g.n[0] * h->g.n[1] * h->g.n[2];
+        fill_sigma_const_kernel<<<grid1d(nc), 256, 0, h->stream()>>>(kgeom(h->g), v,
+                                                                     h->solver.sigma0());
+        h->solver.sigma_changed();
+    });
+}
+
+MGX_API int mgx_nodal_project(mgx_nodal h, double* vel, int vel_ng, double rtol, double atol,
+                              int mode, mgx_solve_info* info) {
+    if (!h) return MGX_ERR_ARG;
+    int status = guarded(h->ctx, [&] {
+        check_mode(mode);
+        MGX_REQUIRE(vel != nullptr, MGX_ERR_ARG, "null velocity");
+        h->ensure_setup();
+        const bool any_nonper = !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
+        MGX_REQUIRE(vel_ng >= (any_nonper ? 1 : 0), MGX_ERR_ARG,
+                    "velocity needs one ghost cell at non-periodic domain faces");
+        MGX_REQUIRE(h->solver.sigma_set, MGX_ERR_STATE, "nodal projector: sigma not set");
+        GatherVelArgs a{};
+        a.g = kgeom(h->g);
+        a.boxes = h->layout.dev_boxes();
+        a.tiles = h->layout.tiles(REG_CELL_GROWN);
+        a.vel = vel;
+        a.vel_ng = vel_ng;
+        a.scale = 1.0;
+        a.const_rho = 1.0;
+        a.ug = h->ug.as<double>();
+        a.sig = h->solver.sigma0();
+        a.ghost_rule = 0;
+        gather_vel_kernel<<<h->layout.ntiles[REG_CELL_GROWN], 256, 0, h->stream()>>>(a);
+        h->compute_rhs();
+        h->zero_phi();
+        h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
+        UpdateArgs u = h->update_args();
+        u.ug = h->ug.as<double>();
+        u.vel = vel;
+        u.vel_ng = vel_ng;
+        update_cells_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(u);
+        h->have_phi = true;
+    });
+    if (status == MGX_OK && info && !info->converged) {
+        h->ctx->last_error = "nodal projector: MLMG failed to converge";
+        status = MGX_ERR_NOT_CONVERGED;
+    }
+    return status;
+}
+
+MGX_API int mgx_nodal_get_phi(mgx_nodal h, double* phi, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(h->have_phi, MGX_ERR_STATE, "getPhi before project");
+        MGX_REQUIRE(phi != nullptr && ng >= 0, MGX_ERR_ARG, "bad phi field");
+        scatter_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kgeom(h->g), h->layout.dev_boxes(), h->layout.tiles(REG_NODE), h->solver.phi, phi, ng,
+            0);
+    });
+}
+
+MGX_API int mgx_nodal_get_gradphi(mgx_nodal h, double* gphi, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(h->have_phi, MGX_ERR_STATE, "getGradPhi before project");
+        MGX_REQUIRE(gphi != nullptr && ng >= 0, MGX_ERR_ARG, "bad gradphi field");
+        UpdateArgs u = h->update_args();
+        u.gp = gphi;
+        u.gp_ng = ng;
+        update_cells_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(u);
+    });
+}
+
+MGX_API int mgx_nodal_apply(mgx_nodal h, const double* phi, int phi_ng, double* out, int out_ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(phi && out && phi_ng >= 0 && out_ng >= 0, MGX_ERR_ARG, "bad apply fields");
+        MGX_REQUIRE(h->solver.sigma_set, MGX_ERR_STATE, "nodal apply: sigma not set");
+        h->ensure_setup();
+        const KGeom kg = kgeom(h->g);
+        gather_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), phi, phi_ng, h->rhs.as<double>());
+        SArgs<double, double> a{};
+        a.g = kg;
+        a.c = make_coef<double>(h->g.dx);
+        a.x = h->rhs.as<double>();
+        a.sig = h->solver.sigma0();
+        a.rout = h->solver.phi;
+        launch_stencil<double, double, KS_APPLY>(h->g, a, h->stream(), false);
+        scatter_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), h->solver.phi, out, out_ng, 0);
+        h->have_phi = false;
+    });
+}
+
+MGX_API int mgx_nodal_solve(mgx_nodal h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
+                            double rtol, double atol, int mode, mgx_solve_info* info) {
+    if (!h) return MGX_ERR_ARG;
+    int status = guarded(h->ctx, [&] {
+        check_mode(mode);
+        MGX_REQUIRE(phi && rhs && phi_ng >= 0 && rhs_ng >= 0, MGX_ERR_ARG, "bad solve fields");
+        h->ensure_setup();
+        const KGeom kg = kgeom(h->g);
+        gather_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), rhs, rhs_ng, h->rhs.as<double>());
+        gather_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), phi, phi_ng, h->solver.phi);
+        h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
+        scatter_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), h->solver.phi, phi, phi_ng, 0);
+        h->have_phi = true;
+    });
+    if (status == MGX_OK && info && !info->converged) {
+        h->ctx->last_error = "nodal MLMG failed to converge";
+        status = MGX_ERR_NOT_CONVERGED;
+    }
+    return status;
+}
+
+MGX_API int mgx_apply_nodal_projection(mgx_nodal h, const double* density, int rho_ng,
+                                       double const_rho, double* vel, int vel_ng,
+                                       const double* vel_old, int vel_old_ng, double* gp,
+                                       int gp_ng, double* p_nd, int p_ng, double scaling_factor,
+                                       int incremental, const double* inflow_vel, double rtol,
+                                       double atol, int mode, mgx_solve_info* info) {
+    if (!h) return MGX_ERR_ARG;
+    int status = guarded(h->ctx, [&] {
+        check_mode(mode);
+        MGX_REQUIRE(vel && gp && p_nd, MGX_ERR_ARG, "velocity, gp and p_nd are required");
+        MGX_REQUIRE(density != nullptr || const_rho > 0.0, MGX_ERR_ARG,
+                    "need a density field or a positive constant density");
+        MGX_REQUIRE(scaling_factor > 0.0, MGX_ERR_ARG, "scaling factor must be positive");
+        h->ensure_setup();
+        const bool any_nonper = !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
+        MGX_REQUIRE(vel_ng >= (any_nonper ? 1 : 0), MGX_ERR_ARG,
+                    "velocity needs one ghost cell at non-periodic domain faces");
+        GatherVelArgs a{};
+        a.g = kgeom(h->g);
+        a.boxes = h->layout.dev_boxes();
+        a.tiles = h->layout.tiles(REG_CELL_GROWN);
+        a.vel = vel;
+        a.vel_ng = vel_ng;
+        a.vel_old = vel_old;
+        a.vold_ng = vel_old_ng;
+        a.gp = incremental ? nullptr : gp;
+        a.gp_ng = gp_ng;
+        a.rho = density;
+        a.rho_ng = rho_ng;
+        a.const_rho = const_rho;
+        a.scale = scaling_factor;
+        a.sigma_from_rho = 1;
+        a.ug = h->ug.as<double>();
+        a.sig = h->solver.sigma0();
+        a.ghost_rule = 1;
+        for (int d = 0; d < 3; ++d) { a.bclo[d] = h->bclo[d]; a.bchi[d] = h->bchi[d]; }
+        if (inflow_vel) for (int q = 0; q < 18; ++q) a.inflow[q] = inflow_vel[q];
+        else for (int d = 0; d < 3; ++d) { a.bclo[d] = a.bchi[d] = MGX_BC_NEUMANN; }
+        gather_vel_kernel<<<h->layout.ntiles[REG_CELL_GROWN], 256, 0, h->stream()>>>(a);
+        h->solver.sigma_changed();
+        h->compute_rhs();
+        h->zero_phi();
+        h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
+        UpdateArgs u = h->update_args();
+        u.ug = h->ug.as<double>();
+        u.vel = vel;
+        u.vel_ng = vel_ng;
+        u.vel_old = vel_old;
+        u.vold_ng = vel_old_ng;
+        u.gp = gp;
+        u.gp_ng = gp_ng;
+        u.gp_add = incremental ? 1 : 0;
+        update_cells_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(u);
+        scatter_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kgeom(h->g), h->layout.dev_boxes(), h->layout.tiles(REG_NODE), h->solver.phi, p_nd,
+            p_ng, incremental ? 1 : 0);
+        h->have_phi = true;
+    });
+    if (status == MGX_OK && info && !info->converged) {
+        h->ctx->last_error = "nodal projection: MLMG failed to converge";
+        status = MGX_ERR_NOT_CONVERGED;
+    }
+    return status;
+}
+
+}  // extern "C"

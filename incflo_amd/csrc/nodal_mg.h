@@ -1,0 +1,131 @@
+// nodal_mg.h -- nodal multigrid hierarchy and the outer MLMG iteration.
+#pragma once
+
+#include <vector>
+
+#include "box_kernels.h"
+#include "mgx_common.h"
+#include "nodal_kernels.h"
+
+namespace mgx {
+
+struct MGOpts {
+    int maxiter = 100;
+    int bottom_maxiter = 100;
+    double bottom_rtol = 1e-4;
+    int max_coarsening_level = 100;
+    int nu1 = 2, nu2 = 2;
+    double omega = 8.0 / 9.0;
+    int verbose = 0;
+    int stall_iters = 3;
+};
+
+inline KGeom kgeom(const NodalGeom& g) {
+    KGeom k{};
+    for (int d = 0; d < 3; ++d) {
+        k.nn[d] = g.nn[d];
+        k.per[d] = g.per[d];
+        k.dlo[d] = g.dlo[d];
+        k.dhi[d] = g.dhi[d];
+        k.sg[d] = g.sg[d];
+        k.n[d] = g.n[d];
+    }
+    return k;
+}
+
+template <class T>
+struct MGLevel {
+    NodalGeom g;
+    KGeom kg;
+    Coef<T> coef;
+    bool iso = true;
+    DevBuf sig_buf, b_buf, x_buf, x2_buf, r_buf;
+    T* sig = nullptr;
+    T* b = nullptr;
+    T* x = nullptr;
+    T* x2 = nullptr;
+    T* r = nullptr;
+};
+
+// Launch one 27-point stencil pass (Jacobi / residual / update+residual).
+template <class T, class TC, int MODE>
+void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine);
+
+// Optional stream-ordered timing of selected launches (event pairs).
+struct LaunchTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    double total_ms = 0.0;
+    long long count = 0;
+    void begin(hipStream_t s);
+    void end(hipStream_t s);
+    void collect();  // after the stream has been synchronised
+    void reset() { total_ms = 0.0; count = 0; used = 0; }
+    ~LaunchTimer();
+};
+
+template <class T>
+class NodalHierarchy {
+   public:
+    std::vector<MGLevel<T>> L;
+    DevBuf bottom_work;
+    LaunchTimer* timer = nullptr;  // times the finest-level smoother sweeps
+    void build(const NodalGeom& g0, int nlev);
+    // level-0 sigma (ghosted, same layout) from a double array, then coarsen
+    void set_sigma(const double* sig0, hipStream_t s);
+    // V-cycle on the correction equation: input L[0].b, output L[0].x.
+    void vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters);
+    void smooth(int lev, int nsweeps, bool from_zero, const MGOpts& o, hipStream_t s);
+    bool built = false;
+};
+
+// Outer MLMG iteration: FP64 residual, V-cycle correction (FP64 or FP32).
+class NodalSolver {
+   public:
+    NodalGeom g0;
+    MGOpts opts;
+    int nlev = 1;
+    NodalHierarchy<double> H64;
+    NodalHierarchy<float> H32;
+    DevBuf sig0_buf;         // level-0 sigma (ghosted, FP64)
+    DevBuf phi_a, phi_b;     // outer iterate (ping-pong)
+    DevBuf bbuf;             // rhs after the singular/Dirichlet fix
+    DevBuf res64;            // FP64 residual (plain mode's level-0 V-cycle rhs)
+    DevBuf res32;            // FP32 residual (mixed mode's level-0 V-cycle rhs)
+    DevBuf scal;             // device scalars: [0..7] u64 norms, doubles after
+    DevBuf part;             // partial sums
+    DevBuf ibuf;             // device ints (bottom iteration counter)
+    unsigned long long* h_pinned = nullptr;
+    double* phi = nullptr;   // current outer iterate (points into phi_a / phi_b)
+    LaunchTimer fine_timer;  // finest-level smoother launches
+    bool sigma_set = false;
+    bool h32_sigma = false;
+
+    ~NodalSolver();
+    void setup(const NodalGeom& g, const MGOpts& o);
+    double* sigma0() { return sig0_buf.as<double>(); }
+    void sigma_changed() { sigma_set = true; h32_sigma = false; }
+    // Solve L phi = b for the canonical rhs b (device, FP64). The initial
+    // guess is this->phi (zeroed by the caller if wanted).
+    void solve(const double* b, double rtol, double atol, int mode, mgx_solve_info* info,
+               hipStream_t s);
+    double read_norm(int slot, hipStream_t s);
+    void zero_norm(int slot, hipStream_t s);
+    double sum(const double* a, long n, hipStream_t s);
+};
+
+// Box list of one level, its device box-offset table and tile tables.
+class BoxLayout {
+   public:
+    std::vector<int> boxes;  // 6 ints per box
+    int nboxes = 0;
+    DevBuf dboxes;
+    DevBuf dtiles[3];
+    int ntiles[3] = {0, 0, 0};
+    void build(const int* boxes6, int nb, const NodalGeom& g);
+    const BoxInfo* dev_boxes() const { return dboxes.as<BoxInfo>(); }
+    const Tile* tiles(int region) const { return dtiles[region].as<Tile>(); }
+};
+
+}  // namespace mgx

@@ -1,0 +1,403 @@
+// nodal_mg.hip -- host side of the nodal multigrid: hierarchy set-up, V-cycle
+// and the outer MLMG iteration in FP64 or as an FP32 correction inside FP64
+// iterative refinement.
+//
+// Algorithm (the same steps as the oracle, oracle/orc_nodal.c):
+//   b <- b with Dirichlet rows zeroed; singular problems: b <- b - mean(b)
+//   r <- b - L phi;  tol = max(atol, rtol ||b||_inf)
+//   repeat: cor <- Vcycle(r); phi <- phi + cor; r <- b - L phi   (FP64)
+//   until ||r||_inf <= tol  (incflo: project(rtol=1e-11, atol=1e-14),
+//   src/projection/incflo_apply_nodal_projection.cpp:215, src/incflo.H:453-454)
+// The V-cycle: weighted Jacobi (nu1 sweeps, the first from zero), residual,
+// full-weighting restriction, recursion, trilinear prolongation, nu2 sweeps;
+// the coarsest level is solved by a one-workgroup FP64 BiCGStab.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mgx_ctx.h"
+#include "nodal_mg.h"
+
+namespace mgx {
+
+template <class T, class TC, int MODE>
+void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
+    constexpr int TX = 64, TY = 4;
+    a.gx = (g.nn[0] + TX - 1) / TX;
+    a.gy = (g.nn[1] + TY - 1) / TY;
+    const long gxy = (long)a.gx * a.gy;
+    int kz = (int)std::min<long>(32, std::max<long>(2, (g.nn[2] * gxy) / 2048));
+    kz = std::min(kz, g.nn[2]);
+    a.kz = kz;
+    a.gz = (g.nn[2] + kz - 1) / kz;
+    const unsigned nb = (unsigned)(gxy * a.gz);
+    dim3 blk(TX, TY);
+    if (g.iso()) {
+        if (fine)
+            nodal_stencil_kernel<T, TC, MODE, true, TX, TY, true><<<nb, blk, 0, s>>>(a);
+        else
+            nodal_stencil_kernel<T, TC, MODE, true, TX, TY, false><<<nb, blk, 0, s>>>(a);
+    } else {
+        nodal_stencil_kernel<T, TC, MODE, false, TX, TY, false><<<nb, blk, 0, s>>>(a);
+    }
+}
+
+void LaunchTimer::begin(hipStream_t s) {
+    if (!on) return;
+    while (ev.size() < used + 2) {
+        hipEvent_t e;
+        MGX_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+    }
+    MGX_HIP(hipEventRecord(ev[used], s));
+}
+
+void LaunchTimer::end(hipStream_t s) {
+    if (!on) return;
+    MGX_HIP(hipEventRecord(ev[used + 1], s));
+    used += 2;
+}
+
+void LaunchTimer::collect() {
+    for (size_t q = 0; q + 1 < used; q += 2) {
+        float ms = 0.f;
+        MGX_HIP(hipEventSynchronize(ev[q + 1]));
+        MGX_HIP(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+        total_ms += ms;
+        ++count;
+    }
+    used = 0;
+}
+
+LaunchTimer::~LaunchTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+template <class T>
+void NodalHierarchy<T>::build(const NodalGeom& g0, int nlev) {
+    L.clear();
+    L.resize(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        MGLevel<T>& lv = L[l];
+        lv.g = (l == 0) ? g0 : coarsen(L[l - 1].g);
+        lv.kg = kgeom(lv.g);
+        lv.coef = make_coef<T>(lv.g.dx);
+        lv.iso = lv.g.iso();
+        const long nn = lv.g.nnode(), ns = lv.g.nsig();
+        lv.sig_buf.alloc(ns * sizeof(T));
+        MGX_HIP(hipMemset(lv.sig_buf.p, 0, ns * sizeof(T)));
+        if (l > 0) lv.b_buf.alloc(nn * sizeof(T));
+        lv.x_buf.alloc(nn * sizeof(T));
+        lv.x2_buf.alloc(nn * sizeof(T));
+        if (l < nlev - 1) lv.r_buf.alloc(nn * sizeof(T));
+        lv.sig = lv.sig_buf.template as<T>();
+        lv.b = lv.b_buf.template as<T>();
+        lv.x = lv.x_buf.template as<T>();
+        lv.x2 = lv.x2_buf.template as<T>();
+        lv.r = lv.r_buf.template as<T>();
+    }
+    bottom_work.alloc(7 * L.back().g.nnode() * sizeof(double));
+    built = true;
+}
+
+template <class T>
+void NodalHierarchy<T>::set_sigma(const double* sig0, hipStream_t s) {
+    const long ns = L[0].g.nsig();
+    if (std::is_same<T, double>::value) {
+        if ((const void*)sig0 != (const void*)L[0].sig)
+            MGX_HIP(hipMemcpyAsync(L[0].sig, sig0, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        cast_kernel<T, double><<<grid1d(ns), 256, 0, s>>>(sig0, L[0].sig, ns);
+    }
+    for (size_t l = 1; l < L.size(); ++l) {
+        const long nc = (long)L[l].g.n[0] * L[l].g.n[1] * L[l].g.n[2];
+        sigma_coarsen_kernel<T><<<grid1d(nc), 256, 0, s>>>(L[l - 1].kg, L[l].kg, L[l - 1].sig,
+                                                           L[l].sig);
+    }
+}
+
+template <class T>
+void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts& o, hipStream_t s) {
+    MGLevel<T>& lv = L[l];
+    const long nn = lv.g.nnode();
+    int sweeps = nsweeps;
+    if (from_zero) {
+        if (sweeps == 0) {
+            zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.x, nn);
+            return;
+        }
+        nodal_jacobi_first_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.kg, lv.coef, lv.b, lv.sig,
+                                                                lv.x, (T)o.omega);
+        --sweeps;
+    }
+    for (int q = 0; q < sweeps; ++q) {
+        SArgs<T, T> a{};
+        a.g = lv.kg;
+        a.c = lv.coef;
+        a.x = lv.x;
+        a.b = lv.b;
+        a.sig = lv.sig;
+        a.xout = lv.x2;
+        a.omega = (T)o.omega;
+        if (l == 0 && timer) timer->begin(s);
+        launch_stencil<T, T, KS_JACOBI>(lv.g, a, s, l == 0);
+        if (l == 0 && timer) timer->end(s);
+        std::swap(lv.x, lv.x2);
+    }
+}
+
+template <class T>
+void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) {
+    const int nl = (int)L.size();
+    for (int l = 0; l < nl - 1; ++l) {
+        MGLevel<T>& lv = L[l];
+        smooth(l, o.nu1, true, o, s);
+        SArgs<T, T> a{};
+        a.g = lv.kg;
+        a.c = lv.coef;
+        a.x = lv.x;
+        a.b = lv.b;
+        a.sig = lv.sig;
+        a.rout = lv.r;
+        launch_stencil<T, T, KS_RESID>(lv.g, a, s, false);
+        const long nc = L[l + 1].g.nnode();
+        nodal_restrict_kernel<T><<<grid1d(nc), 256, 0, s>>>(lv.kg, L[l + 1].kg, lv.r, L[l + 1].b);
+    }
+    {
+        MGLevel<T>& lb = L[nl - 1];
+        const long nb = lb.g.nnode();
+        const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
+        bottom_bicgstab_kernel<T><<<1, threads, 0, s>>>(
+            lb.kg, make_coef<double>(lb.g.dx), lb.sig, lb.b, lb.x, bottom_work.as<double>(),
+            o.bottom_maxiter, o.bottom_rtol, lb.g.singular() ? 1 : 0, bottom_iters);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        MGLevel<T>& lv = L[l];
+        const long nf = lv.g.nnode();
+        nodal_prolong_add_kernel<T><<<grid1d(nf), 256, 0, s>>>(lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);
+        smooth(l, o.nu2, false, o, s);
+    }
+}
+
+template void launch_stencil<double, double, KS_APPLY>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
+
+template class NodalHierarchy<double>;
+template class NodalHierarchy<float>;
+
+// ---------------------------------------------------------------------------
+
+NodalSolver::~NodalSolver() {
+    if (h_pinned) (void)hipHostFree(h_pinned);
+}
+
+void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
+    g0 = g;
+    opts = o;
+    nlev = nodal_nlevels(g, o.max_coarsening_level);
+    const long nn = g.nnode();
+    sig0_buf.alloc(g.nsig() * sizeof(double));
+    MGX_HIP(hipMemset(sig0_buf.p, 0, g.nsig() * sizeof(double)));
+    phi_a.alloc(nn * sizeof(double));
+    phi_b.alloc(nn * sizeof(double));
+    MGX_HIP(hipMemset(phi_a.p, 0, nn * sizeof(double)));
+    phi = phi_a.as<double>();
+    bbuf.alloc(nn * sizeof(double));
+    scal.alloc(64 * sizeof(double));
+    part.alloc(1024 * sizeof(double));
+    ibuf.alloc(16 * sizeof(int));
+    if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
+    H64.built = false;
+    H32.built = false;
+    sigma_set = false;
+    h32_sigma = false;
+}
+
+void NodalSolver::zero_norm(int slot, hipStream_t s) {
+    zero_u64_kernel<<<1, 64, 0, s>>>(scal.as<unsigned long long>() + slot, 1);
+}
+
+double NodalSolver::read_norm(int slot, hipStream_t s) {
+    MGX_HIP(hipMemcpyAsync(h_pinned + slot, scal.as<unsigned long long>() + slot,
+                           sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    if (fine_timer.on) fine_timer.collect();
+    unsigned long long v = h_pinned[slot];
+    double d;
+    std::memcpy(&d, &v, sizeof(d));
+    return d;
+}
+
+double NodalSolver::sum(const double* a, long n, hipStream_t s) {
+    const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+    partial_sum_kernel<double><<<np, 256, 0, s>>>(a, n, part.as<double>());
+    final_sum_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, scal.as<double>() + 32, 1.0);
+    double h;
+    MGX_HIP(hipMemcpyAsync(&h, scal.as<double>() + 32, sizeof(double), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+static void remove_mean(NodalSolver& S, double* a, long n, hipStream_t s) {
+    const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+    partial_sum_kernel<double><<<np, 256, 0, s>>>(a, n, S.part.as<double>());
+    final_sum_kernel<<<1, 256, 0, s>>>(S.part.as<double>(), np, S.scal.as<double>() + 33,
+                                       1.0 / (double)n);
+    subtract_scalar_kernel<double><<<grid1d(n), 256, 0, s>>>(a, n, S.scal.as<double>() + 33);
+}
+
+void NodalSolver::solve(const double* b_in, double rtol, double atol, int mode,
+                        mgx_solve_info* info, hipStream_t s) {
+    MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "nodal solve: sigma has not been set");
+    MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_UNSUPPORTED,
+                "nodal solve: mode not available in this build");
+    const NodalGeom& g = g0;
+    const KGeom kg = kgeom(g);
+    const long nn = g.nnode();
+    const bool mixed = (mode == MGX_MODE_MIXED);
+    H64.timer = &fine_timer;
+    H32.timer = &fine_timer;
+    if (mixed) {
+        if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g, nlev); h32_sigma = false; }
+        if (!h32_sigma) { H32.set_sigma(sigma0(), s); h32_sigma = true; }
+        res32.alloc(nn * sizeof(float));
+        H32.L[0].b = res32.as<float>();
+    } else {
+        if (!H64.built || (int)H64.L.size() != nlev) H64.build(g, nlev);
+        // level-0 sigma of the FP64 hierarchy is the solver's own array
+        if (H64.L[0].sig != sigma0()) {
+            H64.L[0].sig_buf.release();
+            H64.L[0].sig = sigma0();
+        }
+        res64.alloc(nn * sizeof(double));
+        H64.L[0].b = res64.as<double>();
+        H64.set_sigma(sigma0(), s);  // coarse levels; level 0 is shared
+    }
+
+    double* b = bbuf.as<double>();
+    if (b_in != b) MGX_HIP(hipMemcpyAsync(b, b_in, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+    zero_dirichlet_kernel<double><<<grid1d(nn), 256, 0, s>>>(kg, b);
+    zero_dirichlet_kernel<double><<<grid1d(nn), 256, 0, s>>>(kg, phi);
+    if (g.singular()) remove_mean(*this, b, nn, s);
+    zero_norm(0, s);
+    max_abs_kernel<double><<<std::min<long>(1024, grid1d(nn)), 256, 0, s>>>(
+        b, nn, scal.as<unsigned long long>());
+    MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
+    const double bnorm = read_norm(0, s);
+    const Coef<double> c64 = make_coef<double>(g.dx);
+
+    // initial residual
+    zero_norm(1, s);
+    if (mixed) {
+        SArgs<double, float> a{};
+        a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
+        a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
+        launch_stencil<double, float, KS_RESID>(g, a, s, false);
+    } else {
+        SArgs<double, double> a{};
+        a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
+        a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
+        launch_stencil<double, double, KS_RESID>(g, a, s, false);
+    }
+    const double r0 = read_norm(1, s);
+    const double tol = std::max(atol, rtol * bnorm);
+    double rn = r0;
+    int it = 0;
+    bool conv = (r0 <= tol);
+    while (!conv && it < opts.maxiter) {
+        ++it;
+        zero_norm(1, s);
+        double* pout = (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>();
+        if (mixed) {
+            H32.vcycle(opts, s, ibuf.as<int>());
+            SArgs<double, float> a{};
+            a.g = kg; a.c = c64; a.x = phi; a.cor = H32.L[0].x; a.b = b; a.sig = sigma0();
+            a.xout = pout; a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
+            launch_stencil<double, float, KS_UPDRES>(g, a, s, false);
+        } else {
+            H64.vcycle(opts, s, ibuf.as<int>());
+            SArgs<double, double> a{};
+            a.g = kg; a.c = c64; a.x = phi; a.cor = H64.L[0].x; a.b = b; a.sig = sigma0();
+            a.xout = pout; a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
+            launch_stencil<double, double, KS_UPDRES>(g, a, s, false);
+        }
+        phi = pout;
+        rn = read_norm(1, s);
+        if (opts.verbose)
+            fprintf(stderr, "mgx nodal MLMG: iter %d |r| %.6e rel %.3e\n", it, rn,
+                    bnorm > 0 ? rn / bnorm : rn);
+        conv = (rn <= tol);
+    }
+    if (g.singular()) remove_mean(*this, phi, nn, s);
+    int bot = 0;
+    MGX_HIP(hipMemcpyAsync(&bot, ibuf.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    if (info) {
+        info->iters = it;
+        info->converged = conv ? 1 : 0;
+        info->nlevels = nlev;
+        info->fallback_used = 0;
+        info->bnorm = bnorm;
+        info->resnorm0 = r0;
+        info->resnorm = rn;
+        info->rel_resid = bnorm > 0 ? rn / bnorm : rn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+void BoxLayout::build(const int* b6, int nb, const NodalGeom& g) {
+    MGX_REQUIRE(nb > 0 && b6 != nullptr, MGX_ERR_ARG, "box list is empty");
+    boxes.assign(b6, b6 + 6 * nb);
+    nboxes = nb;
+    long vol = 0;
+    for (int b = 0; b < nb; ++b) {
+        for (int d = 0; d < 3; ++d) {
+            const int lo = b6[6 * b + d], hi = b6[6 * b + 3 + d];
+            MGX_REQUIRE(lo >= 0 && hi < g.n[d] && lo <= hi, MGX_ERR_ARG,
+                        "box " + std::to_string(b) + " lies outside the domain");
+        }
+        vol += (long)(b6[6 * b + 3] - b6[6 * b] + 1) * (b6[6 * b + 4] - b6[6 * b + 1] + 1) *
+               (b6[6 * b + 5] - b6[6 * b + 2] + 1);
+    }
+    MGX_REQUIRE(vol == (long)g.n[0] * g.n[1] * g.n[2], MGX_ERR_UNSUPPORTED,
+                "single-level solve needs boxes that tile the whole domain");
+    for (int a = 0; a < nb; ++a)
+        for (int b = a + 1; b < nb; ++b) {
+            bool ov = true;
+            for (int d = 0; d < 3; ++d)
+                if (b6[6 * a + 3 + d] < b6[6 * b + d] || b6[6 * b + 3 + d] < b6[6 * a + d]) ov = false;
+            MGX_REQUIRE(!ov, MGX_ERR_ARG, "boxes overlap");
+        }
+    std::vector<BoxInfo> info(nb);
+    long pc[4] = {0, 0, 0, 0}, pn[4] = {0, 0, 0, 0};
+    std::vector<Tile> tiles[3];
+    for (int b = 0; b < nb; ++b) {
+        BoxInfo& bi = info[b];
+        for (int d = 0; d < 3; ++d) {
+            bi.lo[d] = b6[6 * b + d];
+            bi.len[d] = b6[6 * b + 3 + d] - b6[6 * b + d] + 1;
+            const bool glo = !g.per[d] && bi.lo[d] == 0;
+            const bool ghi = !g.per[d] && bi.lo[d] + bi.len[d] == g.n[d];
+            bi.rlo[d] = bi.lo[d] - (glo ? 1 : 0);
+            bi.rlen[d] = bi.len[d] + (glo ? 1 : 0) + (ghi ? 1 : 0);
+        }
+        for (int q = 0; q < 4; ++q) { bi.pc[q] = pc[q]; bi.pn[q] = pn[q]; }
+        const long a0 = bi.len[0], a1 = bi.len[1], a2 = bi.len[2];
+        pc[0] += a0 * a1 * a2; pc[1] += a0 * a1 + a1 * a2 + a0 * a2; pc[2] += a0 + a1 + a2; pc[3] += 1;
+        const long n0 = a0 + 1, n1 = a1 + 1, n2 = a2 + 1;
+        pn[0] += n0 * n1 * n2; pn[1] += n0 * n1 + n1 * n2 + n0 * n2; pn[2] += n0 + n1 + n2; pn[3] += 1;
+        const long vols[3] = {a0 * a1 * a2, (long)bi.rlen[0] * bi.rlen[1] * bi.rlen[2], n0 * n1 * n2};
+        for (int r = 0; r < 3; ++r)
+            for (long st = 0; st < vols[r]; st += 256) tiles[r].push_back(Tile{b, (int)st});
+    }
+    dboxes.alloc(nb * sizeof(BoxInfo));
+    MGX_HIP(hipMemcpy(dboxes.p, info.data(), nb * sizeof(BoxInfo), hipMemcpyHostToDevice));
+    for (int r = 0; r < 3; ++r) {
+        ntiles[r] = (int)tiles[r].size();
+        dtiles[r].alloc(tiles[r].size() * sizeof(Tile));
+        MGX_HIP(hipMemcpy(dtiles[r].p, tiles[r].data(), tiles[r].size() * sizeof(Tile),
+                          hipMemcpyHostToDevice));
+    }
+}
+
+}  // namespace mgx

@@ -1,0 +1,275 @@
+"""ctypes binding of libmgx.so (include/mgx.h) used by the tests and bench.
+
+This is plumbing over the C ABI: device memory comes from the library's own
+mgx_dev_malloc / memcpy helpers, so no second HIP runtime is loaded.  The
+library is required -- there is no CPU fallback: loading raises when
+libmgx.so is missing and every call raises MgxError on a non-zero status.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmgx.so")
+
+MGX_OK, MGX_ERR_ARG, MGX_ERR_HIP, MGX_ERR_NOT_CONVERGED, MGX_ERR_STATE, MGX_ERR_UNSUPPORTED = range(6)
+BC_PERIODIC, BC_DIRICHLET, BC_NEUMANN, BC_INFLOW, BC_ROBIN = range(5)
+MODE_FP64, MODE_MIXED, MODE_LEARNED = 0, 1, 2
+
+
+class MgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mgx status {code}: {msg}")
+        self.code = code
+
+
+class LevelDesc(C.Structure):
+    _fields_ = [("n_cell", C.c_int * 3), ("prob_lo", C.c_double * 3),
+                ("prob_hi", C.c_double * 3), ("is_periodic", C.c_int * 3),
+                ("nboxes", C.c_int), ("boxes", C.POINTER(C.c_int))]
+
+
+class MLMGOpts(C.Structure):
+    _fields_ = [("verbose", C.c_int), ("bottom_verbose", C.c_int), ("maxiter", C.c_int),
+                ("bottom_maxiter", C.c_int), ("bottom_rtol", C.c_double),
+                ("bottom_atol", C.c_double), ("max_coarsening_level", C.c_int),
+                ("nu1", C.c_int), ("nu2", C.c_int), ("omega", C.c_double),
+                ("stall_iters", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("nlevels", C.c_int),
+                ("fallback_used", C.c_int), ("bnorm", C.c_double), ("resnorm0", C.c_double),
+                ("resnorm", C.c_double), ("rel_resid", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m incflo_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.mgx_version.restype = C.c_char_p
+        L.mgx_ctx_last_error.restype = C.c_char_p
+        L.mgx_ctx_last_error.argtypes = [C.c_void_p]
+        vp = C.c_void_p
+        for name, args in {
+            "mgx_ctx_create": [C.c_int, vp, C.POINTER(vp)],
+            "mgx_ctx_destroy": [vp],
+            "mgx_ctx_synchronize": [vp],
+            "mgx_dev_malloc": [vp, C.c_ulonglong, C.POINTER(vp)],
+            "mgx_dev_free": [vp, vp],
+            "mgx_memcpy_h2d": [vp, vp, vp, C.c_ulonglong],
+            "mgx_memcpy_d2h": [vp, vp, vp, C.c_ulonglong],
+            "mgx_memset_d": [vp, vp, C.c_int, C.c_ulonglong],
+            "mgx_memcpy_d2d": [vp, vp, vp, C.c_ulonglong],
+            "mgx_nodal_set_timing": [vp, C.c_int],
+            "mgx_nodal_get_timing": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
+            "mgx_ctx_record": [vp, C.c_int],
+            "mgx_ctx_elapsed_ms": [vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
+            "mgx_nodal_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
+            "mgx_nodal_destroy": [vp],
+            "mgx_nodal_set_domain_bc": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "mgx_nodal_set_options": [vp, C.POINTER(MLMGOpts)],
+            "mgx_nodal_set_sigma": [vp, vp, C.c_int],
+            "mgx_nodal_set_const_sigma": [vp, C.c_double],
+            "mgx_nodal_project": [vp, vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.POINTER(SolveInfo)],
+            "mgx_nodal_get_phi": [vp, vp, C.c_int],
+            "mgx_nodal_apply": [vp, vp, C.c_int, vp, C.c_int],
+            "mgx_nodal_get_gradphi": [vp, vp, C.c_int],
+            "mgx_nodal_solve": [vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                C.POINTER(SolveInfo)],
+            "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
+                                           vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp,
+                                           C.c_double, C.c_double, C.c_int,
+                                           C.POINTER(SolveInfo)],
+        }.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.mgx_default_mlmg_opts.argtypes = [C.POINTER(MLMGOpts)]
+        L.mgx_default_mlmg_opts.restype = None
+        _lib = L
+    return _lib
+
+
+def default_opts(**kw):
+    o = MLMGOpts()
+    lib().mgx_default_mlmg_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    def __init__(self, device=0, stream=None):
+        self.h = C.c_void_p()
+        st = lib().mgx_ctx_create(device, stream, C.byref(self.h))
+        if st != MGX_OK:
+            raise MgxError(st, f"mgx_ctx_create(device={device}) failed (no GPU?)")
+
+    def check(self, st):
+        if st != MGX_OK:
+            raise MgxError(st, lib().mgx_ctx_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(lib().mgx_ctx_synchronize(self.h))
+
+    def record(self, slot):
+        self.check(lib().mgx_ctx_record(self.h, slot))
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_double()
+        self.check(lib().mgx_ctx_elapsed_ms(self.h, a, b, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().mgx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def array(self, host_or_n, dtype=np.float64):
+        return DeviceArray(self, host_or_n, dtype)
+
+
+class DeviceArray:
+    """A flat device buffer with numpy upload/download."""
+
+    def __init__(self, ctx, host_or_n, dtype=np.float64):
+        self.ctx = ctx
+        if isinstance(host_or_n, np.ndarray):
+            host = np.ascontiguousarray(host_or_n, dtype=dtype)
+            self.n = host.size
+        else:
+            host = None
+            self.n = int(host_or_n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        ctx.check(lib().mgx_dev_malloc(ctx.h, max(self.nbytes, 1), C.byref(self.ptr)))
+        if host is not None:
+            self.upload(host)
+        else:
+            ctx.check(lib().mgx_memset_d(ctx.h, self.ptr, 0, self.nbytes))
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.size == self.n
+        self.ctx.check(lib().mgx_memcpy_h2d(self.ctx.h, self.ptr, host.ctypes.data, self.nbytes))
+
+    def copy_from(self, other):
+        """stream-ordered device-to-device copy"""
+        assert other.nbytes == self.nbytes
+        self.ctx.check(lib().mgx_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        self.ctx.check(lib().mgx_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mgx_dev_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_level(n_cell, boxes, periodic=(1, 1, 1), prob_lo=(0, 0, 0), prob_hi=(1, 1, 1)):
+    d = LevelDesc()
+    for k in range(3):
+        d.n_cell[k] = int(n_cell[k])
+        d.prob_lo[k] = float(prob_lo[k])
+        d.prob_hi[k] = float(prob_hi[k])
+        d.is_periodic[k] = int(periodic[k])
+    flat = np.ascontiguousarray(np.array([list(lo) + list(hi) for lo, hi in boxes], dtype=np.int32))
+    d.nboxes = len(boxes)
+    d._keep = flat
+    d.boxes = flat.ctypes.data_as(C.POINTER(C.c_int))
+    return d
+
+
+class NodalProjector:
+    """Thin mirror of Hydro::NodalProjector over the C ABI (test plumbing)."""
+
+    def __init__(self, ctx, level):
+        self.ctx = ctx
+        self.level = level
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_nodal_create(ctx.h, C.byref(level), C.byref(self.h)))
+
+    def setDomainBC(self, lo, hi):
+        a = (C.c_int * 3)(*lo)
+        b = (C.c_int * 3)(*hi)
+        self.ctx.check(lib().mgx_nodal_set_domain_bc(self.h, a, b))
+
+    def set_options(self, opts):
+        self.ctx.check(lib().mgx_nodal_set_options(self.h, C.byref(opts)))
+
+    def set_timing(self, on):
+        self.ctx.check(lib().mgx_nodal_set_timing(self.h, int(on)))
+
+    def get_timing(self):
+        ms, n = C.c_double(), C.c_longlong()
+        self.ctx.check(lib().mgx_nodal_get_timing(self.h, 0, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def set_sigma(self, sigma_dev, ng=0):
+        self.ctx.check(lib().mgx_nodal_set_sigma(self.h, sigma_dev.ptr, ng))
+
+    def set_const_sigma(self, v):
+        self.ctx.check(lib().mgx_nodal_set_const_sigma(self.h, v))
+
+    def project(self, vel_dev, vel_ng, rtol, atol=0.0, mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_nodal_project(self.h, vel_dev.ptr, vel_ng, rtol, atol, mode,
+                                               C.byref(info)))
+        return info
+
+    def getPhi(self, phi_dev, ng=0):
+        self.ctx.check(lib().mgx_nodal_get_phi(self.h, phi_dev.ptr, ng))
+
+    def getGradPhi(self, g_dev, ng=0):
+        self.ctx.check(lib().mgx_nodal_get_gradphi(self.h, g_dev.ptr, ng))
+
+    def apply(self, phi_dev, phi_ng, out_dev, out_ng):
+        self.ctx.check(lib().mgx_nodal_apply(self.h, phi_dev.ptr, phi_ng, out_dev.ptr, out_ng))
+
+    def solve(self, phi_dev, phi_ng, rhs_dev, rhs_ng, rtol, atol=0.0, mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_nodal_solve(self.h, phi_dev.ptr, phi_ng, rhs_dev.ptr, rhs_ng,
+                                             rtol, atol, mode, C.byref(info)))
+        return info
+
+    def apply_nodal_projection(self, vel, vel_ng, gp, gp_ng, p_nd, p_ng, scaling_factor,
+                               density=None, rho_ng=0, const_rho=0.0, vel_old=None,
+                               vel_old_ng=0, incremental=False, inflow_vel=None, rtol=1e-11,
+                               atol=1e-14, mode=MODE_FP64):
+        info = SolveInfo()
+        iv = None
+        if inflow_vel is not None:
+            iv = np.ascontiguousarray(inflow_vel, dtype=np.float64).reshape(18)
+        st = lib().mgx_apply_nodal_projection(
+            self.h, density.ptr if density is not None else None, rho_ng, const_rho, vel.ptr,
+            vel_ng, vel_old.ptr if vel_old is not None else None, vel_old_ng, gp.ptr, gp_ng,
+            p_nd.ptr, p_ng, scaling_factor, int(incremental),
+            iv.ctypes.data if iv is not None else None, rtol, atol, mode, C.byref(info))
+        self.ctx.check(st)
+        return info
+
+    def close(self):
+        if self.h:
+            lib().mgx_nodal_destroy(self.h)
+            self.h = C.c_void_p()

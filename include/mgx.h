@@ -1,0 +1,185 @@
+/*
+ * mgx.h -- C ABI of the MI355X-native projection solver (libmgx.so).
+ *
+ * This is the drop-in boundary for incflo's projection hot path.  Every entry
+ * point returns an int status (MGX_OK == 0); nothing throws across the ABI.
+ * A non-zero status leaves a message in mgx_ctx_last_error().  The C++ facade
+ * (include/mgx_amrex.H) maps a non-zero status to an abort with that message,
+ * which is what amrex::Abort does at the reference call sites
+ * (src/projection/incflo_projection_bc.cpp:36,74).
+ *
+ * Data layout ("box-offset table" layout).  A level is a list of boxes (cell
+ * index ranges lo..hi inclusive).  A field on a level is ONE device
+ * allocation holding the boxes back to back, in box order; each box is an
+ * AMReX-style Array4: x fastest, then y, z, then component, over the box
+ * grown by the field's ghost width ng.  Cell-centred boxes span
+ * (hi-lo+1+2ng) per direction, nodal boxes (hi-lo+2+2ng).  Box b starts at
+ * ncomp * sum_{b'<b} volume(b').  The caller owns all field memory; the
+ * handle owns the multigrid workspace.
+ *
+ * All device work is enqueued on the context's HIP stream; project/solve
+ * synchronise only to read residual norms and to return the info struct.
+ * A handle is used from one host thread at a time.
+ */
+#ifndef MGX_H
+#define MGX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGX_API __attribute__((visibility("default")))
+
+/* status codes */
+enum {
+    MGX_OK = 0,
+    MGX_ERR_ARG = 1,           /* invalid argument (amrex::Abort at the reference site) */
+    MGX_ERR_HIP = 2,           /* HIP runtime error                                     */
+    MGX_ERR_NOT_CONVERGED = 3, /* MLMG did not reach the tolerance within maxiter       */
+    MGX_ERR_STATE = 4,         /* call out of order (e.g. project before sigma)          */
+    MGX_ERR_UNSUPPORTED = 5    /* valid input this build does not implement              */
+};
+
+/* boundary types: mirrors amrex::LinOpBCType as produced by
+ * incflo::get_projection_bc / get_mac_projection_bc
+ * (src/projection/incflo_projection_bc.cpp:6-78) */
+enum {
+    MGX_BC_PERIODIC = 0,
+    MGX_BC_DIRICHLET = 1,
+    MGX_BC_NEUMANN = 2,
+    MGX_BC_INFLOW = 3,
+    MGX_BC_ROBIN = 4
+};
+
+/* solve modes */
+enum {
+    MGX_MODE_FP64 = 0,   /* plain FP64 MLMG                                            */
+    MGX_MODE_MIXED = 1,  /* FP32 V-cycle as the correction inside FP64 refinement      */
+    MGX_MODE_LEARNED = 2 /* mixed + learned initial guess / coarse correction          */
+};
+
+typedef struct mgx_ctx_s* mgx_ctx;
+typedef struct mgx_nodal_s* mgx_nodal;
+
+/* Geometry + box list of one AMR level (amrex::Geometry + BoxArray). */
+typedef struct {
+    int n_cell[3];        /* domain cells                                  */
+    double prob_lo[3];
+    double prob_hi[3];
+    int is_periodic[3];
+    int nboxes;
+    const int* boxes;     /* nboxes*6 ints: lo[0..2], hi[0..2] (cell index, inclusive) */
+} mgx_level_desc;
+
+/* MLMG knobs: the nodal_proj.* / mac_proj.* ParmParse names
+ * (src/incflo.H:409-445, src/setup/init.cpp:165-177,
+ *  Docs/sphinx_documentation/source/InputsMultigrid.rst:10-36). */
+typedef struct {
+    int verbose;
+    int bottom_verbose;
+    int maxiter;               /* default 100 (nodal) / 200 (MAC)             */
+    int bottom_maxiter;        /* default 100 (nodal) / 200 (MAC)             */
+    double bottom_rtol;        /* default 1e-4                                */
+    double bottom_atol;        /* default -1 (unused)                         */
+    int max_coarsening_level;  /* LPInfo::setMaxCoarseningLevel, default 100  */
+    int nu1;                   /* pre-smoothing sweeps, default 2             */
+    int nu2;                   /* post-smoothing sweeps, default 2            */
+    double omega;              /* weighted-Jacobi factor, default 8/9         */
+    int stall_iters;           /* learned mode: fall back to plain MLMG when
+                                  the residual has not dropped 10x within this
+                                  many outer iterations (default 3)          */
+} mgx_mlmg_opts;
+
+typedef struct {
+    int iters;          /* outer (V-cycle) iterations                        */
+    int converged;
+    int nlevels;
+    int fallback_used;  /* learned mode: 1 if the plain MLMG fallback ran    */
+    double bnorm;       /* ||b||_inf (after the singular-problem mean fix)   */
+    double resnorm0;    /* initial ||b - L phi||_inf                         */
+    double resnorm;     /* final ||b - L phi||_inf (FP64)                    */
+    double rel_resid;   /* resnorm / bnorm                                   */
+} mgx_solve_info;
+
+/* ---- library / context ------------------------------------------------ */
+MGX_API const char* mgx_version(void);
+/* stream: a hipStream_t (NULL = the device's null stream). */
+MGX_API int mgx_ctx_create(int device, void* stream, mgx_ctx* out);
+MGX_API int mgx_ctx_destroy(mgx_ctx ctx);
+MGX_API const char* mgx_ctx_last_error(mgx_ctx ctx);
+MGX_API int mgx_ctx_synchronize(mgx_ctx ctx);
+/* Device-memory plumbing for hosts without their own arena. */
+MGX_API int mgx_dev_malloc(mgx_ctx ctx, unsigned long long bytes, void** out);
+MGX_API int mgx_dev_free(mgx_ctx ctx, void* p);
+MGX_API int mgx_memcpy_h2d(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes);
+MGX_API int mgx_memcpy_d2h(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes);
+MGX_API int mgx_memset_d(mgx_ctx ctx, void* dst, int value, unsigned long long bytes);
+/* stream-ordered device-to-device copy (returns without synchronising) */
+MGX_API int mgx_memcpy_d2d(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes);
+/* Stream-ordered timing: record event `slot` (0..15) on the context's
+ * stream; elapsed milliseconds between two recorded slots. */
+MGX_API int mgx_ctx_record(mgx_ctx ctx, int slot);
+MGX_API int mgx_ctx_elapsed_ms(mgx_ctx ctx, int slot_a, int slot_b, double* ms);
+MGX_API void mgx_default_mlmg_opts(mgx_mlmg_opts* o);
+
+/* ---- nodal projection (Hydro::NodalProjector + MLNodeLaplacian + MLMG) -- */
+/* Hydro::NodalProjector constructor, geometry part
+ * (src/projection/incflo_apply_nodal_projection.cpp:187-192). */
+MGX_API int mgx_nodal_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_nodal* out);
+MGX_API int mgx_nodal_destroy(mgx_nodal h);
+/* NodalProjector::setDomainBC(lo, hi) (:194); values MGX_BC_*. */
+MGX_API int mgx_nodal_set_domain_bc(mgx_nodal h, const int lo[3], const int hi[3]);
+/* LPInfo / nodal_proj.* knobs (:181-182). */
+MGX_API int mgx_nodal_set_options(mgx_nodal h, const mgx_mlmg_opts* opts);
+/* Instrumentation (no reference counterpart; the reference's BL_PROFILE regions,
+ * e.g. src/incflo.cpp:220, play this role): when on, every finest-level
+ * smoother launch is bracketed by HIP events on the context's stream;
+ * get_timing(which=0) returns the summed milliseconds and the launch count. */
+MGX_API int mgx_nodal_set_timing(mgx_nodal h, int on);
+MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long long* count);
+/* Variable sigma, a boxed cell field with ng ghosts (:191, sigma = s/rho at :117). */
+MGX_API int mgx_nodal_set_sigma(mgx_nodal h, const double* sigma, int sigma_ng);
+/* Scalar sigma = s/rho_0 (:186-188). */
+MGX_API int mgx_nodal_set_const_sigma(mgx_nodal h, double sigma);
+/* NodalProjector::project(rtol, atol) (:215): rhs = D.vel (vel: boxed cell
+ * field, 3 comps, vel_ng >= 1 ghosts; ghosts outside non-periodic domain faces
+ * are read as given, e.g. 0 or the inflow velocity), solve L phi = rhs, then
+ * vel -= sigma G(phi) on valid cells, in place. */
+MGX_API int mgx_nodal_project(mgx_nodal h, double* vel, int vel_ng, double rtol, double atol,
+                              int mode, mgx_solve_info* info);
+/* NodalProjector::getPhi() / getGradPhi() (:218-219): copy out the last
+ * projection's phi (boxed nodal, 1 comp) and G(phi) (boxed cell, 3 comps). */
+MGX_API int mgx_nodal_get_phi(mgx_nodal h, double* phi, int phi_ng);
+MGX_API int mgx_nodal_get_gradphi(mgx_nodal h, double* gphi, int gphi_ng);
+/* MLLinOp::apply on level 0: out = L phi with the current sigma (boxed nodal
+ * fields; used to check a solution's residual with the device operator). */
+MGX_API int mgx_nodal_apply(mgx_nodal h, const double* phi, int phi_ng, double* out, int out_ng);
+/* MLMG::solve(sol, rhs, rtol, atol) on the nodal operator: boxed nodal fields
+ * (src/diffusion/DiffusionTensorOp.cpp:188-215 shows the call shape).
+ * phi holds the initial guess on entry. */
+MGX_API int mgx_nodal_solve(mgx_nodal h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
+                            double rtol, double atol, int mode, mgx_solve_info* info);
+/* Whole incflo::ApplyNodalProjection (wrapper + core,
+ * src/projection/incflo_apply_nodal_projection.cpp:29-267) in one call:
+ *   if !incremental: u += (s/rho) gp                          (:38-62)
+ *   if incremental or vel_old given for the small-dt form: u -= vel_old (:64-71)
+ *   sigma = s/rho (or s/const_rho when const_rho > 0)          (:101-121, :184-188)
+ *   ghost cells: 0, inflow faces = inflow_vel[face][comp]      (:137-163)
+ *   project(rtol, atol)                                        (:215)
+ *   gp = G(phi) (+= if incremental), p_nd = phi (+= if incr.)  (:221-256)
+ *   if vel_old given: u += vel_old                             (:85-92)
+ * density: boxed cell field (rho_ng ghosts) or NULL with const_rho > 0;
+ * vel: 3-comp cell field (vel_ng >= 1); gp: 3-comp cell field; p_nd: nodal.
+ * inflow_vel: 6x3 doubles indexed [2*dir + (0 lo | 1 hi)][comp] or NULL. */
+MGX_API int mgx_apply_nodal_projection(mgx_nodal h, const double* density, int rho_ng,
+                                       double const_rho, double* vel, int vel_ng,
+                                       const double* vel_old, int vel_old_ng, double* gp,
+                                       int gp_ng, double* p_nd, int p_ng,
+                                       double scaling_factor, int incremental,
+                                       const double* inflow_vel, double rtol, double atol,
+                                       int mode, mgx_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGX_H */

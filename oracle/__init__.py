@@ -1,0 +1,206 @@
+"""ctypes wrapper around the FP64 CPU oracle (oracle/_build/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry point and the
+CPU-baseline leg of bench.py, where it is the checker.  The product library
+never loads it.
+
+Arrays are numpy float64, x fastest (numpy shape (nz, ny, nx), C order).
+Cell vectors with one ghost layer have shape (3, nz+2, ny+2, nx+2).
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+
+BC_PERIODIC, BC_DIRICHLET, BC_NEUMANN, BC_INFLOW, BC_ROBIN = 0, 1, 2, 3, 4
+
+
+class Geom(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("periodic", C.c_int * 3), ("bclo", C.c_int * 3),
+                ("bchi", C.c_int * 3), ("dx", C.c_double * 3)]
+
+    @staticmethod
+    def make(n, periodic=(1, 1, 1), bclo=(0, 0, 0), bchi=(0, 0, 0), prob_lo=(0, 0, 0),
+             prob_hi=(1, 1, 1)):
+        g = Geom()
+        for d in range(3):
+            g.n[d] = int(n[d])
+            g.periodic[d] = int(periodic[d])
+            g.bclo[d] = int(bclo[d]) if not periodic[d] else BC_PERIODIC
+            g.bchi[d] = int(bchi[d]) if not periodic[d] else BC_PERIODIC
+            g.dx[d] = (prob_hi[d] - prob_lo[d]) / n[d]
+        return g
+
+    def nodes(self):
+        return tuple(self.n[d] if self.periodic[d] else self.n[d] + 1 for d in range(3))
+
+    def node_shape(self):
+        nx, ny, nz = self.nodes()
+        return (nz, ny, nx)
+
+    def cell_shape(self):
+        return (self.n[2], self.n[1], self.n[0])
+
+    def ghost_vec_shape(self):
+        return (3, self.n[2] + 2, self.n[1] + 2, self.n[0] + 2)
+
+    def coarsen(self):
+        g = Geom()
+        for d in range(3):
+            g.n[d] = self.n[d] // 2
+            g.periodic[d] = self.periodic[d]
+            g.bclo[d] = self.bclo[d]
+            g.bchi[d] = self.bchi[d]
+            g.dx[d] = 2 * self.dx[d]
+        return g
+
+
+class Opts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("bottom_max_iter", C.c_int),
+                ("bottom_rtol", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int),
+                ("omega", C.c_double), ("max_coarsening_level", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("bnorm", C.c_double),
+                ("resnorm0", C.c_double), ("resnorm", C.c_double), ("nlevels", C.c_int),
+                ("bottom_iters", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        _lib = C.CDLL(_LIB_PATH)
+    return _lib
+
+
+_dp = C.POINTER(C.c_double)
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().orc_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def nodal_apply(g, sigma, phi):
+    out = np.zeros(g.node_shape())
+    lib().orc_nodal_apply(C.byref(g), _p(sigma), _p(phi), _p(out))
+    return out
+
+
+def nodal_diag(g, sigma):
+    out = np.zeros(g.node_shape())
+    lib().orc_nodal_diag(C.byref(g), _p(sigma), _p(out))
+    return out
+
+
+def nodal_residual(g, sigma, b, x):
+    out = np.zeros(g.node_shape())
+    lib().orc_nodal_residual(C.byref(g), _p(sigma), _p(b), _p(x), _p(out))
+    return out
+
+
+def nodal_jacobi(g, sigma, b, x, omega):
+    out = np.zeros(g.node_shape())
+    lib().orc_nodal_jacobi(C.byref(g), _p(sigma), _p(b), _p(x), _p(out), C.c_double(omega))
+    return out
+
+
+def nodal_restrict(gf, rf):
+    out = np.zeros(gf.coarsen().node_shape())
+    lib().orc_nodal_restrict(C.byref(gf), _p(rf), _p(out))
+    return out
+
+
+def nodal_prolong_add(gf, xc, xf):
+    xf = np.ascontiguousarray(xf, dtype=np.float64).copy()
+    lib().orc_nodal_prolong_add(C.byref(gf), _p(xc), _p(xf))
+    return xf
+
+
+def sigma_coarsen(gf, sf):
+    out = np.zeros(gf.coarsen().cell_shape())
+    lib().orc_sigma_coarsen(C.byref(gf), _p(sf), _p(out))
+    return out
+
+
+def nodal_nlevels(g, max_coarsening_level=100):
+    return lib().orc_nodal_nlevels(C.byref(g), C.c_int(max_coarsening_level))
+
+
+def fill_vel_ghosts(g, ug, inflow_vel=None):
+    iv = None
+    if inflow_vel is not None:
+        iv = np.ascontiguousarray(inflow_vel, dtype=np.float64).reshape(6, 3)
+    lib().orc_fill_vel_ghosts(C.byref(g), _p(ug), _p(iv) if iv is not None else None)
+    return ug
+
+
+def nodal_divergence(g, ug):
+    out = np.zeros(g.node_shape())
+    lib().orc_nodal_divergence(C.byref(g), _p(ug), _p(out))
+    return out
+
+
+def nodal_gradient(g, phi):
+    out = np.zeros((3,) + g.cell_shape())
+    lib().orc_nodal_gradient(C.byref(g), _p(phi), _p(out))
+    return out
+
+
+def nodal_bicgstab(g, sigma, b, x0=None, rtol=1e-4, maxiter=100):
+    x = np.zeros(g.node_shape()) if x0 is None else np.array(x0, dtype=np.float64)
+    its = C.c_int(0)
+    ret = lib().orc_nodal_bicgstab(C.byref(g), _p(sigma), _p(b), _p(x), C.c_double(rtol),
+                                   C.c_int(maxiter), C.byref(its))
+    return x, its.value, ret
+
+
+def nodal_solve(g, sigma, b, x0=None, rtol=1e-10, atol=0.0, opts=None):
+    x = np.zeros(g.node_shape()) if x0 is None else np.array(x0, dtype=np.float64)
+    info = Info()
+    o = opts if opts is not None else default_opts()
+    ret = lib().orc_nodal_solve(C.byref(g), _p(sigma), _p(np.ascontiguousarray(b)), _p(x),
+                                C.c_double(rtol), C.c_double(atol), C.byref(o), C.byref(info))
+    return x, info, ret
+
+
+def nodal_project(g, sigma, vel_g, rtol=1e-10, atol=0.0, opts=None):
+    """vel_g is updated in place (valid cells); returns (phi, gphi, info)."""
+    phi = np.zeros(g.node_shape())
+    gphi = np.zeros((3,) + g.cell_shape())
+    info = Info()
+    o = opts if opts is not None else default_opts()
+    ret = lib().orc_nodal_project(C.byref(g), _p(sigma), _p(vel_g), _p(phi), _p(gphi),
+                                  C.c_double(rtol), C.c_double(atol), C.byref(o),
+                                  C.byref(info))
+    return phi, gphi, info, ret
+
+
+def _setup():
+    L = lib()
+    for name in ["orc_norm_inf", "orc_dot", "orc_mean"]:
+        getattr(L, name).restype = C.c_double
+    L.orc_nodal_nlevels.restype = C.c_int
+
+
+def load():
+    lib()
+    _setup()
+    return _lib

@@ -1,0 +1,78 @@
+"""C-ABI checks that need no GPU: the library loads and exports every entry
+point declared in include/mgx.h; the oracle library loads; the boxed-layout
+helpers round-trip."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"MGX_API\s+[\w\s\*]+?\b(mgx_\w+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    names = _declared("mgx.h")
+    for must in ["mgx_ctx_create", "mgx_nodal_create", "mgx_nodal_project", "mgx_nodal_solve",
+                 "mgx_apply_nodal_projection", "mgx_nodal_get_phi", "mgx_nodal_get_gradphi"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from incflo_amd import mgx
+    L = mgx.lib()  # loading must succeed without a GPU
+    missing = [n for n in _declared("mgx.h") if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", mgx.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = set(re.findall(r" T (mgx_\w+)", out))
+    assert set(_declared("mgx.h")) <= exported
+    # nothing but the C ABI is exported as text symbols from the mgx namespace
+    assert all(s.startswith("mgx_") for s in exported)
+    assert L.mgx_version().startswith(b"mgx")
+
+
+def test_library_is_built_for_gfx950():
+    from incflo_amd import mgx
+    out = subprocess.run(["strings", mgx.LIB_PATH], capture_output=True, text=True).stdout
+    assert "gfx950" in out
+
+
+def test_ctx_create_without_gpu_fails_cleanly():
+    from incflo_amd import mgx
+    h = C.c_void_p()
+    st = mgx.lib().mgx_ctx_create(999, None, C.byref(h))
+    assert st != mgx.MGX_OK and not h.value
+
+
+def test_default_opts():
+    from incflo_amd import mgx
+    o = mgx.default_opts()
+    assert o.maxiter == 100 and o.bottom_maxiter == 100 and o.nu1 == 2 and o.nu2 == 2
+    assert o.max_coarsening_level == 100 and abs(o.bottom_rtol - 1e-4) < 1e-18
+
+
+def test_boxed_layout_roundtrip():
+    from incflo_amd import boxes as B
+    bxs = B.chop_domain((12, 8, 10), 5)
+    assert len(bxs) == 3 * 2 * 2
+    rng = np.random.default_rng(0)
+    cell = rng.standard_normal((3, 10, 8, 12))
+    boxed = B.to_boxed(cell, bxs, 2)
+    assert boxed.size == B.field_size(bxs, 2, 3)
+    back = B.from_boxed(boxed, bxs, 2, (10, 8, 12), ncomp=3)
+    assert np.array_equal(back, cell)
+    node = rng.standard_normal((10, 8, 12))  # periodic nodes
+    nb = B.to_boxed(node, bxs, 0, nodal=True)
+    assert np.array_equal(B.from_boxed(nb, bxs, 0, (10, 8, 12), nodal=True), node)
+
+
+def test_oracle_library_loads(orc):
+    assert orc.nodal_nlevels(orc.Geom.make((256, 256, 256))) == 7
+    assert orc.nodal_nlevels(orc.Geom.make((256, 64, 64), periodic=(0, 0, 0))) == 5

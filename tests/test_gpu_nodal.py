@@ -1,0 +1,281 @@
+"""GPU parity of the nodal projection path against the FP64 CPU oracle.
+
+All calls go through the C ABI (libmgx.so).  Tolerances (stated per test):
+  * operator apply, FP64:           1e-12 relative (summation order only)
+  * FP64 MLMG solve / projection:   1e-10 relative in phi, u, G(phi)
+    (both sides run the same V-cycle; they differ by rounding order)
+  * mixed FP32/FP64 solve:          relative residual <= 1e-10 measured by the
+    oracle's FP64 operator; phi within 1e-7 of the oracle's FP64 solution.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from incflo_amd import boxes as B
+
+pytestmark = pytest.mark.gpu
+
+D, N, INF = 1, 2, 3  # BC_DIRICHLET, BC_NEUMANN, BC_INFLOW
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from incflo_amd import mgx
+    c = mgx.Context(0)
+    yield c
+    c.close()
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _proj(ctx, n, boxes, periodic=(1, 1, 1), bclo=None, bchi=None, prob_hi=(1, 1, 1), opts=None):
+    from incflo_amd import mgx
+    lev = mgx.make_level(n, boxes, periodic=periodic, prob_hi=prob_hi)
+    P = mgx.NodalProjector(ctx, lev)
+    if bclo is not None:
+        P.setDomainBC(bclo, bchi)
+    if opts is not None:
+        P.set_options(opts)
+    return P
+
+
+def _geom(orc, n, periodic=(1, 1, 1), bclo=(0, 0, 0), bchi=(0, 0, 0), prob_hi=(1, 1, 1)):
+    return orc.Geom.make(n, periodic=periodic, bclo=bclo, bchi=bchi, prob_hi=prob_hi)
+
+
+def test_apply_matches_oracle_uneven_boxes(ctx, orc):
+    n, per = (20, 12, 18), (0, 1, 0)
+    bclo, bchi = (N, 0, D), (D, 0, N)
+    _say(f"case apply n={n}")
+    g = _geom(orc, n, per, bclo, bchi, prob_hi=(1.0, 0.6, 0.9))
+    bxs = B.chop_domain(n, 7)
+    rng = np.random.default_rng(1)
+    sigma = 0.5 + rng.random(g.cell_shape())
+    phi = rng.standard_normal(g.node_shape())
+    P = _proj(ctx, n, bxs, per, (N, 0, D), (D, 0, N), prob_hi=(1.0, 0.6, 0.9))
+    sd = ctx.array(B.to_boxed(sigma, bxs, 0))
+    P.set_sigma(sd, 0)
+    pd = ctx.array(B.to_boxed(phi, bxs, 1, nodal=True, periodic=per))
+    od = ctx.array(B.field_size(bxs, 0, 1, True))
+    P.apply(pd, 1, od, 0)
+    out = B.from_boxed(od.download(), bxs, 0, g.node_shape(), nodal=True)
+    ref = orc.nodal_apply(g, sigma, phi)
+    assert _rel(out, ref) < 1e-12
+    P.close()
+
+
+def _solve_case(ctx, orc, n, per, bclo, bchi, prob_hi, seed, mode, mgs=16):
+    from incflo_amd import mgx
+    g = _geom(orc, n, per, bclo, bchi, prob_hi)
+    bxs = B.chop_domain(n, mgs)
+    rng = np.random.default_rng(seed)
+    sigma = 1.0 + 0.5 * np.tanh(rng.standard_normal(g.cell_shape()))
+    b = rng.standard_normal(g.node_shape())
+    P = _proj(ctx, n, bxs, per, bclo if not all(per) else None, bchi, prob_hi)
+    P.set_sigma(ctx.array(B.to_boxed(sigma, bxs, 0)), 0)
+    rd = ctx.array(B.to_boxed(b, bxs, 0, nodal=True, periodic=per))
+    xd = ctx.array(B.field_size(bxs, 0, 1, True))
+    info = P.solve(xd, 0, rd, 0, rtol=1e-12 if mode == mgx.MODE_FP64 else 1e-10, mode=mode)
+    x = B.from_boxed(xd.download(), bxs, 0, g.node_shape(), nodal=True)
+    P.close()
+    return g, sigma, b, x, info
+
+
+@pytest.mark.parametrize("case", ["periodic", "walls_dirichlet", "inflow_outflow"])
+def test_fp64_solve_matches_oracle(ctx, orc, case):
+    from incflo_amd import mgx
+    cfg = {
+        "periodic": ((32, 32, 32), (1, 1, 1), (0, 0, 0), (0, 0, 0), (1, 1, 1)),
+        "walls_dirichlet": ((32, 16, 24), (0, 0, 0), (D, N, N), (N, N, D), (2.0, 1.0, 1.5)),
+        "inflow_outflow": ((48, 16, 16), (0, 1, 1), (INF, 0, 0), (D, 0, 0), (3.0, 1.0, 1.0)),
+    }[case]
+    n, per, bclo, bchi, ph = cfg
+    _say(f"case fp64 solve {case} n={n}")
+    g, sigma, b, x, info = _solve_case(ctx, orc, n, per, bclo, bchi, ph, 7, mgx.MODE_FP64)
+    xo, io, ro = orc.nodal_solve(g, sigma, b, rtol=1e-12)
+    assert info.converged and io.converged
+    assert abs(info.iters - io.iters) <= 1
+    assert _rel(x, xo) < 1e-10
+    # Dirichlet rows carry no equation; singular problems use the mean-free rhs
+    bb = b.copy()
+    for d in range(3):
+        if per[d]:
+            continue
+        ax = 2 - d
+        if bclo[d] == D:
+            bb[(slice(None),) * ax + (0,)] = 0.0
+        if bchi[d] == D:
+            bb[(slice(None),) * ax + (-1,)] = 0.0
+    if not any(D in (bclo[d], bchi[d]) for d in range(3) if not per[d]):
+        bb -= bb.mean()
+    r = orc.nodal_residual(g, sigma, bb, x)
+    assert np.max(np.abs(r)) <= 1e-12 * info.bnorm * 1.01
+
+
+def test_mixed_solve_reaches_1e10_and_matches_fp64(ctx, orc):
+    from incflo_amd import mgx
+    n = (32, 32, 32)
+    _say("case mixed solve periodic 32^3")
+    g, sigma, b, x, info = _solve_case(ctx, orc, n, (1, 1, 1), (0,) * 3, (0,) * 3, (1, 1, 1), 11,
+                                       mgx.MODE_MIXED)
+    assert info.converged
+    bm = b - b.mean()
+    r = orc.nodal_residual(g, sigma, bm, x)
+    assert np.max(np.abs(r)) / np.max(np.abs(bm)) <= 1e-10
+    xo, io, _ = orc.nodal_solve(g, sigma, b, rtol=1e-13)
+    assert _rel(x, xo) < 1e-7
+    # the FP32 V-cycle contracts about as well as the FP64 one
+    assert info.iters <= io.iters + 2
+
+
+def _vel_case(orc, g, rng, inflow=None):
+    ug = np.zeros(g.ghost_vec_shape())
+    ug[:, 1:-1, 1:-1, 1:-1] = rng.standard_normal((3,) + g.cell_shape())
+    orc.fill_vel_ghosts(g, ug, inflow)
+    return ug
+
+
+def test_project_matches_oracle_many_boxes(ctx, orc):
+    n = (32, 24, 16)
+    _say(f"case project n={n}")
+    g = _geom(orc, n, prob_hi=(1.0, 0.75, 0.5))
+    rng = np.random.default_rng(3)
+    ug = _vel_case(orc, g, rng)
+    sigma = 0.01 * (1.0 + 0.5 * np.tanh(rng.standard_normal(g.cell_shape())))
+    bxs = B.chop_domain(n, 8)
+    P = _proj(ctx, n, bxs, prob_hi=(1.0, 0.75, 0.5))
+    P.set_sigma(ctx.array(B.to_boxed(sigma, bxs, 0)), 0)
+    vd = ctx.array(B.set_ghosts_from(np.zeros(B.field_size(bxs, 1, 3)), bxs, 1, ug))
+    info = P.project(vd, 1, rtol=1e-12)
+    phid = ctx.array(B.field_size(bxs, 0, 1, True))
+    gpd = ctx.array(B.field_size(bxs, 0, 3))
+    P.getPhi(phid, 0)
+    P.getGradPhi(gpd, 0)
+    u = B.from_boxed(vd.download(), bxs, 1, g.cell_shape(), ncomp=3)
+    phi = B.from_boxed(phid.download(), bxs, 0, g.node_shape(), nodal=True)
+    gp = B.from_boxed(gpd.download(), bxs, 0, g.cell_shape(), ncomp=3)
+    ugo = ug.copy()
+    phio, gpo, io, ret = orc.nodal_project(g, sigma, ugo, rtol=1e-12)
+    assert info.converged and ret == 0
+    assert _rel(phi, phio) < 1e-10
+    assert _rel(gp, gpo) < 1e-10
+    assert _rel(u, ugo[:, 1:-1, 1:-1, 1:-1]) < 1e-10
+    P.close()
+
+
+def _apply_ref(orc, g, u, rho, gp, s, inflow, incremental=False, u_old=None):
+    """incflo::ApplyNodalProjection composed from oracle pieces."""
+    ustar = u.copy()
+    if not incremental:
+        ustar += (s / rho) * gp
+    if u_old is not None:
+        ustar -= u_old
+    ug = np.zeros(g.ghost_vec_shape())
+    ug[:, 1:-1, 1:-1, 1:-1] = ustar
+    set_inflow = not incremental and u_old is None
+    orc.fill_vel_ghosts(g, ug, inflow if set_inflow else None)
+    sigma = s / rho
+    phi, gphi, info, ret = orc.nodal_project(g, sigma, ug, rtol=1e-12)
+    unew = ug[:, 1:-1, 1:-1, 1:-1].copy()
+    if u_old is not None:
+        unew += u_old
+    return unew, phi, gphi, info
+
+
+@pytest.mark.parametrize("incremental", [False, True])
+def test_apply_nodal_projection_channel(ctx, orc, incremental):
+    from incflo_amd import mgx
+    n, per = (32, 16, 16), (0, 0, 1)
+    bclo, bchi = (INF, N, 0), (D, N, 0)
+    ph = (2.0, 1.0, 1.0)
+    _say(f"case apply_nodal_projection incremental={incremental}")
+    g = _geom(orc, n, per, bclo, bchi, ph)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((3,) + g.cell_shape())
+    u_old = 0.3 * rng.standard_normal((3,) + g.cell_shape()) if incremental else None
+    rho = 1.0 + 0.5 * np.tanh(rng.standard_normal(g.cell_shape()))
+    gp0 = rng.standard_normal((3,) + g.cell_shape())
+    p0 = rng.standard_normal(g.node_shape())
+    inflow = np.zeros((6, 3)); inflow[0] = [1.0, 0.0, 0.0]
+    s = 0.01
+    bxs = B.chop_domain(n, (12, 16, 8))
+    P = _proj(ctx, n, bxs, per, bclo, bchi, ph)
+    vd = ctx.array(B.to_boxed(u, bxs, 1, periodic=per))
+    gpd = ctx.array(B.to_boxed(gp0, bxs, 0))
+    pd = ctx.array(B.to_boxed(p0, bxs, 0, nodal=True, periodic=per))
+    rd = ctx.array(B.to_boxed(rho, bxs, 1, periodic=per))
+    od = ctx.array(B.to_boxed(u_old, bxs, 0)) if incremental else None
+    info = P.apply_nodal_projection(vd, 1, gpd, 0, pd, 0, s, density=rd, rho_ng=1, vel_old=od,
+                                    vel_old_ng=0, incremental=incremental,
+                                    inflow_vel=None if incremental else inflow, rtol=1e-12,
+                                    atol=0.0, mode=mgx.MODE_FP64)
+    assert info.converged
+    unew = B.from_boxed(vd.download(), bxs, 1, g.cell_shape(), ncomp=3)
+    gpn = B.from_boxed(gpd.download(), bxs, 0, g.cell_shape(), ncomp=3)
+    pn = B.from_boxed(pd.download(), bxs, 0, g.node_shape(), nodal=True)
+    ur, phir, gphir, io = _apply_ref(orc, g, u, rho, gp0, s, inflow, incremental, u_old)
+    assert _rel(unew, ur) < 1e-10
+    assert _rel(gpn, gp0 + gphir if incremental else gphir) < 1e-10
+    assert _rel(pn, p0 + phir if incremental else phir) < 1e-10
+    P.close()
+
+
+def test_decomposition_invariance(ctx, orc):
+    """One box vs many uneven boxes give the same answer (test_periodic_godunov pattern)."""
+    from incflo_amd import mgx
+    n = (40, 24, 32)
+    _say("case decomposition invariance")
+    g = _geom(orc, n)
+    rng = np.random.default_rng(9)
+    ug = _vel_case(orc, g, rng)
+    sigma = 1.0 + 0.5 * np.tanh(rng.standard_normal(g.cell_shape()))
+    outs = []
+    for bxs in (B.chop_domain(n, 64), B.chop_domain(n, (9, 7, 11))):
+        P = _proj(ctx, n, bxs)
+        P.set_sigma(ctx.array(B.to_boxed(sigma, bxs, 0)), 0)
+        vd = ctx.array(B.set_ghosts_from(np.zeros(B.field_size(bxs, 1, 3)), bxs, 1, ug))
+        P.project(vd, 1, rtol=1e-12)
+        phid = ctx.array(B.field_size(bxs, 0, 1, True))
+        P.getPhi(phid, 0)
+        outs.append((B.from_boxed(vd.download(), bxs, 1, g.cell_shape(), ncomp=3),
+                     B.from_boxed(phid.download(), bxs, 0, g.node_shape(), nodal=True)))
+        P.close()
+    assert _rel(outs[1][0], outs[0][0]) < 1e-13
+    assert _rel(outs[1][1], outs[0][1]) < 1e-13
+
+
+def test_error_paths(ctx):
+    from incflo_amd import mgx
+    _say("case error paths")
+    bxs = B.chop_domain((16, 16, 16), 16)
+    P = _proj(ctx, (16, 16, 16), bxs, periodic=(0, 1, 1))
+    with pytest.raises(mgx.MgxError) as e:
+        P.setDomainBC((mgx.BC_ROBIN, 0, 0), (D, 0, 0))
+    assert e.value.code == mgx.MGX_ERR_UNSUPPORTED
+    with pytest.raises(mgx.MgxError) as e:
+        P.setDomainBC((N, N, 0), (D, 0, 0))  # periodic mismatch in y
+    assert e.value.code == mgx.MGX_ERR_ARG
+    with pytest.raises(mgx.MgxError) as e:
+        P.set_const_sigma(1.0)  # setDomainBC is required on a non-periodic domain
+    assert e.value.code == mgx.MGX_ERR_STATE
+    P.setDomainBC((N, 0, 0), (D, 0, 0))
+    P.set_const_sigma(1.0)
+    P.set_options(mgx.default_opts(maxiter=1))
+    rng = np.random.default_rng(1)
+    rd = ctx.array(rng.standard_normal(B.field_size(bxs, 0, 1, True)))
+    xd = ctx.array(B.field_size(bxs, 0, 1, True))
+    with pytest.raises(mgx.MgxError) as e:
+        P.solve(xd, 0, rd, 0, rtol=1e-12)
+    assert e.value.code == mgx.MGX_ERR_NOT_CONVERGED
+    P.close()
+    with pytest.raises(mgx.MgxError) as e:
+        mgx.NodalProjector(ctx, mgx.make_level((16, 16, 16), [((0, 0, 0), (15, 15, 20))]))
+    assert e.value.code == mgx.MGX_ERR_ARG
