@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -106,6 +107,21 @@ inline NodalGeom coarsen(const NodalGeom& f) {
         c.sg[d] = c.n[d] + 2;
     }
     return c;
+}
+
+// Weighted-Jacobi factor for a level: omega is meant for isotropic cells, where
+// the Jacobi-preconditioned constant-sigma Q1 symbol peaks at 3/2; on
+// anisotropic cells the peak (the symbol at the corners of [0,pi]^3) is
+//   lmax = max(4 max ih2, 4/3 max pair sum, 4/9 S) / (8/9 S)
+// and omega is scaled by min(1, 1.5/lmax).
+inline double nodal_omega(const NodalGeom& g, double omega) {
+    double ih2[3], S = 0.0;
+    for (int d = 0; d < 3; ++d) { ih2[d] = 1.0 / (g.dx[d] * g.dx[d]); S += ih2[d]; }
+    const double m1 = std::max(ih2[0], std::max(ih2[1], ih2[2]));
+    const double m2 = std::max(ih2[0] + ih2[1], std::max(ih2[0] + ih2[2], ih2[1] + ih2[2]));
+    const double lmax =
+        std::max(4.0 * m1, std::max(4.0 / 3.0 * m2, 4.0 / 9.0 * S)) / (8.0 / 9.0 * S);
+    return omega * std::min(1.0, 1.5 / lmax);
 }
 
 // Coarsen while every cell count is even and the coarse count stays >= 4.

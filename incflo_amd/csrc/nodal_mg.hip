@@ -127,7 +127,7 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
             return;
         }
         nodal_jacobi_first_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.kg, lv.coef, lv.b, lv.sig,
-                                                                lv.x, (T)o.omega);
+                                                                lv.x, (T)nodal_omega(lv.g, o.omega));
         --sweeps;
     }
     for (int q = 0; q < sweeps; ++q) {
@@ -138,7 +138,7 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         a.b = lv.b;
         a.sig = lv.sig;
         a.xout = lv.x2;
-        a.omega = (T)o.omega;
+        a.omega = (T)nodal_omega(lv.g, o.omega);
         if (l == 0 && timer) timer->begin(s);
         launch_stencil<T, T, KS_JACOBI>(lv.g, a, s, l == 0);
         if (l == 0 && timer) timer->end(s);

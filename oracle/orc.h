@@ -74,6 +74,7 @@ void orc_nodal_restrict(const orc_geom* gf, const double* rf, double* rc);
 void orc_nodal_prolong_add(const orc_geom* gf, const double* xc, double* xf);
 void orc_sigma_coarsen(const orc_geom* gf, const double* sf, double* sc);
 int  orc_nodal_nlevels(const orc_geom* g, int max_coarsening_level);
+double orc_nodal_omega(const orc_geom* g, double omega);
 
 /* D.u (nodal, from cell velocity with one ghost layer) and G.phi (cell). */
 void orc_fill_vel_ghosts(const orc_geom* g, double* ug, const double* inflow_vel /*6x3 or NULL*/);

@@ -324,6 +324,22 @@ void orc_sigma_coarsen(const orc_geom* gf, const double* sf, double* sc)
             }
 }
 
+/* Weighted-Jacobi factor for a level: the user's omega is meant for isotropic
+ * cells, where the Jacobi-preconditioned constant-sigma symbol peaks at 3/2.
+ * On anisotropic cells the peak is
+ *   lmax = max(4 max_d ih2_d, 4/3 max_{d<e}(ih2_d+ih2_e), 4/9 S) / (8/9 S),
+ * (the symbol's values at the corners of [0,pi]^3), so omega is scaled by
+ * min(1, 3/2 / lmax) to keep |1 - omega lambda| < 1. */
+double orc_nodal_omega(const orc_geom* g, double omega)
+{
+    double ih2[3], S = 0.0;
+    for (int d = 0; d < 3; ++d) { ih2[d] = 1.0 / (g->dx[d] * g->dx[d]); S += ih2[d]; }
+    double m1 = fmax(ih2[0], fmax(ih2[1], ih2[2]));
+    double m2 = fmax(ih2[0] + ih2[1], fmax(ih2[0] + ih2[2], ih2[1] + ih2[2]));
+    double lmax = fmax(4.0 * m1, fmax(4.0 / 3.0 * m2, 4.0 / 9.0 * S)) / (8.0 / 9.0 * S);
+    return omega * fmin(1.0, 1.5 / lmax);
+}
+
 /* A level is coarsened while every cell count is even and the coarse count is >= 4. */
 int orc_nodal_nlevels(const orc_geom* g, int max_coarsening_level)
 {
@@ -519,7 +535,7 @@ static void vcycle(level_t* L, int lev, int nlev, const orc_mg_opts* o, int sing
     }
     memset(l->x, 0, sizeof(double) * l->nn);
     for (int s = 0; s < o->nu1; ++s) {
-        orc_nodal_jacobi(&l->g, l->sigma, l->b, l->x, l->x2, o->omega);
+        orc_nodal_jacobi(&l->g, l->sigma, l->b, l->x, l->x2, orc_nodal_omega(&l->g, o->omega));
         double* t = l->x; l->x = l->x2; l->x2 = t;
     }
     orc_nodal_residual(&l->g, l->sigma, l->b, l->x, l->r);
@@ -527,7 +543,7 @@ static void vcycle(level_t* L, int lev, int nlev, const orc_mg_opts* o, int sing
     vcycle(L, lev + 1, nlev, o, singular, bot_its);
     orc_nodal_prolong_add(&l->g, L[lev + 1].x, l->x);
     for (int s = 0; s < o->nu2; ++s) {
-        orc_nodal_jacobi(&l->g, l->sigma, l->b, l->x, l->x2, o->omega);
+        orc_nodal_jacobi(&l->g, l->sigma, l->b, l->x, l->x2, orc_nodal_omega(&l->g, o->omega));
         double* t = l->x; l->x = l->x2; l->x2 = t;
     }
 }
