@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cache", action="store_true",
+                    help="keep the generated input in $TMPDIR between runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU "
@@ -103,8 +105,16 @@ def main():
     from incflo_amd import workloads as W
 
     n = args.n
-    log(f"bench: generating {n}^3 variable-density case seed={args.seed}")
-    rho, ustar, dt = W.variable_density_case(n, args.seed)
+    cache = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"mgx_case_{n}_{args.seed}.npz")
+    if args.cache and os.path.exists(cache):
+        z = np.load(cache)
+        rho, ustar, dt = z["rho"], z["ustar"], float(z["dt"])
+        log(f"bench: loaded {cache}")
+    else:
+        log(f"bench: generating {n}^3 variable-density case seed={args.seed}")
+        rho, ustar, dt = W.variable_density_case(n, args.seed)
+        if args.cache:
+            np.savez(cache, rho=rho, ustar=ustar, dt=dt)
     bxs = B.chop_domain((n, n, n), args.max_grid_size)
     ctx = mgx.Context(0)
     P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))

@@ -52,12 +52,13 @@ __device__ __forceinline__ bool tile_point(const BoxInfo* boxes, const Tile* til
         if (region == REG_CELL_GROWN) { lo[d] = b.rlo[d]; len[d] = b.rlen[d]; }
         else { lo[d] = b.lo[d]; len[d] = b.len[d] + (region == REG_NODE ? 1 : 0); }
     }
-    const long lin = (long)t.start + threadIdx.x;
-    const long vol = (long)len[0] * len[1] * len[2];
+    const int lin = t.start + (int)threadIdx.x;
+    const int vol = len[0] * len[1] * len[2];
     if (lin >= vol) return false;
-    i = lo[0] + (int)(lin % len[0]);
-    j = lo[1] + (int)((lin / len[0]) % len[1]);
-    k = lo[2] + (int)(lin / ((long)len[0] * len[1]));
+    const int q = lin / len[0];
+    i = lo[0] + (lin - q * len[0]);
+    j = lo[1] + q % len[1];
+    k = lo[2] + q / len[1];
     return true;
 }
 
@@ -143,12 +144,9 @@ static __global__ void __launch_bounds__(256) gather_vel_kernel(GatherVelArgs a)
 static __global__ void __launch_bounds__(256) nodal_rhs_kernel(KGeom g, const double* __restrict__ ug,
                                                         double* __restrict__ rhs, double fx,
                                                         double fy, double fz) {
-    const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int i = id % g.nn[0];
-    const int j = (id / g.nn[0]) % g.nn[1];
-    const int k = id / ((long)g.nn[0] * g.nn[1]);
+    int i, j, k;
+    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
     const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
     double sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll

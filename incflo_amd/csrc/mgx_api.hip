@@ -11,12 +11,8 @@ using namespace mgx;
 namespace {
 
 __global__ void fill_sigma_const_kernel(KGeom g, double v, double* sig) {
-    const long n = (long)g.n[0] * g.n[1] * g.n[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int i = id % g.n[0];
-    const int j = (id / g.n[0]) % g.n[1];
-    const int k = id / ((long)g.n[0] * g.n[1]);
+    int i, j, k;
+    if (!grid_point(g.n[0], g.n[1], i, j, k)) return;
     write_cell_images(sig, g, i, j, k, v);
 }
 
@@ -57,7 +53,7 @@ struct mgx_nodal_s {
     hipStream_t stream() const { return ctx->stream; }
     void compute_rhs() {
         const KGeom kg = kgeom(g);
-        nodal_rhs_kernel<<<grid1d(g.nnode()), 256, 0, stream()>>>(
+        nodal_rhs_kernel<<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, stream()>>>(
             kg, ug.as<double>(), rhs.as<double>(), 0.25 / g.dx[0], 0.25 / g.dx[1], 0.25 / g.dx[2]);
     }
     void zero_phi() {
@@ -310,7 +306,7 @@ MGX_API int mgx_nodal_set_const_sigma(mgx_nodal h, double v) {
         MGX_REQUIRE(v > 0.0, MGX_ERR_ARG, "sigma must be positive");
         h->ensure_setup();
         const long nc = (long)h->g.n[0] * h->g.n[1] * h->g.n[2];
-        fill_sigma_const_kernel<<<grid1d(nc), 256, 0, h->stream()>>>(kgeom(h->g), v,
+        fill_sigma_const_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(kgeom(h->g), v,
                                                                      h->solver.sigma0());
         h->solver.sigma_changed();
     });

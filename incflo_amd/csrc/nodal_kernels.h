@@ -73,6 +73,18 @@ __device__ __forceinline__ int xcd_remap(int h, int nblocks) {
     return (h & 7) * per + (h >> 3);
 }
 
+// Point kernels run on a 3-D grid of 64 x 4 thread blocks (x, y) and one z
+// plane per block row, so no thread decodes a linear index with divisions.
+__device__ __forceinline__ bool grid_point(int nx, int ny, int& i, int& j, int& k) {
+    i = blockIdx.x * 64 + threadIdx.x;
+    j = blockIdx.y * 4 + threadIdx.y;
+    k = blockIdx.z;
+    return i < nx && j < ny;
+}
+
+inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + 63) / 64, (ny + 3) / 4, nz); }
+inline dim3 block3() { return dim3(64, 4, 1); }
+
 // atomic max for non-negative doubles via their bit pattern
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
@@ -93,7 +105,9 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return red[0];
 }
 
-enum { KS_JACOBI = 0, KS_RESID = 1, KS_UPDRES = 2, KS_APPLY = 3 };
+// JACOBI: x_out = x + w (b - Lx)/diag   RESID: r = b - Lx   UPDRES: x_out = x + cor,
+// r = b - L x_out   APPLY: r = Lx   JFIRST: x_out = w b/diag (first sweep from zero)
+enum { KS_JACOBI = 0, KS_RESID = 1, KS_UPDRES = 2, KS_APPLY = 3, KS_JFIRST = 4 };
 
 template <class T, class TC>
 struct SArgs {
@@ -111,134 +125,217 @@ struct SArgs {
     unsigned long long* norm;  // optional max |r|
 };
 
-// One 27-point sweep over a TX x TY column tile, streaming along z: each
-// thread keeps its 3x3x3 phi neighbourhood and 2x2x2 sigma neighbourhood in
-// registers and receives one new phi plane and one sigma plane per step
-// through LDS (double buffered, one barrier per plane).
+// One 27-point pass over a 64 x 8 node tile, streaming along z.  A workgroup
+// is 64 x 4 threads; thread (tx,ty) owns nodes (i0+tx, j0+2ty) and
+// (i0+tx, j0+2ty+1).  Each thread keeps the 4 x 3 phi rows and 3 x 2 sigma
+// rows of its two nodes for three (phi) / two (sigma) planes in registers and
+// receives one new plane of each per step through LDS (double buffered, one
+// barrier per plane).  The next plane's global loads (and the rhs of the
+// plane being computed) are issued one step ahead, before the barrier, so
+// every wave keeps a full plane of loads in flight while it computes.  All
+// loads are unconditional (clamped indices) so the wait counts stay exact.
 // FINE only separates the finest-level instantiation (a distinct kernel name
 // in profiles); the code is identical.
-template <class T, class TC, int MODE, bool ISO, int TX, int TY, bool FINE>
-static __global__ void __launch_bounds__(TX* TY) nodal_stencil_kernel(SArgs<T, TC> a) {
-    __shared__ T sphi[2][TY + 2][TX + 2];
-    __shared__ T ssig[2][TY + 1][TX + 1];
-    __shared__ double red[(TX * TY) / 64];
+template <class T, bool ISO, int RO, int NPR, int NSR>
+__device__ __forceinline__ T apply27(const T (&P)[3][NPR][3], const T (&S)[2][NSR][2],
+                                     const Coef<T>& cf, T& ssum) {
+    const T c = P[1][RO + 1][1];
+    T acc = 0;
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) {
+            const T lo = S[0][RO + bb][aa], hi = S[1][RO + bb][aa];
+            // corners (one cell each)
+            acc += cf.corner * (lo * (P[0][RO + 2 * bb][2 * aa] - c) + hi * (P[2][RO + 2 * bb][2 * aa] - c));
+            // xy edges (dz = 0): the cell below and above
+            acc += cf.edge[2] * (lo + hi) * (P[1][RO + 2 * bb][2 * aa] - c);
+        }
+#pragma unroll
+    for (int aa = 0; aa < 2; ++aa) {
+        // xz edges (dy = 0) and yz edges (dx = 0)
+        acc += cf.edge[1] * ((S[0][RO][aa] + S[0][RO + 1][aa]) * (P[0][RO + 1][2 * aa] - c) +
+                             (S[1][RO][aa] + S[1][RO + 1][aa]) * (P[2][RO + 1][2 * aa] - c));
+        acc += cf.edge[0] * ((S[0][RO + aa][0] + S[0][RO + aa][1]) * (P[0][RO + 2 * aa][1] - c) +
+                             (S[1][RO + aa][0] + S[1][RO + aa][1]) * (P[2][RO + 2 * aa][1] - c));
+    }
+    const T s0 = S[0][RO][0] + S[0][RO][1] + S[0][RO + 1][0] + S[0][RO + 1][1];
+    const T s1 = S[1][RO][0] + S[1][RO][1] + S[1][RO + 1][0] + S[1][RO + 1][1];
+    if (!ISO) {
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) {
+            acc += cf.face[0] * (S[0][RO][aa] + S[0][RO + 1][aa] + S[1][RO][aa] + S[1][RO + 1][aa]) *
+                   (P[1][RO + 1][2 * aa] - c);
+            acc += cf.face[1] * (S[0][RO + aa][0] + S[0][RO + aa][1] + S[1][RO + aa][0] + S[1][RO + aa][1]) *
+                   (P[1][RO + 2 * aa][1] - c);
+        }
+        acc += cf.face[2] * (s0 * (P[0][RO + 1][1] - c) + s1 * (P[2][RO + 1][1] - c));
+    }
+    ssum = s0 + s1;
+    return acc;
+}
+
+template <class T, class TC, int MODE, bool ISO, bool FINE, int NPT>
+static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T, TC> a) {
+    constexpr int TX = 64, NY = 8, TYT = NY / NPT;   // threads per row, tile rows, thread rows
+    constexpr int NT = TX * TYT;
+    constexpr int PR = NY + 2, PC = TX + 2;          // phi tile incl. halo
+    constexpr int SR = NY + 1, SC = TX + 1;          // sigma tile
+    constexpr int NQP = (PR + TYT - 1) / TYT, NQS = (SR + TYT - 1) / TYT;
+    __shared__ T sphi[2][PR][PC];
+    __shared__ T ssig[2][SR][SC];
+    __shared__ double red[NT / 64];
     const KGeom& g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
     const int nb = a.gx * a.gy * a.gz;
-    const int h = blockIdx.x;
-    const int lb = xcd_remap(h, nb);
+    const int lb = xcd_remap(blockIdx.x, nb);
     const int bx = lb % a.gx, by = (lb / a.gx) % a.gy, bz = lb / (a.gx * a.gy);
-    const int i0 = bx * TX, j0 = by * TY, k0 = bz * a.kz;
+    const int i0 = bx * TX, j0 = by * NY, k0 = bz * a.kz;
     const int k1 = min(k0 + a.kz, g.nn[2]);
-    const int i = i0 + tx, j = j0 + ty;
-    const bool active = (i < g.nn[0]) && (j < g.nn[1]);
+    const int i = i0 + tx, jr = j0 + NPT * ty;   // first node row of this thread
     const long sxy = (long)g.nn[0] * g.nn[1];
     const long ssxy = (long)g.sg[0] * g.sg[1];
 
-    T pm[3][3], p0[3][3], pp[3][3];
-    T slo[2][2], shi[2][2];
+    // fixed per-thread load slots (indices clamped; stores to LDS predicated)
+    const int gi = wrapc(i, g.nn[0], g.per[0]);
+    int prow[NQP];
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < NQP; ++q) prow[q] = wrapc(j0 - 1 + min(ty + TYT * q, PR - 1), g.nn[1], g.per[1]);
+    const int hr = min(tid >> 1, PR - 1);
+    const int hcol = wrapc((tid & 1) ? i0 + TX : i0 - 1, g.nn[0], g.per[0]);
+    const int hrow = wrapc(j0 - 1 + hr, g.nn[1], g.per[1]);
+    const bool hst = tid < 2 * PR;
+    const int si = min(i0 + tx, g.sg[0] - 1);
+    int srow[NQS];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) { pm[q][r] = 0; p0[q][r] = 0; pp[q][r] = 0; }
+    for (int q = 0; q < NQS; ++q) srow[q] = min(j0 + min(ty + TYT * q, SR - 1), g.sg[1] - 1);
+    const int sxr = min(j0 + min(tid, SR - 1), g.sg[1] - 1);
+    const int sxc = min(i0 + TX, g.sg[0] - 1);
+    const bool sxst = tid < SR;
+    long bid[NPT];
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int r = 0; r < 2; ++r) { slo[q][r] = 0; shi[q][r] = 0; }
-    double rmax = 0.0;
+    for (int n = 0; n < NPT; ++n)
+        bid[n] = (long)min(i, g.nn[0] - 1) + (long)g.nn[0] * min(jr + n, g.nn[1] - 1);
 
-    for (int kk = k0 - 1; kk <= k1; ++kk) {
-        const int buf = kk & 1;
-        const int kw = wrapc(kk, g.nn[2], g.per[2]);
-        for (int e = tid; e < (TX + 2) * (TY + 2); e += TX * TY) {
-            const int lx = e % (TX + 2), ly = e / (TX + 2);
-            const int gi = wrapc(i0 - 1 + lx, g.nn[0], g.per[0]);
-            const int gj = wrapc(j0 - 1 + ly, g.nn[1], g.per[1]);
-            const long id = gi + (long)g.nn[0] * gj + sxy * kw;
+    T lp[NQP], lh, ls[NQS], lsx, lb_[NPT];
+#pragma unroll
+    for (int n = 0; n < NPT; ++n) lb_[n] = 0;
+    auto issue = [&](int kk, int kb) {
+        const long kw = (long)wrapc(kk, g.nn[2], g.per[2]) * sxy;
+        const long ks = (long)min(max(kk, 0), g.sg[2] - 1) * ssxy;
+        if (MODE != KS_JFIRST) {
+#pragma unroll
+            for (int q = 0; q < NQP; ++q) {
+                const long id = gi + (long)g.nn[0] * prow[q] + kw;
+                T v = a.x[id];
+                if (MODE == KS_UPDRES) v += (T)a.cor[id];
+                lp[q] = v;
+            }
+            const long id = hcol + (long)g.nn[0] * hrow + kw;
             T v = a.x[id];
             if (MODE == KS_UPDRES) v += (T)a.cor[id];
-            sphi[buf][ly][lx] = v;
+            lh = v;
         }
-        const int ks = min(max(kk, 0), g.sg[2] - 1);
-        for (int e = tid; e < (TX + 1) * (TY + 1); e += TX * TY) {
-            const int lx = e % (TX + 1), ly = e / (TX + 1);
-            const int si = min(i0 + lx, g.sg[0] - 1), sj = min(j0 + ly, g.sg[1] - 1);
-            ssig[buf][ly][lx] = a.sig[si + (long)g.sg[0] * sj + ssxy * ks];
+#pragma unroll
+        for (int q = 0; q < NQS; ++q) ls[q] = a.sig[si + (long)g.sg[0] * srow[q] + ks];
+        lsx = a.sig[sxc + (long)g.sg[0] * sxr + ks];
+        if (MODE != KS_APPLY) {
+            const long kbo = (long)min(max(kb, 0), g.nn[2] - 1) * sxy;
+#pragma unroll
+            for (int n = 0; n < NPT; ++n) lb_[n] = a.b[bid[n] + kbo];
         }
+    };
+
+    T P[3][NPT + 2][3], S[2][NPT + 1][2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < NPT + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[p][r][c] = 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < NPT + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) S[p][r][c] = 0;
+    double rmax = 0.0;
+
+    issue(k0 - 1, k0 - 1);
+    for (int kk = k0 - 1; kk <= k1; ++kk) {
+        const int buf = kk & 1;
+        if (MODE != KS_JFIRST) {
+#pragma unroll
+            for (int q = 0; q < NQP; ++q)
+                if (ty + TYT * q < PR) sphi[buf][ty + TYT * q][tx + 1] = lp[q];
+            if (hst) sphi[buf][hr][(tid & 1) ? PC - 1 : 0] = lh;
+        }
+#pragma unroll
+        for (int q = 0; q < NQS; ++q)
+            if (ty + TYT * q < SR) ssig[buf][ty + TYT * q][tx] = ls[q];
+        if (sxst) ssig[buf][min(tid, SR - 1)][SC - 1] = lsx;
+        T bcur[NPT];  // rhs of plane kk-1
+#pragma unroll
+        for (int n = 0; n < NPT; ++n) bcur[n] = lb_[n];
+        if (kk < k1) issue(kk + 1, kk);
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
+        for (int r = 0; r < NPT + 2; ++r)
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                pm[q][r] = p0[q][r];
-                p0[q][r] = pp[q][r];
-                pp[q][r] = sphi[buf][ty + q][tx + r];
+            for (int c = 0; c < 3; ++c) {
+                P[0][r][c] = P[1][r][c];
+                P[1][r][c] = P[2][r][c];
+                P[2][r][c] = (MODE == KS_JFIRST) ? (T)0 : sphi[buf][NPT * ty + r][tx + c];
             }
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+        for (int r = 0; r < NPT + 1; ++r)
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                slo[q][r] = shi[q][r];
-                shi[q][r] = ssig[buf][ty + q][tx + r];
+            for (int c = 0; c < 2; ++c) {
+                S[0][r][c] = S[1][r][c];
+                S[1][r][c] = ssig[buf][NPT * ty + r][tx + c];
             }
-        const int k = kk - 1;
-        if (kk >= k0 + 1 && active) {
-            const T c = p0[1][1];
-            T acc = 0;
-            // corners
+        if (kk >= k0 + 1) {
+            const int k = kk - 1;
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
+            for (int n = 0; n < NPT; ++n) {
+                const int j = jr + n;
+                if (!(i < g.nn[0] && j < g.nn[1])) continue;
+                if (MODE == KS_JFIRST) {
+                    T ss = 0;
 #pragma unroll
-                for (int aa = 0; aa < 2; ++aa)
-                    acc += a.c.corner * (slo[bb][aa] * (pm[2 * bb][2 * aa] - c) +
-                                         shi[bb][aa] * (pp[2 * bb][2 * aa] - c));
-            // xy edges (dz = 0)
+                    for (int p = 0; p < 2; ++p)
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
-#pragma unroll
-                for (int aa = 0; aa < 2; ++aa)
-                    acc += a.c.edge[2] * (slo[bb][aa] + shi[bb][aa]) * (p0[2 * bb][2 * aa] - c);
-            // xz edges (dy = 0) and yz edges (dx = 0)
-#pragma unroll
-            for (int aa = 0; aa < 2; ++aa) {
-                acc += a.c.edge[1] * ((slo[0][aa] + slo[1][aa]) * (pm[1][2 * aa] - c) +
-                                      (shi[0][aa] + shi[1][aa]) * (pp[1][2 * aa] - c));
-                acc += a.c.edge[0] * ((slo[aa][0] + slo[aa][1]) * (pm[2 * aa][1] - c) +
-                                      (shi[aa][0] + shi[aa][1]) * (pp[2 * aa][1] - c));
-            }
-            if (!ISO) {
-#pragma unroll
-                for (int aa = 0; aa < 2; ++aa) {
-                    acc += a.c.face[0] * (slo[0][aa] + slo[1][aa] + shi[0][aa] + shi[1][aa]) *
-                           (p0[1][2 * aa] - c);
-                    acc += a.c.face[1] * (slo[aa][0] + slo[aa][1] + shi[aa][0] + shi[aa][1]) *
-                           (p0[2 * aa][1] - c);
+                        for (int r = 0; r < 2; ++r) ss += S[p][n + r][0] + S[p][n + r][1];
+                    const long id = i + (long)g.nn[0] * j + sxy * k;
+                    a.xout[id] = (is_dir(g, i, j, k) || ss == (T)0)
+                                     ? (T)0 : a.omega * bcur[n] / (-a.c.diag4 * ss);
+                    continue;
                 }
-                acc += a.c.face[2] * ((slo[0][0] + slo[0][1] + slo[1][0] + slo[1][1]) * (pm[1][1] - c) +
-                                      (shi[0][0] + shi[0][1] + shi[1][0] + shi[1][1]) * (pp[1][1] - c));
-            }
-            const long id = i + (long)g.nn[0] * j + sxy * k;
-            const bool dir = is_dir(g, i, j, k);
-            if (MODE == KS_JACOBI) {
-                const T ssum = slo[0][0] + slo[0][1] + slo[1][0] + slo[1][1] + shi[0][0] +
-                               shi[0][1] + shi[1][0] + shi[1][1];
-                const T diag = -a.c.diag4 * ssum;
-                T v = c;
-                if (dir) v = 0;
-                else if (ssum != (T)0) v = c + a.omega * (a.b[id] - acc) / diag;
-                a.xout[id] = v;
-            } else if (MODE == KS_APPLY) {
-                a.rout[id] = (TC)acc;
-            } else {
-                const T r = dir ? (T)0 : (a.b[id] - acc);
-                a.rout[id] = (TC)r;
-                if (MODE == KS_UPDRES) a.xout[id] = dir ? (T)0 : c;
-                rmax = fmax(rmax, (double)fabs(r));
+                T ssum;
+                const T acc = (n == 0) ? apply27<T, ISO, 0>(P, S, a.c, ssum)
+                                       : apply27<T, ISO, (NPT > 1 ? 1 : 0)>(P, S, a.c, ssum);
+                const T c = P[1][n + 1][1];
+                const long id = i + (long)g.nn[0] * j + sxy * k;
+                const bool dir = is_dir(g, i, j, k);
+                const T bv = bcur[n];
+                if (MODE == KS_JACOBI) {
+                    T v = c;
+                    if (dir) v = 0;
+                    else if (ssum != (T)0) v = c + a.omega * (bv - acc) / (-a.c.diag4 * ssum);
+                    a.xout[id] = v;
+                } else if (MODE == KS_APPLY) {
+                    a.rout[id] = (TC)acc;
+                } else {
+                    const T r = dir ? (T)0 : (bv - acc);
+                    a.rout[id] = (TC)r;
+                    if (MODE == KS_UPDRES) a.xout[id] = dir ? (T)0 : c;
+                    rmax = fmax(rmax, (double)fabs(r));
+                }
             }
         }
     }
     if ((MODE == KS_RESID || MODE == KS_UPDRES) && a.norm) {
-        const double m = block_max<TX * TY>(rmax, red);
+        const double m = block_max<NT>(rmax, red);
         if (tid == 0) atomic_max_nonneg(a.norm, m);
     }
 }
@@ -247,12 +344,9 @@ static __global__ void __launch_bounds__(TX* TY) nodal_stencil_kernel(SArgs<T, T
 template <class T>
 static __global__ void nodal_jacobi_first_kernel(KGeom g, Coef<T> c, const T* __restrict__ b,
                                           const T* __restrict__ sig, T* __restrict__ x, T omega) {
-    const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int i = id % g.nn[0];
-    const int j = (id / g.nn[0]) % g.nn[1];
-    const int k = id / ((long)g.nn[0] * g.nn[1]);
+    int i, j, k;
+    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
     if (is_dir(g, i, j, k)) { x[id] = 0; return; }
     T s = 0;
 #pragma unroll
@@ -267,12 +361,9 @@ static __global__ void nodal_jacobi_first_kernel(KGeom g, Coef<T> c, const T* __
 template <class T>
 static __global__ void nodal_restrict_kernel(KGeom gf, KGeom gc, const T* __restrict__ rf,
                                       T* __restrict__ rc) {
-    const long n = (long)gc.nn[0] * gc.nn[1] * gc.nn[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int I = id % gc.nn[0];
-    const int J = (id / gc.nn[0]) % gc.nn[1];
-    const int K = id / ((long)gc.nn[0] * gc.nn[1]);
+    int I, J, K;
+    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (is_dir(gc, I, J, K)) { rc[id] = 0; return; }
     T acc = 0;
 #pragma unroll
@@ -304,12 +395,9 @@ static __global__ void nodal_restrict_kernel(KGeom gf, KGeom gc, const T* __rest
 template <class T>
 static __global__ void nodal_prolong_add_kernel(KGeom gf, KGeom gc, const T* __restrict__ xc,
                                          T* __restrict__ xf) {
-    const long n = (long)gf.nn[0] * gf.nn[1] * gf.nn[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int i = id % gf.nn[0];
-    const int j = (id / gf.nn[0]) % gf.nn[1];
-    const int k = id / ((long)gf.nn[0] * gf.nn[1]);
+    int i, j, k;
+    if (!grid_point(gf.nn[0], gf.nn[1], i, j, k)) return;
+    const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
     if (is_dir(gf, i, j, k)) return;
     int lo[3], hi[3];
     const int f[3] = {i, j, k};
@@ -353,12 +441,8 @@ __device__ __forceinline__ void write_cell_images(T* arr, const KGeom& g, int ci
 // coarse sigma = mean of the 8 fine cells
 template <class T>
 static __global__ void sigma_coarsen_kernel(KGeom gf, KGeom gc, const T* __restrict__ sf, T* sc) {
-    const long n = (long)gc.n[0] * gc.n[1] * gc.n[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int I = id % gc.n[0];
-    const int J = (id / gc.n[0]) % gc.n[1];
-    const int K = id / ((long)gc.n[0] * gc.n[1]);
+    int I, J, K;
+    if (!grid_point(gc.n[0], gc.n[1], I, J, K)) return;
     T s = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -417,6 +501,68 @@ static __global__ void __launch_bounds__(256) final_sum_kernel(const double* __r
     if (threadIdx.x == 0) out[0] = red[0] * scale;
 }
 
+// per-block sum / min / max (part[0..np), part[np..2np), part[2np..3np))
+static __global__ void __launch_bounds__(256) stats_kernel(const double* __restrict__ a, long n,
+                                                           double* __restrict__ part, int np) {
+    __shared__ double rs[256], rmn[256], rmx[256];
+    double s = 0.0, mn = 1e300, mx = -1e300;
+    for (long id = (long)blockIdx.x * 256 + threadIdx.x; id < n; id += (long)gridDim.x * 256) {
+        const double v = a[id];
+        s += v;
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+    }
+    rs[threadIdx.x] = s;
+    rmn[threadIdx.x] = mn;
+    rmx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            rs[threadIdx.x] += rs[threadIdx.x + o];
+            rmn[threadIdx.x] = fmin(rmn[threadIdx.x], rmn[threadIdx.x + o]);
+            rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = rs[0];
+        part[np + blockIdx.x] = rmn[0];
+        part[2 * np + blockIdx.x] = rmx[0];
+    }
+}
+
+// mean (written to *mean when singular, else 0) and ||a - mean||_inf (as bits in *norm)
+static __global__ void __launch_bounds__(256) final_stats_kernel(const double* __restrict__ part,
+                                                                 int np, long n, int singular,
+                                                                 double* mean,
+                                                                 unsigned long long* norm) {
+    __shared__ double rs[256], rmn[256], rmx[256];
+    double s = 0.0, mn = 1e300, mx = -1e300;
+    for (int id = threadIdx.x; id < np; id += 256) {
+        s += part[id];
+        mn = fmin(mn, part[np + id]);
+        mx = fmax(mx, part[2 * np + id]);
+    }
+    rs[threadIdx.x] = s;
+    rmn[threadIdx.x] = mn;
+    rmx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            rs[threadIdx.x] += rs[threadIdx.x + o];
+            rmn[threadIdx.x] = fmin(rmn[threadIdx.x], rmn[threadIdx.x + o]);
+            rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = singular ? rs[0] / (double)n : 0.0;
+        *mean = m;
+        const double bn = fmax(rmx[0] - m, m - rmn[0]);
+        *norm = (unsigned long long)__double_as_longlong(fmax(bn, 0.0));
+    }
+}
+
 template <class T>
 static __global__ void subtract_scalar_kernel(T* __restrict__ a, long n, const double* __restrict__ s) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,13 +582,9 @@ static __global__ void __launch_bounds__(256) max_abs_kernel(const T* __restrict
 
 template <class T>
 static __global__ void zero_dirichlet_kernel(KGeom g, T* __restrict__ a) {
-    const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
-    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const int i = id % g.nn[0];
-    const int j = (id / g.nn[0]) % g.nn[1];
-    const int k = id / ((long)g.nn[0] * g.nn[1]);
-    if (is_dir(g, i, j, k)) a[id] = 0;
+    int i, j, k;
+    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    if (is_dir(g, i, j, k)) a[i + (long)g.nn[0] * (j + (long)g.nn[1] * k)] = 0;
 }
 
 // ---- bottom solve: BiCGStab (van der Vorst 1992) in one workgroup, FP64 ----
@@ -471,45 +613,44 @@ __device__ __forceinline__ double bmax(double v, double* red) {
 }
 
 // (L x)_n with direct loads, in double; Dirichlet rows masked to 0.
+// (L x)_n with direct loads of the 3x3x3 neighbourhood, in TO precision
+// (same unrolled arithmetic as the streaming kernel).
+template <class TO, class TS, class TX>
+__device__ __forceinline__ TO apply_node_nbhd(const KGeom& g, const Coef<TO>& c, const TS* sig,
+                                              const TX* x, int i, int j, int k, TO& ssum) {
+    TO P[3][3][3], S[2][2][2];
+    int ii[3], jj[3], kk[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        ii[d] = wrapc(i - 1 + d, g.nn[0], g.per[0]);
+        jj[d] = wrapc(j - 1 + d, g.nn[1], g.per[1]);
+        kk[d] = wrapc(k - 1 + d, g.nn[2], g.per[2]);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                P[p][r][q] = (TO)x[ii[q] + (long)g.nn[0] * (jj[r] + (long)g.nn[1] * kk[p])];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                S[p][r][q] = (TO)sig[(i + q) + (long)g.sg[0] * ((j + r) + (long)g.sg[1] * (k + p))];
+    const bool iso = c.face[0] == (TO)0 && c.face[1] == (TO)0 && c.face[2] == (TO)0;
+    return iso ? apply27<TO, true, 0>(P, S, c, ssum) : apply27<TO, false, 0>(P, S, c, ssum);
+}
+
+// (L x)_n in double; Dirichlet rows masked to 0.
 template <class TS>
 __device__ double apply_node_direct(const KGeom& g, const Coef<double>& c, const TS* sig,
                                     const double* x, int i, int j, int k) {
     if (is_dir(g, i, j, k)) return 0.0;
-    double s[2][2][2];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int aa = q & 1, bb = (q >> 1) & 1, cc = (q >> 2) & 1;
-        s[cc][bb][aa] = (double)sig[(i + aa) + (long)g.sg[0] * ((j + bb) + (long)g.sg[1] * (k + cc))];
-    }
-    const double xc = x[i + (long)g.nn[0] * (j + (long)g.nn[1] * k)];
-    double acc = 0.0;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (!dx && !dy && !dz) continue;
-                // sum sigma over the cells adjacent to both n and n+d
-                double ss = 0.0;
-                for (int cc = 0; cc < 2; ++cc) {
-                    if (dz && cc != (dz + 1) / 2) continue;
-                    for (int bb = 0; bb < 2; ++bb) {
-                        if (dy && bb != (dy + 1) / 2) continue;
-                        for (int aa = 0; aa < 2; ++aa) {
-                            if (dx && aa != (dx + 1) / 2) continue;
-                            ss += s[cc][bb][aa];
-                        }
-                    }
-                }
-                const int nz = (dx != 0) + (dy != 0) + (dz != 0);
-                double w;
-                if (nz == 3) w = c.corner;
-                else if (nz == 2) w = c.edge[!dx ? 0 : (!dy ? 1 : 2)];
-                else w = c.face[dx ? 0 : (dy ? 1 : 2)];
-                const int ii = wrapc(i + dx, g.nn[0], g.per[0]);
-                const int jj = wrapc(j + dy, g.nn[1], g.per[1]);
-                const int kk = wrapc(k + dz, g.nn[2], g.per[2]);
-                acc += w * ss * (x[ii + (long)g.nn[0] * (jj + (long)g.nn[1] * kk)] - xc);
-            }
-    return acc;
+    double ssum;
+    return apply_node_nbhd<double, TS, double>(g, c, sig, x, i, j, k, ssum);
 }
 
 template <class TS>
@@ -526,12 +667,9 @@ __device__ void bottom_apply(const KGeom& g, const Coef<double>& c, const TS* si
 }
 
 template <class T>
-static __global__ void __launch_bounds__(1024) bottom_bicgstab_kernel(KGeom g, Coef<double> c,
-                                                               const T* sig, const T* b_in,
-                                                               T* x_out, double* work, int maxiter,
-                                                               double rtol, int singular,
-                                                               int* iters_acc) {
-    __shared__ double red[16];
+__device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, const T* sig,
+                                     const T* b_in, T* x_out, double* work, int maxiter,
+                                     double rtol, int singular, int* iters_acc, double* red) {
     const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
     double* x = work;
     double* r = work + n;
@@ -605,6 +743,256 @@ static __global__ void __launch_bounds__(1024) bottom_bicgstab_kernel(KGeom g, C
     const double xmean = singular ? xs / (double)n : 0.0;
     for (long id = tid; id < n; id += nt) x_out[id] = (T)(x[id] - xmean);
     if (tid == 0 && iters_acc) atomicAdd(iters_acc, it);
+}
+
+template <class T>
+static __global__ void __launch_bounds__(1024) bottom_bicgstab_kernel(KGeom g, Coef<double> c,
+                                                                      const T* sig, const T* b_in,
+                                                                      T* x_out, double* work,
+                                                                      int maxiter, double rtol,
+                                                                      int singular, int* iters_acc) {
+    __shared__ double red[16];
+    bottom_bicgstab_body<T>(g, c, sig, b_in, x_out, work, maxiter, rtol, singular, iters_acc, red);
+}
+
+// ---- coarse levels: the whole V-cycle below a size threshold in one workgroup ----
+// The levels are small enough that one workgroup sweeps each of them in a few
+// microseconds, so running smoothing, residual, restriction, prolongation and
+// the BiCGStab bottom solve back to back with workgroup barriers replaces
+// about 7 launches per level.  Same operations, same order, as the per-level
+// kernels.
+constexpr int kMaxCoarse = 8;
+
+template <class T>
+struct CoarseLevel {
+    KGeom g;
+    Coef<T> c;
+    T omega;
+    const T* sig;
+    T* b;
+    T* x;
+    T* x2;
+    T* r;
+};
+
+template <class T>
+struct CoarseArgs {
+    CoarseLevel<T> L[kMaxCoarse];
+    int nl;           // levels in this launch (the last one is the bottom)
+    int nu1, nu2;
+    Coef<double> cb;  // bottom operator coefficients (double)
+    double* work;
+    int maxiter;
+    double rtol;
+    int singular;
+    int* iters_acc;
+};
+
+template <class T>
+__device__ __forceinline__ void node_of(const KGeom& g, int id, int& i, int& j, int& k) {
+    const int q = id / g.nn[0];
+    i = id - q * g.nn[0];
+    j = q % g.nn[1];
+    k = q / g.nn[1];
+}
+
+// L x at node (i,j,k) with direct loads; also returns the sum of the 8 cell coefficients.
+template <class T>
+__device__ __forceinline__ T apply_node_T(const KGeom& g, const Coef<T>& c, const T* sig,
+                                          const T* x, int i, int j, int k, T& ssum) {
+    return apply_node_nbhd<T, T, T>(g, c, sig, x, i, j, k, ssum);
+}
+
+// Bytes of dynamic LDS the coarse kernel needs for levels L[0..nl-1]: every
+// level's sigma, b, x, x2 and r (r shares x2's storage) and the bottom
+// solver's 7 FP64 work vectors.
+template <class T>
+inline size_t coarse_lds_bytes(const KGeom* g, int nl) {
+    size_t b = 0;
+    for (int l = 0; l < nl; ++l) {
+        const size_t nn = (size_t)g[l].nn[0] * g[l].nn[1] * g[l].nn[2];
+        const size_t ns = (size_t)g[l].sg[0] * g[l].sg[1] * g[l].sg[2];
+        b += ((ns + 3 * nn) * sizeof(T) + 15) / 16 * 16;
+    }
+    const size_t nb = (size_t)g[nl - 1].nn[0] * g[nl - 1].nn[1] * g[nl - 1].nn[2];
+    return b + 7 * nb * sizeof(double);
+}
+
+// The whole coarse hierarchy lives in LDS for the duration of the launch:
+// sigma and the top level's rhs are copied in, the top level's correction is
+// copied out; everything in between never touches global memory.
+template <class T>
+static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseArgs<T> a) {
+    __shared__ double red[16];
+    __shared__ CoarseLevel<T> sl[kMaxCoarse];  // level descriptors with LDS pointers
+    __shared__ double* swork;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) {
+        size_t off = 0;
+        for (int l = 0; l < a.nl; ++l) {
+            CoarseLevel<T> lv = a.L[l];
+            const int nn = lv.g.nn[0] * lv.g.nn[1] * lv.g.nn[2];
+            const int ns = lv.g.sg[0] * lv.g.sg[1] * lv.g.sg[2];
+            T* base = reinterpret_cast<T*>(lds + off);
+            lv.sig = base;
+            lv.b = base + ns;
+            lv.x = base + ns + nn;
+            lv.x2 = base + ns + 2 * nn;
+            lv.r = lv.x2;  // the residual is formed after smoothing, when x2 is free
+            sl[l] = lv;
+            off += ((size_t)(ns + 3 * nn) * sizeof(T) + 15) / 16 * 16;
+        }
+        swork = reinterpret_cast<double*>(lds + off);
+    }
+    __syncthreads();
+    for (int l = 0; l < a.nl; ++l) {
+        const int ns = sl[l].g.sg[0] * sl[l].g.sg[1] * sl[l].g.sg[2];
+        T* dst = const_cast<T*>(sl[l].sig);
+        const T* src = a.L[l].sig;
+        for (int id = tid; id < ns; id += nt) dst[id] = src[id];
+    }
+    {
+        const int nn = sl[0].g.nn[0] * sl[0].g.nn[1] * sl[0].g.nn[2];
+        for (int id = tid; id < nn; id += nt) sl[0].b[id] = a.L[0].b[id];
+    }
+    __syncthreads();
+    T* top_x_global = a.L[0].x;
+    unsigned swmask = 0u;  // bit l: level l's current iterate is in x2
+    auto cur = [&](int l) { return ((swmask >> l) & 1u) ? sl[l].x2 : sl[l].x; };
+    auto oth = [&](int l) { return ((swmask >> l) & 1u) ? sl[l].x : sl[l].x2; };
+    auto jacobi = [&](int l) {
+        const CoarseLevel<T>& lv = sl[l];
+        const int n = lv.g.nn[0] * lv.g.nn[1] * lv.g.nn[2];
+        const T* xin = cur(l);
+        T* xout = oth(l);
+        for (int id = tid; id < n; id += nt) {
+            int i, j, k;
+            node_of<T>(lv.g, id, i, j, k);
+            T ssum;
+            const T acc = apply_node_T<T>(lv.g, lv.c, lv.sig, xin, i, j, k, ssum);
+            T v = xin[id];
+            if (is_dir(lv.g, i, j, k)) v = 0;
+            else if (ssum != (T)0) v = v + lv.omega * (lv.b[id] - acc) / (-lv.c.diag4 * ssum);
+            xout[id] = v;
+        }
+        swmask ^= (1u << l);
+        __syncthreads();
+    };
+    // down
+    for (int l = 0; l < a.nl - 1; ++l) {
+        const CoarseLevel<T>& lv = sl[l];
+        const int n = lv.g.nn[0] * lv.g.nn[1] * lv.g.nn[2];
+        if (a.nu1 > 0) {
+            T* x = cur(l);
+            for (int id = tid; id < n; id += nt) {
+                int i, j, k;
+                node_of<T>(lv.g, id, i, j, k);
+                T ssum = 0;
+                for (int q = 0; q < 8; ++q)
+                    ssum += lv.sig[(i + (q & 1)) + (long)lv.g.sg[0] * ((j + ((q >> 1) & 1)) +
+                                                                    (long)lv.g.sg[1] * (k + ((q >> 2) & 1)))];
+                x[id] = (is_dir(lv.g, i, j, k) || ssum == (T)0) ? (T)0
+                                                               : lv.omega * lv.b[id] / (-lv.c.diag4 * ssum);
+            }
+            __syncthreads();
+            for (int s = 1; s < a.nu1; ++s) jacobi(l);
+        } else {
+            T* x = cur(l);
+            for (int id = tid; id < n; id += nt) x[id] = 0;
+            __syncthreads();
+        }
+        {
+            const T* x = cur(l);
+            for (int id = tid; id < n; id += nt) {
+                int i, j, k;
+                node_of<T>(lv.g, id, i, j, k);
+                T ssum;
+                const T acc = apply_node_T<T>(lv.g, lv.c, lv.sig, x, i, j, k, ssum);
+                oth(l)[id] = is_dir(lv.g, i, j, k) ? (T)0 : lv.b[id] - acc;
+            }
+            __syncthreads();
+        }
+        {
+            const CoarseLevel<T>& cv = sl[l + 1];
+            const T* rf = oth(l);  // the residual just formed
+            const KGeom& gf = lv.g;
+            const KGeom& gc = cv.g;
+            const int nc = gc.nn[0] * gc.nn[1] * gc.nn[2];
+            for (int id = tid; id < nc; id += nt) {
+                int I, J, K;
+                node_of<T>(gc, id, I, J, K);
+                T acc = 0;
+                if (!is_dir(gc, I, J, K)) {
+                    for (int oz = -1; oz <= 1; ++oz) {
+                        int fk = 2 * K + oz;
+                        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
+                        else if (fk < 0 || fk >= gf.nn[2]) continue;
+                        for (int oy = -1; oy <= 1; ++oy) {
+                            int fj = 2 * J + oy;
+                            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
+                            else if (fj < 0 || fj >= gf.nn[1]) continue;
+                            for (int ox = -1; ox <= 1; ++ox) {
+                                int fi = 2 * I + ox;
+                                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
+                                else if (fi < 0 || fi >= gf.nn[0]) continue;
+                                const T w = (ox ? (T)0.25 : (T)0.5) * (oy ? (T)0.25 : (T)0.5) *
+                                            (oz ? (T)0.25 : (T)0.5);
+                                acc += w * rf[fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk)];
+                            }
+                        }
+                    }
+                }
+                cv.b[id] = acc;
+            }
+            __syncthreads();
+        }
+    }
+    // bottom
+    {
+        const int lb = a.nl - 1;
+        bottom_bicgstab_body<T>(sl[lb].g, a.cb, sl[lb].sig, sl[lb].b, cur(lb), swork,
+                                a.maxiter, a.rtol, a.singular, a.iters_acc, red);
+        __syncthreads();
+    }
+    // up
+    for (int l = a.nl - 2; l >= 0; --l) {
+        const CoarseLevel<T>& lv = sl[l];
+        const KGeom& gf = lv.g;
+        const KGeom& gc = sl[l + 1].g;
+        const T* xc = cur(l + 1);
+        T* xf = cur(l);
+        const int nf = gf.nn[0] * gf.nn[1] * gf.nn[2];
+        for (int id = tid; id < nf; id += nt) {
+            int i, j, k;
+            node_of<T>(gf, id, i, j, k);
+            if (is_dir(gf, i, j, k)) continue;
+            const int f[3] = {i, j, k};
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = f[d] >> 1;
+                hi[d] = (f[d] & 1) ? lo[d] + 1 : lo[d];
+                if (gc.per[d] && hi[d] >= gc.nn[d]) hi[d] -= gc.nn[d];
+            }
+            T acc = 0;
+            for (int q = 0; q < 8; ++q) {
+                const int I = (q & 1) ? hi[0] : lo[0];
+                const int J = (q & 2) ? hi[1] : lo[1];
+                const int K = (q & 4) ? hi[2] : lo[2];
+                acc += xc[I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K)];
+            }
+            xf[id] += (T)0.125 * acc;
+        }
+        __syncthreads();
+        for (int s = 0; s < a.nu2; ++s) jacobi(l);
+    }
+    // the top level's correction goes back to global memory
+    {
+        const CoarseLevel<T>& lv = sl[0];
+        const int n = lv.g.nn[0] * lv.g.nn[1] * lv.g.nn[2];
+        const T* xs = cur(0);
+        for (int id = tid; id < n; id += nt) top_x_global[id] = xs[id];
+    }
 }
 
 }  // namespace mgx

@@ -90,7 +90,6 @@ class NodalSolver {
     NodalHierarchy<float> H32;
     DevBuf sig0_buf;         // level-0 sigma (ghosted, FP64)
     DevBuf phi_a, phi_b;     // outer iterate (ping-pong)
-    DevBuf bbuf;             // rhs after the singular/Dirichlet fix
     DevBuf res64;            // FP64 residual (plain mode's level-0 V-cycle rhs)
     DevBuf res32;            // FP32 residual (mixed mode's level-0 V-cycle rhs)
     DevBuf scal;             // device scalars: [0..7] u64 norms, doubles after
@@ -106,9 +105,10 @@ class NodalSolver {
     void setup(const NodalGeom& g, const MGOpts& o);
     double* sigma0() { return sig0_buf.as<double>(); }
     void sigma_changed() { sigma_set = true; h32_sigma = false; }
-    // Solve L phi = b for the canonical rhs b (device, FP64). The initial
+    // Solve L phi = b for the canonical rhs b (device, FP64; modified in place:
+    // Dirichlet rows zeroed, mean removed for singular problems).  The initial
     // guess is this->phi (zeroed by the caller if wanted).
-    void solve(const double* b, double rtol, double atol, int mode, mgx_solve_info* info,
+    void solve(double* b, double rtol, double atol, int mode, mgx_solve_info* info,
                hipStream_t s);
     double read_norm(int slot, hipStream_t s);
     void zero_norm(int slot, hipStream_t s);

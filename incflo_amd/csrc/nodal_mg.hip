@@ -13,6 +13,7 @@
 // the coarsest level is solved by a one-workgroup FP64 BiCGStab.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "mgx_ctx.h"
@@ -20,26 +21,48 @@
 
 namespace mgx {
 
-template <class T, class TC, int MODE>
-void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
-    constexpr int TX = 64, TY = 4;
+// Nodes per thread of the stencil kernel (1: 64x8 threads, 2: 64x4 threads);
+// MGX_STENCIL_NPT overrides the default for tuning runs.
+static int stencil_npt(bool dbl) {
+    static int env = [] {
+        const char* e = getenv("MGX_STENCIL_NPT");
+        return e ? atoi(e) : 0;
+    }();
+    if (env == 1 || env == 2) return env;
+    (void)dbl;
+    return 2;
+}
+
+template <class T, class TC, int MODE, int NPT>
+static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
+    constexpr int TX = 64, NY = 8;
     a.gx = (g.nn[0] + TX - 1) / TX;
-    a.gy = (g.nn[1] + TY - 1) / TY;
+    a.gy = (g.nn[1] + NY - 1) / NY;
     const long gxy = (long)a.gx * a.gy;
-    int kz = (int)std::min<long>(32, std::max<long>(2, (g.nn[2] * gxy) / 2048));
+    // z-planes per workgroup: long runs amortise the two prologue planes on big
+    // levels; small levels are latency bound and get short runs / more blocks
+    int kz = (int)std::min<long>(64, std::max<long>(2, (g.nn[2] * gxy) / 2048));
     kz = std::min(kz, g.nn[2]);
     a.kz = kz;
     a.gz = (g.nn[2] + kz - 1) / kz;
     const unsigned nb = (unsigned)(gxy * a.gz);
-    dim3 blk(TX, TY);
+    dim3 blk(TX, NY / NPT);
     if (g.iso()) {
         if (fine)
-            nodal_stencil_kernel<T, TC, MODE, true, TX, TY, true><<<nb, blk, 0, s>>>(a);
+            nodal_stencil_kernel<T, TC, MODE, true, true, NPT><<<nb, blk, 0, s>>>(a);
         else
-            nodal_stencil_kernel<T, TC, MODE, true, TX, TY, false><<<nb, blk, 0, s>>>(a);
+            nodal_stencil_kernel<T, TC, MODE, true, false, NPT><<<nb, blk, 0, s>>>(a);
     } else {
-        nodal_stencil_kernel<T, TC, MODE, false, TX, TY, false><<<nb, blk, 0, s>>>(a);
+        nodal_stencil_kernel<T, TC, MODE, false, false, NPT><<<nb, blk, 0, s>>>(a);
     }
+}
+
+template <class T, class TC, int MODE>
+void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
+    if (stencil_npt(sizeof(T) == 8) == 1)
+        launch_stencil_npt<T, TC, MODE, 1>(g, a, s, fine);
+    else
+        launch_stencil_npt<T, TC, MODE, 2>(g, a, s, fine);
 }
 
 void LaunchTimer::begin(hipStream_t s) {
@@ -111,7 +134,7 @@ void NodalHierarchy<T>::set_sigma(const double* sig0, hipStream_t s) {
     }
     for (size_t l = 1; l < L.size(); ++l) {
         const long nc = (long)L[l].g.n[0] * L[l].g.n[1] * L[l].g.n[2];
-        sigma_coarsen_kernel<T><<<grid1d(nc), 256, 0, s>>>(L[l - 1].kg, L[l].kg, L[l - 1].sig,
+        sigma_coarsen_kernel<T><<<grid3(L[l].g.n[0], L[l].g.n[1], L[l].g.n[2]), block3(), 0, s>>>(L[l - 1].kg, L[l].kg, L[l - 1].sig,
                                                            L[l].sig);
     }
 }
@@ -126,8 +149,14 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
             zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.x, nn);
             return;
         }
-        nodal_jacobi_first_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.kg, lv.coef, lv.b, lv.sig,
-                                                                lv.x, (T)nodal_omega(lv.g, o.omega));
+        SArgs<T, T> a{};
+        a.g = lv.kg;
+        a.c = lv.coef;
+        a.b = lv.b;
+        a.sig = lv.sig;
+        a.xout = lv.x;
+        a.omega = (T)nodal_omega(lv.g, o.omega);
+        launch_stencil<T, T, KS_JFIRST>(lv.g, a, s, false);
         --sweeps;
     }
     for (int q = 0; q < sweeps; ++q) {
@@ -146,10 +175,36 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
     }
 }
 
+// Levels with at most this many nodes run inside the single-workgroup launch
+// that holds the bottom solve (MGX_COARSE_NODES overrides it for tuning runs).
+// Measured at 256^3: per-level launches for 16^3 and 8^3 beat folding them
+// into the one workgroup (which is then instruction-latency bound on one CU),
+// so by default only the bottom level runs there.
+static long coarse_threshold() {
+    static long v = [] {
+        const char* e = getenv("MGX_COARSE_NODES");
+        return e ? atol(e) : 1L;
+    }();
+    return v;
+}
+
 template <class T>
 void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) {
     const int nl = (int)L.size();
-    for (int l = 0; l < nl - 1; ++l) {
+    // levels lc..nl-1 run in the LDS-resident coarse kernel (when they fit)
+    constexpr size_t kLdsBudget = 150 * 1024;
+    auto lds_for = [&](int first) {
+        KGeom gs[kMaxCoarse];
+        for (int q = 0; q < nl - first; ++q) gs[q] = L[first + q].kg;
+        return coarse_lds_bytes<T>(gs, nl - first);
+    };
+    int lc = nl - 1;
+    const bool bottom_in_lds = lds_for(nl - 1) <= kLdsBudget;
+    if (bottom_in_lds)
+        while (lc > 0 && nl - (lc - 1) <= kMaxCoarse && L[lc - 1].g.nnode() <= coarse_threshold() &&
+               lds_for(lc - 1) <= kLdsBudget)
+            --lc;
+    for (int l = 0; l < lc; ++l) {
         MGLevel<T>& lv = L[l];
         smooth(l, o.nu1, true, o, s);
         SArgs<T, T> a{};
@@ -161,20 +216,54 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         a.rout = lv.r;
         launch_stencil<T, T, KS_RESID>(lv.g, a, s, false);
         const long nc = L[l + 1].g.nnode();
-        nodal_restrict_kernel<T><<<grid1d(nc), 256, 0, s>>>(lv.kg, L[l + 1].kg, lv.r, L[l + 1].b);
+        nodal_restrict_kernel<T><<<grid3(L[l + 1].g.nn[0], L[l + 1].g.nn[1], L[l + 1].g.nn[2]), block3(), 0, s>>>(lv.kg, L[l + 1].kg, lv.r, L[l + 1].b);
     }
-    {
+    if (!bottom_in_lds) {
         MGLevel<T>& lb = L[nl - 1];
         const long nb = lb.g.nnode();
         const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
         bottom_bicgstab_kernel<T><<<1, threads, 0, s>>>(
             lb.kg, make_coef<double>(lb.g.dx), lb.sig, lb.b, lb.x, bottom_work.as<double>(),
             o.bottom_maxiter, o.bottom_rtol, lb.g.singular() ? 1 : 0, bottom_iters);
+    } else {
+        CoarseArgs<T> ca{};
+        ca.nl = nl - lc;
+        long nmax = 0;
+        for (int q = 0; q < ca.nl; ++q) {
+            MGLevel<T>& lv = L[lc + q];
+            ca.L[q].g = lv.kg;
+            ca.L[q].c = lv.coef;
+            ca.L[q].omega = (T)nodal_omega(lv.g, o.omega);
+            ca.L[q].sig = lv.sig;
+            ca.L[q].b = lv.b;
+            ca.L[q].x = lv.x;
+            ca.L[q].x2 = lv.x2;
+            ca.L[q].r = lv.r;
+            nmax = std::max(nmax, lv.g.nnode());
+        }
+        const MGLevel<T>& lb = L[nl - 1];
+        ca.nu1 = o.nu1;
+        ca.nu2 = o.nu2;
+        ca.cb = make_coef<double>(lb.g.dx);
+        ca.work = bottom_work.as<double>();
+        ca.maxiter = o.bottom_maxiter;
+        ca.rtol = o.bottom_rtol;
+        ca.singular = lb.g.singular() ? 1 : 0;
+        ca.iters_acc = bottom_iters;
+        const int threads = nmax >= 512 ? 512 : (nmax >= 256 ? 256 : 64);
+        static bool attr_set = false;
+        if (!attr_set) {
+            // (AMD GPUs grant up to the CU's LDS without it; the call is advisory)
+            (void)hipFuncSetAttribute((const void*)coarse_vcycle_kernel<T>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+            attr_set = true;
+        }
+        coarse_vcycle_kernel<T><<<1, threads, lds_for(lc), s>>>(ca);
     }
-    for (int l = nl - 2; l >= 0; --l) {
+    for (int l = lc - 1; l >= 0; --l) {
         MGLevel<T>& lv = L[l];
         const long nf = lv.g.nnode();
-        nodal_prolong_add_kernel<T><<<grid1d(nf), 256, 0, s>>>(lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);
+        nodal_prolong_add_kernel<T><<<grid3(lv.g.nn[0], lv.g.nn[1], lv.g.nn[2]), block3(), 0, s>>>(lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);
         smooth(l, o.nu2, false, o, s);
     }
 }
@@ -201,9 +290,8 @@ void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
     phi_b.alloc(nn * sizeof(double));
     MGX_HIP(hipMemset(phi_a.p, 0, nn * sizeof(double)));
     phi = phi_a.as<double>();
-    bbuf.alloc(nn * sizeof(double));
     scal.alloc(64 * sizeof(double));
-    part.alloc(1024 * sizeof(double));
+    part.alloc(4096 * sizeof(double));
     ibuf.alloc(16 * sizeof(int));
     if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
     H64.built = false;
@@ -245,8 +333,8 @@ static void remove_mean(NodalSolver& S, double* a, long n, hipStream_t s) {
     subtract_scalar_kernel<double><<<grid1d(n), 256, 0, s>>>(a, n, S.scal.as<double>() + 33);
 }
 
-void NodalSolver::solve(const double* b_in, double rtol, double atol, int mode,
-                        mgx_solve_info* info, hipStream_t s) {
+void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_info* info,
+                        hipStream_t s) {
     MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "nodal solve: sigma has not been set");
     MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_UNSUPPORTED,
                 "nodal solve: mode not available in this build");
@@ -273,14 +361,20 @@ void NodalSolver::solve(const double* b_in, double rtol, double atol, int mode,
         H64.set_sigma(sigma0(), s);  // coarse levels; level 0 is shared
     }
 
-    double* b = bbuf.as<double>();
-    if (b_in != b) MGX_HIP(hipMemcpyAsync(b, b_in, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-    zero_dirichlet_kernel<double><<<grid1d(nn), 256, 0, s>>>(kg, b);
-    zero_dirichlet_kernel<double><<<grid1d(nn), 256, 0, s>>>(kg, phi);
-    if (g.singular()) remove_mean(*this, b, nn, s);
-    zero_norm(0, s);
-    max_abs_kernel<double><<<std::min<long>(1024, grid1d(nn)), 256, 0, s>>>(
-        b, nn, scal.as<unsigned long long>());
+    const bool has_dir = g.dlo[0] || g.dlo[1] || g.dlo[2] || g.dhi[0] || g.dhi[1] || g.dhi[2];
+    if (has_dir) {
+        zero_dirichlet_kernel<double><<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, s>>>(kg, b);
+        zero_dirichlet_kernel<double><<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, s>>>(kg, phi);
+    }
+    {
+        // one pass for sum/min/max of b; ||b - mean||_inf follows from min/max
+        const int np = (int)std::min<long>(1024, std::max<long>(1, (nn + 255) / 256));
+        stats_kernel<<<np, 256, 0, s>>>(b, nn, part.as<double>(), np);
+        final_stats_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, nn, g.singular() ? 1 : 0,
+                                             scal.as<double>() + 33, scal.as<unsigned long long>());
+        if (g.singular())
+            subtract_scalar_kernel<double><<<grid1d(nn), 256, 0, s>>>(b, nn, scal.as<double>() + 33);
+    }
     MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
     const double bnorm = read_norm(0, s);
     const Coef<double> c64 = make_coef<double>(g.dx);
