@@ -111,3 +111,41 @@ def from_boxed(boxed, boxes, ng, shape, ncomp=1, nodal=False):
             out[comp][np.ix_(iz, iy, ix)] = v
         off += ncomp * vol
     return out if ncomp > 1 else out[0]
+
+
+def face_box_dims(lo, hi, ng, d):
+    return tuple(hi[e] - lo[e] + 1 + (1 if e == d else 0) + 2 * ng for e in range(3))
+
+
+def face_field_size(boxes, ng, d):
+    return sum(int(np.prod(face_box_dims(lo, hi, ng, d))) for lo, hi in boxes)
+
+
+def faces_to_boxed(canon, boxes, ng, d):
+    """canonical face array (shape (nz,ny,nx) with n_d + 1 along d) -> boxed (ghost faces 0)."""
+    out = np.zeros(face_field_size(boxes, ng, d))
+    off = 0
+    for lo, hi in boxes:
+        D = face_box_dims(lo, hi, ng, d)
+        blk = np.zeros((D[2], D[1], D[0]))
+        e = [1 if q == d else 0 for q in range(3)]
+        blk[ng:D[2] - ng, ng:D[1] - ng, ng:D[0] - ng] = canon[lo[2]:hi[2] + 1 + e[2],
+                                                              lo[1]:hi[1] + 1 + e[1],
+                                                              lo[0]:hi[0] + 1 + e[0]]
+        out[off:off + blk.size] = blk.ravel()
+        off += blk.size
+    return out
+
+
+def faces_from_boxed(boxed, boxes, ng, d, shape):
+    out = np.zeros(shape)
+    off = 0
+    for lo, hi in boxes:
+        D = face_box_dims(lo, hi, ng, d)
+        vol = int(np.prod(D))
+        blk = boxed[off:off + vol].reshape(D[2], D[1], D[0])
+        e = [1 if q == d else 0 for q in range(3)]
+        out[lo[2]:hi[2] + 1 + e[2], lo[1]:hi[1] + 1 + e[1], lo[0]:hi[0] + 1 + e[0]] = \
+            blk[ng:D[2] - ng, ng:D[1] - ng, ng:D[0] - ng]
+        off += vol
+    return out

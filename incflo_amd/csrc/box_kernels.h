@@ -18,6 +18,7 @@ struct BoxInfo {
     int rlen[3];
     long pc[4];      // prefix sums over earlier boxes for cell-field offsets
     long pn[4];      // ... for nodal-field offsets
+    long pf[3][4];   // ... for face-field offsets (faces normal to x, y, z)
 };
 
 struct Tile {
@@ -25,12 +26,25 @@ struct Tile {
     int start;
 };
 
-enum { REG_CELL = 0, REG_CELL_GROWN = 1, REG_NODE = 2 };
+enum { REG_CELL = 0, REG_CELL_GROWN = 1, REG_NODE = 2, REG_FACE0 = 3, REG_NREG = 6 };
 
 __device__ __forceinline__ long field_offset(const BoxInfo& b, int ng, int ncomp, bool nodal) {
     const long* P = nodal ? b.pn : b.pc;
     const long g = ng;
     return (long)ncomp * (P[0] + 2 * g * P[1] + 4 * g * g * P[2] + 8 * g * g * g * P[3]);
+}
+
+__device__ __forceinline__ long face_offset(const BoxInfo& b, int ng, int d) {
+    const long* P = b.pf[d];
+    const long g = ng;
+    return P[0] + 2 * g * P[1] + 4 * g * g * P[2] + 8 * g * g * g * P[3];
+}
+
+// index of face (i,j,k) (normal d) inside box b of a face field with ghost width ng
+__device__ __forceinline__ long box_index_face(const BoxInfo& b, int ng, int d, int i, int j,
+                                               int k) {
+    const long D0 = b.len[0] + (d == 0) + 2 * ng, D1 = b.len[1] + (d == 1) + 2 * ng;
+    return (i - b.lo[0] + ng) + D0 * ((j - b.lo[1] + ng) + D1 * (long)(k - b.lo[2] + ng));
 }
 
 // index of (i,j,k,comp) inside box b of a field with ghost width ng
@@ -50,6 +64,7 @@ __device__ __forceinline__ bool tile_point(const BoxInfo* boxes, const Tile* til
     int lo[3], len[3];
     for (int d = 0; d < 3; ++d) {
         if (region == REG_CELL_GROWN) { lo[d] = b.rlo[d]; len[d] = b.rlen[d]; }
+        else if (region >= REG_FACE0) { lo[d] = b.lo[d]; len[d] = b.len[d] + (region - REG_FACE0 == d); }
         else { lo[d] = b.lo[d]; len[d] = b.len[d] + (region == REG_NODE ? 1 : 0); }
     }
     const int lin = t.start + (int)threadIdx.x;
@@ -260,6 +275,39 @@ static __global__ void __launch_bounds__(256) gather_nodes_kernel(KGeom g, const
     const BoxInfo& b = boxes[bi];
     phi[i + (long)g.nn[0] * (j + (long)g.nn[1] * k)] =
         in[field_offset(b, ng, 1, true) + box_index(b, ng, true, i, j, k, 0)];
+}
+
+// ---- face and cell fields of the MAC projection -----------------------------
+// canonical face index (faces along d: n_d + 1 of them)
+__device__ __forceinline__ long canon_face(const KGeom& g, int d, int i, int j, int k) {
+    const long f0 = g.n[0] + (d == 0), f1 = g.n[1] + (d == 1);
+    return i + f0 * (j + f1 * (long)k);
+}
+
+// canonical faces <- boxed face field (dir = d), or the reverse when to_box
+static __global__ void __launch_bounds__(256) face_copy_kernel(KGeom g, const BoxInfo* boxes,
+                                                              const Tile* tiles, int d, double* boxed,
+                                                              int ng, double* canon, int to_box) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_FACE0 + d, bi, i, j, k)) return;
+    const BoxInfo& b = boxes[bi];
+    const long o = face_offset(b, ng, d) + box_index_face(b, ng, d, i, j, k);
+    const long c = canon_face(g, d, i, j, k);
+    if (to_box) boxed[o] = canon[c];
+    else canon[c] = boxed[o];
+}
+
+// canonical cells (no ghosts) <- boxed cell field (valid cells), or the reverse
+static __global__ void __launch_bounds__(256) cell_copy_kernel(KGeom g, const BoxInfo* boxes,
+                                                              const Tile* tiles, double* boxed,
+                                                              int ng, double* canon, int to_box) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_CELL, bi, i, j, k)) return;
+    const BoxInfo& b = boxes[bi];
+    const long o = field_offset(b, ng, 1, false) + box_index(b, ng, false, i, j, k, 0);
+    const long c = i + (long)g.n[0] * (j + (long)g.n[1] * k);
+    if (to_box) boxed[o] = canon[c];
+    else canon[c] = boxed[o];
 }
 
 }  // namespace mgx

@@ -1,0 +1,345 @@
+// mac_kernels.h -- HIP kernels of the MAC projection's cell-centred multigrid.
+//
+// Operator (variable face coefficient beta, cells c, faces f):
+//   F_f = beta_f (phi_R - phi_L)/h        interior and periodic faces
+//   F_f = 0                               Neumann boundary faces (walls, inflow)
+//   F_lo = 2 beta_f phi_0 / h,  F_hi = -2 beta_f phi_{n-1} / h   Dirichlet faces
+//   (L phi)_c = sum_d (F_{c+1/2} - F_{c-1/2}) / h_d
+// i.e. div(beta grad phi) for Hydro::MacProjector (AMReX-Hydro, called from
+// src/convection/incflo_compute_MAC_projected_velocities.cpp:294-299).
+// Cells are stored x-fastest without ghosts; face arrays along d hold n_d+1
+// faces (the periodic face n_d equals face 0).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nodal_kernels.h"
+
+namespace mgx {
+
+struct MGeomK {
+    int n[3];
+    int per[3];
+    int dlo[3], dhi[3];  // Dirichlet faces
+};
+
+template <class T>
+struct MCoef {
+    T ih2[3];  // 1/h_d^2
+};
+
+__device__ __forceinline__ long cidx(const MGeomK& g, int i, int j, int k) {
+    return i + (long)g.n[0] * (j + (long)g.n[1] * k);
+}
+
+__device__ __forceinline__ long fidx(const MGeomK& g, int d, int i, int j, int k) {
+    const long f0 = g.n[0] + (d == 0), f1 = g.n[1] + (d == 1);
+    return i + f0 * (j + f1 * (long)k);
+}
+
+// (L phi)_c and the diagonal of L at cell (i,j,k); phi read with direct loads.
+template <class T, class TB, class TP>
+__device__ __forceinline__ T mac_apply_cell(const MGeomK& g, const MCoef<T>& c, const TB* bx,
+                                            const TB* by, const TB* bz, const TP* phi, int i,
+                                            int j, int k, T& diag) {
+    const int p[3] = {i, j, k};
+    const TB* beta[3] = {bx, by, bz};
+    const T pc = (T)phi[cidx(g, i, j, k)];
+    T acc = 0, dg = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        // lo face
+        {
+            int q[3] = {p[0], p[1], p[2]};
+            const T b = (T)beta[d][fidx(g, d, q[0], q[1], q[2])];
+            if (p[d] > 0 || g.per[d]) {
+                q[d] = p[d] > 0 ? p[d] - 1 : g.n[d] - 1;
+                acc += c.ih2[d] * b * ((T)phi[cidx(g, q[0], q[1], q[2])] - pc);
+                dg -= c.ih2[d] * b;
+            } else if (g.dlo[d]) {
+                acc -= 2 * c.ih2[d] * b * pc;
+                dg -= 2 * c.ih2[d] * b;
+            }
+        }
+        // hi face
+        {
+            int q[3] = {p[0], p[1], p[2]};
+            q[d] = p[d] + 1;
+            const T b = (T)beta[d][fidx(g, d, q[0], q[1], q[2])];
+            if (p[d] + 1 < g.n[d] || g.per[d]) {
+                q[d] = p[d] + 1 < g.n[d] ? p[d] + 1 : 0;
+                acc += c.ih2[d] * b * ((T)phi[cidx(g, q[0], q[1], q[2])] - pc);
+                dg -= c.ih2[d] * b;
+            } else if (g.dhi[d]) {
+                acc -= 2 * c.ih2[d] * b * pc;
+                dg -= 2 * c.ih2[d] * b;
+            }
+        }
+    }
+    diag = dg;
+    return acc;
+}
+
+// One colour of red-black Gauss-Seidel, in place.  Threads cover pairs of
+// cells along x; each updates the cell of the requested colour.
+template <class T>
+static __global__ void mac_gsrb_kernel(MGeomK g, MCoef<T> c, const T* __restrict__ bx,
+                                       const T* __restrict__ by, const T* __restrict__ bz,
+                                       const T* __restrict__ b, T* x, int color) {
+    int ih, j, k;
+    if (!grid_point((g.n[0] + 1) / 2, g.n[1], ih, j, k)) return;
+    const int i = 2 * ih + ((j + k + color) & 1);
+    if (i >= g.n[0]) return;
+    T dg;
+    const T acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, x, i, j, k, dg);
+    const long id = cidx(g, i, j, k);
+    if (dg != (T)0) x[id] += (b[id] - acc) / dg;
+}
+
+enum { MK_RESID = 0, MK_UPDRES = 1, MK_APPLY = 2 };
+
+// RESID: r = b - L x.  UPDRES: x_out = x + cor, r = b - L x_out.  APPLY: r = L x.
+template <class T, class TC, int MODE>
+static __global__ void mac_resid_kernel(MGeomK g, MCoef<T> c, const T* __restrict__ bx,
+                                        const T* __restrict__ by, const T* __restrict__ bz,
+                                        const T* __restrict__ b, const T* __restrict__ x,
+                                        const TC* __restrict__ cor, T* __restrict__ xout,
+                                        TC* __restrict__ r, unsigned long long* norm) {
+    __shared__ double red[4];
+    int i, j, k;
+    double m = 0.0;
+    if (grid_point(g.n[0], g.n[1], i, j, k)) {
+        const long id = cidx(g, i, j, k);
+        T dg, acc;
+        if (MODE == MK_UPDRES) {
+            // L(x + cor) = L x + L cor (linear); both read directly
+            T d2;
+            acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, x, i, j, k, dg) +
+                  mac_apply_cell<T, T, TC>(g, c, bx, by, bz, cor, i, j, k, d2);
+            xout[id] = x[id] + (T)cor[id];
+        } else {
+            acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, x, i, j, k, dg);
+        }
+        if (MODE == MK_APPLY) {
+            r[id] = (TC)acc;
+        } else {
+            const T rv = b[id] - acc;
+            r[id] = (TC)rv;
+            m = fabs((double)rv);
+        }
+    }
+    if (MODE != MK_APPLY && norm) {
+        const double bm = block_max<256>(m, red);
+        if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_nonneg(norm, bm);
+    }
+}
+
+// coarse cell = mean of its 8 fine cells
+template <class T>
+static __global__ void mac_restrict_kernel(MGeomK gf, MGeomK gc, const T* __restrict__ rf,
+                                           T* __restrict__ rc) {
+    int I, J, K;
+    if (!grid_point(gc.n[0], gc.n[1], I, J, K)) return;
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        s += rf[cidx(gf, 2 * I + (q & 1), 2 * J + ((q >> 1) & 1), 2 * K + ((q >> 2) & 1))];
+    rc[cidx(gc, I, J, K)] = (T)0.125 * s;
+}
+
+// piecewise-constant prolongation added onto the fine correction
+template <class T>
+static __global__ void mac_prolong_add_kernel(MGeomK gf, MGeomK gc, const T* __restrict__ xc,
+                                              T* __restrict__ xf) {
+    int i, j, k;
+    if (!grid_point(gf.n[0], gf.n[1], i, j, k)) return;
+    xf[cidx(gf, i, j, k)] += xc[cidx(gc, i >> 1, j >> 1, k >> 1)];
+}
+
+// coarse face beta = mean of the 4 fine faces covering it
+template <class T, class TF>
+static __global__ void mac_beta_coarsen_kernel(MGeomK gf, MGeomK gc, int d,
+                                               const TF* __restrict__ bf, T* __restrict__ bc) {
+    const int f0 = gc.n[0] + (d == 0), f1 = gc.n[1] + (d == 1), f2 = gc.n[2] + (d == 2);
+    int I, J, K;
+    if (!grid_point(f0, f1, I, J, K)) return;
+    if (K >= f2) return;
+    const int C[3] = {I, J, K};
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int f[3];
+        int t = 0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (e == d) f[e] = 2 * C[e];
+            else { f[e] = 2 * C[e] + ((q >> t) & 1); ++t; }
+        }
+        s += (T)bf[fidx(gf, d, f[0], f[1], f[2])];
+    }
+    bc[fidx(gc, d, I, J, K)] = (T)0.25 * s;
+}
+
+// Face flux F_f = beta grad(phi) . e_d at face (i,j,k) of direction d (see header).
+template <class T>
+__device__ __forceinline__ T mac_face_flux(const MGeomK& g, const T* beta, const T* phi, int d,
+                                           int i, int j, int k, T ih) {
+    const int p[3] = {i, j, k};
+    const T b = beta[fidx(g, d, i, j, k)];
+    int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+    lo[d] = p[d] - 1;
+    if (g.per[d]) {
+        if (lo[d] < 0) lo[d] += g.n[d];
+        if (hi[d] >= g.n[d]) hi[d] -= g.n[d];
+    } else {
+        if (p[d] == 0) return g.dlo[d] ? 2 * b * phi[cidx(g, hi[0], hi[1], hi[2])] * ih : (T)0;
+        if (p[d] == g.n[d]) return g.dhi[d] ? -2 * b * phi[cidx(g, lo[0], lo[1], lo[2])] * ih : (T)0;
+    }
+    return b * (phi[cidx(g, hi[0], hi[1], hi[2])] - phi[cidx(g, lo[0], lo[1], lo[2])]) * ih;
+}
+
+// rhs = div(u_mac) on cells
+static __global__ void mac_div_kernel(MGeomK g, const double* __restrict__ u,
+                                      const double* __restrict__ v, const double* __restrict__ w,
+                                      double ihx, double ihy, double ihz, double* __restrict__ rhs) {
+    int i, j, k;
+    if (!grid_point(g.n[0], g.n[1], i, j, k)) return;
+    rhs[cidx(g, i, j, k)] = (u[fidx(g, 0, i + 1, j, k)] - u[fidx(g, 0, i, j, k)]) * ihx +
+                            (v[fidx(g, 1, i, j + 1, k)] - v[fidx(g, 1, i, j, k)]) * ihy +
+                            (w[fidx(g, 2, i, j, k + 1)] - w[fidx(g, 2, i, j, k)]) * ihz;
+}
+
+// u_d -= F (projection) or out = -F (getFluxes), on faces of direction d
+static __global__ void mac_face_update_kernel(MGeomK g, int d, const double* __restrict__ beta,
+                                              const double* __restrict__ phi, double ih,
+                                              double* __restrict__ u, int mode_fluxes) {
+    const int f0 = g.n[0] + (d == 0), f1 = g.n[1] + (d == 1), f2 = g.n[2] + (d == 2);
+    int i, j, k;
+    if (!grid_point(f0, f1, i, j, k)) return;
+    if (k >= f2) return;
+    const double F = mac_face_flux<double>(g, beta, phi, d, i, j, k, ih);
+    const long id = fidx(g, d, i, j, k);
+    if (mode_fluxes) u[id] = -F;
+    else u[id] -= F;
+}
+
+// beta_f = dt / (0.5 (rho_L + rho_R)); non-periodic boundary faces use the adjacent cell
+static __global__ void mac_beta_from_rho_kernel(MGeomK g, int d, const double* __restrict__ rho,
+                                                double dt, double* __restrict__ beta) {
+    const int f0 = g.n[0] + (d == 0), f1 = g.n[1] + (d == 1), f2 = g.n[2] + (d == 2);
+    int i, j, k;
+    if (!grid_point(f0, f1, i, j, k)) return;
+    if (k >= f2) return;
+    int L[3] = {i, j, k}, R[3] = {i, j, k};
+    L[d] -= 1;
+    const int n = g.n[d];
+    if (g.per[d]) {
+        if (L[d] < 0) L[d] += n;
+        if (R[d] >= n) R[d] -= n;
+    } else {
+        if (L[d] < 0) L[d] = 0;
+        if (R[d] >= n) R[d] = n - 1;
+    }
+    beta[fidx(g, d, i, j, k)] =
+        dt / (0.5 * (rho[cidx(g, L[0], L[1], L[2])] + rho[cidx(g, R[0], R[1], R[2])]));
+}
+
+// ---- bottom: BiCGStab in one workgroup on the MAC operator (FP64) ----
+template <class T>
+__device__ void mac_bottom_apply(const MGeomK& g, const MCoef<double>& c, const T* bx, const T* by,
+                                 const T* bz, const double* x, double* y, long n) {
+    __syncthreads();
+    for (long id = threadIdx.x; id < n; id += blockDim.x) {
+        const int q = (int)(id / g.n[0]);
+        const int i = (int)(id - (long)q * g.n[0]);
+        const int j = q % g.n[1];
+        const int k = q / g.n[1];
+        double dg;
+        y[id] = mac_apply_cell<double, T, double>(g, c, bx, by, bz, x, i, j, k, dg);
+    }
+    __syncthreads();
+}
+
+template <class T>
+static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef<double> c,
+                                                                 const T* bx, const T* by,
+                                                                 const T* bz, const T* b_in,
+                                                                 T* x_out, double* work,
+                                                                 int maxiter, double rtol,
+                                                                 int singular, int* iters_acc) {
+    __shared__ double red[16];
+    const long n = (long)g.n[0] * g.n[1] * g.n[2];
+    double* x = work;
+    double* r = work + n;
+    double* rh = work + 2 * n;
+    double* p = work + 3 * n;
+    double* v = work + 4 * n;
+    double* s = work + 5 * n;
+    double* t = work + 6 * n;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double bs = 0.0;
+    for (long id = tid; id < n; id += nt) { r[id] = (double)b_in[id]; bs += r[id]; }
+    bs = bsum(bs, red);
+    const double bmean = singular ? bs / (double)n : 0.0;
+    for (long id = tid; id < n; id += nt) {
+        r[id] -= bmean;
+        rh[id] = r[id];
+        x[id] = 0.0; p[id] = 0.0; v[id] = 0.0;
+    }
+    double rn = 0.0;
+    for (long id = tid; id < n; id += nt) rn = fmax(rn, fabs(r[id]));
+    const double rnorm0 = bmax(rn, red);
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    if (rnorm0 > 0.0) {
+        for (it = 1; it <= maxiter; ++it) {
+            double q = 0.0;
+            for (long id = tid; id < n; id += nt) q += rh[id] * r[id];
+            const double rho_new = bsum(q, red);
+            if (rho_new == 0.0) break;
+            const double beta = (rho_new / rho) * (alpha / omega);
+            for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
+            mac_bottom_apply<T>(g, c, bx, by, bz, p, v, n);
+            q = 0.0;
+            for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
+            const double rhv = bsum(q, red);
+            if (rhv == 0.0) break;
+            alpha = rho_new / rhv;
+            double sn = 0.0;
+            for (long id = tid; id < n; id += nt) {
+                x[id] += alpha * p[id];
+                s[id] = r[id] - alpha * v[id];
+                sn = fmax(sn, fabs(s[id]));
+            }
+            sn = bmax(sn, red);
+            if (sn <= rtol * rnorm0) break;
+            mac_bottom_apply<T>(g, c, bx, by, bz, s, t, n);
+            double tt = 0.0, ts = 0.0;
+            for (long id = tid; id < n; id += nt) { tt += t[id] * t[id]; ts += t[id] * s[id]; }
+            tt = bsum(tt, red);
+            ts = bsum(ts, red);
+            if (tt == 0.0) break;
+            omega = ts / tt;
+            double rr = 0.0;
+            for (long id = tid; id < n; id += nt) {
+                x[id] += omega * s[id];
+                r[id] = s[id] - omega * t[id];
+                rr = fmax(rr, fabs(r[id]));
+            }
+            rr = bmax(rr, red);
+            if (rr <= rtol * rnorm0) break;
+            if (omega == 0.0) break;
+            rho = rho_new;
+        }
+        if (it > maxiter) it = maxiter;
+    }
+    __syncthreads();
+    double xs = 0.0;
+    for (long id = tid; id < n; id += nt) xs += x[id];
+    xs = bsum(xs, red);
+    const double xmean = singular ? xs / (double)n : 0.0;
+    for (long id = tid; id < n; id += nt) x_out[id] = (T)(x[id] - xmean);
+    if (tid == 0 && iters_acc) atomicAdd(iters_acc, it);
+}
+
+}  // namespace mgx

@@ -1,0 +1,225 @@
+// mac_mg.hip -- host side of the MAC projection's cell-centred multigrid:
+// red-black Gauss-Seidel V-cycle (cell-average restriction, piecewise-constant
+// prolongation, face-averaged coarse beta, one-workgroup FP64 BiCGStab
+// bottom) inside the same outer MLMG iteration as the nodal solve (FP64
+// residual, FP64 or FP32 V-cycle correction).  Same steps as the oracle
+// (oracle/orc_mac.c).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mac_mg.h"
+#include "mgx_ctx.h"
+
+namespace mgx {
+
+template <class T>
+void MacHierarchy<T>::build(const MacGeom& g0, int nlev) {
+    L.clear();
+    L.resize(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        MacLevel<T>& lv = L[l];
+        lv.g = (l == 0) ? g0 : L[l - 1].g.coarse();
+        lv.kg = lv.g.k();
+        lv.c = make_mcoef<T>(lv.g.dx);
+        for (int d = 0; d < 3; ++d) {
+            lv.beta_buf[d].alloc(lv.g.nface(d) * sizeof(T));
+            lv.beta[d] = lv.beta_buf[d].template as<T>();
+        }
+        const long n = lv.g.ncell();
+        if (l > 0) lv.b_buf.alloc(n * sizeof(T));
+        lv.x_buf.alloc(n * sizeof(T));
+        if (l < nlev - 1) lv.r_buf.alloc(n * sizeof(T));
+        lv.b = lv.b_buf.template as<T>();
+        lv.x = lv.x_buf.template as<T>();
+        lv.r = lv.r_buf.template as<T>();
+    }
+    bottom_work.alloc(7 * L.back().g.ncell() * sizeof(double));
+    built = true;
+}
+
+template <class T>
+void MacHierarchy<T>::set_beta(const double* const b0[3], hipStream_t s) {
+    for (int d = 0; d < 3; ++d) {
+        const long nf = L[0].g.nface(d);
+        cast_kernel<T, double><<<grid1d(nf), 256, 0, s>>>(b0[d], L[0].beta[d], nf);
+    }
+    for (size_t l = 1; l < L.size(); ++l)
+        for (int d = 0; d < 3; ++d) {
+            const MacGeom& gc = L[l].g;
+            mac_beta_coarsen_kernel<T, T>
+                <<<grid3(gc.n[0] + (d == 0), gc.n[1] + (d == 1), gc.n[2] + (d == 2)), block3(), 0, s>>>(
+                    L[l - 1].kg, L[l].kg, d, L[l - 1].beta[d], L[l].beta[d]);
+        }
+}
+
+template <class T>
+static void gsrb(MacLevel<T>& lv, hipStream_t s) {
+    for (int color = 0; color < 2; ++color)
+        mac_gsrb_kernel<T><<<grid3((lv.g.n[0] + 1) / 2, lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
+            lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.b, lv.x, color);
+}
+
+template <class T>
+void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) {
+    const int nl = (int)L.size();
+    for (int l = 0; l < nl - 1; ++l) {
+        MacLevel<T>& lv = L[l];
+        const long n = lv.g.ncell();
+        zero_kernel<T><<<grid1d(n), 256, 0, s>>>(lv.x, n);
+        for (int q = 0; q < o.nu1; ++q) gsrb(lv, s);
+        mac_resid_kernel<T, T, MK_RESID><<<grid3(lv.g.n[0], lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
+            lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.b, lv.x, nullptr, nullptr, lv.r,
+            nullptr);
+        MacLevel<T>& cv = L[l + 1];
+        mac_restrict_kernel<T><<<grid3(cv.g.n[0], cv.g.n[1], cv.g.n[2]), block3(), 0, s>>>(
+            lv.kg, cv.kg, lv.r, cv.b);
+    }
+    {
+        MacLevel<T>& lb = L[nl - 1];
+        const long nb = lb.g.ncell();
+        const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
+        mac_bottom_kernel<T><<<1, threads, 0, s>>>(lb.kg, make_mcoef<double>(lb.g.dx), lb.beta[0],
+                                                   lb.beta[1], lb.beta[2], lb.b, lb.x,
+                                                   bottom_work.as<double>(), o.bottom_maxiter,
+                                                   o.bottom_rtol, lb.g.singular() ? 1 : 0,
+                                                   bottom_iters);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        MacLevel<T>& lv = L[l];
+        mac_prolong_add_kernel<T><<<grid3(lv.g.n[0], lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
+            lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);
+        for (int q = 0; q < o.nu2; ++q) gsrb(lv, s);
+    }
+}
+
+template class MacHierarchy<double>;
+template class MacHierarchy<float>;
+
+// ---------------------------------------------------------------------------
+
+MacSolver::~MacSolver() {
+    if (h_pinned) (void)hipHostFree(h_pinned);
+}
+
+static int mac_nlevels(const MacGeom& g, int max_coarsening_level) {
+    NodalGeom ng{};
+    for (int d = 0; d < 3; ++d) ng.n[d] = g.n[d];
+    return nodal_nlevels(ng, max_coarsening_level);
+}
+
+void MacSolver::setup(const MacGeom& g, const MGOpts& o) {
+    g0 = g;
+    opts = o;
+    nlev = mac_nlevels(g, o.max_coarsening_level);
+    for (int d = 0; d < 3; ++d) beta0[d].alloc(g.nface(d) * sizeof(double));
+    const long n = g.ncell();
+    phi_a.alloc(n * sizeof(double));
+    phi_b.alloc(n * sizeof(double));
+    MGX_HIP(hipMemset(phi_a.p, 0, n * sizeof(double)));
+    phi = phi_a.as<double>();
+    scal.alloc(64 * sizeof(double));
+    part.alloc(4096 * sizeof(double));
+    ibuf.alloc(16 * sizeof(int));
+    if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
+    H64.built = H32.built = false;
+    beta_set = h32_beta = h64_beta = false;
+}
+
+double MacSolver::read_norm(int slot, hipStream_t s) {
+    MGX_HIP(hipMemcpyAsync(h_pinned + slot, scal.as<unsigned long long>() + slot,
+                           sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    double d;
+    std::memcpy(&d, h_pinned + slot, sizeof(d));
+    return d;
+}
+
+void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_info* info,
+                      hipStream_t s) {
+    MGX_REQUIRE(beta_set, MGX_ERR_STATE, "MAC solve: beta has not been set");
+    MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_UNSUPPORTED,
+                "MAC solve: mode not available for the MAC projection");
+    const MacGeom& g = g0;
+    const MGeomK kg = g.k();
+    const long n = g.ncell();
+    const bool mixed = (mode == MGX_MODE_MIXED);
+    const double* b0[3] = {beta(0), beta(1), beta(2)};
+    if (mixed) {
+        if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g, nlev); h32_beta = false; }
+        if (!h32_beta) { H32.set_beta(b0, s); h32_beta = true; }
+        res32.alloc(n * sizeof(float));
+        H32.L[0].b = res32.as<float>();
+    } else {
+        if (!H64.built || (int)H64.L.size() != nlev) { H64.build(g, nlev); h64_beta = false; }
+        if (!h64_beta) { H64.set_beta(b0, s); h64_beta = true; }
+        res64.alloc(n * sizeof(double));
+        H64.L[0].b = res64.as<double>();
+    }
+    {
+        const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+        stats_kernel<<<np, 256, 0, s>>>(b, n, part.as<double>(), np);
+        final_stats_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, n, g.singular() ? 1 : 0,
+                                             scal.as<double>() + 33, scal.as<unsigned long long>());
+        if (g.singular())
+            subtract_scalar_kernel<double><<<grid1d(n), 256, 0, s>>>(b, n, scal.as<double>() + 33);
+    }
+    MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
+    const double bnorm = read_norm(0, s);
+    const MCoef<double> c64 = make_mcoef<double>(g.dx);
+    const dim3 gr = grid3(g.n[0], g.n[1], g.n[2]);
+    unsigned long long* nrm = scal.as<unsigned long long>() + 1;
+
+    zero_u64_kernel<<<1, 64, 0, s>>>(nrm, 1);
+    if (mixed)
+        mac_resid_kernel<double, float, MK_RESID><<<gr, block3(), 0, s>>>(
+            kg, c64, b0[0], b0[1], b0[2], b, phi, nullptr, nullptr, res32.as<float>(), nrm);
+    else
+        mac_resid_kernel<double, double, MK_RESID><<<gr, block3(), 0, s>>>(
+            kg, c64, b0[0], b0[1], b0[2], b, phi, nullptr, nullptr, res64.as<double>(), nrm);
+    const double r0 = read_norm(1, s);
+    const double tol = std::max(atol, rtol * bnorm);
+    double rn = r0;
+    int it = 0;
+    bool conv = (r0 <= tol);
+    while (!conv && it < opts.maxiter) {
+        ++it;
+        zero_u64_kernel<<<1, 64, 0, s>>>(nrm, 1);
+        double* pout = (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>();
+        if (mixed) {
+            H32.vcycle(opts, s, ibuf.as<int>());
+            mac_resid_kernel<double, float, MK_UPDRES><<<gr, block3(), 0, s>>>(
+                kg, c64, b0[0], b0[1], b0[2], b, phi, H32.L[0].x, pout, res32.as<float>(), nrm);
+        } else {
+            H64.vcycle(opts, s, ibuf.as<int>());
+            mac_resid_kernel<double, double, MK_UPDRES><<<gr, block3(), 0, s>>>(
+                kg, c64, b0[0], b0[1], b0[2], b, phi, H64.L[0].x, pout, res64.as<double>(), nrm);
+        }
+        phi = pout;
+        rn = read_norm(1, s);
+        if (opts.verbose)
+            fprintf(stderr, "mgx MAC MLMG: iter %d |r| %.6e rel %.3e\n", it, rn,
+                    bnorm > 0 ? rn / bnorm : rn);
+        conv = (rn <= tol);
+    }
+    if (g.singular()) {
+        const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+        stats_kernel<<<np, 256, 0, s>>>(phi, n, part.as<double>(), np);
+        final_stats_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, n, 1, scal.as<double>() + 34,
+                                             scal.as<unsigned long long>() + 2);
+        subtract_scalar_kernel<double><<<grid1d(n), 256, 0, s>>>(phi, n, scal.as<double>() + 34);
+    }
+    MGX_HIP(hipStreamSynchronize(s));
+    if (info) {
+        info->iters = it;
+        info->converged = conv ? 1 : 0;
+        info->nlevels = nlev;
+        info->fallback_used = 0;
+        info->bnorm = bnorm;
+        info->resnorm0 = r0;
+        info->resnorm = rn;
+        info->rel_resid = bnorm > 0 ? rn / bnorm : rn;
+    }
+}
+
+}  // namespace mgx

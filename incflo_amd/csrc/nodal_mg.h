@@ -121,8 +121,8 @@ class BoxLayout {
     std::vector<int> boxes;  // 6 ints per box
     int nboxes = 0;
     DevBuf dboxes;
-    DevBuf dtiles[3];
-    int ntiles[3] = {0, 0, 0};
+    DevBuf dtiles[REG_NREG];
+    int ntiles[REG_NREG] = {0, 0, 0, 0, 0, 0};
     void build(const int* boxes6, int nb, const NodalGeom& g);
     const BoxInfo* dev_boxes() const { return dboxes.as<BoxInfo>(); }
     const Tile* tiles(int region) const { return dtiles[region].as<Tile>(); }

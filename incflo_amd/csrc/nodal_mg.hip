@@ -463,8 +463,8 @@ void BoxLayout::build(const int* b6, int nb, const NodalGeom& g) {
             MGX_REQUIRE(!ov, MGX_ERR_ARG, "boxes overlap");
         }
     std::vector<BoxInfo> info(nb);
-    long pc[4] = {0, 0, 0, 0}, pn[4] = {0, 0, 0, 0};
-    std::vector<Tile> tiles[3];
+    long pc[4] = {0, 0, 0, 0}, pn[4] = {0, 0, 0, 0}, pf[3][4] = {};
+    std::vector<Tile> tiles[REG_NREG];
     for (int b = 0; b < nb; ++b) {
         BoxInfo& bi = info[b];
         for (int d = 0; d < 3; ++d) {
@@ -475,18 +475,28 @@ void BoxLayout::build(const int* b6, int nb, const NodalGeom& g) {
             bi.rlo[d] = bi.lo[d] - (glo ? 1 : 0);
             bi.rlen[d] = bi.len[d] + (glo ? 1 : 0) + (ghi ? 1 : 0);
         }
-        for (int q = 0; q < 4; ++q) { bi.pc[q] = pc[q]; bi.pn[q] = pn[q]; }
+        for (int q = 0; q < 4; ++q) {
+            bi.pc[q] = pc[q];
+            bi.pn[q] = pn[q];
+            for (int d = 0; d < 3; ++d) bi.pf[d][q] = pf[d][q];
+        }
         const long a0 = bi.len[0], a1 = bi.len[1], a2 = bi.len[2];
         pc[0] += a0 * a1 * a2; pc[1] += a0 * a1 + a1 * a2 + a0 * a2; pc[2] += a0 + a1 + a2; pc[3] += 1;
         const long n0 = a0 + 1, n1 = a1 + 1, n2 = a2 + 1;
         pn[0] += n0 * n1 * n2; pn[1] += n0 * n1 + n1 * n2 + n0 * n2; pn[2] += n0 + n1 + n2; pn[3] += 1;
-        const long vols[3] = {a0 * a1 * a2, (long)bi.rlen[0] * bi.rlen[1] * bi.rlen[2], n0 * n1 * n2};
-        for (int r = 0; r < 3; ++r)
+        long vols[REG_NREG] = {a0 * a1 * a2, (long)bi.rlen[0] * bi.rlen[1] * bi.rlen[2], n0 * n1 * n2};
+        for (int d = 0; d < 3; ++d) {
+            const long f0 = a0 + (d == 0), f1 = a1 + (d == 1), f2 = a2 + (d == 2);
+            pf[d][0] += f0 * f1 * f2; pf[d][1] += f0 * f1 + f1 * f2 + f0 * f2;
+            pf[d][2] += f0 + f1 + f2; pf[d][3] += 1;
+            vols[REG_FACE0 + d] = f0 * f1 * f2;
+        }
+        for (int r = 0; r < REG_NREG; ++r)
             for (long st = 0; st < vols[r]; st += 256) tiles[r].push_back(Tile{b, (int)st});
     }
     dboxes.alloc(nb * sizeof(BoxInfo));
     MGX_HIP(hipMemcpy(dboxes.p, info.data(), nb * sizeof(BoxInfo), hipMemcpyHostToDevice));
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < REG_NREG; ++r) {
         ntiles[r] = (int)tiles[r].size();
         dtiles[r].alloc(tiles[r].size() * sizeof(Tile));
         MGX_HIP(hipMemcpy(dtiles[r].p, tiles[r].data(), tiles[r].size() * sizeof(Tile),

@@ -87,6 +87,18 @@ def lib():
             "mgx_nodal_get_gradphi": [vp, vp, C.c_int],
             "mgx_nodal_solve": [vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_double, C.c_int,
                                 C.POINTER(SolveInfo)],
+            "mgx_mac_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
+            "mgx_mac_destroy": [vp],
+            "mgx_mac_set_domain_bc": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "mgx_mac_set_options": [vp, C.POINTER(MLMGOpts)],
+            "mgx_mac_set_beta": [vp, vp, vp, vp, C.c_int],
+            "mgx_mac_set_const_beta": [vp, C.c_double],
+            "mgx_mac_set_beta_from_density": [vp, vp, C.c_int, C.c_double],
+            "mgx_mac_project": [vp, vp, vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
+                                C.c_int, C.POINTER(SolveInfo)],
+            "mgx_mac_get_fluxes": [vp, vp, vp, vp, C.c_int],
+            "mgx_mac_solve": [vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                              C.POINTER(SolveInfo)],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
                                            vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp,
                                            C.c_double, C.c_double, C.c_int,
@@ -272,4 +284,50 @@ class NodalProjector:
     def close(self):
         if self.h:
             lib().mgx_nodal_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class MacProjector:
+    """Thin mirror of Hydro::MacProjector over the C ABI (test plumbing)."""
+
+    def __init__(self, ctx, level):
+        self.ctx = ctx
+        self.level = level
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_mac_create(ctx.h, C.byref(level), C.byref(self.h)))
+
+    def setDomainBC(self, lo, hi):
+        self.ctx.check(lib().mgx_mac_set_domain_bc(self.h, (C.c_int * 3)(*lo), (C.c_int * 3)(*hi)))
+
+    def set_options(self, opts):
+        self.ctx.check(lib().mgx_mac_set_options(self.h, C.byref(opts)))
+
+    def set_beta(self, bx, by, bz, ng=0):
+        self.ctx.check(lib().mgx_mac_set_beta(self.h, bx.ptr, by.ptr, bz.ptr, ng))
+
+    def updateBeta(self, beta):
+        self.ctx.check(lib().mgx_mac_set_const_beta(self.h, beta))
+
+    def set_beta_from_density(self, rho, rho_ng, dt):
+        self.ctx.check(lib().mgx_mac_set_beta_from_density(self.h, rho.ptr, rho_ng, dt))
+
+    def project(self, u, v, w, umac_ng, rtol, atol=0.0, phi=None, phi_ng=0, mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_mac_project(self.h, u.ptr, v.ptr, w.ptr, umac_ng,
+                                             phi.ptr if phi is not None else None, phi_ng, rtol,
+                                             atol, mode, C.byref(info)))
+        return info
+
+    def getFluxes(self, fx, fy, fz, ng=0):
+        self.ctx.check(lib().mgx_mac_get_fluxes(self.h, fx.ptr, fy.ptr, fz.ptr, ng))
+
+    def solve(self, phi, phi_ng, rhs, rhs_ng, rtol, atol=0.0, mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_mac_solve(self.h, phi.ptr, phi_ng, rhs.ptr, rhs_ng, rtol, atol,
+                                           mode, C.byref(info)))
+        return info
+
+    def close(self):
+        if self.h:
+            lib().mgx_mac_destroy(self.h)
             self.h = C.c_void_p()

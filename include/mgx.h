@@ -179,6 +179,38 @@ MGX_API int mgx_apply_nodal_projection(mgx_nodal h, const double* density, int r
                                        const double* inflow_vel, double rtol, double atol,
                                        int mode, mgx_solve_info* info);
 
+/* ---- MAC projection (Hydro::MacProjector + MLABecLaplacian/MLPoisson + MLMG) */
+typedef struct mgx_mac_s* mgx_mac;
+/* MacProjector(Vector<Geometry>) (src/incflo.cpp:291); the level's box list
+ * lays out the face fields (x-faces of box b span lo..hi+1 in x). */
+MGX_API int mgx_mac_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_mac* out);
+MGX_API int mgx_mac_destroy(mgx_mac h);
+/* setDomainBC (src/convection/incflo_compute_MAC_projected_velocities.cpp:114) with the
+ * get_mac_projection_bc mapping (src/projection/incflo_projection_bc.cpp:44-78):
+ * DIRICHLET for pressure in/outflow, NEUMANN (or INFLOW) for walls and mass inflow. */
+MGX_API int mgx_mac_set_domain_bc(mgx_mac h, const int lo[3], const int hi[3]);
+/* mac_proj.* knobs (maxiter / bottom_maxiter default 200). */
+MGX_API int mgx_mac_set_options(mgx_mac h, const mgx_mlmg_opts* opts);
+/* initProjector(lp_info, beta) / updateCoeffs(beta) (:112, :131): face coefficient
+ * fields (boxed face layout, ng ghosts). */
+MGX_API int mgx_mac_set_beta(mgx_mac h, const double* bx, const double* by, const double* bz,
+                             int ng);
+/* initProjector(ba, dm, lp_info, beta) / updateBeta(beta) (:108, :127): constant beta. */
+MGX_API int mgx_mac_set_const_beta(mgx_mac h, double beta);
+/* incflo's beta_f = dt / (face average of rho) (:70-91), from a boxed cell density. */
+MGX_API int mgx_mac_set_beta_from_density(mgx_mac h, const double* rho, int rho_ng, double dt);
+/* setUMAC + project(rtol, atol) / project(phi, rtol, atol) (:272, :294-299):
+ * solve div(beta grad phi) = div(u_mac), then u_mac -= beta grad phi in place.
+ * phi (boxed cell field, may be NULL) is the initial guess and receives phi. */
+MGX_API int mgx_mac_project(mgx_mac h, double* umac, double* vmac, double* wmac, int umac_ng,
+                            double* phi, int phi_ng, double rtol, double atol, int mode,
+                            mgx_solve_info* info);
+/* getFluxes (src/projection/incflo_apply_cc_projection.cpp:387-389): -beta grad phi on faces. */
+MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, int ng);
+/* MLMG::solve on the MAC operator (boxed cell fields; phi holds the initial guess). */
+MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
+                          double rtol, double atol, int mode, mgx_solve_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -204,3 +204,96 @@ def load():
     lib()
     _setup()
     return _lib
+
+
+# ---- MAC (cell-centred) ---------------------------------------------------
+def face_shape(g, d):
+    n = [g.n[0], g.n[1], g.n[2]]
+    n[d] += 1
+    return (n[2], n[1], n[0])
+
+
+def mac_apply(g, beta, phi):
+    out = np.zeros(g.cell_shape())
+    lib().orc_mac_apply(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]), _p(phi), _p(out))
+    return out
+
+
+def mac_residual(g, beta, b, x):
+    out = np.zeros(g.cell_shape())
+    lib().orc_mac_residual(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]), _p(b), _p(x),
+                           _p(out))
+    return out
+
+
+def mac_gsrb(g, beta, b, x):
+    x = np.array(x, dtype=np.float64)
+    lib().orc_mac_gsrb(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]), _p(b), _p(x))
+    return x
+
+
+def mac_restrict(gf, rf):
+    out = np.zeros(gf.coarsen().cell_shape())
+    lib().orc_mac_restrict(C.byref(gf), _p(rf), _p(out))
+    return out
+
+
+def mac_prolong_add(gf, xc, xf):
+    xf = np.array(xf, dtype=np.float64)
+    lib().orc_mac_prolong_add(C.byref(gf), _p(xc), _p(xf))
+    return xf
+
+
+def mac_beta_coarsen(gf, bf, d):
+    out = np.zeros(face_shape(gf.coarsen(), d))
+    lib().orc_mac_beta_coarsen(C.byref(gf), _p(bf), C.c_int(d), _p(out))
+    return out
+
+
+def mac_beta_from_density(g, rho, dt):
+    b = [np.zeros(face_shape(g, d)) for d in range(3)]
+    lib().orc_mac_beta_from_density(C.byref(g), _p(rho), C.c_double(dt), _p(b[0]), _p(b[1]),
+                                    _p(b[2]))
+    return b
+
+
+def mac_divergence(g, u):
+    out = np.zeros(g.cell_shape())
+    lib().orc_mac_divergence(C.byref(g), _p(u[0]), _p(u[1]), _p(u[2]), _p(out))
+    return out
+
+
+def mac_fluxes(g, beta, phi):
+    f = [np.zeros(face_shape(g, d)) for d in range(3)]
+    lib().orc_mac_fluxes(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]), _p(phi), _p(f[0]),
+                         _p(f[1]), _p(f[2]))
+    return f
+
+
+def mac_bicgstab(g, beta, b, rtol=1e-4, maxiter=200):
+    x = np.zeros(g.cell_shape())
+    its = C.c_int(0)
+    ret = lib().orc_mac_bicgstab(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]), _p(b), _p(x),
+                                 C.c_double(rtol), C.c_int(maxiter), C.byref(its))
+    return x, its.value, ret
+
+
+def mac_solve(g, beta, b, x0=None, rtol=1e-10, atol=0.0, opts=None):
+    x = np.zeros(g.cell_shape()) if x0 is None else np.array(x0, dtype=np.float64)
+    info = Info()
+    o = opts if opts is not None else default_opts(max_iter=200, bottom_max_iter=200)
+    ret = lib().orc_mac_solve(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]),
+                              _p(np.ascontiguousarray(b)), _p(x), C.c_double(rtol),
+                              C.c_double(atol), C.byref(o), C.byref(info))
+    return x, info, ret
+
+
+def mac_project(g, beta, u, rtol=1e-10, atol=0.0, opts=None, phi0=None):
+    """u: list of 3 face arrays, updated in place; returns (phi, info, ret)."""
+    phi = np.zeros(g.cell_shape()) if phi0 is None else np.array(phi0, dtype=np.float64)
+    info = Info()
+    o = opts if opts is not None else default_opts(max_iter=200, bottom_max_iter=200)
+    ret = lib().orc_mac_project(C.byref(g), _p(beta[0]), _p(beta[1]), _p(beta[2]), _p(u[0]),
+                                _p(u[1]), _p(u[2]), _p(phi), C.c_double(rtol), C.c_double(atol),
+                                C.byref(o), C.byref(info))
+    return phi, info, ret

@@ -95,6 +95,33 @@ int orc_nodal_project(const orc_geom* g, const double* sigma, double* vel_g,
                       double* phi, double* gphi, double rtol, double atol,
                       const orc_mg_opts* o, orc_solve_info* info);
 
+/* ---- MAC projection (cell-centred variable-beta Poisson on faces) ----- */
+long orc_mac_face_count(const orc_geom* g, int d);
+int  orc_mac_is_singular(const orc_geom* g);
+void orc_mac_apply(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                   const double* phi, double* out);
+void orc_mac_residual(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                      const double* b, const double* x, double* r);
+void orc_mac_gsrb(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                  const double* b, double* x);
+void orc_mac_restrict(const orc_geom* gf, const double* rf, double* rc);
+void orc_mac_prolong_add(const orc_geom* gf, const double* xc, double* xf);
+void orc_mac_beta_coarsen(const orc_geom* gf, const double* bf, int d, double* bc);
+void orc_mac_beta_from_density(const orc_geom* g, const double* rho, double dt, double* bx,
+                               double* by, double* bz);
+void orc_mac_divergence(const orc_geom* g, const double* u, const double* v, const double* w,
+                        double* div);
+void orc_mac_fluxes(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                    const double* phi, double* fx, double* fy, double* fz);
+int  orc_mac_bicgstab(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                      const double* b, double* x, double rtol, int maxiter, int* iters_out);
+int  orc_mac_solve(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                   const double* b_in, double* x, double rtol, double atol,
+                   const orc_mg_opts* o, orc_solve_info* info);
+int  orc_mac_project(const orc_geom* g, const double* bx, const double* by, const double* bz,
+                     double* u, double* v, double* w, double* phi, double rtol, double atol,
+                     const orc_mg_opts* o, orc_solve_info* info);
+
 /* ---- helpers ---------------------------------------------------------- */
 double orc_norm_inf(const double* a, long n);
 double orc_dot(const double* a, const double* b, long n);
