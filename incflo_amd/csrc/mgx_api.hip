@@ -486,4 +486,24 @@ MGX_API int mgx_apply_nodal_projection(mgx_nodal h, const double* density, int r
     return status;
 }
 
+MGX_API int mgx_nodal_set_unet(mgx_nodal h, mgx_unet u, int level) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(u == nullptr || u->ctx == h->ctx, MGX_ERR_ARG,
+                    "the network must belong to the projector's context");
+        MGX_REQUIRE(level >= 0, MGX_ERR_ARG, "network level must be >= 0");
+        h->solver.net = u ? &u->net : nullptr;
+        h->solver.net_level = level;
+    });
+}
+
+MGX_API int mgx_nodal_training_pair(mgx_nodal h, int level, float* in, float* target) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(in && target, MGX_ERR_ARG, "null output tensors");
+        MGX_REQUIRE(h->have_phi, MGX_ERR_STATE, "training pair: no converged solve yet");
+        h->solver.training_pair(h->rhs.as<double>(), level, in, target, h->stream());
+    });
+}
+
 }  // extern "C"

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "box_kernels.h"
+#include "unet.h"
 #include "mgx_common.h"
 #include "nodal_kernels.h"
 
@@ -100,11 +101,24 @@ class NodalSolver {
     LaunchTimer fine_timer;  // finest-level smoother launches
     bool sigma_set = false;
     bool h32_sigma = false;
+    // learned coarse-level predictor (MGX_MODE_LEARNED)
+    UNet* net = nullptr;
+    int net_level = 2;
+    DevBuf net_in, net_out;
+    int logsig_lvl = -1;  // level whose mean log(sigma) is in scal[40]
 
     ~NodalSolver();
     void setup(const NodalGeom& g, const MGOpts& o);
     double* sigma0() { return sig0_buf.as<double>(); }
-    void sigma_changed() { sigma_set = true; h32_sigma = false; }
+    void sigma_changed() { sigma_set = true; h32_sigma = false; logsig_lvl = -1; }
+    // FP32 correction in H32.L[0].x from the FP32 residual in H32.L[0].b via
+    // the learned predictor at level net_level.
+    void learned_correction(hipStream_t s);
+    void check_learned() const;
+    // features of the current H32.L[0].b at level `lvl` -> in[8*nodes]; scalars in scal
+    void learned_features(int lvl, float* in, hipStream_t s);
+    void prepare_h32(hipStream_t s);
+    void training_pair(const double* b, int lvl, float* in, float* target, hipStream_t s);
     // Solve L phi = b for the canonical rhs b (device, FP64; modified in place:
     // Dirichlet rows zeroed, mean removed for singular problems).  The initial
     // guess is this->phi (zeroed by the caller if wanted).

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "learned_kernels.h"
 #include "mgx_ctx.h"
 #include "nodal_mg.h"
 
@@ -298,6 +299,7 @@ void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
     H32.built = false;
     sigma_set = false;
     h32_sigma = false;
+    logsig_lvl = -1;
 }
 
 void NodalSolver::zero_norm(int slot, hipStream_t s) {
@@ -333,33 +335,113 @@ static void remove_mean(NodalSolver& S, double* a, long n, hipStream_t s) {
     subtract_scalar_kernel<double><<<grid1d(n), 256, 0, s>>>(a, n, S.scal.as<double>() + 33);
 }
 
+void NodalSolver::prepare_h32(hipStream_t s) {
+    if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g0, nlev); h32_sigma = false; }
+    if (!h32_sigma) { H32.set_sigma(sigma0(), s); h32_sigma = true; }
+    res32.alloc(g0.nnode() * sizeof(float));
+    H32.L[0].b = res32.as<float>();
+}
+
+static void prepare_h64(NodalSolver& S, hipStream_t s) {
+    if (!S.H64.built || (int)S.H64.L.size() != S.nlev) S.H64.build(S.g0, S.nlev);
+    // level-0 sigma of the FP64 hierarchy is the solver's own array
+    if (S.H64.L[0].sig != S.sigma0()) {
+        S.H64.L[0].sig_buf.release();
+        S.H64.L[0].sig = S.sigma0();
+    }
+    S.res64.alloc(S.g0.nnode() * sizeof(double));
+    S.H64.L[0].b = S.res64.as<double>();
+    S.H64.set_sigma(S.sigma0(), s);  // coarse levels; level 0 is shared
+}
+
+// ---- learned coarse-level predictor ---------------------------------------
+
+static NodalGeom level_geom(const NodalGeom& g0, int lvl) {
+    NodalGeom g = g0;
+    for (int l = 0; l < lvl; ++l) g = coarsen(g);
+    return g;
+}
+
+void NodalSolver::check_learned() const {
+    MGX_REQUIRE(net != nullptr, MGX_ERR_STATE,
+                "learned mode: no network attached to the nodal projector");
+    MGX_REQUIRE(g0.per[0] && g0.per[1] && g0.per[2], MGX_ERR_UNSUPPORTED,
+                "learned mode: only fully periodic domains are supported");
+    MGX_REQUIRE(net_level >= 0 && net_level < nlev, MGX_ERR_ARG,
+                "learned mode: network level is not a multigrid level of this problem");
+    const NodalGeom gl = level_geom(g0, net_level);
+    const int m = 1 << (net->levels - 1);
+    for (int d = 0; d < 3; ++d)
+        MGX_REQUIRE(gl.nn[d] % m == 0, MGX_ERR_UNSUPPORTED,
+                    "learned mode: the network level's size must be a multiple of 2^(levels-1)");
+}
+
+// H32.L[0].b holds an FP32 right-hand side; restrict it to level lvl and pack
+// the normalised features there.  Scalars land in scal[40..42].
+void NodalSolver::learned_features(int lvl, float* in, hipStream_t s) {
+    for (int l = 0; l < lvl; ++l)
+        nodal_restrict_kernel<float><<<grid3(H32.L[l + 1].g.nn[0], H32.L[l + 1].g.nn[1], H32.L[l + 1].g.nn[2]),
+                                       block3(), 0, s>>>(H32.L[l].kg, H32.L[l + 1].kg, H32.L[l].b,
+                                                          H32.L[l + 1].b);
+    MGLevel<float>& lv = H32.L[lvl];
+    const long n = lv.g.nnode();
+    const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
+    double* sc = scal.as<double>() + 40;
+    const double h2 = std::pow(lv.g.dx[0] * lv.g.dx[1] * lv.g.dx[2], 2.0 / 3.0);
+    if (logsig_lvl != lvl) {
+        feature_partials_kernel<float><<<np, 256, 0, s>>>(lv.kg, lv.b, lv.sig, 1, part.as<double>());
+        feature_scalars_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, n, 1, h2, sc);
+        logsig_lvl = lvl;
+    }
+    feature_partials_kernel<float><<<np, 256, 0, s>>>(lv.kg, lv.b, lv.sig, 0, part.as<double>());
+    feature_scalars_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, n, 0, h2, sc);
+    pack_features_kernel<float><<<grid3(lv.g.nn[0], lv.g.nn[1], lv.g.nn[2]), block3(), 0, s>>>(
+        lv.kg, lv.b, lv.sig, sc, in);
+}
+
+void NodalSolver::learned_correction(hipStream_t s) {
+    const int lc = net_level;
+    MGLevel<float>& lv = H32.L[lc];
+    const long n = lv.g.nnode();
+    net_in.alloc(8 * n * sizeof(float));
+    net_out.alloc(n * sizeof(float));
+    learned_features(lc, net_in.as<float>(), s);
+    NetShape sh{};
+    sh.nx = lv.g.nn[0];
+    sh.ny = lv.g.nn[1];
+    sh.nz = lv.g.nn[2];
+    sh.batch = 1;
+    sh.per[0] = sh.per[1] = sh.per[2] = 1;
+    const hipStream_t saved = net->stream;
+    net->stream = s;
+    net->forward(net_in.as<float>(), sh, net_out.as<float>());
+    net->stream = saved;
+    unscale_kernel<float><<<grid1d(n), 256, 0, s>>>(net_out.as<float>(), n, scal.as<double>() + 40, lv.x);
+    // upward half of a V-cycle from the predicted coarse correction
+    for (int l = lc - 1; l >= 0; --l) {
+        MGLevel<float>& f = H32.L[l];
+        zero_kernel<float><<<grid1d(f.g.nnode()), 256, 0, s>>>(f.x, f.g.nnode());
+        nodal_prolong_add_kernel<float><<<grid3(f.g.nn[0], f.g.nn[1], f.g.nn[2]), block3(), 0, s>>>(
+            f.kg, H32.L[l + 1].kg, H32.L[l + 1].x, f.x);
+        H32.smooth(l, opts.nu2, false, opts, s);
+    }
+}
+
 void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_info* info,
                         hipStream_t s) {
     MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "nodal solve: sigma has not been set");
-    MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_UNSUPPORTED,
-                "nodal solve: mode not available in this build");
+    MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED || mode == MGX_MODE_LEARNED,
+                MGX_ERR_UNSUPPORTED, "nodal solve: unknown mode");
+    const bool learned = (mode == MGX_MODE_LEARNED);
+    if (learned) check_learned();
     const NodalGeom& g = g0;
     const KGeom kg = kgeom(g);
     const long nn = g.nnode();
-    const bool mixed = (mode == MGX_MODE_MIXED);
+    bool mixed = (mode != MGX_MODE_FP64);
     H64.timer = &fine_timer;
     H32.timer = &fine_timer;
-    if (mixed) {
-        if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g, nlev); h32_sigma = false; }
-        if (!h32_sigma) { H32.set_sigma(sigma0(), s); h32_sigma = true; }
-        res32.alloc(nn * sizeof(float));
-        H32.L[0].b = res32.as<float>();
-    } else {
-        if (!H64.built || (int)H64.L.size() != nlev) H64.build(g, nlev);
-        // level-0 sigma of the FP64 hierarchy is the solver's own array
-        if (H64.L[0].sig != sigma0()) {
-            H64.L[0].sig_buf.release();
-            H64.L[0].sig = sigma0();
-        }
-        res64.alloc(nn * sizeof(double));
-        H64.L[0].b = res64.as<double>();
-        H64.set_sigma(sigma0(), s);  // coarse levels; level 0 is shared
-    }
+    if (mixed) prepare_h32(s);
+    else prepare_h64(*this, s);
 
     const bool has_dir = g.dlo[0] || g.dlo[1] || g.dlo[2] || g.dhi[0] || g.dhi[1] || g.dhi[2];
     if (has_dir) {
@@ -379,47 +461,86 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     const double bnorm = read_norm(0, s);
     const Coef<double> c64 = make_coef<double>(g.dx);
 
-    // initial residual
-    zero_norm(1, s);
-    if (mixed) {
-        SArgs<double, float> a{};
-        a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
-        a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
-        launch_stencil<double, float, KS_RESID>(g, a, s, false);
-    } else {
-        SArgs<double, double> a{};
-        a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
-        a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
-        launch_stencil<double, double, KS_RESID>(g, a, s, false);
-    }
-    const double r0 = read_norm(1, s);
-    const double tol = std::max(atol, rtol * bnorm);
-    double rn = r0;
-    int it = 0;
-    bool conv = (r0 <= tol);
-    while (!conv && it < opts.maxiter) {
-        ++it;
+    // r = b - L phi into the V-cycle's level-0 right-hand side
+    auto residual = [&](bool into32) {
         zero_norm(1, s);
-        double* pout = (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>();
-        if (mixed) {
-            H32.vcycle(opts, s, ibuf.as<int>());
+        if (into32) {
+            SArgs<double, float> a{};
+            a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
+            a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
+            launch_stencil<double, float, KS_RESID>(g, a, s, false);
+        } else {
+            SArgs<double, double> a{};
+            a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
+            a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
+            launch_stencil<double, double, KS_RESID>(g, a, s, false);
+        }
+        return read_norm(1, s);
+    };
+    // phi_out = phi + cor, r = b - L phi_out (FP64), returns ||r||_inf
+    auto update = [&](bool from32, double* pout) {
+        zero_norm(1, s);
+        if (from32) {
             SArgs<double, float> a{};
             a.g = kg; a.c = c64; a.x = phi; a.cor = H32.L[0].x; a.b = b; a.sig = sigma0();
             a.xout = pout; a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
             launch_stencil<double, float, KS_UPDRES>(g, a, s, false);
         } else {
-            H64.vcycle(opts, s, ibuf.as<int>());
             SArgs<double, double> a{};
             a.g = kg; a.c = c64; a.x = phi; a.cor = H64.L[0].x; a.b = b; a.sig = sigma0();
             a.xout = pout; a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
             launch_stencil<double, double, KS_UPDRES>(g, a, s, false);
         }
+        return read_norm(1, s);
+    };
+    auto other = [&]() { return (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>(); };
+
+    const double r0 = residual(mixed);
+    const double tol = std::max(atol, rtol * bnorm);
+    double rn = r0;
+    int it = 0;
+    int fallback = 0;
+    bool conv = (r0 <= tol);
+    if (learned && !conv && opts.maxiter > 0) {
+        // learned initial guess / coarse correction; kept only if it reduces ||r||
+        ++it;
+        learned_correction(s);
+        double* pout = other();
+        const double rl = update(true, pout);
+        if (opts.verbose)
+            fprintf(stderr, "mgx nodal MLMG: learned step |r| %.6e (from %.6e)\n", rl, rn);
+        if (rl < rn) {
+            phi = pout;
+            rn = rl;
+        } else {
+            fallback = 1;
+            rn = residual(true);
+        }
+        conv = (rn <= tol);
+    }
+    int stall = 0;
+    while (!conv && it < opts.maxiter) {
+        ++it;
+        double* pout = other();
+        if (mixed) H32.vcycle(opts, s, ibuf.as<int>());
+        else H64.vcycle(opts, s, ibuf.as<int>());
+        const double prev = rn;
+        rn = update(mixed, pout);
         phi = pout;
-        rn = read_norm(1, s);
         if (opts.verbose)
             fprintf(stderr, "mgx nodal MLMG: iter %d |r| %.6e rel %.3e\n", it, rn,
                     bnorm > 0 ? rn / bnorm : rn);
         conv = (rn <= tol);
+        if (learned && mixed && !conv && opts.stall_iters > 0) {
+            // plain FP64 MLMG takes over when the reduced-precision cycle stalls
+            stall = (rn > 0.9 * prev) ? stall + 1 : 0;
+            if (stall >= opts.stall_iters) {
+                fallback = 1;
+                mixed = false;
+                prepare_h64(*this, s);
+                rn = residual(false);
+            }
+        }
     }
     if (g.singular()) remove_mean(*this, phi, nn, s);
     int bot = 0;
@@ -429,12 +550,29 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         info->iters = it;
         info->converged = conv ? 1 : 0;
         info->nlevels = nlev;
-        info->fallback_used = 0;
+        info->fallback_used = fallback;
         info->bnorm = bnorm;
         info->resnorm0 = r0;
         info->resnorm = rn;
         info->rel_resid = bnorm > 0 ? rn / bnorm : rn;
     }
+}
+
+// Training pair for the learned predictor from the canonical rhs b and the
+// converged phi: in = features of b at level lvl, target = phi injected there.
+void NodalSolver::training_pair(const double* b, int lvl, float* in, float* target, hipStream_t s) {
+    MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "training pair: sigma has not been set");
+    MGX_REQUIRE(g0.per[0] && g0.per[1] && g0.per[2], MGX_ERR_UNSUPPORTED,
+                "training pair: only fully periodic domains are supported");
+    MGX_REQUIRE(lvl >= 0 && lvl < nlev, MGX_ERR_ARG, "training pair: level out of range");
+    prepare_h32(s);
+    const long nn = g0.nnode();
+    cast_kernel<float, double><<<grid1d(nn), 256, 0, s>>>(b, res32.as<float>(), nn);
+    learned_features(lvl, in, s);
+    const MGLevel<float>& lv = H32.L[lvl];
+    inject_target_kernel<<<grid3(lv.g.nn[0], lv.g.nn[1], lv.g.nn[2]), block3(), 0, s>>>(
+        H32.L[0].kg, lv.kg, 1 << lvl, phi, scal.as<double>() + 40, target);
+    MGX_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,372 @@
+// unet_kernels.h -- kernels of the learned coarse-level predictor: a 3-D
+// convolutional U-net with BF16 weights and FP32 accumulation on the CDNA4
+// matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Tensors are channels-last FP32: t[(b*S + v)*C + c], v = x + nx*(y + ny*z).
+// A 3x3x3 convolution is an implicit GEMM
+//     out[v][co] = bias[co] + sum_k A[v][k] W[co][k],  k = tap*Cin + ci,
+//     A[v][tap*Cin+ci] = in[v + off(tap)][ci]   (periodic wrap or zero pad),
+// tap = (dz+1)*9 + (dy+1)*3 + (dx+1).  Weights are stored BF16 as W[co][Kp]
+// with K = 27*Cin padded to Kp (a multiple of 32) by zeros, so one MFMA step
+// covers 32 consecutive k.  Cin is a multiple of 8: the 8 k values a lane
+// feeds per step are 8 consecutive input channels of one tap (one 32-byte
+// activation load converted to BF16 in registers).
+//
+// MFMA 16x16x32 operand layout (lane l of a 64-wide wave, i = 0..7):
+//   A[row = l%16][k = 8*(l/16) + i],  B[k = 8*(l/16) + i][col = l%16],
+//   D[row = 4*(l/16) + j][col = l%16], j = 0..3.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mgx {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct ConvGeom {
+    int nx, ny, nz;     // spatial size of one sample
+    int per[3];         // 1: periodic wrap, 0: zero padding
+    int batch;
+    int cin, cout;      // cin % 8 == 0, cout % 16 == 0
+    int kp;             // padded K (multiple of 32)
+};
+
+__device__ __forceinline__ int conv_wrap(int i, int n, int per, bool& valid) {
+    if (i >= 0 && i < n) return i;
+    if (per) return i < 0 ? i + n : i - n;
+    valid = false;
+    return 0;
+}
+
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+    // round to nearest even (finite inputs)
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 to_bf16x8(const float* p, bool valid) {
+    bf16x8 r;
+    if (valid) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        const float4 b = *reinterpret_cast<const float4*>(p + 4);
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const unsigned short s = bf16_bits(v[i]);
+            r[i] = __builtin_bit_cast(__bf16, s);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = __builtin_bit_cast(__bf16, (unsigned short)0);
+    }
+    return r;
+}
+
+// Forward 3x3x3 convolution (+bias, optional fused ReLU).  One wave computes
+// 16 voxels x all output channels (cout/16 accumulator tiles); a workgroup of
+// 4 waves covers 64 consecutive voxels of the batch.
+template <int NT>
+static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, const float* __restrict__ in,
+                                                               const unsigned short* __restrict__ w,
+                                                               const float* __restrict__ bias,
+                                                               float* __restrict__ out, int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long S = (long)g.nx * g.ny * g.nz;
+    const long nvox = S * g.batch;
+    const long v0 = ((long)blockIdx.x * 4 + wave) * 16;
+    if (v0 >= nvox) return;
+    const int row = lane & 15, kq = lane >> 4;
+    // this lane's A row: voxel v0 + row
+    long va = v0 + row;
+    const bool rowok = va < nvox;
+    if (!rowok) va = nvox - 1;
+    const int b = (int)(va / S);
+    const int vs = (int)(va - (long)b * S);
+    const int x = vs % g.nx, y = (vs / g.nx) % g.ny, z = vs / (g.nx * g.ny);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int K = 27 * g.cin;
+    for (int k0 = 0; k0 < g.kp; k0 += 32) {
+        const int k = k0 + 8 * kq;
+        bf16x8 afrag;
+        if (k < K) {
+            const int tap = k / g.cin, ci = k - tap * g.cin;
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            bool valid = rowok;
+            const int xx = conv_wrap(x + dx, g.nx, g.per[0], valid);
+            const int yy = conv_wrap(y + dy, g.ny, g.per[1], valid);
+            const int zz = conv_wrap(z + dz, g.nz, g.per[2], valid);
+            const long nv = (long)b * S + xx + (long)g.nx * (yy + (long)g.ny * zz);
+            afrag = to_bf16x8(in + nv * g.cin + ci, valid);
+        } else {
+            afrag = to_bf16x8(in, false);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + row;
+            const bf16x8 bfrag = *reinterpret_cast<const bf16x8*>(w + (long)co * g.kp + k);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + row;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long v = v0 + 4 * kq + j;
+            if (v < nvox) {
+                float r = acc[t][j] + bv;
+                if (relu) r = fmaxf(r, 0.f);
+                out[v * g.cout + co] = r;
+            }
+        }
+    }
+}
+
+// Weight gradient: dW[co][k] += sum_v dY[v][co] * A[v][k] (row stride ldw).  One wave owns one
+// 16 (co) x 16 (k) tile and a range of voxels; partial sums are added
+// atomically (FP32).
+static __global__ void __launch_bounds__(64) conv3d_dw_kernel(ConvGeom g, const float* __restrict__ in,
+                                                              const float* __restrict__ dy,
+                                                              float* __restrict__ dw, int ldw,
+                                                              int vchunk) {
+    const int lane = threadIdx.x;
+    const int row = lane & 15, kq = lane >> 4;
+    const int ntk = g.kp / 16;
+    const int tile = blockIdx.x % (ntk * (g.cout / 16));
+    const int chunk = blockIdx.x / (ntk * (g.cout / 16));
+    const int co0 = (tile / ntk) * 16, kk0 = (tile % ntk) * 16;
+    const long S = (long)g.nx * g.ny * g.nz;
+    const long nvox = S * g.batch;
+    const long vbeg = (long)chunk * vchunk;
+    const long vend = vbeg + vchunk < nvox ? vbeg + vchunk : nvox;
+    const int K = 27 * g.cin;
+    // this lane's B column: k = kk0 + row
+    const int kcol = kk0 + row;
+    int tap = 0, ci = 0, dz = 0, dyo = 0, dx = 0;
+    const bool kok = kcol < K;
+    if (kok) {
+        tap = kcol / g.cin;
+        ci = kcol - tap * g.cin;
+        dz = tap / 9 - 1;
+        dyo = (tap / 3) % 3 - 1;
+        dx = tap % 3 - 1;
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (long vb = vbeg; vb < vend; vb += 32) {
+        bf16x8 a, bb;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const long v = vb + 8 * kq + i;
+            float av = 0.f, bv = 0.f;
+            if (v < vend) {
+                av = dy[v * g.cout + co0 + row];
+                if (kok) {
+                    const int b = (int)(v / S);
+                    const int vs = (int)(v - (long)b * S);
+                    const int x = vs % g.nx, y = (vs / g.nx) % g.ny, z = vs / (g.nx * g.ny);
+                    bool valid = true;
+                    const int xx = conv_wrap(x + dx, g.nx, g.per[0], valid);
+                    const int yy = conv_wrap(y + dyo, g.ny, g.per[1], valid);
+                    const int zz = conv_wrap(z + dz, g.nz, g.per[2], valid);
+                    if (valid) bv = in[((long)b * S + xx + (long)g.nx * (yy + (long)g.ny * zz)) * g.cin + ci];
+                }
+            }
+            a[i] = __builtin_bit_cast(__bf16, bf16_bits(av));
+            bb[i] = __builtin_bit_cast(__bf16, bf16_bits(bv));
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
+    }
+    // D[row = co offset 4*kq + j][col = k offset row]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int co = co0 + 4 * kq + j;
+        const int kc = kk0 + row;
+        if (kc < K) atomicAdd(&dw[(long)co * ldw + kc], acc[j]);
+    }
+}
+
+// bias gradient: db[c] += sum_v dY[v][c]
+static __global__ void __launch_bounds__(256) bias_grad_kernel(const float* __restrict__ dy, long nvox,
+                                                               int c, float* __restrict__ db) {
+    const int co = threadIdx.x % c;
+    const int lanes = 256 / c * c;
+    if ((int)threadIdx.x >= lanes) return;
+    float s = 0.f;
+    for (long v = (long)blockIdx.x * (256 / c) + threadIdx.x / c; v < nvox;
+         v += (long)gridDim.x * (256 / c))
+        s += dy[v * c + co];
+    atomicAdd(&db[co], s);
+}
+
+// Data-gradient weights: Wt[ci][tap'*Cout + co] = W[co][(26 - tap')*Cin + ci]
+// (flipped taps, transposed channels; dX is then a forward conv of dY).
+static __global__ void weight_transpose_kernel(const unsigned short* __restrict__ w, int cin, int cout,
+                                               int kp, unsigned short* __restrict__ wt, int kpt) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= cin * kpt) return;
+    const int ci = id / kpt, k = id % kpt;
+    unsigned short v = 0;
+    if (k < 27 * cout) {
+        const int tp = k / cout, co = k % cout;
+        v = w[(long)co * kp + (26 - tp) * cin + ci];
+    }
+    wt[id] = v;
+}
+
+// FP32 master weights [co][K] (K = 27*cin) -> BF16 [co][Kp] (zero padded)
+static __global__ void weight_to_bf16_kernel(const float* __restrict__ wm, int cout, int K, int kp,
+                                             unsigned short* __restrict__ wb) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= cout * kp) return;
+    const int co = id / kp, k = id % kp;
+    wb[id] = k < K ? bf16_bits(wm[(long)co * K + k]) : (unsigned short)0;
+}
+
+// dY *= (y > 0)
+static __global__ void relu_bwd_kernel(const float* __restrict__ y, float* __restrict__ dy, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n && !(y[id] > 0.f)) dy[id] = 0.f;
+}
+
+// 2x2x2 average pooling (channels last)
+static __global__ void avgpool_fwd_kernel(const float* __restrict__ in, int nx, int ny, int nz, int c,
+                                          int batch, float* __restrict__ out) {
+    const int ox = nx / 2, oy = ny / 2, oz = nz / 2;
+    const long n = (long)ox * oy * oz * batch * c;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int ch = id % c;
+    long v = id / c;
+    const int X = v % ox; v /= ox;
+    const int Y = v % oy; v /= oy;
+    const int Z = v % oz;
+    const int b = (int)(v / oz);
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const long fv = (long)b * nx * ny * nz + (2 * X + (q & 1)) +
+                        (long)nx * ((2 * Y + ((q >> 1) & 1)) + (long)ny * (2 * Z + ((q >> 2) & 1)));
+        s += in[fv * c + ch];
+    }
+    out[id] = 0.125f * s;
+}
+
+static __global__ void avgpool_bwd_kernel(const float* __restrict__ dout, int nx, int ny, int nz,
+                                          int c, int batch, float* __restrict__ din) {
+    const long n = (long)nx * ny * nz * batch * c;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int ch = id % c;
+    long v = id / c;
+    const int x = v % nx; v /= nx;
+    const int y = v % ny; v /= ny;
+    const int z = v % nz;
+    const int b = (int)(v / nz);
+    const int ox = nx / 2, oy = ny / 2, oz = nz / 2;
+    const long cv = (long)b * ox * oy * oz + (x / 2) + (long)ox * ((y / 2) + (long)oy * (z / 2));
+    din[id] = 0.125f * dout[cv * c + ch];
+}
+
+// nearest x2 upsampling of `in` (c channels) into channels [coff, coff+c) of
+// `out` (ctot channels); the backward sums the 8 children.
+static __global__ void upsample_fwd_kernel(const float* __restrict__ in, int nx, int ny, int nz, int c,
+                                           int batch, float* __restrict__ out, int ctot, int coff) {
+    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
+    const long n = (long)fx * fy * fz * batch * c;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int ch = id % c;
+    long v = id / c;
+    const int x = v % fx; v /= fx;
+    const int y = v % fy; v /= fy;
+    const int z = v % fz;
+    const int b = (int)(v / fz);
+    const long cv = (long)b * nx * ny * nz + (x / 2) + (long)nx * ((y / 2) + (long)ny * (z / 2));
+    const long fv = (long)b * fx * fy * fz + x + (long)fx * (y + (long)fy * z);
+    out[fv * ctot + coff + ch] = in[cv * c + ch];
+}
+
+static __global__ void upsample_bwd_kernel(const float* __restrict__ dout, int nx, int ny, int nz,
+                                           int c, int batch, int ctot, int coff,
+                                           float* __restrict__ din) {
+    const long n = (long)nx * ny * nz * batch * c;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int ch = id % c;
+    long v = id / c;
+    const int X = v % nx; v /= nx;
+    const int Y = v % ny; v /= ny;
+    const int Z = v % nz;
+    const int b = (int)(v / nz);
+    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const long fv = (long)b * fx * fy * fz + (2 * X + (q & 1)) +
+                        (long)fx * ((2 * Y + ((q >> 1) & 1)) + (long)fy * (2 * Z + ((q >> 2) & 1)));
+        s += dout[fv * ctot + coff + ch];
+    }
+    din[id] = s;
+}
+
+// copy c channels between tensors with different channel counts / offsets
+static __global__ void channel_copy_kernel(const float* __restrict__ src, int csrc, int soff,
+                                           float* __restrict__ dst, int cdst, int doff, int c, long nvox,
+                                           int accumulate) {
+    const long n = nvox * c;
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int ch = id % c;
+    const long v = id / c;
+    const float val = src[v * csrc + soff + ch];
+    float& d = dst[v * cdst + doff + ch];
+    d = accumulate ? d + val : val;
+}
+
+// MSE loss 1/N sum (y - t)^2 over channel 0 of y (cy channels); gradient into dy
+static __global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ y, int cy,
+                                                         const float* __restrict__ t, long nvox,
+                                                         float* __restrict__ dy, float* __restrict__ loss) {
+    __shared__ float red[256];
+    float s = 0.f;
+    const float inv = 1.f / (float)nvox;
+    for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (long)gridDim.x * 256) {
+        const float e = y[v * cy] - t[v];
+        s += e * e;
+        dy[v * cy] = 2.f * e * inv;
+        for (int c = 1; c < cy; ++c) dy[v * cy + c] = 0.f;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv);
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master parameters
+static __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                   float* __restrict__ m, float* __restrict__ v, long n, float lr,
+                                   float b1, float b2, float eps, float c1, float c2) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const float gi = g[id];
+    const float mi = b1 * m[id] + (1.f - b1) * gi;
+    const float vi = b2 * v[id] + (1.f - b2) * gi * gi;
+    m[id] = mi;
+    v[id] = vi;
+    p[id] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+static __global__ void zero_f32_kernel(float* a, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) a[id] = 0.f;
+}
+
+}  // namespace mgx

@@ -99,6 +99,22 @@ def lib():
             "mgx_mac_get_fluxes": [vp, vp, vp, vp, C.c_int],
             "mgx_mac_solve": [vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_double, C.c_int,
                               C.POINTER(SolveInfo)],
+            "mgx_unet_create": [vp, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(vp)],
+            "mgx_unet_destroy": [vp],
+            "mgx_unet_num_params": [vp, C.POINTER(C.c_longlong)],
+            "mgx_unet_get_params": [vp, vp, C.c_longlong],
+            "mgx_unet_set_params": [vp, vp, C.c_longlong],
+            "mgx_unet_forward": [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
+            "mgx_unet_gradients": [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_int), vp, C.c_longlong],
+            "mgx_unet_train_step": [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_int), C.c_float, C.POINTER(C.c_float)],
+            "mgx_conv3d_forward": [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.POINTER(C.c_int), C.c_int],
+            "mgx_conv3d_backward": [vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.POINTER(C.c_int)],
+            "mgx_nodal_set_unet": [vp, vp, C.c_int],
+            "mgx_nodal_training_pair": [vp, C.c_int, vp, vp],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
                                            vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp,
                                            C.c_double, C.c_double, C.c_int,
@@ -281,6 +297,13 @@ class NodalProjector:
         self.ctx.check(st)
         return info
 
+    def set_unet(self, net, level):
+        """Attach a UNet for MODE_LEARNED (None detaches)."""
+        self.ctx.check(lib().mgx_nodal_set_unet(self.h, net.h if net is not None else None, level))
+
+    def training_pair(self, level, in_dev, target_dev):
+        self.ctx.check(lib().mgx_nodal_training_pair(self.h, level, in_dev.ptr, target_dev.ptr))
+
     def close(self):
         if self.h:
             lib().mgx_nodal_destroy(self.h)
@@ -331,3 +354,71 @@ class MacProjector:
         if self.h:
             lib().mgx_mac_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def _per(periodic):
+    return (C.c_int * 3)(*[1 if p else 0 for p in periodic])
+
+
+class UNet:
+    """The learned coarse-level predictor (BF16-MFMA 3-D conv U-net)."""
+
+    def __init__(self, ctx, base=16, levels=4, seed=0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_unet_create(ctx.h, base, levels, seed, C.byref(self.h)))
+        n = C.c_longlong()
+        ctx.check(lib().mgx_unet_num_params(self.h, C.byref(n)))
+        self.nparam = n.value
+
+    def get_params(self):
+        out = np.empty(self.nparam, dtype=np.float32)
+        self.ctx.check(lib().mgx_unet_get_params(self.h, out.ctypes.data, self.nparam))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self.ctx.check(lib().mgx_unet_set_params(self.h, p.ctypes.data, self.nparam))
+
+    def forward(self, x_dev, out_dev, shape, periodic=(1, 1, 1)):
+        """shape = (batch, nz, ny, nx); x: [batch][nz][ny][nx][8], out: [batch][nz][ny][nx]."""
+        b, nz, ny, nx = shape
+        self.ctx.check(lib().mgx_unet_forward(self.h, x_dev.ptr, out_dev.ptr, nx, ny, nz, b,
+                                              _per(periodic)))
+
+    def gradients(self, x_dev, dout_dev, shape, periodic=(1, 1, 1)):
+        b, nz, ny, nx = shape
+        g = np.empty(self.nparam, dtype=np.float32)
+        self.ctx.check(lib().mgx_unet_gradients(self.h, x_dev.ptr, dout_dev.ptr, nx, ny, nz, b,
+                                                _per(periodic), g.ctypes.data, self.nparam))
+        return g
+
+    def train_step(self, x_dev, t_dev, shape, lr, periodic=(1, 1, 1)):
+        b, nz, ny, nx = shape
+        loss = C.c_float()
+        self.ctx.check(lib().mgx_unet_train_step(self.h, x_dev.ptr, t_dev.ptr, nx, ny, nz, b,
+                                                 _per(periodic), lr, C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().mgx_unet_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def conv3d_forward(ctx, x_dev, w_dev, b_dev, out_dev, shape, cin, cout, periodic=(1, 1, 1),
+                   relu=False):
+    b, nz, ny, nx = shape
+    ctx.check(lib().mgx_conv3d_forward(ctx.h, x_dev.ptr, w_dev.ptr,
+                                       b_dev.ptr if b_dev is not None else None, out_dev.ptr,
+                                       nx, ny, nz, b, cin, cout, _per(periodic), int(relu)))
+
+
+def conv3d_backward(ctx, x_dev, w_dev, dout_dev, din_dev, dw_dev, db_dev, shape, cin, cout,
+                    periodic=(1, 1, 1)):
+    b, nz, ny, nx = shape
+    opt = lambda a: a.ptr if a is not None else None
+    ctx.check(lib().mgx_conv3d_backward(ctx.h, x_dev.ptr, w_dev.ptr, dout_dev.ptr, opt(din_dev),
+                                        opt(dw_dev), opt(db_dev), nx, ny, nz, b, cin, cout,
+                                        _per(periodic)))

@@ -211,6 +211,52 @@ MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, in
 MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
                           double rtol, double atol, int mode, mgx_solve_info* info);
 
+/* ---- learned coarse-level predictor --------------------------------------
+ * No counterpart in the reference (incflo has no learned component); these
+ * entry points serve MGX_MODE_LEARNED of the nodal solve.  A 3-D conv U-net
+ * (4 levels, base_channels -> 8*base_channels, 3x3x3 kernels, BF16 weights,
+ * FP32 accumulation on the matrix cores) maps the normalised coarse residual
+ * and coarse log(sigma) to the normalised coarse correction.
+ * Tensors are device FP32, channels last: in[batch][nz][ny][nx][8]
+ * (channels 0,1 used), out[batch][nz][ny][nx]; sizes multiples of 8. */
+typedef struct mgx_unet_s* mgx_unet;
+MGX_API int mgx_unet_create(mgx_ctx ctx, int base_channels, int levels, unsigned long long seed,
+                            mgx_unet* out);
+MGX_API int mgx_unet_destroy(mgx_unet u);
+MGX_API int mgx_unet_num_params(mgx_unet u, long long* n);
+/* FP32 master parameters (host arrays of num_params floats). */
+MGX_API int mgx_unet_get_params(mgx_unet u, float* host, long long n);
+MGX_API int mgx_unet_set_params(mgx_unet u, const float* host, long long n);
+MGX_API int mgx_unet_forward(mgx_unet u, const float* in, float* out, int nx, int ny, int nz,
+                             int batch, const int periodic[3]);
+/* forward + backward with the given upstream gradient dL/d(out); the
+ * parameter gradients are copied to host_grads (n = num_params). */
+MGX_API int mgx_unet_gradients(mgx_unet u, const float* in, const float* dout, int nx, int ny,
+                               int nz, int batch, const int periodic[3], float* host_grads,
+                               long long n);
+/* one Adam step on the mean-squared error against target[batch][nz][ny][nx]. */
+MGX_API int mgx_unet_train_step(mgx_unet u, const float* in, const float* target, int nx, int ny,
+                                int nz, int batch, const int periodic[3], float lr, float* loss);
+/* The BF16-MFMA 3x3x3 convolution the U-net is built from (operator level):
+ * out = conv(in, w) + bias (ReLU if relu), w: FP32 [cout][27*cin] with
+ * k = tap*cin + ci, tap = (dz+1)*9 + (dy+1)*3 + (dx+1); periodic or zero padding. */
+MGX_API int mgx_conv3d_forward(mgx_ctx ctx, const float* in, const float* w, const float* bias,
+                               float* out, int nx, int ny, int nz, int batch, int cin, int cout,
+                               const int periodic[3], int relu);
+/* Gradients of that convolution (no ReLU) for upstream dout: din (may be
+ * NULL), dw [cout][27*cin] and db [cout] (either may be NULL), overwritten. */
+MGX_API int mgx_conv3d_backward(mgx_ctx ctx, const float* in, const float* w, const float* dout,
+                                float* din, float* dw, float* db, int nx, int ny, int nz,
+                                int batch, int cin, int cout, const int periodic[3]);
+/* Attach a U-net to a nodal projector for MGX_MODE_LEARNED: it acts on the
+ * multigrid level `level` (coarsened `level` times; sizes there must be
+ * multiples of 8, periodic domains only).  u may be NULL to detach. */
+MGX_API int mgx_nodal_set_unet(mgx_nodal h, mgx_unet u, int level);
+/* After a solve: the network input (normalised coarse rhs and log sigma,
+ * [nz][ny][nx][8] at `level`) and the normalised target (the solution
+ * injected at that level's nodes) -- one training pair for mgx_unet_train_step. */
+MGX_API int mgx_nodal_training_pair(mgx_nodal h, int level, float* in, float* target);
+
 #ifdef __cplusplus
 }
 #endif

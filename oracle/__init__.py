@@ -198,6 +198,7 @@ def _setup():
     for name in ["orc_norm_inf", "orc_dot", "orc_mean"]:
         getattr(L, name).restype = C.c_double
     L.orc_nodal_nlevels.restype = C.c_int
+    L.orc_unet_nparam.restype = C.c_long
 
 
 def load():
@@ -297,3 +298,65 @@ def mac_project(g, beta, u, rtol=1e-10, atol=0.0, opts=None, phi0=None):
                                 _p(u[1]), _p(u[2]), _p(phi), C.c_double(rtol), C.c_double(atol),
                                 C.byref(o), C.byref(info))
     return phi, info, ret
+
+
+# ---- learned predictor: 3x3x3 convolution and U-net ------------------------
+# Tensors are channels last: shape (batch, nz, ny, nx, c), float64.
+def _per(periodic):
+    return (C.c_int * 3)(*[1 if p else 0 for p in periodic])
+
+
+def conv3d_forward(x, w, bias, cout, periodic=(1, 1, 1), relu=False, bf16=False):
+    """x: (batch, nz, ny, nx, cin); w: (cout, 27*cin); bias: (cout,) or None."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    batch, nz, ny, nx, cin = x.shape
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    assert w.shape == (cout, 27 * cin)
+    out = np.zeros((batch, nz, ny, nx, cout))
+    bp = _p(np.ascontiguousarray(bias, dtype=np.float64)) if bias is not None else None
+    lib().orc_conv3d_fwd(_p(x), _p(w), bp, _p(out), nx, ny, nz, batch, cin, cout,
+                         _per(periodic), int(relu), int(bf16))
+    return out
+
+
+def conv3d_backward(x, w, dout, periodic=(1, 1, 1), bf16=False):
+    """Returns (din, dw, db) for out = conv(x, w) + b and upstream dout."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dout = np.ascontiguousarray(dout, dtype=np.float64)
+    batch, nz, ny, nx, cin = x.shape
+    cout = dout.shape[-1]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    din = np.zeros_like(x)
+    dw = np.zeros((cout, 27 * cin))
+    db = np.zeros(cout)
+    lib().orc_conv3d_bwd(_p(x), _p(w), _p(dout), _p(din), _p(dw), _p(db), nx, ny, nz, batch, cin,
+                         cout, _per(periodic), int(bf16))
+    return din, dw, db
+
+
+def unet_nparam(base=16, levels=4, cin0=8):
+    return int(lib().orc_unet_nparam(base, levels, cin0))
+
+
+def unet_forward(params, x, base=16, levels=4, periodic=(1, 1, 1), bf16=False):
+    """x: (batch, nz, ny, nx, cin0) -> (batch, nz, ny, nx)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    batch, nz, ny, nx, cin0 = x.shape
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    assert p.size == unet_nparam(base, levels, cin0)
+    out = np.zeros((batch, nz, ny, nx))
+    lib().orc_unet_forward(_p(p), base, levels, cin0, _p(x), _p(out), nx, ny, nz, batch,
+                           _per(periodic), int(bf16))
+    return out
+
+
+def unet_grad(params, x, dout, base=16, levels=4, periodic=(1, 1, 1), bf16=False):
+    """Gradient of sum(dout * unet(x)) with respect to the flat parameters."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    batch, nz, ny, nx, cin0 = x.shape
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    g = np.zeros_like(p)
+    lib().orc_unet_grad(_p(p), base, levels, cin0, _p(x),
+                        _p(np.ascontiguousarray(dout, dtype=np.float64)), _p(g), nx, ny, nz,
+                        batch, _per(periodic), int(bf16))
+    return g

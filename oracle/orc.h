@@ -122,6 +122,20 @@ int  orc_mac_project(const orc_geom* g, const double* bx, const double* by, cons
                      double* u, double* v, double* w, double* phi, double rtol, double atol,
                      const orc_mg_opts* o, orc_solve_info* info);
 
+/* ---- learned predictor: 3x3x3 convolution and U-net (orc_unet.c) ------- */
+void orc_conv3d_fwd(const double* in, const double* w, const double* bias, double* out, int nx,
+                    int ny, int nz, int batch, int cin, int cout, const int per[3], int relu,
+                    int bf16);
+void orc_conv3d_bwd(const double* in, const double* w, const double* dout, double* din,
+                    double* dw, double* db, int nx, int ny, int nz, int batch, int cin, int cout,
+                    const int per[3], int bf16);
+long orc_unet_nparam(int base, int levels, int cin0);
+void orc_unet_forward(const double* p, int base, int levels, int cin0, const double* in,
+                      double* out, int nx, int ny, int nz, int batch, const int per[3], int bf16);
+void orc_unet_grad(const double* p, int base, int levels, int cin0, const double* in,
+                   const double* dout, double* grad, int nx, int ny, int nz, int batch,
+                   const int per[3], int bf16);
+
 /* ---- helpers ---------------------------------------------------------- */
 double orc_norm_inf(const double* a, long n);
 double orc_dot(const double* a, const double* b, long n);
