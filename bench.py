@@ -88,7 +88,12 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--max-grid-size", type=int, default=32)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64,mixed,learned",
+                    help="learned runs only when trained weights are present")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "incflo_amd", "data", "unet_b16_l4.npy"))
+    ap.add_argument("--net-level", type=int, default=-1,
+                    help="multigrid level the network acts on (default: the level whose size "
+                         "matches the training level's)")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -126,10 +131,28 @@ def main():
     bufs = (vel0, vel, rhod, gp, p)
     nnodes = n ** 3
 
-    modes = {"fp64": mgx.MODE_FP64, "mixed": mgx.MODE_MIXED}
+    modes = {"fp64": mgx.MODE_FP64, "mixed": mgx.MODE_MIXED, "learned": mgx.MODE_LEARNED}
     results = {}
     phis = {}
-    for name in args.modes.split(","):
+    net = None
+    mode_names = args.modes.split(",")
+    if "learned" in mode_names:
+        if os.path.exists(args.weights):
+            meta_path = os.path.splitext(args.weights)[0] + ".json"
+            meta = json.load(open(meta_path)) if os.path.exists(meta_path) else {}
+            level = args.net_level
+            if level < 0:
+                m_train = meta.get("n", 128) >> meta.get("level", 2)
+                level = int(round(np.log2(n / m_train)))
+            params = np.load(args.weights)
+            net = mgx.UNet(ctx, base=meta.get("base", 16), levels=4)
+            net.set_params(params)
+            P.set_unet(net, level)
+            log(f"bench: learned predictor {args.weights} at level {level} ({n >> level}^3)")
+        else:
+            log(f"bench: no trained weights at {args.weights}; learned mode skipped")
+            mode_names.remove("learned")
+    for name in mode_names:
         mode = modes[name]
         log(f"bench: mode {name}: warmup {args.warmup}, steps {args.steps}")
         t, infos, ms, cnt = run_mode(ctx, P, bufs, dt, mode, args.steps, args.warmup, args.rtol)
@@ -140,6 +163,7 @@ def main():
             "projections_per_s": args.steps / t,
             "ms_per_projection": 1e3 * t / args.steps,
             "vcycles": infos[-1].iters,
+            "fallback_used": infos[-1].fallback_used,
             "rel_resid": infos[-1].rel_resid,
             "converged": all(i.converged for i in infos),
             "fine_smoother_avg_us": 1e3 * avg_ms,
@@ -165,13 +189,16 @@ def main():
         for name, phi in phis.items():
             r = orc.nodal_residual(g, sigma, b, phi)
             check[name] = {"oracle_rel_resid": float(np.max(np.abs(r)) / bn)}
-        if "fp64" in phis and "mixed" in phis:
-            check["mixed_vs_fp64_phi_rel"] = float(
-                np.max(np.abs(phis["mixed"] - phis["fp64"])) / np.max(np.abs(phis["fp64"])))
+        for other in ("mixed", "learned"):
+            if "fp64" in phis and other in phis:
+                check[f"{other}_vs_fp64_phi_rel"] = float(
+                    np.max(np.abs(phis[other] - phis["fp64"])) / np.max(np.abs(phis["fp64"])))
         log(f"bench: oracle check {check}")
 
     conv = {k: v for k, v in results.items() if v["converged"]}
-    head = max(conv, key=lambda k: conv[k]["projections_per_s"]) if conv else list(results)[0]
+    # the headline is the learned + mixed-precision form; without trained
+    # weights the mixed form stands in (named in config.mode)
+    head = next((k for k in ("learned", "mixed", "fp64") if k in conv), list(results)[0])
     hr = results[head]
     word = 8 if head == "fp64" else 4
     roofline = None
@@ -212,6 +239,8 @@ def main():
         "cpu_baseline": cpu,
     }
     P.close()
+    if net is not None:
+        net.close()
     ctx.close()
     print(json.dumps(out), flush=True)
 

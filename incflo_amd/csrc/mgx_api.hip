@@ -1,5 +1,7 @@
 // mgx_api.hip -- the C ABI (include/mgx.h): context, device-memory plumbing,
 // and the nodal projection entry points.
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 
@@ -17,6 +19,13 @@ __global__ void fill_sigma_const_kernel(KGeom g, double v, double* sig) {
 }
 
 }  // namespace
+
+static __global__ void __launch_bounds__(256) stream_copy_kernel(float4* __restrict__ dst,
+                                                                 const float4* __restrict__ src,
+                                                                 long n) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        dst[i] = src[i];
+}
 
 struct mgx_nodal_s {
     mgx_ctx_s* ctx = nullptr;
@@ -168,6 +177,18 @@ MGX_API int mgx_memcpy_d2h(mgx_ctx ctx, void* dst, const void* src, unsigned lon
 MGX_API int mgx_memcpy_d2d(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes) {
     return guarded(ctx, [&] {
         MGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+MGX_API int mgx_stream_copy(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes) {
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(dst && src && bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 &&
+                        ((uintptr_t)src % 16) == 0,
+                    MGX_ERR_ARG, "stream copy: 16-byte aligned pointers and sizes required");
+        const long n = (long)(bytes / 16);
+        if (n == 0) return;
+        const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
+        stream_copy_kernel<<<blocks, 256, 0, ctx->stream>>>((float4*)dst, (const float4*)src, n);
     });
 }
 

@@ -70,6 +70,7 @@ def lib():
             "mgx_memcpy_d2h": [vp, vp, vp, C.c_ulonglong],
             "mgx_memset_d": [vp, vp, C.c_int, C.c_ulonglong],
             "mgx_memcpy_d2d": [vp, vp, vp, C.c_ulonglong],
+            "mgx_stream_copy": [vp, vp, vp, C.c_ulonglong],
             "mgx_nodal_set_timing": [vp, C.c_int],
             "mgx_nodal_get_timing": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
             "mgx_ctx_record": [vp, C.c_int],

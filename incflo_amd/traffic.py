@@ -1,0 +1,123 @@
+"""Workload for hardware byte-counter runs of the finest-level smoother.
+
+    rocprofv3 --pmc FETCH_SIZE -d <dir> -o run -- python3 -m incflo_amd.traffic
+    rocprofv3 --pmc WRITE_SIZE -d <dir> -o run -- python3 -m incflo_amd.traffic
+    python3 -m incflo_amd.traffic --summarize <dir-of-FETCH> <dir-of-WRITE>
+
+The program first runs `--copies` launches of the plain streaming copy kernel
+(mgx_stream_copy) over a buffer of known size, then one warm-up and
+`--projections` nodal projections of the 256^3 variable-density case in the
+given mode.  The copy launches calibrate the counters: the factor
+(known bytes) / (counter reading) converts a counter reading of any kernel to
+bytes.  --summarize reads rocprofv3's counter_collection CSVs and prints, per
+kernel family, the average counter value per dispatch and the calibrated
+bytes, in the text form kept under profiles/.
+"""
+import argparse
+import csv
+import glob
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+
+def run(args):
+    from incflo_amd import boxes as B
+    from incflo_amd import mgx, workloads
+    ctx = mgx.Context(0)
+    nbytes = args.copy_mib << 20
+    a = ctx.array(nbytes // 8)
+    b = ctx.array(nbytes // 8)
+    for _ in range(args.copies):
+        ctx.check(mgx.lib().mgx_stream_copy(ctx.h, b.ptr, a.ptr, nbytes))
+    ctx.sync()
+    a.free()
+    b.free()
+    n = args.n
+    rho, ustar, dt = workloads.variable_density_case(n, args.seed)
+    bxs = B.chop_domain((n, n, n), 32)
+    P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))
+    vel0 = ctx.array(B.to_boxed(ustar, bxs, 1))
+    vel = ctx.array(vel0.n)
+    rhod = ctx.array(B.to_boxed(rho, bxs, 0))
+    gp = ctx.array(B.field_size(bxs, 0, 3))
+    p = ctx.array(B.field_size(bxs, 0, 1, True))
+    mode = {"fp64": mgx.MODE_FP64, "mixed": mgx.MODE_MIXED}[args.mode]
+    for q in range(1 + args.projections):
+        vel.copy_from(vel0)
+        ctx.check(mgx.lib().mgx_memset_d(ctx.h, gp.ptr, 0, gp.nbytes))
+        info = P.apply_nodal_projection(vel, 1, gp, 0, p, 0, dt, density=rhod, rtol=1e-10,
+                                        atol=0.0, mode=mode)
+        print(f"projection {q}: {info.iters} V-cycles, rel resid {info.rel_resid:.2e}",
+              flush=True)
+    P.close()
+    ctx.close()
+    print(f"copies: {args.copies} x {nbytes} bytes read + written", flush=True)
+
+
+FAMILIES = [
+    ("stream_copy", "stream_copy_kernel"),
+    ("fine Jacobi FP32", "nodal_stencil_kernel<float, float, 0"),
+    ("fine Jacobi FP64", "nodal_stencil_kernel<double, double, 0"),
+]
+
+
+def _read(d):
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def summarize(dirs, copy_bytes):
+    out = []
+    for d in dirs:
+        rows = _read(d)
+        if not rows:
+            out.append(f"{d}: no counter_collection.csv")
+            continue
+        counter = rows[0]["Counter_Name"]
+        per = {}
+        for r in rows:
+            name = r["Kernel_Name"]
+            for fam, key in FAMILIES:
+                if key in name and ("FINE" not in key):
+                    # the finest level is the largest grid: keep per-grid-size buckets
+                    gsz = r.get("Grid_Size", "")
+                    per.setdefault((fam, gsz), []).append(float(r["Counter_Value"]))
+        copy = [v for (f, _), vals in per.items() if f == "stream_copy" for v in vals]
+        cal = copy_bytes / (np.mean(copy) * 1024.0) if copy else float("nan")
+        out.append(f"counter {counter}: copy kernel reads {np.mean(copy) if copy else float('nan'):.1f} KiB"
+                   f" per dispatch for {copy_bytes} known bytes -> scale {cal:.4f}")
+        for (fam, gsz), vals in sorted(per.items(), key=lambda kv: -max(kv[1])):
+            if fam == "stream_copy":
+                continue
+            m = float(np.mean(vals))
+            out.append(f"  {fam:18s} grid {gsz:>10s}  dispatches {len(vals):4d}  "
+                       f"{counter} {m:12.1f} KiB  calibrated {m * 1024.0 * cal / 1e6:10.2f} MB")
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--mode", default="mixed")
+    ap.add_argument("--projections", type=int, default=1)
+    ap.add_argument("--copies", type=int, default=4)
+    ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--summarize", nargs="+")
+    args = ap.parse_args()
+    if args.summarize:
+        print(summarize(args.summarize, args.copy_mib << 20))
+    else:
+        run(args)
+
+
+if __name__ == "__main__":
+    main()

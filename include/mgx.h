@@ -116,6 +116,10 @@ MGX_API int mgx_memcpy_d2h(mgx_ctx ctx, void* dst, const void* src, unsigned lon
 MGX_API int mgx_memset_d(mgx_ctx ctx, void* dst, int value, unsigned long long bytes);
 /* stream-ordered device-to-device copy (returns without synchronising) */
 MGX_API int mgx_memcpy_d2d(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes);
+/* Instrumentation (no reference counterpart): a plain streaming copy kernel
+ * (16-byte loads and stores; bytes a multiple of 16, 16-byte aligned
+ * pointers) whose known traffic calibrates hardware byte counters. */
+MGX_API int mgx_stream_copy(mgx_ctx ctx, void* dst, const void* src, unsigned long long bytes);
 /* Stream-ordered timing: record event `slot` (0..15) on the context's
  * stream; elapsed milliseconds between two recorded slots. */
 MGX_API int mgx_ctx_record(mgx_ctx ctx, int slot);
