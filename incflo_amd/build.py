@@ -18,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmgx.so")
+FACADE = os.path.join(OBJ, "facade_check")
 ARCH = os.environ.get("MGX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -57,6 +58,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ facade driver (host code over the C ABI): tests/cpp/facade_check.cpp
+    src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+    if os.path.exists(src):
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+               src, "-o", FACADE, "-L", HERE, "-lmgx", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"facade driver build failed:\n{' '.join(cmd)}\n{r.stderr}")
     return LIB
 
 

@@ -1,0 +1,379 @@
+// mgx_amrex.H -- header-only C++ facade over the C ABI (mgx.h) with the call
+// surface incflo uses on AMReX-Hydro's projectors and AMReX's MLMG, so the
+// GPU solver drops in at the same call sites:
+//   NodalProjector  src/projection/incflo_apply_nodal_projection.cpp:181-219
+//   MacProjector    src/convection/incflo_compute_MAC_projected_velocities.cpp:96-300,
+//                   src/incflo.cpp:286-291,
+//                   src/projection/incflo_apply_cc_projection.cpp:321,387-389
+//   MLMG            src/diffusion/DiffusionTensorOp.cpp:188-215
+// Field containers are views (device pointer + layout) of caller-owned
+// buffers in the boxed layout: boxes back to back, per box component-major,
+// x fastest, ngrow ghost layers (AMReX Array4 order).  Errors end the process
+// with the library's message, as amrex::Abort does at the reference's call
+// sites (src/projection/incflo_projection_bc.cpp:36,74).
+#pragma once
+
+#include <array>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "mgx.h"
+
+namespace mgx {
+
+[[noreturn]] inline void Abort(const std::string& msg) {
+    std::fprintf(stderr, "mgx::Abort: %s\n", msg.c_str());
+    std::fflush(stderr);
+    std::abort();
+}
+
+// amrex::LinOpBCType values used by incflo's BC maps
+enum class LinOpBCType : int {
+    Periodic = MGX_BC_PERIODIC,
+    Dirichlet = MGX_BC_DIRICHLET,
+    Neumann = MGX_BC_NEUMANN,
+    inflow = MGX_BC_INFLOW,
+    Robin = MGX_BC_ROBIN,
+};
+
+struct Box {
+    std::array<int, 3> lo, hi;  // cell indices, inclusive
+};
+
+struct BoxArray {
+    std::vector<Box> boxes;
+    std::vector<int> flat() const {
+        std::vector<int> f;
+        for (const Box& b : boxes)
+            f.insert(f.end(), {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]});
+        return f;
+    }
+};
+
+struct Geometry {
+    std::array<int, 3> n_cell;
+    std::array<double, 3> prob_lo{0.0, 0.0, 0.0}, prob_hi{1.0, 1.0, 1.0};
+    std::array<int, 3> is_periodic{1, 1, 1};
+};
+
+// A MultiFab view: device buffer of one level in the boxed layout.
+struct MultiFab {
+    double* p = nullptr;
+    const BoxArray* ba = nullptr;
+    int ncomp = 1;
+    int ngrow = 0;
+};
+
+struct LPInfo {
+    int max_coarsening_level = 100;
+    LPInfo& setMaxCoarseningLevel(int n) {
+        max_coarsening_level = n;
+        return *this;
+    }
+};
+
+// Owns an mgx context (device + stream).  hipStream_t is passed as void*.
+class Context {
+   public:
+    explicit Context(int device = 0, void* stream = nullptr) {
+        if (mgx_ctx_create(device, stream, &h_) != MGX_OK)
+            Abort("mgx_ctx_create failed for device " + std::to_string(device));
+    }
+    ~Context() { mgx_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    mgx_ctx get() const { return h_; }
+    void check(int st, const char* what) const {
+        if (st != MGX_OK) Abort(std::string(what) + ": " + mgx_ctx_last_error(h_));
+    }
+    void synchronize() const { check(mgx_ctx_synchronize(h_), "synchronize"); }
+
+    // The context used by constructors called without one (incflo's call
+    // shapes): the one made current with make_default(), else device 0 on its
+    // null stream, created on first use.
+    static Context& current() {
+        if (!slot()) {
+            static Context dev0(0);
+            slot() = &dev0;
+        }
+        return *slot();
+    }
+    static void make_default(Context& c) { slot() = &c; }
+
+   private:
+    static Context*& slot() {
+        static Context* c = nullptr;
+        return c;
+    }
+    mgx_ctx h_ = nullptr;
+};
+
+namespace detail {
+inline mgx_level_desc level_desc(const Geometry& g, const std::vector<int>& flat) {
+    mgx_level_desc d{};
+    for (int k = 0; k < 3; ++k) {
+        d.n_cell[k] = g.n_cell[k];
+        d.prob_lo[k] = g.prob_lo[k];
+        d.prob_hi[k] = g.prob_hi[k];
+        d.is_periodic[k] = g.is_periodic[k];
+    }
+    d.nboxes = (int)(flat.size() / 6);
+    d.boxes = flat.data();
+    return d;
+}
+inline void one_level(size_t n, const char* who) {
+    if (n != 1) Abort(std::string(who) + ": only single-level hierarchies are supported");
+}
+// device buffer owned by a projector (getPhi / getGradPhi results)
+struct DevField {
+    mgx_ctx ctx = nullptr;
+    void* p = nullptr;
+    MultiFab view;
+    void alloc(mgx_ctx c, const BoxArray* ba, int ncomp, bool nodal) {
+        if (p) return;
+        ctx = c;
+        long long n = 0;
+        for (const Box& b : ba->boxes) {
+            long long v = 1;
+            for (int k = 0; k < 3; ++k) v *= (b.hi[k] - b.lo[k] + 1) + (nodal ? 1 : 0);
+            n += v * ncomp;
+        }
+        if (mgx_dev_malloc(c, (unsigned long long)n * sizeof(double), &p) != MGX_OK)
+            Abort("device allocation failed");
+        view.p = (double*)p;
+        view.ba = ba;
+        view.ncomp = ncomp;
+        view.ngrow = 0;
+    }
+    ~DevField() {
+        if (p) mgx_dev_free(ctx, p);
+    }
+};
+}  // namespace detail
+
+// Hydro::NodalProjector (single level).  project() overwrites vel in place.
+class NodalProjector {
+   public:
+    NodalProjector(Context& ctx, std::vector<MultiFab*> vel, std::vector<MultiFab const*> sigma,
+                   std::vector<Geometry> geom, LPInfo info = LPInfo())
+        : ctx_(ctx), vel_(vel), geom_(geom) {
+        init(info);
+        detail::one_level(sigma.size(), "NodalProjector");
+        ctx_.check(mgx_nodal_set_sigma(h_, sigma[0]->p, sigma[0]->ngrow), "set sigma");
+    }
+    NodalProjector(Context& ctx, std::vector<MultiFab*> vel, double const_sigma,
+                   std::vector<Geometry> geom, LPInfo info = LPInfo())
+        : ctx_(ctx), vel_(vel), geom_(geom) {
+        init(info);
+        ctx_.check(mgx_nodal_set_const_sigma(h_, const_sigma), "set sigma");
+    }
+    NodalProjector(std::vector<MultiFab*> vel, std::vector<MultiFab const*> sigma,
+                   std::vector<Geometry> geom, LPInfo info = LPInfo())
+        : NodalProjector(Context::current(), vel, sigma, geom, info) {}
+    NodalProjector(std::vector<MultiFab*> vel, double const_sigma, std::vector<Geometry> geom,
+                   LPInfo info = LPInfo())
+        : NodalProjector(Context::current(), vel, const_sigma, geom, info) {}
+    ~NodalProjector() { mgx_nodal_destroy(h_); }
+    NodalProjector(const NodalProjector&) = delete;
+    NodalProjector& operator=(const NodalProjector&) = delete;
+
+    void setDomainBC(std::array<LinOpBCType, 3> lo, std::array<LinOpBCType, 3> hi) {
+        int l[3], r[3];
+        for (int k = 0; k < 3; ++k) {
+            l[k] = (int)lo[k];
+            r[k] = (int)hi[k];
+        }
+        ctx_.check(mgx_nodal_set_domain_bc(h_, l, r), "setDomainBC");
+    }
+    // solve mode: MGX_MODE_FP64 (default), MGX_MODE_MIXED, MGX_MODE_LEARNED
+    void setSolveMode(int mode) { mode_ = mode; }
+    void setVerbose(int v) {
+        opts_.verbose = v;
+        ctx_.check(mgx_nodal_set_options(h_, &opts_), "set options");
+    }
+    void project(double rtol, double atol) {
+        ctx_.check(mgx_nodal_project(h_, vel_[0]->p, vel_[0]->ngrow, rtol, atol, mode_, &info_),
+                   "NodalProjector::project");
+    }
+    std::vector<MultiFab*> getPhi() {
+        phi_.alloc(ctx_.get(), vel_[0]->ba, 1, true);
+        ctx_.check(mgx_nodal_get_phi(h_, phi_.view.p, 0), "getPhi");
+        return {&phi_.view};
+    }
+    std::vector<MultiFab*> getGradPhi() {
+        gphi_.alloc(ctx_.get(), vel_[0]->ba, 3, false);
+        ctx_.check(mgx_nodal_get_gradphi(h_, gphi_.view.p, 0), "getGradPhi");
+        return {&gphi_.view};
+    }
+    const mgx_solve_info& info() const { return info_; }
+    mgx_nodal handle() const { return h_; }
+    Context& context() const { return ctx_; }
+
+   private:
+    void init(const LPInfo& info) {
+        detail::one_level(vel_.size(), "NodalProjector");
+        detail::one_level(geom_.size(), "NodalProjector");
+        flat_ = vel_[0]->ba->flat();
+        mgx_level_desc d = detail::level_desc(geom_[0], flat_);
+        ctx_.check(mgx_nodal_create(ctx_.get(), &d, &h_), "NodalProjector");
+        mgx_default_mlmg_opts(&opts_);
+        opts_.max_coarsening_level = info.max_coarsening_level;
+        ctx_.check(mgx_nodal_set_options(h_, &opts_), "set options");
+    }
+    Context& ctx_;
+    std::vector<MultiFab*> vel_;
+    std::vector<Geometry> geom_;
+    std::vector<int> flat_;
+    mgx_nodal h_ = nullptr;
+    mgx_mlmg_opts opts_{};
+    mgx_solve_info info_{};
+    int mode_ = MGX_MODE_FP64;
+    detail::DevField phi_, gphi_;
+};
+
+// Hydro::MacProjector (single level, no EB).
+class MacProjector {
+   public:
+    MacProjector(Context& ctx, std::vector<Geometry> geom) : ctx_(ctx), geom_(geom) {
+        detail::one_level(geom_.size(), "MacProjector");
+    }
+    explicit MacProjector(std::vector<Geometry> geom) : MacProjector(Context::current(), geom) {}
+    ~MacProjector() {
+        if (h_) mgx_mac_destroy(h_);
+    }
+    MacProjector(const MacProjector&) = delete;
+    MacProjector& operator=(const MacProjector&) = delete;
+
+    bool needInitialization() const { return h_ == nullptr; }
+    // initProjector(lp_info, beta) with face coefficients
+    void initProjector(const BoxArray& ba, LPInfo info,
+                       std::vector<std::array<MultiFab const*, 3>> beta) {
+        create(ba, info);
+        updateCoeffs(beta);
+    }
+    // initProjector(ba, dm, lp_info, beta) with a constant coefficient
+    void initProjector(const BoxArray& ba, LPInfo info, double beta) {
+        create(ba, info);
+        updateBeta(beta);
+    }
+    void setDomainBC(std::array<LinOpBCType, 3> lo, std::array<LinOpBCType, 3> hi) {
+        int l[3], r[3];
+        for (int k = 0; k < 3; ++k) {
+            l[k] = (int)lo[k];
+            r[k] = (int)hi[k];
+        }
+        ctx_.check(mgx_mac_set_domain_bc(need(), l, r), "setDomainBC");
+    }
+    void updateBeta(double beta) { ctx_.check(mgx_mac_set_const_beta(need(), beta), "updateBeta"); }
+    void updateCoeffs(std::vector<std::array<MultiFab const*, 3>> beta) {
+        detail::one_level(beta.size(), "MacProjector");
+        ctx_.check(mgx_mac_set_beta(need(), beta[0][0]->p, beta[0][1]->p, beta[0][2]->p,
+                                    beta[0][0]->ngrow),
+                   "updateCoeffs");
+    }
+    void setUMAC(std::vector<std::array<MultiFab*, 3>> umac) {
+        detail::one_level(umac.size(), "MacProjector");
+        umac_ = umac[0];
+    }
+    void setSolveMode(int mode) { mode_ = mode; }
+    void project(double rtol, double atol) { run(nullptr, 0, rtol, atol); }
+    void project(std::vector<MultiFab*> phi, double rtol, double atol) {
+        detail::one_level(phi.size(), "MacProjector");
+        run(phi[0]->p, phi[0]->ngrow, rtol, atol);
+    }
+    // fluxes -beta grad(phi) of the last solve (MLMG::Location::FaceCenter)
+    void getFluxes(std::vector<std::array<MultiFab*, 3>> flux) {
+        detail::one_level(flux.size(), "MacProjector");
+        ctx_.check(mgx_mac_get_fluxes(need(), flux[0][0]->p, flux[0][1]->p, flux[0][2]->p,
+                                      flux[0][0]->ngrow),
+                   "getFluxes");
+    }
+    const mgx_solve_info& info() const { return info_; }
+    mgx_mac handle() const { return h_; }
+    Context& context() const { return ctx_; }
+
+   private:
+    void create(const BoxArray& ba, const LPInfo& info) {
+        if (h_) mgx_mac_destroy(h_);
+        flat_ = ba.flat();
+        mgx_level_desc d = detail::level_desc(geom_[0], flat_);
+        ctx_.check(mgx_mac_create(ctx_.get(), &d, &h_), "MacProjector");
+        mgx_mlmg_opts o;
+        mgx_default_mlmg_opts(&o);
+        o.maxiter = 200;
+        o.bottom_maxiter = 200;
+        o.max_coarsening_level = info.max_coarsening_level;
+        ctx_.check(mgx_mac_set_options(h_, &o), "set options");
+    }
+    mgx_mac need() const {
+        if (!h_) Abort("MacProjector: initProjector has not been called");
+        return h_;
+    }
+    void run(double* phi, int phi_ng, double rtol, double atol) {
+        if (!umac_[0]) Abort("MacProjector::project: setUMAC has not been called");
+        ctx_.check(mgx_mac_project(need(), umac_[0]->p, umac_[1]->p, umac_[2]->p, umac_[0]->ngrow,
+                                   phi, phi_ng, rtol, atol, mode_, &info_),
+                   "MacProjector::project");
+    }
+    Context& ctx_;
+    std::vector<Geometry> geom_;
+    std::vector<int> flat_;
+    mgx_mac h_ = nullptr;
+    std::array<MultiFab*, 3> umac_{nullptr, nullptr, nullptr};
+    mgx_solve_info info_{};
+    int mode_ = MGX_MODE_FP64;
+};
+
+// amrex::MLMG driving one of the two operators directly (setMaxIter, ...,
+// solve(sol, rhs, rtol, atol)); the operator's coefficients and BCs come from
+// the projector it wraps.
+class MLMG {
+   public:
+    explicit MLMG(NodalProjector& op) : ctx_(op.context()), nodal_(op.handle()) { defaults(); }
+    explicit MLMG(MacProjector& op) : ctx_(op.context()), mac_(op.handle()) {
+        defaults();
+        o_.maxiter = 200;
+        o_.bottom_maxiter = 200;
+    }
+    void setMaxIter(int n) { o_.maxiter = n; }
+    void setBottomMaxIter(int n) { o_.bottom_maxiter = n; }
+    void setVerbose(int v) { o_.verbose = v; }
+    void setBottomVerbose(int v) { o_.bottom_verbose = v; }
+    void setBottomTolerance(double t) { o_.bottom_rtol = t; }
+    void setBottomToleranceAbs(double t) { o_.bottom_atol = t; }
+    void setSolveMode(int mode) { mode_ = mode; }
+    // returns the final residual norm, as amrex::MLMG::solve does
+    double solve(std::vector<MultiFab*> sol, std::vector<MultiFab const*> rhs, double rtol,
+                 double atol) {
+        detail::one_level(sol.size(), "MLMG::solve");
+        detail::one_level(rhs.size(), "MLMG::solve");
+        if (nodal_) {
+            ctx_.check(mgx_nodal_set_options(nodal_, &o_), "MLMG options");
+            ctx_.check(mgx_nodal_solve(nodal_, sol[0]->p, sol[0]->ngrow, rhs[0]->p, rhs[0]->ngrow,
+                                       rtol, atol, mode_, &info_),
+                       "MLMG::solve");
+        } else {
+            if (!mac_) Abort("MLMG: the operator has not been initialised");
+            ctx_.check(mgx_mac_set_options(mac_, &o_), "MLMG options");
+            ctx_.check(mgx_mac_solve(mac_, sol[0]->p, sol[0]->ngrow, rhs[0]->p, rhs[0]->ngrow, rtol,
+                                     atol, mode_, &info_),
+                       "MLMG::solve");
+        }
+        return info_.resnorm;
+    }
+    int getNumIters() const { return info_.iters; }
+    const mgx_solve_info& info() const { return info_; }
+
+   private:
+    void defaults() { mgx_default_mlmg_opts(&o_); }
+    Context& ctx_;
+    mgx_nodal nodal_ = nullptr;
+    mgx_mac mac_ = nullptr;
+    mgx_mlmg_opts o_{};
+    mgx_solve_info info_{};
+    int mode_ = MGX_MODE_FP64;
+};
+
+}  // namespace mgx

@@ -76,3 +76,22 @@ def test_boxed_layout_roundtrip():
 def test_oracle_library_loads(orc):
     assert orc.nodal_nlevels(orc.Geom.make((256, 256, 256))) == 7
     assert orc.nodal_nlevels(orc.Geom.make((256, 64, 64), periodic=(0, 0, 0))) == 5
+
+
+def test_cpp_facade_compiles_against_the_abi(tmp_path):
+    """include/mgx_amrex.H (the C++ call surface) compiles on its own and the
+    facade driver links against libmgx.so (no GPU call)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "mgx_amrex.H"\n'
+                   "int main() { mgx::LPInfo i; i.setMaxCoarseningLevel(3);\n"
+                   "  return i.max_coarsening_level == 3 ? 0 : 1; }\n")
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(os.path.join(ROOT, "incflo_amd", "_build", "facade_check"))
