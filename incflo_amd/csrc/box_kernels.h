@@ -5,6 +5,7 @@
 // of uneven sizes are handled by a single launch.
 #pragma once
 
+#include "mgx.h"
 #include <hip/hip_runtime.h>
 
 #include "nodal_kernels.h"

@@ -114,6 +114,12 @@ def lib():
                                    C.c_int, C.POINTER(C.c_int), C.c_int],
             "mgx_conv3d_backward": [vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.POINTER(C.c_int)],
+            "mgx_adv_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
+            "mgx_adv_destroy": [vp],
+            "mgx_adv_extrap_vel_to_faces": [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp, vp, vp,
+                                            C.c_int],
+            "mgx_adv_convective_term": [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int,
+                                        C.c_double, vp, C.c_int, vp, vp, vp, vp, vp, vp],
             "mgx_nodal_set_unet": [vp, vp, C.c_int],
             "mgx_nodal_training_pair": [vp, C.c_int, vp, vp],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
@@ -423,3 +429,31 @@ def conv3d_backward(ctx, x_dev, w_dev, dout_dev, din_dev, dw_dev, db_dev, shape,
     ctx.check(lib().mgx_conv3d_backward(ctx.h, x_dev.ptr, w_dev.ptr, dout_dev.ptr, opt(din_dev),
                                         opt(dw_dev), opt(db_dev), nx, ny, nz, b, cin, cout,
                                         _per(periodic)))
+
+
+class Advection:
+    """Godunov PPM advection pieces of the predictor (periodic, single level)."""
+
+    def __init__(self, ctx, level):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_adv_create(ctx.h, C.byref(level), C.byref(self.h)))
+
+    def extrap_vel_to_faces(self, vel, vel_ng, dt, umac, vmac, wmac, umac_ng=0, force=None,
+                            f_ng=0):
+        self.ctx.check(lib().mgx_adv_extrap_vel_to_faces(
+            self.h, vel.ptr, vel_ng, force.ptr if force is not None else None, f_ng, dt,
+            umac.ptr, vmac.ptr, wmac.ptr, umac_ng))
+
+    def convective_term(self, vel, vel_ng, umac, vmac, wmac, umac_ng, dt, conv, conv_ng=0,
+                        force=None, f_ng=0, faces=(None, None, None), fluxes=(None, None, None)):
+        p = lambda a: a.ptr if a is not None else None
+        self.ctx.check(lib().mgx_adv_convective_term(
+            self.h, vel.ptr, vel_ng, p(force), f_ng, umac.ptr, vmac.ptr, wmac.ptr, umac_ng, dt,
+            conv.ptr, conv_ng, p(faces[0]), p(faces[1]), p(faces[2]), p(fluxes[0]),
+            p(fluxes[1]), p(fluxes[2])))
+
+    def close(self):
+        if self.h:
+            lib().mgx_adv_destroy(self.h)
+            self.h = C.c_void_p()

@@ -215,6 +215,34 @@ MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, in
 MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
                           double rtol, double atol, int mode, mgx_solve_info* info);
 
+/* ---- Godunov advection on the predictor path (periodic, single level, no EB) --
+ * The reference calls AMReX-Hydro's Godunov here; the method (PPM with the
+ * Colella-Woodward limiter, corner-transport-upwind transverse terms,
+ * use_forces_in_trans = false, src/incflo.H:516,525) is restated in
+ * incflo_amd/csrc/godunov_kernels.h.  Boxed layouts as above; faces shared by
+ * two boxes must agree. */
+typedef struct mgx_adv_s* mgx_adv;
+MGX_API int mgx_adv_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_adv* out);
+MGX_API int mgx_adv_destroy(mgx_adv h);
+/* HydroUtils::ExtrapVelToFaces (src/convection/incflo_compute_MAC_projected_velocities.cpp:168-178):
+ * cell velocity (3 comps, vel_ng ghosts) and forces (3 comps, f_ng ghosts,
+ * NULL = none) -> face-normal velocities at t^{n+1/2} (valid faces of the
+ * boxed face fields with umac_ng ghosts are written). */
+MGX_API int mgx_adv_extrap_vel_to_faces(mgx_adv h, const double* vel, int vel_ng,
+                                        const double* force, int f_ng, double dt, double* umac,
+                                        double* vmac, double* wmac, int umac_ng);
+/* ComputeFluxesOnBoxFromState + ComputeDivergence(mult = -1) + ComputeConvectiveTerm
+ * for the velocity in convective form (src/convection/incflo_compute_advection_term.cpp:361-428,
+ * 575-661): edge states of u, v, w by Godunov with the MAC velocities,
+ * fluxes u_mac * state (not area weighted), conv = -div(flux) + u div(u_mac)
+ * (3 comps, conv_ng ghosts).  face_* / flux_* (boxed faces, 3 comps, no
+ * ghosts) may be NULL. */
+MGX_API int mgx_adv_convective_term(mgx_adv h, const double* vel, int vel_ng, const double* force,
+                                    int f_ng, const double* umac, const double* vmac,
+                                    const double* wmac, int umac_ng, double dt, double* conv,
+                                    int conv_ng, double* face_x, double* face_y, double* face_z,
+                                    double* flux_x, double* flux_y, double* flux_z);
+
 /* ---- learned coarse-level predictor --------------------------------------
  * No counterpart in the reference (incflo has no learned component); these
  * entry points serve MGX_MODE_LEARNED of the nodal solve.  A 3-D conv U-net

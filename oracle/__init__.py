@@ -360,3 +360,49 @@ def unet_grad(params, x, dout, base=16, levels=4, periodic=(1, 1, 1), bf16=False
                         _p(np.ascontiguousarray(dout, dtype=np.float64)), _p(g), nx, ny, nz,
                         batch, _per(periodic), int(bf16))
     return g
+
+
+# ---- Godunov PPM advection (periodic, canonical arrays) -----------------------
+# cells (nz, ny, nx); d-faces have n entries along d (face i = low face of cell i)
+def _opt(a):
+    return _p(np.ascontiguousarray(a, dtype=np.float64)) if a is not None else None
+
+
+def ppm_predict(g, s, d, ucell=None, uface=None, dt=0.0):
+    Im = np.zeros(g.cell_shape())
+    Ip = np.zeros(g.cell_shape())
+    lib().orc_ppm_predict(C.byref(g), _p(np.ascontiguousarray(s)), int(d), _opt(ucell),
+                          _opt(uface), C.c_double(dt), _p(Im), _p(Ip))
+    return Im, Ip
+
+
+def godunov_extrap_vel(g, vel, dt, force=None):
+    """vel: (3, nz, ny, nx) -> [umac, vmac, wmac] at t^{n+1/2}."""
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    f = [None] * 3 if force is None else [np.ascontiguousarray(force[n]) for n in range(3)]
+    out = [np.zeros(g.cell_shape()) for _ in range(3)]
+    lib().orc_godunov_extrap_vel(C.byref(g), _p(vel[0]), _p(vel[1]), _p(vel[2]), _opt(f[0]),
+                                 _opt(f[1]), _opt(f[2]), C.c_double(dt), _p(out[0]), _p(out[1]),
+                                 _p(out[2]))
+    return out
+
+
+def godunov_edge_states(g, vel, umac, dt, force=None):
+    """Edge states of the 3 components on all faces: shape (3 faces, 3 comps, nz, ny, nx)."""
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    f = [None] * 3 if force is None else [np.ascontiguousarray(force[n]) for n in range(3)]
+    st = np.zeros((3, 3) + g.cell_shape())
+    um = [np.ascontiguousarray(a, dtype=np.float64) for a in umac]
+    lib().orc_godunov_edge_states(C.byref(g), _p(vel[0]), _p(vel[1]), _p(vel[2]), _opt(f[0]),
+                                  _opt(f[1]), _opt(f[2]), _p(um[0]), _p(um[1]), _p(um[2]),
+                                  C.c_double(dt), _p(st))
+    return st
+
+
+def convective_term(g, vel, umac, states):
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    um = [np.ascontiguousarray(a, dtype=np.float64) for a in umac]
+    conv = np.zeros((3,) + g.cell_shape())
+    lib().orc_convective_term(C.byref(g), _p(vel[0]), _p(vel[1]), _p(vel[2]), _p(um[0]),
+                              _p(um[1]), _p(um[2]), _p(np.ascontiguousarray(states)), _p(conv))
+    return conv

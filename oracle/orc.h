@@ -136,6 +136,20 @@ void orc_unet_grad(const double* p, int base, int levels, int cin0, const double
                    const double* dout, double* grad, int nx, int ny, int nz, int batch,
                    const int per[3], int bf16);
 
+/* ---- Godunov PPM advection (orc_godunov.c), periodic canonical arrays --- */
+void orc_ppm_predict(const orc_geom* g, const double* s, int d, const double* ucell,
+                     const double* uface, double dt, double* Im, double* Ip);
+void orc_godunov_extrap_vel(const orc_geom* g, const double* u, const double* v, const double* w,
+                            const double* fu, const double* fv, const double* fw, double dt,
+                            double* umac, double* vmac, double* wmac);
+void orc_godunov_edge_states(const orc_geom* g, const double* u, const double* v, const double* w,
+                             const double* fu, const double* fv, const double* fw,
+                             const double* umac, const double* vmac, const double* wmac,
+                             double dt, double* states);
+void orc_convective_term(const orc_geom* g, const double* u, const double* v, const double* w,
+                         const double* umac, const double* vmac, const double* wmac,
+                         const double* states, double* conv);
+
 /* ---- helpers ---------------------------------------------------------- */
 double orc_norm_inf(const double* a, long n);
 double orc_dot(const double* a, const double* b, long n);
