@@ -59,6 +59,19 @@ def run_mode(ctx, P, bufs, dt, mode, steps, warmup, rtol):
     return (t1 - t0), infos, ms, cnt
 
 
+def measured_traffic(kernel_key, n):
+    """HBM bytes per launch of the finest-level smoother from the committed
+    counter-run summary (profiles/jacobi_256_traffic.txt, 256^3 only)."""
+    path = os.path.join(ROOT, "profiles", "jacobi_256_traffic.txt")
+    if n != 256 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes" and f[1] == kernel_key:
+            return float(f[2])
+    return None
+
+
 def cpu_baseline(n_full, seed, threads):
     """Oracle FP64 nodal projection on a bounded sample (n_full/2)^3, scaled by node count."""
     import oracle as orc
@@ -205,7 +218,9 @@ def main():
     if hr["fine_smoother_launches"]:
         achieved = hr["fine_smoother_GBps"]
         roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                    "frac": achieved / PEAK_HBM_GBS,
+                    "traffic": measured_traffic("fine_jacobi_fp64" if word == 8 else
+                                                "fine_jacobi_fp32", n),
                     "kernel": f"finest-level weighted-Jacobi sweep ({'FP64' if word == 8 else 'FP32'})",
                     "bytes_per_launch": 4 * word * nnodes}
     cpu = None

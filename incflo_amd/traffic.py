@@ -59,10 +59,14 @@ def run(args):
     print(f"copies: {args.copies} x {nbytes} bytes read + written", flush=True)
 
 
+# (label, substrings that must all occur in the demangled kernel name);
+# FINE = true marks the finest level's launches of the stencil kernel
 FAMILIES = [
-    ("stream_copy", "stream_copy_kernel"),
-    ("fine Jacobi FP32", "nodal_stencil_kernel<float, float, 0"),
-    ("fine Jacobi FP64", "nodal_stencil_kernel<double, double, 0"),
+    ("stream copy", ("stream_copy_kernel",)),
+    ("fine Jacobi FP32", ("nodal_stencil_kernel<float, float, 0, true, true",)),
+    ("fine Jacobi FP64", ("nodal_stencil_kernel<double, double, 0, true, true",)),
+    ("update+residual FP64/FP32", ("nodal_stencil_kernel<double, float, 2,",)),
+    ("update+residual FP64", ("nodal_stencil_kernel<double, double, 2,",)),
 ]
 
 
@@ -75,6 +79,8 @@ def _read(d):
 
 
 def summarize(dirs, copy_bytes):
+    """Per counter run: the calibration factor from the copy kernel (known
+    bytes / counter bytes) and the calibrated bytes per dispatch per family."""
     out = []
     for d in dirs:
         rows = _read(d)
@@ -84,22 +90,21 @@ def summarize(dirs, copy_bytes):
         counter = rows[0]["Counter_Name"]
         per = {}
         for r in rows:
-            name = r["Kernel_Name"]
-            for fam, key in FAMILIES:
-                if key in name and ("FINE" not in key):
-                    # the finest level is the largest grid: keep per-grid-size buckets
-                    gsz = r.get("Grid_Size", "")
-                    per.setdefault((fam, gsz), []).append(float(r["Counter_Value"]))
-        copy = [v for (f, _), vals in per.items() if f == "stream_copy" for v in vals]
-        cal = copy_bytes / (np.mean(copy) * 1024.0) if copy else float("nan")
-        out.append(f"counter {counter}: copy kernel reads {np.mean(copy) if copy else float('nan'):.1f} KiB"
-                   f" per dispatch for {copy_bytes} known bytes -> scale {cal:.4f}")
-        for (fam, gsz), vals in sorted(per.items(), key=lambda kv: -max(kv[1])):
-            if fam == "stream_copy":
-                continue
+            for fam, keys in FAMILIES:
+                if all(k in r["Kernel_Name"] for k in keys):
+                    per.setdefault(fam, []).append(float(r["Counter_Value"]))
+        copy = per.pop("stream copy", [])
+        if not copy:
+            out.append(f"{d}: no copy-kernel dispatches")
+            continue
+        kib = float(np.mean(copy))
+        cal = copy_bytes / (kib * 1024.0)
+        out.append(f"{d}: counter {counter}; copy kernel {kib:.1f} KiB per dispatch for "
+                   f"{copy_bytes} bytes -> scale {cal:.4f}")
+        for fam, vals in per.items():
             m = float(np.mean(vals))
-            out.append(f"  {fam:18s} grid {gsz:>10s}  dispatches {len(vals):4d}  "
-                       f"{counter} {m:12.1f} KiB  calibrated {m * 1024.0 * cal / 1e6:10.2f} MB")
+            out.append(f"  {fam:28s} dispatches {len(vals):4d}  {counter} {m:12.1f} KiB  "
+                       f"calibrated {m * 1024.0 * cal / 1e6:9.2f} MB per dispatch")
     return "\n".join(out)
 
 
