@@ -406,3 +406,67 @@ def convective_term(g, vel, umac, states):
     lib().orc_convective_term(C.byref(g), _p(vel[0]), _p(vel[1]), _p(vel[2]), _p(um[0]),
                               _p(um[1]), _p(um[2]), _p(np.ascontiguousarray(states)), _p(conv))
     return conv
+
+
+# ---- embedded boundary, nodal ------------------------------------------------
+_ip = C.POINTER(C.c_int)
+
+
+def eb_cylinder(g, cx, cy, r, prob_lo=(0.0, 0.0, 0.0)):
+    """-> (ctype (nz,ny,nx) int32: 0 covered / 1 regular / 2 cut, moments (nz,ny,nx,27))"""
+    ct = np.zeros(g.cell_shape(), dtype=np.int32)
+    mom = np.zeros(g.cell_shape() + (27,))
+    lib().orc_eb_cylinder(C.byref(g), (C.c_double * 3)(*prob_lo), C.c_double(cx),
+                          C.c_double(cy), C.c_double(r), ct.ctypes.data_as(_ip), _p(mom))
+    return ct, mom
+
+
+def eb_element(mom27, dx):
+    E = np.zeros(64)
+    lib().orc_eb_element(_p(np.ascontiguousarray(mom27, dtype=np.float64)),
+                         (C.c_double * 3)(*dx), _p(E))
+    return E.reshape(8, 8)
+
+
+def eb_node_mask(g, ctype):
+    m = np.zeros(g.node_shape(), dtype=np.int32)
+    lib().orc_eb_node_mask(C.byref(g), np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                           m.ctypes.data_as(_ip))
+    return m
+
+
+def eb_nodal_apply(g, ctype, mom, sigma, phi):
+    out = np.zeros(g.node_shape())
+    lib().orc_eb_nodal_apply(C.byref(g), np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                             _p(np.ascontiguousarray(mom)), _p(np.ascontiguousarray(sigma)),
+                             _p(np.ascontiguousarray(phi)), _p(out))
+    return out
+
+
+def eb_nodal_divergence(g, ctype, mom, ug):
+    out = np.zeros(g.node_shape())
+    lib().orc_eb_nodal_divergence(C.byref(g),
+                                  np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                                  _p(np.ascontiguousarray(mom)), _p(np.ascontiguousarray(ug)),
+                                  _p(out))
+    return out
+
+
+def eb_nodal_gradient(g, ctype, mom, phi):
+    out = np.zeros((3,) + g.cell_shape())
+    lib().orc_eb_nodal_gradient(C.byref(g),
+                                np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                                _p(np.ascontiguousarray(mom)), _p(np.ascontiguousarray(phi)),
+                                _p(out))
+    return out
+
+
+def eb_nodal_solve(g, ctype, mom, sigma, b, x0=None, rtol=1e-12, maxiter=20000):
+    x = np.zeros(g.node_shape()) if x0 is None else np.array(x0, dtype=np.float64)
+    its = C.c_int(0)
+    ret = lib().orc_eb_nodal_solve(C.byref(g),
+                                   np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                                   _p(np.ascontiguousarray(mom)), _p(np.ascontiguousarray(sigma)),
+                                   _p(np.ascontiguousarray(b)), _p(x), C.c_double(rtol),
+                                   C.c_int(maxiter), C.byref(its))
+    return x, its.value, ret

@@ -150,6 +150,22 @@ void orc_convective_term(const orc_geom* g, const double* u, const double* v, co
                          const double* umac, const double* vmac, const double* wmac,
                          const double* states, double* conv);
 
+/* ---- embedded boundary, nodal (orc_eb.c) ------------------------------- */
+/* ctype per cell: 0 covered, 1 regular, 2 cut; mom: 27 fluid moments per cell */
+void orc_eb_cylinder(const orc_geom* g, const double prob_lo[3], double cx, double cy, double r,
+                     int* ctype, double* mom);
+void orc_eb_element(const double* mom, const double dx[3], double E[64]);
+void orc_eb_node_mask(const orc_geom* g, const int* ctype, int* mask);
+void orc_eb_nodal_apply(const orc_geom* g, const int* ctype, const double* mom,
+                        const double* sigma, const double* phi, double* out);
+void orc_eb_nodal_divergence(const orc_geom* g, const int* ctype, const double* mom,
+                             const double* ug, double* rhs);
+void orc_eb_nodal_gradient(const orc_geom* g, const int* ctype, const double* mom,
+                           const double* phi, double* gphi);
+int  orc_eb_nodal_solve(const orc_geom* g, const int* ctype, const double* mom,
+                        const double* sigma, const double* b_in, double* x, double rtol,
+                        int maxiter, int* iters);
+
 /* ---- helpers ---------------------------------------------------------- */
 double orc_norm_inf(const double* a, long n);
 double orc_dot(const double* a, const double* b, long n);
