@@ -219,6 +219,11 @@ struct UpdateArgs {
     double* vel; int vel_ng;          // u_new written here (NULL: skip)
     const double* vel_old; int vold_ng;
     double* gp; int gp_ng; int gp_add;   // G(phi) written / added (NULL: skip)
+    // embedded boundary (NULL: none): per cell -1 regular / -2 covered / cut
+    // index; cut cells' gradient weights W[24] ([corner][d]) and volume fraction
+    const int* eb_cidx;
+    const double* eb_W;
+    const double* eb_V;
 };
 
 // u_new = u* - sigma G(phi) (+ u_old); gp = G(phi) or gp += G(phi)
@@ -228,7 +233,26 @@ static __global__ void __launch_bounds__(256) update_cells_kernel(UpdateArgs a) 
     const BoxInfo& b = a.boxes[bi];
     const KGeom& g = a.g;
     double gr[3];
-    cell_gradient(g, a.phi, i, j, k, a.f, gr);
+    int ci = -1;
+    if (a.eb_cidx) ci = a.eb_cidx[i + (long)g.n[0] * (j + (long)g.n[1] * k)];
+    if (ci >= 0) {
+        // cut cell: average gradient of the Q1 phi over the fluid part
+        gr[0] = gr[1] = gr[2] = 0.0;
+        for (int q = 0; q < 8; ++q) {
+            int ii = i + (q & 1), jj = j + ((q >> 1) & 1), kk = k + ((q >> 2) & 1);
+            if (ii >= g.nn[0]) ii -= g.nn[0];
+            if (jj >= g.nn[1]) jj -= g.nn[1];
+            if (kk >= g.nn[2]) kk -= g.nn[2];
+            const double v = a.phi[ii + (long)g.nn[0] * (jj + (long)g.nn[1] * kk)];
+            for (int d = 0; d < 3; ++d) gr[d] += a.eb_W[24L * ci + 3 * q + d] * v;
+        }
+        const double vf = a.eb_V[ci];
+        for (int d = 0; d < 3; ++d) gr[d] /= vf;
+    } else if (ci == -2) {
+        gr[0] = gr[1] = gr[2] = 0.0;
+    } else {
+        cell_gradient(g, a.phi, i, j, k, a.f, gr);
+    }
     if (a.vel) {
         const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
         const long gi = (i + 1) + (long)g.sg[0] * ((j + 1) + (long)g.sg[1] * (k + 1));
@@ -239,6 +263,7 @@ static __global__ void __launch_bounds__(256) update_cells_kernel(UpdateArgs a) 
             double u = (a.ug ? a.ug[c * ug3 + gi] : a.vel[vi]) - sigma * gr[c];
             if (a.vel_old) u += a.vel_old[field_offset(b, a.vold_ng, 3, false) +
                                           box_index(b, a.vold_ng, false, i, j, k, c)];
+            if (ci == -2) u = 0.0;  // covered cells carry no velocity
             a.vel[vi] = u;
         }
     }

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 
+#include "eb_nodal.h"
 #include "mgx_ctx.h"
 #include "nodal_mg.h"
 
@@ -42,6 +43,22 @@ struct mgx_nodal_s {
     BoxLayout layout;
     DevBuf ug, rhs;
     bool have_phi = false;
+    // embedded boundary request: 0 none, 1 z-cylinder (eb_cyl = cx, cy, r), 2 moments
+    int eb_kind = 0;
+    double eb_cyl[3] = {0, 0, 0};
+    std::vector<int> eb_type;
+    std::vector<double> eb_mom;
+    std::unique_ptr<EBNodal> eb;
+
+    void build_eb() {
+        solver.eb = nullptr;
+        eb.reset();
+        if (eb_kind == 0) return;
+        if (eb_kind == 1) eb_cylinder_moments(g, prob_lo, eb_cyl[0], eb_cyl[1], eb_cyl[2], eb_type, eb_mom);
+        eb = std::make_unique<EBNodal>();
+        eb->setup(g, solver.nlev, eb_type, eb_mom, stream());
+        solver.eb = eb.get();
+    }
 
     void ensure_setup() {
         if (!dirty) return;
@@ -58,10 +75,17 @@ struct mgx_nodal_s {
         rhs.alloc(g.nnode() * sizeof(double));
         have_phi = false;
         dirty = false;
+        build_eb();
     }
     hipStream_t stream() const { return ctx->stream; }
     void compute_rhs() {
         const KGeom kg = kgeom(g);
+        if (eb) {
+            eb_rhs_kernel<<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, stream()>>>(
+                kg, eb->cidx0(), eb->W0(), eb->mask0(), ug.as<double>(), rhs.as<double>(),
+                0.25 / g.dx[0], 0.25 / g.dx[1], 0.25 / g.dx[2]);
+            return;
+        }
         nodal_rhs_kernel<<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, stream()>>>(
             kg, ug.as<double>(), rhs.as<double>(), 0.25 / g.dx[0], 0.25 / g.dx[1], 0.25 / g.dx[2]);
     }
@@ -76,6 +100,11 @@ struct mgx_nodal_s {
         u.phi = solver.phi;
         u.sig = solver.sigma0();
         for (int d = 0; d < 3; ++d) u.f[d] = 0.25 / g.dx[d];
+        if (eb) {
+            u.eb_cidx = eb->cidx0();
+            u.eb_W = eb->W0();
+            u.eb_V = eb->V0();
+        }
         return u;
     }
 };
@@ -405,13 +434,25 @@ MGX_API int mgx_nodal_apply(mgx_nodal h, const double* phi, int phi_ng, double* 
         const KGeom kg = kgeom(h->g);
         gather_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
             kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), phi, phi_ng, h->rhs.as<double>());
-        SArgs<double, double> a{};
-        a.g = kg;
-        a.c = make_coef<double>(h->g.dx);
-        a.x = h->rhs.as<double>();
-        a.sig = h->solver.sigma0();
-        a.rout = h->solver.phi;
-        launch_stencil<double, double, KS_APPLY>(h->g, a, h->stream(), false);
+        if (h->eb) {
+            h->eb->set_sigma(h->solver.sigma0(), h->stream());
+            EBArgs e{};
+            e.L = h->eb->lev(0);
+            e.x = h->rhs.as<double>();
+            e.xout = h->solver.phi;
+            eb_mask_kernel<<<grid1d(h->g.nnode()), 256, 0, h->stream()>>>(
+                h->eb->mask0(), h->rhs.as<double>(), h->g.nnode());
+            eb_stencil_kernel<<<grid3(h->g.nn[0], h->g.nn[1], h->g.nn[2]), block3(), 0,
+                                h->stream()>>>(e, EB_APPLY);
+        } else {
+            SArgs<double, double> a{};
+            a.g = kg;
+            a.c = make_coef<double>(h->g.dx);
+            a.x = h->rhs.as<double>();
+            a.sig = h->solver.sigma0();
+            a.rout = h->solver.phi;
+            launch_stencil<double, double, KS_APPLY>(h->g, a, h->stream(), false);
+        }
         scatter_nodes_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
             kg, h->layout.dev_boxes(), h->layout.tiles(REG_NODE), h->solver.phi, out, out_ng, 0);
         h->have_phi = false;
@@ -527,4 +568,56 @@ MGX_API int mgx_nodal_training_pair(mgx_nodal h, int level, float* in, float* ta
     });
 }
 
+MGX_API int mgx_nodal_set_eb_cylinder(mgx_nodal h, double cx, double cy, double r) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(r > 0.0, MGX_ERR_ARG, "EB cylinder: radius must be positive");
+        h->eb_kind = 1;
+        h->eb_cyl[0] = cx;
+        h->eb_cyl[1] = cy;
+        h->eb_cyl[2] = r;
+        if (!h->dirty) h->build_eb();
+    });
+}
+
+MGX_API int mgx_nodal_set_eb_moments(mgx_nodal h, const int* type, const double* moments) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        const long nc = (long)h->n_cell[0] * h->n_cell[1] * h->n_cell[2];
+        if (!type) {
+            h->eb_kind = 0;
+        } else {
+            MGX_REQUIRE(moments != nullptr, MGX_ERR_ARG, "EB moments: null moments");
+            for (long c = 0; c < nc; ++c)
+                MGX_REQUIRE(type[c] >= 0 && type[c] <= 2, MGX_ERR_ARG, "EB moments: bad cell type");
+            h->eb_kind = 2;
+            h->eb_type.assign(type, type + nc);
+            h->eb_mom.assign(moments, moments + 27 * nc);
+        }
+        if (!h->dirty) h->build_eb();
+    });
+}
+
 }  // extern "C"
+
+extern "C" MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* lev, double cx, double cy,
+                                               double r, int* type, double* moments) {
+    if (!lev || !type || !moments || r <= 0.0) return MGX_ERR_ARG;
+    try {
+        int per[3], bc[3] = {0, 0, 0};
+        double dx[3];
+        for (int d = 0; d < 3; ++d) {
+            per[d] = lev->is_periodic[d];
+            dx[d] = (lev->prob_hi[d] - lev->prob_lo[d]) / lev->n_cell[d];
+        }
+        NodalGeom g = make_nodal_geom(lev->n_cell, per, bc, bc, dx);
+        std::vector<int> t;
+        std::vector<double> m;
+        eb_cylinder_moments(g, lev->prob_lo, cx, cy, r, t, m);
+        std::memcpy(type, t.data(), t.size() * sizeof(int));
+        std::memcpy(moments, m.data(), m.size() * sizeof(double));
+        return MGX_OK;
+    } catch (...) {
+        return MGX_ERR_ARG;
+    }
+}

@@ -81,6 +81,8 @@ class NodalHierarchy {
     bool built = false;
 };
 
+class EBNodal;
+
 // Outer MLMG iteration: FP64 residual, V-cycle correction (FP64 or FP32).
 class NodalSolver {
    public:
@@ -101,6 +103,8 @@ class NodalSolver {
     LaunchTimer fine_timer;  // finest-level smoother launches
     bool sigma_set = false;
     bool h32_sigma = false;
+    // embedded boundary (owned by the projector handle; nullptr: none)
+    EBNodal* eb = nullptr;
     // learned coarse-level predictor (MGX_MODE_LEARNED)
     UNet* net = nullptr;
     int net_level = 2;

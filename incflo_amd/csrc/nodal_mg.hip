@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "eb_nodal.h"
 #include "learned_kernels.h"
 #include "mgx_ctx.h"
 #include "nodal_mg.h"
@@ -432,6 +433,14 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "nodal solve: sigma has not been set");
     MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED || mode == MGX_MODE_LEARNED,
                 MGX_ERR_UNSUPPORTED, "nodal solve: unknown mode");
+    if (eb && eb->ready) {
+        MGX_REQUIRE(mode == MGX_MODE_FP64, MGX_ERR_UNSUPPORTED,
+                    "nodal solve: embedded boundaries run in FP64 mode only");
+        eb->set_sigma(sigma0(), s);
+        eb->solve(b, phi_a.as<double>(), phi_b.as<double>(), phi, rtol, atol, opts, scal, part,
+                  h_pinned, info, s);
+        return;
+    }
     const bool learned = (mode == MGX_MODE_LEARNED);
     if (learned) check_learned();
     const NodalGeom& g = g0;

@@ -120,6 +120,10 @@ def lib():
                                             C.c_int],
             "mgx_adv_convective_term": [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int,
                                         C.c_double, vp, C.c_int, vp, vp, vp, vp, vp, vp],
+            "mgx_nodal_set_eb_cylinder": [vp, C.c_double, C.c_double, C.c_double],
+            "mgx_nodal_set_eb_moments": [vp, vp, vp],
+            "mgx_eb_cylinder_geometry": [C.POINTER(LevelDesc), C.c_double, C.c_double, C.c_double,
+                                         vp, vp],
             "mgx_nodal_set_unet": [vp, vp, C.c_int],
             "mgx_nodal_training_pair": [vp, C.c_int, vp, vp],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
@@ -304,6 +308,20 @@ class NodalProjector:
         self.ctx.check(st)
         return info
 
+    def set_eb_cylinder(self, cx, cy, r):
+        """Embedded cylinder along z (fluid outside)."""
+        self.ctx.check(lib().mgx_nodal_set_eb_cylinder(self.h, cx, cy, r))
+
+    def set_eb_moments(self, ctype, moments):
+        """ctype: (nz, ny, nx) int 0/1/2; moments: (nz, ny, nx, 27); None removes the EB."""
+        if ctype is None:
+            self.ctx.check(lib().mgx_nodal_set_eb_moments(self.h, None, None))
+            return
+        self._eb_keep = (np.ascontiguousarray(ctype, dtype=np.int32),
+                         np.ascontiguousarray(moments, dtype=np.float64))
+        self.ctx.check(lib().mgx_nodal_set_eb_moments(self.h, self._eb_keep[0].ctypes.data,
+                                                      self._eb_keep[1].ctypes.data))
+
     def set_unet(self, net, level):
         """Attach a UNet for MODE_LEARNED (None detaches)."""
         self.ctx.check(lib().mgx_nodal_set_unet(self.h, net.h if net is not None else None, level))
@@ -457,3 +475,14 @@ class Advection:
         if self.h:
             lib().mgx_adv_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def eb_cylinder_geometry(level, cx, cy, r):
+    """Host-side cylinder geometry of a level: (ctype (nz,ny,nx), moments (nz,ny,nx,27))."""
+    n = [level.n_cell[d] for d in range(3)]
+    ct = np.zeros((n[2], n[1], n[0]), dtype=np.int32)
+    mom = np.zeros((n[2], n[1], n[0], 27))
+    st = lib().mgx_eb_cylinder_geometry(C.byref(level), cx, cy, r, ct.ctypes.data, mom.ctypes.data)
+    if st != MGX_OK:
+        raise MgxError(st, "mgx_eb_cylinder_geometry failed")
+    return ct, mom

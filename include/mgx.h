@@ -183,6 +183,22 @@ MGX_API int mgx_apply_nodal_projection(mgx_nodal h, const double* density, int r
                                        const double* inflow_vel, double rtol, double atol,
                                        int mode, mgx_solve_info* info);
 
+/* ---- embedded boundaries on the nodal projection ----------------------------
+ * The reference builds EB data with EB2::CylinderIF + EB2::Build
+ * (src/embedded_boundaries/eb_cylinder.cpp:53-63) and MLNodeLaplacian
+ * assembles its cut-cell operator; here a cut cell's Q1 element is assembled
+ * from the fluid moments of the cell, coarse levels use Galerkin cut elements.
+ * With EB the solve runs in FP64 (MGX_MODE_FP64).
+ * A cylinder along z with fluid outside (|x - (cx, cy)| >= r): */
+MGX_API int mgx_nodal_set_eb_cylinder(mgx_nodal h, double cx, double cy, double r);
+/* Any geometry: per canonical cell (x fastest, n0*n1*n2 cells) a type (0
+ * covered, 1 regular, 2 cut) and 27 fluid moments int xi^p eta^q zeta^r over
+ * the fluid part of the unit cell (index p + 3q + 9r); type NULL removes the EB. */
+MGX_API int mgx_nodal_set_eb_moments(mgx_nodal h, const int* type, const double* moments);
+/* Host-only: the cylinder's per-cell types and moments on a level (no GPU). */
+MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* level, double cx, double cy, double r,
+                                     int* type, double* moments);
+
 /* ---- MAC projection (Hydro::MacProjector + MLABecLaplacian/MLPoisson + MLMG) */
 typedef struct mgx_mac_s* mgx_mac;
 /* MacProjector(Vector<Geometry>) (src/incflo.cpp:291); the level's box list

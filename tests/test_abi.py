@@ -95,3 +95,19 @@ def test_cpp_facade_compiles_against_the_abi(tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert os.path.exists(os.path.join(ROOT, "incflo_amd", "_build", "facade_check"))
+
+
+def test_eb_geometry_matches_oracle(orc):
+    """The library's cylinder geometry (boundary integrals along the arcs) and
+    the oracle's (exact in y, adaptive quadrature in x) agree cell by cell."""
+    from incflo_amd import boxes as B
+    from incflo_amd import mgx
+    n = (64, 16, 2)
+    lev = mgx.make_level(n, B.chop_domain(n, 16), periodic=(0, 0, 1), prob_hi=(4.0, 1.0, 0.125))
+    ct, mom = mgx.eb_cylinder_geometry(lev, 0.5, 0.5, 0.125)
+    g = orc.Geom.make(n, periodic=(0, 0, 1), bclo=(2, 2, 0), bchi=(1, 2, 0),
+                      prob_hi=(4.0, 1.0, 0.125))
+    cto, momo = orc.eb_cylinder(g, 0.5, 0.5, 0.125)
+    assert np.array_equal(ct, cto)
+    assert (ct == 2).sum() > 0
+    assert np.max(np.abs(mom - momo)) < 1e-12
