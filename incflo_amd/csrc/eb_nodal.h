@@ -18,6 +18,12 @@ namespace mgx {
 void eb_cylinder_moments(const NodalGeom& g, const double prob_lo[3], double cx, double cy,
                          double r, std::vector<int>& type, std::vector<double>& mom);
 
+// Face apertures (fluid fraction of each face, canonical face arrays with
+// n_d + 1 faces along d) and covered-cell flags of the same cylinder.
+void eb_cylinder_apertures(const int n[3], const double prob_lo[3], const double dx[3], double cx,
+                           double cy, double r, std::vector<double> ap[3],
+                           std::vector<unsigned char>& covered);
+
 // FD-normalised element matrix of a cell from its fluid moments (sigma excluded)
 void eb_element_from_moments(const double* mom, const double dx[3], double E[64]);
 // regular element of spacing dx (same normalisation)

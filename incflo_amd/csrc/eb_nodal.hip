@@ -204,6 +204,43 @@ void eb_cylinder_moments(const NodalGeom& g, const double prob_lo[3], double cx,
         }
 }
 
+void eb_cylinder_apertures(const int n[3], const double prob_lo[3], const double dx[3], double cx,
+                           double cy, double r, std::vector<double> ap[3],
+                           std::vector<unsigned char>& covered) {
+    // fraction of the segment [y0, y0 + h] (at abscissa x) outside the disk
+    auto seg = [&](double x, double c_par, double c_perp, double y0, double h) {
+        const double d = x - c_perp;
+        if (std::fabs(d) >= r) return 1.0;
+        const double s = std::sqrt(r * r - d * d);
+        const double lo = std::max(c_par - s, y0), hi = std::min(c_par + s, y0 + h);
+        return hi > lo ? 1.0 - (hi - lo) / h : 1.0;
+    };
+    NodalGeom g{};
+    for (int d = 0; d < 3; ++d) {
+        g.n[d] = n[d];
+        g.dx[d] = dx[d];
+    }
+    std::vector<int> type;
+    std::vector<double> mom;
+    eb_cylinder_moments(g, prob_lo, cx, cy, r, type, mom);
+    const long nc = (long)n[0] * n[1] * n[2];
+    covered.assign(nc, 0);
+    for (long c = 0; c < nc; ++c) covered[c] = type[c] == 0;
+    for (int d = 0; d < 3; ++d) {
+        const int f0 = n[0] + (d == 0), f1 = n[1] + (d == 1), f2 = n[2] + (d == 2);
+        ap[d].assign((size_t)f0 * f1 * f2, 1.0);
+        for (int k = 0; k < f2; ++k)
+            for (int j = 0; j < f1; ++j)
+                for (int i = 0; i < f0; ++i) {
+                    double a;
+                    if (d == 0) a = seg(prob_lo[0] + i * dx[0], cy, cx, prob_lo[1] + j * dx[1], dx[1]);
+                    else if (d == 1) a = seg(prob_lo[1] + j * dx[1], cx, cy, prob_lo[0] + i * dx[0], dx[0]);
+                    else a = mom[27 * (i + (long)n[0] * (j + (long)n[1] * std::min(k, n[2] - 1)))];
+                    ap[d][i + (long)f0 * (j + (long)f1 * k)] = a;
+                }
+    }
+}
+
 void eb_element_from_moments(const double* mom, const double dx[3], double E[64]) {
     for (int a = 0; a < 8; ++a)
         for (int b = 0; b < 8; ++b) {

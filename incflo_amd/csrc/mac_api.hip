@@ -1,6 +1,7 @@
 // mac_api.hip -- C ABI of the MAC projection (Hydro::MacProjector surface).
 #include <memory>
 
+#include "eb_nodal.h"
 #include "mac_mg.h"
 #include "mgx_ctx.h"
 
@@ -30,6 +31,41 @@ struct mgx_mac_s {
     BoxLayout layout;
     DevBuf u[3], rhs, cells;
     bool have_phi = false;
+    // embedded boundary: apertures, raw beta (without apertures), covered cells
+    bool eb_on = false;
+    double eb_cyl[3] = {0, 0, 0};
+    DevBuf eb_ap[3], eb_braw[3], eb_cov;
+    void build_eb() {
+        if (!eb_on) return;
+        std::vector<double> ap[3];
+        std::vector<unsigned char> cov;
+        eb_cylinder_apertures(g.n, prob_lo, g.dx, eb_cyl[0], eb_cyl[1], eb_cyl[2], ap, cov);
+        for (int d = 0; d < 3; ++d) {
+            eb_ap[d].alloc(ap[d].size() * sizeof(double));
+            MGX_HIP(hipMemcpy(eb_ap[d].p, ap[d].data(), ap[d].size() * sizeof(double),
+                              hipMemcpyHostToDevice));
+            eb_braw[d].alloc(ap[d].size() * sizeof(double));
+        }
+        eb_cov.alloc(cov.size());
+        MGX_HIP(hipMemcpy(eb_cov.p, cov.data(), cov.size(), hipMemcpyHostToDevice));
+    }
+    // after a beta setter: keep the raw beta, hand apertures * beta to the solver
+    void eb_post_beta() {
+        if (!eb_on) return;
+        for (int d = 0; d < 3; ++d) {
+            const long nf = g.nface(d);
+            MGX_HIP(hipMemcpyAsync(eb_braw[d].p, solver.beta(d), nf * sizeof(double),
+                                   hipMemcpyDeviceToDevice, stream()));
+            eb_face_scale_kernel<<<grid1d(nf), 256, 0, stream()>>>(eb_ap[d].as<double>(),
+                                                                    solver.beta(d), nf);
+        }
+    }
+    void eb_zero_covered() {
+        if (eb_on)
+            eb_zero_covered_kernel<<<grid1d(g.ncell()), 256, 0, stream()>>>(
+                eb_cov.as<unsigned char>(), solver.phi, g.ncell());
+    }
+    const double* raw_beta(int d) { return eb_on ? eb_braw[d].as<double>() : solver.beta(d); }
 
     KGeom kg() const {
         KGeom k{};
@@ -64,6 +100,7 @@ struct mgx_mac_s {
         cells.alloc(g.ncell() * sizeof(double));
         have_phi = false;
         dirty = false;
+        build_eb();
     }
     void faces_in(int d, const double* boxed, int ng, double* canon) {
         face_copy_kernel<<<layout.ntiles[REG_FACE0 + d], 256, 0, stream()>>>(
@@ -177,6 +214,7 @@ MGX_API int mgx_mac_set_beta(mgx_mac h, const double* bx, const double* by, cons
         h->ensure_setup();
         const double* b[3] = {bx, by, bz};
         for (int d = 0; d < 3; ++d) h->faces_in(d, b[d], ng, h->solver.beta(d));
+        h->eb_post_beta();
         h->solver.beta_changed();
     });
 }
@@ -190,6 +228,7 @@ MGX_API int mgx_mac_set_const_beta(mgx_mac h, double beta) {
             const long nf = h->g.nface(d);
             fill_kernel<<<grid1d(nf), 256, 0, h->stream()>>>(h->solver.beta(d), nf, beta);
         }
+        h->eb_post_beta();
         h->solver.beta_changed();
     });
 }
@@ -203,6 +242,7 @@ MGX_API int mgx_mac_set_beta_from_density(mgx_mac h, const double* rho, int rho_
         for (int d = 0; d < 3; ++d)
             mac_beta_from_rho_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
                 h->g.k(), d, h->cells.as<double>(), dt, h->solver.beta(d));
+        h->eb_post_beta();
         h->solver.beta_changed();
     });
 }
@@ -219,15 +259,28 @@ MGX_API int mgx_mac_project(mgx_mac h, double* u, double* v, double* w, int umac
         double* U[3] = {u, v, w};
         for (int d = 0; d < 3; ++d) h->faces_in(d, U[d], umac_ng, h->u[d].as<double>());
         const MGeomK k = h->g.k();
-        mac_div_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
-            k, h->u[0].as<double>(), h->u[1].as<double>(), h->u[2].as<double>(), 1.0 / h->g.dx[0],
-            1.0 / h->g.dx[1], 1.0 / h->g.dx[2], h->rhs.as<double>());
+        if (h->eb_on)
+            eb_mac_div_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
+                k, h->u[0].as<double>(), h->u[1].as<double>(), h->u[2].as<double>(),
+                h->eb_ap[0].as<double>(), h->eb_ap[1].as<double>(), h->eb_ap[2].as<double>(),
+                1.0 / h->g.dx[0], 1.0 / h->g.dx[1], 1.0 / h->g.dx[2], h->rhs.as<double>());
+        else
+            mac_div_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
+                k, h->u[0].as<double>(), h->u[1].as<double>(), h->u[2].as<double>(),
+                1.0 / h->g.dx[0], 1.0 / h->g.dx[1], 1.0 / h->g.dx[2], h->rhs.as<double>());
         if (phi) h->cells_in(phi, phi_ng, h->solver.phi);
         else MGX_HIP(hipMemsetAsync(h->solver.phi, 0, h->g.ncell() * sizeof(double), h->stream()));
         h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
+        h->eb_zero_covered();
         for (int d = 0; d < 3; ++d) {
-            mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
-                k, d, h->solver.beta(d), h->solver.phi, 1.0 / h->g.dx[d], h->u[d].as<double>(), 0);
+            if (h->eb_on)
+                eb_mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                    k, d, h->raw_beta(d), h->eb_ap[d].as<double>(), h->solver.phi,
+                    1.0 / h->g.dx[d], h->u[d].as<double>(), 0);
+            else
+                mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                    k, d, h->solver.beta(d), h->solver.phi, 1.0 / h->g.dx[d], h->u[d].as<double>(),
+                    0);
             h->faces_out(d, U[d], umac_ng, h->u[d].as<double>());
         }
         if (phi) h->cells_out(phi, phi_ng, h->solver.phi);
@@ -248,8 +301,14 @@ MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, in
         double* F[3] = {fx, fy, fz};
         const MGeomK k = h->g.k();
         for (int d = 0; d < 3; ++d) {
-            mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
-                k, d, h->solver.beta(d), h->solver.phi, 1.0 / h->g.dx[d], h->u[d].as<double>(), 1);
+            if (h->eb_on)
+                eb_mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                    k, d, h->raw_beta(d), h->eb_ap[d].as<double>(), h->solver.phi,
+                    1.0 / h->g.dx[d], h->u[d].as<double>(), 1);
+            else
+                mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                    k, d, h->solver.beta(d), h->solver.phi, 1.0 / h->g.dx[d], h->u[d].as<double>(),
+                    1);
             h->faces_out(d, F[d], ng, h->u[d].as<double>());
         }
     });
@@ -266,6 +325,7 @@ MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs,
         h->cells_in(rhs, rhs_ng, h->rhs.as<double>());
         h->cells_in(phi, phi_ng, h->solver.phi);
         h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
+        h->eb_zero_covered();
         h->cells_out(phi, phi_ng, h->solver.phi);
         h->have_phi = true;
     });
@@ -274,6 +334,26 @@ MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs,
         status = MGX_ERR_NOT_CONVERGED;
     }
     return status;
+}
+
+MGX_API int mgx_mac_set_eb_cylinder(mgx_mac h, double cx, double cy, double r) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(r > 0.0, MGX_ERR_ARG, "EB cylinder: radius must be positive");
+        MGX_REQUIRE(!h->solver.beta_set, MGX_ERR_STATE,
+                    "MAC projector: set the embedded boundary before the coefficients");
+        bool dir = false;
+        for (int d = 0; d < 3; ++d)
+            if (!h->periodic[d] && (h->bclo[d] == MGX_BC_DIRICHLET || h->bchi[d] == MGX_BC_DIRICHLET))
+                dir = true;
+        MGX_REQUIRE(dir, MGX_ERR_UNSUPPORTED,
+                    "MAC projector with an embedded boundary needs a Dirichlet (outflow) face");
+        h->eb_on = true;
+        h->eb_cyl[0] = cx;
+        h->eb_cyl[1] = cy;
+        h->eb_cyl[2] = r;
+        if (!h->dirty) h->build_eb();
+    });
 }
 
 }  // extern "C"

@@ -342,4 +342,60 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
     if (tid == 0 && iters_acc) atomicAdd(iters_acc, it);
 }
 
+// ---- embedded boundary (cut cells) -------------------------------------
+// With face apertures a_f (fluid fraction of each face) the EB operator in
+// symmetric form is the regular operator with face coefficients a_f beta_f;
+// its right-hand side is div(a u) = V div_EB(u); covered cells have only
+// zero-aperture faces and drop out (zero rows).
+
+// beta *= aperture on the faces of one direction
+static __global__ void eb_face_scale_kernel(const double* __restrict__ ap, double* __restrict__ beta,
+                                            long nf) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < nf) beta[id] *= ap[id];
+}
+
+// rhs = div(a u) on cells
+static __global__ void eb_mac_div_kernel(MGeomK g, const double* __restrict__ u,
+                                         const double* __restrict__ v, const double* __restrict__ w,
+                                         const double* __restrict__ ax, const double* __restrict__ ay,
+                                         const double* __restrict__ az, double ihx, double ihy,
+                                         double ihz, double* __restrict__ rhs) {
+    int i, j, k;
+    if (!grid_point(g.n[0], g.n[1], i, j, k)) return;
+    const long x1 = fidx(g, 0, i + 1, j, k), x0 = fidx(g, 0, i, j, k);
+    const long y1 = fidx(g, 1, i, j + 1, k), y0 = fidx(g, 1, i, j, k);
+    const long z1 = fidx(g, 2, i, j, k + 1), z0 = fidx(g, 2, i, j, k);
+    rhs[cidx(g, i, j, k)] = (ax[x1] * u[x1] - ax[x0] * u[x0]) * ihx +
+                            (ay[y1] * v[y1] - ay[y0] * v[y0]) * ihy +
+                            (az[z1] * w[z1] - az[z0] * w[z0]) * ihz;
+}
+
+// u -= beta grad(phi) on faces with fluid, u = 0 on fully covered faces
+// (mode_fluxes: out = -beta grad(phi) there, 0 on covered faces)
+static __global__ void eb_mac_face_update_kernel(MGeomK g, int d, const double* __restrict__ beta,
+                                                 const double* __restrict__ ap,
+                                                 const double* __restrict__ phi, double ih,
+                                                 double* __restrict__ u, int mode_fluxes) {
+    const int f0 = g.n[0] + (d == 0), f1 = g.n[1] + (d == 1), f2 = g.n[2] + (d == 2);
+    int i, j, k;
+    if (!grid_point(f0, f1, i, j, k)) return;
+    if (k >= f2) return;
+    const long id = fidx(g, d, i, j, k);
+    if (ap[id] == 0.0) {
+        u[id] = 0.0;
+        return;
+    }
+    const double F = mac_face_flux<double>(g, beta, phi, d, i, j, k, ih);
+    if (mode_fluxes) u[id] = -F;
+    else u[id] -= F;
+}
+
+// phi = 0 in covered cells
+static __global__ void eb_zero_covered_kernel(const unsigned char* __restrict__ covered,
+                                              double* __restrict__ phi, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n && covered[id]) phi[id] = 0.0;
+}
+
 }  // namespace mgx

@@ -109,6 +109,10 @@ static int mac_nlevels(const MacGeom& g, int max_coarsening_level) {
 }
 
 void MacSolver::setup(const MacGeom& g, const MGOpts& o) {
+    // face coefficients do not depend on the boundary types or the multigrid
+    // options: a re-setup for new ones keeps them
+    const bool keep_beta = beta_set && g.n[0] == g0.n[0] && g.n[1] == g0.n[1] && g.n[2] == g0.n[2] &&
+                           g.per[0] == g0.per[0] && g.per[1] == g0.per[1] && g.per[2] == g0.per[2];
     g0 = g;
     opts = o;
     nlev = mac_nlevels(g, o.max_coarsening_level);
@@ -123,7 +127,8 @@ void MacSolver::setup(const MacGeom& g, const MGOpts& o) {
     ibuf.alloc(16 * sizeof(int));
     if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
     H64.built = H32.built = false;
-    beta_set = h32_beta = h64_beta = false;
+    beta_set = keep_beta;
+    h32_beta = h64_beta = false;
 }
 
 double MacSolver::read_norm(int slot, hipStream_t s) {

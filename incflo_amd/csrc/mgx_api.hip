@@ -621,3 +621,20 @@ extern "C" MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* lev, doubl
         return MGX_ERR_ARG;
     }
 }
+
+extern "C" MGX_API int mgx_eb_cylinder_apertures(const mgx_level_desc* lev, double cx, double cy,
+                                                double r, double* ax, double* ay, double* az) {
+    if (!lev || !ax || !ay || !az || r <= 0.0) return MGX_ERR_ARG;
+    try {
+        double dx[3];
+        for (int d = 0; d < 3; ++d) dx[d] = (lev->prob_hi[d] - lev->prob_lo[d]) / lev->n_cell[d];
+        std::vector<double> ap[3];
+        std::vector<unsigned char> cov;
+        eb_cylinder_apertures(lev->n_cell, lev->prob_lo, dx, cx, cy, r, ap, cov);
+        double* out[3] = {ax, ay, az};
+        for (int d = 0; d < 3; ++d) std::memcpy(out[d], ap[d].data(), ap[d].size() * sizeof(double));
+        return MGX_OK;
+    } catch (...) {
+        return MGX_ERR_ARG;
+    }
+}

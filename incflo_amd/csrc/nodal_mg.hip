@@ -282,12 +282,19 @@ NodalSolver::~NodalSolver() {
 }
 
 void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
+    // sigma (cells with one ghost layer: periodic images or 0) does not depend
+    // on the boundary types or the multigrid options, so a re-setup for new
+    // ones (e.g. setDomainBC after the coefficients, as incflo calls them)
+    // keeps it
+    const bool keep_sigma = sigma_set && g.n[0] == g0.n[0] && g.n[1] == g0.n[1] &&
+                            g.n[2] == g0.n[2] && g.per[0] == g0.per[0] && g.per[1] == g0.per[1] &&
+                            g.per[2] == g0.per[2];
     g0 = g;
     opts = o;
     nlev = nodal_nlevels(g, o.max_coarsening_level);
     const long nn = g.nnode();
     sig0_buf.alloc(g.nsig() * sizeof(double));
-    MGX_HIP(hipMemset(sig0_buf.p, 0, g.nsig() * sizeof(double)));
+    if (!keep_sigma) MGX_HIP(hipMemset(sig0_buf.p, 0, g.nsig() * sizeof(double)));
     phi_a.alloc(nn * sizeof(double));
     phi_b.alloc(nn * sizeof(double));
     MGX_HIP(hipMemset(phi_a.p, 0, nn * sizeof(double)));
@@ -298,7 +305,7 @@ void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
     if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
     H64.built = false;
     H32.built = false;
-    sigma_set = false;
+    sigma_set = keep_sigma;
     h32_sigma = false;
     logsig_lvl = -1;
 }

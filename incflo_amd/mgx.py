@@ -124,6 +124,9 @@ def lib():
             "mgx_nodal_set_eb_moments": [vp, vp, vp],
             "mgx_eb_cylinder_geometry": [C.POINTER(LevelDesc), C.c_double, C.c_double, C.c_double,
                                          vp, vp],
+            "mgx_mac_set_eb_cylinder": [vp, C.c_double, C.c_double, C.c_double],
+            "mgx_eb_cylinder_apertures": [C.POINTER(LevelDesc), C.c_double, C.c_double, C.c_double,
+                                          vp, vp, vp],
             "mgx_nodal_set_unet": [vp, vp, C.c_int],
             "mgx_nodal_training_pair": [vp, C.c_int, vp, vp],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
@@ -366,6 +369,10 @@ class MacProjector:
                                              atol, mode, C.byref(info)))
         return info
 
+    def set_eb_cylinder(self, cx, cy, r):
+        """Embedded cylinder along z (fluid outside); before the coefficients."""
+        self.ctx.check(lib().mgx_mac_set_eb_cylinder(self.h, cx, cy, r))
+
     def getFluxes(self, fx, fy, fz, ng=0):
         self.ctx.check(lib().mgx_mac_get_fluxes(self.h, fx.ptr, fy.ptr, fz.ptr, ng))
 
@@ -486,3 +493,18 @@ def eb_cylinder_geometry(level, cx, cy, r):
     if st != MGX_OK:
         raise MgxError(st, "mgx_eb_cylinder_geometry failed")
     return ct, mom
+
+
+def eb_cylinder_apertures(level, cx, cy, r):
+    """Host-side face apertures [ax, ay, az] (canonical faces, (nz, ny, nx) + 1 along d)."""
+    n = [level.n_cell[d] for d in range(3)]
+    out = []
+    for d in range(3):
+        shape = [n[2], n[1], n[0]]
+        shape[2 - d] += 1
+        out.append(np.zeros(shape))
+    st = lib().mgx_eb_cylinder_apertures(C.byref(level), cx, cy, r, out[0].ctypes.data,
+                                         out[1].ctypes.data, out[2].ctypes.data)
+    if st != MGX_OK:
+        raise MgxError(st, "mgx_eb_cylinder_apertures failed")
+    return out

@@ -195,6 +195,10 @@ MGX_API int mgx_nodal_set_eb_cylinder(mgx_nodal h, double cx, double cy, double 
  * covered, 1 regular, 2 cut) and 27 fluid moments int xi^p eta^q zeta^r over
  * the fluid part of the unit cell (index p + 3q + 9r); type NULL removes the EB. */
 MGX_API int mgx_nodal_set_eb_moments(mgx_nodal h, const int* type, const double* moments);
+/* Host-only: the cylinder's face apertures on a level (canonical face arrays,
+ * n_d + 1 faces along d, x fastest) (no GPU). */
+MGX_API int mgx_eb_cylinder_apertures(const mgx_level_desc* level, double cx, double cy, double r,
+                                      double* ax, double* ay, double* az);
 /* Host-only: the cylinder's per-cell types and moments on a level (no GPU). */
 MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* level, double cx, double cy, double r,
                                      int* type, double* moments);
@@ -227,6 +231,12 @@ MGX_API int mgx_mac_project(mgx_mac h, double* umac, double* vmac, double* wmac,
                             mgx_solve_info* info);
 /* getFluxes (src/projection/incflo_apply_cc_projection.cpp:387-389): -beta grad phi on faces. */
 MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, int ng);
+/* EB form (MacProjector with the EB factory, src/incflo.cpp:286-289; MLEBABecLap):
+ * a cylinder along z (fluid outside).  Face apertures a_f weight the
+ * coefficients (operator div(a beta grad)), the right-hand side is div(a u),
+ * covered cells drop out and fully covered faces get u = 0.  Call after
+ * setDomainBC and before the coefficients; needs a Dirichlet face. */
+MGX_API int mgx_mac_set_eb_cylinder(mgx_mac h, double cx, double cy, double r);
 /* MLMG::solve on the MAC operator (boxed cell fields; phi holds the initial guess). */
 MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
                           double rtol, double atol, int mode, mgx_solve_info* info);

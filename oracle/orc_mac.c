@@ -163,7 +163,7 @@ void orc_mac_gsrb(const orc_geom* g, const double* bx, const double* by, const d
                     const double dg = diag_cell(g, beta, i, j, k);
                     if (dg == 0.0) continue;
                     const double Lx = apply_cell(g, beta, x, i, j, k);
-                    x[id] += (b[id] - Lx) / dg;
+                    if (dg != 0.0) x[id] += (b[id] - Lx) / dg; /* zero rows: covered cells */
                 }
     }
 }

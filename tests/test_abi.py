@@ -111,3 +111,28 @@ def test_eb_geometry_matches_oracle(orc):
     assert np.array_equal(ct, cto)
     assert (ct == 2).sum() > 0
     assert np.max(np.abs(mom - momo)) < 1e-12
+
+
+def test_eb_apertures_by_sampling():
+    """Face apertures of the cylinder against dense sampling of the faces
+    (1e-5) and the z-apertures against the cells' volume fractions."""
+    from incflo_amd import boxes as B
+    from incflo_amd import mgx
+    n = (32, 16, 2)
+    cx, cy, r = 0.5, 0.5, 0.2
+    lev = mgx.make_level(n, B.chop_domain(n, 16), periodic=(0, 0, 1), prob_hi=(2.0, 1.0, 0.125))
+    ax, ay, az = mgx.eb_cylinder_apertures(lev, cx, cy, r)
+    h = 1.0 / 16
+    s = (np.arange(200000) + 0.5) / 200000
+    for (i, j) in [(5, 4), (6, 6), (10, 8), (11, 11), (8, 2)]:
+        x = i * h
+        y = (j + s) * h
+        ref = np.mean((x - cx) ** 2 + (y - cy) ** 2 >= r * r)
+        assert abs(ax[0, j, i] - ref) < 1e-5
+        yy = j * h
+        xx = (i + s) * h
+        ref = np.mean((xx - cx) ** 2 + (yy - cy) ** 2 >= r * r)
+        assert abs(ay[0, j, i] - ref) < 1e-5
+    ct, mom = mgx.eb_cylinder_geometry(lev, cx, cy, r)
+    assert np.allclose(az[0], mom[0, ..., 0]) and np.allclose(az[2], mom[1, ..., 0])
+    assert np.all((ax >= 0) & (ax <= 1)) and ax.min() == 0.0

@@ -130,3 +130,49 @@ def test_eb_nodal_projection_channel(ctx, orc):
     assert np.all(unew[:, ct == 0] == 0.0)
     _say(f"  V-cycles {info.iters}, CG iterations {its}")
     P.close()
+
+
+def test_eb_mac_projection_channel(ctx, orc):
+    """EB MAC projection (apertures weight the coefficients and the
+    divergence) against the oracle's MAC solver on the same symmetric system:
+    div(a beta grad phi) = div(a u), u_new = u - beta grad phi where a > 0."""
+    from incflo_amd import mgx
+    _say("case eb mac projection")
+    n, ph = (32, 16, 16), (2.0, 1.0, 1.0)
+    per, bclo, bchi = (0, 0, 1), (N, N, 0), (D, N, 0)
+    g = orc.Geom.make(n, periodic=per, bclo=bclo, bchi=bchi, prob_hi=ph)
+    bxs = B.chop_domain(n, (16, 8, 8))
+    lev = mgx.make_level(n, bxs, periodic=per, prob_hi=ph)
+    M = mgx.MacProjector(ctx, lev)
+    M.setDomainBC(bclo, bchi)
+    M.set_eb_cylinder(CX, CY, R)
+    ap = mgx.eb_cylinder_apertures(lev, CX, CY, R)
+    rng = np.random.default_rng(6)
+    rho = 1.0 + 0.3 * rng.random(g.cell_shape())
+    dt = 0.01
+    u = [rng.standard_normal(orc.face_shape(g, d)) for d in range(3)]
+    u[2][-1] = u[2][0]  # periodic z: the duplicate face carries the same value
+    M.set_beta_from_density(ctx.array(B.to_boxed(rho, bxs, 1, periodic=per)), 1, dt)
+    ud = [ctx.array(B.faces_to_boxed(u[d], bxs, 0, d)) for d in range(3)]
+    phid = ctx.array(B.field_size(bxs, 0, 1))
+    info = M.project(ud[0], ud[1], ud[2], 0, 1e-12, 0.0, phi=phid, phi_ng=0)
+    assert info.converged
+    un = [B.faces_from_boxed(ud[d].download(), bxs, 0, d, orc.face_shape(g, d)) for d in range(3)]
+    phi = B.from_boxed(phid.download(), bxs, 0, g.cell_shape())
+    beta = orc.mac_beta_from_density(g, rho, dt)
+    ba = [beta[d] * ap[d] for d in range(3)]
+    b = orc.mac_divergence(g, [ap[d] * u[d] for d in range(3)])
+    phio, io, ret = orc.mac_solve(g, ba, b, rtol=1e-13)
+    assert ret == 0
+    ct, mom = orc.eb_cylinder(g, CX, CY, R)
+    fluid = ct > 0
+    assert np.max(np.abs(phi[fluid] - phio[fluid])) <= 1e-9 * np.max(np.abs(phio[fluid]))
+    assert np.all(phi[~fluid] == 0.0)
+    F = orc.mac_fluxes(g, beta, phio)
+    for d in range(3):
+        ref = np.where(ap[d] > 0, u[d] + F[d], 0.0)
+        assert np.max(np.abs(un[d] - ref)) <= 1e-9 * np.max(np.abs(ref)), d
+    # the EB divergence of the projected velocity is the solver's residual
+    div1 = orc.mac_divergence(g, [ap[d] * un[d] for d in range(3)])
+    assert np.max(np.abs(div1)) <= 1.01e-12 * np.max(np.abs(b))
+    M.close()
