@@ -1,0 +1,329 @@
+// amr_kernels.h -- kernels of the two-level composite nodal solve
+// (refinement ratio 2; the reference projects all AMR levels in one
+// NodalProjector, src/projection/incflo_apply_nodal_projection.cpp:124-215).
+//
+// Composite problem: conforming Q1 elements on the composite mesh.  F is the
+// union of the fine boxes; its fine nodes are unknowns (AMR_DOF), fine nodes
+// on the coarse-fine boundary are fixed by the coarse nodes (AMR_BDRY:
+// coincident ones take the coarse value, the others the trilinear
+// interpolation of the coarse face / edge), fine nodes away from F are unused
+// (AMR_OUT).  Coarse nodes are unknowns unless every cell around them is
+// covered by F.  Both levels live on dense canonical arrays (the fine one at
+// twice the coarse resolution over the whole domain); the fine cell
+// coefficient is 0 outside F, so the fine stencil sees only the elements of F.
+//   fine unknown m:   (A x)_m = (L_f x~)_m
+//   coarse unknown n: (A x)_n = (L_c^out x_c)_n + R(L_f x~ on dF)_n
+// with L_c^out the coarse operator with covered cells' coefficient 0 and R the
+// full-weighting restriction (P^T / 8).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "box_kernels.h"
+#include "nodal_kernels.h"
+
+namespace mgx {
+
+enum { AMR_OUT = 0, AMR_DOF = 1, AMR_BDRY = 2 };
+
+// trilinear interpolation of the coarse nodes at fine node (i, j, k) (the
+// weights of nodal_prolong_add_kernel)
+template <class T>
+__device__ __forceinline__ T amr_interp(const KGeom& gc, const T* __restrict__ xc, int i, int j,
+                                        int k) {
+    int lo[3], hi[3];
+    const int f[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = f[d] >> 1;
+        hi[d] = (f[d] & 1) ? lo[d] + 1 : lo[d];
+        if (gc.per[d] && hi[d] >= gc.nn[d]) hi[d] -= gc.nn[d];
+    }
+    T acc = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int I = (q & 1) ? hi[0] : lo[0];
+        const int J = (q & 2) ? hi[1] : lo[1];
+        const int K = (q & 4) ? hi[2] : lo[2];
+        acc += xc[I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K)];
+    }
+    return (T)0.125 * acc;
+}
+
+// fine field on the closure of F: dF nodes from the coarse field, 0 outside
+template <class T>
+static __global__ void __launch_bounds__(256) amr_fill_kernel(KGeom gf, KGeom gc,
+                                                              const unsigned char* __restrict__ ft,
+                                                              const T* __restrict__ xc,
+                                                              T* __restrict__ xf) {
+    int i, j, k;
+    if (!grid_point(gf.nn[0], gf.nn[1], i, j, k)) return;
+    const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
+    const int t = ft[id];
+    if (t == AMR_BDRY) xf[id] = amr_interp(gc, xc, i, j, k);
+    else if (t == AMR_OUT) xf[id] = 0;
+}
+
+// coarse correction onto the fine level: unknowns += P e, dF = P e, outside 0
+template <class T>
+static __global__ void __launch_bounds__(256) amr_prolong_kernel(KGeom gf, KGeom gc,
+                                                                 const unsigned char* __restrict__ ft,
+                                                                 const T* __restrict__ ec,
+                                                                 T* __restrict__ ef) {
+    int i, j, k;
+    if (!grid_point(gf.nn[0], gf.nn[1], i, j, k)) return;
+    const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
+    const int t = ft[id];
+    if (t == AMR_OUT) { ef[id] = 0; return; }
+    const T p = amr_interp(gc, ec, i, j, k);
+    ef[id] = (t == AMR_DOF) ? ef[id] + p : p;
+}
+
+// full-weighting sum over the 27 fine nodes around coarse node (I, J, K) of
+// the values selected by want (a node type): sum_o w_o v(2I + o)
+template <class TV>
+__device__ __forceinline__ double amr_restrict_sel(const KGeom& gf, const unsigned char* ft,
+                                                   const TV* v, int want, int I, int J, int K) {
+    double acc = 0.0;
+    for (int oz = -1; oz <= 1; ++oz) {
+        int fk = 2 * K + oz;
+        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
+        else if (fk < 0 || fk >= gf.nn[2]) continue;
+        const double wz = oz == 0 ? 0.5 : 0.25;
+        for (int oy = -1; oy <= 1; ++oy) {
+            int fj = 2 * J + oy;
+            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
+            else if (fj < 0 || fj >= gf.nn[1]) continue;
+            const double wy = oy == 0 ? 0.5 : 0.25;
+            for (int ox = -1; ox <= 1; ++ox) {
+                int fi = 2 * I + ox;
+                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
+                else if (fi < 0 || fi >= gf.nn[0]) continue;
+                const long m = fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk);
+                if (ft[m] != want) continue;
+                const double wx = ox == 0 ? 0.5 : 0.25;
+                acc += wx * wy * wz * (double)v[m];
+            }
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void wave_max_atomic(unsigned long long* norm, double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(norm, v);
+}
+
+// coarse unknowns: out = base + sgn R(y on dF); every other coarse node 0;
+// optional max |out| (every lane reaches the shuffles)
+template <class TO>
+static __global__ void __launch_bounds__(256) amr_coarse_combine_kernel(
+    KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
+    const double* __restrict__ base, const double* __restrict__ y, double sgn, TO* __restrict__ out,
+    unsigned long long* norm) {
+    int I, J, K;
+    const bool valid = grid_point(gc.nn[0], gc.nn[1], I, J, K);
+    double r = 0.0;
+    if (valid) {
+        const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
+        if (cdof[id] && !is_dir(gc, I, J, K))
+            r = base[id] + sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K);
+        out[id] = (TO)r;
+    }
+    if (norm) wave_max_atomic(norm, fabs(r));
+}
+
+// fine unknowns: r = b - y (cast to TO), 0 elsewhere; optional max |r|
+template <class TO>
+static __global__ void __launch_bounds__(256) amr_fine_resid_kernel(
+    KGeom gf, const unsigned char* __restrict__ ft, const double* __restrict__ b,
+    const double* __restrict__ y, TO* __restrict__ out, unsigned long long* norm) {
+    int i, j, k;
+    const bool valid = grid_point(gf.nn[0], gf.nn[1], i, j, k);
+    double r = 0.0;
+    if (valid) {
+        const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
+        if (ft[id] == AMR_DOF) r = b[id] - (y ? y[id] : 0.0);
+        out[id] = (TO)r;
+    }
+    if (norm) wave_max_atomic(norm, fabs(r));
+}
+
+// right-hand side of the coarse-grid correction (all coarse nodes):
+//   b_C = [n unknown] (r_c - R(y on dF)) + R((r_f - y) on fine unknowns)
+// with r the composite residual and y = L_f e after the fine pre-smoothing
+template <class T>
+static __global__ void __launch_bounds__(256) amr_coarse_rhs_kernel(
+    KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
+    const T* __restrict__ rf, const T* __restrict__ yf, const T* __restrict__ rc, T* __restrict__ bC) {
+    int I, J, K;
+    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
+    if (is_dir(gc, I, J, K)) { bC[id] = 0; return; }
+    double acc = 0.0;
+    for (int oz = -1; oz <= 1; ++oz) {
+        int fk = 2 * K + oz;
+        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
+        else if (fk < 0 || fk >= gf.nn[2]) continue;
+        const double wz = oz == 0 ? 0.5 : 0.25;
+        for (int oy = -1; oy <= 1; ++oy) {
+            int fj = 2 * J + oy;
+            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
+            else if (fj < 0 || fj >= gf.nn[1]) continue;
+            const double wy = oy == 0 ? 0.5 : 0.25;
+            for (int ox = -1; ox <= 1; ++ox) {
+                int fi = 2 * I + ox;
+                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
+                else if (fi < 0 || fi >= gf.nn[0]) continue;
+                const long m = fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk);
+                const int t = ft[m];
+                const double w = (ox == 0 ? 0.5 : 0.25) * wy * wz;
+                if (t == AMR_DOF) acc += w * ((double)rf[m] - (double)yf[m]);
+                else if (t == AMR_BDRY && cdof[id]) acc -= w * (double)yf[m];
+            }
+        }
+    }
+    if (cdof[id]) acc += (double)rc[id];
+    bC[id] = (T)acc;
+}
+
+// dst += src where mask (and not a Dirichlet node of g)
+template <class T>
+static __global__ void __launch_bounds__(256) amr_add_kernel(KGeom g, const unsigned char* __restrict__ mask,
+                                                             unsigned char want, const T* __restrict__ src,
+                                                             double* __restrict__ dst) {
+    int i, j, k;
+    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
+    if (mask[id] == want && !is_dir(g, i, j, k)) dst[id] += (double)src[id];
+}
+
+// v = v where mask == want, else 0
+static __global__ void __launch_bounds__(256) amr_select_kernel(KGeom g, const unsigned char* __restrict__ mask,
+                                                                unsigned char want, double* __restrict__ v) {
+    int i, j, k;
+    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
+    if (mask[id] != want) v[id] = 0.0;
+}
+
+// per-block sums of v over the nodes whose mask equals want (mask NULL: all),
+// in a fixed order (deterministic)
+static __global__ void __launch_bounds__(256) amr_sum_kernel(const double* __restrict__ v,
+                                                             const unsigned char* __restrict__ mask,
+                                                             unsigned char want, long n,
+                                                             double* __restrict__ part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256)
+        if (!mask || mask[q] == want) s += v[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out[0] = (wa * sum(pa) + wb * sum(pb)) / count  (one thread, fixed order)
+static __global__ void amr_mean_kernel(const double* __restrict__ pa, int na, double wa,
+                                       const double* __restrict__ pb, int nb, double wb,
+                                       double count, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < na; ++q) a += pa[q];
+    for (int q = 0; q < nb; ++q) b += pb[q];
+    out[0] = (wa * a + wb * b) / count;
+}
+
+// v -= f * mu[0] where mask == want (mask NULL: everywhere), or where mask != want when invert
+static __global__ void __launch_bounds__(256) amr_shift_kernel(double* __restrict__ v,
+                                                               const unsigned char* __restrict__ mask,
+                                                               unsigned char want, int invert, long n,
+                                                               const double* __restrict__ mu, double f) {
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    if (mask && ((mask[q] == want) == (invert != 0))) return;
+    v[q] -= f * mu[0];
+}
+
+// coarse nodes under F (not unknowns) take the coincident fine value
+static __global__ void __launch_bounds__(256) amr_inject_kernel(KGeom gf, KGeom gc,
+                                                                const unsigned char* __restrict__ cdof,
+                                                                const double* __restrict__ xf,
+                                                                double* __restrict__ xc) {
+    int I, J, K;
+    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
+    if (cdof[id]) return;
+    xc[id] = xf[2 * I + (long)gf.nn[0] * (2 * J + (long)gf.nn[1] * 2 * K)];
+}
+
+// coarse cell coefficient with covered cells set to 0 (ghost images included)
+static __global__ void __launch_bounds__(256) amr_sigma_out_kernel(KGeom gc, const unsigned char* __restrict__ cov,
+                                                                   const double* __restrict__ sig,
+                                                                   double* __restrict__ out) {
+    int i, j, k;
+    if (!grid_point(gc.n[0], gc.n[1], i, j, k)) return;
+    const long c = i + (long)gc.n[0] * (j + (long)gc.n[1] * k);
+    const double v = cov[c] ? 0.0 : sig[(i + 1) + (long)gc.sg[0] * ((j + 1) + (long)gc.sg[1] * (k + 1))];
+    write_cell_images(out, gc, i, j, k, v);
+}
+
+// covered coarse cells (ghost images included) of a ghosted 3-comp velocity -> 0
+static __global__ void __launch_bounds__(256) amr_zero_covered_kernel(KGeom gc, const unsigned char* __restrict__ cov,
+                                                                      double* __restrict__ ug) {
+    int i, j, k;
+    if (!grid_point(gc.n[0], gc.n[1], i, j, k)) return;
+    if (!cov[i + (long)gc.n[0] * (j + (long)gc.n[1] * k)]) return;
+    const long ug3 = (long)gc.sg[0] * gc.sg[1] * gc.sg[2];
+    for (int c = 0; c < 3; ++c) write_cell_images(ug + c * ug3, gc, i, j, k, 0.0);
+}
+
+// average_down onto covered coarse cells of the caller's coarse boxes:
+// u_c = mean of the 8 projected fine children, gp_c = mean of their G phi
+// (src/projection/incflo_apply_nodal_projection.cpp:258-266)
+struct AvgDownArgs {
+    KGeom gc, gf;
+    const BoxInfo* boxes;   // coarse layout
+    const Tile* tiles;
+    const unsigned char* cov;
+    const double* phif;     // canonical fine phi (closure of F filled)
+    const double* ugf;      // canonical ghosted fine u*
+    const double* sigf;     // canonical ghosted fine sigma
+    double ff[3];           // fine 1/(4 h_d)
+    double* vel; int vel_ng;
+    double* gp; int gp_ng;
+};
+
+static __global__ void __launch_bounds__(256) amr_avgdown_kernel(AvgDownArgs a) {
+    int bi, i, j, k;
+    if (!tile_point(a.boxes, a.tiles, REG_CELL, bi, i, j, k)) return;
+    if (!a.cov[i + (long)a.gc.n[0] * (j + (long)a.gc.n[1] * k)]) return;
+    const BoxInfo& b = a.boxes[bi];
+    const KGeom& gf = a.gf;
+    const long ug3 = (long)gf.sg[0] * gf.sg[1] * gf.sg[2];
+    double us[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < 8; ++q) {
+        const int fi = 2 * i + (q & 1), fj = 2 * j + ((q >> 1) & 1), fk = 2 * k + ((q >> 2) & 1);
+        double gr[3];
+        cell_gradient(gf, a.phif, fi, fj, fk, a.ff, gr);
+        const long gi = (fi + 1) + (long)gf.sg[0] * ((fj + 1) + (long)gf.sg[1] * (fk + 1));
+        const double s = a.sigf[gi];
+        for (int c = 0; c < 3; ++c) {
+            gs[c] += gr[c];
+            us[c] += a.ugf[c * ug3 + gi] - s * gr[c];
+        }
+    }
+    if (a.vel) {
+        const long off = field_offset(b, a.vel_ng, 3, false);
+        for (int c = 0; c < 3; ++c) a.vel[off + box_index(b, a.vel_ng, false, i, j, k, c)] = 0.125 * us[c];
+    }
+    if (a.gp) {
+        const long off = field_offset(b, a.gp_ng, 3, false);
+        for (int c = 0; c < 3; ++c) a.gp[off + box_index(b, a.gp_ng, false, i, j, k, c)] = 0.125 * gs[c];
+    }
+}
+
+}  // namespace mgx

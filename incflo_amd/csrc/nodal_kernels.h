@@ -123,6 +123,12 @@ struct SArgs {
     int kz;          // node planes per workgroup
     int gx, gy, gz;  // tile grid
     unsigned long long* norm;  // optional max |r|
+    TC* x1out;       // RESID/UPDRES, optional: omega1 r / diag, the first Jacobi
+    double omega1;   //   iterate from zero of the next V-cycle (saves its pass)
+    // optional node types (composite AMR levels): 1 unknown, 2 fixed (JACOBI
+    // keeps the value), 0 outside (JACOBI / JFIRST write 0); RESID writes
+    // r = 0 on every node that is not an unknown
+    const unsigned char* nmask;
 };
 
 // One 27-point pass over a 64 x 8 node tile, streaming along z.  A workgroup
@@ -307,7 +313,8 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
 #pragma unroll
                         for (int r = 0; r < 2; ++r) ss += S[p][n + r][0] + S[p][n + r][1];
                     const long id = i + (long)g.nn[0] * j + sxy * k;
-                    a.xout[id] = (is_dir(g, i, j, k) || ss == (T)0)
+                    const bool fixed = a.nmask && a.nmask[id] != 1;
+                    a.xout[id] = (is_dir(g, i, j, k) || ss == (T)0 || fixed)
                                      ? (T)0 : a.omega * bcur[n] / (-a.c.diag4 * ss);
                     continue;
                 }
@@ -316,19 +323,24 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
                                        : apply27<T, ISO, (NPT > 1 ? 1 : 0)>(P, S, a.c, ssum);
                 const T c = P[1][n + 1][1];
                 const long id = i + (long)g.nn[0] * j + sxy * k;
+                const int nt = a.nmask ? a.nmask[id] : 1;
                 const bool dir = is_dir(g, i, j, k);
                 const T bv = bcur[n];
                 if (MODE == KS_JACOBI) {
                     T v = c;
-                    if (dir) v = 0;
-                    else if (ssum != (T)0) v = c + a.omega * (bv - acc) / (-a.c.diag4 * ssum);
+                    if (dir || nt == 0) v = 0;
+                    else if (nt == 1 && ssum != (T)0) v = c + a.omega * (bv - acc) / (-a.c.diag4 * ssum);
                     a.xout[id] = v;
                 } else if (MODE == KS_APPLY) {
                     a.rout[id] = (TC)acc;
                 } else {
-                    const T r = dir ? (T)0 : (bv - acc);
+                    const T r = (dir || nt != 1) ? (T)0 : (bv - acc);
                     a.rout[id] = (TC)r;
                     if (MODE == KS_UPDRES) a.xout[id] = dir ? (T)0 : c;
+                    if (a.x1out)
+                        a.x1out[id] = (dir || ssum == (T)0)
+                                          ? (TC)0
+                                          : (TC)(a.omega1 * (double)r / (-(double)a.c.diag4 * (double)ssum));
                     rmax = fmax(rmax, (double)fabs(r));
                 }
             }
@@ -360,11 +372,16 @@ static __global__ void nodal_jacobi_first_kernel(KGeom g, Coef<T> c, const T* __
 // Full weighting: rc(I) = sum_{o in {-1,0,1}^3} prod_d w(o_d) rf(2I+o), w = (1/4,1/2,1/4).
 template <class T>
 static __global__ void nodal_restrict_kernel(KGeom gf, KGeom gc, const T* __restrict__ rf,
-                                      T* __restrict__ rc) {
+                                      T* __restrict__ rc, const T* __restrict__ sigc = nullptr,
+                                      T diag4c = 0, T omegac = 0, T* __restrict__ xc1 = nullptr) {
     int I, J, K;
     if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
-    if (is_dir(gc, I, J, K)) { rc[id] = 0; return; }
+    if (is_dir(gc, I, J, K)) {
+        rc[id] = 0;
+        if (xc1) xc1[id] = 0;
+        return;
+    }
     T acc = 0;
 #pragma unroll
     for (int oz = -1; oz <= 1; ++oz) {
@@ -389,6 +406,14 @@ static __global__ void nodal_restrict_kernel(KGeom gf, KGeom gc, const T* __rest
         }
     }
     rc[id] = acc;
+    if (xc1) {
+        // first coarse Jacobi sweep from zero: omega rc / diag (same as KS_JFIRST)
+        T ss = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            ss += sigc[(I + (q & 1)) + (long)gc.sg[0] * ((J + ((q >> 1) & 1)) + (long)gc.sg[1] * (K + ((q >> 2) & 1)))];
+        xc1[id] = (ss != (T)0) ? omegac * acc / (-diag4c * ss) : (T)0;
+    }
 }
 
 // Trilinear interpolation of coarse nodes added onto fine nodes.

@@ -77,7 +77,12 @@ class NodalHierarchy {
     void set_sigma(const double* sig0, hipStream_t s);
     // V-cycle on the correction equation: input L[0].b, output L[0].x.
     void vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters);
-    void smooth(int lev, int nsweeps, bool from_zero, const MGOpts& o, hipStream_t s);
+    // x_ready: L[lev].x already holds the first sweep from zero (written by the
+    // pass that produced L[lev].b), so that sweep is skipped
+    void smooth(int lev, int nsweeps, bool from_zero, const MGOpts& o, hipStream_t s,
+                bool x_ready = false);
+    // set by the outer iteration when its residual pass also wrote L[0].x
+    bool x0_ready = false;
     bool built = false;
 };
 
@@ -141,7 +146,8 @@ class BoxLayout {
     DevBuf dboxes;
     DevBuf dtiles[REG_NREG];
     int ntiles[REG_NREG] = {0, 0, 0, 0, 0, 0};
-    void build(const int* boxes6, int nb, const NodalGeom& g);
+    // partial: the boxes need not tile the domain (an AMR fine level)
+    void build(const int* boxes6, int nb, const NodalGeom& g, bool partial = false);
     const BoxInfo* dev_boxes() const { return dboxes.as<BoxInfo>(); }
     const Tile* tiles(int region) const { return dtiles[region].as<Tile>(); }
 };

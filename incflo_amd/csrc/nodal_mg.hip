@@ -142,11 +142,14 @@ void NodalHierarchy<T>::set_sigma(const double* sig0, hipStream_t s) {
 }
 
 template <class T>
-void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts& o, hipStream_t s) {
+void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts& o, hipStream_t s,
+                               bool x_ready) {
     MGLevel<T>& lv = L[l];
     const long nn = lv.g.nnode();
     int sweeps = nsweeps;
-    if (from_zero) {
+    if (from_zero && x_ready && sweeps > 0) {
+        --sweeps;
+    } else if (from_zero) {
         if (sweeps == 0) {
             zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.x, nn);
             return;
@@ -206,9 +209,11 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         while (lc > 0 && nl - (lc - 1) <= kMaxCoarse && L[lc - 1].g.nnode() <= coarse_threshold() &&
                lds_for(lc - 1) <= kLdsBudget)
             --lc;
+    bool ready = x0_ready && o.nu1 > 0;
+    x0_ready = false;
     for (int l = 0; l < lc; ++l) {
         MGLevel<T>& lv = L[l];
-        smooth(l, o.nu1, true, o, s);
+        smooth(l, o.nu1, true, o, s, ready);
         SArgs<T, T> a{};
         a.g = lv.kg;
         a.c = lv.coef;
@@ -217,8 +222,13 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         a.sig = lv.sig;
         a.rout = lv.r;
         launch_stencil<T, T, KS_RESID>(lv.g, a, s, false);
-        const long nc = L[l + 1].g.nnode();
-        nodal_restrict_kernel<T><<<grid3(L[l + 1].g.nn[0], L[l + 1].g.nn[1], L[l + 1].g.nn[2]), block3(), 0, s>>>(lv.kg, L[l + 1].kg, lv.r, L[l + 1].b);
+        // a coarse level smoothed by its own launches gets its first sweep
+        // from zero out of the restriction
+        MGLevel<T>& lc1 = L[l + 1];
+        ready = (l + 1 < lc) && o.nu1 > 0;
+        nodal_restrict_kernel<T><<<grid3(lc1.g.nn[0], lc1.g.nn[1], lc1.g.nn[2]), block3(), 0, s>>>(
+            lv.kg, lc1.kg, lv.r, lc1.b, lc1.sig, lc1.coef.diag4, (T)nodal_omega(lc1.g, o.omega),
+            ready ? lc1.x : nullptr);
     }
     if (!bottom_in_lds) {
         MGLevel<T>& lb = L[nl - 1];
@@ -271,6 +281,12 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
 }
 
 template void launch_stencil<double, double, KS_APPLY>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
+template void launch_stencil<double, double, KS_RESID>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
+template void launch_stencil<double, double, KS_JACOBI>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
+template void launch_stencil<double, double, KS_JFIRST>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
+template void launch_stencil<float, float, KS_APPLY>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool);
+template void launch_stencil<float, float, KS_JACOBI>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool);
+template void launch_stencil<float, float, KS_JFIRST>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool);
 
 template class NodalHierarchy<double>;
 template class NodalHierarchy<float>;
@@ -477,6 +493,21 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     const double bnorm = read_norm(0, s);
     const Coef<double> c64 = make_coef<double>(g.dx);
 
+    // The outer residual pass also writes the next V-cycle's first Jacobi
+    // sweep from zero (omega r / diag) into the hierarchy's spare level-0
+    // array (the live one may be the correction this pass reads).
+    auto first_sweep = [&](auto& a, auto& H) {
+        if (opts.nu1 > 0 && H.L.size() > 1) {
+            a.x1out = H.L[0].x2;
+            a.omega1 = nodal_omega(g, opts.omega);
+        }
+    };
+    auto done_first_sweep = [&](auto& a, auto& H) {
+        if (a.x1out) {
+            std::swap(H.L[0].x, H.L[0].x2);
+            H.x0_ready = true;
+        }
+    };
     // r = b - L phi into the V-cycle's level-0 right-hand side
     auto residual = [&](bool into32) {
         zero_norm(1, s);
@@ -484,12 +515,16 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
             SArgs<double, float> a{};
             a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
             a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
+            first_sweep(a, H32);
             launch_stencil<double, float, KS_RESID>(g, a, s, false);
+            done_first_sweep(a, H32);
         } else {
             SArgs<double, double> a{};
             a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
             a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
+            first_sweep(a, H64);
             launch_stencil<double, double, KS_RESID>(g, a, s, false);
+            done_first_sweep(a, H64);
         }
         return read_norm(1, s);
     };
@@ -500,12 +535,16 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
             SArgs<double, float> a{};
             a.g = kg; a.c = c64; a.x = phi; a.cor = H32.L[0].x; a.b = b; a.sig = sigma0();
             a.xout = pout; a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
+            first_sweep(a, H32);
             launch_stencil<double, float, KS_UPDRES>(g, a, s, false);
+            done_first_sweep(a, H32);
         } else {
             SArgs<double, double> a{};
             a.g = kg; a.c = c64; a.x = phi; a.cor = H64.L[0].x; a.b = b; a.sig = sigma0();
             a.xout = pout; a.rout = res64.as<double>(); a.norm = scal.as<unsigned long long>() + 1;
+            first_sweep(a, H64);
             launch_stencil<double, double, KS_UPDRES>(g, a, s, false);
+            done_first_sweep(a, H64);
         }
         return read_norm(1, s);
     };
@@ -593,7 +632,7 @@ void NodalSolver::training_pair(const double* b, int lvl, float* in, float* targ
 
 // ---------------------------------------------------------------------------
 
-void BoxLayout::build(const int* b6, int nb, const NodalGeom& g) {
+void BoxLayout::build(const int* b6, int nb, const NodalGeom& g, bool partial) {
     MGX_REQUIRE(nb > 0 && b6 != nullptr, MGX_ERR_ARG, "box list is empty");
     boxes.assign(b6, b6 + 6 * nb);
     nboxes = nb;
@@ -607,7 +646,7 @@ void BoxLayout::build(const int* b6, int nb, const NodalGeom& g) {
         vol += (long)(b6[6 * b + 3] - b6[6 * b] + 1) * (b6[6 * b + 4] - b6[6 * b + 1] + 1) *
                (b6[6 * b + 5] - b6[6 * b + 2] + 1);
     }
-    MGX_REQUIRE(vol == (long)g.n[0] * g.n[1] * g.n[2], MGX_ERR_UNSUPPORTED,
+    MGX_REQUIRE(partial || vol == (long)g.n[0] * g.n[1] * g.n[2], MGX_ERR_UNSUPPORTED,
                 "single-level solve needs boxes that tile the whole domain");
     for (int a = 0; a < nb; ++a)
         for (int b = a + 1; b < nb; ++b) {

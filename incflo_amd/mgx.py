@@ -129,6 +129,14 @@ def lib():
                                           vp, vp, vp],
             "mgx_nodal_set_unet": [vp, vp, C.c_int],
             "mgx_nodal_training_pair": [vp, C.c_int, vp, vp],
+            "mgx_amr_nodal_create": [vp, C.POINTER(LevelDesc), C.c_int, C.POINTER(vp)],
+            "mgx_amr_nodal_destroy": [vp],
+            "mgx_amr_nodal_set_domain_bc": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "mgx_amr_nodal_set_options": [vp, C.POINTER(MLMGOpts)],
+            "mgx_amr_nodal_set_sigma": [vp, C.POINTER(vp), C.c_int],
+            "mgx_amr_nodal_project": [vp, C.POINTER(vp), C.c_int, C.POINTER(vp), C.c_int,
+                                      C.POINTER(vp), C.c_int, C.c_double, C.c_double, C.c_int,
+                                      C.POINTER(SolveInfo)],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
                                            vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp,
                                            C.c_double, C.c_double, C.c_int,
@@ -508,3 +516,45 @@ def eb_cylinder_apertures(level, cx, cy, r):
     if st != MGX_OK:
         raise MgxError(st, "mgx_eb_cylinder_apertures failed")
     return out
+
+
+def _ptrs(arrs):
+    """array of device pointers (None -> NULL) for the per-level arguments"""
+    return (C.c_void_p * len(arrs))(*[a.ptr if a is not None else None for a in arrs])
+
+
+class AmrNodalProjector:
+    """Composite two-level nodal projection (mgx_amr_nodal_*): levels[0] tiles
+    the domain, levels[1] holds the fine boxes (refinement ratio 2)."""
+
+    def __init__(self, ctx, levels):
+        self.ctx = ctx
+        self.levels = levels
+        arr = (LevelDesc * len(levels))(*levels)
+        self._arr = arr
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_amr_nodal_create(ctx.h, arr, len(levels), C.byref(self.h)))
+
+    def setDomainBC(self, lo, hi):
+        a = (C.c_int * 3)(*lo)
+        b = (C.c_int * 3)(*hi)
+        self.ctx.check(lib().mgx_amr_nodal_set_domain_bc(self.h, a, b))
+
+    def set_options(self, opts):
+        self.ctx.check(lib().mgx_amr_nodal_set_options(self.h, C.byref(opts)))
+
+    def set_sigma(self, sigmas, ng=0):
+        self.ctx.check(lib().mgx_amr_nodal_set_sigma(self.h, _ptrs(sigmas), ng))
+
+    def project(self, vels, vel_ng, rtol, atol=0.0, phis=None, phi_ng=0, gphis=None, gphi_ng=0,
+                mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_amr_nodal_project(
+            self.h, _ptrs(vels), vel_ng, _ptrs(phis) if phis else None, phi_ng,
+            _ptrs(gphis) if gphis else None, gphi_ng, rtol, atol, mode, C.byref(info)))
+        return info
+
+    def close(self):
+        if self.h:
+            lib().mgx_amr_nodal_destroy(self.h)
+            self.h = C.c_void_p()

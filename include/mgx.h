@@ -203,6 +203,37 @@ MGX_API int mgx_eb_cylinder_apertures(const mgx_level_desc* level, double cx, do
 MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* level, double cx, double cy, double r,
                                      int* type, double* moments);
 
+/* ---- AMR: composite two-level nodal projection --------------------------
+ * incflo projects every AMR level with one NodalProjector
+ * (src/projection/incflo_apply_nodal_projection.cpp:124-215: vel and sigma
+ * per level, project(rtol, atol), getPhi / getGradPhi per level) and then
+ * averages gp down (:258-266).  levels[0] tiles the domain; levels[1] (if
+ * nlevels == 2) has n_cell = 2 x levels[0].n_cell, the same domain and
+ * periodicity, and boxes aligned with coarse cells (even lo, odd hi) that
+ * stay off non-periodic domain faces.  The composite problem is the
+ * conforming Q1 discretisation on the composite mesh (fine nodes inside the
+ * fine region, coarse nodes elsewhere; coarse-fine boundary nodes follow the
+ * coarse level); covered coarse cells receive the average of their fine
+ * children for the velocity and G phi, coarse nodes under the fine region the
+ * fine value. */
+typedef struct mgx_amr_s* mgx_amr_nodal;
+MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int nlevels,
+                                 mgx_amr_nodal* out);
+MGX_API int mgx_amr_nodal_destroy(mgx_amr_nodal h);
+/* setDomainBC (:194), as mgx_nodal_set_domain_bc. */
+MGX_API int mgx_amr_nodal_set_domain_bc(mgx_amr_nodal h, const int lo[3], const int hi[3]);
+MGX_API int mgx_amr_nodal_set_options(mgx_amr_nodal h, const mgx_mlmg_opts* opts);
+/* sigma per level (:101-121): sigma[lev] is a boxed cell field of level lev (ng ghosts). */
+MGX_API int mgx_amr_nodal_set_sigma(mgx_amr_nodal h, const double* const* sigma, int sigma_ng);
+/* project(rtol, atol) over the levels (:215): vel[lev] (boxed cell, 3 comps,
+ * vel_ng ghosts) is overwritten in place; phi[lev] (boxed nodal) and
+ * gphi[lev] (boxed cell, 3 comps) receive getPhi() / getGradPhi() (:218-219);
+ * phi / gphi or their entries may be NULL.  MGX_MODE_FP64 or MGX_MODE_MIXED. */
+MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_ng,
+                                  double* const* phi, int phi_ng, double* const* gphi,
+                                  int gphi_ng, double rtol, double atol, int mode,
+                                  mgx_solve_info* info);
+
 /* ---- MAC projection (Hydro::MacProjector + MLABecLaplacian/MLPoisson + MLMG) */
 typedef struct mgx_mac_s* mgx_mac;
 /* MacProjector(Vector<Geometry>) (src/incflo.cpp:291); the level's box list

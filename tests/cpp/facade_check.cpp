@@ -44,6 +44,8 @@ struct Dev {
     }
     Dev(mgx::Context& ctx, size_t count) : c(ctx), n(count) {
         c.check(mgx_dev_malloc(c.get(), n * sizeof(double), &p), "malloc");
+        // zero, like a fresh MultiFab set to 0 (phi doubles as the initial guess)
+        c.check(mgx_memset_d(c.get(), p, 0, n * sizeof(double)), "memset");
     }
     ~Dev() { mgx_dev_free(c.get(), p); }
     std::vector<double> get() const {
