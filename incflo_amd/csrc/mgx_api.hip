@@ -329,6 +329,18 @@ MGX_API int mgx_nodal_set_timing(mgx_nodal h, int on) {
     });
 }
 
+MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        h->solver.H64.graphs = on != 0;
+        h->solver.H32.graphs = on != 0;
+        if (!on) {
+            h->solver.H64.drop_graphs();
+            h->solver.H32.drop_graphs();
+        }
+    });
+}
+
 MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long long* count) {
     if (!h) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {

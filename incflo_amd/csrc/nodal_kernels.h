@@ -181,7 +181,7 @@ __device__ __forceinline__ T apply27(const T (&P)[3][NPR][3], const T (&S)[2][NS
     return acc;
 }
 
-template <class T, class TC, int MODE, bool ISO, bool FINE, int NPT>
+template <class T, class TC, int MODE, bool ISO, bool FINE, int NPT, bool MASK = false>
 static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T, TC> a) {
     constexpr int TX = 64, NY = 8, TYT = NY / NPT;   // threads per row, tile rows, thread rows
     constexpr int NT = TX * TYT;
@@ -313,7 +313,7 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
 #pragma unroll
                         for (int r = 0; r < 2; ++r) ss += S[p][n + r][0] + S[p][n + r][1];
                     const long id = i + (long)g.nn[0] * j + sxy * k;
-                    const bool fixed = a.nmask && a.nmask[id] != 1;
+                    const bool fixed = MASK && a.nmask[id] != 1;
                     a.xout[id] = (is_dir(g, i, j, k) || ss == (T)0 || fixed)
                                      ? (T)0 : a.omega * bcur[n] / (-a.c.diag4 * ss);
                     continue;
@@ -323,7 +323,7 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
                                        : apply27<T, ISO, (NPT > 1 ? 1 : 0)>(P, S, a.c, ssum);
                 const T c = P[1][n + 1][1];
                 const long id = i + (long)g.nn[0] * j + sxy * k;
-                const int nt = a.nmask ? a.nmask[id] : 1;
+                const int nt = MASK ? a.nmask[id] : 1;
                 const bool dir = is_dir(g, i, j, k);
                 const T bv = bcur[n];
                 if (MODE == KS_JACOBI) {
@@ -441,6 +441,134 @@ static __global__ void nodal_prolong_add_kernel(KGeom gf, KGeom gc, const T* __r
         acc += xc[I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K)];
     }
     xf[id] += (T)0.125 * acc;
+}
+
+// The same prolongation with one thread per coarse node: the 8 coarse values
+// of the cell above (I, J, K) are loaded once and the 8 fine nodes
+// (2I + a, 2J + b, 2K + c), a, b, c in {0, 1}, are updated (same sums as
+// nodal_prolong_add_kernel, so the results are bitwise equal).
+template <class T>
+static __global__ void __launch_bounds__(256) nodal_prolong_add_coarse_kernel(
+    KGeom gf, KGeom gc, const T* __restrict__ xc, T* __restrict__ xf) {
+    int I, J, K;
+    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    int up[3] = {I + 1, J + 1, K + 1};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (up[d] >= gc.nn[d]) up[d] = gc.per[d] ? up[d] - gc.nn[d] : gc.nn[d] - 1;
+    T c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int II = (q & 1) ? up[0] : I, JJ = (q & 2) ? up[1] : J, KK = (q & 4) ? up[2] : K;
+        c[q] = xc[II + (long)gc.nn[0] * (JJ + (long)gc.nn[1] * KK)];
+    }
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+        const int a = f & 1, b = (f >> 1) & 1, cc = (f >> 2) & 1;
+        const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + cc;
+        if (i >= gf.nn[0] || j >= gf.nn[1] || k >= gf.nn[2] || is_dir(gf, i, j, k)) continue;
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            // corner q of the fine node's interpolation stencil: the upper
+            // coarse node along d only when the fine index is odd there
+            const int sx = (q & 1) & a, sy = ((q >> 1) & 1) & b, sz = ((q >> 2) & 1) & cc;
+            acc += c[sx | (sy << 1) | (sz << 2)];
+        }
+        xf[i + (long)gf.nn[0] * (j + (long)gf.nn[1] * (long)k)] += (T)0.125 * acc;
+    }
+}
+
+// Full-weighting restriction streamed along z with the fine plane staged in
+// LDS: a 64 x 4 workgroup owns a 64 x 4 column of coarse nodes over kz coarse
+// planes; every fine plane of its 129 x 9 footprint is read from HBM once,
+// restricted in-plane from LDS and combined along z (1/4, 1/2, 1/4) in
+// registers.  Same weights as nodal_restrict_kernel; optional first coarse
+// Jacobi sweep from zero (xc1) as there.
+template <class T>
+static __global__ void __launch_bounds__(256) nodal_restrict_tiled_kernel(
+    KGeom gf, KGeom gc, const T* __restrict__ rf, T* __restrict__ rc, const T* __restrict__ sigc,
+    T diag4c, T omegac, T* __restrict__ xc1, int kz, int gx, int gy, int gz) {
+    constexpr int TX = 64, TY = 4, FC = 2 * TX + 1, FR = 2 * TY + 1;
+    __shared__ T tile[FR][FC];
+    const int nb = gx * gy * gz;
+    const int lb = xcd_remap(blockIdx.x, nb);
+    const int bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+    const int I0 = bx * TX, J0 = by * TY, K0 = bz * kz;
+    const int K1 = min(K0 + kz, gc.nn[2]);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int I = I0 + tx, J = J0 + ty;
+    const long sxy = (long)gf.nn[0] * gf.nn[1];
+    // fixed per-thread tile slots: fine (column, row) offsets and validity
+    constexpr int NQ = (FR * FC + 255) / 256;
+    long off[NQ];
+    bool ok[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int e = tid + 256 * q;
+        const int r = e / FC, c = e - (e / FC) * FC;
+        int fi = 2 * I0 - 1 + c, fj = 2 * J0 - 1 + r;
+        bool v = e < FR * FC;
+        if (gf.per[0]) { fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi); }
+        else v = v && fi >= 0 && fi < gf.nn[0];
+        if (gf.per[1]) { fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj); }
+        else v = v && fj >= 0 && fj < gf.nn[1];
+        fi = min(max(fi, 0), gf.nn[0] - 1);
+        fj = min(max(fj, 0), gf.nn[1] - 1);
+        off[q] = fi + (long)gf.nn[0] * fj;
+        ok[q] = v;
+    }
+    T acc = 0;
+    for (int p = 2 * K0 - 1; p <= 2 * K1 - 1; ++p) {
+        int fk = p;
+        bool zv = true;
+        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
+        else zv = fk >= 0 && fk < gf.nn[2];
+        fk = min(max(fk, 0), gf.nn[2] - 1);
+        T v[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) v[q] = (ok[q] && zv) ? rf[off[q] + sxy * fk] : (T)0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q;
+            if (e < FR * FC) tile[e / FC][e - (e / FC) * FC] = v[q];
+        }
+        __syncthreads();
+        T s2 = 0;
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy) {
+            const T wy = oy == 0 ? (T)0.5 : (T)0.25;
+#pragma unroll
+            for (int ox = -1; ox <= 1; ++ox) {
+                const T wx = ox == 0 ? (T)0.5 : (T)0.25;
+                s2 += wx * wy * tile[2 * ty + 1 + oy][2 * tx + 1 + ox];
+            }
+        }
+        __syncthreads();
+        if ((p & 1) == 0) {
+            acc += (T)0.5 * s2;
+            continue;
+        }
+        // odd plane p = 2K + 1 closes coarse plane K = (p - 1) / 2 and opens K + 1
+        if (p > 2 * K0 - 1) {
+            acc += (T)0.25 * s2;
+            const int K = (p - 1) >> 1;
+            if (I < gc.nn[0] && J < gc.nn[1]) {
+                const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
+                const bool dir = is_dir(gc, I, J, K);
+                rc[id] = dir ? (T)0 : acc;
+                if (xc1) {
+                    T ss = 0;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        ss += sigc[(I + (q & 1)) + (long)gc.sg[0] * ((J + ((q >> 1) & 1)) +
+                                                                   (long)gc.sg[1] * (K + ((q >> 2) & 1)))];
+                    xc1[id] = (dir || ss == (T)0) ? (T)0 : omegac * acc / (-diag4c * ss);
+                }
+            }
+        }
+        acc = (T)0.25 * s2;
+    }
 }
 
 // Write a valid cell's value into a ghosted cell array and its periodic images.

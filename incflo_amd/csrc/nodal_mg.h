@@ -61,6 +61,7 @@ struct LaunchTimer {
     long long count = 0;
     void begin(hipStream_t s);
     void end(hipStream_t s);
+    void reserve(size_t pairs);  // events for `pairs` more launches (before a capture)
     void collect();  // after the stream has been synchronised
     void reset() { total_ms = 0.0; count = 0; used = 0; }
     ~LaunchTimer();
@@ -84,6 +85,35 @@ class NodalHierarchy {
     // set by the outer iteration when its residual pass also wrote L[0].x
     bool x0_ready = false;
     bool built = false;
+    // Replay the V-cycle as a HIP graph: the launch sequence is the same every
+    // cycle once each level starts from its own pair of buffers, so it is
+    // captured once per (level-0 buffers, options) and replayed with one
+    // launch, which removes the host launch gaps between the small coarse-level
+    // kernels.  Off: plain stream launches (identical kernels and results).
+    bool graphs = true;
+    ~NodalHierarchy() {
+        drop_graphs();
+        if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+    }
+    void drop_graphs();
+
+   private:
+    struct CycleGraph {
+        T* x0 = nullptr;
+        const T* b0 = nullptr;
+        bool ready0 = false;
+        bool timed = false;
+        int* bottom_iters = nullptr;
+        MGOpts o;
+        hipGraphExec_t exec = nullptr;
+        std::vector<T*> xs, x2s;  // buffer roles after the cycle
+    };
+    std::vector<CycleGraph> cache_;
+    // capture happens on a private stream (the caller's may be the null
+    // stream, which cannot be captured); the graph is then launched on the
+    // caller's stream
+    hipStream_t cap_stream_ = nullptr;
+    void vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_iters);
 };
 
 class EBNodal;

@@ -49,6 +49,14 @@ static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s
     a.gz = (g.nn[2] + kz - 1) / kz;
     const unsigned nb = (unsigned)(gxy * a.gz);
     dim3 blk(TX, NY / NPT);
+    if (a.nmask) {
+        // composite-level sweeps (node types); kept out of the plain kernels
+        if (g.iso())
+            nodal_stencil_kernel<T, TC, MODE, true, false, NPT, true><<<nb, blk, 0, s>>>(a);
+        else
+            nodal_stencil_kernel<T, TC, MODE, false, false, NPT, true><<<nb, blk, 0, s>>>(a);
+        return;
+    }
     if (g.iso()) {
         if (fine)
             nodal_stencil_kernel<T, TC, MODE, true, true, NPT><<<nb, blk, 0, s>>>(a);
@@ -67,6 +75,20 @@ void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine
         launch_stencil_npt<T, TC, MODE, 2>(g, a, s, fine);
 }
 
+// Full-weighting restriction onto coarse level gc (z-streamed, LDS-staged);
+// xc1 (optional) receives the coarse first Jacobi sweep from zero.
+template <class T>
+static void launch_restrict(const KGeom& kf, const NodalGeom& gc, const KGeom& kc, const T* rf, T* rc,
+                            const T* sigc, T diag4c, T omegac, T* xc1, hipStream_t s) {
+    const int gx = (gc.nn[0] + 63) / 64, gy = (gc.nn[1] + 3) / 4;
+    const long gxy = (long)gx * gy;
+    int kz = (int)std::min<long>(64, std::max<long>(1, (gc.nn[2] * gxy) / 2048));
+    kz = std::min(kz, gc.nn[2]);
+    const int gz = (gc.nn[2] + kz - 1) / kz;
+    nodal_restrict_tiled_kernel<T><<<(unsigned)(gxy * gz), dim3(64, 4), 0, s>>>(
+        kf, kc, rf, rc, sigc, diag4c, omegac, xc1, kz, gx, gy, gz);
+}
+
 void LaunchTimer::begin(hipStream_t s) {
     if (!on) return;
     while (ev.size() < used + 2) {
@@ -81,6 +103,14 @@ void LaunchTimer::end(hipStream_t s) {
     if (!on) return;
     MGX_HIP(hipEventRecord(ev[used + 1], s));
     used += 2;
+}
+
+void LaunchTimer::reserve(size_t pairs) {
+    while (ev.size() < used + 2 * pairs) {
+        hipEvent_t e;
+        MGX_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+    }
 }
 
 void LaunchTimer::collect() {
@@ -99,7 +129,24 @@ LaunchTimer::~LaunchTimer() {
 }
 
 template <class T>
+void NodalHierarchy<T>::drop_graphs() {
+    for (auto& c : cache_)
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    cache_.clear();
+}
+
+template <class T>
 void NodalHierarchy<T>::build(const NodalGeom& g0, int nlev) {
+    drop_graphs();
+    static bool attr_set = false;
+    if (!attr_set) {
+        // the coarse-level kernel may use up to 150 KiB of LDS (advisory on
+        // AMD GPUs, which grant up to the CU's LDS without it); set outside
+        // any stream capture
+        (void)hipFuncSetAttribute((const void*)coarse_vcycle_kernel<T>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
+        attr_set = true;
+    }
     L.clear();
     L.resize(nlev);
     for (int l = 0; l < nlev; ++l) {
@@ -193,8 +240,104 @@ static long coarse_threshold() {
     return v;
 }
 
+static bool same_opts(const MGOpts& a, const MGOpts& b) {
+    return a.nu1 == b.nu1 && a.nu2 == b.nu2 && a.omega == b.omega &&
+           a.bottom_maxiter == b.bottom_maxiter && a.bottom_rtol == b.bottom_rtol;
+}
+
 template <class T>
 void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) {
+    // Levels below 0 are recomputed from scratch every cycle: start each from
+    // its own buffer pair, so every cycle launches the same sequence.
+    for (size_t l = 1; l < L.size(); ++l) {
+        L[l].x = L[l].x_buf.template as<T>();
+        L[l].x2 = L[l].x2_buf.template as<T>();
+    }
+    if (!x0_ready) {
+        L[0].x = L[0].x_buf.template as<T>();
+        L[0].x2 = L[0].x2_buf.template as<T>();
+    }
+    // Timed cycles (HIP events around the finest-level sweeps) run with plain
+    // launches: events recorded inside a graph cannot be timed.
+    const bool timed = timer && timer->on;
+    if (!graphs || timed) {
+        vcycle_body(o, s, bottom_iters);
+        return;
+    }
+    const bool ready0 = x0_ready && o.nu1 > 0;
+    CycleGraph* hit = nullptr;
+    for (auto& c : cache_)
+        if (c.x0 == L[0].x && c.b0 == L[0].b && c.ready0 == ready0 && c.timed == timed &&
+            c.bottom_iters == bottom_iters && same_opts(c.o, o))
+            hit = &c;
+    if (!hit) {
+        CycleGraph c;
+        c.x0 = L[0].x;
+        c.b0 = L[0].b;
+        c.ready0 = ready0;
+        c.timed = timed;
+        c.bottom_iters = bottom_iters;
+        c.o = o;
+        const bool saved_ready = x0_ready;
+        if (!cap_stream_ && hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            cap_stream_ = nullptr;
+            graphs = false;
+            vcycle_body(o, s, bottom_iters);
+            return;
+        }
+        hipStream_t cs = cap_stream_;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            graphs = false;
+            vcycle_body(o, s, bottom_iters);
+            return;
+        }
+        bool body_ok = true;
+        try {
+            vcycle_body(o, cs, bottom_iters);
+        } catch (...) {
+            body_ok = false;  // e.g. a call the runtime does not capture
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (e == hipSuccess && body_ok) e = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess || !body_ok) {
+            // capture not available here: run the cycle with plain launches
+            if (o.verbose)
+                fprintf(stderr, "mgx: V-cycle graph capture failed (%s); plain launches\n",
+                        hipGetErrorString(e));
+            (void)hipGetLastError();
+            graphs = false;
+            x0_ready = saved_ready;
+            for (size_t l = 1; l < L.size(); ++l) {
+                L[l].x = L[l].x_buf.template as<T>();
+                L[l].x2 = L[l].x2_buf.template as<T>();
+            }
+            L[0].x = c.x0;
+            L[0].x2 = (c.x0 == L[0].x_buf.template as<T>()) ? L[0].x2_buf.template as<T>()
+                                                            : L[0].x_buf.template as<T>();
+            vcycle_body(o, s, bottom_iters);
+            return;
+        }
+        for (auto& lv : L) {
+            c.xs.push_back(lv.x);
+            c.x2s.push_back(lv.x2);
+        }
+        cache_.push_back(std::move(c));
+        hit = &cache_.back();
+    }
+    MGX_HIP(hipGraphLaunch(hit->exec, s));
+    for (size_t l = 0; l < L.size(); ++l) {
+        L[l].x = hit->xs[l];
+        L[l].x2 = hit->x2s[l];
+    }
+    x0_ready = false;
+}
+
+template <class T>
+void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_iters) {
     const int nl = (int)L.size();
     // levels lc..nl-1 run in the LDS-resident coarse kernel (when they fit)
     constexpr size_t kLdsBudget = 150 * 1024;
@@ -226,9 +369,8 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         // from zero out of the restriction
         MGLevel<T>& lc1 = L[l + 1];
         ready = (l + 1 < lc) && o.nu1 > 0;
-        nodal_restrict_kernel<T><<<grid3(lc1.g.nn[0], lc1.g.nn[1], lc1.g.nn[2]), block3(), 0, s>>>(
-            lv.kg, lc1.kg, lv.r, lc1.b, lc1.sig, lc1.coef.diag4, (T)nodal_omega(lc1.g, o.omega),
-            ready ? lc1.x : nullptr);
+        launch_restrict<T>(lv.kg, lc1.g, lc1.kg, lv.r, lc1.b, lc1.sig, lc1.coef.diag4,
+                           (T)nodal_omega(lc1.g, o.omega), ready ? lc1.x : nullptr, s);
     }
     if (!bottom_in_lds) {
         MGLevel<T>& lb = L[nl - 1];
@@ -263,19 +405,14 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         ca.singular = lb.g.singular() ? 1 : 0;
         ca.iters_acc = bottom_iters;
         const int threads = nmax >= 512 ? 512 : (nmax >= 256 ? 256 : 64);
-        static bool attr_set = false;
-        if (!attr_set) {
-            // (AMD GPUs grant up to the CU's LDS without it; the call is advisory)
-            (void)hipFuncSetAttribute((const void*)coarse_vcycle_kernel<T>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
-            attr_set = true;
-        }
         coarse_vcycle_kernel<T><<<1, threads, lds_for(lc), s>>>(ca);
     }
     for (int l = lc - 1; l >= 0; --l) {
         MGLevel<T>& lv = L[l];
         const long nf = lv.g.nnode();
-        nodal_prolong_add_kernel<T><<<grid3(lv.g.nn[0], lv.g.nn[1], lv.g.nn[2]), block3(), 0, s>>>(lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);
+        const NodalGeom& gcl = L[l + 1].g;
+        nodal_prolong_add_coarse_kernel<T><<<grid3(gcl.nn[0], gcl.nn[1], gcl.nn[2]), block3(), 0, s>>>(
+            lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);
         smooth(l, o.nu2, false, o, s);
     }
 }

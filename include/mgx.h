@@ -141,6 +141,11 @@ MGX_API int mgx_nodal_set_options(mgx_nodal h, const mgx_mlmg_opts* opts);
  * get_timing(which=0) returns the summed milliseconds and the launch count. */
 MGX_API int mgx_nodal_set_timing(mgx_nodal h, int on);
 MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long long* count);
+/* Launch mode (no reference counterpart; for callers that must not use HIP
+ * graphs, and for tests): on (default), each V-cycle is captured once as a
+ * HIP graph and replayed; off, its kernels are launched one by one.  The
+ * kernels and results are the same either way. */
+MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on);
 /* Variable sigma, a boxed cell field with ng ghosts (:191, sigma = s/rho at :117). */
 MGX_API int mgx_nodal_set_sigma(mgx_nodal h, const double* sigma, int sigma_ng);
 /* Scalar sigma = s/rho_0 (:186-188). */
