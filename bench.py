@@ -115,6 +115,8 @@ def main():
                     help="multigrid level the network acts on (default: the level whose size "
                          "matches the training level's)")
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--vcycle-bits", type=int, default=32, choices=(16, 32),
+                    help="storage of the mixed forms' finest V-cycle level")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cache", action="store_true",
@@ -143,6 +145,7 @@ def main():
     bxs = B.chop_domain((n, n, n), args.max_grid_size)
     ctx = mgx.Context(0)
     P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))
+    P.set_vcycle_bits(args.vcycle_bits)
     vel0 = ctx.array(B.to_boxed(ustar, bxs, 1))
     vel = ctx.array(vel0.n)
     rhod = ctx.array(B.to_boxed(rho, bxs, 0))
@@ -176,7 +179,9 @@ def main():
         mode = modes[name]
         log(f"bench: mode {name}: warmup {args.warmup}, steps {args.steps}")
         t, infos, ms, cnt = run_mode(ctx, P, bufs, dt, mode, args.steps, args.warmup, args.rtol)
-        word = 8 if mode == mgx.MODE_FP64 else 4
+        # stored word of the finest-level V-cycle arrays (x, b, sigma): FP64,
+        # or FP16 in the mixed forms (FP32 with --vcycle-bits 32)
+        word = 8 if mode == mgx.MODE_FP64 else args.vcycle_bits // 8
         avg_ms = ms / max(cnt, 1)
         smoother_bytes = 4 * word * nnodes
         results[name] = {
@@ -220,15 +225,16 @@ def main():
     # weights the mixed form stands in (named in config.mode)
     head = next((k for k in ("learned", "mixed", "fp64") if k in conv), list(results)[0])
     hr = results[head]
-    word = 8 if head == "fp64" else 4
+    word = 8 if head == "fp64" else args.vcycle_bits // 8
+    wname = {8: "FP64", 4: "FP32", 2: "FP16-stored, FP32 arithmetic"}[word]
     roofline = None
     if hr["fine_smoother_launches"]:
         achieved = hr["fine_smoother_GBps"]
         roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": achieved / PEAK_HBM_GBS,
-                    "traffic": measured_traffic("fine_jacobi_fp64" if word == 8 else
-                                                "fine_jacobi_fp32", n),
-                    "kernel": f"finest-level weighted-Jacobi sweep ({'FP64' if word == 8 else 'FP32'})",
+                    "traffic": measured_traffic({8: "fine_jacobi_fp64", 4: "fine_jacobi_fp32",
+                                                 2: "fine_jacobi_fp16"}[word], n),
+                    "kernel": f"finest-level weighted-Jacobi sweep ({wname})",
                     "bytes_per_launch": 4 * word * nnodes}
     cpu = None
     if not args.no_cpu:

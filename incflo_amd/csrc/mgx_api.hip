@@ -341,6 +341,14 @@ MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on) {
     });
 }
 
+MGX_API int mgx_nodal_set_vcycle_bits(mgx_nodal h, int bits) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(bits == 16 || bits == 32, MGX_ERR_ARG, "V-cycle storage must be 16 or 32 bits");
+        h->solver.use_half = bits == 16;
+    });
+}
+
 MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long long* count) {
     if (!h) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {

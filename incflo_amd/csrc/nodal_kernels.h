@@ -15,9 +15,30 @@
 // holding the periodic image or 0.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace mgx {
+
+// Arithmetic type of a storage type: FP16-stored V-cycle arrays are computed
+// in FP32; FP32 and FP64 compute in their own type.
+template <class T> struct AccOf { using type = T; };
+template <> struct AccOf<__half> { using type = float; };
+template <class T> using acc_t = typename AccOf<T>::type;
+
+// load / store through the arithmetic type (FP16 conversions round to nearest even)
+template <class TA, class T>
+__device__ __forceinline__ TA ld(const T& v) { return (TA)v; }
+template <class TA>
+__device__ __forceinline__ TA ld(const __half& v) { return (TA)__half2float(v); }
+template <class T, class TA>
+__device__ __forceinline__ T st(TA v) { return (T)v; }
+template <>
+__device__ __forceinline__ __half st<__half, float>(float v) { return __float2half(v); }
+template <>
+__device__ __forceinline__ __half st<__half, double>(double v) { return __float2half((float)v); }
 
 struct KGeom {
     int nn[3];
@@ -112,14 +133,14 @@ enum { KS_JACOBI = 0, KS_RESID = 1, KS_UPDRES = 2, KS_APPLY = 3, KS_JFIRST = 4 }
 template <class T, class TC>
 struct SArgs {
     KGeom g;
-    Coef<T> c;
+    Coef<acc_t<T>> c;
     const T* x;      // iterate (JACOBI, RESID) or outer phi (UPDRES)
     const TC* cor;   // UPDRES: correction added to x before the stencil
     const T* b;
     const T* sig;    // sigma with ghosts
     T* xout;         // JACOBI: new iterate; UPDRES: x + cor
     TC* rout;        // RESID/UPDRES: residual
-    T omega;
+    acc_t<T> omega;
     int kz;          // node planes per workgroup
     int gx, gy, gz;  // tile grid
     unsigned long long* norm;  // optional max |r|
@@ -129,6 +150,9 @@ struct SArgs {
     // keeps the value), 0 outside (JACOBI / JFIRST write 0); RESID writes
     // r = 0 on every node that is not an unknown
     const unsigned char* nmask;
+    // scaled reduced-precision V-cycle (0 means 1): UPDRES adds cor_scale*cor;
+    // RESID/UPDRES store rscale*r and x1 from x1scale*r
+    double cor_scale, rscale, x1scale;
 };
 
 // One 27-point pass over a 64 x 8 node tile, streaming along z.  A workgroup
@@ -142,54 +166,64 @@ struct SArgs {
 // loads are unconditional (clamped indices) so the wait counts stay exact.
 // FINE only separates the finest-level instantiation (a distinct kernel name
 // in profiles); the code is identical.
+// reciprocal for the smoother's diagonal scaling: the hardware reciprocal
+// (1 ulp) in FP32, a division in FP64
+template <class TA>
+__device__ __forceinline__ TA diag_rcp(TA x) { return (TA)1 / x; }
+template <>
+__device__ __forceinline__ float diag_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <class T, bool ISO, int RO, int NPR, int NSR>
-__device__ __forceinline__ T apply27(const T (&P)[3][NPR][3], const T (&S)[2][NSR][2],
-                                     const Coef<T>& cf, T& ssum) {
-    const T c = P[1][RO + 1][1];
+__device__ __forceinline__ T apply27(const T (&P0)[NPR][3], const T (&P1)[NPR][3],
+                                     const T (&P2)[NPR][3], const T (&S0)[NSR][2],
+                                     const T (&S1)[NSR][2], const Coef<T>& cf, T& ssum) {
+    // planes: P0 below, P1 centre, P2 above; cell planes S0 below, S1 above
+    const T c = P1[RO + 1][1];
     T acc = 0;
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
         for (int aa = 0; aa < 2; ++aa) {
-            const T lo = S[0][RO + bb][aa], hi = S[1][RO + bb][aa];
+            const T lo = S0[RO + bb][aa], hi = S1[RO + bb][aa];
             // corners (one cell each)
-            acc += cf.corner * (lo * (P[0][RO + 2 * bb][2 * aa] - c) + hi * (P[2][RO + 2 * bb][2 * aa] - c));
+            acc += cf.corner * (lo * (P0[RO + 2 * bb][2 * aa] - c) + hi * (P2[RO + 2 * bb][2 * aa] - c));
             // xy edges (dz = 0): the cell below and above
-            acc += cf.edge[2] * (lo + hi) * (P[1][RO + 2 * bb][2 * aa] - c);
+            acc += cf.edge[2] * (lo + hi) * (P1[RO + 2 * bb][2 * aa] - c);
         }
 #pragma unroll
     for (int aa = 0; aa < 2; ++aa) {
         // xz edges (dy = 0) and yz edges (dx = 0)
-        acc += cf.edge[1] * ((S[0][RO][aa] + S[0][RO + 1][aa]) * (P[0][RO + 1][2 * aa] - c) +
-                             (S[1][RO][aa] + S[1][RO + 1][aa]) * (P[2][RO + 1][2 * aa] - c));
-        acc += cf.edge[0] * ((S[0][RO + aa][0] + S[0][RO + aa][1]) * (P[0][RO + 2 * aa][1] - c) +
-                             (S[1][RO + aa][0] + S[1][RO + aa][1]) * (P[2][RO + 2 * aa][1] - c));
+        acc += cf.edge[1] * ((S0[RO][aa] + S0[RO + 1][aa]) * (P0[RO + 1][2 * aa] - c) +
+                             (S1[RO][aa] + S1[RO + 1][aa]) * (P2[RO + 1][2 * aa] - c));
+        acc += cf.edge[0] * ((S0[RO + aa][0] + S0[RO + aa][1]) * (P0[RO + 2 * aa][1] - c) +
+                             (S1[RO + aa][0] + S1[RO + aa][1]) * (P2[RO + 2 * aa][1] - c));
     }
-    const T s0 = S[0][RO][0] + S[0][RO][1] + S[0][RO + 1][0] + S[0][RO + 1][1];
-    const T s1 = S[1][RO][0] + S[1][RO][1] + S[1][RO + 1][0] + S[1][RO + 1][1];
+    const T s0 = S0[RO][0] + S0[RO][1] + S0[RO + 1][0] + S0[RO + 1][1];
+    const T s1 = S1[RO][0] + S1[RO][1] + S1[RO + 1][0] + S1[RO + 1][1];
     if (!ISO) {
 #pragma unroll
         for (int aa = 0; aa < 2; ++aa) {
-            acc += cf.face[0] * (S[0][RO][aa] + S[0][RO + 1][aa] + S[1][RO][aa] + S[1][RO + 1][aa]) *
-                   (P[1][RO + 1][2 * aa] - c);
-            acc += cf.face[1] * (S[0][RO + aa][0] + S[0][RO + aa][1] + S[1][RO + aa][0] + S[1][RO + aa][1]) *
-                   (P[1][RO + 2 * aa][1] - c);
+            acc += cf.face[0] * (S0[RO][aa] + S0[RO + 1][aa] + S1[RO][aa] + S1[RO + 1][aa]) *
+                   (P1[RO + 1][2 * aa] - c);
+            acc += cf.face[1] * (S0[RO + aa][0] + S0[RO + aa][1] + S1[RO + aa][0] + S1[RO + aa][1]) *
+                   (P1[RO + 2 * aa][1] - c);
         }
-        acc += cf.face[2] * (s0 * (P[0][RO + 1][1] - c) + s1 * (P[2][RO + 1][1] - c));
+        acc += cf.face[2] * (s0 * (P0[RO + 1][1] - c) + s1 * (P2[RO + 1][1] - c));
     }
     ssum = s0 + s1;
     return acc;
 }
 
 template <class T, class TC, int MODE, bool ISO, bool FINE, int NPT, bool MASK = false>
-static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T, TC> a) {
-    constexpr int TX = 64, NY = 8, TYT = NY / NPT;   // threads per row, tile rows, thread rows
+static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_kernel(SArgs<T, TC> a) {
+    using TA = acc_t<T>;  // arithmetic type (T is the storage type of x, b, sigma)
+    constexpr int TX = 64, NY = (NPT == 1 ? 8 : 4 * NPT), TYT = NY / NPT;  // threads per row, tile rows, thread rows
     constexpr int NT = TX * TYT;
     constexpr int PR = NY + 2, PC = TX + 2;          // phi tile incl. halo
     constexpr int SR = NY + 1, SC = TX + 1;          // sigma tile
     constexpr int NQP = (PR + TYT - 1) / TYT, NQS = (SR + TYT - 1) / TYT;
-    __shared__ T sphi[2][PR][PC];
-    __shared__ T ssig[2][SR][SC];
+    __shared__ TA sphi[2][PR][PC];
+    __shared__ TA ssig[2][SR][SC];
     __shared__ double red[NT / 64];
     const KGeom& g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
@@ -223,7 +257,10 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
     for (int n = 0; n < NPT; ++n)
         bid[n] = (long)min(i, g.nn[0] - 1) + (long)g.nn[0] * min(jr + n, g.nn[1] - 1);
 
-    T lp[NQP], lh, ls[NQS], lsx, lb_[NPT];
+    const double cs = (MODE == KS_UPDRES && a.cor_scale != 0.0) ? a.cor_scale : 1.0;
+    const double rsc = (a.rscale != 0.0) ? a.rscale : 1.0;
+    const double x1s = (a.x1scale != 0.0) ? a.x1scale : 1.0;
+    TA lp[NQP], lh, ls[NQS], lsx, lb_[NPT];
 #pragma unroll
     for (int n = 0; n < NPT; ++n) lb_[n] = 0;
     auto issue = [&](int kk, int kb) {
@@ -233,26 +270,26 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
 #pragma unroll
             for (int q = 0; q < NQP; ++q) {
                 const long id = gi + (long)g.nn[0] * prow[q] + kw;
-                T v = a.x[id];
-                if (MODE == KS_UPDRES) v += (T)a.cor[id];
+                TA v = ld<TA>(a.x[id]);
+                if (MODE == KS_UPDRES) v += (TA)(cs * (double)ld<acc_t<TC>>(a.cor[id]));
                 lp[q] = v;
             }
             const long id = hcol + (long)g.nn[0] * hrow + kw;
-            T v = a.x[id];
-            if (MODE == KS_UPDRES) v += (T)a.cor[id];
+            TA v = ld<TA>(a.x[id]);
+            if (MODE == KS_UPDRES) v += (TA)(cs * (double)ld<acc_t<TC>>(a.cor[id]));
             lh = v;
         }
 #pragma unroll
-        for (int q = 0; q < NQS; ++q) ls[q] = a.sig[si + (long)g.sg[0] * srow[q] + ks];
-        lsx = a.sig[sxc + (long)g.sg[0] * sxr + ks];
+        for (int q = 0; q < NQS; ++q) ls[q] = ld<TA>(a.sig[si + (long)g.sg[0] * srow[q] + ks]);
+        lsx = ld<TA>(a.sig[sxc + (long)g.sg[0] * sxr + ks]);
         if (MODE != KS_APPLY) {
             const long kbo = (long)min(max(kb, 0), g.nn[2] - 1) * sxy;
 #pragma unroll
-            for (int n = 0; n < NPT; ++n) lb_[n] = a.b[bid[n] + kbo];
+            for (int n = 0; n < NPT; ++n) lb_[n] = ld<TA>(a.b[bid[n] + kbo]);
         }
     };
 
-    T P[3][NPT + 2][3], S[2][NPT + 1][2];
+    TA P[3][NPT + 2][3], S[2][NPT + 1][2];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -267,8 +304,13 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
             for (int c = 0; c < 2; ++c) S[p][r][c] = 0;
     double rmax = 0.0;
 
-    issue(k0 - 1, k0 - 1);
-    for (int kk = k0 - 1; kk <= k1; ++kk) {
+    // One step per z-plane; the register planes rotate (P: 3, S: 2), so the
+    // loop is unrolled by 6 and every step addresses its planes by constant
+    // indices instead of copying them down.
+    auto step = [&](auto JJ, int kk) {
+        constexpr int J = decltype(JJ)::value;
+        constexpr int PN = J % 3, PM = (J + 2) % 3, PB = (J + 1) % 3;  // above, centre, below
+        constexpr int SN = J % 2, SB = (J + 1) % 2;                     // cell planes above, below
         const int buf = kk & 1;
         if (MODE != KS_JFIRST) {
 #pragma unroll
@@ -280,7 +322,7 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
         for (int q = 0; q < NQS; ++q)
             if (ty + TYT * q < SR) ssig[buf][ty + TYT * q][tx] = ls[q];
         if (sxst) ssig[buf][min(tid, SR - 1)][SC - 1] = lsx;
-        T bcur[NPT];  // rhs of plane kk-1
+        TA bcur[NPT];  // rhs of plane kk-1
 #pragma unroll
         for (int n = 0; n < NPT; ++n) bcur[n] = lb_[n];
         if (kk < k1) issue(kk + 1, kk);
@@ -288,18 +330,12 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
 #pragma unroll
         for (int r = 0; r < NPT + 2; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                P[0][r][c] = P[1][r][c];
-                P[1][r][c] = P[2][r][c];
-                P[2][r][c] = (MODE == KS_JFIRST) ? (T)0 : sphi[buf][NPT * ty + r][tx + c];
-            }
+            for (int c = 0; c < 3; ++c)
+                P[PN][r][c] = (MODE == KS_JFIRST) ? (TA)0 : sphi[buf][NPT * ty + r][tx + c];
 #pragma unroll
         for (int r = 0; r < NPT + 1; ++r)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                S[0][r][c] = S[1][r][c];
-                S[1][r][c] = ssig[buf][NPT * ty + r][tx + c];
-            }
+            for (int c = 0; c < 2; ++c) S[SN][r][c] = ssig[buf][NPT * ty + r][tx + c];
         if (kk >= k0 + 1) {
             const int k = kk - 1;
 #pragma unroll
@@ -307,44 +343,80 @@ static __global__ void __launch_bounds__(512 / NPT) nodal_stencil_kernel(SArgs<T
                 const int j = jr + n;
                 if (!(i < g.nn[0] && j < g.nn[1])) continue;
                 if (MODE == KS_JFIRST) {
-                    T ss = 0;
+                    TA ss = 0;
 #pragma unroll
-                    for (int p = 0; p < 2; ++p)
-#pragma unroll
-                        for (int r = 0; r < 2; ++r) ss += S[p][n + r][0] + S[p][n + r][1];
+                    for (int r = 0; r < 2; ++r)
+                        ss += S[SB][n + r][0] + S[SB][n + r][1] + S[SN][n + r][0] + S[SN][n + r][1];
                     const long id = i + (long)g.nn[0] * j + sxy * k;
                     const bool fixed = MASK && a.nmask[id] != 1;
-                    a.xout[id] = (is_dir(g, i, j, k) || ss == (T)0 || fixed)
-                                     ? (T)0 : a.omega * bcur[n] / (-a.c.diag4 * ss);
+                    a.xout[id] = st<T>((is_dir(g, i, j, k) || ss == (TA)0 || fixed)
+                                           ? (TA)0 : a.omega * bcur[n] * diag_rcp(-a.c.diag4 * ss));
                     continue;
                 }
-                T ssum;
-                const T acc = (n == 0) ? apply27<T, ISO, 0>(P, S, a.c, ssum)
-                                       : apply27<T, ISO, (NPT > 1 ? 1 : 0)>(P, S, a.c, ssum);
-                const T c = P[1][n + 1][1];
+                TA ssum;
+                // row offset n as a template argument (n is a constant after unrolling)
+                const TA acc =
+                    (n == 0) ? apply27<TA, ISO, 0>(P[PB], P[PM], P[PN], S[SB], S[SN], a.c, ssum)
+                    : (n == 1) ? apply27<TA, ISO, (NPT > 1 ? 1 : 0)>(P[PB], P[PM], P[PN], S[SB], S[SN], a.c, ssum)
+                    : (n == 2) ? apply27<TA, ISO, (NPT > 2 ? 2 : 0)>(P[PB], P[PM], P[PN], S[SB], S[SN], a.c, ssum)
+                               : apply27<TA, ISO, (NPT > 3 ? 3 : 0)>(P[PB], P[PM], P[PN], S[SB], S[SN], a.c, ssum);
+                const TA c = P[PM][n + 1][1];
                 const long id = i + (long)g.nn[0] * j + sxy * k;
                 const int nt = MASK ? a.nmask[id] : 1;
                 const bool dir = is_dir(g, i, j, k);
-                const T bv = bcur[n];
+                const TA bv = bcur[n];
                 if (MODE == KS_JACOBI) {
-                    T v = c;
+                    TA v = c;
                     if (dir || nt == 0) v = 0;
-                    else if (nt == 1 && ssum != (T)0) v = c + a.omega * (bv - acc) / (-a.c.diag4 * ssum);
-                    a.xout[id] = v;
+                    else if (nt == 1 && ssum != (TA)0)
+                        v = c + (bv - acc) * (a.omega * diag_rcp(-a.c.diag4 * ssum));
+                    a.xout[id] = st<T>(v);
                 } else if (MODE == KS_APPLY) {
-                    a.rout[id] = (TC)acc;
+                    a.rout[id] = st<TC>(acc);
                 } else {
-                    const T r = (dir || nt != 1) ? (T)0 : (bv - acc);
-                    a.rout[id] = (TC)r;
-                    if (MODE == KS_UPDRES) a.xout[id] = dir ? (T)0 : c;
+                    const TA r = (dir || nt != 1) ? (TA)0 : (bv - acc);
+                    a.rout[id] = st<TC>((acc_t<TC>)(rsc * (double)r));
+                    if (MODE == KS_UPDRES) a.xout[id] = st<T>(dir ? (TA)0 : c);
                     if (a.x1out)
-                        a.x1out[id] = (dir || ssum == (T)0)
-                                          ? (TC)0
-                                          : (TC)(a.omega1 * (double)r / (-(double)a.c.diag4 * (double)ssum));
+                        a.x1out[id] = st<TC>((acc_t<TC>)((dir || ssum == (TA)0)
+                                              ? 0.0
+                                              : a.omega1 * x1s * (double)r / (-(double)a.c.diag4 * (double)ssum)));
                     rmax = fmax(rmax, (double)fabs(r));
                 }
             }
         }
+    };
+    issue(k0 - 1, k0 - 1);
+    if (sizeof(TA) == 4) {
+        // FP32 arithmetic: a plain loop with the planes copied down measures
+        // faster (the 6-fold unrolled body costs more than the copies it saves)
+        for (int kk = k0 - 1; kk <= k1; ++kk) {
+#pragma unroll
+            for (int r = 0; r < NPT + 2; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    P[0][r][c] = P[1][r][c];
+                    P[1][r][c] = P[2][r][c];
+                }
+#pragma unroll
+            for (int r = 0; r < NPT + 1; ++r)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) S[1][r][c] = S[0][r][c];
+            step(std::integral_constant<int, 2>{}, kk);
+        }
+    } else
+    for (int kk = k0 - 1; kk <= k1; kk += 6) {
+        step(std::integral_constant<int, 0>{}, kk);
+        if (kk + 1 > k1) break;
+        step(std::integral_constant<int, 1>{}, kk + 1);
+        if (kk + 2 > k1) break;
+        step(std::integral_constant<int, 2>{}, kk + 2);
+        if (kk + 3 > k1) break;
+        step(std::integral_constant<int, 3>{}, kk + 3);
+        if (kk + 4 > k1) break;
+        step(std::integral_constant<int, 4>{}, kk + 4);
+        if (kk + 5 > k1) break;
+        step(std::integral_constant<int, 5>{}, kk + 5);
     }
     if ((MODE == KS_RESID || MODE == KS_UPDRES) && a.norm) {
         const double m = block_max<NT>(rmax, red);
@@ -447,9 +519,9 @@ static __global__ void nodal_prolong_add_kernel(KGeom gf, KGeom gc, const T* __r
 // of the cell above (I, J, K) are loaded once and the 8 fine nodes
 // (2I + a, 2J + b, 2K + c), a, b, c in {0, 1}, are updated (same sums as
 // nodal_prolong_add_kernel, so the results are bitwise equal).
-template <class T>
+template <class T, class TF = T>
 static __global__ void __launch_bounds__(256) nodal_prolong_add_coarse_kernel(
-    KGeom gf, KGeom gc, const T* __restrict__ xc, T* __restrict__ xf) {
+    KGeom gf, KGeom gc, const T* __restrict__ xc, TF* __restrict__ xf) {
     int I, J, K;
     if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
     int up[3] = {I + 1, J + 1, K + 1};
@@ -475,7 +547,8 @@ static __global__ void __launch_bounds__(256) nodal_prolong_add_coarse_kernel(
             const int sx = (q & 1) & a, sy = ((q >> 1) & 1) & b, sz = ((q >> 2) & 1) & cc;
             acc += c[sx | (sy << 1) | (sz << 2)];
         }
-        xf[i + (long)gf.nn[0] * (j + (long)gf.nn[1] * (long)k)] += (T)0.125 * acc;
+        TF& dst = xf[i + (long)gf.nn[0] * (j + (long)gf.nn[1] * (long)k)];
+        dst = st<TF>(ld<T>(dst) + (T)0.125 * acc);
     }
 }
 
@@ -485,9 +558,9 @@ static __global__ void __launch_bounds__(256) nodal_prolong_add_coarse_kernel(
 // restricted in-plane from LDS and combined along z (1/4, 1/2, 1/4) in
 // registers.  Same weights as nodal_restrict_kernel; optional first coarse
 // Jacobi sweep from zero (xc1) as there.
-template <class T>
+template <class T, class TF = T>
 static __global__ void __launch_bounds__(256) nodal_restrict_tiled_kernel(
-    KGeom gf, KGeom gc, const T* __restrict__ rf, T* __restrict__ rc, const T* __restrict__ sigc,
+    KGeom gf, KGeom gc, const TF* __restrict__ rf, T* __restrict__ rc, const T* __restrict__ sigc,
     T diag4c, T omegac, T* __restrict__ xc1, int kz, int gx, int gy, int gz) {
     constexpr int TX = 64, TY = 4, FC = 2 * TX + 1, FR = 2 * TY + 1;
     __shared__ T tile[FR][FC];
@@ -527,7 +600,7 @@ static __global__ void __launch_bounds__(256) nodal_restrict_tiled_kernel(
         fk = min(max(fk, 0), gf.nn[2] - 1);
         T v[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) v[q] = (ok[q] && zv) ? rf[off[q] + sxy * fk] : (T)0;
+        for (int q = 0; q < NQ; ++q) v[q] = (ok[q] && zv) ? ld<T>(rf[off[q] + sxy * fk]) : (T)0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int e = tid + 256 * q;
@@ -794,7 +867,8 @@ __device__ __forceinline__ TO apply_node_nbhd(const KGeom& g, const Coef<TO>& c,
             for (int q = 0; q < 2; ++q)
                 S[p][r][q] = (TO)sig[(i + q) + (long)g.sg[0] * ((j + r) + (long)g.sg[1] * (k + p))];
     const bool iso = c.face[0] == (TO)0 && c.face[1] == (TO)0 && c.face[2] == (TO)0;
-    return iso ? apply27<TO, true, 0>(P, S, c, ssum) : apply27<TO, false, 0>(P, S, c, ssum);
+    return iso ? apply27<TO, true, 0>(P[0], P[1], P[2], S[0], S[1], c, ssum)
+               : apply27<TO, false, 0>(P[0], P[1], P[2], S[0], S[1], c, ssum);
 }
 
 // (L x)_n in double; Dirichlet rows masked to 0.

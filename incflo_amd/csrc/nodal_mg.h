@@ -149,7 +149,32 @@ class NodalSolver {
     ~NodalSolver();
     void setup(const NodalGeom& g, const MGOpts& o);
     double* sigma0() { return sig0_buf.as<double>(); }
-    void sigma_changed() { sigma_set = true; h32_sigma = false; logsig_lvl = -1; }
+    // Mixed-mode V-cycle with an FP16-stored finest level (FP32 arithmetic):
+    // the operator is scaled to L' = L dx0^2 / sigma_max and the residual to
+    // r / s (s: the last residual norm), which keeps every stored value in
+    // FP16's normal range; the levels below are an FP32 hierarchy on the
+    // once-coarsened grid with the same scaling.
+    struct HalfTop {
+        NodalGeom g;            // level 0, dx scaled by 1 / dx0
+        KGeom kg;
+        Coef<float> coef;
+        DevBuf sig, xa, xb, b, r, sub_b, sub_sig;
+        __half* x = nullptr;
+        __half* x2 = nullptr;
+        NodalHierarchy<float> sub;
+        double sigmax = 1.0, dx0 = 1.0;
+        bool ready = false;     // x holds the first sweep from zero
+        bool built = false, sigma_ok = false;
+    } H16;
+    bool use_half = false;  // FP32 storage by default: the sweep is issue-bound, FP16 gains < 1%
+    void prepare_h16(hipStream_t s);
+    void vcycle16(hipStream_t s);
+    void sigma_changed() {
+        sigma_set = true;
+        h32_sigma = false;
+        H16.sigma_ok = false;
+        logsig_lvl = -1;
+    }
     // FP32 correction in H32.L[0].x from the FP32 residual in H32.L[0].b via
     // the learned predictor at level net_level.
     void learned_correction(hipStream_t s);

@@ -30,20 +30,29 @@ static int stencil_npt(bool dbl) {
         const char* e = getenv("MGX_STENCIL_NPT");
         return e ? atoi(e) : 0;
     }();
-    if (env == 1 || env == 2) return env;
+    if (env == 1 || env == 2 || env == 4) return env;
     (void)dbl;
     return 2;
 }
 
 template <class T, class TC, int MODE, int NPT>
 static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
-    constexpr int TX = 64, NY = 8;
+    constexpr int TX = 64, NY = (NPT == 1 ? 8 : 4 * NPT);
     a.gx = (g.nn[0] + TX - 1) / TX;
     a.gy = (g.nn[1] + NY - 1) / NY;
     const long gxy = (long)a.gx * a.gy;
     // z-planes per workgroup: long runs amortise the two prologue planes on big
     // levels; small levels are latency bound and get short runs / more blocks
-    int kz = (int)std::min<long>(64, std::max<long>(2, (g.nn[2] * gxy) / 2048));
+    // z-planes per workgroup from a target workgroup count: long z-runs
+    // amortise the two halo planes and the pipeline start; measured best at
+    // 256^3 (MI355X): 1024 workgroups for FP32/FP16 storage, 512 for FP64.
+    // MGX_STENCIL_BLOCKS overrides it for tuning runs.
+    static const long env_target = [] {
+        const char* e = getenv("MGX_STENCIL_BLOCKS");
+        return e ? atol(e) : 0L;
+    }();
+    const long target = env_target > 0 ? env_target : (sizeof(T) == 8 ? 512L : 1024L);
+    int kz = (int)std::min<long>(256, std::max<long>(2, (g.nn[2] * gxy) / target));
     kz = std::min(kz, g.nn[2]);
     a.kz = kz;
     a.gz = (g.nn[2] + kz - 1) / kz;
@@ -69,23 +78,26 @@ static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s
 
 template <class T, class TC, int MODE>
 void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
-    if (stencil_npt(sizeof(T) == 8) == 1)
+    const int npt = stencil_npt(sizeof(T) == 8);
+    if (npt == 1)
         launch_stencil_npt<T, TC, MODE, 1>(g, a, s, fine);
+    else if (npt == 4)
+        launch_stencil_npt<T, TC, MODE, 4>(g, a, s, fine);
     else
         launch_stencil_npt<T, TC, MODE, 2>(g, a, s, fine);
 }
 
 // Full-weighting restriction onto coarse level gc (z-streamed, LDS-staged);
 // xc1 (optional) receives the coarse first Jacobi sweep from zero.
-template <class T>
-static void launch_restrict(const KGeom& kf, const NodalGeom& gc, const KGeom& kc, const T* rf, T* rc,
+template <class T, class TF = T>
+static void launch_restrict(const KGeom& kf, const NodalGeom& gc, const KGeom& kc, const TF* rf, T* rc,
                             const T* sigc, T diag4c, T omegac, T* xc1, hipStream_t s) {
     const int gx = (gc.nn[0] + 63) / 64, gy = (gc.nn[1] + 3) / 4;
     const long gxy = (long)gx * gy;
     int kz = (int)std::min<long>(64, std::max<long>(1, (gc.nn[2] * gxy) / 2048));
     kz = std::min(kz, gc.nn[2]);
     const int gz = (gc.nn[2] + kz - 1) / kz;
-    nodal_restrict_tiled_kernel<T><<<(unsigned)(gxy * gz), dim3(64, 4), 0, s>>>(
+    nodal_restrict_tiled_kernel<T, TF><<<(unsigned)(gxy * gz), dim3(64, 4), 0, s>>>(
         kf, kc, rf, rc, sigc, diag4c, omegac, xc1, kz, gx, gy, gz);
 }
 
@@ -457,6 +469,7 @@ void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
     ibuf.alloc(16 * sizeof(int));
     if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
     H64.built = false;
+    H16.built = false;
     H32.built = false;
     sigma_set = keep_sigma;
     h32_sigma = false;
@@ -501,6 +514,107 @@ void NodalSolver::prepare_h32(hipStream_t s) {
     if (!h32_sigma) { H32.set_sigma(sigma0(), s); h32_sigma = true; }
     res32.alloc(g0.nnode() * sizeof(float));
     H32.L[0].b = res32.as<float>();
+}
+
+// out = (TO)(in * scale)
+template <class TO>
+static __global__ void scale_cast_kernel(const double* __restrict__ in, TO* __restrict__ out, long n,
+                                         double scale) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) out[id] = st<TO>((acc_t<TO>)(in[id] * scale));
+}
+
+void NodalSolver::prepare_h16(hipStream_t s) {
+    HalfTop& T = H16;
+    if (!T.built) {
+        T.dx0 = g0.dx[0];
+        T.g = g0;
+        for (int d = 0; d < 3; ++d) T.g.dx[d] = g0.dx[d] / T.dx0;
+        T.kg = kgeom(T.g);
+        T.coef = make_coef<float>(T.g.dx);
+        const long nn = g0.nnode();
+        T.sig.alloc(g0.nsig() * sizeof(__half));
+        for (DevBuf* b : {&T.xa, &T.xb, &T.b, &T.r}) b->alloc(nn * sizeof(__half));
+        T.x = T.xa.as<__half>();
+        T.x2 = T.xb.as<__half>();
+        const NodalGeom gs = coarsen(T.g);
+        T.sub.build(gs, nlev - 1);
+        T.sub.timer = nullptr;
+        T.sub_b.alloc(gs.nnode() * sizeof(float));
+        T.sub.L[0].b = T.sub_b.as<float>();
+        T.sub_sig.alloc(gs.nsig() * sizeof(double));
+        T.built = true;
+        T.sigma_ok = false;
+        T.ready = false;
+    }
+    if (!T.sigma_ok) {
+        zero_norm(2, s);
+        max_abs_kernel<double><<<256, 256, 0, s>>>(sigma0(), g0.nsig(), scal.as<unsigned long long>() + 2);
+        const double smax = read_norm(2, s);
+        MGX_REQUIRE(smax > 0.0, MGX_ERR_STATE, "nodal solve: sigma is zero everywhere");
+        T.sigmax = smax;
+        scale_cast_kernel<__half><<<grid1d(g0.nsig()), 256, 0, s>>>(sigma0(), T.sig.as<__half>(),
+                                                                    g0.nsig(), 1.0 / smax);
+        const NodalGeom gs = coarsen(T.g);
+        MGX_HIP(hipMemsetAsync(T.sub_sig.p, 0, gs.nsig() * sizeof(double), s));
+        sigma_coarsen_kernel<double><<<grid3(gs.n[0], gs.n[1], gs.n[2]), block3(), 0, s>>>(
+            kgeom(T.g), kgeom(gs), sigma0(), T.sub_sig.as<double>());
+        scale_cast_kernel<double><<<grid1d(gs.nsig()), 256, 0, s>>>(
+            T.sub_sig.as<double>(), T.sub_sig.as<double>(), gs.nsig(), 1.0 / smax);
+        T.sub.set_sigma(T.sub_sig.as<double>(), s);
+        T.sigma_ok = true;
+    }
+}
+
+// One V-cycle on the FP16 level-0 residual H16.b; the correction (scaled) ends in H16.x.
+void NodalSolver::vcycle16(hipStream_t s) {
+    HalfTop& T = H16;
+    const MGOpts& o = opts;
+    const long nn = g0.nnode();
+    SArgs<__half, __half> a{};
+    a.g = T.kg;
+    a.c = T.coef;
+    a.b = T.b.as<__half>();
+    a.sig = T.sig.as<__half>();
+    a.omega = (float)nodal_omega(T.g, o.omega);
+    int sweeps = o.nu1;
+    if (sweeps > 0) {
+        if (!T.ready) {
+            a.xout = T.x;
+            launch_stencil<__half, __half, KS_JFIRST>(T.g, a, s, false);
+        }
+        --sweeps;
+    } else {
+        zero_kernel<__half><<<grid1d(nn), 256, 0, s>>>(T.x, nn);
+    }
+    T.ready = false;
+    auto jacobi = [&]() {
+        a.x = T.x;
+        a.xout = T.x2;
+        fine_timer.begin(s);
+        launch_stencil<__half, __half, KS_JACOBI>(T.g, a, s, true);
+        fine_timer.end(s);
+        std::swap(T.x, T.x2);
+    };
+    for (int q = 0; q < sweeps; ++q) jacobi();
+    {
+        SArgs<__half, __half> r = a;
+        r.x = T.x;
+        r.xout = nullptr;
+        r.rout = T.r.as<__half>();
+        launch_stencil<__half, __half, KS_RESID>(T.g, r, s, false);
+    }
+    MGLevel<float>& c1 = T.sub.L[0];
+    c1.x = c1.x_buf.as<float>();
+    c1.x2 = c1.x2_buf.as<float>();
+    const bool x1 = o.nu1 > 0 && T.sub.L.size() > 1;
+    launch_restrict<float, __half>(T.kg, c1.g, c1.kg, T.r.as<__half>(), c1.b, c1.sig, c1.coef.diag4,
+                                   (float)nodal_omega(c1.g, o.omega), x1 ? c1.x : nullptr, s);
+    T.sub.x0_ready = x1;
+    T.sub.vcycle(o, s, ibuf.as<int>());
+    nodal_prolong_add_coarse_kernel<float, __half>
+        <<<grid3(c1.g.nn[0], c1.g.nn[1], c1.g.nn[2]), block3(), 0, s>>>(T.kg, c1.kg, c1.x, T.x);
+    for (int q = 0; q < o.nu2; ++q) jacobi();
 }
 
 static void prepare_h64(NodalSolver& S, hipStream_t s) {
@@ -607,10 +721,14 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     const KGeom kg = kgeom(g);
     const long nn = g.nnode();
     bool mixed = (mode != MGX_MODE_FP64);
+    // the mixed V-cycle runs on the FP16-stored finest level when there is a
+    // level below it; the learned step works on the FP32 hierarchy
+    const bool half = mixed && use_half && nlev >= 2;
     H64.timer = &fine_timer;
     H32.timer = &fine_timer;
-    if (mixed) prepare_h32(s);
-    else prepare_h64(*this, s);
+    if (mixed && (!half || learned)) prepare_h32(s);
+    if (half) prepare_h16(s);
+    if (!mixed) prepare_h64(*this, s);
 
     const bool has_dir = g.dlo[0] || g.dlo[1] || g.dlo[2] || g.dhi[0] || g.dhi[1] || g.dhi[2];
     if (has_dir) {
@@ -645,10 +763,37 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
             H.x0_ready = true;
         }
     };
-    // r = b - L phi into the V-cycle's level-0 right-hand side
-    auto residual = [&](bool into32) {
+    // FP16 level: the stored residual is r / s16 and the stored first sweep
+    // belongs to the scaled operator (L dx0^2 / sigma_max)
+    double s16 = 1.0;
+    auto half_scales = [&](SArgs<double, __half>& a, double sc) {
+        s16 = sc > 0.0 ? sc : 1.0;
+        a.rscale = 1.0 / s16;
+        if (opts.nu1 > 0) {
+            a.x1out = H16.x2;
+            a.omega1 = nodal_omega(g, opts.omega);
+            a.x1scale = H16.sigmax / (s16 * H16.dx0 * H16.dx0);
+        }
+    };
+    auto half_done = [&](const SArgs<double, __half>& a) {
+        if (a.x1out) {
+            std::swap(H16.x, H16.x2);
+            H16.ready = true;
+        }
+    };
+    enum { P64 = 0, P32 = 1, P16 = 2 };
+    // r = b - L phi into the V-cycle's level-0 right-hand side; sc: a scale
+    // for the FP16 residual (an estimate of ||r||_inf)
+    auto residual = [&](int prec, double sc) {
         zero_norm(1, s);
-        if (into32) {
+        if (prec == P16) {
+            SArgs<double, __half> a{};
+            a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
+            a.rout = H16.b.as<__half>(); a.norm = scal.as<unsigned long long>() + 1;
+            half_scales(a, sc);
+            launch_stencil<double, __half, KS_RESID>(g, a, s, false);
+            half_done(a);
+        } else if (prec == P32) {
             SArgs<double, float> a{};
             a.g = kg; a.c = c64; a.x = phi; a.b = b; a.sig = sigma0();
             a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
@@ -666,9 +811,17 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         return read_norm(1, s);
     };
     // phi_out = phi + cor, r = b - L phi_out (FP64), returns ||r||_inf
-    auto update = [&](bool from32, double* pout) {
+    auto update = [&](int prec, double* pout, double sc) {
         zero_norm(1, s);
-        if (from32) {
+        if (prec == P16) {
+            SArgs<double, __half> a{};
+            a.g = kg; a.c = c64; a.x = phi; a.cor = H16.x; a.b = b; a.sig = sigma0();
+            a.cor_scale = H16.dx0 * H16.dx0 * s16 / H16.sigmax;
+            a.xout = pout; a.rout = H16.b.as<__half>(); a.norm = scal.as<unsigned long long>() + 1;
+            half_scales(a, sc);
+            launch_stencil<double, __half, KS_UPDRES>(g, a, s, false);
+            half_done(a);
+        } else if (prec == P32) {
             SArgs<double, float> a{};
             a.g = kg; a.c = c64; a.x = phi; a.cor = H32.L[0].x; a.b = b; a.sig = sigma0();
             a.xout = pout; a.rout = res32.as<float>(); a.norm = scal.as<unsigned long long>() + 1;
@@ -687,7 +840,8 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     };
     auto other = [&]() { return (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>(); };
 
-    const double r0 = residual(mixed);
+    const int vprec = mixed ? (half ? P16 : P32) : P64;
+    const double r0 = residual((learned || !half) ? (mixed ? P32 : P64) : P16, bnorm);
     const double tol = std::max(atol, rtol * bnorm);
     double rn = r0;
     int it = 0;
@@ -698,26 +852,29 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         ++it;
         learned_correction(s);
         double* pout = other();
-        const double rl = update(true, pout);
+        const double rl = update(P32, pout, rn);
         if (opts.verbose)
             fprintf(stderr, "mgx nodal MLMG: learned step |r| %.6e (from %.6e)\n", rl, rn);
         if (rl < rn) {
             phi = pout;
             rn = rl;
+            if (half) rn = residual(P16, rn);
         } else {
             fallback = 1;
-            rn = residual(true);
+            rn = residual(vprec, rn);
         }
         conv = (rn <= tol);
     }
+    int prec = vprec;
     int stall = 0;
     while (!conv && it < opts.maxiter) {
         ++it;
         double* pout = other();
-        if (mixed) H32.vcycle(opts, s, ibuf.as<int>());
+        if (prec == P16) vcycle16(s);
+        else if (prec == P32) H32.vcycle(opts, s, ibuf.as<int>());
         else H64.vcycle(opts, s, ibuf.as<int>());
         const double prev = rn;
-        rn = update(mixed, pout);
+        rn = update(prec, pout, rn);
         phi = pout;
         if (opts.verbose)
             fprintf(stderr, "mgx nodal MLMG: iter %d |r| %.6e rel %.3e\n", it, rn,
@@ -729,8 +886,9 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
             if (stall >= opts.stall_iters) {
                 fallback = 1;
                 mixed = false;
+                prec = P64;
                 prepare_h64(*this, s);
-                rn = residual(false);
+                rn = residual(P64, rn);
             }
         }
     }

@@ -49,6 +49,8 @@ class UNet {
     DevBuf dbuf[4];            // gradient scratch
     std::vector<DevBuf> gact_; // gradients of the activation slots
     DevBuf loss_buf, ybuf;
+    DevBuf ksws;                    // split-K partial sums of the forward convolutions
+    static constexpr long kKsWsFloats = 4L << 20;
 
     void create(int base_channels, int levels_, uint64_t seed, hipStream_t s);
     void forward(const float* in, const NetShape& sh, float* out);

@@ -65,19 +65,9 @@ inline unsigned g1(long n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 static void launch_conv(const ConvGeom& cg, const float* in, const unsigned short* w,
-                        const float* bias, float* out, int relu, hipStream_t s) {
-    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
-    const unsigned nb = (unsigned)((nvox + 63) / 64);
-    switch (cg.cout / 16) {
-        case 1: conv3d_fwd_kernel<1><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 2: conv3d_fwd_kernel<2><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 3: conv3d_fwd_kernel<3><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 4: conv3d_fwd_kernel<4><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 6: conv3d_fwd_kernel<6><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 8: conv3d_fwd_kernel<8><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 12: conv3d_fwd_kernel<12><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        default: throw Error(MGX_ERR_UNSUPPORTED, "conv3d: unsupported output channel count");
-    }
+                        const float* bias, float* out, int relu, float* ws, long ws_floats,
+                        hipStream_t s) {
+    conv3d_forward_launch(cg, in, w, bias, out, relu, ws, ws_floats, s);
 }
 
 static void launch_dw(const ConvGeom& cg, const float* in, const float* dy, float* dw, int ldw,
@@ -99,6 +89,7 @@ void UNet::create(int base_channels, int levels_, uint64_t seed, hipStream_t s) 
     levels = levels_;
     base = base_channels;
     stream = s;
+    ksws.alloc(kKsWsFloats * sizeof(float));
     conv.clear();
     long off = 0, wb = 0, wt = 0;
     auto add = [&](int cin, int cout) {
@@ -229,7 +220,7 @@ void UNet::conv_fwd(const ConvLayer& L, const float* in, int lvl, float* out, bo
     cg.cin = L.cin;
     cg.cout = L.cout;
     cg.kp = L.kp;
-    launch_conv(cg, in, w, bias, out, relu ? 1 : 0, stream);
+    launch_conv(cg, in, w, bias, out, relu ? 1 : 0, ksws.as<float>(), kKsWsFloats, stream);
 }
 
 static ConvGeom geom_at(const NetShape& sh, int lvl, int cin, int cout, int kp) {
@@ -330,7 +321,7 @@ void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
         bias_grad_kernel<<<bblocks, 256, 0, stream>>>(dy, nv, L.cout, G + L.boff);
         if (dx) {
             ConvGeom ct = geom_at(sh, lvl, L.cout, L.cin, L.kpt);
-            launch_conv(ct, dy, wt + L.wt_off, nullptr, dx, 0, stream);
+            launch_conv(ct, dy, wt + L.wt_off, nullptr, dx, 0, ksws.as<float>(), kKsWsFloats, stream);
         }
     };
 

@@ -30,18 +30,7 @@ __global__ void to_bf16_padded(const float* __restrict__ w, int cout, int K, int
 
 void conv_launch(const ConvGeom& cg, const float* in, const unsigned short* w, const float* bias,
                  float* out, int relu, hipStream_t s) {
-    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
-    const unsigned nb = (unsigned)((nvox + 63) / 64);
-    switch (cg.cout / 16) {
-        case 1: conv3d_fwd_kernel<1><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 2: conv3d_fwd_kernel<2><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 3: conv3d_fwd_kernel<3><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 4: conv3d_fwd_kernel<4><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 6: conv3d_fwd_kernel<6><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 8: conv3d_fwd_kernel<8><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        case 12: conv3d_fwd_kernel<12><<<nb, 256, 0, s>>>(cg, in, w, bias, out, relu); break;
-        default: throw Error(MGX_ERR_UNSUPPORTED, "conv3d: output channels must be 16,32,48,64,96,128 or 192");
-    }
+    conv3d_forward_launch(cg, in, w, bias, out, relu, nullptr, 0, s);
 }
 
 int pad32(int k) { return (k + 31) / 32 * 32; }

@@ -19,6 +19,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
+#include "mgx_common.h"
+
 namespace mgx {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -67,11 +71,17 @@ __device__ __forceinline__ bf16x8 to_bf16x8(const float* p, bool valid) {
 // Forward 3x3x3 convolution (+bias, optional fused ReLU).  One wave computes
 // 16 voxels x all output channels (cout/16 accumulator tiles); a workgroup of
 // 4 waves covers 64 consecutive voxels of the batch.
+// Split-K: with part != nullptr, blockIdx.y selects a contiguous range of the
+// K steps and the raw partial sums go to part[split][v][co] (bias and ReLU are
+// applied by conv3d_splitk_reduce_kernel); small levels (few voxels, long K)
+// then keep many waves in flight instead of a few long dependent chains.
 template <int NT>
 static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, const float* __restrict__ in,
                                                                const unsigned short* __restrict__ w,
                                                                const float* __restrict__ bias,
-                                                               float* __restrict__ out, int relu) {
+                                                               float* __restrict__ out, int relu,
+                                                               float* __restrict__ part = nullptr,
+                                                               int ksplit = 1) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long S = (long)g.nx * g.ny * g.nz;
     const long nvox = S * g.batch;
@@ -89,7 +99,9 @@ static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, cons
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int K = 27 * g.cin;
-    for (int k0 = 0; k0 < g.kp; k0 += 32) {
+    const int steps = g.kp / 32, per = (steps + ksplit - 1) / ksplit;
+    const int s0 = (int)blockIdx.y * per, s1 = min(steps, s0 + per);
+    for (int k0 = 32 * s0; k0 < 32 * s1; k0 += 32) {
         const int k = k0 + 8 * kq;
         bf16x8 afrag;
         if (k < K) {
@@ -111,6 +123,19 @@ static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, cons
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc[t], 0, 0, 0);
         }
     }
+    if (part) {
+        float* dst = part + (long)blockIdx.y * nvox * g.cout;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + row;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long v = v0 + 4 * kq + j;
+                if (v < nvox) dst[v * g.cout + co] = acc[t][j];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int co = t * 16 + row;
@@ -124,6 +149,56 @@ static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, cons
                 out[v * g.cout + co] = r;
             }
         }
+    }
+}
+
+// out[v][co] = sum_s part[s][v][co] + bias[co] (fixed order), optional ReLU
+static __global__ void __launch_bounds__(256) conv3d_splitk_reduce_kernel(
+    long n, int cout, int ksplit, const float* __restrict__ part, const float* __restrict__ bias,
+    float* __restrict__ out, int relu) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float r = 0.f;
+    for (int q = 0; q < ksplit; ++q) r += part[(long)q * n + e];
+    if (bias) r += bias[e % cout];
+    if (relu) r = fmaxf(r, 0.f);
+    out[e] = r;
+}
+
+// Forward conv launch.  ws (ws_floats floats, may be null) enables split-K
+// when the voxel count alone gives too few waves to fill the GPU.
+inline void conv3d_forward_launch(const ConvGeom& cg, const float* in, const unsigned short* w,
+                                  const float* bias, float* out, int relu, float* ws,
+                                  long ws_floats, hipStream_t s) {
+    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
+    const unsigned nb = (unsigned)((nvox + 63) / 64);
+    const int steps = cg.kp / 32;
+    int ksplit = 1;
+    if (ws) {
+        const long waves = 4L * nb;
+        ksplit = (int)std::min<long>(std::max<long>(1, 1024 / waves), std::max(1, steps / 4));
+        while (ksplit > 1 && (long)ksplit * nvox * cg.cout > ws_floats) --ksplit;
+        if (ksplit > 1) {
+            const int per = (steps + ksplit - 1) / ksplit;
+            ksplit = (steps + per - 1) / per;  // no empty splits
+        }
+    }
+    float* part = ksplit > 1 ? ws : nullptr;
+    const dim3 grid(nb, ksplit);
+    switch (cg.cout / 16) {
+        case 1: conv3d_fwd_kernel<1><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 2: conv3d_fwd_kernel<2><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 3: conv3d_fwd_kernel<3><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 4: conv3d_fwd_kernel<4><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 6: conv3d_fwd_kernel<6><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 8: conv3d_fwd_kernel<8><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 12: conv3d_fwd_kernel<12><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        default: throw Error(MGX_ERR_UNSUPPORTED, "conv3d: output channels must be 16,32,48,64,96,128 or 192");
+    }
+    if (part) {
+        const long n = nvox * cg.cout;
+        conv3d_splitk_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, cg.cout, ksplit, part,
+                                                                               bias, out, relu);
     }
 }
 

@@ -73,6 +73,7 @@ def lib():
             "mgx_stream_copy": [vp, vp, vp, C.c_ulonglong],
             "mgx_nodal_set_timing": [vp, C.c_int],
             "mgx_nodal_set_graphs": [vp, C.c_int],
+            "mgx_nodal_set_vcycle_bits": [vp, C.c_int],
             "mgx_nodal_get_timing": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
             "mgx_ctx_record": [vp, C.c_int],
             "mgx_ctx_elapsed_ms": [vp, C.c_int, C.c_int, C.POINTER(C.c_double)],
@@ -274,6 +275,9 @@ class NodalProjector:
 
     def set_graphs(self, on):
         self.ctx.check(lib().mgx_nodal_set_graphs(self.h, int(on)))
+
+    def set_vcycle_bits(self, bits):
+        self.ctx.check(lib().mgx_nodal_set_vcycle_bits(self.h, int(bits)))
 
     def get_timing(self):
         ms, n = C.c_double(), C.c_longlong()

@@ -146,6 +146,12 @@ MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long 
  * HIP graph and replayed; off, its kernels are launched one by one.  The
  * kernels and results are the same either way. */
 MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on);
+/* Storage of the finest level of the mixed-mode V-cycle (no reference
+ * counterpart; MLMG runs in FP64 throughout): 32 (default) stores FP32; 16
+ * stores x, b and sigma in FP16 with the operator and the residual scaled
+ * into FP16's range and computes in FP32.  The outer residual, the update
+ * and the convergence test are FP64 either way. */
+MGX_API int mgx_nodal_set_vcycle_bits(mgx_nodal h, int bits);
 /* Variable sigma, a boxed cell field with ng ghosts (:191, sigma = s/rho at :117). */
 MGX_API int mgx_nodal_set_sigma(mgx_nodal h, const double* sigma, int sigma_ng);
 /* Scalar sigma = s/rho_0 (:186-188). */

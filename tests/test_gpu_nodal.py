@@ -314,3 +314,41 @@ def test_graph_replay_matches_plain_launches(ctx, orc):
         for (va, pa, ia), (vb, pb, ib) in zip(results[False], results[True]):
             assert ia == ib
             assert np.array_equal(va, vb) and np.array_equal(pa, pb)
+
+
+@pytest.mark.parametrize("sigma_scale", [1e-7, 1.0, 1e5])
+def test_fp16_vcycle_storage_converges_like_fp32(ctx, orc, sigma_scale):
+    """The FP16-stored finest V-cycle level (operator scaled by dx0^2/sigma_max,
+    residual by its last norm) reaches the FP64 tolerance in at most one more
+    V-cycle than FP32 storage, for coefficients far from 1 in either direction
+    (their FP16 values would underflow / overflow unscaled); the answer's
+    residual is checked with the oracle's FP64 operator."""
+    from incflo_amd import mgx, workloads
+    n = 48
+    _say(f"case fp16 v-cycle sigma scale {sigma_scale:g}")
+    rho, ustar, dt = workloads.variable_density_case(n, 7)
+    bxs = B.chop_domain((n, n, n), 16)
+    lev = mgx.make_level((n, n, n), bxs)
+    g = _geom(orc, (n, n, n))
+    sigma = sigma_scale * dt / rho
+    iters = {}
+    for bits in (32, 16):
+        P = mgx.NodalProjector(ctx, lev)
+        P.set_vcycle_bits(bits)
+        P.set_sigma(ctx.array(B.to_boxed(sigma, bxs, 0)), 0)
+        vel = ctx.array(B.to_boxed(ustar, bxs, 1))
+        info = P.project(vel, 1, 1e-10, 0.0, mode=mgx.MODE_MIXED)
+        assert info.converged and info.rel_resid <= 1e-10
+        phi = ctx.array(B.field_size(bxs, 0, 1, True))
+        P.getPhi(phi, 0)
+        x = B.from_boxed(phi.download(), bxs, 0, g.node_shape(), nodal=True)
+        ug = np.zeros(g.ghost_vec_shape())
+        ug[:, 1:-1, 1:-1, 1:-1] = ustar
+        orc.fill_vel_ghosts(g, ug)
+        bb = orc.nodal_divergence(g, ug)
+        bb -= bb.mean()
+        r = orc.nodal_residual(g, sigma, bb, x)
+        assert np.max(np.abs(r)) <= 1.01e-10 * np.max(np.abs(bb))
+        iters[bits] = info.iters
+        P.close()
+    assert iters[16] <= iters[32] + 1, iters
