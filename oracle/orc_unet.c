@@ -64,7 +64,8 @@ void orc_conv3d_fwd(const double* in, const double* w, const double* bias, doubl
     const long K = 27L * cin;
     double* wr = (double*)malloc(sizeof(double) * cout * K);
     for (long q = 0; q < cout * K; ++q) wr[q] = R(w[q], bf16);
-#pragma omp parallel for schedule(static)
+    /* threads only for work that outweighs their start-up (tiny finite-difference cases) */
+#pragma omp parallel for schedule(static) if (nv * K * cout > 4000000L)
     for (long v = 0; v < nv; ++v) {
         const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)((v / ((long)nx * ny)) % nz);
         const int b = (int)(v / ((long)nx * ny * nz));
@@ -91,7 +92,7 @@ void orc_conv3d_bwd(const double* in, const double* w, const double* dout, doubl
     const long K = 27L * cin;
     if (din) {
         /* din[u][ci] = sum_{tap,co} w[co][tap,ci] dout[u - off(tap)][co] */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (nv * K * cout > 4000000L)
         for (long u = 0; u < nv; ++u) {
             const int x = (int)(u % nx), y = (int)((u / nx) % ny), z = (int)((u / ((long)nx * ny)) % nz);
             const int b = (int)(u / ((long)nx * ny * nz));
@@ -108,7 +109,7 @@ void orc_conv3d_bwd(const double* in, const double* w, const double* dout, doubl
         }
     }
     if (dw) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (nv * K * cout > 4000000L)
         for (int co = 0; co < cout; ++co) {
             double* acc = dw + (long)co * K;
             for (long k = 0; k < K; ++k) acc[k] = 0.0;
