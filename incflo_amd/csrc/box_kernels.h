@@ -303,6 +303,18 @@ static __global__ void __launch_bounds__(256) gather_nodes_kernel(KGeom g, const
         in[field_offset(b, ng, 1, true) + box_index(b, ng, true, i, j, k, 0)];
 }
 
+// canonical node flags (1: value != 0) <- boxed nodal int field
+static __global__ void __launch_bounds__(256) gather_node_flags_kernel(KGeom g, const BoxInfo* boxes,
+                                                                     const Tile* tiles, const int* in,
+                                                                     int ng, unsigned char* out) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_NODE, bi, i, j, k)) return;
+    if (i >= g.nn[0] || j >= g.nn[1] || k >= g.nn[2]) return;  // periodic duplicate
+    const BoxInfo& b = boxes[bi];
+    out[i + (long)g.nn[0] * (j + (long)g.nn[1] * k)] =
+        in[field_offset(b, ng, 1, true) + box_index(b, ng, true, i, j, k, 0)] != 0;
+}
+
 // ---- face and cell fields of the MAC projection -----------------------------
 // canonical face index (faces along d: n_d + 1 of them)
 __device__ __forceinline__ long canon_face(const KGeom& g, int d, int i, int j, int k) {

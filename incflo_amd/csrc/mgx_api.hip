@@ -44,6 +44,9 @@ struct mgx_nodal_s {
     DevBuf ug, rhs;
     bool have_phi = false;
     // embedded boundary request: 0 none, 1 z-cylinder (eb_cyl = cx, cy, r), 2 moments
+    // overset mask (canonical node flags) as the caller set it
+    DevBuf omask;
+    bool has_omask = false;
     int eb_kind = 0;
     double eb_cyl[3] = {0, 0, 0};
     std::vector<int> eb_type;
@@ -76,6 +79,7 @@ struct mgx_nodal_s {
         have_phi = false;
         dirty = false;
         build_eb();
+        if (has_omask) solver.set_mask(omask.as<unsigned char>(), ctx->stream);
     }
     hipStream_t stream() const { return ctx->stream; }
     void compute_rhs() {
@@ -338,6 +342,29 @@ MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on) {
             h->solver.H64.drop_graphs();
             h->solver.H32.drop_graphs();
         }
+    });
+}
+
+MGX_API int mgx_nodal_set_overset_mask(mgx_nodal h, const int* mask, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        h->ensure_setup();
+        if (!mask) {
+            h->has_omask = false;
+            h->solver.set_mask(nullptr, h->stream());
+            return;
+        }
+        MGX_REQUIRE(ng >= 0 && ng <= 8, MGX_ERR_ARG, "overset mask ghost width out of range");
+        MGX_REQUIRE(h->eb_kind == 0, MGX_ERR_UNSUPPORTED,
+                    "overset mask with embedded boundaries is not supported");
+        const long nn = h->g.nnode();
+        h->omask.alloc(nn);
+        MGX_HIP(hipMemsetAsync(h->omask.p, 1, nn, h->stream()));
+        gather_node_flags_kernel<<<h->layout.ntiles[REG_NODE], 256, 0, h->stream()>>>(
+            kgeom(h->g), h->layout.dev_boxes(), h->layout.tiles(REG_NODE), mask, ng,
+            h->omask.as<unsigned char>());
+        h->has_omask = true;
+        h->solver.set_mask(h->omask.as<unsigned char>(), h->stream());
     });
 }
 

@@ -880,23 +880,30 @@ __device__ double apply_node_direct(const KGeom& g, const Coef<double>& c, const
     return apply_node_nbhd<double, TS, double>(g, c, sig, x, i, j, k, ssum);
 }
 
+// A node held at zero: a Dirichlet node, or mask 0 of an overset mask (m may be null).
+__device__ __forceinline__ bool node_fixed(const KGeom& g, const unsigned char* m, long id, int i,
+                                           int j, int k) {
+    return is_dir(g, i, j, k) || (m && !m[id]);
+}
+
 template <class TS>
-__device__ void bottom_apply(const KGeom& g, const Coef<double>& c, const TS* sig, const double* x,
-                             double* y, long n) {
+__device__ void bottom_apply(const KGeom& g, const Coef<double>& c, const TS* sig,
+                             const unsigned char* m, const double* x, double* y, long n) {
     __syncthreads();
     for (long id = threadIdx.x; id < n; id += blockDim.x) {
         const int i = id % g.nn[0];
         const int j = (id / g.nn[0]) % g.nn[1];
         const int k = id / ((long)g.nn[0] * g.nn[1]);
-        y[id] = apply_node_direct(g, c, sig, x, i, j, k);
+        y[id] = (m && !m[id]) ? 0.0 : apply_node_direct(g, c, sig, x, i, j, k);
     }
     __syncthreads();
 }
 
 template <class T>
 __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, const T* sig,
-                                     const T* b_in, T* x_out, double* work, int maxiter,
-                                     double rtol, int singular, int* iters_acc, double* red) {
+                                     const unsigned char* m, const T* b_in, T* x_out,
+                                     double* work, int maxiter, double rtol, int singular,
+                                     int* iters_acc, double* red) {
     const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
     double* x = work;
     double* r = work + n;
@@ -906,9 +913,12 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
     double* s = work + 5 * n;
     double* t = work + 6 * n;
     const int tid = threadIdx.x, nt = blockDim.x;
-    // b (mean-free when singular) into r; x = 0
+    // b (mean-free when singular; zero at overset-masked nodes) into r; x = 0
     double bs = 0.0;
-    for (long id = tid; id < n; id += nt) { r[id] = (double)b_in[id]; bs += r[id]; }
+    for (long id = tid; id < n; id += nt) {
+        r[id] = (m && !m[id]) ? 0.0 : (double)b_in[id];
+        bs += r[id];
+    }
     bs = bsum(bs, red);
     const double bmean = singular ? bs / (double)n : 0.0;
     for (long id = tid; id < n; id += nt) {
@@ -929,7 +939,7 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
             if (rho_new == 0.0) break;
             const double beta = (rho_new / rho) * (alpha / omega);
             for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
-            bottom_apply(g, c, sig, p, v, n);
+            bottom_apply(g, c, sig, m, p, v, n);
             q = 0.0;
             for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
             const double rhv = bsum(q, red);
@@ -943,7 +953,7 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
             }
             sn = bmax(sn, red);
             if (sn <= rtol * rnorm0) break;
-            bottom_apply(g, c, sig, s, t, n);
+            bottom_apply(g, c, sig, m, s, t, n);
             double tt = 0.0, ts = 0.0;
             for (long id = tid; id < n; id += nt) { tt += t[id] * t[id]; ts += t[id] * s[id]; }
             tt = bsum(tt, red);
@@ -974,12 +984,15 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
 
 template <class T>
 static __global__ void __launch_bounds__(1024) bottom_bicgstab_kernel(KGeom g, Coef<double> c,
-                                                                      const T* sig, const T* b_in,
-                                                                      T* x_out, double* work,
-                                                                      int maxiter, double rtol,
-                                                                      int singular, int* iters_acc) {
+                                                                      const T* sig,
+                                                                      const unsigned char* m,
+                                                                      const T* b_in, T* x_out,
+                                                                      double* work, int maxiter,
+                                                                      double rtol, int singular,
+                                                                      int* iters_acc) {
     __shared__ double red[16];
-    bottom_bicgstab_body<T>(g, c, sig, b_in, x_out, work, maxiter, rtol, singular, iters_acc, red);
+    bottom_bicgstab_body<T>(g, c, sig, m, b_in, x_out, work, maxiter, rtol, singular, iters_acc,
+                            red);
 }
 
 // ---- coarse levels: the whole V-cycle below a size threshold in one workgroup ----
@@ -1000,6 +1013,7 @@ struct CoarseLevel {
     T* x;
     T* x2;
     T* r;
+    const unsigned char* m;  // overset mask (global memory), null without one
 };
 
 template <class T>
@@ -1099,7 +1113,7 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
             T ssum;
             const T acc = apply_node_T<T>(lv.g, lv.c, lv.sig, xin, i, j, k, ssum);
             T v = xin[id];
-            if (is_dir(lv.g, i, j, k)) v = 0;
+            if (node_fixed(lv.g, lv.m, id, i, j, k)) v = 0;
             else if (ssum != (T)0) v = v + lv.omega * (lv.b[id] - acc) / (-lv.c.diag4 * ssum);
             xout[id] = v;
         }
@@ -1119,8 +1133,9 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
                 for (int q = 0; q < 8; ++q)
                     ssum += lv.sig[(i + (q & 1)) + (long)lv.g.sg[0] * ((j + ((q >> 1) & 1)) +
                                                                     (long)lv.g.sg[1] * (k + ((q >> 2) & 1)))];
-                x[id] = (is_dir(lv.g, i, j, k) || ssum == (T)0) ? (T)0
-                                                               : lv.omega * lv.b[id] / (-lv.c.diag4 * ssum);
+                x[id] = (node_fixed(lv.g, lv.m, id, i, j, k) || ssum == (T)0)
+                            ? (T)0
+                            : lv.omega * lv.b[id] / (-lv.c.diag4 * ssum);
             }
             __syncthreads();
             for (int s = 1; s < a.nu1; ++s) jacobi(l);
@@ -1136,7 +1151,7 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
                 node_of<T>(lv.g, id, i, j, k);
                 T ssum;
                 const T acc = apply_node_T<T>(lv.g, lv.c, lv.sig, x, i, j, k, ssum);
-                oth(l)[id] = is_dir(lv.g, i, j, k) ? (T)0 : lv.b[id] - acc;
+                oth(l)[id] = node_fixed(lv.g, lv.m, id, i, j, k) ? (T)0 : lv.b[id] - acc;
             }
             __syncthreads();
         }
@@ -1150,7 +1165,7 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
                 int I, J, K;
                 node_of<T>(gc, id, I, J, K);
                 T acc = 0;
-                if (!is_dir(gc, I, J, K)) {
+                if (!node_fixed(gc, cv.m, id, I, J, K)) {
                     for (int oz = -1; oz <= 1; ++oz) {
                         int fk = 2 * K + oz;
                         if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
@@ -1178,7 +1193,7 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
     // bottom
     {
         const int lb = a.nl - 1;
-        bottom_bicgstab_body<T>(sl[lb].g, a.cb, sl[lb].sig, sl[lb].b, cur(lb), swork,
+        bottom_bicgstab_body<T>(sl[lb].g, a.cb, sl[lb].sig, sl[lb].m, sl[lb].b, cur(lb), swork,
                                 a.maxiter, a.rtol, a.singular, a.iters_acc, red);
         __syncthreads();
     }
@@ -1193,7 +1208,7 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
         for (int id = tid; id < nf; id += nt) {
             int i, j, k;
             node_of<T>(gf, id, i, j, k);
-            if (is_dir(gf, i, j, k)) continue;
+            if (node_fixed(gf, lv.m, id, i, j, k)) continue;
             const int f[3] = {i, j, k};
             int lo[3], hi[3];
             for (int d = 0; d < 3; ++d) {

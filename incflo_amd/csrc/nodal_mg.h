@@ -84,6 +84,11 @@ class NodalHierarchy {
                 bool x_ready = false);
     // set by the outer iteration when its residual pass also wrote L[0].x
     bool x0_ready = false;
+    // optional overset mask per level (1 solve, 0 fixed at 0; empty: none)
+    std::vector<const unsigned char*> masks;
+    bool bottom_fixed = false;  // the bottom level's overset mask holds a node at zero
+    const unsigned char* mask(int l) const { return masks.empty() ? nullptr : masks[l]; }
+    bool bottom_singular() const { return L.back().g.singular() && !(mask(0) && bottom_fixed); }
     bool built = false;
     // Replay the V-cycle as a HIP graph: the launch sequence is the same every
     // cycle once each level starts from its own pair of buffers, so it is
@@ -167,6 +172,12 @@ class NodalSolver {
         bool built = false, sigma_ok = false;
     } H16;
     bool use_half = false;  // FP32 storage by default: the sweep is issue-bound, FP16 gains < 1%
+    // overset mask (MLNodeLaplacian::setOversetMask): per level, 1 = solve,
+    // 0 = node fixed at phi = 0; coarse levels by injection
+    std::vector<DevBuf> mask_lv;
+    bool mask_on = false, mask_fixed = false, mask_bottom_fixed = false;
+    // level-0 mask (canonical nodes, device); nullptr removes it
+    void set_mask(const unsigned char* m0, hipStream_t s);
     void prepare_h16(hipStream_t s);
     void vcycle16(hipStream_t s);
     void sigma_changed() {

@@ -220,6 +220,7 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         a.sig = lv.sig;
         a.xout = lv.x;
         a.omega = (T)nodal_omega(lv.g, o.omega);
+        a.nmask = mask(l);
         launch_stencil<T, T, KS_JFIRST>(lv.g, a, s, false);
         --sweeps;
     }
@@ -232,6 +233,7 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         a.sig = lv.sig;
         a.xout = lv.x2;
         a.omega = (T)nodal_omega(lv.g, o.omega);
+        a.nmask = mask(l);
         if (l == 0 && timer) timer->begin(s);
         launch_stencil<T, T, KS_JACOBI>(lv.g, a, s, l == 0);
         if (l == 0 && timer) timer->end(s);
@@ -376,11 +378,13 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
         a.b = lv.b;
         a.sig = lv.sig;
         a.rout = lv.r;
+        a.nmask = mask(l);
         launch_stencil<T, T, KS_RESID>(lv.g, a, s, false);
         // a coarse level smoothed by its own launches gets its first sweep
-        // from zero out of the restriction
+        // from zero out of the restriction (not with an overset mask: the
+        // fused sweep does not know it)
         MGLevel<T>& lc1 = L[l + 1];
-        ready = (l + 1 < lc) && o.nu1 > 0;
+        ready = (l + 1 < lc) && o.nu1 > 0 && masks.empty();
         launch_restrict<T>(lv.kg, lc1.g, lc1.kg, lv.r, lc1.b, lc1.sig, lc1.coef.diag4,
                            (T)nodal_omega(lc1.g, o.omega), ready ? lc1.x : nullptr, s);
     }
@@ -389,8 +393,9 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
         const long nb = lb.g.nnode();
         const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
         bottom_bicgstab_kernel<T><<<1, threads, 0, s>>>(
-            lb.kg, make_coef<double>(lb.g.dx), lb.sig, lb.b, lb.x, bottom_work.as<double>(),
-            o.bottom_maxiter, o.bottom_rtol, lb.g.singular() ? 1 : 0, bottom_iters);
+            lb.kg, make_coef<double>(lb.g.dx), lb.sig, mask(nl - 1), lb.b, lb.x,
+            bottom_work.as<double>(), o.bottom_maxiter, o.bottom_rtol, bottom_singular() ? 1 : 0,
+            bottom_iters);
     } else {
         CoarseArgs<T> ca{};
         ca.nl = nl - lc;
@@ -405,6 +410,7 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
             ca.L[q].x = lv.x;
             ca.L[q].x2 = lv.x2;
             ca.L[q].r = lv.r;
+            ca.L[q].m = mask(lc + q);
             nmax = std::max(nmax, lv.g.nnode());
         }
         const MGLevel<T>& lb = L[nl - 1];
@@ -414,7 +420,7 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
         ca.work = bottom_work.as<double>();
         ca.maxiter = o.bottom_maxiter;
         ca.rtol = o.bottom_rtol;
-        ca.singular = lb.g.singular() ? 1 : 0;
+        ca.singular = bottom_singular() ? 1 : 0;
         ca.iters_acc = bottom_iters;
         const int threads = nmax >= 512 ? 512 : (nmax >= 256 ? 256 : 64);
         coarse_vcycle_kernel<T><<<1, threads, lds_for(lc), s>>>(ca);
@@ -470,6 +476,8 @@ void NodalSolver::setup(const NodalGeom& g, const MGOpts& o) {
     if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
     H64.built = false;
     H16.built = false;
+    mask_on = mask_fixed = false;  // the handle re-applies its mask after a set-up
+    mask_lv.clear();
     H32.built = false;
     sigma_set = keep_sigma;
     h32_sigma = false;
@@ -522,6 +530,58 @@ static __global__ void scale_cast_kernel(const double* __restrict__ in, TO* __re
                                          double scale) {
     const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id < n) out[id] = st<TO>((acc_t<TO>)(in[id] * scale));
+}
+
+// coarse overset mask by injection: node I takes the fine node 2I
+static __global__ void mask_inject_kernel(KGeom gf, KGeom gc, const unsigned char* __restrict__ mf,
+                                          unsigned char* __restrict__ mc) {
+    int I, J, K;
+    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    mc[I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K)] =
+        mf[2 * I + (long)gf.nn[0] * (2 * J + (long)gf.nn[1] * 2 * K)];
+}
+
+// b = 0 where the mask is 0
+static __global__ void mask_zero_kernel(const unsigned char* __restrict__ m, double* __restrict__ b, long n) {
+    const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n && !m[id]) b[id] = 0.0;
+}
+
+void NodalSolver::set_mask(const unsigned char* m0, hipStream_t s) {
+    // captured V-cycles hold the old mask pointers
+    H64.drop_graphs();
+    H32.drop_graphs();
+    H64.masks.clear();
+    H32.masks.clear();
+    mask_on = false;
+    mask_fixed = false;
+    mask_lv.clear();
+    if (!m0) return;
+    mask_lv.resize(nlev);
+    NodalGeom gl = g0, gprev = g0;
+    for (int l = 0; l < nlev; ++l) {
+        mask_lv[l].alloc(gl.nnode());
+        if (l == 0) {
+            MGX_HIP(hipMemcpyAsync(mask_lv[0].p, m0, gl.nnode(), hipMemcpyDeviceToDevice, s));
+        } else {
+            mask_inject_kernel<<<grid3(gl.nn[0], gl.nn[1], gl.nn[2]), block3(), 0, s>>>(
+                kgeom(gprev), kgeom(gl), mask_lv[l - 1].as<unsigned char>(), mask_lv[l].as<unsigned char>());
+        }
+        gprev = gl;
+        if (l + 1 < nlev) gl = coarsen(gl);
+    }
+    std::vector<unsigned char> h(g0.nnode());
+    MGX_HIP(hipMemcpyAsync(h.data(), m0, g0.nnode(), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    for (unsigned char v : h) if (!v) { mask_fixed = true; break; }
+    // the bottom problem is singular unless its own (injected) mask fixes a node
+    const long nb = gl.nnode();  // gl is the bottom level here
+    std::vector<unsigned char> hb(nb);
+    MGX_HIP(hipMemcpyAsync(hb.data(), mask_lv[nlev - 1].p, nb, hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    mask_bottom_fixed = false;
+    for (unsigned char v : hb) if (!v) { mask_bottom_fixed = true; break; }
+    mask_on = true;
 }
 
 void NodalSolver::prepare_h16(hipStream_t s) {
@@ -708,6 +768,8 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED || mode == MGX_MODE_LEARNED,
                 MGX_ERR_UNSUPPORTED, "nodal solve: unknown mode");
     if (eb && eb->ready) {
+        MGX_REQUIRE(!mask_on, MGX_ERR_UNSUPPORTED,
+                    "nodal solve: an overset mask with embedded boundaries is not supported");
         MGX_REQUIRE(mode == MGX_MODE_FP64, MGX_ERR_UNSUPPORTED,
                     "nodal solve: embedded boundaries run in FP64 mode only");
         eb->set_sigma(sigma0(), s);
@@ -723,12 +785,29 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     bool mixed = (mode != MGX_MODE_FP64);
     // the mixed V-cycle runs on the FP16-stored finest level when there is a
     // level below it; the learned step works on the FP32 hierarchy
-    const bool half = mixed && use_half && nlev >= 2;
+    MGX_REQUIRE(!(mask_on && learned), MGX_ERR_UNSUPPORTED,
+                "nodal solve: the learned mode does not take an overset mask");
+    MGX_REQUIRE(!mask_on || opts.nu2 >= 1, MGX_ERR_UNSUPPORTED,
+                "nodal solve: an overset mask needs at least one post-smoothing sweep");
+    const bool half = mixed && use_half && nlev >= 2 && !mask_on;
     H64.timer = &fine_timer;
     H32.timer = &fine_timer;
     if (mixed && (!half || learned)) prepare_h32(s);
     if (half) prepare_h16(s);
     if (!mixed) prepare_h64(*this, s);
+    // overset mask on every level of the hierarchy in use; a problem with a
+    // fixed node is not singular
+    const unsigned char* mask0 = mask_on ? mask_lv[0].as<unsigned char>() : nullptr;
+    auto attach = [&](auto& H) {
+        H.masks.clear();
+        H.bottom_fixed = mask_bottom_fixed;
+        if (mask_on)
+            for (int l = 0; l < nlev; ++l) H.masks.push_back(mask_lv[l].as<unsigned char>());
+    };
+    if (H32.built) attach(H32);
+    if (H64.built) attach(H64);
+    const bool sing = g0.singular() && !mask_fixed;
+    if (mask0) mask_zero_kernel<<<grid1d(nn), 256, 0, s>>>(mask0, b, nn);
 
     const bool has_dir = g.dlo[0] || g.dlo[1] || g.dlo[2] || g.dhi[0] || g.dhi[1] || g.dhi[2];
     if (has_dir) {
@@ -739,9 +818,9 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         // one pass for sum/min/max of b; ||b - mean||_inf follows from min/max
         const int np = (int)std::min<long>(1024, std::max<long>(1, (nn + 255) / 256));
         stats_kernel<<<np, 256, 0, s>>>(b, nn, part.as<double>(), np);
-        final_stats_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, nn, g.singular() ? 1 : 0,
+        final_stats_kernel<<<1, 256, 0, s>>>(part.as<double>(), np, nn, sing ? 1 : 0,
                                              scal.as<double>() + 33, scal.as<unsigned long long>());
-        if (g.singular())
+        if (sing)
             subtract_scalar_kernel<double><<<grid1d(nn), 256, 0, s>>>(b, nn, scal.as<double>() + 33);
     }
     MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
@@ -752,7 +831,8 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     // sweep from zero (omega r / diag) into the hierarchy's spare level-0
     // array (the live one may be the correction this pass reads).
     auto first_sweep = [&](auto& a, auto& H) {
-        if (opts.nu1 > 0 && H.L.size() > 1) {
+        a.nmask = mask0;
+        if (opts.nu1 > 0 && H.L.size() > 1 && !mask0) {
             a.x1out = H.L[0].x2;
             a.omega1 = nodal_omega(g, opts.omega);
         }
@@ -888,11 +968,12 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
                 mixed = false;
                 prec = P64;
                 prepare_h64(*this, s);
+                attach(H64);
                 rn = residual(P64, rn);
             }
         }
     }
-    if (g.singular()) remove_mean(*this, phi, nn, s);
+    if (sing) remove_mean(*this, phi, nn, s);
     int bot = 0;
     MGX_HIP(hipMemcpyAsync(&bot, ibuf.p, sizeof(int), hipMemcpyDeviceToHost, s));
     MGX_HIP(hipStreamSynchronize(s));

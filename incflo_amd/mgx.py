@@ -73,6 +73,7 @@ def lib():
             "mgx_stream_copy": [vp, vp, vp, C.c_ulonglong],
             "mgx_nodal_set_timing": [vp, C.c_int],
             "mgx_nodal_set_graphs": [vp, C.c_int],
+            "mgx_nodal_set_overset_mask": [vp, vp, C.c_int],
             "mgx_nodal_set_vcycle_bits": [vp, C.c_int],
             "mgx_nodal_get_timing": [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)],
             "mgx_ctx_record": [vp, C.c_int],
@@ -275,6 +276,10 @@ class NodalProjector:
 
     def set_graphs(self, on):
         self.ctx.check(lib().mgx_nodal_set_graphs(self.h, int(on)))
+
+    def setOversetMask(self, mask_dev, ng=0):
+        """mask_dev: device int32 boxed nodal field (or None to remove)"""
+        self.ctx.check(lib().mgx_nodal_set_overset_mask(self.h, mask_dev.ptr if mask_dev else None, ng))
 
     def set_vcycle_bits(self, bits):
         self.ctx.check(lib().mgx_nodal_set_vcycle_bits(self.h, int(bits)))

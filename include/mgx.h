@@ -206,6 +206,11 @@ MGX_API int mgx_nodal_set_eb_cylinder(mgx_nodal h, double cx, double cy, double 
  * covered, 1 regular, 2 cut) and 27 fluid moments int xi^p eta^q zeta^r over
  * the fluid part of the unit cell (index p + 3q + 9r); type NULL removes the EB. */
 MGX_API int mgx_nodal_set_eb_moments(mgx_nodal h, const int* type, const double* moments);
+/* getLinOp().setOversetMask(lev, mask) (src/projection/incflo_apply_nodal_projection.cpp:203-211),
+ * which incflo uses for mixed boundaries (src/boundary_conditions/incflo_set_bcs.cpp:11-53):
+ * a boxed nodal int field (ng ghosts), 1 = solve, 0 = node fixed at phi = 0;
+ * NULL removes it.  Not with embedded boundaries or the learned mode. */
+MGX_API int mgx_nodal_set_overset_mask(mgx_nodal h, const int* mask, int ng);
 /* Host-only: the cylinder's face apertures on a level (canonical face arrays,
  * n_d + 1 faces along d, x fastest) (no GPU). */
 MGX_API int mgx_eb_cylinder_apertures(const mgx_level_desc* level, double cx, double cy, double r,

@@ -470,3 +470,56 @@ def eb_nodal_solve(g, ctype, mom, sigma, b, x0=None, rtol=1e-12, maxiter=20000):
                                    _p(np.ascontiguousarray(b)), _p(x), C.c_double(rtol),
                                    C.c_int(maxiter), C.byref(its))
     return x, its.value, ret
+
+
+def dirichlet_node_mask(g):
+    """1 on nodes the domain boundary conditions leave free, 0 on Dirichlet nodes"""
+    m = np.ones(g.node_shape(), bool)
+    for d, ax in ((0, 2), (1, 1), (2, 0)):
+        if g.periodic[d]:
+            continue
+        sl = [slice(None)] * 3
+        if g.bclo[d] == BC_DIRICHLET:
+            sl[ax] = 0
+            m[tuple(sl)] = False
+        if g.bchi[d] == BC_DIRICHLET:
+            sl[ax] = -1
+            m[tuple(sl)] = False
+    return m
+
+
+def nodal_masked_solve(g, sigma, b, mask, rtol=1e-13, maxiter=20000):
+    """Nodal solve with an overset mask (MLNodeLaplacian::setOversetMask, used by
+    incflo for mixed boundaries, src/projection/incflo_apply_nodal_projection.cpp:203-211):
+    nodes with mask 0 are fixed at phi = 0 and carry no equation.  Jacobi-
+    preconditioned conjugate gradients on the remaining nodes (the operator
+    restricted to them is symmetric negative definite once a node is fixed).
+    Returns (phi, iterations, 0 on success)."""
+    m = np.asarray(mask, bool) & dirichlet_node_mask(g)
+    mf = m.astype(np.float64)
+
+    def A(x):
+        return -mf * nodal_apply(g, sigma, mf * x)
+
+    d = -mf * nodal_diag(g, sigma)
+    d = np.where(m & (d > 0), d, 1.0)
+    r = -mf * np.asarray(b, np.float64)
+    x = np.zeros(g.node_shape())
+    bn = np.abs(r).max()
+    if bn == 0.0:
+        return x, 0, 0
+    z = r / d
+    p = z.copy()
+    rz = np.sum(r * z)
+    for it in range(1, maxiter + 1):
+        q = A(p)
+        alpha = rz / np.sum(p * q)
+        x += alpha * p
+        r -= alpha * q
+        if np.abs(r).max() <= rtol * bn:
+            return x, it, 0
+        z = r / d
+        rz2 = np.sum(r * z)
+        p = z + (rz2 / rz) * p
+        rz = rz2
+    return x, maxiter, 1

@@ -352,3 +352,66 @@ def test_fp16_vcycle_storage_converges_like_fp32(ctx, orc, sigma_scale):
         iters[bits] = info.iters
         P.close()
     assert iters[16] <= iters[32] + 1, iters
+
+
+@pytest.mark.parametrize("mode_name", ["fp64", "mixed"])
+def test_overset_mask_mixed_boundary(ctx, orc, mode_name):
+    """Mixed x-hi face as incflo builds it for the nodal projection
+    (src/boundary_conditions/incflo_set_bcs.cpp:11-53): the face is an inflow
+    (Neumann) boundary and an overset mask fixes phi = 0 on its lower half
+    (outflow part).  GPU projection vs the oracle's masked solve; a mask that
+    covers the whole face equals a Dirichlet face."""
+    from incflo_amd import mgx
+    _say(f"case overset mask {mode_name}")
+    mode = mgx.MODE_FP64 if mode_name == "fp64" else mgx.MODE_MIXED
+    n, per = (32, 16, 16), (0, 0, 1)
+    bclo, bchi = (INF, N, 0), (INF, N, 0)
+    ph = (2.0, 1.0, 1.0)
+    g = _geom(orc, n, per, bclo, bchi, prob_hi=ph)
+    bxs = B.chop_domain(n, 16)
+    rng = np.random.default_rng(8)
+    sigma = 0.01 * (0.5 + rng.random(g.cell_shape()))
+    ug = np.zeros(g.ghost_vec_shape())
+    ug[:, 1:-1, 1:-1, 1:-1] = rng.standard_normal((3,) + g.cell_shape())
+    inflow = np.zeros((6, 3))
+    inflow[0] = [1.0, 0.0, 0.0]   # inflow at x-lo; the mixed x-hi face carries 0
+    orc.fill_vel_ghosts(g, ug, inflow)
+    mask = np.ones(g.node_shape(), bool)
+    mask[:, : g.node_shape()[1] // 2, -1] = False
+    b = orc.nodal_divergence(g, ug)
+    xo, its, ret = orc.nodal_masked_solve(g, sigma, b, mask, rtol=1e-13)
+    assert ret == 0
+    P = _proj(ctx, n, bxs, per, bclo, bchi, prob_hi=ph)
+    P.set_sigma(ctx.array(B.to_boxed(sigma, bxs, 0)), 0)
+    mk = ctx.array(B.to_boxed(mask.astype(np.float64), bxs, 0, nodal=True, periodic=per).astype(np.int32),
+                   np.int32)
+    P.setOversetMask(mk, 0)
+    vel = B.to_boxed(ug[:, 1:-1, 1:-1, 1:-1], bxs, 1, periodic=per)
+    vel = B.set_ghosts_from(vel, bxs, 1, ug)
+    vd = ctx.array(vel)
+    rtol = 1e-12 if mode_name == "fp64" else 1e-10
+    info = P.project(vd, 1, rtol, 0.0, mode=mode)
+    assert info.converged
+    phid = ctx.array(B.field_size(bxs, 0, 1, True))
+    P.getPhi(phid, 0)
+    x = B.from_boxed(phid.download(), bxs, 0, g.node_shape(), nodal=True)
+    assert np.all(x[~mask] == 0.0)
+    assert _rel(x, xo) <= (1e-9 if mode_name == "fp64" else 1e-7)
+    # the whole face masked = a Dirichlet face
+    full = np.ones(g.node_shape(), bool)
+    full[:, :, -1] = False
+    P.setOversetMask(ctx.array(B.to_boxed(full.astype(np.float64), bxs, 0, nodal=True,
+                                          periodic=per).astype(np.int32), np.int32), 0)
+    vd2 = ctx.array(vel)
+    P.project(vd2, 1, 1e-12, 0.0, mode=mgx.MODE_FP64)
+    P.getPhi(phid, 0)
+    xm = B.from_boxed(phid.download(), bxs, 0, g.node_shape(), nodal=True)
+    P.close()
+    Q = _proj(ctx, n, bxs, per, bclo, (D, N, 0), prob_hi=ph)
+    Q.set_sigma(ctx.array(B.to_boxed(sigma, bxs, 0)), 0)
+    vd3 = ctx.array(vel)
+    Q.project(vd3, 1, 1e-12, 0.0, mode=mgx.MODE_FP64)
+    Q.getPhi(phid, 0)
+    xd = B.from_boxed(phid.download(), bxs, 0, g.node_shape(), nodal=True)
+    Q.close()
+    assert _rel(xm, xd) <= 1e-9
