@@ -323,6 +323,37 @@ __device__ __forceinline__ long canon_face(const KGeom& g, int d, int i, int j, 
 }
 
 // canonical faces <- boxed face field (dir = d), or the reverse when to_box
+// The ghost cell just outside the domain face (d, side) of a boxed cell field
+// (ng >= 1), one value per boundary face (plane element t: x fastest over the
+// two remaining directions), read from the box holding the adjacent valid
+// cell; NaN where no box holds it.
+static __global__ void __launch_bounds__(256) gather_face_ghost_kernel(KGeom g, const BoxInfo* boxes,
+                                                                       int nboxes,
+                                                                       const double* __restrict__ in,
+                                                                       int ng, int d, int side,
+                                                                       double* __restrict__ out) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int a0 = d == 0 ? 1 : 0, a1 = d == 2 ? 1 : 2;
+    const long m0 = g.n[a0];
+    if (t >= m0 * g.n[a1]) return;
+    int p[3];
+    p[d] = side ? g.n[d] - 1 : 0;
+    p[a0] = (int)(t % m0);
+    p[a1] = (int)(t / m0);
+    double v = __builtin_nan("");
+    for (int bi = 0; bi < nboxes; ++bi) {
+        const BoxInfo& b = boxes[bi];
+        if (p[0] < b.lo[0] || p[0] >= b.lo[0] + b.len[0] || p[1] < b.lo[1] ||
+            p[1] >= b.lo[1] + b.len[1] || p[2] < b.lo[2] || p[2] >= b.lo[2] + b.len[2])
+            continue;
+        int q[3] = {p[0], p[1], p[2]};
+        q[d] += side ? 1 : -1;
+        v = in[field_offset(b, ng, 1, false) + box_index(b, ng, false, q[0], q[1], q[2], 0)];
+        break;
+    }
+    out[t] = v;
+}
+
 static __global__ void __launch_bounds__(256) face_copy_kernel(KGeom g, const BoxInfo* boxes,
                                                               const Tile* tiles, int d, double* boxed,
                                                               int ng, double* canon, int to_box) {

@@ -1,5 +1,7 @@
 // mac_api.hip -- C ABI of the MAC projection (Hydro::MacProjector surface).
+#include <cmath>
 #include <memory>
+#include <vector>
 
 #include "eb_nodal.h"
 #include "mac_mg.h"
@@ -31,10 +33,10 @@ struct mgx_mac_s {
     BoxLayout layout;
     DevBuf u[3], rhs, cells;
     bool have_phi = false;
-    // embedded boundary: apertures, raw beta (without apertures), covered cells
+    // embedded boundary: apertures and covered cells
     bool eb_on = false;
     double eb_cyl[3] = {0, 0, 0};
-    DevBuf eb_ap[3], eb_braw[3], eb_cov;
+    DevBuf eb_ap[3], eb_cov;
     void build_eb() {
         if (!eb_on) return;
         std::vector<double> ap[3];
@@ -44,28 +46,84 @@ struct mgx_mac_s {
             eb_ap[d].alloc(ap[d].size() * sizeof(double));
             MGX_HIP(hipMemcpy(eb_ap[d].p, ap[d].data(), ap[d].size() * sizeof(double),
                               hipMemcpyHostToDevice));
-            eb_braw[d].alloc(ap[d].size() * sizeof(double));
         }
         eb_cov.alloc(cov.size());
         MGX_HIP(hipMemcpy(eb_cov.p, cov.data(), cov.size(), hipMemcpyHostToDevice));
-    }
-    // after a beta setter: keep the raw beta, hand apertures * beta to the solver
-    void eb_post_beta() {
-        if (!eb_on) return;
-        for (int d = 0; d < 3; ++d) {
-            const long nf = g.nface(d);
-            MGX_HIP(hipMemcpyAsync(eb_braw[d].p, solver.beta(d), nf * sizeof(double),
-                                   hipMemcpyDeviceToDevice, stream()));
-            eb_face_scale_kernel<<<grid1d(nf), 256, 0, stream()>>>(eb_ap[d].as<double>(),
-                                                                    solver.beta(d), nf);
-        }
     }
     void eb_zero_covered() {
         if (eb_on)
             eb_zero_covered_kernel<<<grid1d(g.ncell()), 256, 0, stream()>>>(
                 eb_cov.as<unsigned char>(), solver.phi, g.ncell());
     }
-    const double* raw_beta(int d) { return eb_on ? eb_braw[d].as<double>() : solver.beta(d); }
+
+    // Robin faces (setLevelBC robin_a / robin_b / robin_f): per side, the
+    // beta scale a h / (a h + 2 b) and the constant c = f / (a h/2 + b)
+    bool rob_set = false;    // coefficients received for the current BCs
+    bool rob_has_f = false;  // some f != 0
+    bool rob_fixes = false;  // some a > 0 (the problem is not singular)
+    DevBuf rob_c[3][2];
+    bool robin_side(int d, int side) const {
+        return !periodic[d] && (side ? bchi[d] : bclo[d]) == MGX_BC_ROBIN;
+    }
+    bool any_robin() const {
+        for (int d = 0; d < 3; ++d)
+            if (robin_side(d, 0) || robin_side(d, 1)) return true;
+        return false;
+    }
+    long plane(int d) const { return (long)g.n[d == 0 ? 1 : 0] * g.n[d == 2 ? 1 : 2]; }
+
+    // The caller's face coefficients.  With apertures the solver runs on
+    // a beta and braw keeps the caller's; otherwise the solver's arrays are
+    // the caller's.  (The solver scales the Robin planes itself, per level.)
+    DevBuf braw[3];
+    bool raw_valid = false;  // braw holds the caller's current beta
+    bool need_raw() const { return eb_on; }
+    double* beta_dst(int d) {
+        if (!need_raw()) return solver.beta(d);
+        braw[d].alloc(g.nface(d) * sizeof(double));
+        return braw[d].as<double>();
+    }
+    const double* raw_beta(int d) { return raw_valid ? braw[d].as<double>() : solver.beta(d); }
+    // solver beta = caller beta x apertures
+    void derive_beta() {
+        if (!need_raw()) {
+            if (raw_valid)  // back to the caller's values
+                for (int d = 0; d < 3; ++d)
+                    MGX_HIP(hipMemcpyAsync(solver.beta(d), braw[d].p, g.nface(d) * sizeof(double),
+                                           hipMemcpyDeviceToDevice, stream()));
+            raw_valid = false;
+            return;
+        }
+        for (int d = 0; d < 3; ++d) {
+            const long nf = g.nface(d);
+            MGX_HIP(hipMemcpyAsync(solver.beta(d), braw[d].p, nf * sizeof(double),
+                                   hipMemcpyDeviceToDevice, stream()));
+            eb_face_scale_kernel<<<grid1d(nf), 256, 0, stream()>>>(eb_ap[d].as<double>(),
+                                                                    solver.beta(d), nf);
+        }
+    }
+    // after a beta setter wrote beta_dst()
+    void post_beta() {
+        raw_valid = need_raw();
+        derive_beta();
+        solver.beta_changed();
+    }
+    void robin_rhs(double* rhs) {
+        if (!rob_has_f) return;
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side)
+                if (robin_side(d, side))
+                    robin_rhs_kernel<<<grid1d(plane(d)), 256, 0, stream()>>>(
+                        g.k(), d, side, raw_beta(d), rob_c[d][side].as<double>(), 1.0 / g.dx[d],
+                        rhs);
+    }
+    void robin_flux(int d, double* u) {
+        if (!rob_has_f) return;
+        for (int side = 0; side < 2; ++side)
+            if (robin_side(d, side))
+                robin_flux_kernel<<<grid1d(plane(d)), 256, 0, stream()>>>(
+                    g.k(), d, side, raw_beta(d), rob_c[d][side].as<double>(), u);
+    }
 
     KGeom kg() const {
         KGeom k{};
@@ -86,8 +144,10 @@ struct mgx_mac_s {
         for (int d = 0; d < 3; ++d) {
             g.n[d] = n_cell[d];
             g.per[d] = periodic[d];
-            g.dlo[d] = (!periodic[d] && bclo[d] == MGX_BC_DIRICHLET) ? 1 : 0;
-            g.dhi[d] = (!periodic[d] && bchi[d] == MGX_BC_DIRICHLET) ? 1 : 0;
+            g.dlo[d] = (!periodic[d] && (bclo[d] == MGX_BC_DIRICHLET || bclo[d] == MGX_BC_ROBIN)) ? 1 : 0;
+            g.dhi[d] = (!periodic[d] && (bchi[d] == MGX_BC_DIRICHLET || bchi[d] == MGX_BC_ROBIN)) ? 1 : 0;
+            g.rlo[d] = robin_side(d, 0) ? 1 : 0;
+            g.rhi[d] = robin_side(d, 1) ? 1 : 0;
             g.dx[d] = (prob_hi[d] - prob_lo[d]) / n_cell[d];
         }
         solver.setup(g, opts);
@@ -98,9 +158,26 @@ struct mgx_mac_s {
         for (int d = 0; d < 3; ++d) u[d].alloc(g.nface(d) * sizeof(double));
         rhs.alloc(g.ncell() * sizeof(double));
         cells.alloc(g.ncell() * sizeof(double));
+        g.robin_fixes = rob_set && rob_fixes;
+        solver.set_robin_fixes(g.robin_fixes);
         have_phi = false;
         dirty = false;
         build_eb();
+        // coefficients kept across the set-up: re-derive the solver's
+        if (solver.beta_set) {
+            if (!raw_valid && need_raw()) {
+                for (int d = 0; d < 3; ++d) {
+                    braw[d].alloc(g.nface(d) * sizeof(double));
+                    MGX_HIP(hipMemcpyAsync(braw[d].p, solver.beta(d), g.nface(d) * sizeof(double),
+                                           hipMemcpyDeviceToDevice, stream()));
+                }
+                raw_valid = true;
+            }
+            derive_beta();
+            solver.beta_changed();
+        } else {
+            raw_valid = false;
+        }
     }
     void faces_in(int d, const double* boxed, int ng, double* canon) {
         face_copy_kernel<<<layout.ntiles[REG_FACE0 + d], 256, 0, stream()>>>(
@@ -178,12 +255,14 @@ MGX_API int mgx_mac_set_domain_bc(mgx_mac h, const int lo[3], const int hi[3]) {
                             "MAC projector: undefined BC type");
                 MGX_REQUIRE((bc == MGX_BC_PERIODIC) == (h->periodic[d] != 0), MGX_ERR_ARG,
                             "MAC projector: periodic BC must match the geometry's periodicity");
-                MGX_REQUIRE(bc != MGX_BC_ROBIN, MGX_ERR_UNSUPPORTED,
-                            "MAC projector: Robin (mixed) boundaries are not implemented");
+                MGX_REQUIRE(bc != MGX_BC_ROBIN || !h->eb_on, MGX_ERR_UNSUPPORTED,
+                            "MAC projector: Robin faces with an embedded boundary are not supported");
             }
         for (int d = 0; d < 3; ++d) { h->bclo[d] = lo[d]; h->bchi[d] = hi[d]; }
         h->bc_set = true;
         h->dirty = true;
+        h->rob_set = h->rob_has_f = h->rob_fixes = false;  // setLevelBC follows
+        h->solver.clear_robin();
     });
 }
 
@@ -213,9 +292,8 @@ MGX_API int mgx_mac_set_beta(mgx_mac h, const double* bx, const double* by, cons
         MGX_REQUIRE(bx && by && bz && ng >= 0, MGX_ERR_ARG, "bad beta fields");
         h->ensure_setup();
         const double* b[3] = {bx, by, bz};
-        for (int d = 0; d < 3; ++d) h->faces_in(d, b[d], ng, h->solver.beta(d));
-        h->eb_post_beta();
-        h->solver.beta_changed();
+        for (int d = 0; d < 3; ++d) h->faces_in(d, b[d], ng, h->beta_dst(d));
+        h->post_beta();
     });
 }
 
@@ -226,10 +304,9 @@ MGX_API int mgx_mac_set_const_beta(mgx_mac h, double beta) {
         h->ensure_setup();
         for (int d = 0; d < 3; ++d) {
             const long nf = h->g.nface(d);
-            fill_kernel<<<grid1d(nf), 256, 0, h->stream()>>>(h->solver.beta(d), nf, beta);
+            fill_kernel<<<grid1d(nf), 256, 0, h->stream()>>>(h->beta_dst(d), nf, beta);
         }
-        h->eb_post_beta();
-        h->solver.beta_changed();
+        h->post_beta();
     });
 }
 
@@ -241,9 +318,8 @@ MGX_API int mgx_mac_set_beta_from_density(mgx_mac h, const double* rho, int rho_
         h->cells_in(rho, rho_ng, h->cells.as<double>());
         for (int d = 0; d < 3; ++d)
             mac_beta_from_rho_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
-                h->g.k(), d, h->cells.as<double>(), dt, h->solver.beta(d));
-        h->eb_post_beta();
-        h->solver.beta_changed();
+                h->g.k(), d, h->cells.as<double>(), dt, h->beta_dst(d));
+        h->post_beta();
     });
 }
 
@@ -256,6 +332,8 @@ MGX_API int mgx_mac_project(mgx_mac h, double* u, double* v, double* w, int umac
                     "MAC projection modes: FP64 or mixed");
         h->ensure_setup();
         MGX_REQUIRE(h->solver.beta_set, MGX_ERR_STATE, "MAC projector: beta not set");
+        MGX_REQUIRE(h->rob_set || !h->any_robin(), MGX_ERR_STATE,
+                    "MAC projector: Robin faces need setLevelBC coefficients");
         double* U[3] = {u, v, w};
         for (int d = 0; d < 3; ++d) h->faces_in(d, U[d], umac_ng, h->u[d].as<double>());
         const MGeomK k = h->g.k();
@@ -268,6 +346,7 @@ MGX_API int mgx_mac_project(mgx_mac h, double* u, double* v, double* w, int umac
             mac_div_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
                 k, h->u[0].as<double>(), h->u[1].as<double>(), h->u[2].as<double>(),
                 1.0 / h->g.dx[0], 1.0 / h->g.dx[1], 1.0 / h->g.dx[2], h->rhs.as<double>());
+        h->robin_rhs(h->rhs.as<double>());
         if (phi) h->cells_in(phi, phi_ng, h->solver.phi);
         else MGX_HIP(hipMemsetAsync(h->solver.phi, 0, h->g.ncell() * sizeof(double), h->stream()));
         h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
@@ -279,8 +358,10 @@ MGX_API int mgx_mac_project(mgx_mac h, double* u, double* v, double* w, int umac
                     1.0 / h->g.dx[d], h->u[d].as<double>(), 0);
             else
                 mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
-                    k, d, h->solver.beta(d), h->solver.phi, 1.0 / h->g.dx[d], h->u[d].as<double>(),
+                    k, d, h->solver.fine_beta(d, h->stream()), h->solver.phi, 1.0 / h->g.dx[d],
+                    h->u[d].as<double>(),
                     0);
+            h->robin_flux(d, h->u[d].as<double>());
             h->faces_out(d, U[d], umac_ng, h->u[d].as<double>());
         }
         if (phi) h->cells_out(phi, phi_ng, h->solver.phi);
@@ -307,8 +388,10 @@ MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, in
                     1.0 / h->g.dx[d], h->u[d].as<double>(), 1);
             else
                 mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
-                    k, d, h->solver.beta(d), h->solver.phi, 1.0 / h->g.dx[d], h->u[d].as<double>(),
+                    k, d, h->solver.fine_beta(d, h->stream()), h->solver.phi, 1.0 / h->g.dx[d],
+                    h->u[d].as<double>(),
                     1);
+            h->robin_flux(d, h->u[d].as<double>());
             h->faces_out(d, F[d], ng, h->u[d].as<double>());
         }
     });
@@ -322,7 +405,10 @@ MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs,
         MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_ARG,
                     "MAC solve modes: FP64 or mixed");
         h->ensure_setup();
+        MGX_REQUIRE(h->rob_set || !h->any_robin(), MGX_ERR_STATE,
+                    "MAC solve: Robin faces need setLevelBC coefficients");
         h->cells_in(rhs, rhs_ng, h->rhs.as<double>());
+        h->robin_rhs(h->rhs.as<double>());
         h->cells_in(phi, phi_ng, h->solver.phi);
         h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
         h->eb_zero_covered();
@@ -334,6 +420,57 @@ MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs,
         status = MGX_ERR_NOT_CONVERGED;
     }
     return status;
+}
+
+MGX_API int mgx_mac_set_level_bc(mgx_mac h, const double* robin_a, const double* robin_b,
+                                 const double* robin_f, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(robin_a && robin_b && robin_f && ng >= 1, MGX_ERR_ARG,
+                    "setLevelBC: robin_a / robin_b / robin_f need at least one ghost cell");
+        h->ensure_setup();
+        const KGeom kg = h->kg();
+        bool fixes = false, has_f = false;
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                if (!h->robin_side(d, side)) continue;
+                const long np = h->plane(d);
+                DevBuf tmp;
+                tmp.alloc(3 * np * sizeof(double));
+                const double* src[3] = {robin_a, robin_b, robin_f};
+                for (int q = 0; q < 3; ++q)
+                    gather_face_ghost_kernel<<<grid1d(np), 256, 0, h->stream()>>>(
+                        kg, h->layout.dev_boxes(), h->layout.nboxes, src[q], ng, d, side,
+                        tmp.as<double>() + q * np);
+                std::vector<double> v(3 * np), c(np);
+                MGX_HIP(hipMemcpyAsync(v.data(), tmp.p, 3 * np * sizeof(double),
+                                       hipMemcpyDeviceToHost, h->stream()));
+                MGX_HIP(hipStreamSynchronize(h->stream()));
+                const double hd = h->g.dx[d];
+                for (long t = 0; t < np; ++t) {
+                    const double a = v[t], b = v[np + t], f = v[2 * np + t];
+                    MGX_REQUIRE(std::isfinite(a) && std::isfinite(b) && std::isfinite(f) && a >= 0.0 &&
+                                    b >= 0.0 && a * hd / 2 + b > 0.0,
+                                MGX_ERR_ARG,
+                                "setLevelBC: Robin coefficients need a, b >= 0 and a h/2 + b > 0 "
+                                "on every face of a Robin side");
+                    c[t] = f / (a * hd / 2 + b);
+                    if (a > 0.0) fixes = true;
+                    if (f != 0.0) has_f = true;
+                }
+                h->rob_c[d][side].alloc(np * sizeof(double));
+                MGX_HIP(hipMemcpyAsync(h->rob_c[d][side].p, c.data(), np * sizeof(double),
+                                       hipMemcpyHostToDevice, h->stream()));
+                MGX_HIP(hipStreamSynchronize(h->stream()));
+                h->solver.set_robin(d, side, std::vector<double>(v.begin(), v.begin() + np),
+                                    std::vector<double>(v.begin() + np, v.begin() + 2 * np));
+            }
+        h->rob_set = h->any_robin();
+        h->rob_fixes = fixes;
+        h->rob_has_f = has_f;
+        h->g.robin_fixes = h->rob_set && fixes;
+        h->solver.set_robin_fixes(h->g.robin_fixes);
+    });
 }
 
 MGX_API int mgx_mac_set_eb_cylinder(mgx_mac h, double cx, double cy, double r) {
@@ -348,6 +485,8 @@ MGX_API int mgx_mac_set_eb_cylinder(mgx_mac h, double cx, double cy, double r) {
                 dir = true;
         MGX_REQUIRE(dir, MGX_ERR_UNSUPPORTED,
                     "MAC projector with an embedded boundary needs a Dirichlet (outflow) face");
+        MGX_REQUIRE(!h->any_robin(), MGX_ERR_UNSUPPORTED,
+                    "MAC projector: Robin faces with an embedded boundary are not supported");
         h->eb_on = true;
         h->eb_cyl[0] = cx;
         h->eb_cyl[1] = cy;

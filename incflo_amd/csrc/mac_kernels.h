@@ -342,6 +342,70 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
     if (tid == 0 && iters_acc) atomicAdd(iters_acc, it);
 }
 
+// ---- Robin domain faces: a phi + b dphi/dn = f ---------------------------
+// The face value of phi is the mean of the boundary cell and its ghost and
+// the outward derivative their difference over h, so the relation fixes
+//   dphi/dn = (f - a phi_0) / (a h/2 + b).
+// The linear part is the Dirichlet-face flux 2 beta' phi_0 / h with the face
+// coefficient scaled to beta' = beta a h / (a h + 2 b) (a Dirichlet face at
+// a = 1, b = 0; a Neumann face at a = 0), so the operator kernels run
+// unchanged with the side marked as a Dirichlet side.  The constant part
+// beta c, c = f / (a h/2 + b), enters the right-hand side of the boundary
+// cell (-beta c / h) and the face flux (+beta c on a high face, -beta c on a
+// low one).  Planes hold one value per boundary face, x fastest over the two
+// remaining directions.
+
+// plane element t of side (d, side) -> the face of the face array of direction d
+__device__ __forceinline__ bool robin_face(const MGeomK& g, int d, int side, long t, int& i, int& j,
+                                           int& k) {
+    const int a0 = d == 0 ? 1 : 0, a1 = d == 2 ? 1 : 2;  // the two remaining directions
+    const long m0 = g.n[a0];
+    if (t >= m0 * g.n[a1]) return false;
+    int p[3];
+    p[d] = side ? g.n[d] : 0;
+    p[a0] = (int)(t % m0);
+    p[a1] = (int)(t / m0);
+    i = p[0];
+    j = p[1];
+    k = p[2];
+    return true;
+}
+
+// beta *= scale on the faces of side (d, side)
+template <class T>
+static __global__ void robin_beta_kernel(MGeomK g, int d, int side, T* __restrict__ beta,
+                                         const double* __restrict__ scale) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!robin_face(g, d, side, t, i, j, k)) return;
+    const long f = fidx(g, d, i, j, k);
+    beta[f] = (T)((double)beta[f] * scale[t]);
+}
+
+// rhs of the boundary cell -= beta c / h  (beta: the unscaled face array)
+static __global__ void robin_rhs_kernel(MGeomK g, int d, int side, const double* __restrict__ beta,
+                                        const double* __restrict__ c, double ih,
+                                        double* __restrict__ rhs) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!robin_face(g, d, side, t, i, j, k)) return;
+    int p[3] = {i, j, k};
+    if (side) p[d] -= 1;
+    rhs[cidx(g, p[0], p[1], p[2])] -= beta[fidx(g, d, i, j, k)] * c[t] * ih;
+}
+
+// the constant part of the face flux: u -= (+-) beta c (projection), and the
+// same for getFluxes' -F
+static __global__ void robin_flux_kernel(MGeomK g, int d, int side, const double* __restrict__ beta,
+                                         const double* __restrict__ c, double* __restrict__ u) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!robin_face(g, d, side, t, i, j, k)) return;
+    const long f = fidx(g, d, i, j, k);
+    const double F = beta[f] * c[t];
+    u[f] -= side ? F : -F;
+}
+
 // ---- embedded boundary (cut cells) -------------------------------------
 // With face apertures a_f (fluid fraction of each face) the EB operator in
 // symmetric form is the regular operator with face coefficients a_f beta_f;

@@ -12,7 +12,9 @@ namespace mgx {
 struct MacGeom {
     int n[3];
     int per[3];
-    int dlo[3], dhi[3];
+    int dlo[3], dhi[3];  // Dirichlet-form sides (Dirichlet, or Robin with scaled beta)
+    int rlo[3], rhi[3];  // of those, the Robin sides
+    bool robin_fixes;    // some Robin face has a > 0
     double dx[3];
     long ncell() const { return (long)n[0] * n[1] * n[2]; }
     long nface(int d) const {
@@ -20,8 +22,8 @@ struct MacGeom {
     }
     bool singular() const {
         for (int d = 0; d < 3; ++d)
-            if (!per[d] && (dlo[d] || dhi[d])) return false;
-        return true;
+            if (!per[d] && ((dlo[d] && !rlo[d]) || (dhi[d] && !rhi[d]))) return false;
+        return !robin_fixes;
     }
     MGeomK k() const {
         MGeomK g{};
@@ -66,8 +68,11 @@ class MacHierarchy {
     DevBuf bottom_work;
     bool built = false;
     void build(const MacGeom& g0, int nlev);
-    // level-0 beta from FP64 face arrays, then coarsen
-    void set_beta(const double* const beta0[3], hipStream_t s);
+    // level-0 beta from FP64 face arrays, then coarsen; with Robin sides the
+    // boundary planes of each level are then scaled by rscale(l, d, side)
+    // (null: not a Robin side)
+    void set_beta(const double* const beta0[3], hipStream_t s,
+                  const std::vector<const double*>* rscale = nullptr);
     void vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters);
 };
 
@@ -90,7 +95,32 @@ class MacSolver {
     ~MacSolver();
     void setup(const MacGeom& g, const MGOpts& o);
     double* beta(int d) { return beta0[d].as<double>(); }
-    void beta_changed() { beta_set = true; h32_beta = false; h64_beta = false; }
+    void beta_changed() { beta_set = true; h32_beta = false; h64_beta = false; beta0s_ok = false; }
+    // Robin data changed whether the problem is singular: rebuild the levels
+    void set_robin_fixes(bool v) {
+        g0.robin_fixes = v;
+        H64.built = H32.built = false;
+    }
+    // Robin sides: level-0 a and b per boundary face (x fastest over the two
+    // remaining directions).  Each level's boundary coefficient is its
+    // face-averaged beta times a_l h_l / (a_l h_l + 2 b_l), with a_l, b_l the
+    // face averages of a, b (the Robin relation on that level's mesh).
+    void set_robin(int d, int side, std::vector<double> a, std::vector<double> b);
+    void clear_robin();
+    // level-0 beta with the Robin planes scaled (the operator's face coefficient)
+    const double* fine_beta(int d, hipStream_t s);
+
+   private:
+    bool robin_on = false;
+    std::vector<double> rob_a[3][2], rob_b[3][2];
+    std::vector<DevBuf> rob_scale[3][2];  // per level
+    std::vector<const double*> rob_ptr;   // [l * 6 + 2 d + side]
+    int rob_levels = 0;                   // levels rob_scale was built for
+    DevBuf beta0s[3];
+    bool beta0s_ok = false;
+    void robin_prepare(hipStream_t s);
+
+   public:
     // Solve L phi = b (b modified in place: mean removed when singular);
     // initial guess this->phi.
     void solve(double* b, double rtol, double atol, int mode, mgx_solve_info* info, hipStream_t s);

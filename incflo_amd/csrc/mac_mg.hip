@@ -39,7 +39,8 @@ void MacHierarchy<T>::build(const MacGeom& g0, int nlev) {
 }
 
 template <class T>
-void MacHierarchy<T>::set_beta(const double* const b0[3], hipStream_t s) {
+void MacHierarchy<T>::set_beta(const double* const b0[3], hipStream_t s,
+                               const std::vector<const double*>* rscale) {
     for (int d = 0; d < 3; ++d) {
         const long nf = L[0].g.nface(d);
         cast_kernel<T, double><<<grid1d(nf), 256, 0, s>>>(b0[d], L[0].beta[d], nf);
@@ -51,6 +52,18 @@ void MacHierarchy<T>::set_beta(const double* const b0[3], hipStream_t s) {
                 <<<grid3(gc.n[0] + (d == 0), gc.n[1] + (d == 1), gc.n[2] + (d == 2)), block3(), 0, s>>>(
                     L[l - 1].kg, L[l].kg, d, L[l - 1].beta[d], L[l].beta[d]);
         }
+    // Robin planes last: every level was coarsened from unscaled values
+    if (rscale)
+        for (size_t l = 0; l < L.size(); ++l)
+            for (int d = 0; d < 3; ++d)
+                for (int side = 0; side < 2; ++side) {
+                    const double* sc = (*rscale)[l * 6 + 2 * d + side];
+                    if (!sc) continue;
+                    const MacGeom& gl = L[l].g;
+                    const long np = (long)gl.n[d == 0 ? 1 : 0] * gl.n[d == 2 ? 1 : 2];
+                    robin_beta_kernel<T><<<grid1d(np), 256, 0, s>>>(L[l].kg, d, side, L[l].beta[d],
+                                                                    sc);
+                }
 }
 
 template <class T>
@@ -129,6 +142,101 @@ void MacSolver::setup(const MacGeom& g, const MGOpts& o) {
     H64.built = H32.built = false;
     beta_set = keep_beta;
     h32_beta = h64_beta = false;
+    rob_levels = 0;  // the level count may have changed
+    beta0s_ok = false;
+}
+
+void MacSolver::set_robin(int d, int side, std::vector<double> a, std::vector<double> b) {
+    rob_a[d][side] = std::move(a);
+    rob_b[d][side] = std::move(b);
+    robin_on = true;
+    rob_levels = 0;
+    beta0s_ok = false;
+    h32_beta = h64_beta = false;
+}
+
+void MacSolver::clear_robin() {
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            rob_a[d][side].clear();
+            rob_b[d][side].clear();
+            rob_scale[d][side].clear();
+        }
+    robin_on = false;
+    rob_levels = 0;
+    rob_ptr.clear();
+    beta0s_ok = false;
+    h32_beta = h64_beta = false;
+}
+
+void MacSolver::robin_prepare(hipStream_t s) {
+    if (!robin_on) return;
+    if (rob_levels != nlev) {
+        rob_ptr.assign((size_t)nlev * 6, nullptr);
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                rob_scale[d][side].clear();
+                if (rob_a[d][side].empty()) continue;
+                rob_scale[d][side].resize(nlev);
+                const int a0 = d == 0 ? 1 : 0, a1 = d == 2 ? 1 : 2;
+                int m0 = g0.n[a0], m1 = g0.n[a1];
+                std::vector<double> a = rob_a[d][side], b = rob_b[d][side];
+                double h = g0.dx[d];
+                for (int l = 0; l < nlev; ++l) {
+                    if (l > 0) {  // face averages over 2 x 2 fine faces
+                        const int c0 = m0 / 2, c1 = m1 / 2;
+                        std::vector<double> ac((size_t)c0 * c1), bc((size_t)c0 * c1);
+                        for (int q = 0; q < c1; ++q)
+                            for (int p = 0; p < c0; ++p) {
+                                double sa = 0, sb = 0;
+                                for (int e = 0; e < 4; ++e) {
+                                    const long f = (2 * p + (e & 1)) + (long)m0 * (2 * q + (e >> 1));
+                                    sa += a[f];
+                                    sb += b[f];
+                                }
+                                ac[p + (long)c0 * q] = 0.25 * sa;
+                                bc[p + (long)c0 * q] = 0.25 * sb;
+                            }
+                        a.swap(ac);
+                        b.swap(bc);
+                        m0 = c0;
+                        m1 = c1;
+                        h *= 2;
+                    }
+                    std::vector<double> sc(a.size());
+                    for (size_t t = 0; t < a.size(); ++t) sc[t] = a[t] * h / (a[t] * h + 2.0 * b[t]);
+                    DevBuf& buf = rob_scale[d][side][l];
+                    buf.alloc(sc.size() * sizeof(double));
+                    MGX_HIP(hipMemcpyAsync(buf.p, sc.data(), sc.size() * sizeof(double),
+                                           hipMemcpyHostToDevice, s));
+                    MGX_HIP(hipStreamSynchronize(s));
+                    rob_ptr[(size_t)l * 6 + 2 * d + side] = buf.as<double>();
+                }
+            }
+        rob_levels = nlev;
+        beta0s_ok = false;
+    }
+    if (!beta0s_ok) {
+        for (int d = 0; d < 3; ++d) {
+            const long nf = g0.nface(d);
+            beta0s[d].alloc(nf * sizeof(double));
+            MGX_HIP(hipMemcpyAsync(beta0s[d].p, beta(d), nf * sizeof(double), hipMemcpyDeviceToDevice, s));
+            for (int side = 0; side < 2; ++side) {
+                const double* sc = rob_ptr[2 * d + side];
+                if (!sc) continue;
+                const long np = (long)g0.n[d == 0 ? 1 : 0] * g0.n[d == 2 ? 1 : 2];
+                robin_beta_kernel<double><<<grid1d(np), 256, 0, s>>>(g0.k(), d, side,
+                                                                     beta0s[d].as<double>(), sc);
+            }
+        }
+        beta0s_ok = true;
+    }
+}
+
+const double* MacSolver::fine_beta(int d, hipStream_t s) {
+    if (!robin_on) return beta(d);
+    robin_prepare(s);
+    return beta0s[d].as<double>();
 }
 
 double MacSolver::read_norm(int slot, hipStream_t s) {
@@ -149,15 +257,20 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     const MGeomK kg = g.k();
     const long n = g.ncell();
     const bool mixed = (mode == MGX_MODE_MIXED);
-    const double* b0[3] = {beta(0), beta(1), beta(2)};
+    robin_prepare(s);
+    // unscaled coefficients feed the level coarsening; the operator uses the
+    // Robin-scaled level-0 ones
+    const double* braw0[3] = {beta(0), beta(1), beta(2)};
+    const double* b0[3] = {fine_beta(0, s), fine_beta(1, s), fine_beta(2, s)};
+    const std::vector<const double*>* rs = robin_on ? &rob_ptr : nullptr;
     if (mixed) {
         if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g, nlev); h32_beta = false; }
-        if (!h32_beta) { H32.set_beta(b0, s); h32_beta = true; }
+        if (!h32_beta) { H32.set_beta(braw0, s, rs); h32_beta = true; }
         res32.alloc(n * sizeof(float));
         H32.L[0].b = res32.as<float>();
     } else {
         if (!H64.built || (int)H64.L.size() != nlev) { H64.build(g, nlev); h64_beta = false; }
-        if (!h64_beta) { H64.set_beta(b0, s); h64_beta = true; }
+        if (!h64_beta) { H64.set_beta(braw0, s, rs); h64_beta = true; }
         res64.alloc(n * sizeof(double));
         H64.L[0].b = res64.as<double>();
     }

@@ -128,6 +128,7 @@ def lib():
             "mgx_eb_cylinder_geometry": [C.POINTER(LevelDesc), C.c_double, C.c_double, C.c_double,
                                          vp, vp],
             "mgx_mac_set_eb_cylinder": [vp, C.c_double, C.c_double, C.c_double],
+            "mgx_mac_set_level_bc": [vp, vp, vp, vp, C.c_int],
             "mgx_eb_cylinder_apertures": [C.POINTER(LevelDesc), C.c_double, C.c_double, C.c_double,
                                           vp, vp, vp],
             "mgx_nodal_set_unet": [vp, vp, C.c_int],
@@ -393,6 +394,12 @@ class MacProjector:
     def set_eb_cylinder(self, cx, cy, r):
         """Embedded cylinder along z (fluid outside); before the coefficients."""
         self.ctx.check(lib().mgx_mac_set_eb_cylinder(self.h, cx, cy, r))
+
+    def setLevelBC(self, robin_a, robin_b, robin_f, ng=1):
+        """Robin coefficients a phi + b dphi/dn = f, boxed cell fields whose
+        first ghost cell outside each Robin face holds the values."""
+        self.ctx.check(lib().mgx_mac_set_level_bc(self.h, robin_a.ptr, robin_b.ptr, robin_f.ptr,
+                                                  ng))
 
     def getFluxes(self, fx, fy, fz, ng=0):
         self.ctx.check(lib().mgx_mac_get_fluxes(self.h, fx.ptr, fy.ptr, fz.ptr, ng))

@@ -258,8 +258,18 @@ MGX_API int mgx_mac_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_mac* ou
 MGX_API int mgx_mac_destroy(mgx_mac h);
 /* setDomainBC (src/convection/incflo_compute_MAC_projected_velocities.cpp:114) with the
  * get_mac_projection_bc mapping (src/projection/incflo_projection_bc.cpp:44-78):
- * DIRICHLET for pressure in/outflow, NEUMANN (or INFLOW) for walls and mass inflow. */
+ * DIRICHLET for pressure in/outflow, NEUMANN (or INFLOW) for walls and mass inflow,
+ * ROBIN for mixed faces (coefficients from mgx_mac_set_level_bc). */
 MGX_API int mgx_mac_set_domain_bc(mgx_mac h, const int lo[3], const int hi[3]);
+/* setLevelBC(lev, nullptr, &robin_a, &robin_b, &robin_f)
+ * (src/convection/incflo_compute_MAC_projected_velocities.cpp:115-122, data from
+ * src/boundary_conditions/incflo_set_bcs.cpp:122-190): coefficients of
+ * a phi + b dphi/dn = f on the MGX_BC_ROBIN sides, read from the first ghost
+ * cell outside each such face of three boxed cell fields (ng >= 1 ghosts).
+ * Needs a >= 0, b >= 0, a h/2 + b > 0 there.  Call after setDomainBC (which
+ * clears them); the solve refuses Robin sides without them. */
+MGX_API int mgx_mac_set_level_bc(mgx_mac h, const double* robin_a, const double* robin_b,
+                                 const double* robin_f, int ng);
 /* mac_proj.* knobs (maxiter / bottom_maxiter default 200). */
 MGX_API int mgx_mac_set_options(mgx_mac h, const mgx_mlmg_opts* opts);
 /* initProjector(lp_info, beta) / updateCoeffs(beta) (:112, :131): face coefficient

@@ -266,6 +266,14 @@ class MacProjector {
         }
         ctx_.check(mgx_mac_set_domain_bc(need(), l, r), "setDomainBC");
     }
+    // setLevelBC(lev, nullptr, &robin_a, &robin_b, &robin_f): Robin (mixed) face data
+    void setLevelBC(int lev, const MultiFab* levelbcdata, const MultiFab* robin_a,
+                    const MultiFab* robin_b, const MultiFab* robin_f) {
+        if (lev != 0 || levelbcdata) Abort("MacProjector::setLevelBC: level 0, no Dirichlet data");
+        ctx_.check(mgx_mac_set_level_bc(need(), robin_a->p, robin_b->p, robin_f->p,
+                                        robin_a->ngrow),
+                   "setLevelBC");
+    }
     void updateBeta(double beta) { ctx_.check(mgx_mac_set_const_beta(need(), beta), "updateBeta"); }
     void updateCoeffs(std::vector<std::array<MultiFab const*, 3>> beta) {
         detail::one_level(beta.size(), "MacProjector");

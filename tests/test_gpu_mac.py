@@ -135,12 +135,140 @@ def test_mac_error_paths(ctx):
     _say("case mac error paths")
     bxs = B.chop_domain((16, 16, 16), 8)
     P = mgx.MacProjector(ctx, mgx.make_level((16, 16, 16), bxs, periodic=(0, 1, 1)))
-    with pytest.raises(mgx.MgxError) as e:
-        P.setDomainBC((mgx.BC_ROBIN, 0, 0), (D_, 0, 0))
-    assert e.value.code == mgx.MGX_ERR_UNSUPPORTED
     P.setDomainBC((N_, 0, 0), (D_, 0, 0))
     ud = [ctx.array(B.face_field_size(bxs, 0, d)) for d in range(3)]
     with pytest.raises(mgx.MgxError) as e:
         P.project(ud[0], ud[1], ud[2], 0, 1e-10)
     assert e.value.code == mgx.MGX_ERR_STATE
+    # a Robin side needs setLevelBC's coefficients before a solve
+    P.setDomainBC((mgx.BC_ROBIN, 0, 0), (D_, 0, 0))
+    P.updateBeta(1.0)
+    with pytest.raises(mgx.MgxError) as e:
+        P.project(ud[0], ud[1], ud[2], 0, 1e-10)
+    assert e.value.code == mgx.MGX_ERR_STATE
+    # b < 0 is refused
+    A = np.zeros((1, 18, 18, 18))
+    A[0, :, :, 0] = 1.0
+    Bv = np.full((1, 18, 18, 18), -1.0)
+    bo = [ctx.array(B.set_ghosts_from(np.zeros(B.field_size(bxs, 1)), bxs, 1, X, ncomp=1))
+          for X in (A, Bv, np.zeros_like(A))]
+    with pytest.raises(mgx.MgxError) as e:
+        P.setLevelBC(bo[0], bo[1], bo[2], 1)
+    assert e.value.code == mgx.MGX_ERR_ARG
     P.close()
+
+
+def _robin_boxed(ctx, g, bxs, rob):
+    """Robin planes -> three boxed cell fields (ng 1) holding them in the
+    ghost cells outside the faces (incflo's make_robinBC_MFs layout)."""
+    nz, ny, nx = g.cell_shape()
+    out = []
+    for q in range(3):
+        X = np.zeros((1, nz + 2, ny + 2, nx + 2))
+        for (d, s), planes in rob.items():
+            v = planes[q]
+            gi = 0 if s == 0 else g.n[d] + 1
+            if d == 0:
+                X[0, 1:-1, 1:-1, gi] = v
+            elif d == 1:
+                X[0, 1:-1, gi, 1:-1] = v
+            else:
+                X[0, gi, 1:-1, 1:-1] = v
+        out.append(ctx.array(B.set_ghosts_from(np.zeros(B.field_size(bxs, 1)), bxs, 1, X,
+                                               ncomp=1)))
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mac_robin_mixed_face_projection(ctx, orc, mode):
+    """incflo's mixed face in the MAC projection: Robin with a = 1, b = 0 on
+    the outflow half and a = 0, b = 1 on the inflow half, f = 0
+    (src/prob/prob_bc.cpp:97-197) against the oracle (oracle/mac_robin.py)."""
+    from incflo_amd import mgx
+    from oracle import mac_robin as R
+    _say(f"case mac robin mixed face mode={mode}")
+    n, per, ph = (32, 16, 16), (0, 0, 1), (2.0, 1.0, 1.0)
+    bclo, bchi = (N_, N_, 0), (mgx.BC_ROBIN, N_, 0)
+    g = orc.Geom.make(n, periodic=per, bclo=bclo, bchi=bchi, prob_hi=ph)
+    rng = np.random.default_rng(21)
+    rho = 1.0 + 0.5 * np.tanh(rng.standard_normal(g.cell_shape()))
+    u = _periodic_consistent(orc, g, [rng.standard_normal(orc.face_shape(g, d)) for d in range(3)])
+    dt = 0.01
+    shp = R.plane_shape(g, 0)
+    a = np.zeros(shp)
+    a[:, : shp[1] // 2] = 1.0
+    rob = {(0, 1): (a, 1.0 - a, np.zeros(shp))}
+    bxs = B.chop_domain(n, (16, 8, 8))
+    P = _mac(ctx, n, bxs, per, bclo, bchi, ph)
+    P.set_beta_from_density(ctx.array(B.to_boxed(rho, bxs, 1, periodic=per)), 1, dt)
+    ra, rb, rf = _robin_boxed(ctx, g, bxs, rob)
+    P.setLevelBC(ra, rb, rf, 1)
+    ud = [ctx.array(B.faces_to_boxed(u[d], bxs, 0, d)) for d in range(3)]
+    phid = ctx.array(B.field_size(bxs, 0, 1))
+    rtol = 1e-12 if mode == mgx.MODE_FP64 else 1e-10
+    info = P.project(ud[0], ud[1], ud[2], 0, rtol, 0.0, phi=phid, phi_ng=0, mode=mode)
+    assert info.converged
+    un = [B.faces_from_boxed(ud[d].download(), bxs, 0, d, orc.face_shape(g, d)) for d in range(3)]
+    phi = B.from_boxed(phid.download(), bxs, 0, g.cell_shape())
+    beta = orc.mac_beta_from_density(g, rho, dt)
+    uo, phio = R.robin_project(g, beta, rob, u)
+    tol = 1e-9 if mode == mgx.MODE_FP64 else 1e-7
+    assert _rel(phi, phio) < tol
+    for d in range(3):
+        assert _rel(un[d], uo[d]) < tol, d
+    top = a == 0
+    assert np.array_equal(un[0][:, :, -1][top], u[0][:, :, -1][top])
+    f = [ctx.array(B.face_field_size(bxs, 0, d)) for d in range(3)]
+    P.getFluxes(f[0], f[1], f[2], 0)
+    Fo = R.robin_fluxes(g, beta, rob, phio)
+    for d in range(3):
+        fd = B.faces_from_boxed(f[d].download(), bxs, 0, d, orc.face_shape(g, d))
+        assert _rel(fd, -Fo[d]) < tol, d
+    P.close()
+
+
+def test_mac_robin_general_solve(ctx, orc):
+    """MLMG::solve on the MAC operator with general Robin data (a, b > 0,
+    f != 0) on two sides, Dirichlet nowhere, against the oracle's direct
+    solve; a = 1, b = 0, f = 0 reproduces the Dirichlet side exactly."""
+    from incflo_amd import mgx
+    from oracle import mac_robin as R
+    _say("case mac robin general solve")
+    n, per, ph = (24, 16, 32), (0, 0, 0), (1.5, 1.0, 2.0)
+    bclo, bchi = (mgx.BC_ROBIN, N_, N_), (N_, mgx.BC_ROBIN, N_)
+    g = orc.Geom.make(n, periodic=per, bclo=bclo, bchi=bchi, prob_hi=ph)
+    rng = np.random.default_rng(22)
+    beta = [0.5 + rng.random(orc.face_shape(g, d)) for d in range(3)]
+    rob = {}
+    for (d, s) in [(0, 0), (1, 1)]:
+        shp = R.plane_shape(g, d)
+        rob[(d, s)] = (rng.random(shp), 0.05 + 0.1 * rng.random(shp), rng.standard_normal(shp))
+    rhs = rng.standard_normal(g.cell_shape())
+    bxs = B.chop_domain(n, (8, 16, 16))
+    P = _mac(ctx, n, bxs, per, bclo, bchi, ph)
+    P.set_beta(*[ctx.array(B.faces_to_boxed(beta[d], bxs, 0, d)) for d in range(3)], 0)
+    ra, rb, rf = _robin_boxed(ctx, g, bxs, rob)
+    P.setLevelBC(ra, rb, rf, 1)
+    rd = ctx.array(B.to_boxed(rhs, bxs, 0))
+    xd = ctx.array(B.field_size(bxs, 0, 1))
+    info = P.solve(xd, 0, rd, 0, 1e-12)
+    assert info.converged
+    x = B.from_boxed(xd.download(), bxs, 0, g.cell_shape())
+    xo = R.robin_solve(g, beta, rob, rhs)
+    assert _rel(x, xo) < 1e-9
+    P.close()
+    # Dirichlet limit: identical to a Dirichlet x-lo side
+    rob1 = {(d, s): (np.ones_like(v[0]), np.zeros_like(v[0]), np.zeros_like(v[0]))
+            for (d, s), v in rob.items()}
+    outs = []
+    for lo, hi in [(bclo, bchi), ((D_, N_, N_), (N_, D_, N_))]:
+        P = _mac(ctx, n, bxs, per, lo, hi, ph)
+        P.set_beta(*[ctx.array(B.faces_to_boxed(beta[d], bxs, 0, d)) for d in range(3)], 0)
+        if lo[0] == mgx.BC_ROBIN:
+            P.setLevelBC(*_robin_boxed(ctx, g, bxs, rob1), 1)
+        xd = ctx.array(B.field_size(bxs, 0, 1))
+        info = P.solve(xd, 0, rd, 0, 1e-12)
+        assert info.converged
+        outs.append(xd.download())
+        P.close()
+    assert np.array_equal(outs[0], outs[1])
