@@ -1,6 +1,7 @@
 // adv_api.hip -- C ABI of the Godunov advection pieces on incflo's predictor
 // path: ExtrapVelToFaces and the velocity's edge states / fluxes /
 // convective term (periodic, single level, no EB).
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -18,6 +19,7 @@ struct mgx_adv_s {
     long N = 0;
     DevBuf work;  // all stage arrays
     DevBuf qbuf, fbuf, abuf, obuf, cbuf, flbuf;
+    DevBuf sbuf, sfbuf, fsbuf;  // scalars, their forces, their face states
     hipStream_t stream() const { return ctx->stream; }
 };
 
@@ -27,16 +29,18 @@ namespace {
 constexpr int kWorkArrays = 18 + 3 + 9 + 27 + 9;
 
 GdArrays bind(mgx_adv_s* h, const double* q, const double* f, const double* adv, double dt,
-              bool extrap) {
+              bool extrap, int nc = 3, const int* cons = nullptr) {
     const long N = h->N;
     h->work.alloc((size_t)kWorkArrays * N * sizeof(double));
     double* w = h->work.as<double>();
     GdArrays a{};
     for (int n = 0; n < 3; ++n) {
-        a.q[n] = q + n * N;
-        a.f[n] = f ? f + n * N : nullptr;
+        a.q[n] = n < nc ? q + n * N : nullptr;
+        a.f[n] = (f && n < nc) ? f + n * N : nullptr;
         a.adv[n] = adv ? adv + n * N : nullptr;
+        a.cons[n] = (cons && n < nc) ? (cons[n] ? 1 : 0) : 0;
     }
+    a.nc = nc;
     long s = 0;
     for (int d = 0; d < 3; ++d)
         for (int n = 0; n < 3; ++n) {
@@ -163,7 +167,7 @@ MGX_API int mgx_adv_convective_term(mgx_adv h, const double* vel, int vel_ng, co
         const bool want_flux = flux_x || flux_y || flux_z;
         if (want_flux) h->flbuf.alloc(9 * N * sizeof(double));
         gd_conv_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
-            h->g, a, h->cbuf.as<double>(), want_flux ? h->flbuf.as<double>() : nullptr);
+            h->g, a, h->cbuf.as<double>(), want_flux ? h->flbuf.as<double>() : nullptr, 0, 3);
         gd_cells_out_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
             h->g, h->layout.dev_boxes(), h->layout.tiles(REG_CELL), conv, conv_ng, 3,
             h->cbuf.as<double>());
@@ -181,6 +185,80 @@ MGX_API int mgx_adv_convective_term(mgx_adv h, const double* vel, int vel_ng, co
                 gd_faces_kernel<<<h->layout.ntiles[REG_FACE0 + d], 256, 0, h->stream()>>>(
                     h->g, h->layout.dev_boxes(), h->layout.tiles(REG_FACE0 + d), d, fluxes[d], 0, 3,
                     h->flbuf.as<double>() + 3 * d * N, N, 1);
+        }
+        MGX_HIP(hipStreamSynchronize(h->stream()));
+    });
+}
+
+MGX_API int mgx_adv_scalar_term(mgx_adv h, const double* state, int s_ng, int ncomp,
+                                const int* iconserv, const double* force, int f_ng,
+                                const double* umac, const double* vmac, const double* wmac,
+                                int umac_ng, double dt, double* conv, int conv_ng, double* face_x,
+                                double* face_y, double* face_z, double* flux_x, double* flux_y,
+                                double* flux_z) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(state && umac && vmac && wmac && conv && dt > 0.0 && ncomp >= 1 && ncomp <= 64,
+                    MGX_ERR_ARG, "scalar advection: null field, bad component count or dt");
+        check_ng(s_ng, "state");
+        check_ng(f_ng, "forces");
+        check_ng(umac_ng, "umac");
+        check_ng(conv_ng, "conv");
+        const long N = h->N;
+        h->sbuf.alloc((size_t)ncomp * N * sizeof(double));
+        gd_cells_in_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
+            h->g, h->layout.dev_boxes(), h->layout.tiles(REG_CELL), state, s_ng, ncomp,
+            h->sbuf.as<double>());
+        if (force) {
+            h->sfbuf.alloc((size_t)ncomp * N * sizeof(double));
+            gd_cells_in_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
+                h->g, h->layout.dev_boxes(), h->layout.tiles(REG_CELL), force, f_ng, ncomp,
+                h->sfbuf.as<double>());
+        }
+        h->abuf.alloc(3 * N * sizeof(double));
+        const double* src[3] = {umac, vmac, wmac};
+        for (int d = 0; d < 3; ++d)
+            gd_faces_kernel<<<h->layout.ntiles[REG_FACE0 + d], 256, 0, h->stream()>>>(
+                h->g, h->layout.dev_boxes(), h->layout.tiles(REG_FACE0 + d), d,
+                const_cast<double*>(src[d]), umac_ng, 1, h->abuf.as<double>() + d * N, N, 0);
+        h->cbuf.alloc((size_t)ncomp * N * sizeof(double));
+        double* faces[3] = {face_x, face_y, face_z};
+        double* fluxes[3] = {flux_x, flux_y, flux_z};
+        const bool want_face = face_x || face_y || face_z;
+        const bool want_flux = flux_x || flux_y || flux_z;
+        if (want_face) h->fsbuf.alloc((size_t)3 * ncomp * N * sizeof(double));
+        if (want_flux) h->flbuf.alloc((size_t)3 * ncomp * N * sizeof(double));
+        // the work arrays hold three components at a time
+        for (int n0 = 0; n0 < ncomp; n0 += 3) {
+            const int nc = std::min(3, ncomp - n0);
+            int cons[3] = {0, 0, 0};
+            for (int q = 0; q < nc; ++q) cons[q] = iconserv ? iconserv[n0 + q] : 0;
+            GdArrays a = bind(h, h->sbuf.as<double>() + n0 * N,
+                              force ? h->sfbuf.as<double>() + n0 * N : nullptr,
+                              h->abuf.as<double>(), dt, false, nc, cons);
+            run_stages(h, a);
+            gd_conv_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
+                h->g, a, h->cbuf.as<double>(), want_flux ? h->flbuf.as<double>() : nullptr, n0,
+                ncomp);
+            if (want_face)
+                for (int d = 0; d < 3; ++d)
+                    for (int q = 0; q < nc; ++q)
+                        MGX_HIP(hipMemcpyAsync(h->fsbuf.as<double>() + (long)(d * ncomp + n0 + q) * N,
+                                               a.out[d][q], N * sizeof(double),
+                                               hipMemcpyDeviceToDevice, h->stream()));
+        }
+        gd_cells_out_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
+            h->g, h->layout.dev_boxes(), h->layout.tiles(REG_CELL), conv, conv_ng, ncomp,
+            h->cbuf.as<double>());
+        for (int d = 0; d < 3; ++d) {
+            if (faces[d])
+                gd_faces_kernel<<<h->layout.ntiles[REG_FACE0 + d], 256, 0, h->stream()>>>(
+                    h->g, h->layout.dev_boxes(), h->layout.tiles(REG_FACE0 + d), d, faces[d], 0,
+                    ncomp, h->fsbuf.as<double>() + (long)d * ncomp * N, N, 1);
+            if (fluxes[d])
+                gd_faces_kernel<<<h->layout.ntiles[REG_FACE0 + d], 256, 0, h->stream()>>>(
+                    h->g, h->layout.dev_boxes(), h->layout.tiles(REG_FACE0 + d), d, fluxes[d], 0,
+                    ncomp, h->flbuf.as<double>() + (long)d * ncomp * N, N, 1);
         }
         MGX_HIP(hipStreamSynchronize(h->stream()));
     });

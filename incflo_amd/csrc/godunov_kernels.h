@@ -47,6 +47,8 @@ struct GdArrays {
     double* out[3][3];       // final states [d][n] (extrapolation: [d][d] only)
     double dt;
     int extrap;              // 1: ExtrapVelToFaces, 0: edge states with u_mac
+    int nc;                  // components in q (3 for the velocity, 1..3 for scalars)
+    int cons[3];             // conservative transverse corrections for component n
 };
 
 __device__ __forceinline__ int gd_wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -112,7 +114,7 @@ static __global__ void __launch_bounds__(256) gd_ppm_kernel(GdGeom g, GdArrays a
         }
         const long m2 = gd_sh(g, i, j, k, d, -2), m1 = gd_sh(g, i, j, k, d, -1);
         const long p1 = gd_sh(g, i, j, k, d, 1), p2 = gd_sh(g, i, j, k, d, 2);
-        for (int n = 0; n < 3; ++n) {
+        for (int n = 0; n < a.nc; ++n) {
             const double* s = a.q[n];
             const double s0 = s[c];
             double sL, sR;
@@ -143,7 +145,7 @@ static __global__ void __launch_bounds__(256) gd_edge_kernel(GdGeom g, GdArrays 
         const long lc = gd_sh(g, i, j, k, d, -1);
         const double av = a.extrap ? gd_burgers(a.Ip[d][d][lc], a.Im[d][d][f]) : a.adv[d][f];
         a.ad[d][f] = av;
-        for (int n = 0; n < 3; ++n) a.E[d][n][f] = gd_upwind(a.Ip[d][n][lc], a.Im[d][n][f], av);
+        for (int n = 0; n < a.nc; ++n) a.E[d][n][f] = gd_upwind(a.Ip[d][n][lc], a.Im[d][n][f], av);
     }
 }
 
@@ -161,10 +163,20 @@ static __global__ void __launch_bounds__(256) gd_trans_kernel(GdGeom g, GdArrays
             const double c = a.dt / (3.0 * g.dx[e]);
             const long lcp = gd_sh(g, li, lj, lk, e, 1), hcp = gd_sh(g, i, j, k, e, 1);
             const double* ae = a.ad[e];
-            for (int n = 0; n < 3; ++n) {
+            for (int n = 0; n < a.nc; ++n) {
                 const double* Ee = a.E[e][n];
-                const double lo = a.Ip[d][n][lc] - c * 0.5 * (ae[lcp] + ae[lc]) * (Ee[lcp] - Ee[lc]);
-                const double hi = a.Im[d][n][f] - c * 0.5 * (ae[hcp] + ae[f]) * (Ee[hcp] - Ee[f]);
+                double lo, hi;
+                if (a.cons[n]) {
+                    // flux differences minus q times the velocity difference
+                    const double* q = a.q[n];
+                    lo = a.Ip[d][n][lc] - c * (ae[lcp] * Ee[lcp] - ae[lc] * Ee[lc]) +
+                         c * q[lc] * (ae[lcp] - ae[lc]);
+                    hi = a.Im[d][n][f] - c * (ae[hcp] * Ee[hcp] - ae[f] * Ee[f]) +
+                         c * q[f] * (ae[hcp] - ae[f]);
+                } else {
+                    lo = a.Ip[d][n][lc] - c * 0.5 * (ae[lcp] + ae[lc]) * (Ee[lcp] - Ee[lc]);
+                    hi = a.Im[d][n][f] - c * 0.5 * (ae[hcp] + ae[f]) * (Ee[hcp] - Ee[f]);
+                }
                 a.T[d][e][n][f] = gd_upwind(lo, hi, a.ad[d][f]);
             }
         }
@@ -186,14 +198,25 @@ static __global__ void __launch_bounds__(256) gd_final_kernel(GdGeom g, GdArrays
         const long h1 = gd_sh(g, i, j, k, e1, 1), h2 = gd_sh(g, i, j, k, e2, 1);
         const double* a1 = a.ad[e1];
         const double* a2 = a.ad[e2];
-        for (int n = 0; n < 3; ++n) {
+        for (int n = 0; n < a.nc; ++n) {
             if (a.extrap && n != d) continue;
             const double* t1 = a.T[e1][e2][n];
             const double* t2 = a.T[e2][e1][n];
-            double lo = a.Ip[d][n][lc] - c1 * 0.5 * (a1[l1] + a1[lc]) * (t1[l1] - t1[lc]) -
-                        c2 * 0.5 * (a2[l2] + a2[lc]) * (t2[l2] - t2[lc]);
-            double hi = a.Im[d][n][f] - c1 * 0.5 * (a1[h1] + a1[f]) * (t1[h1] - t1[f]) -
-                        c2 * 0.5 * (a2[h2] + a2[f]) * (t2[h2] - t2[f]);
+            double lo, hi;
+            if (a.cons[n]) {
+                const double* q = a.q[n];
+                lo = a.Ip[d][n][lc] - c1 * (a1[l1] * t1[l1] - a1[lc] * t1[lc]) +
+                     c1 * q[lc] * (a1[l1] - a1[lc]) - c2 * (a2[l2] * t2[l2] - a2[lc] * t2[lc]) +
+                     c2 * q[lc] * (a2[l2] - a2[lc]);
+                hi = a.Im[d][n][f] - c1 * (a1[h1] * t1[h1] - a1[f] * t1[f]) +
+                     c1 * q[f] * (a1[h1] - a1[f]) - c2 * (a2[h2] * t2[h2] - a2[f] * t2[f]) +
+                     c2 * q[f] * (a2[h2] - a2[f]);
+            } else {
+                lo = a.Ip[d][n][lc] - c1 * 0.5 * (a1[l1] + a1[lc]) * (t1[l1] - t1[lc]) -
+                     c2 * 0.5 * (a2[l2] + a2[lc]) * (t2[l2] - t2[lc]);
+                hi = a.Im[d][n][f] - c1 * 0.5 * (a1[h1] + a1[f]) * (t1[h1] - t1[f]) -
+                     c2 * 0.5 * (a2[h2] + a2[f]) * (t2[h2] - t2[f]);
+            }
             if (a.f[n]) {
                 lo += 0.5 * a.dt * a.f[n][lc];
                 hi += 0.5 * a.dt * a.f[n][f];
@@ -203,9 +226,11 @@ static __global__ void __launch_bounds__(256) gd_final_kernel(GdGeom g, GdArrays
     }
 }
 
-// 6. conv[n] = -div(u_mac S[.][n]) + q_n div(u_mac); optional fluxes u_mac * S
+// 6. conv[n] = -div(u_mac S[.][n]) + q_n div(u_mac) (the last term only for
+// components in convective form); optional fluxes u_mac * S.  Outputs of
+// component n go to slot n0 + n of ntot (conv: [slot][N], flux: [d][slot][N]).
 static __global__ void __launch_bounds__(256) gd_conv_kernel(GdGeom g, GdArrays a, double* conv,
-                                                             double* flux) {
+                                                             double* flux, int n0, int ntot) {
     int i, j, k;
     if (!grid_point(g.n[0], g.n[1], i, j, k) || k >= g.n[2]) return;
     const long N = (long)g.n[0] * g.n[1] * g.n[2];
@@ -213,16 +238,16 @@ static __global__ void __launch_bounds__(256) gd_conv_kernel(GdGeom g, GdArrays 
     double divu = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) divu += (a.adv[d][gd_sh(g, i, j, k, d, 1)] - a.adv[d][c]) / g.dx[d];
-    for (int n = 0; n < 3; ++n) {
+    for (int n = 0; n < a.nc; ++n) {
         double div = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double* st = a.out[d][n];
             const long cp = gd_sh(g, i, j, k, d, 1);
             div += (a.adv[d][cp] * st[cp] - a.adv[d][c] * st[c]) / g.dx[d];
-            if (flux) flux[(long)(3 * d + n) * N + c] = a.adv[d][c] * st[c];
+            if (flux) flux[(long)(d * ntot + n0 + n) * N + c] = a.adv[d][c] * st[c];
         }
-        conv[(long)n * N + c] = -div + a.q[n][c] * divu;
+        conv[(long)(n0 + n) * N + c] = a.cons[n] ? -div : -div + a.q[n][c] * divu;
     }
 }
 

@@ -326,6 +326,22 @@ MGX_API int mgx_adv_convective_term(mgx_adv h, const double* vel, int vel_ng, co
                                     int conv_ng, double* face_x, double* face_y, double* face_z,
                                     double* flux_x, double* flux_y, double* flux_z);
 
+/* ComputeFluxesOnBoxFromState (is_velocity = false) + ComputeDivergence(mult = -1)
+ * + ComputeConvectiveTerm for density and tracers
+ * (src/convection/incflo_compute_advection_term.cpp:446-561, 666-760): edge states
+ * of ncomp cell scalars (state_ng ghosts) with the MAC velocities, fluxes
+ * u_mac * state, conv = -div(flux), plus state * div(u_mac) for components
+ * with iconserv[n] == 0 (iconserv NULL: all convective).  Conservative
+ * components (density; rho * tracer for conservative tracers, which the caller
+ * forms as incflo does at :498-511) take conservative transverse terms.
+ * face_* / flux_* (boxed faces, ncomp comps, no ghosts) may be NULL. */
+MGX_API int mgx_adv_scalar_term(mgx_adv h, const double* state, int state_ng, int ncomp,
+                                const int* iconserv, const double* force, int f_ng,
+                                const double* umac, const double* vmac, const double* wmac,
+                                int umac_ng, double dt, double* conv, int conv_ng, double* face_x,
+                                double* face_y, double* face_z, double* flux_x, double* flux_y,
+                                double* flux_z);
+
 /* ---- learned coarse-level predictor --------------------------------------
  * No counterpart in the reference (incflo has no learned component); these
  * entry points serve MGX_MODE_LEARNED of the nodal solve.  A 3-D conv U-net

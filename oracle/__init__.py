@@ -408,6 +408,31 @@ def convective_term(g, vel, umac, states):
     return conv
 
 
+def godunov_scalar_states(g, s, umac, dt, iconserv=None, force=None):
+    """Edge states of ncomp scalars (s: (ncomp, nz, ny, nx)) with the MAC
+    velocities: shape (3 faces, ncomp, nz, ny, nx)."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    nc = s.shape[0]
+    ic = (C.c_int * nc)(*([0] * nc if iconserv is None else [int(v) for v in iconserv]))
+    st = np.zeros((3, nc) + g.cell_shape())
+    um = [np.ascontiguousarray(a, dtype=np.float64) for a in umac]
+    lib().orc_godunov_scalar_states(C.byref(g), _p(s), nc, ic, _opt(force), _p(um[0]), _p(um[1]),
+                                    _p(um[2]), C.c_double(dt), _p(st))
+    return st
+
+
+def scalar_advection(g, s, umac, states, iconserv=None):
+    """conv = -div(u_mac s_face) (+ s div(u_mac) for convective components)."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    nc = s.shape[0]
+    ic = (C.c_int * nc)(*([0] * nc if iconserv is None else [int(v) for v in iconserv]))
+    um = [np.ascontiguousarray(a, dtype=np.float64) for a in umac]
+    conv = np.zeros((nc,) + g.cell_shape())
+    lib().orc_scalar_advection(C.byref(g), _p(s), nc, ic, _p(um[0]), _p(um[1]), _p(um[2]),
+                               _p(np.ascontiguousarray(states)), _p(conv))
+    return conv
+
+
 # ---- embedded boundary, nodal ------------------------------------------------
 _ip = C.POINTER(C.c_int)
 

@@ -146,6 +146,12 @@ void orc_godunov_edge_states(const orc_geom* g, const double* u, const double* v
                              const double* fu, const double* fv, const double* fw,
                              const double* umac, const double* vmac, const double* wmac,
                              double dt, double* states);
+void orc_godunov_scalar_states(const orc_geom* g, const double* s, int ncomp, const int* iconserv,
+                               const double* force, const double* umac, const double* vmac,
+                               const double* wmac, double dt, double* states);
+void orc_scalar_advection(const orc_geom* g, const double* s, int ncomp, const int* iconserv,
+                          const double* umac, const double* vmac, const double* wmac,
+                          const double* states, double* conv);
 void orc_convective_term(const orc_geom* g, const double* u, const double* v, const double* w,
                          const double* umac, const double* vmac, const double* wmac,
                          const double* states, double* conv);

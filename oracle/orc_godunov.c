@@ -325,3 +325,142 @@ void orc_convective_term(const orc_geom* g, const double* u, const double* v, co
                 }
             }
 }
+
+/* ---- scalars: density and tracers (edge-state mode with u_mac) -----------
+ * incflo advects density and tracers with the MAC velocities
+ * (src/convection/incflo_compute_advection_term.cpp:446-561): edge states by
+ * the same Godunov steps, then fluxes u_mac s_face and conv = -div(F)
+ * (ComputeDivergence, mult = -1, :666-742), plus s div(u_mac) for the
+ * components advected in convective form (ComputeConvectiveTerm, :744-760).
+ * A component with iconserv = 1 (density always, tracers by
+ * trac_is_conservative, :28-35) takes its transverse corrections in
+ * conservative form: the difference of the transverse fluxes minus the cell
+ * value times the difference of the transverse velocities,
+ *     - c [(a s)_+ - (a s)_-] + c q [a_+ - a_-],
+ * which vanishes for a constant state like the convective form
+ * - c avg(a) (s_+ - s_-) used otherwise. */
+static void trans1_cons(const pg_t* p, int d, int e, const double* q, const double* Imd,
+                        const double* Ipd, const double* edge_e, const double* ae, const double* ad,
+                        double dt, double* out) {
+    const double c = dt / (3.0 * p->dx[e]);
+    for (int k = 0; k < p->n[2]; ++k)
+        for (int j = 0; j < p->n[1]; ++j)
+            for (int i = 0; i < p->n[0]; ++i) {
+                const long f = at(p, i, j, k);
+                const long lc = sh(p, i, j, k, d, -1);
+                const int li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
+                const long lcp = sh(p, li, lj, lk, e, 1), hcp = sh(p, i, j, k, e, 1);
+                const double lo = Ipd[lc] - c * (ae[lcp] * edge_e[lcp] - ae[lc] * edge_e[lc]) +
+                                  c * q[lc] * (ae[lcp] - ae[lc]);
+                const double hi = Imd[f] - c * (ae[hcp] * edge_e[hcp] - ae[f] * edge_e[f]) +
+                                  c * q[f] * (ae[hcp] - ae[f]);
+                out[f] = upwind(lo, hi, ad[f]);
+            }
+}
+
+static void final_states_cons(const pg_t* p, int d, int e1, int e2, const double* q,
+                              const double* Imd, const double* Ipd, const double* t_e1,
+                              const double* a_e1, const double* t_e2, const double* a_e2,
+                              const double* force, double dt, long f, int i, int j, int k,
+                              double* lo, double* hi) {
+    const long lc = sh(p, i, j, k, d, -1);
+    const int li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
+    const double c1 = dt / (2.0 * p->dx[e1]), c2 = dt / (2.0 * p->dx[e2]);
+    const long l1 = sh(p, li, lj, lk, e1, 1), l2 = sh(p, li, lj, lk, e2, 1);
+    const long h1 = sh(p, i, j, k, e1, 1), h2 = sh(p, i, j, k, e2, 1);
+    double l = Ipd[lc] - c1 * (a_e1[l1] * t_e1[l1] - a_e1[lc] * t_e1[lc]) +
+               c1 * q[lc] * (a_e1[l1] - a_e1[lc]) -
+               c2 * (a_e2[l2] * t_e2[l2] - a_e2[lc] * t_e2[lc]) + c2 * q[lc] * (a_e2[l2] - a_e2[lc]);
+    double h = Imd[f] - c1 * (a_e1[h1] * t_e1[h1] - a_e1[f] * t_e1[f]) +
+               c1 * q[f] * (a_e1[h1] - a_e1[f]) -
+               c2 * (a_e2[h2] * t_e2[h2] - a_e2[f] * t_e2[f]) + c2 * q[f] * (a_e2[h2] - a_e2[f]);
+    if (force) {
+        l += 0.5 * dt * force[lc];
+        h += 0.5 * dt * force[f];
+    }
+    *lo = l;
+    *hi = h;
+}
+
+/* edge states of ncomp cell scalars s[n*N..] on every face direction:
+ * states[(d * ncomp + n) * N + f]; force may be NULL (else ncomp comps) */
+void orc_godunov_scalar_states(const orc_geom* g, const double* s, int ncomp, const int* iconserv,
+                               const double* force, const double* umac, const double* vmac,
+                               const double* wmac, double dt, double* states) {
+    pg_t p = pg(g);
+    const long N = (long)p.n[0] * p.n[1] * p.n[2];
+    const double* ad[3] = {umac, vmac, wmac};
+    double *Im[3], *Ip[3], *E[3], *T[3][3];
+    for (int d = 0; d < 3; ++d) {
+        Im[d] = dalloc(N);
+        Ip[d] = dalloc(N);
+        E[d] = dalloc(N);
+        for (int e = 0; e < 3; ++e) T[d][e] = (e == d) ? NULL : dalloc(N);
+    }
+    for (int n = 0; n < ncomp; ++n) {
+        const double* q = s + (long)n * N;
+        const double* fn = force ? force + (long)n * N : NULL;
+        const int cons = iconserv ? iconserv[n] : 0;
+        for (int d = 0; d < 3; ++d) {
+            orc_ppm_predict(g, q, d, NULL, ad[d], dt, Im[d], Ip[d]);
+            edges(&p, d, Im[d], Ip[d], ad[d], E[d]);
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int e = 0; e < 3; ++e) {
+                if (e == d) continue;
+                if (cons)
+                    trans1_cons(&p, d, e, q, Im[d], Ip[d], E[e], ad[e], ad[d], dt, T[d][e]);
+                else
+                    trans1(&p, d, e, Im[d], Ip[d], E[e], ad[e], ad[d], dt, T[d][e]);
+            }
+        for (int d = 0; d < 3; ++d) {
+            const int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+            double* o = states + (long)(d * ncomp + n) * N;
+            for (int k = 0; k < p.n[2]; ++k)
+                for (int j = 0; j < p.n[1]; ++j)
+                    for (int i = 0; i < p.n[0]; ++i) {
+                        const long f = at(&p, i, j, k);
+                        double lo, hi;
+                        if (cons)
+                            final_states_cons(&p, d, e1, e2, q, Im[d], Ip[d], T[e1][e2], ad[e1],
+                                              T[e2][e1], ad[e2], fn, dt, f, i, j, k, &lo, &hi);
+                        else
+                            final_states(&p, d, e1, e2, Im[d], Ip[d], T[e1][e2], ad[e1],
+                                         T[e2][e1], ad[e2], fn, dt, f, i, j, k, &lo, &hi);
+                        o[f] = upwind(lo, hi, ad[d][f]);
+                    }
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        free(Im[d]);
+        free(Ip[d]);
+        free(E[d]);
+        for (int e = 0; e < 3; ++e) free(T[d][e]);
+    }
+}
+
+/* conv[n] = -div(u_mac s_face[n]) (+ s_n div(u_mac) when iconserv[n] == 0) */
+void orc_scalar_advection(const orc_geom* g, const double* s, int ncomp, const int* iconserv,
+                          const double* umac, const double* vmac, const double* wmac,
+                          const double* states, double* conv) {
+    pg_t p = pg(g);
+    const long N = (long)p.n[0] * p.n[1] * p.n[2];
+    const double* a[3] = {umac, vmac, wmac};
+    for (int k = 0; k < p.n[2]; ++k)
+        for (int j = 0; j < p.n[1]; ++j)
+            for (int i = 0; i < p.n[0]; ++i) {
+                const long c = at(&p, i, j, k);
+                double divu = 0.0;
+                for (int d = 0; d < 3; ++d) divu += (a[d][sh(&p, i, j, k, d, 1)] - a[d][c]) / p.dx[d];
+                for (int n = 0; n < ncomp; ++n) {
+                    double div = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const double* st = states + (long)(d * ncomp + n) * N;
+                        const long cp = sh(&p, i, j, k, d, 1);
+                        div += (a[d][cp] * st[cp] - a[d][c] * st[c]) / p.dx[d];
+                    }
+                    const int cons = iconserv ? iconserv[n] : 0;
+                    conv[(long)n * N + c] = -div + (cons ? 0.0 : s[(long)n * N + c] * divu);
+                }
+            }
+}
