@@ -75,16 +75,17 @@ def to_boxed(canon, boxes, ng, nodal=False, ghost_value=None, periodic=(1, 1, 1)
     return out
 
 
-def set_ghosts_from(boxed, boxes, ng, ghosted, ncomp=3):
+def set_ghosts_from(boxed, boxes, ng, ghosted, ncomp=3, gw=1):
     """Overwrite every cell (valid + ghost) of every box from a canonical array
-    with one ghost layer (shape (ncomp, nz+2, ny+2, nx+2)); ng must be 1."""
-    assert ng == 1
+    with gw ghost layers (shape (ncomp, nz+2gw, ny+2gw, nx+2gw)); ng <= gw."""
+    assert ng <= gw
     off = 0
     for lo, hi in boxes:
         D = box_dims(lo, hi, ng, False)
         vol = int(np.prod(D))
+        o = [lo[q] + gw - ng for q in range(3)]
         for comp in range(ncomp):
-            blk = ghosted[comp, lo[2]:lo[2] + D[2], lo[1]:lo[1] + D[1], lo[0]:lo[0] + D[0]]
+            blk = ghosted[comp, o[2]:o[2] + D[2], o[1]:o[1] + D[1], o[0]:o[0] + D[0]]
             boxed[off + comp * vol: off + (comp + 1) * vol] = blk.ravel()
         off += ncomp * vol
     return boxed

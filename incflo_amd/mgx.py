@@ -121,8 +121,9 @@ def lib():
             "mgx_adv_destroy": [vp],
             "mgx_adv_extrap_vel_to_faces": [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp, vp, vp,
                                             C.c_int],
-            "mgx_adv_scalar_term": [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp, vp, vp,
+            "mgx_adv_scalar_term": [vp, vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp,
                                     C.c_int, C.c_double, vp, C.c_int, vp, vp, vp, vp, vp, vp],
+            "mgx_adv_set_velocity_bcrec": [vp, vp],
             "mgx_adv_convective_term": [vp, vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int,
                                         C.c_double, vp, C.c_int, vp, vp, vp, vp, vp, vp],
             "mgx_nodal_set_eb_cylinder": [vp, C.c_double, C.c_double, C.c_double],
@@ -508,16 +509,23 @@ class Advection:
             conv.ptr, conv_ng, p(faces[0]), p(faces[1]), p(faces[2]), p(fluxes[0]),
             p(fluxes[1]), p(fluxes[2])))
 
+    def set_velocity_bcrec(self, bcrec):
+        """amrex::BCType values, shape (3, 6): lo x, y, z, hi x, y, z per component."""
+        b = (C.c_int * 18)(*[int(v) for v in np.asarray(bcrec).ravel()])
+        self.ctx.check(lib().mgx_adv_set_velocity_bcrec(self.h, b))
+
     def scalar_term(self, state, state_ng, ncomp, umac, vmac, wmac, umac_ng, dt, conv,
                     conv_ng=0, iconserv=None, force=None, f_ng=0, faces=(None, None, None),
-                    fluxes=(None, None, None)):
+                    fluxes=(None, None, None), bcrec=None):
         """Density / tracer advection: edge states, fluxes and conv (see mgx.h)."""
         p = lambda a: a.ptr if a is not None else None
         ic = None if iconserv is None else (C.c_int * ncomp)(*[int(v) for v in iconserv])
+        bc = None if bcrec is None else (C.c_int * (6 * ncomp))(
+            *[int(v) for v in np.asarray(bcrec).ravel()])
         self.ctx.check(lib().mgx_adv_scalar_term(
-            self.h, state.ptr, state_ng, ncomp, ic, p(force), f_ng, umac.ptr, vmac.ptr, wmac.ptr,
-            umac_ng, dt, conv.ptr, conv_ng, p(faces[0]), p(faces[1]), p(faces[2]), p(fluxes[0]),
-            p(fluxes[1]), p(fluxes[2])))
+            self.h, state.ptr, state_ng, ncomp, ic, bc, p(force), f_ng, umac.ptr, vmac.ptr,
+            wmac.ptr, umac_ng, dt, conv.ptr, conv_ng, p(faces[0]), p(faces[1]), p(faces[2]),
+            p(fluxes[0]), p(fluxes[1]), p(fluxes[2])))
 
     def close(self):
         if self.h:

@@ -298,15 +298,23 @@ MGX_API int mgx_mac_set_eb_cylinder(mgx_mac h, double cx, double cy, double r);
 MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
                           double rtol, double atol, int mode, mgx_solve_info* info);
 
-/* ---- Godunov advection on the predictor path (periodic, single level, no EB) --
+/* ---- Godunov advection on the predictor path (single level, no EB) --------
  * The reference calls AMReX-Hydro's Godunov here; the method (PPM with the
  * Colella-Woodward limiter, corner-transport-upwind transverse terms,
  * use_forces_in_trans = false, src/incflo.H:516,525) is restated in
  * incflo_amd/csrc/godunov_kernels.h.  Boxed layouts as above; faces shared by
- * two boxes must agree. */
+ * two boxes must agree.  Domain faces of non-periodic directions follow
+ * amrex::BCType records (6 ints per component: lo x, y, z, hi x, y, z; ext_dir
+ * 3, foextrap 2, hoextrap 4, reflect_even 1, reflect_odd -1; periodic
+ * directions int_dir 0), as incflo's get_velocity_bcrec / get_density_bcrec /
+ * get_tracer_bcrec; cell states then carry 2 filled ghost cells (incflo's
+ * FillPatch; an ext_dir ghost holds the boundary value). */
 typedef struct mgx_adv_s* mgx_adv;
 MGX_API int mgx_adv_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_adv* out);
 MGX_API int mgx_adv_destroy(mgx_adv h);
+/* get_velocity_bcrec() (src/incflo.H): 3 x 6 BCType values; required before
+ * the velocity calls on a bounded domain. */
+MGX_API int mgx_adv_set_velocity_bcrec(mgx_adv h, const int* bcrec);
 /* HydroUtils::ExtrapVelToFaces (src/convection/incflo_compute_MAC_projected_velocities.cpp:168-178):
  * cell velocity (3 comps, vel_ng ghosts) and forces (3 comps, f_ng ghosts,
  * NULL = none) -> face-normal velocities at t^{n+1/2} (valid faces of the
@@ -331,12 +339,14 @@ MGX_API int mgx_adv_convective_term(mgx_adv h, const double* vel, int vel_ng, co
  * (src/convection/incflo_compute_advection_term.cpp:446-561, 666-760): edge states
  * of ncomp cell scalars (state_ng ghosts) with the MAC velocities, fluxes
  * u_mac * state, conv = -div(flux), plus state * div(u_mac) for components
- * with iconserv[n] == 0 (iconserv NULL: all convective).  Conservative
+ * with iconserv[n] == 0 (iconserv NULL: all convective); bcrec: ncomp x 6
+ * BCType values (NULL on periodic domains).  Conservative
  * components (density; rho * tracer for conservative tracers, which the caller
  * forms as incflo does at :498-511) take conservative transverse terms.
  * face_* / flux_* (boxed faces, ncomp comps, no ghosts) may be NULL. */
 MGX_API int mgx_adv_scalar_term(mgx_adv h, const double* state, int state_ng, int ncomp,
-                                const int* iconserv, const double* force, int f_ng,
+                                const int* iconserv, const int* bcrec, const double* force,
+                                int f_ng,
                                 const double* umac, const double* vmac, const double* wmac,
                                 int umac_ng, double dt, double* conv, int conv_ng, double* face_x,
                                 double* face_y, double* face_z, double* flux_x, double* flux_y,

@@ -433,6 +433,52 @@ def scalar_advection(g, s, umac, states, iconserv=None):
     return conv
 
 
+# General form with domain boundaries: cell states carry 2 ghost layers
+# (shape (ncomp, nz+4, ny+4, nx+4), filled by the caller), bc is an int array
+# (ncomp, 6) of amrex::BCType values (lo x, y, z, hi x, y, z), faces have
+# n_d + 1 entries along d (face_shape).
+BCT_RODD, BCT_INT, BCT_REVEN, BCT_FOE, BCT_EXT, BCT_HOE = -1, 0, 1, 2, 3, 4
+
+
+def _bc_arr(bc, ncomp):
+    a = np.ascontiguousarray(np.asarray(bc, dtype=np.int32).reshape(ncomp, 6))
+    return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def gd_extrap_vel(g, vel_g, bc, dt, force=None):
+    vel_g = np.ascontiguousarray(vel_g, dtype=np.float64)
+    b, bp = _bc_arr(bc, 3)
+    out = [np.zeros(face_shape(g, d)) for d in range(3)]
+    lib().orc_gd_extrap_vel(C.byref(g), _p(vel_g), bp, _opt(force), C.c_double(dt), _p(out[0]),
+                            _p(out[1]), _p(out[2]))
+    return out
+
+
+def gd_edge_states(g, s_g, bc, umac, dt, iconserv=None, force=None):
+    """-> [states on x-faces (ncomp, ...), y-faces, z-faces]"""
+    s_g = np.ascontiguousarray(s_g, dtype=np.float64)
+    nc = s_g.shape[0]
+    b, bp = _bc_arr(bc, nc)
+    ic = (C.c_int * nc)(*([0] * nc if iconserv is None else [int(v) for v in iconserv]))
+    um = [np.ascontiguousarray(a, dtype=np.float64) for a in umac]
+    out = [np.zeros((nc,) + face_shape(g, d)) for d in range(3)]
+    lib().orc_gd_edge_states(C.byref(g), nc, _p(s_g), bp, ic, _opt(force), _p(um[0]), _p(um[1]),
+                             _p(um[2]), C.c_double(dt), _p(out[0]), _p(out[1]), _p(out[2]))
+    return out
+
+
+def gd_conv(g, s, umac, states, iconserv=None):
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    nc = s.shape[0]
+    ic = (C.c_int * nc)(*([0] * nc if iconserv is None else [int(v) for v in iconserv]))
+    um = [np.ascontiguousarray(a, dtype=np.float64) for a in umac]
+    st = [np.ascontiguousarray(a, dtype=np.float64) for a in states]
+    conv = np.zeros((nc,) + g.cell_shape())
+    lib().orc_gd_conv(C.byref(g), nc, _p(s), ic, _p(um[0]), _p(um[1]), _p(um[2]), _p(st[0]),
+                      _p(st[1]), _p(st[2]), _p(conv))
+    return conv
+
+
 # ---- embedded boundary, nodal ------------------------------------------------
 _ip = C.POINTER(C.c_int)
 

@@ -152,6 +152,17 @@ void orc_godunov_scalar_states(const orc_geom* g, const double* s, int ncomp, co
 void orc_scalar_advection(const orc_geom* g, const double* s, int ncomp, const int* iconserv,
                           const double* umac, const double* vmac, const double* wmac,
                           const double* states, double* conv);
+/* general form: ghosted cell states (2 layers), amrex::BCType per component
+ * and side (bc[6 n + 3 side + d]), faces with n_d + 1 entries along d */
+void orc_gd_extrap_vel(const orc_geom* g, const double* vel_g, const int* bc, const double* force,
+                       double dt, double* umac, double* vmac, double* wmac);
+void orc_gd_edge_states(const orc_geom* g, int ncomp, const double* s_g, const int* bc,
+                        const int* iconserv, const double* force, const double* umac,
+                        const double* vmac, const double* wmac, double dt, double* sx, double* sy,
+                        double* sz);
+void orc_gd_conv(const orc_geom* g, int ncomp, const double* s, const int* iconserv,
+                 const double* umac, const double* vmac, const double* wmac, const double* sx,
+                 const double* sy, const double* sz, double* conv);
 void orc_convective_term(const orc_geom* g, const double* u, const double* v, const double* w,
                          const double* umac, const double* vmac, const double* wmac,
                          const double* states, double* conv);

@@ -202,13 +202,130 @@ def test_scalar_advection_matches_oracle(ctx, orc):
     A.close()
 
 
+EXT, FOE, HOE, REVEN, RODD = 3, 2, 4, 1, -1
+
+
+def test_bounded_channel_matches_oracle(ctx, orc):
+    """Channel: inflow (ext_dir) at x-lo, outflow (foextrap) at x-hi, no-slip
+    walls (ext_dir 0) in y, periodic z; velocity extrapolation, velocity
+    convective term and density / tracer advection against the oracle."""
+    from incflo_amd import mgx
+    n, mgs = (24, 16, 12), (8, 8, 6)
+    per = (0, 0, 1)
+    _say(f"case bounded channel n={n}")
+    bxs = B.chop_domain(n, mgs)
+    g = orc.Geom.make(n, periodic=per, bclo=(3, 3, 0), bchi=(2, 3, 0))
+    rng = np.random.default_rng(9)
+    vel = _field(n, 10)
+    vel[0] += 1.0
+    G = 3  # ghost layers the caller fills (incflo: 3 for the velocity)
+    vg = np.pad(vel, ((0, 0), (G, G), (G, G), (G, G)), mode="wrap")
+    inflow = np.array([1.2, 0.1, -0.05])
+    for c in range(3):
+        vg[c, :, :, :G] = inflow[c]                               # inflow face value
+        vg[c, :, :, -G:] = vg[c, :, :, -G - 1:-G]                 # foextrap
+        vg[c, :, :G, :] = 0.0                                     # no-slip walls
+        vg[c, :, -G:, :] = 0.0
+    bc = np.zeros((3, 6), int)
+    bc[:, 0], bc[:, 3] = EXT, FOE
+    bc[:, 1], bc[:, 4] = EXT, EXT
+    dt = 0.3 / max(n) / np.max(np.abs(vel))
+    lev = mgx.make_level(n, bxs, periodic=per)
+    A = mgx.Advection(ctx, lev)
+    A.set_velocity_bcrec(bc)
+    vd = ctx.array(B.set_ghosts_from(np.zeros(B.field_size(bxs, G, 3)), bxs, G, vg, ncomp=3, gw=G))
+    um = _faces_out(ctx, bxs, n)
+    A.extrap_vel_to_faces(vd, G, dt, um[0], um[1], um[2], umac_ng=1)
+    o2 = vg[:, G - 2:vg.shape[1] - G + 2, G - 2:vg.shape[2] - G + 2, G - 2:vg.shape[3] - G + 2]
+    ref = orc.gd_extrap_vel(g, o2, bc, dt)
+    for d in range(3):
+        _, full = _canon_faces(um[d], bxs, 1, d, n)
+        assert np.max(np.abs(full - ref[d])) <= 1e-12 * np.max(np.abs(ref[d])), d
+    assert np.all(ref[0][:, :, 0] == inflow[0]) and np.all(ref[1][:, 0, :] == 0.0)
+    # velocity convective term with those MAC velocities
+    umd = [ctx.array(B.faces_to_boxed(ref[d], bxs, 1, d)) for d in range(3)]
+    conv = ctx.array(B.field_size(bxs, 0, 3))
+    faces = [ctx.array(3 * B.face_field_size(bxs, 0, d)) for d in range(3)]
+    flux = [ctx.array(3 * B.face_field_size(bxs, 0, d)) for d in range(3)]
+    A.convective_term(vd, G, umd[0], umd[1], umd[2], 1, dt, conv, 0, faces=faces, fluxes=flux)
+    st = orc.gd_edge_states(g, o2, bc, ref, dt)
+    cref = orc.gd_conv(g, vel, ref, st)
+    got = B.from_boxed(conv.download(), bxs, 0, (n[2], n[1], n[0]), ncomp=3)
+    assert np.max(np.abs(got - cref)) <= 1e-11 * np.max(np.abs(cref))
+    for d in range(3):
+        S = _split_faces_full(faces[d].download(), bxs, d, 3, n)
+        F = _split_faces_full(flux[d].download(), bxs, d, 3, n)
+        for q in range(3):
+            assert np.max(np.abs(S[q] - st[d][q])) <= 1e-12 * np.max(np.abs(st[d][q])), (d, q)
+            fr = ref[d] * st[d][q]
+            assert np.max(np.abs(F[q] - fr)) <= 1e-12 * max(np.max(np.abs(fr)), 1e-300), (d, q)
+    # density (conservative) and a tracer (convective, hoextrap walls)
+    sc = np.stack([1.0 + 0.2 * _field(n, 11)[0], _field(n, 12)[1]])
+    sgh = np.pad(sc, ((0, 0), (G, G), (G, G), (G, G)), mode="wrap")
+    sgh[0, :, :, :G] = 1.3
+    sgh[1, :, :, :G] = 0.4
+    sgh[:, :, :, -G:] = sgh[:, :, :, -G - 1:-G]
+    sgh[:, :, :G, :] = sgh[:, :, G:G + 1, :]
+    sgh[:, :, -G:, :] = sgh[:, :, -G - 1:-G, :]
+    sbc = np.zeros((2, 6), int)
+    sbc[:, 0], sbc[:, 3] = EXT, FOE
+    sbc[0, 1], sbc[0, 4] = FOE, FOE
+    sbc[1, 1], sbc[1, 4] = HOE, HOE
+    sd = ctx.array(B.set_ghosts_from(np.zeros(B.field_size(bxs, G, 2)), bxs, G, sgh, ncomp=2, gw=G))
+    sconv = ctx.array(B.field_size(bxs, 0, 2))
+    sfaces = [ctx.array(2 * B.face_field_size(bxs, 0, d)) for d in range(3)]
+    A.scalar_term(sd, G, 2, umd[0], umd[1], umd[2], 1, dt, sconv, 0, iconserv=[1, 0],
+                  faces=sfaces, bcrec=sbc)
+    s2 = sgh[:, G - 2:sgh.shape[1] - G + 2, G - 2:sgh.shape[2] - G + 2, G - 2:sgh.shape[3] - G + 2]
+    sst = orc.gd_edge_states(g, s2, sbc, ref, dt, iconserv=[1, 0])
+    scref = orc.gd_conv(g, sc, ref, sst, iconserv=[1, 0])
+    sgot = B.from_boxed(sconv.download(), bxs, 0, (n[2], n[1], n[0]), ncomp=2)
+    for q in range(2):
+        assert np.max(np.abs(sgot[q] - scref[q])) <= 1e-11 * np.max(np.abs(scref[q])), q
+    for d in range(3):
+        S = _split_faces_full(sfaces[d].download(), bxs, d, 2, n)
+        for q in range(2):
+            assert np.max(np.abs(S[q] - sst[d][q])) <= 1e-12 * np.max(np.abs(sst[d][q])), (d, q)
+    assert np.all(sst[0][0][:, :, 0] == 1.3)
+    A.close()
+
+
+def _split_faces_full(raw, bxs, d, ncomp, n):
+    """as _split_faces, keeping all n_d + 1 faces"""
+    per = B.face_field_size(bxs, 0, d)
+    boxed = [np.zeros(per) for _ in range(ncomp)]
+    off = 0
+    for lo, hi in bxs:
+        vol = int(np.prod(B.face_box_dims(lo, hi, 0, d)))
+        for q in range(ncomp):
+            boxed[q][off:off + vol] = raw[ncomp * off + q * vol: ncomp * off + (q + 1) * vol]
+        off += vol
+    shape = [n[2], n[1], n[0]]
+    shape[2 - d] += 1
+    return [B.faces_from_boxed(boxed[q], bxs, 0, d, tuple(shape)) for q in range(ncomp)]
+
+
 def test_advection_errors(ctx):
     from incflo_amd import mgx
     n = (16, 16, 16)
     bxs = B.chop_domain(n, 8)
-    with pytest.raises(mgx.MgxError) as e:
-        mgx.Advection(ctx, mgx.make_level(n, bxs, periodic=(0, 1, 1)))
-    assert e.value.code == mgx.MGX_ERR_UNSUPPORTED
+    Ab = mgx.Advection(ctx, mgx.make_level(n, bxs, periodic=(0, 1, 1)))
+    vb = ctx.array(B.field_size(bxs, 2, 3))
+    umb = _faces_out(ctx, bxs, n)
+    with pytest.raises(mgx.MgxError) as e:  # no velocity BC record on a bounded domain
+        Ab.extrap_vel_to_faces(vb, 2, 0.01, umb[0], umb[1], umb[2], umac_ng=1)
+    assert e.value.code == mgx.MGX_ERR_STATE
+    bc = np.zeros((3, 6), int)
+    with pytest.raises(mgx.MgxError) as e:  # x is not periodic: int_dir refused there
+        Ab.set_velocity_bcrec(bc)
+    assert e.value.code == mgx.MGX_ERR_ARG
+    bc[:, 0], bc[:, 3] = EXT, FOE
+    Ab.set_velocity_bcrec(bc)
+    with pytest.raises(mgx.MgxError) as e:  # one ghost cell is too few for PPM at the faces
+        Ab.extrap_vel_to_faces(ctx.array(B.field_size(bxs, 1, 3)), 1, 0.01, umb[0], umb[1],
+                               umb[2], umac_ng=1)
+    assert e.value.code == mgx.MGX_ERR_ARG
+    Ab.close()
     A = mgx.Advection(ctx, mgx.make_level(n, bxs))
     vd = ctx.array(B.field_size(bxs, 3, 3))
     um = _faces_out(ctx, bxs, n)

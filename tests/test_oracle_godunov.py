@@ -218,3 +218,100 @@ def test_scalar_conservative_form_conserves_and_reduces_to_convective(orc):
     a = orc.godunov_scalar_states(g, s, U, dt, iconserv=[1])
     b = orc.godunov_scalar_states(g, s, U, dt, iconserv=[0])
     assert np.max(np.abs(a - b)) <= 1e-14
+
+
+# ---- domain boundaries (inflow ext_dir, outflow foextrap, walls) ------------
+EXT, FOE, HOE, REVEN, RODD, INT = 3, 2, 4, 1, -1, 0
+
+
+def _ghosted(s, per, lo_fill, hi_fill):
+    """s: (nc, nz, ny, nx) -> (nc, nz+4, ny+4, nx+4); periodic dims wrap,
+    x (the only non-periodic dim here) takes lo_fill / hi_fill ghost columns
+    (callables (c, k) -> (nc, nz+4, ny+4) values for ghost column c) or 'edge'."""
+    out = np.pad(s, ((0, 0), (2, 2), (2, 2), (2, 2)), mode="wrap")
+    if not per[0]:
+        for c, fill in ((0, lo_fill), (1, lo_fill), (-1, hi_fill), (-2, hi_fill)):
+            src = 2 if c >= 0 else -3
+            out[:, :, :, c] = out[:, :, :, src] if fill == "edge" else fill(c)
+    return out
+
+
+def test_boundaries_constant_state_kept(orc):
+    # inflow / outflow in x, periodic y, z; a constant state with matching
+    # ghosts stays exactly constant on every face, both transverse forms
+    n = (12, 8, 10)
+    g = orc.Geom.make(n, periodic=(0, 1, 1), bclo=(3, 0, 0), bchi=(2, 0, 0))
+    rng = np.random.default_rng(21)
+    um = [0.5 * rng.standard_normal(orc.face_shape(g, d)) for d in range(3)]
+    for d in (1, 2):  # periodic directions: the duplicate face repeats face 0
+        ax = 2 - d
+        sl0 = [slice(None)] * 3; sl0[ax] = 0
+        sl1 = [slice(None)] * 3; sl1[ax] = -1
+        um[d][tuple(sl1)] = um[d][tuple(sl0)]
+    s = np.full((2,) + g.cell_shape(), 1.7)
+    sg = _ghosted(s, (0, 1, 1), "edge", "edge")
+    bc = np.zeros((2, 6), int)
+    bc[:, 0], bc[:, 3] = EXT, FOE
+    st = orc.gd_edge_states(g, sg, bc, um, 0.3 / 12, iconserv=[1, 0])
+    for d in range(3):
+        assert np.max(np.abs(st[d] - 1.7)) <= 1e-14, d
+    conv = orc.gd_conv(g, s, um, st, iconserv=[1, 0])
+    divu = sum(np.diff(um[d], axis=2 - d) * n[d] for d in range(3))
+    assert np.max(np.abs(conv[0] + 1.7 * divu)) <= 1e-12 * np.abs(divu).max()
+    assert np.max(np.abs(conv[1])) <= 1e-12 * np.abs(divu).max()
+
+
+def _inflow_error(orc, m, U=0.8):
+    """s(x, t) = f(x - U t) through an inflow face at x = 0 (ext_dir ghosts
+    hold the exact boundary values at t^{n+1/2}) and out through x = 1
+    (foextrap); returns the max error of the x-face states at t^{n+1/2}."""
+    n = (m, 4, 4)
+    g = orc.Geom.make(n, periodic=(0, 1, 1), bclo=(3, 0, 0), bchi=(2, 0, 0),
+                      prob_hi=(1.0, 4.0 / m, 4.0 / m))
+    h = 1.0 / m
+    dt = 0.5 * h / U
+    f = lambda x: np.sin(2 * np.pi * x) + 0.5 * np.cos(4 * np.pi * x)
+    F = lambda x: -np.cos(2 * np.pi * x) / (2 * np.pi) + 0.5 * np.sin(4 * np.pi * x) / (4 * np.pi)
+    xl = np.arange(-2, m + 2) * h
+    avg = (F(xl + h) - F(xl)) / h  # cell averages at t^n, ghosts included
+    s = np.broadcast_to(avg[2:-2], (1,) + g.cell_shape()).copy()
+    sg = _ghosted(s, (0, 1, 1), "edge", "edge")
+    sg[:, :, :, 0] = sg[:, :, :, 1] = f(0.0)  # boundary value on the inflow face
+    sg[:, :, :, -2:] = avg[-2:]              # smooth continuation downstream
+    um = [np.full(orc.face_shape(g, 0), U), np.zeros(orc.face_shape(g, 1)),
+          np.zeros(orc.face_shape(g, 2))]
+    bc = np.zeros((1, 6), int)
+    bc[0, 0], bc[0, 3] = EXT, FOE
+    st = orc.gd_edge_states(g, sg, bc, um, dt)
+    xf = np.arange(m + 1) * h
+    exact = f(xf - U * dt / 2)
+    assert np.all(st[0][0, :, :, 0] == f(0.0))  # the inflow face carries the boundary value
+    return np.max(np.abs(st[0][0, 0, 0, 1:] - exact[1:]))
+
+
+def test_boundaries_inflow_translation_converges(orc):
+    e1, e2 = _inflow_error(orc, 32), _inflow_error(orc, 64)
+    assert e2 < 5e-3
+    assert e1 / e2 > 3.0, (e1, e2)
+
+
+def test_boundaries_walls_and_outflow_rules(orc):
+    # x-lo no-slip wall (all components ext_dir 0), x-hi outflow (foextrap):
+    # the wall face carries u = 0 and zero tangential states; on the outflow
+    # face an inward normal velocity is clipped to 0
+    n = (10, 8, 8)
+    g = orc.Geom.make(n, periodic=(0, 1, 1), bclo=(3, 0, 0), bchi=(2, 0, 0))
+    rng = np.random.default_rng(22)
+    vel = rng.standard_normal((3,) + g.cell_shape())
+    vel[0][:, :, -1] = -np.abs(vel[0][:, :, -1]) - 0.5  # flowing back into the domain at x-hi
+    vg = _ghosted(vel, (0, 1, 1), lambda c: 0.0, "edge")
+    bc = np.zeros((3, 6), int)
+    bc[:, 0], bc[:, 3] = EXT, FOE
+    dt = 0.2 / 10
+    um = orc.gd_extrap_vel(g, vg, bc, dt)
+    assert np.all(um[0][:, :, 0] == 0.0)
+    assert np.all(um[0][:, :, -1] == 0.0)
+    st = orc.gd_edge_states(g, vg, bc, um, dt)
+    assert np.all(st[0][:, :, :, 0] == 0.0)
+    # the outflow face takes the interior trace (not the ghost) for every component
+    assert np.max(np.abs(st[0][1:, :, :, -1])) > 1e-3
