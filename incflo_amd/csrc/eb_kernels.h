@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "box_kernels.h"
 #include "nodal_kernels.h"
 
 namespace mgx {
@@ -231,6 +232,56 @@ static __global__ void __launch_bounds__(256) eb_rhs_kernel(KGeom g, const int* 
         }
     }
     rhs[id] = acc;
+}
+
+// EB inflow velocity: rhs += sum over the cut cells around each active node of
+// B[corner] . u_eb (B: 24 per cut cell, (1/V) int over the body surface of
+// N_p n_d dA with n out of the fluid; ueb: 3 x cells, x fastest)
+static __global__ void __launch_bounds__(256) eb_inflow_rhs_kernel(KGeom g, const int* __restrict__ cidx,
+                                                                   const double* __restrict__ B,
+                                                                   const unsigned char* __restrict__ mask,
+                                                                   const double* __restrict__ ueb,
+                                                                   double* __restrict__ rhs) {
+    int i, j, k;
+    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
+    if (!mask[id]) return;
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    double acc = 0.0;
+    for (int a = 0; a < 8; ++a) {
+        const int cd[3] = {a & 1, (a >> 1) & 1, (a >> 2) & 1};
+        const int n3[3] = {i, j, k};
+        int c[3];
+        bool inside = true;
+        for (int d = 0; d < 3; ++d) {
+            int cc = n3[d] - 1 + cd[d];
+            if (g.per[d]) cc = cc < 0 ? cc + g.n[d] : (cc >= g.n[d] ? cc - g.n[d] : cc);
+            else if (cc < 0 || cc >= g.n[d]) inside = false;
+            c[d] = cc;
+        }
+        if (!inside) continue;
+        const long cl = c[0] + (long)g.n[0] * (c[1] + (long)g.n[1] * c[2]);
+        const int ci = cidx[cl];
+        if (ci < 0) continue;
+        const int p = (1 - cd[0]) | ((1 - cd[1]) << 1) | ((1 - cd[2]) << 2);
+        for (int d = 0; d < 3; ++d) acc += B[24L * ci + 3 * p + d] * ueb[d * nc + cl];
+    }
+    rhs[id] += acc;
+}
+
+// canonical cells (ncomp x cells, no ghosts) <- boxed cells (valid region)
+static __global__ void __launch_bounds__(256) gather_cells_kernel(KGeom g, const BoxInfo* boxes,
+                                                                  const Tile* tiles,
+                                                                  const double* __restrict__ in,
+                                                                  int ng, int ncomp,
+                                                                  double* __restrict__ out) {
+    int bi, i, j, k;
+    if (!tile_point(boxes, tiles, REG_CELL, bi, i, j, k)) return;
+    const BoxInfo& b = boxes[bi];
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    const long o = field_offset(b, ng, ncomp, false);
+    const long c = i + (long)g.n[0] * (j + (long)g.n[1] * k);
+    for (int q = 0; q < ncomp; ++q) out[q * nc + c] = in[o + box_index(b, ng, false, i, j, k, q)];
 }
 
 }  // namespace mgx

@@ -18,6 +18,13 @@ namespace mgx {
 void eb_cylinder_moments(const NodalGeom& g, const double prob_lo[3], double cx, double cy,
                          double r, std::vector<int>& type, std::vector<double>& mom);
 
+// Body-surface weights of the same cylinder, 24 per cell ([corner][d], zero
+// outside cut cells): (1/V) int over the surface in the cell of N_p n_d dA with
+// n out of the fluid, for the EB inflow velocity term of the divergence.
+void eb_cylinder_surface_weights(const NodalGeom& g, const double prob_lo[3], double cx,
+                                 double cy, double r, const std::vector<int>& type,
+                                 std::vector<double>& B);
+
 // Face apertures (fluid fraction of each face, canonical face arrays with
 // n_d + 1 faces along d) and covered-cell flags of the same cylinder.
 void eb_cylinder_apertures(const int n[3], const double prob_lo[3], const double dx[3], double cx,

@@ -29,13 +29,12 @@ const double kGL16W[16] = {0.0271524594117541, 0.0622535239386479, 0.09515851168
                            0.1246289712555339, 0.0951585116824928, 0.0622535239386479,
                            0.0271524594117541};
 
-// moments (unit cell) of the part of the cell [x0,x0+hx]x[y0,y0+hy] inside the
-// disk: Green's theorem with Q = hx X^{p+1} Y^q / (p+1), counter-clockwise
-// boundary = arcs of the circle inside the cell + the cell's right edge inside
-// the disk (the left edge has X = 0, horizontal edges dy = 0).
-void covered_moments(double x0, double y0, double hx, double hy, double cx, double cy, double r,
-                     double Mc[9]) {
-    for (int q = 0; q < 9; ++q) Mc[q] = 0.0;
+// Calls f(t, w) for the Gauss-Legendre points (angle t, weight w in angle) of
+// the arcs of the circle that lie inside the cell [x0,x0+hx]x[y0,y0+hy]
+// (counter-clockwise, pieces of at most pi/8, 16 points each).
+template <class F>
+void for_arc_points(double x0, double y0, double hx, double hy, double cx, double cy, double r,
+                    F&& f) {
     const double x1 = x0 + hx, y1 = y0 + hy;
     const double twopi = 2.0 * M_PI;
     auto norm = [&](double t) {
@@ -78,21 +77,8 @@ void covered_moments(double x0, double y0, double hx, double hy, double cx, doub
         const int np = std::max(1, (int)std::ceil((tb - ta) / (M_PI / 8)));
         for (int s = 0; s < np; ++s) {
             const double a = ta + (tb - ta) * s / np, b = ta + (tb - ta) * (s + 1) / np;
-            for (int g = 0; g < 16; ++g) {
-                const double t = 0.5 * (a + b) + 0.5 * (b - a) * kGL16X[g];
-                const double w = 0.5 * (b - a) * kGL16W[g];
-                const double X = (cx + r * std::cos(t) - x0) / hx, Y = (cy + r * std::sin(t) - y0) / hy;
-                const double dy = r * std::cos(t);
-                double Xp = X;  // X^{p+1}
-                for (int p = 0; p < 3; ++p) {
-                    double Yq = 1.0;
-                    for (int q = 0; q < 3; ++q) {
-                        Mc[p * 3 + q] += w * hx * Xp * Yq / (p + 1) * dy;
-                        Yq *= Y;
-                    }
-                    Xp *= X;
-                }
-            }
+            for (int g = 0; g < 16; ++g)
+                f(0.5 * (a + b) + 0.5 * (b - a) * kGL16X[g], 0.5 * (b - a) * kGL16W[g]);
         }
     };
     if (th.empty()) {
@@ -104,6 +90,29 @@ void covered_moments(double x0, double y0, double hx, double hy, double cx, doub
             if (inside(0.5 * (ta + tb))) arc(ta, tb);
         }
     }
+}
+
+// moments (unit cell) of the part of the cell [x0,x0+hx]x[y0,y0+hy] inside the
+// disk: Green's theorem with Q = hx X^{p+1} Y^q / (p+1), counter-clockwise
+// boundary = arcs of the circle inside the cell + the cell's right edge inside
+// the disk (the left edge has X = 0, horizontal edges dy = 0).
+void covered_moments(double x0, double y0, double hx, double hy, double cx, double cy, double r,
+                     double Mc[9]) {
+    for (int q = 0; q < 9; ++q) Mc[q] = 0.0;
+    const double x1 = x0 + hx, y1 = y0 + hy;
+    for_arc_points(x0, y0, hx, hy, cx, cy, r, [&](double t, double w) {
+        const double X = (cx + r * std::cos(t) - x0) / hx, Y = (cy + r * std::sin(t) - y0) / hy;
+        const double dy = r * std::cos(t);
+        double Xp = X;  // X^{p+1}
+        for (int p = 0; p < 3; ++p) {
+            double Yq = 1.0;
+            for (int q = 0; q < 3; ++q) {
+                Mc[p * 3 + q] += w * hx * Xp * Yq / (p + 1) * dy;
+                Yq *= Y;
+            }
+            Xp *= X;
+        }
+    });
     // right edge x = x1, upwards, the part inside the disk
     const double dxe = x1 - cx;
     if (std::fabs(dxe) < r) {
@@ -117,6 +126,28 @@ void covered_moments(double x0, double y0, double hx, double hy, double cx, doub
         }
     }
     for (int q = 0; q < 9; ++q) Mc[q] /= hx * hy;
+}
+
+// (1/V) int over the body surface in the cell of N_p n dA, n out of the fluid
+// (toward the axis), corners p (x bit 0, y bit 1, z bit 2), [p][d]: the
+// z-factor of N_p integrates to 1/2 over the cell height and n_z = 0
+void surface_weights(double x0, double y0, double hx, double hy, double cx, double cy, double r,
+                     double B[24]) {
+    double B2[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for_arc_points(x0, y0, hx, hy, cx, cy, r, [&](double t, double w) {
+        const double X = (cx + r * std::cos(t) - x0) / hx, Y = (cy + r * std::sin(t) - y0) / hy;
+        const double nx = -std::cos(t), ny = -std::sin(t), ds = r * w;
+        for (int c = 0; c < 4; ++c) {
+            const double N = ((c & 1) ? X : 1.0 - X) * ((c & 2) ? Y : 1.0 - Y);
+            B2[c][0] += N * nx * ds;
+            B2[c][1] += N * ny * ds;
+        }
+    });
+    for (int p = 0; p < 8; ++p) {
+        B[3 * p + 0] = 0.5 * B2[p & 3][0] / (hx * hy);
+        B[3 * p + 1] = 0.5 * B2[p & 3][1] / (hx * hy);
+        B[3 * p + 2] = 0.0;
+    }
 }
 
 // integral over the fluid of prod_e f_e(x_e), each f_e a polynomial with
@@ -200,6 +231,24 @@ void eb_cylinder_moments(const NodalGeom& g, const double prob_lo[3], double cx,
                     for (int p = 0; p < 3; ++p)
                         for (int q = 0; q < 3; ++q)
                             for (int s = 0; s < 3; ++s) m[p + 3 * q + 9 * s] = m2[p * 3 + q] / (s + 1);
+            }
+        }
+}
+
+void eb_cylinder_surface_weights(const NodalGeom& g, const double prob_lo[3], double cx,
+                                 double cy, double r, const std::vector<int>& type,
+                                 std::vector<double>& B) {
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    B.assign(24 * nc, 0.0);
+    for (int j = 0; j < g.n[1]; ++j)
+        for (int i = 0; i < g.n[0]; ++i) {
+            if (type[i + (long)g.n[0] * j] != 2) continue;  // the type is z-independent
+            double b[24];
+            surface_weights(prob_lo[0] + i * g.dx[0], prob_lo[1] + j * g.dx[1], g.dx[0], g.dx[1], cx,
+                            cy, r, b);
+            for (int k = 0; k < g.n[2]; ++k) {
+                const long c = i + (long)g.n[0] * (j + (long)g.n[1] * k);
+                std::copy(b, b + 24, &B[24 * c]);
             }
         }
 }

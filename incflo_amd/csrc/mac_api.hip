@@ -37,6 +37,8 @@ struct mgx_mac_s {
     bool eb_on = false;
     double eb_cyl[3] = {0, 0, 0};
     DevBuf eb_ap[3], eb_cov;
+    DevBuf ueb;  // EB inflow velocity (3 x cells)
+    bool has_ueb = false;
     void build_eb() {
         if (!eb_on) return;
         std::vector<double> ap[3];
@@ -346,6 +348,11 @@ MGX_API int mgx_mac_project(mgx_mac h, double* u, double* v, double* w, int umac
             mac_div_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
                 k, h->u[0].as<double>(), h->u[1].as<double>(), h->u[2].as<double>(),
                 1.0 / h->g.dx[0], 1.0 / h->g.dx[1], 1.0 / h->g.dx[2], h->rhs.as<double>());
+        if (h->eb_on && h->has_ueb)
+            eb_mac_inflow_kernel<<<grid3(h->g.n[0], h->g.n[1], h->g.n[2]), block3(), 0, h->stream()>>>(
+                k, h->ueb.as<double>(), h->eb_ap[0].as<double>(), h->eb_ap[1].as<double>(),
+                h->eb_ap[2].as<double>(), 1.0 / h->g.dx[0], 1.0 / h->g.dx[1], 1.0 / h->g.dx[2],
+                h->rhs.as<double>());
         h->robin_rhs(h->rhs.as<double>());
         if (phi) h->cells_in(phi, phi_ng, h->solver.phi);
         else MGX_HIP(hipMemsetAsync(h->solver.phi, 0, h->g.ncell() * sizeof(double), h->stream()));
@@ -470,6 +477,25 @@ MGX_API int mgx_mac_set_level_bc(mgx_mac h, const double* robin_a, const double*
         h->rob_has_f = has_f;
         h->g.robin_fixes = h->rob_set && fixes;
         h->solver.set_robin_fixes(h->g.robin_fixes);
+    });
+}
+
+MGX_API int mgx_mac_set_eb_inflow_velocity(mgx_mac h, const double* eb_vel, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        if (!eb_vel) {
+            h->has_ueb = false;
+            return;
+        }
+        MGX_REQUIRE(ng >= 0 && ng <= 8, MGX_ERR_ARG, "EB inflow velocity: bad ghost width");
+        MGX_REQUIRE(h->eb_on, MGX_ERR_STATE, "EB inflow velocity: no embedded boundary set");
+        h->ensure_setup();
+        const long nc = h->g.ncell();
+        h->ueb.alloc(3 * nc * sizeof(double));
+        gather_cells_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
+            h->kg(), h->layout.dev_boxes(), h->layout.tiles(REG_CELL), eb_vel, ng, 3,
+            h->ueb.as<double>());
+        h->has_ueb = true;
     });
 }
 

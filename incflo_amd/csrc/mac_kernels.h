@@ -455,6 +455,24 @@ static __global__ void eb_mac_face_update_kernel(MGeomK g, int d, const double* 
     else u[id] -= F;
 }
 
+// EB inflow velocity (setEBInflowVelocity) in the MAC divergence: the body
+// surface in a cell carries int u_eb . n dA with n out of the fluid, and by the
+// divergence theorem on the fluid part of the cell int n_d dA = -(a_d+ - a_d-) h^2,
+// so rhs += -sum_d (a_d+ - a_d-) u_eb,d / h_d (zero in regular cells)
+static __global__ void eb_mac_inflow_kernel(MGeomK g, const double* __restrict__ ueb,
+                                            const double* __restrict__ ax,
+                                            const double* __restrict__ ay,
+                                            const double* __restrict__ az, double ihx, double ihy,
+                                            double ihz, double* __restrict__ rhs) {
+    int i, j, k;
+    if (!grid_point(g.n[0], g.n[1], i, j, k)) return;
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    const long c = cidx(g, i, j, k);
+    rhs[c] -= (ax[fidx(g, 0, i + 1, j, k)] - ax[fidx(g, 0, i, j, k)]) * ueb[c] * ihx +
+              (ay[fidx(g, 1, i, j + 1, k)] - ay[fidx(g, 1, i, j, k)]) * ueb[nc + c] * ihy +
+              (az[fidx(g, 2, i, j, k + 1)] - az[fidx(g, 2, i, j, k)]) * ueb[2 * nc + c] * ihz;
+}
+
 // phi = 0 in covered cells
 static __global__ void eb_zero_covered_kernel(const unsigned char* __restrict__ covered,
                                               double* __restrict__ phi, long n) {

@@ -52,6 +52,10 @@ struct mgx_nodal_s {
     std::vector<int> eb_type;
     std::vector<double> eb_mom;
     std::unique_ptr<EBNodal> eb;
+    // EB inflow velocity (setEBInflowVelocity): body-surface weights per cut
+    // cell (cylinder geometry) and the caller's velocity (3 x cells)
+    DevBuf eb_bcut, ueb;
+    bool has_ueb = false;
 
     void build_eb() {
         solver.eb = nullptr;
@@ -61,6 +65,16 @@ struct mgx_nodal_s {
         eb = std::make_unique<EBNodal>();
         eb->setup(g, solver.nlev, eb_type, eb_mom, stream());
         solver.eb = eb.get();
+        if (eb_kind == 1) {
+            std::vector<double> bw;
+            eb_cylinder_surface_weights(g, prob_lo, eb_cyl[0], eb_cyl[1], eb_cyl[2], eb_type, bw);
+            const std::vector<int>& cut = eb->L[0].cut_cells;
+            std::vector<double> bc(24 * std::max<size_t>(cut.size(), 1), 0.0);
+            for (size_t q = 0; q < cut.size(); ++q)
+                std::copy(&bw[24L * cut[q]], &bw[24L * cut[q]] + 24, &bc[24 * q]);
+            eb_bcut.alloc(bc.size() * sizeof(double));
+            MGX_HIP(hipMemcpy(eb_bcut.p, bc.data(), bc.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
     }
 
     void ensure_setup() {
@@ -88,6 +102,10 @@ struct mgx_nodal_s {
             eb_rhs_kernel<<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, stream()>>>(
                 kg, eb->cidx0(), eb->W0(), eb->mask0(), ug.as<double>(), rhs.as<double>(),
                 0.25 / g.dx[0], 0.25 / g.dx[1], 0.25 / g.dx[2]);
+            if (has_ueb && eb_kind == 1)
+                eb_inflow_rhs_kernel<<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, stream()>>>(
+                    kg, eb->cidx0(), eb_bcut.as<double>(), eb->mask0(), ueb.as<double>(),
+                    rhs.as<double>());
             return;
         }
         nodal_rhs_kernel<<<grid3(g.nn[0], g.nn[1], g.nn[2]), block3(), 0, stream()>>>(
@@ -627,6 +645,26 @@ MGX_API int mgx_nodal_set_eb_cylinder(mgx_nodal h, double cx, double cy, double 
     });
 }
 
+MGX_API int mgx_nodal_set_eb_inflow_velocity(mgx_nodal h, const double* eb_vel, int ng) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        if (!eb_vel) {
+            h->has_ueb = false;
+            return;
+        }
+        MGX_REQUIRE(ng >= 0 && ng <= 8, MGX_ERR_ARG, "EB inflow velocity: bad ghost width");
+        MGX_REQUIRE(h->eb_kind == 1, MGX_ERR_UNSUPPORTED,
+                    "EB inflow velocity: needs the cylinder geometry (its surface weights)");
+        h->ensure_setup();
+        const long nc = (long)h->n_cell[0] * h->n_cell[1] * h->n_cell[2];
+        h->ueb.alloc(3 * nc * sizeof(double));
+        gather_cells_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(
+            kgeom(h->g), h->layout.dev_boxes(), h->layout.tiles(REG_CELL), eb_vel, ng, 3,
+            h->ueb.as<double>());
+        h->has_ueb = true;
+    });
+}
+
 MGX_API int mgx_nodal_set_eb_moments(mgx_nodal h, const int* type, const double* moments) {
     if (!h) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {
@@ -663,6 +701,28 @@ extern "C" MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* lev, doubl
         eb_cylinder_moments(g, lev->prob_lo, cx, cy, r, t, m);
         std::memcpy(type, t.data(), t.size() * sizeof(int));
         std::memcpy(moments, m.data(), m.size() * sizeof(double));
+        return MGX_OK;
+    } catch (...) {
+        return MGX_ERR_ARG;
+    }
+}
+
+extern "C" MGX_API int mgx_eb_cylinder_surface_weights(const mgx_level_desc* lev, double cx,
+                                                      double cy, double r, double* weights) {
+    if (!lev || !weights || r <= 0.0) return MGX_ERR_ARG;
+    try {
+        int per[3], bc[3] = {0, 0, 0};
+        double dx[3];
+        for (int d = 0; d < 3; ++d) {
+            per[d] = lev->is_periodic[d];
+            dx[d] = (lev->prob_hi[d] - lev->prob_lo[d]) / lev->n_cell[d];
+        }
+        NodalGeom g = make_nodal_geom(lev->n_cell, per, bc, bc, dx);
+        std::vector<int> t;
+        std::vector<double> m, b;
+        eb_cylinder_moments(g, lev->prob_lo, cx, cy, r, t, m);
+        eb_cylinder_surface_weights(g, lev->prob_lo, cx, cy, r, t, b);
+        std::memcpy(weights, b.data(), b.size() * sizeof(double));
         return MGX_OK;
     } catch (...) {
         return MGX_ERR_ARG;

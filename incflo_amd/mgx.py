@@ -132,6 +132,10 @@ def lib():
                                          vp, vp],
             "mgx_mac_set_eb_cylinder": [vp, C.c_double, C.c_double, C.c_double],
             "mgx_mac_set_level_bc": [vp, vp, vp, vp, C.c_int],
+            "mgx_mac_set_eb_inflow_velocity": [vp, vp, C.c_int],
+            "mgx_nodal_set_eb_inflow_velocity": [vp, vp, C.c_int],
+            "mgx_eb_cylinder_surface_weights": [C.POINTER(LevelDesc), C.c_double, C.c_double,
+                                                C.c_double, vp],
             "mgx_eb_cylinder_apertures": [C.POINTER(LevelDesc), C.c_double, C.c_double, C.c_double,
                                           vp, vp, vp],
             "mgx_nodal_set_unet": [vp, vp, C.c_int],
@@ -281,6 +285,11 @@ class NodalProjector:
     def set_graphs(self, on):
         self.ctx.check(lib().mgx_nodal_set_graphs(self.h, int(on)))
 
+    def setEBInflowVelocity(self, eb_vel, ng=0):
+        """Body-surface velocity (device boxed cell field, 3 comps; None removes it)."""
+        self.ctx.check(lib().mgx_nodal_set_eb_inflow_velocity(
+            self.h, eb_vel.ptr if eb_vel is not None else None, ng))
+
     def setOversetMask(self, mask_dev, ng=0):
         """mask_dev: device int32 boxed nodal field (or None to remove)"""
         self.ctx.check(lib().mgx_nodal_set_overset_mask(self.h, mask_dev.ptr if mask_dev else None, ng))
@@ -397,6 +406,10 @@ class MacProjector:
     def set_eb_cylinder(self, cx, cy, r):
         """Embedded cylinder along z (fluid outside); before the coefficients."""
         self.ctx.check(lib().mgx_mac_set_eb_cylinder(self.h, cx, cy, r))
+
+    def setEBInflowVelocity(self, eb_vel, ng=0):
+        self.ctx.check(lib().mgx_mac_set_eb_inflow_velocity(
+            self.h, eb_vel.ptr if eb_vel is not None else None, ng))
 
     def setLevelBC(self, robin_a, robin_b, robin_f, ng=1):
         """Robin coefficients a phi + b dphi/dn = f, boxed cell fields whose
@@ -542,6 +555,16 @@ def eb_cylinder_geometry(level, cx, cy, r):
     if st != MGX_OK:
         raise MgxError(st, "mgx_eb_cylinder_geometry failed")
     return ct, mom
+
+
+def eb_cylinder_surface_weights(level, cx, cy, r):
+    """Host-side body-surface weights (nz, ny, nx, 8, 3) of the cylinder."""
+    n = [level.n_cell[d] for d in range(3)]
+    out = np.zeros((n[2], n[1], n[0], 8, 3))
+    st = lib().mgx_eb_cylinder_surface_weights(C.byref(level), cx, cy, r, out.ctypes.data)
+    if st != MGX_OK:
+        raise MgxError(st, "mgx_eb_cylinder_surface_weights failed")
+    return out
 
 
 def eb_cylinder_apertures(level, cx, cy, r):

@@ -206,11 +206,22 @@ MGX_API int mgx_nodal_set_eb_cylinder(mgx_nodal h, double cx, double cy, double 
  * covered, 1 regular, 2 cut) and 27 fluid moments int xi^p eta^q zeta^r over
  * the fluid part of the unit cell (index p + 3q + 9r); type NULL removes the EB. */
 MGX_API int mgx_nodal_set_eb_moments(mgx_nodal h, const int* type, const double* moments);
+
+/* getLinOp().setEBInflowVelocity(lev, eb_vel) (src/projection/incflo_apply_nodal_projection.cpp:196-200):
+ * a velocity on the body surface (boxed cell field, 3 comps, ng ghosts; the
+ * values of cut cells are used) adds int over the surface of N (u_eb . n) to
+ * the divergence, n out of the fluid.  Cylinder geometry only; NULL removes it. */
+MGX_API int mgx_nodal_set_eb_inflow_velocity(mgx_nodal h, const double* eb_vel, int ng);
 /* getLinOp().setOversetMask(lev, mask) (src/projection/incflo_apply_nodal_projection.cpp:203-211),
  * which incflo uses for mixed boundaries (src/boundary_conditions/incflo_set_bcs.cpp:11-53):
  * a boxed nodal int field (ng ghosts), 1 = solve, 0 = node fixed at phi = 0;
  * NULL removes it.  Not with embedded boundaries or the learned mode. */
 MGX_API int mgx_nodal_set_overset_mask(mgx_nodal h, const int* mask, int ng);
+/* Host-only: body-surface weights of the cylinder, 24 per cell ([corner][d],
+ * x bit 0, y bit 1, z bit 2), (1/V) int over the surface in the cell of
+ * N_p n_d dA with n out of the fluid; zero outside cut cells (no GPU). */
+MGX_API int mgx_eb_cylinder_surface_weights(const mgx_level_desc* level, double cx, double cy,
+                                            double r, double* weights);
 /* Host-only: the cylinder's face apertures on a level (canonical face arrays,
  * n_d + 1 faces along d, x fastest) (no GPU). */
 MGX_API int mgx_eb_cylinder_apertures(const mgx_level_desc* level, double cx, double cy, double r,
@@ -294,6 +305,11 @@ MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, in
  * covered cells drop out and fully covered faces get u = 0.  Call after
  * setDomainBC and before the coefficients; needs a Dirichlet face. */
 MGX_API int mgx_mac_set_eb_cylinder(mgx_mac h, double cx, double cy, double r);
+/* setEBInflowVelocity(lev, eb_vel) (src/convection/incflo_compute_MAC_projected_velocities.cpp:277-279):
+ * velocity on the body surface (boxed cell field, 3 comps): the surface flux
+ * -sum_d (a_d+ - a_d-) u_eb,d / h_d joins the divergence of each cut cell.
+ * NULL removes it. */
+MGX_API int mgx_mac_set_eb_inflow_velocity(mgx_mac h, const double* eb_vel, int ng);
 /* MLMG::solve on the MAC operator (boxed cell fields; phi holds the initial guess). */
 MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,
                           double rtol, double atol, int mode, mgx_solve_info* info);

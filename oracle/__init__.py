@@ -523,6 +523,24 @@ def eb_nodal_divergence(g, ctype, mom, ug):
     return out
 
 
+def eb_cylinder_boundary(g, ctype, mom, cx, cy, r, prob_lo=(0.0, 0.0, 0.0)):
+    """Body-surface weights B (nz,ny,nx,8,3): (1/V) int N_p n_d dA, n out of the fluid."""
+    B = np.zeros(g.cell_shape() + (8, 3))
+    lib().orc_eb_cylinder_boundary(C.byref(g), (C.c_double * 3)(*prob_lo), C.c_double(cx),
+                                   C.c_double(cy), C.c_double(r),
+                                   np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                                   _p(np.ascontiguousarray(mom)), _p(B))
+    return B
+
+
+def eb_inflow_rhs(g, ctype, B, ueb, rhs):
+    """rhs + the body-surface term of the EB inflow velocity ueb (3, nz, ny, nx)."""
+    out = np.array(rhs, dtype=np.float64)
+    lib().orc_eb_inflow_rhs(C.byref(g), np.ascontiguousarray(ctype, np.int32).ctypes.data_as(_ip),
+                            _p(np.ascontiguousarray(B)), _p(np.ascontiguousarray(ueb)), _p(out))
+    return out
+
+
 def eb_nodal_gradient(g, ctype, mom, phi):
     out = np.zeros((3,) + g.cell_shape())
     lib().orc_eb_nodal_gradient(C.byref(g),

@@ -168,6 +168,10 @@ void orc_convective_term(const orc_geom* g, const double* u, const double* v, co
                          const double* states, double* conv);
 
 /* ---- embedded boundary, nodal (orc_eb.c) ------------------------------- */
+void orc_eb_cylinder_boundary(const orc_geom* g, const double prob_lo[3], double cx, double cy,
+                              double r, const int* ctype, const double* mom, double* B);
+void orc_eb_inflow_rhs(const orc_geom* g, const int* ctype, const double* B, const double* ueb,
+                       double* rhs);
 /* ctype per cell: 0 covered, 1 regular, 2 cut; mom: 27 fluid moments per cell */
 void orc_eb_cylinder(const orc_geom* g, const double prob_lo[3], double cx, double cy, double r,
                      int* ctype, double* mom);

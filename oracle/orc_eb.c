@@ -533,3 +533,100 @@ int orc_eb_nodal_solve(const orc_geom* g, const int* ctype, const double* mom,
     free(r); free(z); free(p); free(q); free(dg); free(mask); free(K);
     return conv ? 0 : 1;
 }
+
+/* ---- EB inflow velocity (setEBInflowVelocity) ---------------------------
+ * With a velocity u_eb on the body surface the weak divergence gains the
+ * surface term: b_n += sum_cells (1/V) int_{body surface in cell} N_n (u_eb . n) dA,
+ * n pointing out of the fluid (src/projection/incflo_apply_nodal_projection.cpp:196-200).
+ * The surface weights B[p][d] = (1/V) int N_p n_d dA follow from the divergence
+ * theorem on the fluid part of the cell: the volume integral of d N_p / d x_d
+ * (the weights W above, from the fluid moments) minus the integrals over the
+ * fluid parts of the cell faces, which for the z-cylinder are exact segment
+ * integrals (x- and y-faces) and the 2-D fluid moments (z-faces).  This is
+ * independent of the product's arc quadrature. */
+
+/* int over the fluid part of the horizontal line eta = const of xi^q (exact) */
+static void row_moments(const sq_t* S, double eta, double f[3])
+{
+    const double y = S->y0 + S->hy * eta;
+    const double dy = y - S->cy;
+    double lo = 1.0, hi = 0.0;
+    if (dy * dy < S->r * S->r) {
+        const double h = sqrt(S->r * S->r - dy * dy);
+        lo = fmax((S->cx - h - S->x0) / S->hx, 0.0);
+        hi = fmin((S->cx + h - S->x0) / S->hx, 1.0);
+    }
+    for (int q = 0; q < 3; ++q) {
+        double v = 1.0 / (q + 1);
+        if (hi > lo) v -= (pow(hi, q + 1) - pow(lo, q + 1)) / (q + 1);
+        f[q] = v;
+    }
+}
+
+void orc_eb_cylinder_boundary(const orc_geom* g, const double prob_lo[3], double cx, double cy,
+                              double r, const int* ctype, const double* mom, double* B)
+{
+    const long nc = (long)g->n[0] * g->n[1] * g->n[2];
+    for (long c = 0; c < nc; ++c) {
+        double* b = B + 24 * c;
+        for (int q = 0; q < 24; ++q) b[q] = 0.0;
+        if (ctype[c] != 2) continue;
+        const int i = (int)(c % g->n[0]), j = (int)((c / g->n[0]) % g->n[1]);
+        sq_t S = {prob_lo[0] + i * g->dx[0], prob_lo[1] + j * g->dx[1], g->dx[0], g->dx[1], cx, cy, r};
+        double W[8][3];
+        eb_grad_weights(mom + 27 * c, g->dx, W);
+        double fx[2][3], fy[2][3];  /* fluid moments along the x-faces (in eta) and y-faces (in xi) */
+        line_moments(&S, 0.0, fx[0]);
+        line_moments(&S, 1.0, fx[1]);
+        row_moments(&S, 0.0, fy[0]);
+        row_moments(&S, 1.0, fy[1]);
+        const double* m = mom + 27 * c;
+        for (int p = 0; p < 8; ++p) {
+            const int bx = p & 1, by = (p >> 1) & 1, bz = (p >> 2) & 1;
+            /* x: the x-face the corner lies on, int N_y over its fluid segment, int N_z = 1/2 */
+            const double Ny = by ? fx[bx][1] : fx[bx][0] - fx[bx][1];
+            b[3 * p + 0] = W[p][0] - (bx ? 1.0 : -1.0) / g->dx[0] * Ny * 0.5;
+            const double Nx = bx ? fy[by][1] : fy[by][0] - fy[by][1];
+            b[3 * p + 1] = W[p][1] - (by ? 1.0 : -1.0) / g->dx[1] * Nx * 0.5;
+            /* z: the 2-D fluid moment of N_x N_y (z-extruded fluid) */
+            const double a00 = m[0], a10 = m[1], a01 = m[3], a11 = m[4];
+            const double nxny = bx ? (by ? a11 : a10 - a11) : (by ? a01 - a11 : a00 - a10 - a01 + a11);
+            b[3 * p + 2] = W[p][2] - (bz ? 1.0 : -1.0) / g->dx[2] * nxny;
+        }
+    }
+}
+
+/* rhs += sum over the cut cells around each active node of B[p] . u_eb
+ * (ueb: 3 x cells, x fastest) */
+void orc_eb_inflow_rhs(const orc_geom* g, const int* ctype, const double* B, const double* ueb,
+                       double* rhs)
+{
+    const int nx = nn_of(g, 0), ny = nn_of(g, 1), nz = nn_of(g, 2);
+    const long nc = (long)g->n[0] * g->n[1] * g->n[2];
+    int* mask = (int*)malloc(sizeof(int) * (size_t)nx * ny * nz);
+    orc_eb_node_mask(g, ctype, mask);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const long id = IDX(i, j, k, nx, ny);
+                if (!mask[id]) continue;
+                int n[3] = {i, j, k};
+                double acc = 0.0;
+                for (int a = 0; a < 8; ++a) {
+                    int c[3], inside = 1;
+                    for (int d = 0; d < 3; ++d) {
+                        int cw = n[d] - 1 + ((a >> d) & 1);
+                        if (g->periodic[d]) cw = ((cw % g->n[d]) + g->n[d]) % g->n[d];
+                        if (cw < 0 || cw >= g->n[d]) inside = 0;
+                        c[d] = cw;
+                    }
+                    if (!inside) continue;
+                    const long cc = IDX(c[0], c[1], c[2], g->n[0], g->n[1]);
+                    if (ctype[cc] != 2) continue;
+                    const int p = (1 - (a & 1)) | ((1 - ((a >> 1) & 1)) << 1) | ((1 - ((a >> 2) & 1)) << 2);
+                    for (int d = 0; d < 3; ++d) acc += B[24 * cc + 3 * p + d] * ueb[d * nc + cc];
+                }
+                rhs[id] += acc;
+            }
+    free(mask);
+}

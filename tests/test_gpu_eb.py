@@ -201,3 +201,128 @@ def test_eb_mac_projection_channel(ctx, orc):
     div1 = orc.mac_divergence(g, [ap[d] * un[d] for d in range(3)])
     assert np.max(np.abs(div1)) <= 1.01e-12 * np.max(np.abs(b))
     M.close()
+
+
+def test_eb_inflow_velocity_uniform_flow_is_kept(ctx, orc):
+    """setEBInflowVelocity: a uniform flow that also crosses the body surface
+    is divergence free, so the projection keeps it (and phi stays 0); without
+    the body velocity the same field is changed at the cut cells."""
+    from incflo_amd import mgx
+    _say("case eb inflow uniform flow")
+    g, bxs, P, ct, mom = _setup(ctx, orc)
+    U = np.array([0.7, 0.0, 0.25])
+    u = np.broadcast_to(U[:, None, None, None], (3,) + g.cell_shape()).copy()
+    inflow = np.zeros((6, 3))
+    inflow[0] = U
+    ueb = ctx.array(B.to_boxed(u, bxs, 0))
+    vol = mom[..., 0]
+    fluid = vol >= 1e-3
+
+    def project(with_eb_vel):
+        P.setEBInflowVelocity(ueb if with_eb_vel else None, 0)
+        vd = ctx.array(B.to_boxed(u, bxs, 1, periodic=(0, 0, 1)))
+        gpd = ctx.array(B.field_size(bxs, 0, 3))
+        pd = ctx.array(B.field_size(bxs, 0, 1, True))
+        info = P.apply_nodal_projection(vd, 1, gpd, 0, pd, 0, 0.01, const_rho=1.0,
+                                        inflow_vel=inflow, rtol=1e-12, atol=1e-12,
+                                        mode=mgx.MODE_FP64)
+        assert info.converged
+        return (B.from_boxed(vd.download(), bxs, 1, g.cell_shape(), ncomp=3),
+                B.from_boxed(pd.download(), bxs, 0, g.node_shape(), nodal=True))
+
+    un, p = project(True)
+    assert np.max(np.abs(un - u)[:, fluid]) <= 1e-9
+    assert np.max(np.abs(p)) <= 1e-9
+    un0, _ = project(False)
+    assert np.max(np.abs(un0 - u)[:, fluid]) > 1e-3
+    P.close()
+
+
+def test_eb_inflow_velocity_projection_matches_oracle(ctx, orc):
+    """Random velocity and a radial body velocity: GPU projection against the
+    oracle's (divergence + body-surface term, CG solve, EB gradient)."""
+    from incflo_amd import mgx
+    _say("case eb inflow projection")
+    g, bxs, P, ct, mom = _setup(ctx, orc)
+    rng = np.random.default_rng(31)
+    u = np.zeros((3,) + g.cell_shape())
+    u[0] = 1.0
+    u += 0.1 * rng.standard_normal(u.shape)
+    u *= (ct > 0)
+    h = 2.0 / 32
+    xc = (np.arange(32) + 0.5) * h
+    yc = (np.arange(16) + 0.5) / 16
+    Y, X = np.meshgrid(yc, xc, indexing="ij")
+    ueb = np.zeros((3,) + g.cell_shape())
+    ueb[0] = 0.3 * (X - CX) / R
+    ueb[1] = 0.3 * (Y - CY) / R
+    s = 0.01
+    inflow = np.zeros((6, 3))
+    inflow[0] = [1.0, 0.0, 0.0]
+    P.setEBInflowVelocity(ctx.array(B.to_boxed(ueb, bxs, 0)), 0)
+    vd = ctx.array(B.to_boxed(u, bxs, 1, periodic=(0, 0, 1)))
+    gpd = ctx.array(B.field_size(bxs, 0, 3))
+    pd = ctx.array(B.field_size(bxs, 0, 1, True))
+    info = P.apply_nodal_projection(vd, 1, gpd, 0, pd, 0, s, const_rho=1.0, inflow_vel=inflow,
+                                    rtol=1e-12, atol=0.0, mode=mgx.MODE_FP64)
+    assert info.converged
+    unew = B.from_boxed(vd.download(), bxs, 1, g.cell_shape(), ncomp=3)
+    p = B.from_boxed(pd.download(), bxs, 0, g.node_shape(), nodal=True)
+    ug = np.zeros(g.ghost_vec_shape())
+    ug[:, 1:-1, 1:-1, 1:-1] = u
+    orc.fill_vel_ghosts(g, ug, inflow)
+    Bw = orc.eb_cylinder_boundary(g, ct, mom, CX, CY, R)
+    b = orc.eb_inflow_rhs(g, ct, Bw, ueb, orc.eb_nodal_divergence(g, ct, mom, ug))
+    sigma = np.full(g.cell_shape(), s)
+    phi, its, ret = orc.eb_nodal_solve(g, ct, mom, sigma, b, rtol=1e-13)
+    assert ret == 0
+    mask = orc.eb_node_mask(g, ct).astype(bool)
+    r = np.where(mask, b - orc.eb_nodal_apply(g, ct, mom, sigma, p), 0.0)
+    assert np.max(np.abs(r)) <= 5e-11 * np.max(np.abs(b))
+    G = orc.eb_nodal_gradient(g, ct, mom, phi)
+    uref = (u - sigma[None] * G) * (ct > 0)
+    cells = mom[..., 0] >= 1e-3
+    assert np.max(np.abs(unew - uref)[:, cells]) <= 1e-8 * np.max(np.abs(uref[:, cells]))
+    P.close()
+
+
+def test_eb_mac_inflow_velocity(ctx, orc):
+    """MAC setEBInflowVelocity: a uniform flow crossing the body is kept; a
+    random field is projected to zero divergence including the body flux."""
+    from incflo_amd import mgx
+    _say("case eb mac inflow")
+    n, ph = (32, 16, 16), (2.0, 1.0, 1.0)
+    per, bclo, bchi = (0, 0, 1), (N, N, 0), (D, N, 0)
+    g = orc.Geom.make(n, periodic=per, bclo=bclo, bchi=bchi, prob_hi=ph)
+    bxs = B.chop_domain(n, (16, 8, 8))
+    lev = mgx.make_level(n, bxs, periodic=per, prob_hi=ph)
+    ap = mgx.eb_cylinder_apertures(lev, CX, CY, R)
+    ct, mom = orc.eb_cylinder(g, CX, CY, R)
+    U = np.array([0.6, 0.0, -0.3])
+    for case in ("uniform", "random"):
+        M = mgx.MacProjector(ctx, lev)
+        M.setDomainBC(bclo, bchi)
+        M.set_eb_cylinder(CX, CY, R)
+        M.updateBeta(0.01)
+        rng = np.random.default_rng(32)
+        if case == "uniform":
+            u = [np.full(orc.face_shape(g, d), U[d]) for d in range(3)]
+            ueb = np.broadcast_to(U[:, None, None, None], (3,) + g.cell_shape()).copy()
+        else:
+            u = [rng.standard_normal(orc.face_shape(g, d)) for d in range(3)]
+            u[2][-1] = u[2][0]
+            ueb = 0.5 * rng.standard_normal((3,) + g.cell_shape())
+        M.setEBInflowVelocity(ctx.array(B.to_boxed(ueb, bxs, 0)), 0)
+        ud = [ctx.array(B.faces_to_boxed(u[d], bxs, 0, d)) for d in range(3)]
+        info = M.project(ud[0], ud[1], ud[2], 0, 1e-12, 1e-12)
+        assert info.converged
+        un = [B.faces_from_boxed(ud[d].download(), bxs, 0, d, orc.face_shape(g, d)) for d in range(3)]
+        div = orc.mac_divergence(g, [ap[d] * un[d] for d in range(3)])
+        body = -sum(np.diff(ap[d], axis=2 - d) * n[d] / ph[d] * ueb[d] for d in range(3))
+        tot = (div + body) * (ct > 0)
+        if case == "uniform":
+            for d in range(3):
+                assert np.max(np.abs(np.where(ap[d] > 0, un[d] - u[d], 0.0))) <= 1e-10
+        scale = np.max(np.abs(orc.mac_divergence(g, [ap[d] * u[d] for d in range(3)]) + body))
+        assert np.max(np.abs(tot)) <= 1e-10 * max(scale, 1.0)
+        M.close()

@@ -187,3 +187,48 @@ def test_cut_cell_moments_match_adaptive_quadrature(orc):
                 pts = sorted(e for e in pts if 0 < e < 1)
                 ref, _ = quad(inner, 0.0, 1.0, points=pts or None, epsabs=1e-15, limit=400)
                 assert abs(mom[0, j, i, p + 3 * q] - ref) < 1e-11, (i, j, p, q)
+
+
+def test_surface_weights_uniform_flow_invariant(orc):
+    """A uniform velocity with the same velocity on the body is divergence
+    free: the fluid-volume divergence plus the body-surface term vanishes at
+    every active node (the cell-face integrals cancel between neighbours);
+    without the surface term it does not.  The surface normals of a cut cell
+    sum (over corners) to the body's projected length, n_z = 0, and for a
+    whole cut ring the x-normal integral cancels."""
+    g, ct, mom = _case()
+    B = orc.eb_cylinder_boundary(g, ct, mom, CX, CY, R)
+    U = np.array([0.7, -0.4, 0.25])
+    ug = np.zeros(g.ghost_vec_shape())
+    for d in range(3):
+        ug[d] = U[d]
+    ueb = np.broadcast_to(U[:, None, None, None], (3,) + g.cell_shape()).copy()
+    div = orc.eb_nodal_divergence(g, ct, mom, ug)
+    assert np.max(np.abs(div)) > 1e-2
+    tot = orc.eb_inflow_rhs(g, ct, B, ueb, div)
+    assert np.max(np.abs(tot)) <= 1e-13 * np.max(np.abs(div))
+    assert np.all(B[..., 2] == 0.0) or np.max(np.abs(B[..., 2])) < 1e-13
+    cut = ct == 2
+    # sum over corners and cut cells of one z-layer: int over the whole circle of n = 0
+    assert abs(B[0][cut[0]][:, :, 0].sum()) < 1e-12 * np.abs(B[0][cut[0]][:, :, 0]).sum()
+    # total |n_x| ds over the circle: 4 r (per unit height); B carries 1/V and the
+    # cell volume is h^2 (h_z) with the z-average 1/2 per corner layer
+    h = 1.0 / 16
+    sx = np.abs(B[0][cut[0]][:, :4, 0].sum(axis=1) + B[0][cut[0]][:, 4:, 0].sum(axis=1)).sum()
+    assert abs(sx * h * h - 4 * R) < 0.02 * 4 * R
+
+
+def test_library_surface_weights_match_oracle(orc):
+    """The library's weights (Gauss-Legendre along the arcs) against the
+    oracle's (divergence theorem from the fluid moments and exact face segments)."""
+    from incflo_amd import boxes as B
+    from incflo_amd import mgx
+    n = (32, 16, 2)
+    cx, cy, r = 0.5, 0.5, 0.2
+    lev = mgx.make_level(n, B.chop_domain(n, 16), periodic=(0, 0, 1), prob_hi=(2.0, 1.0, 0.125))
+    Wl = mgx.eb_cylinder_surface_weights(lev, cx, cy, r)
+    g = orc.Geom.make(n, periodic=(0, 0, 1), bclo=(2, 2, 0), bchi=(1, 2, 0), prob_hi=(2.0, 1.0, 0.125))
+    ct, mom = orc.eb_cylinder(g, cx, cy, r)
+    Wo = orc.eb_cylinder_boundary(g, ct, mom, cx, cy, r)
+    assert (ct == 2).sum() > 0
+    assert np.max(np.abs(Wl - Wo)) <= 1e-11 * np.max(np.abs(Wo))
