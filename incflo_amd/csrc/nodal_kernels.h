@@ -173,12 +173,53 @@ __device__ __forceinline__ TA diag_rcp(TA x) { return (TA)1 / x; }
 template <>
 __device__ __forceinline__ float diag_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// p advanced by a 32-bit byte offset: with p uniform across the wavefront the
+// access compiles to a scalar-base + 32-bit vector-offset load or store, so a
+// streaming kernel needs no per-thread 64-bit address arithmetic per plane
+template <class V>
+__device__ __forceinline__ V* byte_add(V* p, unsigned off) {
+    return reinterpret_cast<V*>(reinterpret_cast<char*>(p) + off);
+}
+template <class V>
+__device__ __forceinline__ const V* byte_add(const V* p, unsigned off) {
+    return reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + off);
+}
+
 template <class T, bool ISO, int RO, int NPR, int NSR>
 __device__ __forceinline__ T apply27(const T (&P0)[NPR][3], const T (&P1)[NPR][3],
                                      const T (&P2)[NPR][3], const T (&S0)[NSR][2],
                                      const T (&S1)[NSR][2], const Coef<T>& cf, T& ssum) {
     // planes: P0 below, P1 centre, P2 above; cell planes S0 below, S1 above
     const T c = P1[RO + 1][1];
+    if constexpr (ISO && std::is_same<T, float>::value) {
+        // FP32: weighted neighbour sums per weight class, minus the centre times
+        // the total weight (every cell enters each class once, so the weights
+        // sum to ssum (corner + edge0 + edge1 + edge2)).  This skips the 20
+        // differences P - c; the rounding it adds is about eps32 (n / 2 pi)^2
+        // relative on the smoothest modes (5e-4 at 256^3), far below what one
+        // V-cycle removes, and the FP64 residual keeps the difference form.
+        T ac = 0, a2 = 0, a1 = 0, a0 = 0;
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa) {
+                const T lo = S0[RO + bb][aa], hi = S1[RO + bb][aa];
+                ac += lo * P0[RO + 2 * bb][2 * aa] + hi * P2[RO + 2 * bb][2 * aa];
+                a2 += (lo + hi) * P1[RO + 2 * bb][2 * aa];
+            }
+#pragma unroll
+        for (int aa = 0; aa < 2; ++aa) {
+            a1 += (S0[RO][aa] + S0[RO + 1][aa]) * P0[RO + 1][2 * aa] +
+                  (S1[RO][aa] + S1[RO + 1][aa]) * P2[RO + 1][2 * aa];
+            a0 += (S0[RO + aa][0] + S0[RO + aa][1]) * P0[RO + 2 * aa][1] +
+                  (S1[RO + aa][0] + S1[RO + aa][1]) * P2[RO + 2 * aa][1];
+        }
+        const T s0 = S0[RO][0] + S0[RO][1] + S0[RO + 1][0] + S0[RO + 1][1];
+        const T s1 = S1[RO][0] + S1[RO][1] + S1[RO + 1][0] + S1[RO + 1][1];
+        ssum = s0 + s1;
+        const T wsum = cf.corner + cf.edge[0] + cf.edge[1] + cf.edge[2];
+        return cf.corner * ac + cf.edge[2] * a2 + cf.edge[1] * a1 + cf.edge[0] * a0 - c * (ssum * wsum);
+    }
     T acc = 0;
 #pragma unroll
     for (int bb = 0; bb < 2; ++bb)
@@ -252,40 +293,71 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
     const int sxr = min(j0 + min(tid, SR - 1), g.sg[1] - 1);
     const int sxc = min(i0 + TX, g.sg[0] - 1);
     const bool sxst = tid < SR;
-    long bid[NPT];
+    // in-plane byte offsets of every load and store of this thread; the plane
+    // bases (k * plane size) are uniform and added as pointers
+    unsigned opx[NQP], opc[NQP], ohx, ohc, osg[NQS], osx, obb[NPT], oox[NPT], ooc[NPT], oel[NPT];
 #pragma unroll
-    for (int n = 0; n < NPT; ++n)
-        bid[n] = (long)min(i, g.nn[0] - 1) + (long)g.nn[0] * min(jr + n, g.nn[1] - 1);
+    for (int q = 0; q < NQP; ++q) {
+        const unsigned e = (unsigned)(gi + g.nn[0] * prow[q]);
+        opx[q] = e * (unsigned)sizeof(T);
+        opc[q] = e * (unsigned)sizeof(TC);
+    }
+    {
+        const unsigned e = (unsigned)(hcol + g.nn[0] * hrow);
+        ohx = e * (unsigned)sizeof(T);
+        ohc = e * (unsigned)sizeof(TC);
+    }
+#pragma unroll
+    for (int q = 0; q < NQS; ++q) osg[q] = (unsigned)(si + g.sg[0] * srow[q]) * (unsigned)sizeof(T);
+    osx = (unsigned)(sxc + g.sg[0] * sxr) * (unsigned)sizeof(T);
+#pragma unroll
+    for (int n = 0; n < NPT; ++n) {
+        obb[n] = (unsigned)(min(i, g.nn[0] - 1) + g.nn[0] * min(jr + n, g.nn[1] - 1)) * (unsigned)sizeof(T);
+        const unsigned e = (unsigned)(i + g.nn[0] * (jr + n));  // used only for nodes inside
+        oel[n] = e;
+        oox[n] = e * (unsigned)sizeof(T);
+        ooc[n] = e * (unsigned)sizeof(TC);
+    }
 
     const double cs = (MODE == KS_UPDRES && a.cor_scale != 0.0) ? a.cor_scale : 1.0;
     const double rsc = (a.rscale != 0.0) ? a.rscale : 1.0;
     const double x1s = (a.x1scale != 0.0) ? a.x1scale : 1.0;
-    TA lp[NQP], lh, ls[NQS], lsx, lb_[NPT];
+    // registers of one plane's loads: phi rows + halo, sigma rows + column, rhs
+    struct LoadSet {
+        TA lp[NQP], lh, ls[NQS], lsx, lb[NPT];
+    };
+    LoadSet L0, L1;
 #pragma unroll
-    for (int n = 0; n < NPT; ++n) lb_[n] = 0;
-    auto issue = [&](int kk, int kb) {
+    for (int n = 0; n < NPT; ++n) L0.lb[n] = L1.lb[n] = 0;
+    auto issue = [&](LoadSet& ld_, int kk, int kb) {
+        TA* lp = ld_.lp;
+        TA& lh = ld_.lh;
+        TA* ls = ld_.ls;
+        TA& lsx = ld_.lsx;
+        TA* lb_ = ld_.lb;
         const long kw = (long)wrapc(kk, g.nn[2], g.per[2]) * sxy;
         const long ks = (long)min(max(kk, 0), g.sg[2] - 1) * ssxy;
         if (MODE != KS_JFIRST) {
+            const T* xk = a.x + kw;
+            const TC* ck = (MODE == KS_UPDRES) ? a.cor + kw : nullptr;
 #pragma unroll
             for (int q = 0; q < NQP; ++q) {
-                const long id = gi + (long)g.nn[0] * prow[q] + kw;
-                TA v = ld<TA>(a.x[id]);
-                if (MODE == KS_UPDRES) v += (TA)(cs * (double)ld<acc_t<TC>>(a.cor[id]));
+                TA v = ld<TA>(*byte_add(xk, opx[q]));
+                if (MODE == KS_UPDRES) v += (TA)(cs * (double)ld<acc_t<TC>>(*byte_add(ck, opc[q])));
                 lp[q] = v;
             }
-            const long id = hcol + (long)g.nn[0] * hrow + kw;
-            TA v = ld<TA>(a.x[id]);
-            if (MODE == KS_UPDRES) v += (TA)(cs * (double)ld<acc_t<TC>>(a.cor[id]));
+            TA v = ld<TA>(*byte_add(xk, ohx));
+            if (MODE == KS_UPDRES) v += (TA)(cs * (double)ld<acc_t<TC>>(*byte_add(ck, ohc)));
             lh = v;
         }
+        const T* sk = a.sig + ks;
 #pragma unroll
-        for (int q = 0; q < NQS; ++q) ls[q] = ld<TA>(a.sig[si + (long)g.sg[0] * srow[q] + ks]);
-        lsx = ld<TA>(a.sig[sxc + (long)g.sg[0] * sxr + ks]);
+        for (int q = 0; q < NQS; ++q) ls[q] = ld<TA>(*byte_add(sk, osg[q]));
+        lsx = ld<TA>(*byte_add(sk, osx));
         if (MODE != KS_APPLY) {
-            const long kbo = (long)min(max(kb, 0), g.nn[2] - 1) * sxy;
+            const T* bk = a.b + (long)min(max(kb, 0), g.nn[2] - 1) * sxy;
 #pragma unroll
-            for (int n = 0; n < NPT; ++n) lb_[n] = ld<TA>(a.b[bid[n] + kbo]);
+            for (int n = 0; n < NPT; ++n) lb_[n] = ld<TA>(*byte_add(bk, obb[n]));
         }
     };
 
@@ -307,7 +379,9 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
     // One step per z-plane; the register planes rotate (P: 3, S: 2), so the
     // loop is unrolled by 6 and every step addresses its planes by constant
     // indices instead of copying them down.
-    auto step = [&](auto JJ, int kk) {
+    // cur: the loads of plane kk (issued `ahead` planes earlier); the same
+    // registers then receive plane kk + ahead
+    auto step = [&](auto JJ, int kk, LoadSet& cur, int ahead) {
         constexpr int J = decltype(JJ)::value;
         constexpr int PN = J % 3, PM = (J + 2) % 3, PB = (J + 1) % 3;  // above, centre, below
         constexpr int SN = J % 2, SB = (J + 1) % 2;                     // cell planes above, below
@@ -315,17 +389,17 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
         if (MODE != KS_JFIRST) {
 #pragma unroll
             for (int q = 0; q < NQP; ++q)
-                if (ty + TYT * q < PR) sphi[buf][ty + TYT * q][tx + 1] = lp[q];
-            if (hst) sphi[buf][hr][(tid & 1) ? PC - 1 : 0] = lh;
+                if (ty + TYT * q < PR) sphi[buf][ty + TYT * q][tx + 1] = cur.lp[q];
+            if (hst) sphi[buf][hr][(tid & 1) ? PC - 1 : 0] = cur.lh;
         }
 #pragma unroll
         for (int q = 0; q < NQS; ++q)
-            if (ty + TYT * q < SR) ssig[buf][ty + TYT * q][tx] = ls[q];
-        if (sxst) ssig[buf][min(tid, SR - 1)][SC - 1] = lsx;
+            if (ty + TYT * q < SR) ssig[buf][ty + TYT * q][tx] = cur.ls[q];
+        if (sxst) ssig[buf][min(tid, SR - 1)][SC - 1] = cur.lsx;
         TA bcur[NPT];  // rhs of plane kk-1
 #pragma unroll
-        for (int n = 0; n < NPT; ++n) bcur[n] = lb_[n];
-        if (kk < k1) issue(kk + 1, kk);
+        for (int n = 0; n < NPT; ++n) bcur[n] = cur.lb[n];
+        if (kk + ahead <= k1) issue(cur, kk + ahead, kk + ahead - 1);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < NPT + 2; ++r)
@@ -347,10 +421,10 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
 #pragma unroll
                     for (int r = 0; r < 2; ++r)
                         ss += S[SB][n + r][0] + S[SB][n + r][1] + S[SN][n + r][0] + S[SN][n + r][1];
-                    const long id = i + (long)g.nn[0] * j + sxy * k;
-                    const bool fixed = MASK && a.nmask[id] != 1;
-                    a.xout[id] = st<T>((is_dir(g, i, j, k) || ss == (TA)0 || fixed)
-                                           ? (TA)0 : a.omega * bcur[n] * diag_rcp(-a.c.diag4 * ss));
+                    const bool fixed = MASK && a.nmask[sxy * k + oel[n]] != 1;
+                    *byte_add(a.xout + sxy * k, oox[n]) =
+                        st<T>((is_dir(g, i, j, k) || ss == (TA)0 || fixed)
+                                  ? (TA)0 : a.omega * bcur[n] * diag_rcp(-a.c.diag4 * ss));
                     continue;
                 }
                 TA ssum;
@@ -361,8 +435,8 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
                     : (n == 2) ? apply27<TA, ISO, (NPT > 2 ? 2 : 0)>(P[PB], P[PM], P[PN], S[SB], S[SN], a.c, ssum)
                                : apply27<TA, ISO, (NPT > 3 ? 3 : 0)>(P[PB], P[PM], P[PN], S[SB], S[SN], a.c, ssum);
                 const TA c = P[PM][n + 1][1];
-                const long id = i + (long)g.nn[0] * j + sxy * k;
-                const int nt = MASK ? a.nmask[id] : 1;
+                const long kp = sxy * k;  // uniform plane base
+                const int nt = MASK ? a.nmask[kp + oel[n]] : 1;
                 const bool dir = is_dir(g, i, j, k);
                 const TA bv = bcur[n];
                 if (MODE == KS_JACOBI) {
@@ -370,27 +444,32 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
                     if (dir || nt == 0) v = 0;
                     else if (nt == 1 && ssum != (TA)0)
                         v = c + (bv - acc) * (a.omega * diag_rcp(-a.c.diag4 * ssum));
-                    a.xout[id] = st<T>(v);
+                    *byte_add(a.xout + kp, oox[n]) = st<T>(v);
                 } else if (MODE == KS_APPLY) {
-                    a.rout[id] = st<TC>(acc);
+                    *byte_add(a.rout + kp, ooc[n]) = st<TC>(acc);
                 } else {
                     const TA r = (dir || nt != 1) ? (TA)0 : (bv - acc);
-                    a.rout[id] = st<TC>((acc_t<TC>)(rsc * (double)r));
-                    if (MODE == KS_UPDRES) a.xout[id] = st<T>(dir ? (TA)0 : c);
+                    *byte_add(a.rout + kp, ooc[n]) = st<TC>((acc_t<TC>)(rsc * (double)r));
+                    if (MODE == KS_UPDRES) *byte_add(a.xout + kp, oox[n]) = st<T>(dir ? (TA)0 : c);
                     if (a.x1out)
-                        a.x1out[id] = st<TC>((acc_t<TC>)((dir || ssum == (TA)0)
-                                              ? 0.0
-                                              : a.omega1 * x1s * (double)r / (-(double)a.c.diag4 * (double)ssum)));
+                        *byte_add(a.x1out + kp, ooc[n]) =
+                            st<TC>((acc_t<TC>)((dir || ssum == (TA)0)
+                                                   ? 0.0
+                                                   : a.omega1 * x1s * (double)r /
+                                                         (-(double)a.c.diag4 * (double)ssum)));
                     rmax = fmax(rmax, (double)fabs(r));
                 }
             }
         }
     };
-    issue(k0 - 1, k0 - 1);
     if (sizeof(TA) == 4) {
-        // FP32 arithmetic: a plain loop with the planes copied down measures
-        // faster (the 6-fold unrolled body costs more than the copies it saves)
-        for (int kk = k0 - 1; kk <= k1; ++kk) {
+        // FP32 arithmetic: a plain loop with the planes copied down (cheaper than
+        // the 6-fold unrolled rotation); the Jacobi sweeps issue their loads two
+        // planes ahead through two register sets (the sweep is bound by load
+        // latency, not by issue: 18% fewer VALU instructions changed nothing,
+        // the second set cut the sweep by 9%, and it fits the 128-VGPR budget
+        // of 4 waves per SIMD)
+        auto shift = [&]() {
 #pragma unroll
             for (int r = 0; r < NPT + 2; ++r)
 #pragma unroll
@@ -402,21 +481,39 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
             for (int r = 0; r < NPT + 1; ++r)
 #pragma unroll
                 for (int c = 0; c < 2; ++c) S[1][r][c] = S[0][r][c];
-            step(std::integral_constant<int, 2>{}, kk);
+        };
+        // (the FP32 residual passes keep one plane: their extra registers would
+        // cost a wave per SIMD, which measured slower)
+        constexpr bool two = (MODE == KS_JACOBI || MODE == KS_JFIRST);
+        issue(L0, k0 - 1, k0 - 2);
+        if (two && k0 <= k1) issue(L1, k0, k0 - 1);
+        for (int kk = k0 - 1; kk <= k1; kk += 2) {
+            shift();
+            step(std::integral_constant<int, 2>{}, kk, L0, two ? 2 : 1);
+            if (kk + 1 > k1) break;
+            shift();
+            step(std::integral_constant<int, 2>{}, kk + 1, two ? L1 : L0, two ? 2 : 1);
         }
-    } else
-    for (int kk = k0 - 1; kk <= k1; kk += 6) {
-        step(std::integral_constant<int, 0>{}, kk);
-        if (kk + 1 > k1) break;
-        step(std::integral_constant<int, 1>{}, kk + 1);
-        if (kk + 2 > k1) break;
-        step(std::integral_constant<int, 2>{}, kk + 2);
-        if (kk + 3 > k1) break;
-        step(std::integral_constant<int, 3>{}, kk + 3);
-        if (kk + 4 > k1) break;
-        step(std::integral_constant<int, 4>{}, kk + 4);
-        if (kk + 5 > k1) break;
-        step(std::integral_constant<int, 5>{}, kk + 5);
+    } else {
+        // FP64: loads two planes ahead as well (the second register set keeps
+        // the kernels within 256 VGPRs, i.e. at the same 2 waves per SIMD)
+        constexpr int A = 2;
+        issue(L0, k0 - 1, k0 - 2);
+        if (A == 2 && k0 <= k1) issue(L1, k0, k0 - 1);
+        LoadSet& Lb = (A == 2) ? L1 : L0;
+        for (int kk = k0 - 1; kk <= k1; kk += 6) {
+            step(std::integral_constant<int, 0>{}, kk, L0, A);
+            if (kk + 1 > k1) break;
+            step(std::integral_constant<int, 1>{}, kk + 1, Lb, A);
+            if (kk + 2 > k1) break;
+            step(std::integral_constant<int, 2>{}, kk + 2, L0, A);
+            if (kk + 3 > k1) break;
+            step(std::integral_constant<int, 3>{}, kk + 3, Lb, A);
+            if (kk + 4 > k1) break;
+            step(std::integral_constant<int, 4>{}, kk + 4, L0, A);
+            if (kk + 5 > k1) break;
+            step(std::integral_constant<int, 5>{}, kk + 5, Lb, A);
+        }
     }
     if ((MODE == KS_RESID || MODE == KS_UPDRES) && a.norm) {
         const double m = block_max<NT>(rmax, red);
