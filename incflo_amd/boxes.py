@@ -150,3 +150,30 @@ def faces_from_boxed(boxed, boxes, ng, d, shape):
             blk[ng:D[2] - ng, ng:D[1] - ng, ng:D[0] - ng]
         off += vol
     return out
+
+
+def faces_to_boxed_n(canon, boxes, ng, d):
+    """ncomp canonical face arrays (ncomp, ...) -> boxed, component-major per box."""
+    nc = canon.shape[0]
+    parts = []
+    for lo, hi in boxes:
+        per = [faces_to_boxed(canon[q], [(lo, hi)], ng, d) for q in range(nc)]
+        parts.append(np.concatenate(per))
+    return np.concatenate(parts)
+
+
+def faces_from_boxed_n(boxed, boxes, ng, d, shape, nc):
+    """inverse of faces_to_boxed_n: -> (nc,) + shape"""
+    out = np.zeros((nc,) + tuple(shape))
+    off = 0
+    for lo, hi in boxes:
+        vol = int(np.prod(face_box_dims(lo, hi, ng, d)))
+        for q in range(nc):
+            out[q] += 0.0
+            blk = faces_from_boxed(boxed[off + q * vol: off + (q + 1) * vol], [(lo, hi)], ng, d,
+                                   shape)
+            e = [1 if r == d else 0 for r in range(3)]
+            out[q][lo[2]:hi[2] + 1 + e[2], lo[1]:hi[1] + 1 + e[1], lo[0]:hi[0] + 1 + e[0]] = \
+                blk[lo[2]:hi[2] + 1 + e[2], lo[1]:hi[1] + 1 + e[1], lo[0]:hi[0] + 1 + e[0]]
+        off += nc * vol
+    return out

@@ -622,4 +622,74 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
     return status;
 }
 
+// ---- AMR synchronisation ---------------------------------------------------
+
+namespace {
+
+// both levels' layouts (the fine boxes need not tile the domain), checked for
+// ratio 2 and coarse-aligned fine boxes
+void sync_layouts(const mgx_level_desc* c, const mgx_level_desc* f, BoxLayout& lc, BoxLayout& lf,
+                  KGeom& kf) {
+    MGX_REQUIRE(c && f && c->boxes && f->boxes && c->nboxes > 0 && f->nboxes > 0, MGX_ERR_ARG,
+                "average down: bad level descriptions");
+    double dxc[3], dxf[3];
+    int per[3], zero[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+        MGX_REQUIRE(f->n_cell[d] == 2 * c->n_cell[d], MGX_ERR_UNSUPPORTED,
+                    "average down: refinement ratio 2 only");
+        per[d] = c->is_periodic[d] ? 1 : 0;
+        dxc[d] = (c->prob_hi[d] - c->prob_lo[d]) / c->n_cell[d];
+        dxf[d] = (f->prob_hi[d] - f->prob_lo[d]) / f->n_cell[d];
+    }
+    for (int b = 0; b < f->nboxes; ++b)
+        for (int d = 0; d < 3; ++d)
+            MGX_REQUIRE(f->boxes[6 * b + d] % 2 == 0 && f->boxes[6 * b + 3 + d] % 2 == 1,
+                        MGX_ERR_ARG, "average down: fine boxes must align with coarse cells");
+    const NodalGeom gc = make_nodal_geom(c->n_cell, per, zero, zero, dxc);
+    const NodalGeom gf = make_nodal_geom(f->n_cell, per, zero, zero, dxf);
+    lc.build(c->boxes, c->nboxes, gc);
+    lf.build(f->boxes, f->nboxes, gf, true);
+    kf = kgeom(gf);
+}
+
+}  // namespace
+
+MGX_API int mgx_amr_average_down_faces(mgx_ctx ctx, const mgx_level_desc* crse,
+                                       const mgx_level_desc* fine, const double* const* fine_faces,
+                                       int fine_ng, double* const* crse_faces, int crse_ng,
+                                       int ncomp) {
+    if (!ctx) return MGX_ERR_ARG;
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(fine_faces && crse_faces && ncomp >= 1 && fine_ng >= 0 && crse_ng >= 0,
+                    MGX_ERR_ARG, "average_down_faces: bad fields");
+        BoxLayout lc, lf;
+        KGeom kf;
+        sync_layouts(crse, fine, lc, lf, kf);
+        for (int d = 0; d < 3; ++d) {
+            MGX_REQUIRE(fine_faces[d] && crse_faces[d], MGX_ERR_ARG, "average_down_faces: null face field");
+            average_down_faces_kernel<<<lf.ntiles[REG_FACE0 + d], 256, 0, ctx->stream>>>(
+                kf, lf.dev_boxes(), lf.tiles(REG_FACE0 + d), d, fine_faces[d], fine_ng,
+                lc.dev_boxes(), lc.nboxes, crse_faces[d], crse_ng, ncomp);
+        }
+        MGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+MGX_API int mgx_amr_average_down(mgx_ctx ctx, const mgx_level_desc* crse,
+                                 const mgx_level_desc* fine, const double* fine_cells, int fine_ng,
+                                 double* crse_cells, int crse_ng, int ncomp) {
+    if (!ctx) return MGX_ERR_ARG;
+    return guarded(ctx, [&] {
+        MGX_REQUIRE(fine_cells && crse_cells && ncomp >= 1 && fine_ng >= 0 && crse_ng >= 0,
+                    MGX_ERR_ARG, "average_down: bad fields");
+        BoxLayout lc, lf;
+        KGeom kf;
+        sync_layouts(crse, fine, lc, lf, kf);
+        average_down_cells_kernel<<<lf.ntiles[REG_CELL], 256, 0, ctx->stream>>>(
+            kf, lf.dev_boxes(), lf.tiles(REG_CELL), fine_cells, fine_ng, lc.dev_boxes(), lc.nboxes,
+            crse_cells, crse_ng, ncomp);
+        MGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 }  // extern "C"

@@ -380,3 +380,76 @@ static __global__ void __launch_bounds__(256) cell_copy_kernel(KGeom g, const Bo
 }
 
 }  // namespace mgx
+
+namespace mgx {
+
+// ---- AMR synchronisation (refinement ratio 2): average_down_faces and
+// average_down of cell fields.  One thread per fine face (cell) whose indices
+// are all even: it averages the 4 fine faces (8 fine cells) over one coarse
+// face (cell) and writes every coarse box that holds that face (cell).
+
+__device__ __forceinline__ bool box_holds(const BoxInfo& b, int i, int j, int k, int d) {
+    // d < 0: cell (i,j,k); else face (i,j,k) normal to d (lo..hi+1 along d)
+    const int p[3] = {i, j, k};
+    for (int e = 0; e < 3; ++e) {
+        const int hi = b.lo[e] + b.len[e] - 1 + (e == d ? 1 : 0);
+        if (p[e] < b.lo[e] || p[e] > hi) return false;
+    }
+    return true;
+}
+
+static __global__ void __launch_bounds__(256) average_down_faces_kernel(
+    KGeom gf, const BoxInfo* fboxes, const Tile* ftiles, int d, const double* __restrict__ fine,
+    int fng, const BoxInfo* cboxes, int ncboxes, double* __restrict__ crse, int cng, int ncomp) {
+    int bi, i, j, k;
+    if (!tile_point(fboxes, ftiles, REG_FACE0 + d, bi, i, j, k)) return;
+    if ((i | j | k) & 1) return;
+    const BoxInfo& b = fboxes[bi];
+    const long fvol = (long)(b.len[0] + (d == 0) + 2 * fng) * (b.len[1] + (d == 1) + 2 * fng) *
+                      (b.len[2] + (d == 2) + 2 * fng);
+    const long fo = (long)ncomp * face_offset(b, fng, d);
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    for (int cb = 0; cb < ncboxes; ++cb) {
+        const BoxInfo& c = cboxes[cb];
+        if (!box_holds(c, I, J, K, d)) continue;
+        const long cvol = (long)(c.len[0] + (d == 0) + 2 * cng) * (c.len[1] + (d == 1) + 2 * cng) *
+                          (c.len[2] + (d == 2) + 2 * cng);
+        const long co = (long)ncomp * face_offset(c, cng, d) + box_index_face(c, cng, d, I, J, K);
+        for (int q = 0; q < ncomp; ++q) {
+            double s = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                int p[3] = {i, j, k};
+                p[e1] += t & 1;
+                p[e2] += t >> 1;
+                s += fine[fo + q * fvol + box_index_face(b, fng, d, p[0], p[1], p[2])];
+            }
+            crse[co + q * cvol] = 0.25 * s;
+        }
+    }
+}
+
+static __global__ void __launch_bounds__(256) average_down_cells_kernel(
+    KGeom gf, const BoxInfo* fboxes, const Tile* ftiles, const double* __restrict__ fine, int fng,
+    const BoxInfo* cboxes, int ncboxes, double* __restrict__ crse, int cng, int ncomp) {
+    int bi, i, j, k;
+    if (!tile_point(fboxes, ftiles, REG_CELL, bi, i, j, k)) return;
+    if ((i | j | k) & 1) return;
+    const BoxInfo& b = fboxes[bi];
+    const long fo = field_offset(b, fng, ncomp, false);
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    for (int cb = 0; cb < ncboxes; ++cb) {
+        const BoxInfo& c = cboxes[cb];
+        if (!box_holds(c, I, J, K, -1)) continue;
+        const long co = field_offset(c, cng, ncomp, false);
+        for (int q = 0; q < ncomp; ++q) {
+            double s = 0.0;
+            for (int t = 0; t < 8; ++t)
+                s += fine[fo + box_index(b, fng, false, i + (t & 1), j + ((t >> 1) & 1), k + (t >> 2), q)];
+            crse[co + box_index(c, cng, false, I, J, K, q)] = 0.125 * s;
+        }
+        break;  // a cell belongs to one box
+    }
+}
+
+}  // namespace mgx

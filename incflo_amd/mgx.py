@@ -132,6 +132,10 @@ def lib():
                                          vp, vp],
             "mgx_mac_set_eb_cylinder": [vp, C.c_double, C.c_double, C.c_double],
             "mgx_mac_set_level_bc": [vp, vp, vp, vp, C.c_int],
+            "mgx_amr_average_down_faces": [vp, C.POINTER(LevelDesc), C.POINTER(LevelDesc), vp,
+                                           C.c_int, vp, C.c_int, C.c_int],
+            "mgx_amr_average_down": [vp, C.POINTER(LevelDesc), C.POINTER(LevelDesc), vp, C.c_int,
+                                     vp, C.c_int, C.c_int],
             "mgx_mac_set_eb_inflow_velocity": [vp, vp, C.c_int],
             "mgx_nodal_set_eb_inflow_velocity": [vp, vp, C.c_int],
             "mgx_eb_cylinder_surface_weights": [C.POINTER(LevelDesc), C.c_double, C.c_double,
@@ -585,6 +589,20 @@ def eb_cylinder_apertures(level, cx, cy, r):
 def _ptrs(arrs):
     """array of device pointers (None -> NULL) for the per-level arguments"""
     return (C.c_void_p * len(arrs))(*[a.ptr if a is not None else None for a in arrs])
+
+
+def amr_average_down_faces(ctx, crse, fine, fine_faces, fine_ng, crse_faces, crse_ng, ncomp=1):
+    """average_down_faces over two levels (ratio 2): device face arrays per direction."""
+    ff = (C.c_void_p * 3)(*[a.ptr for a in fine_faces])
+    cf = (C.c_void_p * 3)(*[a.ptr for a in crse_faces])
+    ctx.check(lib().mgx_amr_average_down_faces(ctx.h, C.byref(crse), C.byref(fine), ff, fine_ng,
+                                               cf, crse_ng, ncomp))
+
+
+def amr_average_down(ctx, crse, fine, fine_cells, fine_ng, crse_cells, crse_ng, ncomp=1):
+    """average_down of a cell field over two levels (ratio 2)."""
+    ctx.check(lib().mgx_amr_average_down(ctx.h, C.byref(crse), C.byref(fine), fine_cells.ptr,
+                                         fine_ng, crse_cells.ptr, crse_ng, ncomp))
 
 
 class AmrNodalProjector:

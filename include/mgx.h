@@ -261,6 +261,21 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
                                   int gphi_ng, double rtol, double atol, int mode,
                                   mgx_solve_info* info);
 
+/* AMR synchronisation, refinement ratio 2 (fine boxes aligned with coarse cells):
+ * average_down_faces (src/convection/incflo_compute_advection_term.cpp:559-569):
+ * every coarse face under the fine level (its closure included) becomes the
+ * mean of the 4 fine faces on it, in every coarse box holding it; boxed face
+ * fields with ncomp components (fine_faces[d] / crse_faces[d]). */
+MGX_API int mgx_amr_average_down_faces(mgx_ctx ctx, const mgx_level_desc* crse,
+                                       const mgx_level_desc* fine, const double* const* fine_faces,
+                                       int fine_ng, double* const* crse_faces, int crse_ng,
+                                       int ncomp);
+/* average_down of cell fields (src/incflo_update_density.cpp:47-51,
+ * src/incflo_update_tracer.cpp:61-65): coarse cell = mean of its 8 fine cells. */
+MGX_API int mgx_amr_average_down(mgx_ctx ctx, const mgx_level_desc* crse,
+                                 const mgx_level_desc* fine, const double* fine_cells, int fine_ng,
+                                 double* crse_cells, int crse_ng, int ncomp);
+
 /* ---- MAC projection (Hydro::MacProjector + MLABecLaplacian/MLPoisson + MLMG) */
 typedef struct mgx_mac_s* mgx_mac;
 /* MacProjector(Vector<Geometry>) (src/incflo.cpp:291); the level's box list

@@ -264,3 +264,51 @@ class AmrGeom:
         ucn = uc - sig_c[None] * gc
         ucn = np.where(self.cov[None], self.average_down(ufn), ucn)
         return ucn, ufn, phc, phf, gc, gf, its
+
+
+# ---- synchronisation (ratio 2) ------------------------------------------------
+def average_down_faces(fine_cells_mask, ff, fc):
+    """average_down_faces (src/convection/incflo_compute_advection_term.cpp:559-569)
+    on canonical face arrays: fc[d] (coarse, n_c + 1 along d) gets, on every
+    coarse face whose 4 fine faces belong to the fine level, their mean.
+    fine_cells_mask: (nz, ny, nx) bool of the fine level's cells; ff[d]: fine
+    faces (n_f + 1 along d).  Returns new coarse arrays."""
+    out = [np.array(a, dtype=np.float64) for a in fc]
+    F = np.asarray(fine_cells_mask, bool)
+    for d in range(3):
+        ax = 2 - d
+        # a fine face belongs to the fine level when a fine cell lies on either side
+        pad = [(0, 0)] * 3
+        pad[ax] = (1, 1)
+        Fp = np.pad(F, pad)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        has = Fp[tuple(lo)] | Fp[tuple(hi)]
+        # the 4 fine faces of each coarse face: even index along d, pairs across
+        sl = [slice(None)] * 3
+        sl[ax] = slice(0, None, 2)
+        f_ev = ff[d][tuple(sl)]
+        h_ev = has[tuple(sl)]
+        tang = [a for a in range(3) if a != ax]
+        s = 0.0
+        ok = True
+        for t0 in (0, 1):
+            for t1 in (0, 1):
+                idx = [slice(None)] * 3
+                idx[tang[0]] = slice(t0, None, 2)
+                idx[tang[1]] = slice(t1, None, 2)
+                s = s + f_ev[tuple(idx)]
+                ok = ok & h_ev[tuple(idx)]
+        out[d] = np.where(ok, 0.25 * s, out[d])
+    return out
+
+
+def average_down_cells(fine_cells_mask, uf, uc):
+    """average_down of a cell field: coarse cell = mean of its 8 fine cells
+    where those are fine cells; uf (nz_f, ny_f, nx_f), uc (nz_c, ny_c, nx_c)."""
+    F = np.asarray(fine_cells_mask, bool)
+    s = sum(uf[a::2, b::2, c::2] for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    cov = F[0::2, 0::2, 0::2]
+    return np.where(cov, 0.125 * s, uc)

@@ -167,3 +167,45 @@ def test_amr_rejects_bad_levels(ctx):
     with pytest.raises(mgx.MgxError) as e:
         mgx.AmrNodalProjector(ctx, [lev_w, wall])
     assert e.value.code == mgx.MGX_ERR_UNSUPPORTED
+
+
+def test_average_down_faces_and_cells_match_oracle(ctx):
+    """average_down_faces / average_down through the C ABI against the oracle,
+    on a fine level of two boxes (one on the periodic seam side)."""
+    from incflo_amd import mgx
+    from oracle import amr as A
+    print("case average down", flush=True)
+    nc, nf = (16, 16, 8), (32, 32, 16)
+    cb = B.chop_domain(nc, 8)
+    fb = [((4, 8, 0), (11, 19, 7)), ((20, 2, 4), (31, 9, 15))]
+    lc = mgx.make_level(nc, cb)
+    lf = mgx.make_level(nf, fb)
+    F = np.zeros((nf[2], nf[1], nf[0]), bool)
+    for lo, hi in fb:
+        F[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+    rng = np.random.default_rng(41)
+    fshape = lambda n, d: tuple(n[2 - a] + (1 if a == 2 - d else 0) for a in range(3))
+    ff = [rng.standard_normal((2,) + fshape(nf, d)) for d in range(3)]
+    fc = [rng.standard_normal((2,) + fshape(nc, d)) for d in range(3)]
+    for d in range(3):  # periodic: the duplicate coarse face repeats face 0
+        ax = 3 - d
+        sl0 = [slice(None)] * 4; sl0[ax] = 0
+        sl1 = [slice(None)] * 4; sl1[ax] = -1
+        fc[d][tuple(sl1)] = fc[d][tuple(sl0)]
+    fd = [ctx.array(B.faces_to_boxed_n(ff[d], fb, 0, d)) for d in range(3)]
+    cd = [ctx.array(B.faces_to_boxed_n(fc[d], cb, 0, d)) for d in range(3)]
+    mgx.amr_average_down_faces(ctx, lc, lf, fd, 0, cd, 0, ncomp=2)
+    for d in range(3):
+        got = B.faces_from_boxed_n(cd[d].download(), cb, 0, d, fshape(nc, d), 2)
+        for q in range(2):
+            ref = A.average_down_faces(F, [ff[e][q] for e in range(3)], [fc[e][q] for e in range(3)])[d]
+            assert np.max(np.abs(got[q] - ref)) <= 1e-15 * np.max(np.abs(ref)), (d, q)
+    uf = rng.standard_normal((3, nf[2], nf[1], nf[0]))
+    uc = rng.standard_normal((3, nc[2], nc[1], nc[0]))
+    ufd = ctx.array(B.to_boxed(uf, fb, 1))
+    ucd = ctx.array(B.to_boxed(uc, cb, 1))
+    mgx.amr_average_down(ctx, lc, lf, ufd, 1, ucd, 1, ncomp=3)
+    got = B.from_boxed(ucd.download(), cb, 1, (nc[2], nc[1], nc[0]), ncomp=3)
+    for q in range(3):
+        ref = A.average_down_cells(F, uf[q], uc[q])
+        assert np.max(np.abs(got[q] - ref)) <= 1e-15 * np.max(np.abs(ref))

@@ -189,3 +189,42 @@ def test_projection_solves_composite_system(orc):
     # coarse phi under F is the coincident fine value
     assert np.array_equal(phc[~A.cdof], phf[::2, ::2, ::2][~A.cdof])
     assert abs(phc.mean()) < 1e-14 * np.abs(phc).max()
+
+
+def test_average_down_faces_injection_and_divergence():
+    """average_down_faces returns an injected coarse field unchanged and makes
+    the coarse divergence of every covered coarse cell the mean of its 8 fine
+    cells' divergences (the fine faces inside a coarse cell cancel)."""
+    from oracle import amr as A
+    nc = (8, 8, 8)
+    rng = np.random.default_rng(40)
+    F = np.zeros((16, 16, 16), bool)
+    F[4:12, 2:10, 6:14] = True  # fine cells (aligned with coarse cells)
+    fshape = lambda n, d: tuple(n[2 - a] + (1 if a == 2 - d else 0) for a in range(3))
+    nf = (16, 16, 16)
+    fc = [rng.standard_normal(fshape(nc, d)) for d in range(3)]
+    # injection: fine faces on coarse face planes take the coarse value
+    ff = []
+    for d in range(3):
+        a = rng.standard_normal(fshape(nf, d))
+        ax = 2 - d
+        rep = np.repeat(np.repeat(fc[d], 2, axis=[x for x in range(3) if x != ax][0]),
+                        2, axis=[x for x in range(3) if x != ax][1])
+        sl = [slice(None)] * 3
+        sl[ax] = slice(0, None, 2)
+        a[tuple(sl)] = rep
+        ff.append(a)
+    out = A.average_down_faces(F, ff, fc)
+    for d in range(3):
+        assert np.array_equal(out[d], fc[d])
+    # divergence: random fine faces
+    ff = [rng.standard_normal(fshape(nf, d)) for d in range(3)]
+    out = A.average_down_faces(F, ff, fc)
+    divf = sum(np.diff(ff[d], axis=2 - d) * nf[d] for d in range(3))
+    divc = sum(np.diff(out[d], axis=2 - d) * nc[d] for d in range(3))
+    mean8 = sum(divf[a::2, b::2, c::2] for a in (0, 1) for b in (0, 1) for c in (0, 1)) / 8
+    cov = F[0::2, 0::2, 0::2]
+    assert cov.sum() == 4 * 4 * 4
+    assert np.max(np.abs(divc - mean8)[cov]) <= 1e-12 * np.max(np.abs(mean8))
+    # faces away from the fine region keep their values
+    assert np.array_equal(out[0][0], fc[0][0])
