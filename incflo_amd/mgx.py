@@ -152,6 +152,13 @@ def lib():
             "mgx_amr_nodal_project": [vp, C.POINTER(vp), C.c_int, C.POINTER(vp), C.c_int,
                                       C.POINTER(vp), C.c_int, C.c_double, C.c_double, C.c_int,
                                       C.POINTER(SolveInfo)],
+            "mgx_amr_mac_create": [vp, C.POINTER(LevelDesc), C.c_int, C.POINTER(vp)],
+            "mgx_amr_mac_destroy": [vp],
+            "mgx_amr_mac_set_domain_bc": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "mgx_amr_mac_set_options": [vp, C.POINTER(MLMGOpts)],
+            "mgx_amr_mac_set_beta": [vp, C.POINTER(vp), C.c_int],
+            "mgx_amr_mac_project": [vp, C.POINTER(vp), C.c_int, C.POINTER(vp), C.c_int, C.c_double,
+                                    C.c_double, C.c_int, C.POINTER(SolveInfo)],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
                                            vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp,
                                            C.c_double, C.c_double, C.c_int,
@@ -639,4 +646,44 @@ class AmrNodalProjector:
     def close(self):
         if self.h:
             lib().mgx_amr_nodal_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class AmrMacProjector:
+    """Composite two-level MAC projection (mgx_amr_mac_*): levels[0] tiles the
+    domain, levels[1] holds the fine boxes (refinement ratio 2).  Face fields
+    are passed per level as [x, y, z]."""
+
+    def __init__(self, ctx, levels):
+        self.ctx = ctx
+        self.levels = levels
+        arr = (LevelDesc * len(levels))(*levels)
+        self._arr = arr
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_amr_mac_create(ctx.h, arr, len(levels), C.byref(self.h)))
+
+    def setDomainBC(self, lo, hi):
+        a = (C.c_int * 3)(*lo)
+        b = (C.c_int * 3)(*hi)
+        self.ctx.check(lib().mgx_amr_mac_set_domain_bc(self.h, a, b))
+
+    def set_options(self, opts):
+        self.ctx.check(lib().mgx_amr_mac_set_options(self.h, C.byref(opts)))
+
+    def initProjector(self, betas, ng=0):
+        """betas: per level a list of 3 boxed face arrays"""
+        self.ctx.check(lib().mgx_amr_mac_set_beta(self.h, _ptrs([b for lev in betas for b in lev]), ng))
+
+    updateCoeffs = initProjector
+
+    def project(self, umacs, umac_ng, rtol, atol=0.0, phis=None, phi_ng=0, mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_amr_mac_project(
+            self.h, _ptrs([u for lev in umacs for u in lev]), umac_ng,
+            _ptrs(phis) if phis else None, phi_ng, rtol, atol, mode, C.byref(info)))
+        return info
+
+    def close(self):
+        if self.h:
+            lib().mgx_amr_mac_destroy(self.h)
             self.h = C.c_void_p()

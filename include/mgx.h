@@ -261,6 +261,38 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
                                   int gphi_ng, double rtol, double atol, int mode,
                                   mgx_solve_info* info);
 
+/* ---- AMR: composite two-level MAC projection ----------------------------
+ * incflo builds one MacProjector over all AMR levels
+ * (src/convection/incflo_compute_MAC_projected_velocities.cpp:96-133:
+ * initProjector with beta per level, setDomainBC; :272-300: setUMAC with the
+ * face velocities of every level, project(rtol, atol) or project(phi, rtol,
+ * atol)), and the projector averages the fine faces down (:301-302).  Level
+ * rules as for mgx_amr_nodal_create.  The composite problem: fine cells in the
+ * fine region and uncovered coarse cells are the unknowns; every face between
+ * two of them carries beta (phi_j - phi_i) / (distance of the cell centres),
+ * 1.5 h_f across a fine face on the coarse-fine boundary; a coarse cell next
+ * to the fine region takes the 4 fine fluxes on each such face (reflux).
+ * Boundary types: Dirichlet, Neumann or inflow (no Robin). */
+typedef struct mgx_amr_mac_s* mgx_amr_mac;
+MGX_API int mgx_amr_mac_create(mgx_ctx ctx, const mgx_level_desc* levels, int nlevels,
+                               mgx_amr_mac* out);
+MGX_API int mgx_amr_mac_destroy(mgx_amr_mac h);
+/* setDomainBC (:114), as mgx_mac_set_domain_bc without ROBIN. */
+MGX_API int mgx_amr_mac_set_domain_bc(mgx_amr_mac h, const int lo[3], const int hi[3]);
+/* mac_proj.* knobs (maxiter / bottom_maxiter default 200). */
+MGX_API int mgx_amr_mac_set_options(mgx_amr_mac h, const mgx_mlmg_opts* opts);
+/* initProjector(info, beta) / updateCoeffs(beta) (:112, :131): beta[3 lev + d]
+ * is the boxed face field of level lev, direction d (ng ghosts). */
+MGX_API int mgx_amr_mac_set_beta(mgx_amr_mac h, const double* const* beta, int ng);
+/* setUMAC(umac) + project(phi, rtol, atol) (:272-299): umac[3 lev + d] (boxed
+ * faces, umac_ng ghosts) is overwritten with the projected velocity, coarse
+ * faces under the fine level with the mean of the fine faces on them;
+ * phi[lev] (boxed cells, may be NULL) receives phi, covered coarse cells the
+ * mean of their 8 fine cells.  MGX_MODE_FP64 or MGX_MODE_MIXED. */
+MGX_API int mgx_amr_mac_project(mgx_amr_mac h, double* const* umac, int umac_ng,
+                                double* const* phi, int phi_ng, double rtol, double atol,
+                                int mode, mgx_solve_info* info);
+
 /* AMR synchronisation, refinement ratio 2 (fine boxes aligned with coarse cells):
  * average_down_faces (src/convection/incflo_compute_advection_term.cpp:559-569):
  * every coarse face under the fine level (its closure included) becomes the
