@@ -9,9 +9,11 @@ p_nd = phi.  Before each step the projected velocity is restored from a
 pristine device copy (a device-to-device copy inside the timed region, so the
 reported rate is slightly conservative).
 
-Modes: fp64 (plain FP64 MLMG, the denominator) and mixed (FP32 V-cycle inside
-FP64 iterative refinement).  The headline `value` is the fastest converged
-mode among those run; `modes` lists every mode.
+Modes: fp64 (plain FP64 MLMG, the denominator), mixed (FP32 V-cycles inside
+FP64 iterative refinement, up to 3 per FP64 update) and learned (mixed plus the
+learned first step, when trained weights are present).  The headline `value`
+is the learned + mixed form (mixed without weights); `modes` lists every mode
+with its V-cycle and FP64-update counts.
 
 Prints ONE JSON line.  Only one GPU is used (--gpus > 1 exits).
 """
@@ -117,6 +119,9 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--vcycle-bits", type=int, default=32, choices=(16, 32),
                     help="storage of the mixed forms' finest V-cycle level")
+    ap.add_argument("--cycles-per-update", type=int, default=0,
+                    help="FP32 V-cycles per FP64 refinement step in the mixed forms "
+                         "(0: the library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cache", action="store_true",
@@ -146,6 +151,8 @@ def main():
     ctx = mgx.Context(0)
     P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))
     P.set_vcycle_bits(args.vcycle_bits)
+    if args.cycles_per_update:
+        P.set_options(mgx.default_opts(cycles_per_update=args.cycles_per_update))
     vel0 = ctx.array(B.to_boxed(ustar, bxs, 1))
     vel = ctx.array(vel0.n)
     rhod = ctx.array(B.to_boxed(rho, bxs, 0))
@@ -187,7 +194,8 @@ def main():
         results[name] = {
             "projections_per_s": args.steps / t,
             "ms_per_projection": 1e3 * t / args.steps,
-            "vcycles": infos[-1].iters,
+            "vcycles": infos[-1].vcycles,
+            "fp64_updates": infos[-1].iters,
             "fallback_used": infos[-1].fallback_used,
             "rel_resid": infos[-1].rel_resid,
             "converged": all(i.converged for i in infos),

@@ -573,6 +573,7 @@ class AmrMac {
         MGX_HIP(hipStreamSynchronize(s));
         if (info) {
             info->iters = it;
+            info->vcycles = it;
             info->converged = conv ? 1 : 0;
             info->nlevels = nlev + 1;
             info->fallback_used = 0;

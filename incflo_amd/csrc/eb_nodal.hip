@@ -692,6 +692,7 @@ void EBNodal::solve(double* b, double* phi_a, double* phi_b, double*& phi, doubl
     MGX_HIP(hipStreamSynchronize(s));
     if (info) {
         info->iters = it;
+        info->vcycles = it;
         info->converged = conv ? 1 : 0;
         info->nlevels = (int)L.size();
         info->fallback_used = 0;

@@ -330,6 +330,7 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     MGX_HIP(hipStreamSynchronize(s));
     if (info) {
         info->iters = it;
+        info->vcycles = it;
         info->converged = conv ? 1 : 0;
         info->nlevels = nlev;
         info->fallback_used = 0;

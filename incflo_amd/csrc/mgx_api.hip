@@ -150,6 +150,8 @@ static void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
     m.nu2 = o->nu2;
     m.omega = o->omega;
     m.stall_iters = o->stall_iters;
+    MGX_REQUIRE(o->cycles_per_update <= 8, MGX_ERR_ARG, "cycles_per_update must be at most 8");
+    m.cycles_per_update = o->cycles_per_update < 1 ? 1 : o->cycles_per_update;  // 0: the default
 }
 
 extern "C" {
@@ -170,6 +172,7 @@ MGX_API void mgx_default_mlmg_opts(mgx_mlmg_opts* o) {
     o->nu2 = m.nu2;
     o->omega = m.omega;
     o->stall_iters = m.stall_iters;
+    o->cycles_per_update = m.cycles_per_update;
 }
 
 MGX_API int mgx_ctx_create(int device, void* stream, mgx_ctx* out) {

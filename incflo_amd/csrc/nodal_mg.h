@@ -19,6 +19,7 @@ struct MGOpts {
     double omega = 8.0 / 9.0;
     int verbose = 0;
     int stall_iters = 3;
+    int cycles_per_update = 3;  // most FP32 V-cycles per FP64 refinement step (mixed modes)
 };
 
 inline KGeom kgeom(const NodalGeom& g) {
@@ -84,6 +85,9 @@ class NodalHierarchy {
                 bool x_ready = false);
     // set by the outer iteration when its residual pass also wrote L[0].x
     bool x0_ready = false;
+    // level-0 x holds an initial guess (the correction so far): the next cycle
+    // pre-smooths from it instead of from zero
+    bool x0_guess = false;
     // optional overset mask per level (1 solve, 0 fixed at 0; empty: none)
     std::vector<const unsigned char*> masks;
     bool bottom_fixed = false;  // the bottom level's overset mask holds a node at zero
@@ -107,6 +111,7 @@ class NodalHierarchy {
         T* x0 = nullptr;
         const T* b0 = nullptr;
         bool ready0 = false;
+        bool guess0 = false;
         bool timed = false;
         int* bottom_iters = nullptr;
         MGOpts o;

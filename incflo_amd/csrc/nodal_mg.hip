@@ -267,7 +267,7 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         L[l].x = L[l].x_buf.template as<T>();
         L[l].x2 = L[l].x2_buf.template as<T>();
     }
-    if (!x0_ready) {
+    if (!x0_ready && !x0_guess) {
         L[0].x = L[0].x_buf.template as<T>();
         L[0].x2 = L[0].x2_buf.template as<T>();
     }
@@ -279,9 +279,11 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         return;
     }
     const bool ready0 = x0_ready && o.nu1 > 0;
+    const bool guess0 = x0_guess;
     CycleGraph* hit = nullptr;
     for (auto& c : cache_)
-        if (c.x0 == L[0].x && c.b0 == L[0].b && c.ready0 == ready0 && c.timed == timed &&
+        if (c.x0 == L[0].x && c.b0 == L[0].b && c.ready0 == ready0 && c.guess0 == guess0 &&
+            c.timed == timed &&
             c.bottom_iters == bottom_iters && same_opts(c.o, o))
             hit = &c;
     if (!hit) {
@@ -289,10 +291,12 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         c.x0 = L[0].x;
         c.b0 = L[0].b;
         c.ready0 = ready0;
+        c.guess0 = guess0;
         c.timed = timed;
         c.bottom_iters = bottom_iters;
         c.o = o;
         const bool saved_ready = x0_ready;
+        const bool saved_guess = x0_guess;
         if (!cap_stream_ && hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess) {
             (void)hipGetLastError();
             cap_stream_ = nullptr;
@@ -325,6 +329,7 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
             (void)hipGetLastError();
             graphs = false;
             x0_ready = saved_ready;
+            x0_guess = saved_guess;
             for (size_t l = 1; l < L.size(); ++l) {
                 L[l].x = L[l].x_buf.template as<T>();
                 L[l].x2 = L[l].x2_buf.template as<T>();
@@ -348,6 +353,7 @@ void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters
         L[l].x2 = hit->x2s[l];
     }
     x0_ready = false;
+    x0_guess = false;
 }
 
 template <class T>
@@ -367,10 +373,13 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
                lds_for(lc - 1) <= kLdsBudget)
             --lc;
     bool ready = x0_ready && o.nu1 > 0;
+    const bool guess = x0_guess;
     x0_ready = false;
+    x0_guess = false;
     for (int l = 0; l < lc; ++l) {
         MGLevel<T>& lv = L[l];
-        smooth(l, o.nu1, true, o, s, ready);
+        // level 0 with an initial guess pre-smooths from it
+        smooth(l, o.nu1, !(l == 0 && guess), o, s, ready);
         SArgs<T, T> a{};
         a.g = lv.kg;
         a.c = lv.coef;
@@ -922,6 +931,7 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
 
     const int vprec = mixed ? (half ? P16 : P32) : P64;
     const double r0 = residual((learned || !half) ? (mixed ? P32 : P64) : P16, bnorm);
+    int vcycles = 0;
     const double tol = std::max(atol, rtol * bnorm);
     double rn = r0;
     int it = 0;
@@ -930,6 +940,7 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     if (learned && !conv && opts.maxiter > 0) {
         // learned initial guess / coarse correction; kept only if it reduces ||r||
         ++it;
+        ++vcycles;
         learned_correction(s);
         double* pout = other();
         const double rl = update(P32, pout, rn);
@@ -947,15 +958,32 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     }
     int prec = vprec;
     int stall = 0;
+    int ncyc = opts.cycles_per_update;  // FP32 cycles of the next refinement step
     while (!conv && it < opts.maxiter) {
         ++it;
         double* pout = other();
+        int ran = 1;
         if (prec == P16) vcycle16(s);
-        else if (prec == P32) H32.vcycle(opts, s, ibuf.as<int>());
-        else H64.vcycle(opts, s, ibuf.as<int>());
+        else if (prec == P32) {
+            // further FP32 cycles on the same FP64 residual, each from the
+            // correction so far (the FP64 update and residual run once)
+            H32.vcycle(opts, s, ibuf.as<int>());
+            for (; ran < ncyc; ++ran) {
+                H32.x0_guess = true;
+                H32.vcycle(opts, s, ibuf.as<int>());
+            }
+        } else H64.vcycle(opts, s, ibuf.as<int>());
+        vcycles += ran;
         const double prev = rn;
         rn = update(prec, pout, rn);
         phi = pout;
+        if (prec == P32 && opts.cycles_per_update > 1 && rn > tol && rn < prev) {
+            // the next step runs the cycles the measured per-cycle contraction
+            // says are left, at most cycles_per_update
+            const double rho = std::pow(rn / prev, 1.0 / ran);
+            const double need = std::ceil(std::log(tol / rn) / std::log(rho));
+            ncyc = (int)std::max(1.0, std::min((double)opts.cycles_per_update, need));
+        }
         if (opts.verbose)
             fprintf(stderr, "mgx nodal MLMG: iter %d |r| %.6e rel %.3e\n", it, rn,
                     bnorm > 0 ? rn / bnorm : rn);
@@ -979,6 +1007,7 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     MGX_HIP(hipStreamSynchronize(s));
     if (info) {
         info->iters = it;
+        info->vcycles = vcycles;
         info->converged = conv ? 1 : 0;
         info->nlevels = nlev;
         info->fallback_used = fallback;

@@ -35,13 +35,13 @@ class MLMGOpts(C.Structure):
                 ("bottom_maxiter", C.c_int), ("bottom_rtol", C.c_double),
                 ("bottom_atol", C.c_double), ("max_coarsening_level", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("omega", C.c_double),
-                ("stall_iters", C.c_int)]
+                ("stall_iters", C.c_int), ("cycles_per_update", C.c_int)]
 
 
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("nlevels", C.c_int),
                 ("fallback_used", C.c_int), ("bnorm", C.c_double), ("resnorm0", C.c_double),
-                ("resnorm", C.c_double), ("rel_resid", C.c_double)]
+                ("resnorm", C.c_double), ("rel_resid", C.c_double), ("vcycles", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -88,6 +88,13 @@ typedef struct {
     int stall_iters;           /* learned mode: fall back to plain MLMG when
                                   the residual has not dropped 10x within this
                                   many outer iterations (default 3)          */
+    int cycles_per_update;     /* nodal mixed / learned modes (no reference
+                                  counterpart): FP32 V-cycles per FP64
+                                  refinement step; each after the first starts
+                                  from the correction so far; the last step
+                                  runs only the cycles the measured
+                                  contraction says are left (default 3 for
+                                  the nodal solver; 0 means the default)     */
 } mgx_mlmg_opts;
 
 typedef struct {
@@ -99,6 +106,9 @@ typedef struct {
     double resnorm0;    /* initial ||b - L phi||_inf                         */
     double resnorm;     /* final ||b - L phi||_inf (FP64)                    */
     double rel_resid;   /* resnorm / bnorm                                   */
+    int vcycles;        /* V-cycles run (iters, or more in the mixed forms
+                           with cycles_per_update > 1; a learned step
+                           counts as one)                                   */
 } mgx_solve_info;
 
 /* ---- library / context ------------------------------------------------ */

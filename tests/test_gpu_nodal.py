@@ -132,8 +132,10 @@ def test_mixed_solve_reaches_1e10_and_matches_fp64(ctx, orc):
     assert np.max(np.abs(r)) / np.max(np.abs(bm)) <= 1e-10
     xo, io, _ = orc.nodal_solve(g, sigma, b, rtol=1e-13)
     assert _rel(x, xo) < 1e-7
-    # the FP32 V-cycle contracts about as well as the FP64 one
-    assert info.iters <= io.iters + 2
+    # the FP32 V-cycle contracts about as well as the FP64 one (several FP32
+    # cycles per FP64 refinement step: compare V-cycles, not steps)
+    assert info.vcycles <= io.iters + 2
+    assert info.iters < info.vcycles
 
 
 def _vel_case(orc, g, rng, inflow=None):
@@ -349,7 +351,7 @@ def test_fp16_vcycle_storage_converges_like_fp32(ctx, orc, sigma_scale):
         bb -= bb.mean()
         r = orc.nodal_residual(g, sigma, bb, x)
         assert np.max(np.abs(r)) <= 1.01e-10 * np.max(np.abs(bb))
-        iters[bits] = info.iters
+        iters[bits] = info.vcycles
         P.close()
     assert iters[16] <= iters[32] + 1, iters
 
