@@ -113,6 +113,7 @@ struct mgx_nodal_s {
     }
     void zero_phi() {
         MGX_HIP(hipMemsetAsync(solver.phi, 0, g.nnode() * sizeof(double), stream()));
+        solver.phi_zero = true;
     }
     UpdateArgs update_args() {
         UpdateArgs u{};

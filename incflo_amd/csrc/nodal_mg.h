@@ -145,6 +145,8 @@ class NodalSolver {
     DevBuf ibuf;             // device ints (bottom iteration counter)
     unsigned long long* h_pinned = nullptr;
     double* phi = nullptr;   // current outer iterate (points into phi_a / phi_b)
+    // the caller zeroed phi before this solve: the first residual is b itself
+    bool phi_zero = false;
     LaunchTimer fine_timer;  // finest-level smoother launches
     bool sigma_set = false;
     bool h32_sigma = false;

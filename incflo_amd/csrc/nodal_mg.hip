@@ -773,6 +773,8 @@ void NodalSolver::learned_correction(hipStream_t s) {
 
 void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_info* info,
                         hipStream_t s) {
+    const bool zero_start = phi_zero;  // one solve only, also when it fails
+    phi_zero = false;
     MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "nodal solve: sigma has not been set");
     MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED || mode == MGX_MODE_LEARNED,
                 MGX_ERR_UNSUPPORTED, "nodal solve: unknown mode");
@@ -930,7 +932,20 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     auto other = [&]() { return (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>(); };
 
     const int vprec = mixed ? (half ? P16 : P32) : P64;
-    const double r0 = residual((learned || !half) ? (mixed ? P32 : P64) : P16, bnorm);
+    const int prec0 = (learned || !half) ? (mixed ? P32 : P64) : P16;
+    const bool from_zero = zero_start && prec0 != P16;
+    // phi = 0: r0 = b (Dirichlet rows zeroed, mean removed), ||r0|| = ||b||;
+    // the first V-cycle then runs its own first sweep
+    const double r0 = from_zero ? bnorm : residual(prec0, bnorm);
+    if (from_zero) {
+        if (prec0 == P32) {
+            cast_kernel<float, double><<<grid1d(nn), 256, 0, s>>>(b, res32.as<float>(), nn);
+            H32.x0_ready = false;
+        } else {
+            MGX_HIP(hipMemcpyAsync(res64.p, b, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+            H64.x0_ready = false;
+        }
+    }
     int vcycles = 0;
     const double tol = std::max(atol, rtol * bnorm);
     double rn = r0;
