@@ -77,3 +77,51 @@ def variable_density_case(n, seed):
     dt = 0.5 * h / max(umax, 1e-30)
     ustar = advected_velocity(u, dt, h)
     return rho, ustar, dt
+
+
+def average_down(a):
+    """8-cell means: (..., 2n, 2n, 2n) -> (..., n, n, n)"""
+    s = a.shape
+    b = a.reshape(s[:-3] + (s[-3] // 2, 2, s[-2] // 2, 2, s[-1] // 2, 2))
+    return b.mean(axis=(-5, -3, -1))
+
+
+def sphere_fine_boxes(nc, seed, frac=0.2, max_grid_size=32, blocking_factor=8,
+                      rmin=0.05, rmax=0.15):
+    """Fine boxes (refinement ratio 2, periodic unit cube of nc^3 coarse cells)
+    covering seeded random spheres of radius rmin..rmax until they hold
+    `frac` of the volume.  Fine cells are tagged in blocking_factor^3 blocks
+    (a block whose centre lies in a sphere);
+    each max_grid_size^3 chunk of the fine domain contributes the bounding box
+    of its tagged blocks, so the boxes are aligned with the blocking factor and
+    of uneven sizes (the kind of layout AMReX's clustering produces)."""
+    rng = np.random.default_rng(seed + 7919)
+    nf = 2 * nc
+    bf = blocking_factor
+    nb = nf // bf
+    c = (np.arange(nb) + 0.5) / nb
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    inside = np.zeros((nb, nb, nb), bool)
+    while inside.mean() < frac:
+        p = rng.random(3)
+        r = rng.uniform(rmin, rmax)
+        d2 = 0.0
+        for q, A in zip(p, (X, Y, Z)):
+            dq = np.abs(A - q)
+            dq = np.minimum(dq, 1.0 - dq)
+            d2 = d2 + dq * dq
+        inside |= d2 <= r * r
+    tag = inside  # blocks whose centre lies in a sphere
+    m = max_grid_size // bf
+    boxes = []
+    for K in range(0, nb, m):
+        for J in range(0, nb, m):
+            for I in range(0, nb, m):
+                t = tag[K:K + m, J:J + m, I:I + m]
+                if not t.any():
+                    continue
+                ks, js, is_ = (np.flatnonzero(t.any(axis=a)) for a in ((1, 2), (0, 2), (0, 1)))
+                lo = (bf * (I + is_[0]), bf * (J + js[0]), bf * (K + ks[0]))
+                hi = (bf * (I + is_[-1] + 1) - 1, bf * (J + js[-1] + 1) - 1, bf * (K + ks[-1] + 1) - 1)
+                boxes.append((lo, hi))
+    return boxes
