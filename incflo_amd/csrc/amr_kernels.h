@@ -109,9 +109,11 @@ __device__ __forceinline__ double amr_restrict_sel(const KGeom& gf, const unsign
     return acc;
 }
 
+// max over the block (256 threads, every one calls this), one atomic per block
 __device__ __forceinline__ void wave_max_atomic(unsigned long long* norm, double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(norm, v);
+    __shared__ double red[4];
+    const double m = block_max<256>(v, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_nonneg(norm, m);
 }
 
 // coarse unknowns: out = base + sgn R(y on dF); every other coarse node 0;

@@ -106,9 +106,12 @@ __device__ __forceinline__ bool grid_point(int nx, int ny, int& i, int& j, int& 
 inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + 63) / 64, (ny + 3) / 4, nz); }
 inline dim3 block3() { return dim3(64, 4, 1); }
 
-// atomic max for non-negative doubles via their bit pattern
+// atomic max for non-negative doubles via their bit pattern; the atomic is
+// issued only when v exceeds the stored value (with many blocks on one
+// address most skip it, where unconditional atomics serialise in L2)
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
-    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    if (bits > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, bits);
 }
 
 template <int NT>
