@@ -76,7 +76,7 @@ class AmrNodal {
     NodalHierarchy<double> H64;
     NodalHierarchy<float> H32;
     DevBuf bC;                          // level-0 rhs of the coarse correction (FP64-sized)
-    DevBuf sigc, sigc_out, sigf, sigf32;
+    DevBuf sigc, sigc_out, sigc_out32, sigf, sigf32;
     DevBuf ftype, cdof, ccov;
     DevBuf xc, bc, rtmp, xf, bf, yf;
     DevBuf ef_a, ef_b, rfT, yT, rcT;    // V-cycle work (FP64-sized, reused for FP32)
@@ -134,6 +134,7 @@ class AmrNodal {
         const long nnc = gc.nnode(), nnf = gf.nnode();
         sigc.alloc(gc.nsig() * sizeof(double));
         sigc_out.alloc(gc.nsig() * sizeof(double));
+        sigc_out32.alloc(gc.nsig() * sizeof(float));
         sigf.alloc(gf.nsig() * sizeof(double));
         sigf32.alloc(gf.nsig() * sizeof(float));
         // ghost cells outside non-periodic faces stay 0 (no element there)
@@ -159,6 +160,8 @@ class AmrNodal {
             kc, ccov.as<unsigned char>(), sigc.as<double>(), sigc_out.as<double>());
         cast_kernel<float, double><<<grid1d(gf.nsig()), 256, 0, s>>>(sigf.as<double>(),
                                                                     sigf32.as<float>(), gf.nsig());
+        cast_kernel<float, double><<<grid1d(gc.nsig()), 256, 0, s>>>(sigc_out.as<double>(),
+                                                                    sigc_out32.as<float>(), gc.nsig());
         sigma_set = true;
         h64_sigma = h32_sigma = false;
     }
@@ -226,7 +229,7 @@ class AmrNodal {
     // one composite V-cycle on the residual in rfT / rcT; the correction is
     // added to (xc, xf)
     template <class T>
-    void vcycle(NodalHierarchy<T>& H, const T* sfT, hipStream_t s) {
+    void vcycle(NodalHierarchy<T>& H, const T* sfT, const T* scoT, hipStream_t s) {
         const KGeom kc = kgeom(gc), kf = kgeom(gf);
         const unsigned char* ft = ftype.as<unsigned char>();
         T* e = ef_a.as<T>();
@@ -267,6 +270,27 @@ class AmrNodal {
             a.xout = e2;
             launch_stencil<T, T, KS_JACOBI>(gf, a, s, false);
             std::swap(e, e2);
+        }
+        {
+            // the coarse unknowns next to F: one Jacobi sweep with the
+            // composite operator (e on dF follows the coarse correction)
+            SArgs<T, T> af{};
+            af.g = kf;
+            af.c = a.c;
+            af.x = e;
+            af.sig = sfT;
+            af.rout = yT.as<T>();
+            launch_stencil<T, T, KS_APPLY>(gf, af, s, false);
+            SArgs<T, T> ac{};
+            ac.g = kc;
+            ac.c = make_coef<T>(gc.dx);
+            ac.x = H.L[0].x;
+            ac.sig = scoT;
+            ac.rout = rtmp.as<T>();
+            launch_stencil<T, T, KS_APPLY>(gc, ac, s, false);
+            amr_coarse_jacobi_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+                kf, kc, ft, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), yT.as<T>(), H.L[0].sig,
+                make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), H.L[0].x);
         }
         amr_add_kernel<T><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
             kf, ft, AMR_DOF, e, xf.as<double>());
@@ -319,10 +343,10 @@ class AmrNodal {
         while (!conv && it < opts.maxiter) {
             ++it;
             if (mixed) {
-                vcycle<float>(H32, sigf32.as<float>(), s);
+                vcycle<float>(H32, sigf32.as<float>(), sigc_out32.as<float>(), s);
                 rn = residual<float>(s);
             } else {
-                vcycle<double>(H64, sigf.as<double>(), s);
+                vcycle<double>(H64, sigf.as<double>(), sigc_out.as<double>(), s);
                 rn = residual<double>(s);
             }
             if (opts.verbose)

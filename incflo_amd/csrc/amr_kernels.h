@@ -189,6 +189,51 @@ static __global__ void __launch_bounds__(256) amr_coarse_rhs_kernel(
     bC[id] = (T)acc;
 }
 
+// One Jacobi sweep on the coarse unknowns of the composite correction
+// equation, after the fine post-smoothing: with yc = L_c,out ec (coarse
+// elements outside F) and yf = L_f e (the fine correction, dF following the
+// coarse one), t = r_c - (yc + R(yf on dF)) and ec += omega t / diag, diag of
+// the coarse operator with the full coarse sigma (near dF the composite
+// diagonal differs by up to ~20%, which a damped Jacobi sweep tolerates).
+// Smoothing the coarse unknowns next to F with the composite operator is what
+// the coarse correction alone leaves undone.
+template <class T>
+static __global__ void __launch_bounds__(256) amr_coarse_jacobi_kernel(
+    KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
+    const T* __restrict__ rc, const T* __restrict__ yc, const T* __restrict__ yf,
+    const T* __restrict__ sig, double diag4, double omega, T* __restrict__ ec) {
+    int I, J, K;
+    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
+    if (!cdof[id] || is_dir(gc, I, J, K)) return;
+    double acc = (double)yc[id];
+    for (int oz = -1; oz <= 1; ++oz) {
+        int fk = 2 * K + oz;
+        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
+        else if (fk < 0 || fk >= gf.nn[2]) continue;
+        const double wz = oz == 0 ? 0.5 : 0.25;
+        for (int oy = -1; oy <= 1; ++oy) {
+            int fj = 2 * J + oy;
+            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
+            else if (fj < 0 || fj >= gf.nn[1]) continue;
+            const double wy = oy == 0 ? 0.5 : 0.25;
+            for (int ox = -1; ox <= 1; ++ox) {
+                int fi = 2 * I + ox;
+                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
+                else if (fi < 0 || fi >= gf.nn[0]) continue;
+                const long m = fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk);
+                if (ft[m] == AMR_BDRY) acc += (ox == 0 ? 0.5 : 0.25) * wy * wz * (double)yf[m];
+            }
+        }
+    }
+    double ss = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        ss += (double)sig[(I + (q & 1)) + (long)gc.sg[0] * ((J + ((q >> 1) & 1)) + (long)gc.sg[1] * (K + ((q >> 2) & 1)))];
+    const double diag = -diag4 * ss;
+    if (diag != 0.0) ec[id] = (T)((double)ec[id] + omega * ((double)rc[id] - acc) / diag);
+}
+
 // dst += src where mask (and not a Dirichlet node of g)
 template <class T>
 static __global__ void __launch_bounds__(256) amr_add_kernel(KGeom g, const unsigned char* __restrict__ mask,
