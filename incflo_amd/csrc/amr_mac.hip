@@ -109,31 +109,35 @@ static __global__ void amac_fine_kernel(MGeomK gf, MGeomK gc, AmacCoef<T> c,
     }
 }
 
-// Coarse rows on the uncovered cells: r = b - L (xc, xf); xc NULL = 0.
-// Covered cells get r = 0.  Domain faces: Dirichlet (value 0 at h_c/2) or
-// Neumann as on the single level.
-template <class T, class TX, class TO>
+// Coarse rows on the uncovered cells.
+// RESID: r = b - L (xc, xf); xc NULL = 0; covered cells get r = 0.
+// GSRB:  one colour of red-black Gauss-Seidel on xc (uncovered cells, the
+//        fine values xf held), in place: smooths the coarse cells next to the
+//        fine region with the composite rows.
+// Domain faces: Dirichlet (value 0 at h_c/2) or Neumann as on the single level.
+template <class T, class TX, class TO, int MODE>
 static __global__ void amac_coarse_kernel(MGeomK gc, MGeomK gf, AmacCoef<T> c,
                                           const unsigned char* __restrict__ cov,
                                           const T* __restrict__ bcx, const T* __restrict__ bcy,
                                           const T* __restrict__ bcz, const T* __restrict__ bfx,
                                           const T* __restrict__ bfy, const T* __restrict__ bfz,
-                                          const TX* __restrict__ xc, const TX* __restrict__ xf,
+                                          TX* xc, const TX* __restrict__ xf,
                                           const T* __restrict__ b, TO* __restrict__ r,
-                                          unsigned long long* norm) {
+                                          unsigned long long* norm, int color) {
     __shared__ double red[4];
     int I, J, K;
     double m = 0.0;
     if (grid_point(gc.n[0], gc.n[1], I, J, K)) {
         const long id = cidx(gc, I, J, K);
+        const bool mine = MODE != AM_GSRB || (((I + J + K + color) & 1) == 0);
         if (cov[id]) {
-            r[id] = (TO)0;
-        } else {
+            if (MODE == AM_RESID) r[id] = (TO)0;
+        } else if (mine) {
             const int P[3] = {I, J, K};
             const T* betac[3] = {bcx, bcy, bcz};
             const T* betaf[3] = {bfx, bfy, bfz};
             const T xp = xc ? (T)xc[id] : (T)0;
-            T acc = 0;
+            T acc = 0, dg = 0;
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -145,7 +149,10 @@ static __global__ void amac_coarse_kernel(MGeomK gc, MGeomK gf, AmacCoef<T> c,
                     const T bet = betac[d][fidx(gc, d, F[0], F[1], F[2])];
                     if (Q[d] < 0 || Q[d] >= gc.n[d]) {
                         if (!gc.per[d]) {
-                            if (s ? gc.dhi[d] : gc.dlo[d]) acc -= 2 * c.ihc2[d] * bet * xp;
+                            if (s ? gc.dhi[d] : gc.dlo[d]) {
+                                acc -= 2 * c.ihc2[d] * bet * xp;
+                                dg -= 2 * c.ihc2[d] * bet;
+                            }
                             continue;
                         }
                         Q[d] = wrap_idx(Q[d], gc.n[d]);
@@ -154,6 +161,7 @@ static __global__ void amac_coarse_kernel(MGeomK gc, MGeomK gf, AmacCoef<T> c,
                     if (!cov[qid]) {
                         const T xq = xc ? (T)xc[qid] : (T)0;
                         acc += c.ihc2[d] * bet * (xq - xp);
+                        dg -= c.ihc2[d] * bet;
                         continue;
                     }
                     // coarse-fine face: the 4 fine cells of Q next to it, each
@@ -173,14 +181,19 @@ static __global__ void amac_coarse_kernel(MGeomK gc, MGeomK gf, AmacCoef<T> c,
                             const T bf = betaf[d][fidx(gf, d, fa[0], fa[1], fa[2])];
                             const T t = bf * c.ihf2[d] * (T)(1.0 / 12.0);
                             acc += t * ((T)xf[cidx(gf, f[0], f[1], f[2])] - xp);
+                            dg -= t;
                         }
                 }
-            const T rv = b[id] - acc;
-            r[id] = (TO)rv;
-            m = fabs((double)rv);
+            if (MODE == AM_GSRB) {
+                if (dg != (T)0) xc[id] = (TX)(xp + (b[id] - acc) / dg);
+            } else {
+                const T rv = b[id] - acc;
+                r[id] = (TO)rv;
+                m = fabs((double)rv);
+            }
         }
     }
-    if (norm) {
+    if (MODE == AM_RESID && norm) {
         const double bm = block_max<256>(m, red);
         if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_nonneg(norm, bm);
     }
@@ -467,11 +480,11 @@ class AmrMac {
             kf, kc, c, fmask.as<unsigned char>(), betaf[0].as<double>(), betaf[1].as<double>(),
             betaf[2].as<double>(), xf.as<double>(), xc.as<double>(), bf.as<double>(), rf.as<TO>(),
             scal.as<unsigned long long>() + 1, 0);
-        amac_coarse_kernel<double, double, TO><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
+        amac_coarse_kernel<double, double, TO, AM_RESID><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
             kc, kf, c, cmask.as<unsigned char>(), betac[0].as<double>(), betac[1].as<double>(),
             betac[2].as<double>(), betaf[0].as<double>(), betaf[1].as<double>(), betaf[2].as<double>(),
             xc.as<double>(), xf.as<double>(), bc.as<double>(), rc.as<TO>(),
-            scal.as<unsigned long long>() + 1);
+            scal.as<unsigned long long>() + 1, 0);
         return read_norm(1, s);
     }
 
@@ -500,15 +513,20 @@ class AmrMac {
             kf, kc, c, fmask.as<unsigned char>(), bfT[0], bfT[1], bfT[2], e, nullptr, rf.as<T>(),
             rf2.as<T>(), nullptr, 0);
         const T* const bcT[3] = {H.L[0].beta[0], H.L[0].beta[1], H.L[0].beta[2]};
-        amac_coarse_kernel<T, T, T><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
+        amac_coarse_kernel<T, T, T, AM_RESID><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
             kc, kf, c, cmask.as<unsigned char>(), bcT[0], bcT[1], bcT[2], bfT[0], bfT[1], bfT[2],
-            nullptr, e, rc.as<T>(), rc2.as<T>(), nullptr);
+            nullptr, e, rc.as<T>(), rc2.as<T>(), nullptr, 0);
         amac_restrict_kernel<T><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
             kc, kf, cmask.as<unsigned char>(), rc2.as<T>(), rf2.as<T>(), H.L[0].b);
         H.vcycle(opts, s, ibuf.as<int>());
         amac_prolong_kernel<T><<<grid3(gf.n[0], gf.n[1], gf.n[2]), block3(), 0, s>>>(
             kf, kc, fmask.as<unsigned char>(), H.L[0].x, e);
         for (int q = 0; q < opts.nu2; ++q) gsrb_fine<T>(bfT, e, H.L[0].x, rf.as<T>(), s);
+        // the coarse cells next to F with the composite rows (fine values held)
+        for (int color = 0; color < 2; ++color)
+            amac_coarse_kernel<T, T, T, AM_GSRB><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
+                kc, kf, c, cmask.as<unsigned char>(), bcT[0], bcT[1], bcT[2], bfT[0], bfT[1], bfT[2],
+                H.L[0].x, e, rc.as<T>(), nullptr, nullptr, color);
         amac_add_kernel<T><<<grid1d(nf), 256, 0, s>>>(fmask.as<unsigned char>(), 1, e, xf.as<double>(), nf);
         amac_add_kernel<T><<<grid1d(nc), 256, 0, s>>>(cmask.as<unsigned char>(), 0, H.L[0].x,
                                                       xc.as<double>(), nc);
