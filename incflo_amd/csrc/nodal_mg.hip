@@ -952,8 +952,32 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     int it = 0;
     int fallback = 0;
     bool conv = (r0 <= tol);
-    if (learned && !conv && opts.maxiter > 0) {
-        // learned initial guess / coarse correction; kept only if it reduces ||r||
+    if (learned && !conv && opts.maxiter > 0 && !half) {
+        // learned initial guess / coarse correction, judged in FP32 on the
+        // correction equation: kept (as the first refinement step's initial
+        // guess, its V-cycles pre-smooth from it) only if ||r - L e|| < ||r||;
+        // the FP64 residual after that step judges the iterate as always
+        ++vcycles;
+        learned_correction(s);
+        MGLevel<float>& l0 = H32.L[0];
+        SArgs<float, float> a{};
+        a.g = l0.kg;
+        a.c = l0.coef;
+        a.x = l0.x;
+        a.b = l0.b;
+        a.sig = l0.sig;
+        a.rout = l0.r;
+        zero_norm(2, s);
+        a.norm = scal.as<unsigned long long>() + 2;
+        launch_stencil<float, float, KS_RESID>(l0.g, a, s, false);
+        const double rl = read_norm(2, s);
+        if (opts.verbose)
+            fprintf(stderr, "mgx nodal MLMG: learned step |r - L e| %.6e (from %.6e)\n", rl, rn);
+        H32.x0_ready = false;
+        if (rl < rn) H32.x0_guess = true;
+        else fallback = 1;
+    } else if (learned && !conv && opts.maxiter > 0) {
+        // FP16-stored cycles: the step is an FP64 update of its own
         ++it;
         ++vcycles;
         learned_correction(s);
