@@ -67,6 +67,9 @@ class MacHierarchy {
     std::vector<MacLevel<T>> L;
     DevBuf bottom_work;
     bool built = false;
+    // level-0 x holds an initial guess (the correction so far): the next cycle
+    // smooths from it instead of from zero
+    bool x0_guess = false;
     void build(const MacGeom& g0, int nlev);
     // level-0 beta from FP64 face arrays, then coarsen; with Robin sides the
     // boundary planes of each level are then scaled by rscale(l, d, side)

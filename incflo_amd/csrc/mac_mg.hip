@@ -79,7 +79,7 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
     for (int l = 0; l < nl - 1; ++l) {
         MacLevel<T>& lv = L[l];
         const long n = lv.g.ncell();
-        zero_kernel<T><<<grid1d(n), 256, 0, s>>>(lv.x, n);
+        if (l > 0 || !x0_guess) zero_kernel<T><<<grid1d(n), 256, 0, s>>>(lv.x, n);
         for (int q = 0; q < o.nu1; ++q) gsrb(lv, s);
         mac_resid_kernel<T, T, MK_RESID><<<grid3(lv.g.n[0], lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
             lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.b, lv.x, nullptr, nullptr, lv.r,
@@ -98,6 +98,7 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
                                                    o.bottom_rtol, lb.g.singular() ? 1 : 0,
                                                    bottom_iters);
     }
+    x0_guess = false;
     for (int l = nl - 2; l >= 0; --l) {
         MacLevel<T>& lv = L[l];
         mac_prolong_add_kernel<T><<<grid3(lv.g.n[0], lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
@@ -299,13 +300,23 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     const double tol = std::max(atol, rtol * bnorm);
     double rn = r0;
     int it = 0;
+    int vcycles = 0;
     bool conv = (r0 <= tol);
+    int ncyc = opts.cycles_per_update;  // FP32 cycles of the next refinement step
     while (!conv && it < opts.maxiter) {
         ++it;
         zero_u64_kernel<<<1, 64, 0, s>>>(nrm, 1);
         double* pout = (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>();
+        int ran = 1;
+        const double prev = rn;
         if (mixed) {
+            // further FP32 cycles on the same FP64 residual, each from the
+            // correction so far
             H32.vcycle(opts, s, ibuf.as<int>());
+            for (; ran < ncyc; ++ran) {
+                H32.x0_guess = true;
+                H32.vcycle(opts, s, ibuf.as<int>());
+            }
             mac_resid_kernel<double, float, MK_UPDRES><<<gr, block3(), 0, s>>>(
                 kg, c64, b0[0], b0[1], b0[2], b, phi, H32.L[0].x, pout, res32.as<float>(), nrm);
         } else {
@@ -315,10 +326,17 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
         }
         phi = pout;
         rn = read_norm(1, s);
+        vcycles += ran;
         if (opts.verbose)
             fprintf(stderr, "mgx MAC MLMG: iter %d |r| %.6e rel %.3e\n", it, rn,
                     bnorm > 0 ? rn / bnorm : rn);
         conv = (rn <= tol);
+        if (mixed && opts.cycles_per_update > 1 && !conv && rn < prev) {
+            // the cycles the measured per-cycle contraction says are left
+            const double rho = std::pow(rn / prev, 1.0 / ran);
+            const double need = std::ceil(std::log(tol / rn) / std::log(rho));
+            ncyc = (int)std::max(1.0, std::min((double)opts.cycles_per_update, need));
+        }
     }
     if (g.singular()) {
         const int np = (int)std::min<long>(1024, std::max<long>(1, (n + 255) / 256));
@@ -330,7 +348,7 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     MGX_HIP(hipStreamSynchronize(s));
     if (info) {
         info->iters = it;
-        info->vcycles = it;
+        info->vcycles = vcycles;
         info->converged = conv ? 1 : 0;
         info->nlevels = nlev;
         info->fallback_used = 0;
