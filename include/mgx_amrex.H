@@ -153,20 +153,22 @@ struct DevField {
 };
 }  // namespace detail
 
-// Hydro::NodalProjector (single level).  project() overwrites vel in place.
+// Hydro::NodalProjector over one level, or two (refinement ratio 2: the
+// composite projection of mgx_amr_nodal_*, src/projection/
+// incflo_apply_nodal_projection.cpp:124-215).  project() overwrites vel in place.
 class NodalProjector {
    public:
     NodalProjector(Context& ctx, std::vector<MultiFab*> vel, std::vector<MultiFab const*> sigma,
                    std::vector<Geometry> geom, LPInfo info = LPInfo())
-        : ctx_(ctx), vel_(vel), geom_(geom) {
+        : ctx_(ctx), vel_(vel), geom_(geom), sigma_(sigma) {
         init(info);
-        detail::one_level(sigma.size(), "NodalProjector");
-        ctx_.check(mgx_nodal_set_sigma(h_, sigma[0]->p, sigma[0]->ngrow), "set sigma");
+        set_sigma();
     }
     NodalProjector(Context& ctx, std::vector<MultiFab*> vel, double const_sigma,
                    std::vector<Geometry> geom, LPInfo info = LPInfo())
         : ctx_(ctx), vel_(vel), geom_(geom) {
         init(info);
+        if (ha_) Abort("NodalProjector: two levels need a sigma field per level");
         ctx_.check(mgx_nodal_set_const_sigma(h_, const_sigma), "set sigma");
     }
     NodalProjector(std::vector<MultiFab*> vel, std::vector<MultiFab const*> sigma,
@@ -175,7 +177,10 @@ class NodalProjector {
     NodalProjector(std::vector<MultiFab*> vel, double const_sigma, std::vector<Geometry> geom,
                    LPInfo info = LPInfo())
         : NodalProjector(Context::current(), vel, const_sigma, geom, info) {}
-    ~NodalProjector() { mgx_nodal_destroy(h_); }
+    ~NodalProjector() {
+        if (h_) mgx_nodal_destroy(h_);
+        if (ha_) mgx_amr_nodal_destroy(ha_);
+    }
     NodalProjector(const NodalProjector&) = delete;
     NodalProjector& operator=(const NodalProjector&) = delete;
 
@@ -185,24 +190,58 @@ class NodalProjector {
             l[k] = (int)lo[k];
             r[k] = (int)hi[k];
         }
-        ctx_.check(mgx_nodal_set_domain_bc(h_, l, r), "setDomainBC");
+        if (ha_) {
+            // the two-level handle re-sets up its levels: hand sigma over again
+            ctx_.check(mgx_amr_nodal_set_domain_bc(ha_, l, r), "setDomainBC");
+            set_sigma();
+        } else {
+            ctx_.check(mgx_nodal_set_domain_bc(h_, l, r), "setDomainBC");
+        }
     }
-    // solve mode: MGX_MODE_FP64 (default), MGX_MODE_MIXED, MGX_MODE_LEARNED
+    // solve mode: MGX_MODE_FP64 (default), MGX_MODE_MIXED, MGX_MODE_LEARNED (one level)
     void setSolveMode(int mode) { mode_ = mode; }
     void setVerbose(int v) {
         opts_.verbose = v;
-        ctx_.check(mgx_nodal_set_options(h_, &opts_), "set options");
+        if (ha_) {
+            ctx_.check(mgx_amr_nodal_set_options(ha_, &opts_), "set options");
+            set_sigma();
+        } else {
+            ctx_.check(mgx_nodal_set_options(h_, &opts_), "set options");
+        }
     }
     void project(double rtol, double atol) {
+        if (ha_) {
+            for (int l = 0; l < 2; ++l) {
+                phis_[l].alloc(ctx_.get(), vel_[l]->ba, 1, true);
+                gphis_[l].alloc(ctx_.get(), vel_[l]->ba, 3, false);
+            }
+            if (vel_[0]->ngrow != vel_[1]->ngrow) Abort("NodalProjector: velocity ghost widths differ");
+            double* v[2] = {vel_[0]->p, vel_[1]->p};
+            double* ph[2] = {phis_[0].view.p, phis_[1].view.p};
+            double* gp[2] = {gphis_[0].view.p, gphis_[1].view.p};
+            ctx_.check(mgx_amr_nodal_project(ha_, v, vel_[0]->ngrow, ph, 0, gp, 0, rtol, atol, mode_,
+                                             &info_),
+                       "NodalProjector::project");
+            projected_ = true;
+            return;
+        }
         ctx_.check(mgx_nodal_project(h_, vel_[0]->p, vel_[0]->ngrow, rtol, atol, mode_, &info_),
                    "NodalProjector::project");
     }
     std::vector<MultiFab*> getPhi() {
+        if (ha_) {
+            if (!projected_) Abort("getPhi before project");
+            return {&phis_[0].view, &phis_[1].view};
+        }
         phi_.alloc(ctx_.get(), vel_[0]->ba, 1, true);
         ctx_.check(mgx_nodal_get_phi(h_, phi_.view.p, 0), "getPhi");
         return {&phi_.view};
     }
     std::vector<MultiFab*> getGradPhi() {
+        if (ha_) {
+            if (!projected_) Abort("getGradPhi before project");
+            return {&gphis_[0].view, &gphis_[1].view};
+        }
         gphi_.alloc(ctx_.get(), vel_[0]->ba, 3, false);
         ctx_.check(mgx_nodal_get_gradphi(h_, gphi_.view.p, 0), "getGradPhi");
         return {&gphi_.view};
@@ -213,6 +252,17 @@ class NodalProjector {
 
    private:
     void init(const LPInfo& info) {
+        if (vel_.size() == 2 && geom_.size() == 2) {
+            flat_ = vel_[0]->ba->flat();
+            flat1_ = vel_[1]->ba->flat();
+            mgx_level_desc d[2] = {detail::level_desc(geom_[0], flat_),
+                                   detail::level_desc(geom_[1], flat1_)};
+            ctx_.check(mgx_amr_nodal_create(ctx_.get(), d, 2, &ha_), "NodalProjector");
+            mgx_default_mlmg_opts(&opts_);
+            opts_.max_coarsening_level = info.max_coarsening_level;
+            ctx_.check(mgx_amr_nodal_set_options(ha_, &opts_), "set options");
+            return;
+        }
         detail::one_level(vel_.size(), "NodalProjector");
         detail::one_level(geom_.size(), "NodalProjector");
         flat_ = vel_[0]->ba->flat();
@@ -222,31 +272,61 @@ class NodalProjector {
         opts_.max_coarsening_level = info.max_coarsening_level;
         ctx_.check(mgx_nodal_set_options(h_, &opts_), "set options");
     }
+    void set_sigma() {
+        if (sigma_.empty()) return;
+        if (ha_) {
+            if (sigma_.size() != 2 || sigma_[0]->ngrow != sigma_[1]->ngrow)
+                Abort("NodalProjector: two levels need sigma fields of equal ghost width");
+            const double* sg[2] = {sigma_[0]->p, sigma_[1]->p};
+            ctx_.check(mgx_amr_nodal_set_sigma(ha_, sg, sigma_[0]->ngrow), "set sigma");
+        } else {
+            detail::one_level(sigma_.size(), "NodalProjector");
+            ctx_.check(mgx_nodal_set_sigma(h_, sigma_[0]->p, sigma_[0]->ngrow), "set sigma");
+        }
+    }
     Context& ctx_;
     std::vector<MultiFab*> vel_;
     std::vector<Geometry> geom_;
-    std::vector<int> flat_;
+    std::vector<MultiFab const*> sigma_;
+    std::vector<int> flat_, flat1_;
     mgx_nodal h_ = nullptr;
+    mgx_amr_nodal ha_ = nullptr;  // two levels
     mgx_mlmg_opts opts_{};
     mgx_solve_info info_{};
     int mode_ = MGX_MODE_FP64;
+    bool projected_ = false;
     detail::DevField phi_, gphi_;
+    detail::DevField phis_[2], gphis_[2];
 };
 
-// Hydro::MacProjector (single level, no EB).
+// Hydro::MacProjector (no EB) over one level, or two (refinement ratio 2: the
+// composite projection of mgx_amr_mac_*, src/convection/
+// incflo_compute_MAC_projected_velocities.cpp:96-133, 272-302).
 class MacProjector {
    public:
     MacProjector(Context& ctx, std::vector<Geometry> geom) : ctx_(ctx), geom_(geom) {
-        detail::one_level(geom_.size(), "MacProjector");
+        if (geom_.size() != 1 && geom_.size() != 2)
+            Abort("MacProjector: one level, or two with refinement ratio 2");
     }
     explicit MacProjector(std::vector<Geometry> geom) : MacProjector(Context::current(), geom) {}
     ~MacProjector() {
         if (h_) mgx_mac_destroy(h_);
+        if (ha_) mgx_amr_mac_destroy(ha_);
     }
     MacProjector(const MacProjector&) = delete;
     MacProjector& operator=(const MacProjector&) = delete;
 
-    bool needInitialization() const { return h_ == nullptr; }
+    bool needInitialization() const { return h_ == nullptr && ha_ == nullptr; }
+    // initProjector(lp_info, beta) (:112): the BoxArrays are the coefficients'
+    void initProjector(LPInfo info, std::vector<std::array<MultiFab const*, 3>> beta) {
+        if (beta.size() == 2 && geom_.size() == 2) {
+            create_amr(*beta[0][0]->ba, *beta[1][0]->ba, info);
+            updateCoeffs(beta);
+            return;
+        }
+        detail::one_level(beta.size(), "MacProjector");
+        initProjector(*beta[0][0]->ba, info, beta);
+    }
     // initProjector(lp_info, beta) with face coefficients
     void initProjector(const BoxArray& ba, LPInfo info,
                        std::vector<std::array<MultiFab const*, 3>> beta) {
@@ -264,35 +344,68 @@ class MacProjector {
             l[k] = (int)lo[k];
             r[k] = (int)hi[k];
         }
+        if (ha_) {
+            // the two-level handle re-sets up its levels: hand beta over again
+            ctx_.check(mgx_amr_mac_set_domain_bc(ha_, l, r), "setDomainBC");
+            if (!beta_.empty()) updateCoeffs(beta_);
+            return;
+        }
         ctx_.check(mgx_mac_set_domain_bc(need(), l, r), "setDomainBC");
     }
     // setLevelBC(lev, nullptr, &robin_a, &robin_b, &robin_f): Robin (mixed) face data
     void setLevelBC(int lev, const MultiFab* levelbcdata, const MultiFab* robin_a,
                     const MultiFab* robin_b, const MultiFab* robin_f) {
-        if (lev != 0 || levelbcdata) Abort("MacProjector::setLevelBC: level 0, no Dirichlet data");
+        if (lev != 0 || levelbcdata || ha_)
+            Abort("MacProjector::setLevelBC: one level, level 0, no Dirichlet data");
         ctx_.check(mgx_mac_set_level_bc(need(), robin_a->p, robin_b->p, robin_f->p,
                                         robin_a->ngrow),
                    "setLevelBC");
     }
-    void updateBeta(double beta) { ctx_.check(mgx_mac_set_const_beta(need(), beta), "updateBeta"); }
+    void updateBeta(double beta) {
+        if (ha_) Abort("MacProjector::updateBeta: two levels need beta fields");
+        ctx_.check(mgx_mac_set_const_beta(need(), beta), "updateBeta");
+    }
     void updateCoeffs(std::vector<std::array<MultiFab const*, 3>> beta) {
+        if (ha_) {
+            if (beta.size() != 2 || beta[0][0]->ngrow != beta[1][0]->ngrow)
+                Abort("MacProjector: two levels need beta fields of equal ghost width");
+            beta_ = beta;
+            const double* b[6] = {beta[0][0]->p, beta[0][1]->p, beta[0][2]->p,
+                                  beta[1][0]->p, beta[1][1]->p, beta[1][2]->p};
+            ctx_.check(mgx_amr_mac_set_beta(ha_, b, beta[0][0]->ngrow), "updateCoeffs");
+            return;
+        }
         detail::one_level(beta.size(), "MacProjector");
         ctx_.check(mgx_mac_set_beta(need(), beta[0][0]->p, beta[0][1]->p, beta[0][2]->p,
                                     beta[0][0]->ngrow),
                    "updateCoeffs");
     }
     void setUMAC(std::vector<std::array<MultiFab*, 3>> umac) {
+        if (ha_) {
+            if (umac.size() != 2) Abort("MacProjector::setUMAC: a face velocity per level");
+            umac_ = umac[0];
+            umac1_ = umac[1];
+            return;
+        }
         detail::one_level(umac.size(), "MacProjector");
         umac_ = umac[0];
     }
     void setSolveMode(int mode) { mode_ = mode; }
     void project(double rtol, double atol) { run(nullptr, 0, rtol, atol); }
     void project(std::vector<MultiFab*> phi, double rtol, double atol) {
+        if (ha_) {
+            if (phi.size() != 2 || phi[0]->ngrow != phi[1]->ngrow)
+                Abort("MacProjector::project: phi per level, equal ghost widths");
+            double* ph[2] = {phi[0]->p, phi[1]->p};
+            run_amr(ph, phi[0]->ngrow, rtol, atol);
+            return;
+        }
         detail::one_level(phi.size(), "MacProjector");
         run(phi[0]->p, phi[0]->ngrow, rtol, atol);
     }
     // fluxes -beta grad(phi) of the last solve (MLMG::Location::FaceCenter)
     void getFluxes(std::vector<std::array<MultiFab*, 3>> flux) {
+        if (ha_) Abort("MacProjector::getFluxes: one level only");
         detail::one_level(flux.size(), "MacProjector");
         ctx_.check(mgx_mac_get_fluxes(need(), flux[0][0]->p, flux[0][1]->p, flux[0][2]->p,
                                       flux[0][0]->ngrow),
@@ -315,11 +428,35 @@ class MacProjector {
         o.max_coarsening_level = info.max_coarsening_level;
         ctx_.check(mgx_mac_set_options(h_, &o), "set options");
     }
+    void create_amr(const BoxArray& ba0, const BoxArray& ba1, const LPInfo& info) {
+        if (ha_) mgx_amr_mac_destroy(ha_);
+        flat_ = ba0.flat();
+        flat1_ = ba1.flat();
+        mgx_level_desc d[2] = {detail::level_desc(geom_[0], flat_), detail::level_desc(geom_[1], flat1_)};
+        ctx_.check(mgx_amr_mac_create(ctx_.get(), d, 2, &ha_), "MacProjector");
+        mgx_mlmg_opts o;
+        mgx_default_mlmg_opts(&o);
+        o.maxiter = 200;
+        o.bottom_maxiter = 200;
+        o.max_coarsening_level = info.max_coarsening_level;
+        ctx_.check(mgx_amr_mac_set_options(ha_, &o), "set options");
+    }
+    void run_amr(double* const* phi, int phi_ng, double rtol, double atol) {
+        if (!umac_[0] || !umac1_[0]) Abort("MacProjector::project: setUMAC has not been called");
+        if (umac_[0]->ngrow != umac1_[0]->ngrow) Abort("MacProjector: face velocity ghost widths differ");
+        double* u[6] = {umac_[0]->p, umac_[1]->p, umac_[2]->p, umac1_[0]->p, umac1_[1]->p, umac1_[2]->p};
+        ctx_.check(mgx_amr_mac_project(ha_, u, umac_[0]->ngrow, phi, phi_ng, rtol, atol, mode_, &info_),
+                   "MacProjector::project");
+    }
     mgx_mac need() const {
         if (!h_) Abort("MacProjector: initProjector has not been called");
         return h_;
     }
     void run(double* phi, int phi_ng, double rtol, double atol) {
+        if (ha_) {
+            run_amr(nullptr, 0, rtol, atol);
+            return;
+        }
         if (!umac_[0]) Abort("MacProjector::project: setUMAC has not been called");
         ctx_.check(mgx_mac_project(need(), umac_[0]->p, umac_[1]->p, umac_[2]->p, umac_[0]->ngrow,
                                    phi, phi_ng, rtol, atol, mode_, &info_),
@@ -327,9 +464,12 @@ class MacProjector {
     }
     Context& ctx_;
     std::vector<Geometry> geom_;
-    std::vector<int> flat_;
+    std::vector<int> flat_, flat1_;
     mgx_mac h_ = nullptr;
+    mgx_amr_mac ha_ = nullptr;  // two levels
+    std::vector<std::array<MultiFab const*, 3>> beta_;
     std::array<MultiFab*, 3> umac_{nullptr, nullptr, nullptr};
+    std::array<MultiFab*, 3> umac1_{nullptr, nullptr, nullptr};
     mgx_solve_info info_{};
     int mode_ = MGX_MODE_FP64;
 };

@@ -135,6 +135,77 @@ int main(int argc, char** argv) {
     write_file(dir + "/umac_out1.bin", u1.get());
     write_file(dir + "/umac_out2.bin", u2.get());
     write_file(dir + "/macphi.bin", macphi.get());
+
+    // --- two levels (refinement ratio 2), when the test wrote a fine box list:
+    // NodalProjector / MacProjector over Vector<MultiFab*> of both levels ---
+    if (FILE* probe = std::fopen((dir + "/fboxes.bin").c_str(), "rb")) {
+        std::fclose(probe);
+        const std::vector<double> fb = read_file(dir + "/fboxes.bin");
+        mgx::BoxArray fba;
+        for (size_t q = 0; q + 5 < fb.size(); q += 6)
+            fba.boxes.push_back({{(int)fb[q], (int)fb[q + 1], (int)fb[q + 2]},
+                                 {(int)fb[q + 3], (int)fb[q + 4], (int)fb[q + 5]}});
+        mgx::Geometry fgeom;
+        fgeom.n_cell = {2 * n, 2 * n, 2 * n};
+        auto count = [](const mgx::BoxArray& b, int extra) {
+            long long c = 0;
+            for (const mgx::Box& x : b.boxes)
+                c += (long long)(x.hi[0] - x.lo[0] + 1 + extra) * (x.hi[1] - x.lo[1] + 1 + extra) *
+                     (x.hi[2] - x.lo[2] + 1 + extra);
+            return c;
+        };
+        using BC = mgx::LinOpBCType;
+        Dev vc(ctx, read_file(dir + "/amr_velc.bin")), vf(ctx, read_file(dir + "/amr_velf.bin"));
+        Dev sc(ctx, read_file(dir + "/amr_sigc.bin")), sf(ctx, read_file(dir + "/amr_sigf.bin"));
+        mgx::MultiFab vcm{(double*)vc.p, &ba, 3, 0}, vfm{(double*)vf.p, &fba, 3, 0};
+        mgx::MultiFab scm{(double*)sc.p, &ba, 1, 0}, sfm{(double*)sf.p, &fba, 1, 0};
+        {
+            mgx::NodalProjector np({&vcm, &vfm}, {&scm, &sfm}, {geom, fgeom}, info);
+            np.setDomainBC({BC::Periodic, BC::Periodic, BC::Periodic},
+                           {BC::Periodic, BC::Periodic, BC::Periodic});
+            np.project(1e-12, 0.0);
+            std::printf("amr nodal: %d iterations, rel resid %.3e\n", np.info().iters,
+                        np.info().rel_resid);
+            write_file(dir + "/amr_phic.bin", get_view(ctx, np.getPhi()[0], count(ba, 1)));
+            write_file(dir + "/amr_phif.bin", get_view(ctx, np.getPhi()[1], count(fba, 1)));
+            write_file(dir + "/amr_gphic.bin", get_view(ctx, np.getGradPhi()[0], 3 * count(ba, 0)));
+            write_file(dir + "/amr_gphif.bin", get_view(ctx, np.getGradPhi()[1], 3 * count(fba, 0)));
+        }
+        write_file(dir + "/amr_velc_out.bin", vc.get());
+        write_file(dir + "/amr_velf_out.bin", vf.get());
+        std::vector<Dev*> keep;
+        mgx::MultiFab bm[2][3], um[2][3];
+        for (int l = 0; l < 2; ++l)
+            for (int d = 0; d < 3; ++d) {
+                const char* lv = l ? "f" : "c";
+                Dev* b = new Dev(ctx, read_file(dir + "/amr_beta" + lv + std::to_string(d) + ".bin"));
+                Dev* u = new Dev(ctx, read_file(dir + "/amr_umac" + lv + std::to_string(d) + ".bin"));
+                keep.push_back(b);
+                keep.push_back(u);
+                bm[l][d] = {(double*)b->p, l ? &fba : &ba, 1, 0};
+                um[l][d] = {(double*)u->p, l ? &fba : &ba, 1, 0};
+            }
+        Dev mpc(ctx, (size_t)count(ba, 0)), mpf(ctx, (size_t)count(fba, 0));
+        mgx::MultiFab mpcm{(double*)mpc.p, &ba, 1, 0}, mpfm{(double*)mpf.p, &fba, 1, 0};
+        {
+            mgx::MacProjector mp({geom, fgeom});
+            mp.initProjector(info, {{&bm[0][0], &bm[0][1], &bm[0][2]},
+                                    {&bm[1][0], &bm[1][1], &bm[1][2]}});
+            mp.setDomainBC({BC::Periodic, BC::Periodic, BC::Periodic},
+                           {BC::Periodic, BC::Periodic, BC::Periodic});
+            mp.setUMAC({{&um[0][0], &um[0][1], &um[0][2]}, {&um[1][0], &um[1][1], &um[1][2]}});
+            mp.project({&mpcm, &mpfm}, 1e-12, 0.0);
+            std::printf("amr mac: %d iterations, rel resid %.3e\n", mp.info().iters,
+                        mp.info().rel_resid);
+        }
+        for (int l = 0; l < 2; ++l)
+            for (int d = 0; d < 3; ++d)
+                write_file(dir + "/amr_umac" + (l ? "f" : "c") + std::to_string(d) + "_out.bin",
+                           keep[2 * (3 * l + d) + 1]->get());
+        write_file(dir + "/amr_macphic.bin", mpc.get());
+        write_file(dir + "/amr_macphif.bin", mpf.get());
+        for (Dev* q : keep) delete q;
+    }
     std::printf("facade_check ok\n");
     return 0;
 }
