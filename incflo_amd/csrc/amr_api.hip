@@ -91,6 +91,9 @@ class AmrNodal {
     }
 
     void setup(const NodalGeom& g0, const std::vector<int>& fboxes, const MGOpts& o) {
+        // the boxes and the periodicity are fixed per handle, so a set-up for
+        // new boundary types or options keeps the coefficient arrays
+        const bool keep = sigma_set;
         gc = g0;
         gf = refine_geom(g0);
         opts = o;
@@ -138,9 +141,11 @@ class AmrNodal {
         sigf.alloc(gf.nsig() * sizeof(double));
         sigf32.alloc(gf.nsig() * sizeof(float));
         // ghost cells outside non-periodic faces stay 0 (no element there)
-        MGX_HIP(hipMemset(sigc.p, 0, gc.nsig() * sizeof(double)));
-        MGX_HIP(hipMemset(sigc_out.p, 0, gc.nsig() * sizeof(double)));
-        MGX_HIP(hipMemset(sigf.p, 0, gf.nsig() * sizeof(double)));
+        if (!keep) {
+            MGX_HIP(hipMemset(sigc.p, 0, gc.nsig() * sizeof(double)));
+            MGX_HIP(hipMemset(sigc_out.p, 0, gc.nsig() * sizeof(double)));
+            MGX_HIP(hipMemset(sigf.p, 0, gf.nsig() * sizeof(double)));
+        }
         for (DevBuf* b : {&xc, &bc, &rtmp, &rcT}) b->alloc(nnc * sizeof(double));
         for (DevBuf* b : {&xf, &bf, &yf, &ef_a, &ef_b, &rfT, &yT}) b->alloc(nnf * sizeof(double));
         bC.alloc(nnc * sizeof(double));
@@ -150,7 +155,8 @@ class AmrNodal {
         if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
         H64.built = false;
         H32.built = false;
-        sigma_set = false;
+        sigma_set = keep;
+        h64_sigma = h32_sigma = false;
     }
 
     // sigc / sigf (ghosted FP64) are filled by the caller

@@ -380,6 +380,9 @@ class AmrMac {
     }
 
     void setup(const MacGeom& g0, const std::vector<int>& fboxes, const MGOpts& o) {
+        // the boxes and the periodicity are fixed per handle, so a set-up for
+        // new boundary types or options keeps the coefficient arrays
+        const bool keep = beta_set;
         gc = g0;
         gf = g0;
         for (int d = 0; d < 3; ++d) {
@@ -412,7 +415,7 @@ class AmrMac {
             betac[d].alloc(gc.nface(d) * sizeof(double));
             betaf[d].alloc(gf.nface(d) * sizeof(double));
             betaf32[d].alloc(gf.nface(d) * sizeof(float));
-            MGX_HIP(hipMemset(betaf[d].p, 0, gf.nface(d) * sizeof(double)));
+            if (!keep) MGX_HIP(hipMemset(betaf[d].p, 0, gf.nface(d) * sizeof(double)));
         }
         for (DevBuf* b : {&xc, &bc, &rc, &rc2}) b->alloc(gc.ncell() * sizeof(double));
         for (DevBuf* b : {&xf, &bf, &rf, &rf2, &ef}) b->alloc(gf.ncell() * sizeof(double));
@@ -421,7 +424,8 @@ class AmrMac {
         ibuf.alloc(16 * sizeof(int));
         if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
         H64.built = H32.built = false;
-        beta_set = false;
+        beta_set = keep;
+        h64_beta = h32_beta = false;
     }
 
     // betac / betaf (FP64 canonical faces) were filled by the caller

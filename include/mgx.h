@@ -257,8 +257,8 @@ typedef struct mgx_amr_s* mgx_amr_nodal;
 MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int nlevels,
                                  mgx_amr_nodal* out);
 MGX_API int mgx_amr_nodal_destroy(mgx_amr_nodal h);
-/* setDomainBC (:194), as mgx_nodal_set_domain_bc.  This and set_options set
- * the levels up again: hand sigma over (again) after them. */
+/* setDomainBC (:194), as mgx_nodal_set_domain_bc (sigma handed over before
+ * is kept). */
 MGX_API int mgx_amr_nodal_set_domain_bc(mgx_amr_nodal h, const int lo[3], const int hi[3]);
 MGX_API int mgx_amr_nodal_set_options(mgx_amr_nodal h, const mgx_mlmg_opts* opts);
 /* sigma per level (:101-121): sigma[lev] is a boxed cell field of level lev (ng ghosts). */
@@ -288,8 +288,8 @@ typedef struct mgx_amr_mac_s* mgx_amr_mac;
 MGX_API int mgx_amr_mac_create(mgx_ctx ctx, const mgx_level_desc* levels, int nlevels,
                                mgx_amr_mac* out);
 MGX_API int mgx_amr_mac_destroy(mgx_amr_mac h);
-/* setDomainBC (:114), as mgx_mac_set_domain_bc without ROBIN.  This and
- * set_options set the levels up again: hand beta over (again) after them. */
+/* setDomainBC (:114), as mgx_mac_set_domain_bc without ROBIN (beta handed
+ * over before is kept). */
 MGX_API int mgx_amr_mac_set_domain_bc(mgx_amr_mac h, const int lo[3], const int hi[3]);
 /* mac_proj.* knobs (maxiter / bottom_maxiter default 200). */
 MGX_API int mgx_amr_mac_set_options(mgx_amr_mac h, const mgx_mlmg_opts* opts);

@@ -209,3 +209,62 @@ def test_average_down_faces_and_cells_match_oracle(ctx):
     for q in range(3):
         ref = A.average_down_cells(F, uf[q], uc[q])
         assert np.max(np.abs(got[q] - ref)) <= 1e-15 * np.max(np.abs(ref))
+
+
+def test_amr_coefficients_survive_resetup(ctx, orc):
+    """set_domain_bc / set_options after the coefficients (incflo's order) set
+    the levels up again and keep sigma / beta: the result equals the one
+    without the extra call."""
+    from incflo_amd import mgx
+    _say("case amr coefficients kept across set-up")
+    nc = (16, 16, 16)
+    fboxes = [(b[:3], b[3:]) for b in FINE]
+    A, sig_c, sig_f, uc, uf = _case(orc, nc, FINE, seed=4)
+    cboxes = B.chop_domain(nc, 8)
+    lev_c = mgx.make_level(nc, cboxes)
+    lev_f = mgx.make_level(tuple(2 * v for v in nc), fboxes)
+    outs = []
+    for extra in (False, True):
+        P = mgx.AmrNodalProjector(ctx, [lev_c, lev_f])
+        P.set_sigma([ctx.array(B.to_boxed(sig_c, cboxes, 0)), ctx.array(B.to_boxed(sig_f, fboxes, 0))], 0)
+        if extra:
+            P.set_options(mgx.default_opts())
+            P.setDomainBC((0, 0, 0), (0, 0, 0))
+        v = [ctx.array(B.to_boxed(uc, cboxes, 0)), ctx.array(B.to_boxed(uf, fboxes, 0))]
+        info = P.project(v, 0, 1e-12, 0.0)
+        assert info.converged
+        outs.append([a.download() for a in v])
+        P.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    # MAC: beta kept across set_options
+    rng = np.random.default_rng(9)
+    gc = orc.Geom.make(nc)
+    gf = orc.Geom.make(tuple(2 * v for v in nc))
+    bc = [0.5 + rng.random(orc.face_shape(gc, d)) for d in range(3)]
+    bf = [0.5 + rng.random(orc.face_shape(gf, d)) for d in range(3)]
+    for d in range(3):
+        ax = 2 - d
+        for b in (bc[d], bf[d]):
+            sl0 = [slice(None)] * 3
+            sln = [slice(None)] * 3
+            sl0[ax] = 0
+            sln[ax] = -1
+            b[tuple(sln)] = b[tuple(sl0)]
+    u0 = [[rng.standard_normal(orc.face_shape(gc, d)) for d in range(3)],
+          [rng.standard_normal(orc.face_shape(gf, d)) for d in range(3)]]
+    outs = []
+    for extra in (False, True):
+        M = mgx.AmrMacProjector(ctx, [lev_c, lev_f])
+        M.initProjector([[ctx.array(B.faces_to_boxed(bc[d], cboxes, 0, d)) for d in range(3)],
+                         [ctx.array(B.faces_to_boxed(bf[d], fboxes, 0, d)) for d in range(3)]], 0)
+        if extra:
+            M.set_options(mgx.default_opts(maxiter=200, bottom_maxiter=200))
+        ud = [[ctx.array(B.faces_to_boxed(u0[0][d], cboxes, 0, d)) for d in range(3)],
+              [ctx.array(B.faces_to_boxed(u0[1][d], fboxes, 0, d)) for d in range(3)]]
+        info = M.project(ud, 0, 1e-12)
+        assert info.converged
+        outs.append([a.download() for lev in ud for a in lev])
+        M.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
