@@ -114,7 +114,7 @@ def test_facade_two_levels_matches_direct_abi(ctx, tmp_path):
     bxs = B.chop_domain((n, n, n), mgs)
     fboxes = [((8, 8, 4), (19, 15, 11)), ((20, 0, 20), (27, 7, 31))]
     nf = 2 * n
-    # single-level inputs the driver's first part reads
+    # single-level inputs the first part of facade_check reads
     uc1 = rng.standard_normal((3, n, n, n))
     B.to_boxed(uc1, bxs, 1).tofile(tmp_path / "vel.bin")
     B.to_boxed(1.0 + rng.random((n, n, n)), bxs, 0).tofile(tmp_path / "sigma.bin")
