@@ -51,7 +51,7 @@ def spectral_velocity(n, seed):
     c[0] -= kd * KX
     c[1] -= kd * KY
     c[2] -= kd * KZ
-    u = np.stack([np.fft.irfftn(ci, s=(n, n, n)) for ci in c])
+    u = np.stack([np.fft.irfftn(ci, s=(n, n, n), axes=(0, 1, 2)) for ci in c])
     u /= np.max(np.sqrt(np.sum(u * u, axis=0)))
     return u
 
