@@ -767,7 +767,9 @@ void NodalSolver::learned_correction(hipStream_t s) {
         zero_kernel<float><<<grid1d(f.g.nnode()), 256, 0, s>>>(f.x, f.g.nnode());
         nodal_prolong_add_kernel<float><<<grid3(f.g.nn[0], f.g.nn[1], f.g.nn[2]), block3(), 0, s>>>(
             f.kg, H32.L[l + 1].kg, H32.L[l + 1].x, f.x);
-        H32.smooth(l, opts.nu2, false, opts, s);
+        // not level 0: the first refinement step's V-cycles pre-smooth from
+        // this guess there
+        H32.smooth(l, l == 0 ? 0 : opts.nu2, false, opts, s);
     }
 }
 
