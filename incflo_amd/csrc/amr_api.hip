@@ -345,8 +345,8 @@ class AmrNodal {
         const double r0 = mixed ? residual<float>(s) : residual<double>(s);
         double rn = r0;
         int it = 0;
-        bool conv = rn <= tol;
-        while (!conv && it < opts.maxiter) {
+        bool conv = std::isfinite(bnorm) && rn <= tol;
+        while (!conv && it < opts.maxiter && std::isfinite(rn)) {
             ++it;
             if (mixed) {
                 vcycle<float>(H32, sigf32.as<float>(), sigc_out32.as<float>(), s);
@@ -505,14 +505,9 @@ MGX_API int mgx_amr_nodal_set_domain_bc(mgx_amr_nodal h, const int lo[3], const 
 MGX_API int mgx_amr_nodal_set_options(mgx_amr_nodal h, const mgx_mlmg_opts* o) {
     if (!h || !o) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {
-        h->opts.maxiter = o->maxiter;
-        h->opts.bottom_maxiter = o->bottom_maxiter;
-        h->opts.bottom_rtol = o->bottom_rtol;
-        h->opts.max_coarsening_level = o->max_coarsening_level;
-        h->opts.nu1 = o->nu1;
-        h->opts.nu2 = o->nu2;
-        h->opts.omega = o->omega;
-        h->opts.verbose = o->verbose;
+        MGOpts m;
+        to_opts(o, m);
+        h->opts = m;
         h->dirty = true;
     });
 }

@@ -132,7 +132,7 @@ static __global__ void __launch_bounds__(256) amr_coarse_combine_kernel(
             r = base[id] + sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K);
         out[id] = (TO)r;
     }
-    if (norm) wave_max_atomic(norm, fabs(r));
+    if (norm) wave_max_atomic(norm, nabs(r));
 }
 
 // fine unknowns: r = b - y (cast to TO), 0 elsewhere; optional max |r|
@@ -148,7 +148,7 @@ static __global__ void __launch_bounds__(256) amr_fine_resid_kernel(
         if (ft[id] == AMR_DOF) r = b[id] - (y ? y[id] : 0.0);
         out[id] = (TO)r;
     }
-    if (norm) wave_max_atomic(norm, fabs(r));
+    if (norm) wave_max_atomic(norm, nabs(r));
 }
 
 // right-hand side of the coarse-grid correction (all coarse nodes):

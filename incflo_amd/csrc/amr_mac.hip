@@ -97,7 +97,7 @@ static __global__ void amac_fine_kernel(MGeomK gf, MGeomK gc, AmacCoef<T> c,
             } else {
                 const T rv = b[id] - acc;
                 r[id] = (TO)rv;
-                m = fabs((double)rv);
+                m = nabs((double)rv);
             }
         } else if (MODE == AM_RESID) {
             r[id] = (TO)0;
@@ -189,7 +189,7 @@ static __global__ void amac_coarse_kernel(MGeomK gc, MGeomK gf, AmacCoef<T> c,
             } else {
                 const T rv = b[id] - acc;
                 r[id] = (TO)rv;
-                m = fabs((double)rv);
+                m = nabs((double)rv);
             }
         }
     }
@@ -572,10 +572,10 @@ class AmrMac {
         const double r0 = mixed ? residual<float>(s) : bnorm;
         double rn = r0;
         int it = 0;
-        bool conv = rn <= tol;
+        bool conv = std::isfinite(bnorm) && rn <= tol;
         const float* bf32[3] = {betaf32[0].as<float>(), betaf32[1].as<float>(), betaf32[2].as<float>()};
         const double* bf64[3] = {betaf[0].as<double>(), betaf[1].as<double>(), betaf[2].as<double>()};
-        while (!conv && it < opts.maxiter) {
+        while (!conv && it < opts.maxiter && std::isfinite(rn)) {
             ++it;
             if (mixed) {
                 vcycle<float>(H32, bf32, s);
@@ -780,13 +780,9 @@ MGX_API int mgx_amr_mac_set_domain_bc(mgx_amr_mac h, const int lo[3], const int 
 MGX_API int mgx_amr_mac_set_options(mgx_amr_mac h, const mgx_mlmg_opts* o) {
     if (!h || !o) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {
-        h->opts.maxiter = o->maxiter;
-        h->opts.bottom_maxiter = o->bottom_maxiter;
-        h->opts.bottom_rtol = o->bottom_rtol;
-        h->opts.max_coarsening_level = o->max_coarsening_level;
-        h->opts.nu1 = o->nu1;
-        h->opts.nu2 = o->nu2;
-        h->opts.verbose = o->verbose;
+        MGOpts m;
+        to_opts(o, m);
+        h->opts = m;
         h->dirty = true;
     });
 }

@@ -120,7 +120,7 @@ static __global__ void __launch_bounds__(256) eb_stencil_kernel(EBArgs a, int mo
     }
     if ((mode == EB_RESID || mode == EB_UPDRES) && a.norm) {
         // one atomic per wave; every lane stays active through the shuffles
-        double v = fabs(r);
+        double v = nabs(r);
         for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
         if ((threadIdx.x & 63) == 0) atomic_max_nonneg(a.norm, v);
     }

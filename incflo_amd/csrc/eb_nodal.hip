@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(1024) eb_bottom_kernel(EBLev L, const double* 
         x[id] = 0.0;
         p[id] = 0.0;
         v[id] = 0.0;
-        rn = fmax(rn, fabs(r[id]));
+        rn = fmax(rn, nabs(r[id]));
     }
     const double r0 = bmax(rn, red);
     double rho = 1.0, alpha = 1.0, omega = 1.0;
@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(1024) eb_bottom_kernel(EBLev L, const double* 
             for (long id = tid; id < n; id += nt) {
                 x[id] += alpha * p[id];
                 sv[id] = r[id] - alpha * v[id];
-                sn = fmax(sn, fabs(sv[id]));
+                sn = fmax(sn, nabs(sv[id]));
             }
             sn = bmax(sn, red);
             if (sn <= rtol * r0) break;
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(1024) eb_bottom_kernel(EBLev L, const double* 
             for (long id = tid; id < n; id += nt) {
                 x[id] += omega * sv[id];
                 r[id] = sv[id] - omega * t[id];
-                rr = fmax(rr, fabs(r[id]));
+                rr = fmax(rr, nabs(r[id]));
             }
             rr = bmax(rr, red);
             if (rr <= rtol * r0 || omega == 0.0) break;
@@ -660,8 +660,8 @@ void EBNodal::solve(double* b, double* phi_a, double* phi_b, double*& phi, doubl
     const double tol = std::max(atol, rtol * bnorm);
     double rn = r0;
     int it = 0;
-    bool conv = rn <= tol;
-    while (!conv && it < o.maxiter) {
+    bool conv = std::isfinite(bnorm) && rn <= tol;
+    while (!conv && it < o.maxiter && std::isfinite(rn)) {
         ++it;
         vcycle(o, s);
         double* pout = (phi == phi_a) ? phi_b : phi_a;

@@ -253,6 +253,8 @@ MGX_API int mgx_mac_set_domain_bc(mgx_mac h, const int lo[3], const int hi[3]) {
         for (int d = 0; d < 3; ++d)
             for (int side = 0; side < 2; ++side) {
                 const int bc = side ? hi[d] : lo[d];
+                MGX_REQUIRE(bc != MGX_BC_INOUT, MGX_ERR_UNSUPPORTED,
+                            "MAC projector: direction-dependent (in/out) faces need the outflow rescaling of HydroUtils::enforceInOutSolvability, which this build does not implement");
                 MGX_REQUIRE(bc >= MGX_BC_PERIODIC && bc <= MGX_BC_ROBIN, MGX_ERR_ARG,
                             "MAC projector: undefined BC type");
                 MGX_REQUIRE((bc == MGX_BC_PERIODIC) == (h->periodic[d] != 0), MGX_ERR_ARG,
@@ -271,17 +273,10 @@ MGX_API int mgx_mac_set_domain_bc(mgx_mac h, const int lo[3], const int hi[3]) {
 MGX_API int mgx_mac_set_options(mgx_mac h, const mgx_mlmg_opts* o) {
     if (!h) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {
-        MGX_REQUIRE(o != nullptr && o->maxiter >= 1 && o->bottom_maxiter >= 1 && o->nu1 >= 0 &&
-                        o->nu2 >= 0 && o->nu1 + o->nu2 >= 1,
-                    MGX_ERR_ARG, "bad MLMG options");
-        const bool relevel = o->max_coarsening_level != h->opts.max_coarsening_level;
-        h->opts.verbose = o->verbose;
-        h->opts.maxiter = o->maxiter;
-        h->opts.bottom_maxiter = o->bottom_maxiter;
-        h->opts.bottom_rtol = o->bottom_rtol;
-        h->opts.max_coarsening_level = std::max(0, o->max_coarsening_level);
-        h->opts.nu1 = o->nu1;
-        h->opts.nu2 = o->nu2;
+        MGOpts m;
+        to_opts(o, m);
+        const bool relevel = m.max_coarsening_level != h->opts.max_coarsening_level;
+        h->opts = m;
         if (relevel) h->dirty = true;
         else h->solver.opts = h->opts;
     });

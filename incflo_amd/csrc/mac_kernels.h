@@ -125,7 +125,7 @@ static __global__ void mac_resid_kernel(MGeomK g, MCoef<T> c, const T* __restric
         } else {
             const T rv = b[id] - acc;
             r[id] = (TC)rv;
-            m = fabs((double)rv);
+            m = nabs((double)rv);
         }
     }
     if (MODE != MK_APPLY && norm) {
@@ -287,7 +287,7 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
         x[id] = 0.0; p[id] = 0.0; v[id] = 0.0;
     }
     double rn = 0.0;
-    for (long id = tid; id < n; id += nt) rn = fmax(rn, fabs(r[id]));
+    for (long id = tid; id < n; id += nt) rn = fmax(rn, nabs(r[id]));
     const double rnorm0 = bmax(rn, red);
     double rho = 1.0, alpha = 1.0, omega = 1.0;
     int it = 0;
@@ -309,7 +309,7 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
             for (long id = tid; id < n; id += nt) {
                 x[id] += alpha * p[id];
                 s[id] = r[id] - alpha * v[id];
-                sn = fmax(sn, fabs(s[id]));
+                sn = fmax(sn, nabs(s[id]));
             }
             sn = bmax(sn, red);
             if (sn <= rtol * rnorm0) break;
@@ -324,7 +324,7 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
             for (long id = tid; id < n; id += nt) {
                 x[id] += omega * s[id];
                 r[id] = s[id] - omega * t[id];
-                rr = fmax(rr, fabs(r[id]));
+                rr = fmax(rr, nabs(r[id]));
             }
             rr = bmax(rr, red);
             if (rr <= rtol * rnorm0) break;

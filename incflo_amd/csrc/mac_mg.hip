@@ -301,9 +301,9 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     double rn = r0;
     int it = 0;
     int vcycles = 0;
-    bool conv = (r0 <= tol);
+    bool conv = std::isfinite(bnorm) && r0 <= tol;
     int ncyc = opts.cycles_per_update;  // FP32 cycles of the next refinement step
-    while (!conv && it < opts.maxiter) {
+    while (!conv && it < opts.maxiter && std::isfinite(rn)) {
         ++it;
         zero_u64_kernel<<<1, 64, 0, s>>>(nrm, 1);
         double* pout = (phi == phi_a.as<double>()) ? phi_b.as<double>() : phi_a.as<double>();

@@ -137,23 +137,6 @@ static void check_mode(int mode) {
                 MGX_ERR_ARG, "unknown solve mode " + std::to_string(mode));
 }
 
-static void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
-    MGX_REQUIRE(o->maxiter >= 1 && o->bottom_maxiter >= 1, MGX_ERR_ARG, "maxiter must be >= 1");
-    MGX_REQUIRE(o->nu1 >= 0 && o->nu2 >= 0 && o->nu1 + o->nu2 >= 1, MGX_ERR_ARG,
-                "smoothing sweeps must be non-negative with nu1+nu2 >= 1");
-    MGX_REQUIRE(o->omega > 0.0 && o->omega < 2.0, MGX_ERR_ARG, "omega must lie in (0,2)");
-    m.verbose = o->verbose;
-    m.maxiter = o->maxiter;
-    m.bottom_maxiter = o->bottom_maxiter;
-    m.bottom_rtol = o->bottom_rtol;
-    m.max_coarsening_level = o->max_coarsening_level < 0 ? 0 : o->max_coarsening_level;
-    m.nu1 = o->nu1;
-    m.nu2 = o->nu2;
-    m.omega = o->omega;
-    m.stall_iters = o->stall_iters;
-    MGX_REQUIRE(o->cycles_per_update <= 8, MGX_ERR_ARG, "cycles_per_update must be at most 8");
-    m.cycles_per_update = o->cycles_per_update < 1 ? 1 : o->cycles_per_update;  // 0: the default
-}
 
 extern "C" {
 
@@ -319,6 +302,8 @@ MGX_API int mgx_nodal_set_domain_bc(mgx_nodal h, const int lo[3], const int hi[3
         for (int d = 0; d < 3; ++d) {
             for (int side = 0; side < 2; ++side) {
                 const int bc = side ? hi[d] : lo[d];
+                MGX_REQUIRE(bc != MGX_BC_INOUT, MGX_ERR_UNSUPPORTED,
+                            "nodal projector: direction-dependent (in/out) faces need the outflow rescaling of HydroUtils::enforceInOutSolvability, which this build does not implement");
                 MGX_REQUIRE(bc >= MGX_BC_PERIODIC && bc <= MGX_BC_ROBIN, MGX_ERR_ARG,
                             "nodal projector: undefined BC type");
                 MGX_REQUIRE((bc == MGX_BC_PERIODIC) == (h->periodic[d] != 0), MGX_ERR_ARG,

@@ -106,6 +106,13 @@ __device__ __forceinline__ bool grid_point(int nx, int ny, int& i, int& j, int& 
 inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + 63) / 64, (ny + 3) / 4, nz); }
 inline dim3 block3() { return dim3(64, 4, 1); }
 
+// |v| for max-norm reductions, with a NaN mapped to +inf: fmax drops NaN
+// operands, so without this a non-finite residual would count as 0 and a
+// diverged solve could pass its tolerance.
+__host__ __device__ __forceinline__ double nabs(double v) {
+    return v == v ? fabs(v) : __builtin_inf();
+}
+
 // atomic max for non-negative doubles via their bit pattern; the atomic is
 // issued only when v exceeds the stored value (with many blocks on one
 // address most skip it, where unconditional atomics serialise in L2)
@@ -460,7 +467,7 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
                                                    ? 0.0
                                                    : a.omega1 * x1s * (double)r /
                                                          (-(double)a.c.diag4 * (double)ssum)));
-                    rmax = fmax(rmax, (double)fabs(r));
+                    rmax = fmax(rmax, nabs((double)r));
                 }
             }
         }
@@ -884,8 +891,11 @@ static __global__ void __launch_bounds__(256) final_stats_kernel(const double* _
     if (threadIdx.x == 0) {
         const double m = singular ? rs[0] / (double)n : 0.0;
         *mean = m;
-        const double bn = fmax(rmx[0] - m, m - rmn[0]);
-        *norm = (unsigned long long)__double_as_longlong(fmax(bn, 0.0));
+        const double d1 = rmx[0] - m, d2 = m - rmn[0];
+        double bn = d1 > d2 ? d1 : d2;
+        // a non-finite entry makes the norm +inf (fmax would drop a NaN)
+        if (!(rs[0] == rs[0]) || !(d1 == d1) || !(d2 == d2)) bn = __builtin_inf();
+        *norm = (unsigned long long)__double_as_longlong(bn > 0.0 ? bn : 0.0);
     }
 }
 
@@ -901,7 +911,7 @@ static __global__ void __launch_bounds__(256) max_abs_kernel(const T* __restrict
     __shared__ double red[4];
     double m = 0.0;
     for (long id = (long)blockIdx.x * 256 + threadIdx.x; id < n; id += (long)gridDim.x * 256)
-        m = fmax(m, fabs((double)a[id]));
+        m = fmax(m, nabs((double)a[id]));
     m = block_max<256>(m, red);
     if (threadIdx.x == 0) atomic_max_nonneg(out, m);
 }
@@ -1027,7 +1037,7 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
         x[id] = 0.0; p[id] = 0.0; v[id] = 0.0;
     }
     double rn = 0.0;
-    for (long id = tid; id < n; id += nt) rn = fmax(rn, fabs(r[id]));
+    for (long id = tid; id < n; id += nt) rn = fmax(rn, nabs(r[id]));
     const double rnorm0 = bmax(rn, red);
     double rho = 1.0, alpha = 1.0, omega = 1.0;
     int it = 0;
@@ -1049,7 +1059,7 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
             for (long id = tid; id < n; id += nt) {
                 x[id] += alpha * p[id];
                 s[id] = r[id] - alpha * v[id];
-                sn = fmax(sn, fabs(s[id]));
+                sn = fmax(sn, nabs(s[id]));
             }
             sn = bmax(sn, red);
             if (sn <= rtol * rnorm0) break;
@@ -1064,7 +1074,7 @@ __device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, cons
             for (long id = tid; id < n; id += nt) {
                 x[id] += omega * s[id];
                 r[id] = s[id] - omega * t[id];
-                rr = fmax(rr, fabs(r[id]));
+                rr = fmax(rr, nabs(r[id]));
             }
             rr = bmax(rr, red);
             if (rr <= rtol * rnorm0) break;

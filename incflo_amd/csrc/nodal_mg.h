@@ -22,6 +22,29 @@ struct MGOpts {
     int cycles_per_update = 3;  // most FP32 V-cycles per FP64 refinement step (mixed modes)
 };
 
+// The C options struct -> MGOpts, validated the same way for every solver
+// handle.  cycles_per_update 0 keeps the built-in default.
+inline void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
+    MGX_REQUIRE(o != nullptr, MGX_ERR_ARG, "null MLMG options");
+    MGX_REQUIRE(o->maxiter >= 1 && o->bottom_maxiter >= 1, MGX_ERR_ARG, "maxiter must be >= 1");
+    MGX_REQUIRE(o->nu1 >= 0 && o->nu2 >= 0 && o->nu1 + o->nu2 >= 1, MGX_ERR_ARG,
+                "smoothing sweeps must be non-negative with nu1+nu2 >= 1");
+    MGX_REQUIRE(o->omega > 0.0 && o->omega < 2.0, MGX_ERR_ARG, "omega must lie in (0,2)");
+    MGX_REQUIRE(o->cycles_per_update >= 0 && o->cycles_per_update <= 8, MGX_ERR_ARG,
+                "cycles_per_update must lie in 0..8");
+    m.verbose = o->verbose;
+    m.maxiter = o->maxiter;
+    m.bottom_maxiter = o->bottom_maxiter;
+    m.bottom_rtol = o->bottom_rtol;
+    m.max_coarsening_level = o->max_coarsening_level < 0 ? 0 : o->max_coarsening_level;
+    m.nu1 = o->nu1;
+    m.nu2 = o->nu2;
+    m.omega = o->omega;
+    m.stall_iters = o->stall_iters;
+    m.cycles_per_update = o->cycles_per_update == 0 ? MGOpts{}.cycles_per_update
+                                                     : o->cycles_per_update;
+}
+
 inline KGeom kgeom(const NodalGeom& g) {
     KGeom k{};
     for (int d = 0; d < 3; ++d) {

@@ -953,7 +953,8 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     double rn = r0;
     int it = 0;
     int fallback = 0;
-    bool conv = (r0 <= tol);
+    // a non-finite norm (NaN mapped to +inf in the reductions) ends the solve unconverged
+    bool conv = std::isfinite(bnorm) && r0 <= tol;
     if (learned && !conv && opts.maxiter > 0 && !half) {
         // learned initial guess / coarse correction, judged in FP32 on the
         // correction equation: kept (as the first refinement step's initial
@@ -1000,7 +1001,7 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     int prec = vprec;
     int stall = 0;
     int ncyc = opts.cycles_per_update;  // FP32 cycles of the next refinement step
-    while (!conv && it < opts.maxiter) {
+    while (!conv && it < opts.maxiter && std::isfinite(rn)) {
         ++it;
         double* pout = other();
         int ran = 1;

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmgx.so")
 
 MGX_OK, MGX_ERR_ARG, MGX_ERR_HIP, MGX_ERR_NOT_CONVERGED, MGX_ERR_STATE, MGX_ERR_UNSUPPORTED = range(6)
-BC_PERIODIC, BC_DIRICHLET, BC_NEUMANN, BC_INFLOW, BC_ROBIN = range(5)
+BC_PERIODIC, BC_DIRICHLET, BC_NEUMANN, BC_INFLOW, BC_ROBIN, BC_INOUT = range(6)
 MODE_FP64, MODE_MIXED, MODE_LEARNED = 0, 1, 2
 
 

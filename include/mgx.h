@@ -48,7 +48,17 @@ enum {
     MGX_BC_DIRICHLET = 1,
     MGX_BC_NEUMANN = 2,
     MGX_BC_INFLOW = 3,
-    MGX_BC_ROBIN = 4
+    MGX_BC_ROBIN = 4,
+    /* incflo's direction_dependent ("dd") faces
+     * (src/boundary_conditions/boundary_conditions.cpp:82-87): the reference
+     * maps them to nodal inflow / MAC Neumann and, because such a face lets
+     * fluid both in and out, rescales the outflowing velocities with
+     * HydroUtils::enforceInOutSolvability before each projection
+     * (src/projection/incflo_apply_nodal_projection.cpp:166-178,
+     * src/convection/incflo_compute_MAC_projected_velocities.cpp:266-270).
+     * That rescaling is not implemented: set_domain_bc refuses this value
+     * with MGX_ERR_UNSUPPORTED rather than return a different velocity. */
+    MGX_BC_INOUT = 5
 };
 
 /* solve modes */
@@ -85,16 +95,19 @@ typedef struct {
     int nu1;                   /* pre-smoothing sweeps, default 2             */
     int nu2;                   /* post-smoothing sweeps, default 2            */
     double omega;              /* weighted-Jacobi factor, default 8/9         */
-    int stall_iters;           /* learned mode: fall back to plain MLMG when
-                                  the residual has not dropped 10x within this
-                                  many outer iterations (default 3)          */
-    int cycles_per_update;     /* nodal mixed / learned modes (no reference
+    int stall_iters;           /* learned mode: fall back to plain FP64 MLMG
+                                  after this many consecutive outer
+                                  iterations that each reduced the residual
+                                  by less than 10% (default 3; 0 disables)   */
+    int cycles_per_update;     /* mixed / learned modes (no reference
                                   counterpart): FP32 V-cycles per FP64
-                                  refinement step; each after the first starts
-                                  from the correction so far; the last step
-                                  runs only the cycles the measured
-                                  contraction says are left (default 3 for
-                                  the nodal solver; 0 means the default)     */
+                                  refinement step, 0..8; each after the first
+                                  starts from the correction so far; the last
+                                  step runs only the cycles the measured
+                                  contraction says are left.  Honoured by the
+                                  single-level nodal and MAC solvers (the
+                                  two-level AMR solvers run one composite
+                                  cycle per step).  0 means the default, 3.  */
 } mgx_mlmg_opts;
 
 typedef struct {
