@@ -3,6 +3,7 @@
 #include <memory>
 #include <vector>
 
+#include "cc_kernels.h"
 #include "eb_nodal.h"
 #include "mac_mg.h"
 #include "mgx_ctx.h"
@@ -32,6 +33,7 @@ struct mgx_mac_s {
     MacSolver solver;
     BoxLayout layout;
     DevBuf u[3], rhs, cells;
+    DevBuf ug;  // cell-centred projection: ghosted canonical velocity
     bool have_phi = false;
     // embedded boundary: apertures and covered cells
     bool eb_on = false;
@@ -397,6 +399,129 @@ MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, in
             h->faces_out(d, F[d], ng, h->u[d].as<double>());
         }
     });
+}
+
+MGX_API int mgx_apply_cc_projection(mgx_mac h, const double* density, int rho_ng,
+                                    double const_rho, double* vel, int vel_ng,
+                                    const double* vel_old, int vel_old_ng, double* gp, int gp_ng,
+                                    double* p_cc, int p_ng, double scaling_factor, int incremental,
+                                    const double* inflow_vel, double* umac, double* vmac,
+                                    double* wmac, int umac_ng, double rtol, double atol, int mode,
+                                    mgx_solve_info* info) {
+    if (!h) return MGX_ERR_ARG;
+    int status = guarded(h->ctx, [&] {
+        MGX_REQUIRE(vel && gp && p_cc && vel_ng >= 1 && gp_ng >= 0 && p_ng >= 0, MGX_ERR_ARG,
+                    "cc projection: velocity (>= 1 ghost), gp and p_cc are required");
+        MGX_REQUIRE(density != nullptr || const_rho > 0.0, MGX_ERR_ARG,
+                    "cc projection: need a density field or a positive constant density");
+        MGX_REQUIRE(scaling_factor > 0.0, MGX_ERR_ARG, "cc projection: scaling factor must be positive");
+        MGX_REQUIRE(!incremental || vel_old, MGX_ERR_ARG, "cc projection: incremental needs vel_old");
+        MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_ARG,
+                    "cc projection modes: FP64 or mixed");
+        MGX_REQUIRE(!h->eb_on, MGX_ERR_UNSUPPORTED,
+                    "cc projection with embedded boundaries (the reference aborts there too, "
+                    "src/projection/incflo_apply_cc_projection.cpp:377)");
+        h->ensure_setup();
+        MGX_REQUIRE(h->rob_set || !h->any_robin(), MGX_ERR_STATE,
+                    "cc projection: Robin faces need setLevelBC coefficients");
+        const KGeom kg = h->kg();
+        const MacGeom& g = h->g;
+        h->ug.alloc(3 * (size_t)kg.sg[0] * kg.sg[1] * kg.sg[2] * sizeof(double));
+        // u' = u (+ s/rho gp) (- u_old), ghosts 0 / inflow (:209-250)
+        GatherVelArgs a{};
+        a.g = kg;
+        a.boxes = h->layout.dev_boxes();
+        a.tiles = h->layout.tiles(REG_CELL_GROWN);
+        a.vel = vel;
+        a.vel_ng = vel_ng;
+        a.vel_old = incremental ? vel_old : nullptr;
+        a.vold_ng = vel_old_ng;
+        a.gp = incremental ? nullptr : gp;
+        a.gp_ng = gp_ng;
+        a.rho = density;
+        a.rho_ng = rho_ng;
+        a.const_rho = density ? 0.0 : const_rho;
+        a.scale = scaling_factor;
+        a.sigma_from_rho = 0;
+        a.ug = h->ug.as<double>();
+        a.ghost_rule = 1;
+        for (int d = 0; d < 3; ++d) { a.bclo[d] = h->bclo[d]; a.bchi[d] = h->bchi[d]; }
+        if (inflow_vel && !incremental) for (int q = 0; q < 18; ++q) a.inflow[q] = inflow_vel[q];
+        else for (int d = 0; d < 3; ++d) { a.bclo[d] = a.bchi[d] = MGX_BC_NEUMANN; }
+        gather_vel_kernel<<<h->layout.ntiles[REG_CELL_GROWN], 256, 0, h->stream()>>>(a);
+        // beta = s / face-average(rho), or s / ro_0 (:253-320)
+        if (density) {
+            h->cells_in(density, rho_ng, h->cells.as<double>());
+            for (int d = 0; d < 3; ++d)
+                mac_beta_from_rho_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                    g.k(), d, h->cells.as<double>(), scaling_factor, h->beta_dst(d));
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                const long nf = g.nface(d);
+                fill_kernel<<<grid1d(nf), 256, 0, h->stream()>>>(h->beta_dst(d), nf,
+                                                                scaling_factor / const_rho);
+            }
+        }
+        h->post_beta();
+        // MOL face velocities (:329-343)
+        for (int d = 0; d < 3; ++d) {
+            const long nf = g.nface(d);
+            mol_faces_kernel<<<grid1d(nf), 256, 0, h->stream()>>>(
+                kg, d, (!g.per[d] && h->bclo[d] == MGX_BC_DIRICHLET) ? 1 : 0,
+                (!g.per[d] && h->bchi[d] == MGX_BC_DIRICHLET) ? 1 : 0, h->ug.as<double>(),
+                h->u[d].as<double>());
+        }
+        // MAC projection from phi = 0 (:345-348)
+        const MGeomK k = g.k();
+        mac_div_kernel<<<grid3(g.n[0], g.n[1], g.n[2]), block3(), 0, h->stream()>>>(
+            k, h->u[0].as<double>(), h->u[1].as<double>(), h->u[2].as<double>(), 1.0 / g.dx[0],
+            1.0 / g.dx[1], 1.0 / g.dx[2], h->rhs.as<double>());
+        h->robin_rhs(h->rhs.as<double>());
+        MGX_HIP(hipMemsetAsync(h->solver.phi, 0, g.ncell() * sizeof(double), h->stream()));
+        h->solver.solve(h->rhs.as<double>(), rtol, atol, mode, info, h->stream());
+        double* U[3] = {umac, vmac, wmac};
+        for (int d = 0; d < 3; ++d) {
+            const double* bd = h->solver.fine_beta(d, h->stream());
+            if (U[d]) {  // the projected face velocities, when asked for
+                mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                    k, d, bd, h->solver.phi, 1.0 / g.dx[d], h->u[d].as<double>(), 0);
+                h->robin_flux(d, h->u[d].as<double>());
+                h->faces_out(d, U[d], umac_ng, h->u[d].as<double>());
+            }
+            // getFluxes (:362-371): F = -beta grad phi
+            mac_face_update_kernel<<<h->face_grid(d), block3(), 0, h->stream()>>>(
+                k, d, bd, h->solver.phi, 1.0 / g.dx[d], h->u[d].as<double>(), 1);
+            h->robin_flux(d, h->u[d].as<double>());
+        }
+        // u += gphi, p_cc, gp (:373-458)
+        CCUpdateArgs c{};
+        c.g = kg;
+        c.boxes = h->layout.dev_boxes();
+        c.tiles = h->layout.tiles(REG_CELL);
+        c.ug = h->ug.as<double>();
+        for (int d = 0; d < 3; ++d) c.F[d] = h->u[d].as<double>();
+        c.phi = h->solver.phi;
+        c.vel = vel;
+        c.vel_ng = vel_ng;
+        c.vel_old = vel_old;
+        c.vold_ng = vel_old_ng;
+        c.gp = gp;
+        c.gp_ng = gp_ng;
+        c.p = p_cc;
+        c.p_ng = p_ng;
+        c.rho = density;
+        c.rho_ng = rho_ng;
+        c.const_rho = density ? 0.0 : const_rho;
+        c.scale = scaling_factor;
+        c.incremental = incremental ? 1 : 0;
+        cc_update_kernel<<<h->layout.ntiles[REG_CELL], 256, 0, h->stream()>>>(c);
+        h->have_phi = true;
+    });
+    if (status == MGX_OK && info && !info->converged) {
+        h->ctx->last_error = "cc projection: MLMG failed to converge";
+        status = MGX_ERR_NOT_CONVERGED;
+    }
+    return status;
 }
 
 MGX_API int mgx_mac_solve(mgx_mac h, double* phi, int phi_ng, const double* rhs, int rhs_ng,

@@ -7,6 +7,11 @@
 //   (L phi)_c = sum_d (F_{c+1/2} - F_{c-1/2}) / h_d
 // i.e. div(beta grad phi) for Hydro::MacProjector (AMReX-Hydro, called from
 // src/convection/incflo_compute_MAC_projected_velocities.cpp:294-299).
+// With a cell coefficient A (acoef, optional) the operator is
+//   (L phi)_c = sum_d (F_{c+1/2} - F_{c-1/2}) / h_d - A_c phi_c,
+// the Helmholtz form -(alpha a - beta div(b grad)) of MLABecLaplacian with
+// A = alpha a and the face coefficients beta b (incflo's implicit viscous
+// solve, src/diffusion/DiffusionScalarOp.cpp:396-478).
 // Cells are stored x-fastest without ghosts; face arrays along d hold n_d+1
 // faces (the periodic face n_d equals face 0).
 #pragma once
@@ -40,8 +45,8 @@ __device__ __forceinline__ long fidx(const MGeomK& g, int d, int i, int j, int k
 // (L phi)_c and the diagonal of L at cell (i,j,k); phi read with direct loads.
 template <class T, class TB, class TP>
 __device__ __forceinline__ T mac_apply_cell(const MGeomK& g, const MCoef<T>& c, const TB* bx,
-                                            const TB* by, const TB* bz, const TP* phi, int i,
-                                            int j, int k, T& diag) {
+                                            const TB* by, const TB* bz, const TB* acoef,
+                                            const TP* phi, int i, int j, int k, T& diag) {
     const int p[3] = {i, j, k};
     const TB* beta[3] = {bx, by, bz};
     const T pc = (T)phi[cidx(g, i, j, k)];
@@ -76,6 +81,11 @@ __device__ __forceinline__ T mac_apply_cell(const MGeomK& g, const MCoef<T>& c, 
             }
         }
     }
+    if (acoef) {
+        const T av = (T)acoef[cidx(g, i, j, k)];
+        acc -= av * pc;
+        dg -= av;
+    }
     diag = dg;
     return acc;
 }
@@ -85,13 +95,14 @@ __device__ __forceinline__ T mac_apply_cell(const MGeomK& g, const MCoef<T>& c, 
 template <class T>
 static __global__ void mac_gsrb_kernel(MGeomK g, MCoef<T> c, const T* __restrict__ bx,
                                        const T* __restrict__ by, const T* __restrict__ bz,
-                                       const T* __restrict__ b, T* x, int color) {
+                                       const T* __restrict__ a, const T* __restrict__ b, T* x,
+                                       int color) {
     int ih, j, k;
     if (!grid_point((g.n[0] + 1) / 2, g.n[1], ih, j, k)) return;
     const int i = 2 * ih + ((j + k + color) & 1);
     if (i >= g.n[0]) return;
     T dg;
-    const T acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, x, i, j, k, dg);
+    const T acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, a, x, i, j, k, dg);
     const long id = cidx(g, i, j, k);
     if (dg != (T)0) x[id] += (b[id] - acc) / dg;
 }
@@ -102,6 +113,7 @@ enum { MK_RESID = 0, MK_UPDRES = 1, MK_APPLY = 2 };
 template <class T, class TC, int MODE>
 static __global__ void mac_resid_kernel(MGeomK g, MCoef<T> c, const T* __restrict__ bx,
                                         const T* __restrict__ by, const T* __restrict__ bz,
+                                        const T* __restrict__ a,
                                         const T* __restrict__ b, const T* __restrict__ x,
                                         const TC* __restrict__ cor, T* __restrict__ xout,
                                         TC* __restrict__ r, unsigned long long* norm) {
@@ -114,11 +126,11 @@ static __global__ void mac_resid_kernel(MGeomK g, MCoef<T> c, const T* __restric
         if (MODE == MK_UPDRES) {
             // L(x + cor) = L x + L cor (linear); both read directly
             T d2;
-            acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, x, i, j, k, dg) +
-                  mac_apply_cell<T, T, TC>(g, c, bx, by, bz, cor, i, j, k, d2);
+            acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, a, x, i, j, k, dg) +
+                  mac_apply_cell<T, T, TC>(g, c, bx, by, bz, a, cor, i, j, k, d2);
             xout[id] = x[id] + (T)cor[id];
         } else {
-            acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, x, i, j, k, dg);
+            acc = mac_apply_cell<T, T, T>(g, c, bx, by, bz, a, x, i, j, k, dg);
         }
         if (MODE == MK_APPLY) {
             r[id] = (TC)acc;
@@ -247,7 +259,7 @@ static __global__ void mac_beta_from_rho_kernel(MGeomK g, int d, const double* _
 // ---- bottom: BiCGStab in one workgroup on the MAC operator (FP64) ----
 template <class T>
 __device__ void mac_bottom_apply(const MGeomK& g, const MCoef<double>& c, const T* bx, const T* by,
-                                 const T* bz, const double* x, double* y, long n) {
+                                 const T* bz, const T* a, const double* x, double* y, long n) {
     __syncthreads();
     for (long id = threadIdx.x; id < n; id += blockDim.x) {
         const int q = (int)(id / g.n[0]);
@@ -255,7 +267,7 @@ __device__ void mac_bottom_apply(const MGeomK& g, const MCoef<double>& c, const 
         const int j = q % g.n[1];
         const int k = q / g.n[1];
         double dg;
-        y[id] = mac_apply_cell<double, T, double>(g, c, bx, by, bz, x, i, j, k, dg);
+        y[id] = mac_apply_cell<double, T, double>(g, c, bx, by, bz, a, x, i, j, k, dg);
     }
     __syncthreads();
 }
@@ -263,7 +275,8 @@ __device__ void mac_bottom_apply(const MGeomK& g, const MCoef<double>& c, const 
 template <class T>
 static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef<double> c,
                                                                  const T* bx, const T* by,
-                                                                 const T* bz, const T* b_in,
+                                                                 const T* bz, const T* a,
+                                                                 const T* b_in,
                                                                  T* x_out, double* work,
                                                                  int maxiter, double rtol,
                                                                  int singular, int* iters_acc) {
@@ -299,7 +312,7 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
             if (rho_new == 0.0) break;
             const double beta = (rho_new / rho) * (alpha / omega);
             for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
-            mac_bottom_apply<T>(g, c, bx, by, bz, p, v, n);
+            mac_bottom_apply<T>(g, c, bx, by, bz, a, p, v, n);
             q = 0.0;
             for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
             const double rhv = bsum(q, red);
@@ -313,7 +326,7 @@ static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef
             }
             sn = bmax(sn, red);
             if (sn <= rtol * rnorm0) break;
-            mac_bottom_apply<T>(g, c, bx, by, bz, s, t, n);
+            mac_bottom_apply<T>(g, c, bx, by, bz, a, s, t, n);
             double tt = 0.0, ts = 0.0;
             for (long id = tid; id < n; id += nt) { tt += t[id] * t[id]; ts += t[id] * s[id]; }
             tt = bsum(tt, red);

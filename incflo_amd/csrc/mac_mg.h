@@ -15,12 +15,14 @@ struct MacGeom {
     int dlo[3], dhi[3];  // Dirichlet-form sides (Dirichlet, or Robin with scaled beta)
     int rlo[3], rhi[3];  // of those, the Robin sides
     bool robin_fixes;    // some Robin face has a > 0
+    bool has_a;          // Helmholtz form: a cell coefficient A > 0 (never singular)
     double dx[3];
     long ncell() const { return (long)n[0] * n[1] * n[2]; }
     long nface(int d) const {
         return (long)(n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2));
     }
     bool singular() const {
+        if (has_a) return false;
         for (int d = 0; d < 3; ++d)
             if (!per[d] && ((dlo[d] && !rlo[d]) || (dhi[d] && !rhi[d]))) return false;
         return !robin_fixes;
@@ -54,8 +56,9 @@ struct MacLevel {
     MacGeom g;
     MGeomK kg;
     MCoef<T> c;
-    DevBuf beta_buf[3], b_buf, x_buf, r_buf;
+    DevBuf beta_buf[3], a_buf, b_buf, x_buf, r_buf;
     T* beta[3] = {nullptr, nullptr, nullptr};
+    T* a = nullptr;  // Helmholtz cell coefficient (null: none)
     T* b = nullptr;
     T* x = nullptr;
     T* r = nullptr;
@@ -76,6 +79,9 @@ class MacHierarchy {
     // (null: not a Robin side)
     void set_beta(const double* const beta0[3], hipStream_t s,
                   const std::vector<const double*>* rscale = nullptr);
+    // level-0 Helmholtz coefficient A from an FP64 cell array; coarse levels
+    // take the mean of their 8 fine cells (needs g0.has_a)
+    void set_acoef(const double* a0, hipStream_t s);
     void vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters);
 };
 
@@ -87,6 +93,10 @@ class MacSolver {
     MacHierarchy<double> H64;
     MacHierarchy<float> H32;
     DevBuf beta0[3];      // level-0 face coefficients (FP64)
+    DevBuf acoef0;        // level-0 Helmholtz cell coefficient A (FP64; g0.has_a)
+    bool acoef_set = false, h32_a = false, h64_a = false;
+    double* acoef() { return g0.has_a ? acoef0.as<double>() : nullptr; }
+    void acoef_changed() { acoef_set = true; h32_a = false; h64_a = false; }
     DevBuf phi_a, phi_b;  // outer iterate
     DevBuf res64, res32;
     DevBuf scal, part, ibuf;

@@ -27,6 +27,12 @@ void MacHierarchy<T>::build(const MacGeom& g0, int nlev) {
             lv.beta[d] = lv.beta_buf[d].template as<T>();
         }
         const long n = lv.g.ncell();
+        if (lv.g.has_a) {
+            lv.a_buf.alloc(n * sizeof(T));
+            lv.a = lv.a_buf.template as<T>();
+        } else {
+            lv.a = nullptr;
+        }
         if (l > 0) lv.b_buf.alloc(n * sizeof(T));
         lv.x_buf.alloc(n * sizeof(T));
         if (l < nlev - 1) lv.r_buf.alloc(n * sizeof(T));
@@ -67,10 +73,22 @@ void MacHierarchy<T>::set_beta(const double* const b0[3], hipStream_t s,
 }
 
 template <class T>
+void MacHierarchy<T>::set_acoef(const double* a0, hipStream_t s) {
+    if (!L[0].a) return;
+    const long n0 = L[0].g.ncell();
+    cast_kernel<T, double><<<grid1d(n0), 256, 0, s>>>(a0, L[0].a, n0);
+    for (size_t l = 1; l < L.size(); ++l) {
+        const MacGeom& gc = L[l].g;
+        mac_restrict_kernel<T><<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
+            L[l - 1].kg, L[l].kg, L[l - 1].a, L[l].a);
+    }
+}
+
+template <class T>
 static void gsrb(MacLevel<T>& lv, hipStream_t s) {
     for (int color = 0; color < 2; ++color)
         mac_gsrb_kernel<T><<<grid3((lv.g.n[0] + 1) / 2, lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
-            lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.b, lv.x, color);
+            lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.a, lv.b, lv.x, color);
 }
 
 template <class T>
@@ -82,8 +100,8 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
         if (l > 0 || !x0_guess) zero_kernel<T><<<grid1d(n), 256, 0, s>>>(lv.x, n);
         for (int q = 0; q < o.nu1; ++q) gsrb(lv, s);
         mac_resid_kernel<T, T, MK_RESID><<<grid3(lv.g.n[0], lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
-            lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.b, lv.x, nullptr, nullptr, lv.r,
-            nullptr);
+            lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.a, lv.b, lv.x, nullptr, nullptr,
+            lv.r, nullptr);
         MacLevel<T>& cv = L[l + 1];
         mac_restrict_kernel<T><<<grid3(cv.g.n[0], cv.g.n[1], cv.g.n[2]), block3(), 0, s>>>(
             lv.kg, cv.kg, lv.r, cv.b);
@@ -93,7 +111,7 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
         const long nb = lb.g.ncell();
         const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
         mac_bottom_kernel<T><<<1, threads, 0, s>>>(lb.kg, make_mcoef<double>(lb.g.dx), lb.beta[0],
-                                                   lb.beta[1], lb.beta[2], lb.b, lb.x,
+                                                   lb.beta[1], lb.beta[2], lb.a, lb.b, lb.x,
                                                    bottom_work.as<double>(), o.bottom_maxiter,
                                                    o.bottom_rtol, lb.g.singular() ? 1 : 0,
                                                    bottom_iters);
@@ -132,6 +150,8 @@ void MacSolver::setup(const MacGeom& g, const MGOpts& o) {
     nlev = mac_nlevels(g, o.max_coarsening_level);
     for (int d = 0; d < 3; ++d) beta0[d].alloc(g.nface(d) * sizeof(double));
     const long n = g.ncell();
+    if (g.has_a) acoef0.alloc(n * sizeof(double));
+    h32_a = h64_a = false;
     phi_a.alloc(n * sizeof(double));
     phi_b.alloc(n * sizeof(double));
     MGX_HIP(hipMemset(phi_a.p, 0, n * sizeof(double)));
@@ -142,6 +162,7 @@ void MacSolver::setup(const MacGeom& g, const MGOpts& o) {
     if (!h_pinned) MGX_HIP(hipHostMalloc((void**)&h_pinned, 64 * sizeof(unsigned long long), 0));
     H64.built = H32.built = false;
     beta_set = keep_beta;
+    acoef_set = acoef_set && keep_beta && g.has_a;
     h32_beta = h64_beta = false;
     rob_levels = 0;  // the level count may have changed
     beta0s_ok = false;
@@ -264,14 +285,18 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     const double* braw0[3] = {beta(0), beta(1), beta(2)};
     const double* b0[3] = {fine_beta(0, s), fine_beta(1, s), fine_beta(2, s)};
     const std::vector<const double*>* rs = robin_on ? &rob_ptr : nullptr;
+    MGX_REQUIRE(!g.has_a || acoef_set, MGX_ERR_STATE, "Helmholtz solve: a coefficient not set");
+    const double* A0 = acoef();
     if (mixed) {
-        if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g, nlev); h32_beta = false; }
+        if (!H32.built || (int)H32.L.size() != nlev) { H32.build(g, nlev); h32_beta = h32_a = false; }
         if (!h32_beta) { H32.set_beta(braw0, s, rs); h32_beta = true; }
+        if (A0 && !h32_a) { H32.set_acoef(A0, s); h32_a = true; }
         res32.alloc(n * sizeof(float));
         H32.L[0].b = res32.as<float>();
     } else {
-        if (!H64.built || (int)H64.L.size() != nlev) { H64.build(g, nlev); h64_beta = false; }
+        if (!H64.built || (int)H64.L.size() != nlev) { H64.build(g, nlev); h64_beta = h64_a = false; }
         if (!h64_beta) { H64.set_beta(braw0, s, rs); h64_beta = true; }
+        if (A0 && !h64_a) { H64.set_acoef(A0, s); h64_a = true; }
         res64.alloc(n * sizeof(double));
         H64.L[0].b = res64.as<double>();
     }
@@ -292,10 +317,10 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
     zero_u64_kernel<<<1, 64, 0, s>>>(nrm, 1);
     if (mixed)
         mac_resid_kernel<double, float, MK_RESID><<<gr, block3(), 0, s>>>(
-            kg, c64, b0[0], b0[1], b0[2], b, phi, nullptr, nullptr, res32.as<float>(), nrm);
+            kg, c64, b0[0], b0[1], b0[2], A0, b, phi, nullptr, nullptr, res32.as<float>(), nrm);
     else
         mac_resid_kernel<double, double, MK_RESID><<<gr, block3(), 0, s>>>(
-            kg, c64, b0[0], b0[1], b0[2], b, phi, nullptr, nullptr, res64.as<double>(), nrm);
+            kg, c64, b0[0], b0[1], b0[2], A0, b, phi, nullptr, nullptr, res64.as<double>(), nrm);
     const double r0 = read_norm(1, s);
     const double tol = std::max(atol, rtol * bnorm);
     double rn = r0;
@@ -318,11 +343,11 @@ void MacSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve_i
                 H32.vcycle(opts, s, ibuf.as<int>());
             }
             mac_resid_kernel<double, float, MK_UPDRES><<<gr, block3(), 0, s>>>(
-                kg, c64, b0[0], b0[1], b0[2], b, phi, H32.L[0].x, pout, res32.as<float>(), nrm);
+                kg, c64, b0[0], b0[1], b0[2], A0, b, phi, H32.L[0].x, pout, res32.as<float>(), nrm);
         } else {
             H64.vcycle(opts, s, ibuf.as<int>());
             mac_resid_kernel<double, double, MK_UPDRES><<<gr, block3(), 0, s>>>(
-                kg, c64, b0[0], b0[1], b0[2], b, phi, H64.L[0].x, pout, res64.as<double>(), nrm);
+                kg, c64, b0[0], b0[1], b0[2], A0, b, phi, H64.L[0].x, pout, res64.as<double>(), nrm);
         }
         phi = pout;
         rn = read_norm(1, s);

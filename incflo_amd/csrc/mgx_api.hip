@@ -332,6 +332,43 @@ MGX_API int mgx_nodal_set_options(mgx_nodal h, const mgx_mlmg_opts* o) {
     });
 }
 
+MGX_API int mgx_nodal_autotune(mgx_nodal h, int mode, char* report, int report_len) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        check_mode(mode);
+        h->ensure_setup();
+        const std::string r = h->solver.autotune(mode, h->stream());
+        if (report && report_len > 0) {
+            const size_t n = std::min<size_t>(r.size(), (size_t)report_len - 1);
+            std::memcpy(report, r.data(), n);
+            report[n] = 0;
+        }
+    });
+}
+
+MGX_API int mgx_tuning_get(char* buf, int len) {
+    try {
+        const std::string t = tuning_table();
+        if (!buf || len <= (int)t.size()) return MGX_ERR_ARG;
+        std::memcpy(buf, t.c_str(), t.size() + 1);
+        return MGX_OK;
+    } catch (...) {
+        return MGX_ERR_ARG;
+    }
+}
+
+MGX_API int mgx_tuning_set(const char* text) {
+    if (!text) return MGX_ERR_ARG;
+    try {
+        set_tuning_table(text);
+        return MGX_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (...) {
+        return MGX_ERR_ARG;
+    }
+}
+
 MGX_API int mgx_nodal_set_timing(mgx_nodal h, int on) {
     if (!h) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {

@@ -1,6 +1,7 @@
 // nodal_mg.h -- nodal multigrid hierarchy and the outer MLMG iteration.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "box_kernels.h"
@@ -232,7 +233,15 @@ class NodalSolver {
     double read_norm(int slot, hipStream_t s);
     void zero_norm(int slot, hipStream_t s);
     double sum(const double* a, long n, hipStream_t s);
+    // time candidate launch shapes on this problem's levels (mode's
+    // hierarchy) and keep the fastest in the process-wide table; returns a
+    // text report (one line per tuned pass)
+    std::string autotune(int mode, hipStream_t s);
 };
+
+// the process-wide launch-shape table as text, and loading one back
+std::string tuning_table();
+void set_tuning_table(const std::string& text);
 
 // Box list of one level, its device box-offset table and tile tables.
 class BoxLayout {

@@ -15,6 +15,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
 
 #include "eb_nodal.h"
 #include "learned_kernels.h"
@@ -23,36 +27,60 @@
 
 namespace mgx {
 
-// Nodes per thread of the stencil kernel (1: 64x8 threads, 2: 64x4 threads);
-// MGX_STENCIL_NPT overrides the default for tuning runs.
-static int stencil_npt(bool dbl) {
-    static int env = [] {
-        const char* e = getenv("MGX_STENCIL_NPT");
-        return e ? atoi(e) : 0;
-    }();
-    if (env == 1 || env == 2 || env == 4) return env;
-    (void)dbl;
-    return 2;
+// ---- launch shapes of the stencil kernel ------------------------------------
+// A shape is the nodes per thread (NPT: 1 -> 64x8 threads, 2 -> 64x4, 4 -> 64x2
+// threads per workgroup of 64 x 8 nodes), a target workgroup count that sets
+// the z-run length, and the shortest z-run.  Defaults: NPT 2, 1024 workgroups
+// for 4-byte storage and 512 for FP64 (measured best at 256^3 on MI355X), z-runs
+// of at least 2 planes.  mgx_nodal_autotune times candidates on the handle's
+// own levels and stores the fastest per (level shape, storage, pass) in this
+// process-wide table (mgx_tuning_get / mgx_tuning_set read and write it).
+struct StencilShape {
+    int npt = 2;
+    long target = 1024;
+    int kzmin = 2;
+};
+struct ShapeKey {
+    int nn0, nn1, nn2, word, cword, mode, fine;
+    bool operator<(const ShapeKey& o) const {
+        const int a[7] = {nn0, nn1, nn2, word, cword, mode, fine};
+        const int b[7] = {o.nn0, o.nn1, o.nn2, o.word, o.cword, o.mode, o.fine};
+        return std::lexicographical_compare(a, a + 7, b, b + 7);
+    }
+};
+static std::mutex g_tune_mtx;
+static std::map<ShapeKey, StencilShape> g_shapes;
+// levels with at most this many nodes run inside the single-workgroup launch
+// that holds the bottom solve (1: only the bottom level; measured at 256^3:
+// per-level launches for 16^3 and 8^3 beat folding them into one workgroup,
+// which is then instruction-latency bound on one CU)
+static long g_coarse_nodes = 1;
+
+static StencilShape stencil_shape(const NodalGeom& g, int word, int cword, int mode, bool fine) {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    auto it = g_shapes.find(ShapeKey{g.nn[0], g.nn[1], g.nn[2], word, cword, mode, fine ? 1 : 0});
+    if (it != g_shapes.end()) return it->second;
+    StencilShape d;
+    d.target = word == 8 ? 512 : 1024;
+    return d;
+}
+
+static long coarse_threshold() {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    return g_coarse_nodes;
 }
 
 template <class T, class TC, int MODE, int NPT>
-static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
+static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine,
+                               const StencilShape& sh) {
     constexpr int TX = 64, NY = (NPT == 1 ? 8 : 4 * NPT);
     a.gx = (g.nn[0] + TX - 1) / TX;
     a.gy = (g.nn[1] + NY - 1) / NY;
     const long gxy = (long)a.gx * a.gy;
-    // z-planes per workgroup: long runs amortise the two prologue planes on big
-    // levels; small levels are latency bound and get short runs / more blocks
-    // z-planes per workgroup from a target workgroup count: long z-runs
-    // amortise the two halo planes and the pipeline start; measured best at
-    // 256^3 (MI355X): 1024 workgroups for FP32/FP16 storage, 512 for FP64.
-    // MGX_STENCIL_BLOCKS overrides it for tuning runs.
-    static const long env_target = [] {
-        const char* e = getenv("MGX_STENCIL_BLOCKS");
-        return e ? atol(e) : 0L;
-    }();
-    const long target = env_target > 0 ? env_target : (sizeof(T) == 8 ? 512L : 1024L);
-    int kz = (int)std::min<long>(256, std::max<long>(2, (g.nn[2] * gxy) / target));
+    // z-planes per workgroup from the target workgroup count: long z-runs
+    // amortise the two halo planes and the pipeline start on big levels;
+    // small levels are latency bound and want short runs / more workgroups
+    int kz = (int)std::min<long>(256, std::max<long>(sh.kzmin, (g.nn[2] * gxy) / sh.target));
     kz = std::min(kz, g.nn[2]);
     a.kz = kz;
     a.gz = (g.nn[2] + kz - 1) / kz;
@@ -77,14 +105,20 @@ static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s
 }
 
 template <class T, class TC, int MODE>
-void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
-    const int npt = stencil_npt(sizeof(T) == 8);
-    if (npt == 1)
-        launch_stencil_npt<T, TC, MODE, 1>(g, a, s, fine);
-    else if (npt == 4)
-        launch_stencil_npt<T, TC, MODE, 4>(g, a, s, fine);
+static void launch_stencil_shape(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine,
+                                 const StencilShape& sh) {
+    if (sh.npt == 1)
+        launch_stencil_npt<T, TC, MODE, 1>(g, a, s, fine, sh);
+    else if (sh.npt == 4)
+        launch_stencil_npt<T, TC, MODE, 4>(g, a, s, fine, sh);
     else
-        launch_stencil_npt<T, TC, MODE, 2>(g, a, s, fine);
+        launch_stencil_npt<T, TC, MODE, 2>(g, a, s, fine, sh);
+}
+
+template <class T, class TC, int MODE>
+void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
+    launch_stencil_shape<T, TC, MODE>(
+        g, a, s, fine, stencil_shape(g, (int)sizeof(T), (int)sizeof(TC), MODE, fine));
 }
 
 // Full-weighting restriction onto coarse level gc (z-streamed, LDS-staged);
@@ -239,19 +273,6 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         if (l == 0 && timer) timer->end(s);
         std::swap(lv.x, lv.x2);
     }
-}
-
-// Levels with at most this many nodes run inside the single-workgroup launch
-// that holds the bottom solve (MGX_COARSE_NODES overrides it for tuning runs).
-// Measured at 256^3: per-level launches for 16^3 and 8^3 beat folding them
-// into the one workgroup (which is then instruction-latency bound on one CU),
-// so by default only the bottom level runs there.
-static long coarse_threshold() {
-    static long v = [] {
-        const char* e = getenv("MGX_COARSE_NODES");
-        return e ? atol(e) : 1L;
-    }();
-    return v;
 }
 
 static bool same_opts(const MGOpts& a, const MGOpts& b) {
@@ -696,6 +717,235 @@ static void prepare_h64(NodalSolver& S, hipStream_t s) {
     S.res64.alloc(S.g0.nnode() * sizeof(double));
     S.H64.L[0].b = S.res64.as<double>();
     S.H64.set_sigma(S.sigma0(), s);  // coarse levels; level 0 is shared
+}
+
+// ---- autotuning of the launch shapes ----------------------------------------
+
+namespace {
+
+struct TuneTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TuneTimer() {
+        MGX_HIP(hipEventCreate(&e0));
+        MGX_HIP(hipEventCreate(&e1));
+    }
+    ~TuneTimer() {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    // microseconds per call of f, after two warm-up calls
+    template <class F>
+    double time(F&& f, hipStream_t s, int reps = 5) {
+        f();
+        f();
+        MGX_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) f();
+        MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return 1e3 * ms / reps;
+    }
+};
+
+// Candidate shapes, one per distinct (NPT, z-run) they give on this level.
+std::vector<StencilShape> shape_candidates(const NodalGeom& g) {
+    std::vector<StencilShape> out;
+    std::vector<std::pair<int, int>> seen;
+    for (int npt : {1, 2, 4}) {
+        const int ny = npt == 1 ? 8 : 4 * npt;
+        const long gxy = (long)((g.nn[0] + 63) / 64) * ((g.nn[1] + ny - 1) / ny);
+        for (long target : {128L, 256L, 512L, 1024L, 2048L, 4096L, 8192L, 16384L})
+            for (int kzmin : {1, 2}) {
+                int kz = (int)std::min<long>(256, std::max<long>(kzmin, (g.nn[2] * gxy) / target));
+                kz = std::min(kz, g.nn[2]);
+                const std::pair<int, int> key{npt, kz};
+                if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                seen.push_back(key);
+                StencilShape sh;
+                sh.npt = npt;
+                sh.target = target;
+                sh.kzmin = kzmin;
+                out.push_back(sh);
+            }
+    }
+    return out;
+}
+
+template <class T, class TC, int MODE>
+void tune_pass(const NodalGeom& g, const SArgs<T, TC>& a, hipStream_t s, bool fine, TuneTimer& tt,
+               std::ostringstream& rep) {
+    const StencilShape def = stencil_shape(g, (int)sizeof(T), (int)sizeof(TC), MODE, fine);
+    const double t_def = tt.time([&] { launch_stencil_shape<T, TC, MODE>(g, a, s, fine, def); }, s);
+    StencilShape best = def;
+    double t_best = t_def;
+    for (const StencilShape& c : shape_candidates(g)) {
+        const double t = tt.time([&] { launch_stencil_shape<T, TC, MODE>(g, a, s, fine, c); }, s);
+        if (t < 0.97 * t_best) {  // keep the incumbent unless clearly faster
+            t_best = t;
+            best = c;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        g_shapes[ShapeKey{g.nn[0], g.nn[1], g.nn[2], (int)sizeof(T), (int)sizeof(TC), MODE,
+                          fine ? 1 : 0}] = best;
+    }
+    rep << "stencil " << g.nn[0] << " " << g.nn[1] << " " << g.nn[2] << " " << sizeof(T) << " "
+        << sizeof(TC) << " " << MODE << " " << (fine ? 1 : 0) << " npt " << best.npt << " target "
+        << best.target << " kzmin " << best.kzmin << " us " << t_best << " default_us " << t_def
+        << "\n";
+}
+
+template <class T>
+void tune_levels(NodalHierarchy<T>& H, const MGOpts& o, hipStream_t s, TuneTimer& tt,
+                 std::ostringstream& rep) {
+    for (size_t l = 0; l < H.L.size(); ++l) {
+        MGLevel<T>& lv = H.L[l];
+        const long nn = lv.g.nnode();
+        zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.x_buf.template as<T>(), nn);
+        zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.x2_buf.template as<T>(), nn);
+        zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.b, nn);
+        if (lv.r) zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.r, nn);
+        SArgs<T, T> a{};
+        a.g = lv.kg;
+        a.c = lv.coef;
+        a.x = lv.x_buf.template as<T>();
+        a.b = lv.b;
+        a.sig = lv.sig;
+        a.xout = lv.x2_buf.template as<T>();
+        a.rout = lv.r ? lv.r : lv.x2_buf.template as<T>();
+        a.omega = (T)nodal_omega(lv.g, o.omega);
+        tune_pass<T, T, KS_JACOBI>(lv.g, a, s, l == 0, tt, rep);
+        if (l + 1 < H.L.size()) tune_pass<T, T, KS_RESID>(lv.g, a, s, false, tt, rep);
+        tune_pass<T, T, KS_JFIRST>(lv.g, a, s, false, tt, rep);
+    }
+}
+
+}  // namespace
+
+std::string NodalSolver::autotune(int mode, hipStream_t s) {
+    MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "autotune: sigma has not been set (project once first)");
+    MGX_REQUIRE(!eb && !mask_on, MGX_ERR_UNSUPPORTED,
+                "autotune: plain (no EB, no overset mask) levels only");
+    std::ostringstream rep;
+    rep.precision(4);
+    TuneTimer tt;
+    const bool mixed = mode != MGX_MODE_FP64;
+    const long nn = g0.nnode();
+    DevBuf tb, tx, ty;
+    tb.alloc(nn * sizeof(double));
+    tx.alloc(nn * sizeof(double));
+    ty.alloc(nn * sizeof(double));
+    zero_kernel<double><<<grid1d(nn), 256, 0, s>>>(tb.as<double>(), nn);
+    zero_kernel<double><<<grid1d(nn), 256, 0, s>>>(tx.as<double>(), nn);
+    const Coef<double> c64 = make_coef<double>(g0.dx);
+    auto outer = [&](auto& H, auto zero_cor) {
+        using TC = typename std::remove_pointer<decltype(zero_cor)>::type;
+        SArgs<double, TC> a{};
+        a.g = kgeom(g0);
+        a.c = c64;
+        a.x = tx.as<double>();
+        a.b = tb.as<double>();
+        a.sig = sigma0();
+        a.cor = H.L[0].x_buf.template as<TC>();
+        a.xout = ty.as<double>();
+        a.rout = H.L[0].b;
+        a.norm = scal.as<unsigned long long>() + 3;
+        if (opts.nu1 > 0 && H.L.size() > 1) {
+            a.x1out = H.L[0].x2_buf.template as<TC>();
+            a.omega1 = nodal_omega(g0, opts.omega);
+        }
+        tune_pass<double, TC, KS_RESID>(g0, a, s, false, tt, rep);
+        tune_pass<double, TC, KS_UPDRES>(g0, a, s, false, tt, rep);
+    };
+    auto coarse = [&](auto& H) {
+        // the level count folded into the single-workgroup coarse launch
+        std::vector<long> cands{1};
+        for (auto& lv : H.L)
+            if (lv.g.nnode() <= 40000 && lv.g.nnode() > 1) cands.push_back(lv.g.nnode());
+        long best = coarse_threshold();
+        double t_best = 1e300, t_def = 0.0;
+        const long def = best;
+        for (long c : cands) {
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mtx);
+                g_coarse_nodes = c;
+            }
+            H.drop_graphs();
+            const double t = tt.time([&] {
+                H.x0_ready = false;
+                H.x0_guess = false;
+                H.vcycle(opts, s, ibuf.as<int>());
+            }, s);
+            if (c == def) t_def = t;
+            if (t < t_best) {
+                t_best = t;
+                best = c;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mtx);
+            g_coarse_nodes = best;
+        }
+        H.drop_graphs();
+        rep << "coarse_nodes " << best << " vcycle_us " << t_best << " default_us " << t_def << "\n";
+    };
+    if (mixed) {
+        prepare_h32(s);
+        tune_levels(H32, opts, s, tt, rep);
+        outer(H32, (float*)nullptr);
+        coarse(H32);
+        H32.x0_ready = H32.x0_guess = false;
+    } else {
+        prepare_h64(*this, s);
+        tune_levels(H64, opts, s, tt, rep);
+        outer(H64, (double*)nullptr);
+        coarse(H64);
+        H64.x0_ready = H64.x0_guess = false;
+    }
+    MGX_HIP(hipStreamSynchronize(s));
+    return rep.str();
+}
+
+std::string tuning_table() {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    std::ostringstream o;
+    for (auto& kv : g_shapes) {
+        const ShapeKey& k = kv.first;
+        o << "stencil " << k.nn0 << " " << k.nn1 << " " << k.nn2 << " " << k.word << " " << k.cword
+          << " " << k.mode << " " << k.fine << " " << kv.second.npt << " " << kv.second.target << " "
+          << kv.second.kzmin << "\n";
+    }
+    o << "coarse_nodes " << g_coarse_nodes << "\n";
+    return o.str();
+}
+
+void set_tuning_table(const std::string& text) {
+    std::istringstream in(text);
+    std::string tag;
+    std::map<ShapeKey, StencilShape> shapes;
+    long coarse = 1;
+    while (in >> tag) {
+        if (tag == "stencil") {
+            ShapeKey k{};
+            StencilShape sh;
+            in >> k.nn0 >> k.nn1 >> k.nn2 >> k.word >> k.cword >> k.mode >> k.fine >> sh.npt >>
+                sh.target >> sh.kzmin;
+            MGX_REQUIRE(!in.fail() && (sh.npt == 1 || sh.npt == 2 || sh.npt == 4) && sh.target > 0 &&
+                            (sh.kzmin == 1 || sh.kzmin == 2),
+                        MGX_ERR_ARG, "tuning table: bad stencil line");
+            shapes[k] = sh;
+        } else if (tag == "coarse_nodes") {
+            in >> coarse;
+            MGX_REQUIRE(!in.fail() && coarse >= 1, MGX_ERR_ARG, "tuning table: bad coarse_nodes");
+        } else {
+            throw Error(MGX_ERR_ARG, "tuning table: unknown line '" + tag + "'");
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    g_shapes = std::move(shapes);
+    g_coarse_nodes = coarse;
 }
 
 // ---- learned coarse-level predictor ---------------------------------------

@@ -159,6 +159,30 @@ def lib():
             "mgx_amr_mac_set_beta": [vp, C.POINTER(vp), C.c_int],
             "mgx_amr_mac_project": [vp, C.POINTER(vp), C.c_int, C.POINTER(vp), C.c_int, C.c_double,
                                     C.c_double, C.c_int, C.POINTER(SolveInfo)],
+            "mgx_apply_cc_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int, vp,
+                                        C.c_int, vp, C.c_int, C.c_double, C.c_int, vp, vp, vp,
+                                        vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                        C.POINTER(SolveInfo)],
+            "mgx_nodal_autotune": [vp, C.c_int, C.c_char_p, C.c_int],
+            "mgx_tuning_get": [C.c_char_p, C.c_int],
+            "mgx_tuning_set": [C.c_char_p],
+            "mgx_step_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
+            "mgx_step_destroy": [vp],
+            "mgx_fill_boundary": [vp, vp, C.c_int, C.c_int],
+            "mgx_compute_vel_forces": [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int,
+                                       vp, C.c_int],
+            "mgx_update_velocity": [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp,
+                                    C.c_int, C.c_double, vp, vp, vp, C.c_int, C.c_double],
+            "mgx_update_density": [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp, C.c_int, vp,
+                                   C.c_int],
+            "mgx_diff_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
+            "mgx_diff_destroy": [vp],
+            "mgx_diff_set_domain_bc": [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "mgx_diff_set_options": [vp, C.POINTER(MLMGOpts)],
+            "mgx_diff_set_eta": [vp, vp, C.c_int, C.c_double],
+            "mgx_diff_apply": [vp, vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int],
+            "mgx_diff_solve": [vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_double, C.c_double,
+                               C.c_double, C.c_int, C.POINTER(SolveInfo)],
             "mgx_apply_nodal_projection": [vp, vp, C.c_int, C.c_double, vp, C.c_int, vp, C.c_int,
                                            vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, vp,
                                            C.c_double, C.c_double, C.c_int,
@@ -171,6 +195,21 @@ def lib():
         L.mgx_default_mlmg_opts.restype = None
         _lib = L
     return _lib
+
+
+def tuning_get():
+    """the process-wide launch-shape table as text"""
+    buf = C.create_string_buffer(1 << 16)
+    st = lib().mgx_tuning_get(buf, len(buf))
+    if st != MGX_OK:
+        raise MgxError(st, "mgx_tuning_get failed")
+    return buf.value.decode()
+
+
+def tuning_set(text):
+    st = lib().mgx_tuning_set(text.encode())
+    if st != MGX_OK:
+        raise MgxError(st, "mgx_tuning_set: bad table")
 
 
 def default_opts(**kw):
@@ -313,6 +352,12 @@ class NodalProjector:
         self.ctx.check(lib().mgx_nodal_get_timing(self.h, 0, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def autotune(self, mode=MODE_FP64):
+        """mgx_nodal_autotune: returns the text report."""
+        buf = C.create_string_buffer(1 << 16)
+        self.ctx.check(lib().mgx_nodal_autotune(self.h, mode, buf, len(buf)))
+        return buf.value.decode()
+
     def set_sigma(self, sigma_dev, ng=0):
         self.ctx.check(lib().mgx_nodal_set_sigma(self.h, sigma_dev.ptr, ng))
 
@@ -427,6 +472,23 @@ class MacProjector:
         first ghost cell outside each Robin face holds the values."""
         self.ctx.check(lib().mgx_mac_set_level_bc(self.h, robin_a.ptr, robin_b.ptr, robin_f.ptr,
                                                   ng))
+
+    def apply_cc_projection(self, vel, vel_ng, gp, gp_ng, p_cc, p_ng, scaling_factor,
+                            density=None, rho_ng=0, const_rho=0.0, vel_old=None, vel_old_ng=0,
+                            incremental=False, inflow_vel=None, umac=(None, None, None),
+                            umac_ng=0, rtol=1e-11, atol=1e-14, mode=MODE_FP64):
+        """incflo::ApplyCCProjection on this projector (mgx_apply_cc_projection)."""
+        info = SolveInfo()
+        iv = None
+        if inflow_vel is not None:
+            iv = np.ascontiguousarray(inflow_vel, dtype=np.float64).reshape(18)
+        p = lambda a: a.ptr if a is not None else None
+        self.ctx.check(lib().mgx_apply_cc_projection(
+            self.h, p(density), rho_ng, const_rho, vel.ptr, vel_ng, p(vel_old), vel_old_ng,
+            gp.ptr, gp_ng, p_cc.ptr, p_ng, scaling_factor, int(incremental),
+            iv.ctypes.data if iv is not None else None, p(umac[0]), p(umac[1]), p(umac[2]),
+            umac_ng, rtol, atol, mode, C.byref(info)))
+        return info
 
     def getFluxes(self, fx, fy, fz, ng=0):
         self.ctx.check(lib().mgx_mac_get_fluxes(self.h, fx.ptr, fy.ptr, fz.ptr, ng))
@@ -554,6 +616,88 @@ class Advection:
     def close(self):
         if self.h:
             lib().mgx_adv_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def _ptr(a):
+    return a.ptr if a is not None else None
+
+
+def _d3(v):
+    return None if v is None else (C.c_double * 3)(*[float(x) for x in v])
+
+
+class Step:
+    """Predictor-step pieces on one level (mgx_step_*): periodic ghost
+    filling, forces, velocity and density updates."""
+
+    def __init__(self, ctx, level):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_step_create(ctx.h, C.byref(level), C.byref(self.h)))
+
+    def fill_boundary(self, field, ng, ncomp):
+        self.ctx.check(lib().mgx_fill_boundary(self.h, field.ptr, ng, ncomp))
+
+    def compute_vel_forces(self, gp, gp_ng, rho, rho_ng, forces, f_ng, divtau=None, divtau_ng=0,
+                           gravity=None, gp0=None, include_pressure_gradient=True):
+        self.ctx.check(lib().mgx_compute_vel_forces(
+            self.h, _ptr(gp), gp_ng, rho.ptr, rho_ng, _ptr(divtau), divtau_ng, _d3(gravity),
+            _d3(gp0), 1 if include_pressure_gradient else 0, forces.ptr, f_ng))
+
+    def update_velocity(self, vel, vel_ng, vel_old, vo_ng, conv, conv_ng, forces, f_ng, dt,
+                        divtau=None, divtau_ng=0, divtau_coef=0.0, rho_old=None, rho_new=None,
+                        rho_nph=None, rho_ng=0):
+        self.ctx.check(lib().mgx_update_velocity(
+            self.h, vel.ptr, vel_ng, vel_old.ptr, vo_ng, conv.ptr, conv_ng, forces.ptr, f_ng,
+            _ptr(divtau), divtau_ng, divtau_coef, _ptr(rho_old), _ptr(rho_new), _ptr(rho_nph),
+            rho_ng, dt))
+
+    def update_density(self, rho_old, ro_ng, drdt, c_ng, dt, rho_new, rn_ng, rho_nph=None,
+                       rh_ng=0):
+        self.ctx.check(lib().mgx_update_density(self.h, rho_old.ptr, ro_ng, drdt.ptr, c_ng, dt,
+                                                rho_new.ptr, rn_ng, _ptr(rho_nph), rh_ng))
+
+    def close(self):
+        if self.h:
+            lib().mgx_step_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Diffusion:
+    """Cell-centred viscous operator and Helmholtz solve (mgx_diff_*),
+    DiffusionScalarOp's component solves."""
+
+    def __init__(self, ctx, level):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        ctx.check(lib().mgx_diff_create(ctx.h, C.byref(level), C.byref(self.h)))
+
+    def setDomainBC(self, lo, hi, comp=-1):
+        l = (C.c_int * 3)(*lo)
+        u = (C.c_int * 3)(*hi)
+        self.ctx.check(lib().mgx_diff_set_domain_bc(self.h, comp, l, u))
+
+    def set_options(self, opts):
+        self.ctx.check(lib().mgx_diff_set_options(self.h, C.byref(opts)))
+
+    def set_eta(self, eta=None, ng=0, const_eta=0.0):
+        self.ctx.check(lib().mgx_diff_set_eta(self.h, _ptr(eta), ng, const_eta))
+
+    def apply(self, field, ng, ncomp, out, out_ng=0, rho=None, rho_ng=0):
+        self.ctx.check(lib().mgx_diff_apply(self.h, field.ptr, ng, ncomp, _ptr(rho), rho_ng,
+                                            out.ptr, out_ng))
+
+    def solve(self, field, ng, ncomp, dt, acoef=None, a_ng=0, rtol=1e-11, atol=1e-14,
+              mode=MODE_FP64):
+        info = SolveInfo()
+        self.ctx.check(lib().mgx_diff_solve(self.h, field.ptr, ng, ncomp, _ptr(acoef), a_ng, dt,
+                                            rtol, atol, mode, C.byref(info)))
+        return info
+
+    def close(self):
+        if self.h:
+            lib().mgx_diff_destroy(self.h)
             self.h = C.c_void_p()
 
 

@@ -175,6 +175,24 @@ MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on);
  * into FP16's range and computes in FP32.  The outer residual, the update
  * and the convergence test are FP64 either way. */
 MGX_API int mgx_nodal_set_vcycle_bits(mgx_nodal h, int bits);
+/* Launch-shape autotuner (no reference counterpart; AMReX picks fixed GPU
+ * launch sizes).  Times candidate launch shapes of the nodal stencil kernel
+ * (nodes per thread 1/2/4, z-run length through a target workgroup count)
+ * on every level of this handle's hierarchy for `mode` (FP64 or the FP32
+ * V-cycle of the mixed forms), for the Jacobi, first-sweep and residual
+ * passes and the level-0 FP64 residual / update passes, then the number of
+ * coarsest levels run inside the single-workgroup bottom launch (whole
+ * V-cycles timed), and keeps the fastest of each in a process-wide table
+ * keyed by level shape, storage and pass.  Needs sigma (one projection or
+ * set_sigma first); results of later solves are unchanged (launch shapes
+ * only).  report (may be NULL) receives one text line per tuned pass with
+ * the chosen shape and its time against the default's. */
+MGX_API int mgx_nodal_autotune(mgx_nodal h, int mode, char* report, int report_len);
+/* The tuning table as text ("stencil nn0 nn1 nn2 word cword pass fine npt
+ * target kzmin" lines and "coarse_nodes N"), for caching per device model;
+ * mgx_tuning_set loads such a text (replacing the table). */
+MGX_API int mgx_tuning_get(char* buf, int len);
+MGX_API int mgx_tuning_set(const char* text);
 /* Variable sigma, a boxed cell field with ng ghosts (:191, sigma = s/rho at :117). */
 MGX_API int mgx_nodal_set_sigma(mgx_nodal h, const double* sigma, int sigma_ng);
 /* Scalar sigma = s/rho_0 (:186-188). */
@@ -371,6 +389,29 @@ MGX_API int mgx_mac_project(mgx_mac h, double* umac, double* vmac, double* wmac,
                             mgx_solve_info* info);
 /* getFluxes (src/projection/incflo_apply_cc_projection.cpp:387-389): -beta grad phi on faces. */
 MGX_API int mgx_mac_get_fluxes(mgx_mac h, double* fx, double* fy, double* fz, int ng);
+/* incflo::ApplyCCProjection (src/projection/incflo_apply_cc_projection.cpp:199-487),
+ * the projection incflo runs instead of the nodal one when use_cc_proj is set
+ * (dispatch in incflo::ApplyProjection, src/incflo.cpp:213-231), on the MAC
+ * handle's geometry, boundary types and options (single level, no EB):
+ *   if !incremental: u += (s/rho) gp;  if incremental: u -= vel_old
+ *   ghost cells 0, faces marked MGX_BC_INFLOW the inflow_vel table (6x3,
+ *     [2*dir + side][comp], not when incremental)
+ *   u_mac = MOL::ExtrapVelToFaces(u) (limited-slope upwind face velocity)
+ *   beta = s / face-average(rho) (or s / const_rho), MAC projection of u_mac
+ *     from phi = 0, fluxes F = -beta grad phi
+ *   u += mean of the two fluxes per direction (average_mac_to_ccvel),
+ *   p_cc = phi, gp = -(that mean) rho / s (both += / -= when incremental);
+ *   if incremental: u += vel_old.
+ * density: boxed cells or NULL with const_rho > 0; vel: 3 comps, vel_ng >= 1;
+ * gp: 3 comps; p_cc: 1 comp; umac/vmac/wmac (may be NULL): the projected face
+ * velocities. */
+MGX_API int mgx_apply_cc_projection(mgx_mac h, const double* density, int rho_ng,
+                                    double const_rho, double* vel, int vel_ng,
+                                    const double* vel_old, int vel_old_ng, double* gp, int gp_ng,
+                                    double* p_cc, int p_ng, double scaling_factor, int incremental,
+                                    const double* inflow_vel, double* umac, double* vmac,
+                                    double* wmac, int umac_ng, double rtol, double atol, int mode,
+                                    mgx_solve_info* info);
 /* EB form (MacProjector with the EB factory, src/incflo.cpp:286-289; MLEBABecLap):
  * a cylinder along z (fluid outside).  Face apertures a_f weight the
  * coefficients (operator div(a beta grad)), the right-hand side is div(a u),
@@ -439,6 +480,76 @@ MGX_API int mgx_adv_scalar_term(mgx_adv h, const double* state, int state_ng, in
                                 int umac_ng, double dt, double* conv, int conv_ng, double* face_x,
                                 double* face_y, double* face_z, double* flux_x, double* flux_y,
                                 double* flux_z);
+
+/* ---- the predictor step around the projections (single level) -----------
+ * incflo::ApplyPredictor (src/incflo_apply_predictor.cpp) runs, between the
+ * MAC and the nodal projection, the forces, the convective term, the density
+ * and velocity updates and the viscous solve.  These entry points keep that
+ * work on the device, on the caller's boxed fields (layout as above). */
+typedef struct mgx_step_s* mgx_step;
+MGX_API int mgx_step_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_step* out);
+MGX_API int mgx_step_destroy(mgx_step h);
+/* MultiFab::FillBoundary(geom.periodicity()) of a boxed cell field with ng
+ * ghosts and ncomp components: every ghost cell that lies inside the domain
+ * or across a periodic face takes the valid value there; ghosts outside
+ * non-periodic faces are left as they are (the caller's physical BC fill,
+ * as incflo's fillpatch / fillphysbc). */
+MGX_API int mgx_fill_boundary(mgx_step h, double* field, int ng, int ncomp);
+/* incflo::compute_vel_forces (src/incflo_compute_forces.cpp:50-151, the
+ * non-Boussinesq branch :134-148): forces = -(gp + gp0)/rho + gravity
+ * (without gp when include_pressure_gradient == 0), plus divtau when given
+ * (the Godunov forcing, src/convection/incflo_compute_MAC_projected_velocities.cpp:24-57).
+ * gravity / gp0: 3 doubles or NULL (0). */
+MGX_API int mgx_compute_vel_forces(mgx_step h, const double* gp, int gp_ng, const double* rho,
+                                   int rho_ng, const double* divtau, int divtau_ng,
+                                   const double gravity[3], const double gp0[3],
+                                   int include_pressure_gradient, double* forces, int f_ng);
+/* incflo::update_velocity, predictor (src/incflo_update_velocity.cpp:14-143):
+ *   velocity form (rho_* NULL):  vel = vel_old + dt (conv + forces + c divtau)
+ *   momentum form:  vel = (rho_old vel_old + dt (conv + rho_nph forces + c divtau)) / rho_new
+ * with c = divtau_coef: 1 explicit, 0.5 Crank-Nicolson, 0 (or divtau NULL)
+ * implicit; the implicit and Crank-Nicolson forms are followed by
+ * mgx_diff_solve (:322-334). */
+MGX_API int mgx_update_velocity(mgx_step h, double* vel, int vel_ng, const double* vel_old,
+                                int vo_ng, const double* conv, int conv_ng, const double* forces,
+                                int f_ng, const double* divtau, int divtau_ng, double divtau_coef,
+                                const double* rho_old, const double* rho_new,
+                                const double* rho_nph, int rho_ng, double dt);
+/* incflo::update_density, predictor (src/incflo_update_density.cpp:14-64):
+ * rho_new = rho_old + dt drdt, rho_nph = (rho_old + rho_new)/2 (rho_nph may be NULL). */
+MGX_API int mgx_update_density(mgx_step h, const double* rho_old, int ro_ng, const double* drdt,
+                               int c_ng, double dt, double* rho_new, int rn_ng, double* rho_nph,
+                               int rh_ng);
+
+/* ---- cell-centred viscous operator and Helmholtz solve ---------------------
+ * DiffusionScalarOp (src/diffusion/DiffusionScalarOp.cpp): per component,
+ * MLABecLaplacian with alpha = 1, a = rho, beta = dt and b = eta averaged to
+ * faces (average_scalar_eta_to_faces, src/diffusion/incflo_diffusion.cpp:289-380).
+ * The same red-black Gauss-Seidel multigrid as the MAC projection, with the
+ * cell coefficient on every level (coarse a = mean of 8 fine cells).
+ * Dirichlet faces take their value from the field's first ghost cell
+ * (setLevelBC(lev, &vel), two-point face flux as mg_maxorder = 2). */
+typedef struct mgx_diff_s* mgx_diff;
+MGX_API int mgx_diff_create(mgx_ctx ctx, const mgx_level_desc* level, mgx_diff* out);
+MGX_API int mgx_diff_destroy(mgx_diff h);
+/* get_diffuse_velocity_bc / get_diffuse_scalar_bc (src/diffusion/incflo_diffusion.cpp:156-260):
+ * LinOpBCType per side for component comp (-1: all); periodic, Dirichlet or Neumann. */
+MGX_API int mgx_diff_set_domain_bc(mgx_diff h, int comp, const int lo[3], const int hi[3]);
+/* scalar_diffusion.* knobs (DiffusionScalarOp.cpp:130-146; maxiter 100). */
+MGX_API int mgx_diff_set_options(mgx_diff h, const mgx_mlmg_opts* opts);
+/* viscosity eta: boxed cell field (ng ghosts) or a positive constant when eta is NULL. */
+MGX_API int mgx_diff_set_eta(mgx_diff h, const double* eta, int ng, double const_eta);
+/* compute_divtau (scalar form, :584-760) / compute_laps (:482-580):
+ * out_c = div(eta grad field_c) [/ rho when rho is given], c < ncomp. */
+MGX_API int mgx_diff_apply(mgx_diff h, const double* field, int ng, int ncomp, const double* rho,
+                           int rho_ng, double* out, int out_ng);
+/* diffuse_vel_components / diffuse_scalar (:332-480, MLMG::solve to rtol, atol):
+ * every component c < ncomp solves (a - dt div(eta grad)) x = a field_c with
+ * a = acoef (boxed cell field) or 1 when NULL; field is the initial guess and
+ * receives x.  info sums iterations and V-cycles over the components. */
+MGX_API int mgx_diff_solve(mgx_diff h, double* field, int ng, int ncomp, const double* acoef,
+                           int a_ng, double dt, double rtol, double atol, int mode,
+                           mgx_solve_info* info);
 
 /* ---- learned coarse-level predictor --------------------------------------
  * No counterpart in the reference (incflo has no learned component); these

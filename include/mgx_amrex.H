@@ -474,6 +474,78 @@ class MacProjector {
     int mode_ = MGX_MODE_FP64;
 };
 
+// One level of incflo's LevelData (src/incflo.H:618-656) as the projections use it.
+struct ProjectionFields {
+    MultiFab* velocity = nullptr;          // 3 comps, >= 1 ghost
+    const MultiFab* velocity_o = nullptr;  // incremental projections
+    const MultiFab* density = nullptr;     // or const_rho > 0
+    double const_rho = 0.0;
+    MultiFab* gp = nullptr;                // 3 comps
+    MultiFab* p_nd = nullptr;              // nodal (nodal projection)
+    MultiFab* p_cc = nullptr;              // cells (cell-centred projection)
+};
+
+// incflo::ApplyProjection (src/incflo.cpp:213-231): with use_cc_proj the
+// cell-centred projection on incflo's persistent MAC projector
+// (ApplyCCProjection, src/projection/incflo_apply_cc_projection.cpp:199-487;
+// the projector is initialised with a constant beta and the MAC boundary
+// types when it needs it, :282-310), otherwise the nodal projection with a
+// per-call projector (ApplyNodalProjection, src/projection/incflo_apply_nodal_projection.cpp:29-267).
+// inflow: the 6x3 inflow velocities [2*dir + side][comp] or nullptr.
+inline mgx_solve_info ApplyProjection(Context& ctx, bool use_cc_proj, MacProjector& macproj,
+                                      const Geometry& geom, const ProjectionFields& f,
+                                      std::array<LinOpBCType, 3> nodal_lo,
+                                      std::array<LinOpBCType, 3> nodal_hi,
+                                      std::array<LinOpBCType, 3> mac_lo,
+                                      std::array<LinOpBCType, 3> mac_hi, double scaling_factor,
+                                      bool incremental, const double* inflow, double rtol,
+                                      double atol, int mode = MGX_MODE_FP64) {
+    mgx_solve_info info{};
+    if (!f.velocity || !f.gp || (!f.density && !(f.const_rho > 0.0)))
+        Abort("ApplyProjection: velocity, gp and a density are required");
+    const double* rho = f.density ? f.density->p : nullptr;
+    const int rho_ng = f.density ? f.density->ngrow : 0;
+    const double* vo = incremental && f.velocity_o ? f.velocity_o->p : nullptr;
+    const int vo_ng = f.velocity_o ? f.velocity_o->ngrow : 0;
+    if (use_cc_proj) {
+        if (!f.p_cc) Abort("ApplyProjection: the cell-centred projection writes p_cc");
+        if (macproj.needInitialization()) {
+            macproj.initProjector(*f.velocity->ba, LPInfo(), scaling_factor);
+            macproj.setDomainBC(mac_lo, mac_hi);
+        }
+        ctx.check(mgx_apply_cc_projection(macproj.handle(), rho, rho_ng, f.const_rho,
+                                          f.velocity->p, f.velocity->ngrow, vo, vo_ng, f.gp->p,
+                                          f.gp->ngrow, f.p_cc->p, f.p_cc->ngrow, scaling_factor,
+                                          incremental ? 1 : 0, inflow, nullptr, nullptr, nullptr,
+                                          0, rtol, atol, mode, &info),
+                  "ApplyCCProjection");
+        return info;
+    }
+    if (!f.p_nd) Abort("ApplyProjection: the nodal projection writes p_nd");
+    const std::vector<int> flat = f.velocity->ba->flat();
+    mgx_level_desc d = detail::level_desc(geom, flat);
+    mgx_nodal h = nullptr;
+    ctx.check(mgx_nodal_create(ctx.get(), &d, &h), "NodalProjector");
+    int l[3], r[3];
+    for (int k = 0; k < 3; ++k) {
+        l[k] = (int)nodal_lo[k];
+        r[k] = (int)nodal_hi[k];
+    }
+    int st = mgx_nodal_set_domain_bc(h, l, r);
+    if (st == MGX_OK)
+        st = mgx_apply_nodal_projection(h, rho, rho_ng, f.const_rho, f.velocity->p,
+                                        f.velocity->ngrow, vo, vo_ng, f.gp->p, f.gp->ngrow,
+                                        f.p_nd->p, f.p_nd->ngrow, scaling_factor,
+                                        incremental ? 1 : 0, inflow, rtol, atol, mode, &info);
+    if (st != MGX_OK) {
+        const std::string msg = mgx_ctx_last_error(ctx.get());
+        mgx_nodal_destroy(h);
+        Abort("ApplyNodalProjection: " + msg);
+    }
+    mgx_nodal_destroy(h);
+    return info;
+}
+
 // amrex::MLMG driving one of the two operators directly (setMaxIter, ...,
 // solve(sol, rhs, rtol, atol)); the operator's coefficients and BCs come from
 // the projector it wraps.

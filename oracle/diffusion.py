@@ -24,9 +24,9 @@ The discretisation restated here (this build's own, second order):
     F_hi = b_f (g - u_{n-1}) / (h/2)
     F = 0                                Neumann face (homogeneous)
     (L u)_c = sum_d (F_hi - F_lo) / h_d
-AMReX's MLABecLaplacian extrapolates Dirichlet faces with more cells (its
-maxorder), which this build does not pin: parity unpinned at Dirichlet faces;
-periodic and Neumann faces are the same 7-point operator.
+incflo runs these solves with mg_maxorder = 2 (src/diffusion/DiffusionScalarOp.H:65),
+i.e. the ghost value is extrapolated linearly from the face value and the
+first interior cell, 2 g - u_0, which is the two-point Dirichlet flux above.
 
 Boundary values follow incflo's convention that a Dirichlet face value lives
 in the ghost cell outside it (setLevelBC(lev, &phi), DiffusionScalarOp.cpp:446):

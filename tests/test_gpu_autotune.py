@@ -1,0 +1,76 @@
+"""The launch-shape autotuner (mgx_nodal_autotune): launch shapes do not
+change any node's arithmetic, so projections after tuning are bit-identical
+to those before (within round-off when the tuner folds more coarse levels
+into the single-workgroup bottom launch); the
+table round-trips through mgx_tuning_get / mgx_tuning_set; the tuned shapes
+are not slower than the defaults (measured inside the tuner)."""
+import sys
+
+import numpy as np
+import pytest
+
+from incflo_amd import boxes as B
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from incflo_amd import mgx
+    c = mgx.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_autotune_keeps_results_bitwise(ctx, mode):
+    from incflo_amd import mgx, workloads
+    n = 64
+    _say(f"case autotune n={n} mode={mode}")
+    rho, ustar, dt = workloads.variable_density_case(n, 7)
+    bxs = B.chop_domain((n, n, n), 32)
+    P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))
+    rd = ctx.array(B.to_boxed(rho, bxs, 0))
+    saved = mgx.tuning_get()
+
+    def run():
+        vel = ctx.array(B.to_boxed(ustar, bxs, 1))
+        gp = ctx.array(B.field_size(bxs, 0, 3))
+        p = ctx.array(B.field_size(bxs, 0, 1, True))
+        info = P.apply_nodal_projection(vel, 1, gp, 0, p, 0, dt, density=rd, rtol=1e-10,
+                                        mode=mode)
+        return p.download(), vel.download(), info
+
+    p0, v0, i0 = run()
+    rep = P.autotune(mode)
+    _say(rep)
+    lines = [l for l in rep.splitlines() if l.startswith("stencil")]
+    assert lines and any(l.startswith("coarse_nodes") for l in rep.splitlines())
+    for l in lines:
+        f = l.split()
+        us, dus = float(f[f.index("us") + 1]), float(f[f.index("default_us") + 1])
+        assert us <= dus * 1.0001, l
+    p1, v1, i1 = run()
+    table = mgx.tuning_get()
+    if "coarse_nodes 1\n" in table:
+        # launch shapes only: every node is computed by the same arithmetic
+        assert i1.vcycles == i0.vcycles
+        assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+    else:
+        # more levels in the single-workgroup bottom launch: same cycle,
+        # different rounding
+        assert abs(i1.vcycles - i0.vcycles) <= 1 and i1.converged
+        assert np.max(np.abs(p1 - p0)) <= 1e-8 * np.max(np.abs(p0))
+    assert "coarse_nodes" in table
+    mgx.tuning_set(saved)
+    assert mgx.tuning_get() == saved
+    mgx.tuning_set(table)
+    assert mgx.tuning_get() == table
+    with pytest.raises(mgx.MgxError):
+        mgx.tuning_set("stencil 1 2")
+    P.close()
