@@ -2,10 +2,13 @@
 // the cell-centred viscous operator and Helmholtz solve (incflo's
 // DiffusionScalarOp on the MAC hierarchy with a cell coefficient), forces,
 // the velocity / density updates and periodic ghost filling.
+#include <algorithm>
 #include <array>
 #include <cmath>
+#include <string>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "mac_mg.h"
@@ -60,6 +63,34 @@ struct GrownTiles {
         return it->second.first.as<Tile>();
     }
 };
+
+// ---- fill_boundary without allocation ----------------------------------------
+// A fixed device array stages the canonical copy of the field, so
+// mgx_fill_boundary_noalloc allocates nothing.  Each GPU the library runs on
+// holds its own copy of the array until the process ends; every call on one
+// GPU shares it (calls that overlap on one GPU race).
+constexpr long kFillScratchDoubles = 8L << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kFillScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mtx;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// this GPU's address of `scratch`, looked up once per device with the
+// device current (the array's name is only a host-side stand-in)
+double* scratch_ptr() {
+    int dev = 0;
+    MGX_HIP(hipGetDevice(&dev));
+    MGX_REQUIRE(dev >= 0 && dev < kMaxDevices, MGX_ERR_UNSUPPORTED,
+                "fill_boundary_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        MGX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
 
 }  // namespace
 
@@ -196,6 +227,34 @@ MGX_API int mgx_fill_boundary(mgx_step h, double* field, int ng, int ncomp) {
         const Tile* t = h->grown.get(h->layout.boxes, ng, nt);
         fill_ghosts_kernel<<<nt, 256, 0, h->stream()>>>(k, h->layout.dev_boxes(), t, field, ng, ncomp,
                                                         h->canon.as<double>());
+    });
+}
+
+MGX_API int mgx_fill_boundary_noalloc(mgx_step h, double* field, int ng, int ncomp) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        MGX_REQUIRE(field && ng >= 0 && ncomp >= 1, MGX_ERR_ARG, "fill_boundary_noalloc: bad field");
+        const long nc = (long)h->g.n[0] * h->g.n[1] * h->g.n[2];
+        MGX_REQUIRE((long)ncomp * nc <= kFillScratchDoubles, MGX_ERR_ARG,
+                    "fill_boundary_noalloc: ncomp x cells = " + std::to_string((long)ncomp * nc) +
+                        " exceeds the fixed scratch of " + std::to_string(kFillScratchDoubles) +
+                        " doubles (use mgx_fill_boundary)");
+        if (ng == 0) return;
+        double* canon = scratch_ptr();
+        const KGeom k = cell_kgeom(h->g);
+        canon_all_kernel<<<h->ntile(), 256, 0, h->stream()>>>(
+            k, h->layout.dev_boxes(), h->layout.tiles(REG_CELL), field, ng, ncomp, canon);
+        // grid (chunk, box) over the grown boxes: no tile table to allocate
+        long vmax = 0;
+        const std::vector<int>& b6 = h->layout.boxes;
+        for (int b = 0; b < h->layout.nboxes; ++b) {
+            long vol = 1;
+            for (int d = 0; d < 3; ++d) vol *= (long)(b6[6 * b + 3 + d] - b6[6 * b + d] + 1 + 2 * ng);
+            vmax = std::max(vmax, vol);
+        }
+        const dim3 grid((unsigned)((vmax + 255) / 256), (unsigned)h->layout.nboxes);
+        fill_ghosts_boxgrid_kernel<<<grid, 256, 0, h->stream()>>>(k, h->layout.dev_boxes(), field, ng,
+                                                                  ncomp, canon);
     });
 }
 

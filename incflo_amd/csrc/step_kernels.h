@@ -108,6 +108,36 @@ static __global__ void __launch_bounds__(256) fill_ghosts_kernel(KGeom g, const 
     for (int q = 0; q < ncomp; ++q) boxed[base + box_index(b, ng, false, p[0], p[1], p[2], q)] = canon[q * nc + c];
 }
 
+// the same fill with a (chunk, box) grid: blockIdx.y is the box, blockIdx.x
+// a 256-cell chunk of its grown region (chunks past a smaller box return)
+static __global__ void __launch_bounds__(256) fill_ghosts_boxgrid_kernel(
+    KGeom g, const BoxInfo* boxes, double* __restrict__ boxed, int ng, int ncomp,
+    const double* __restrict__ canon) {
+    const BoxInfo& b = boxes[blockIdx.y];
+    int len[3], lo[3];
+    for (int d = 0; d < 3; ++d) { lo[d] = b.lo[d] - ng; len[d] = b.len[d] + 2 * ng; }
+    const long lin = (long)blockIdx.x * 256 + threadIdx.x;
+    if (lin >= (long)len[0] * len[1] * len[2]) return;
+    const int qq = (int)(lin / len[0]);
+    int p[3] = {lo[0] + (int)(lin - (long)qq * len[0]), lo[1] + qq % len[1], lo[2] + qq / len[1]};
+    bool valid = true;
+    for (int d = 0; d < 3; ++d)
+        if (p[d] < b.lo[d] || p[d] >= b.lo[d] + b.len[d]) valid = false;
+    if (valid) return;
+    int w[3];
+    for (int d = 0; d < 3; ++d) {
+        w[d] = p[d];
+        if (w[d] < 0 || w[d] >= g.n[d]) {
+            if (!g.per[d]) return;  // physical boundary ghost: not ours
+            w[d] = ((w[d] % g.n[d]) + g.n[d]) % g.n[d];
+        }
+    }
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    const long c = w[0] + (long)g.n[0] * (w[1] + (long)g.n[1] * w[2]);
+    const long base = field_offset(b, ng, ncomp, false);
+    for (int q = 0; q < ncomp; ++q) boxed[base + box_index(b, ng, false, p[0], p[1], p[2], q)] = canon[q * nc + c];
+}
+
 // ---- predictor step pieces (boxed, valid cells, 3 components) -------------
 struct ForceArgs {
     const BoxInfo* boxes;

@@ -169,6 +169,7 @@ def lib():
             "mgx_step_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
             "mgx_step_destroy": [vp],
             "mgx_fill_boundary": [vp, vp, C.c_int, C.c_int],
+            "mgx_fill_boundary_noalloc": [vp, vp, C.c_int, C.c_int],
             "mgx_compute_vel_forces": [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int,
                                        vp, C.c_int],
             "mgx_update_velocity": [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp,
@@ -638,6 +639,9 @@ class Step:
 
     def fill_boundary(self, field, ng, ncomp):
         self.ctx.check(lib().mgx_fill_boundary(self.h, field.ptr, ng, ncomp))
+
+    def fill_boundary_noalloc(self, field, ng, ncomp):
+        self.ctx.check(lib().mgx_fill_boundary_noalloc(self.h, field.ptr, ng, ncomp))
 
     def compute_vel_forces(self, gp, gp_ng, rho, rho_ng, forces, f_ng, divtau=None, divtau_ng=0,
                            gravity=None, gp0=None, include_pressure_gradient=True):

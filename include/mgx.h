@@ -495,6 +495,14 @@ MGX_API int mgx_step_destroy(mgx_step h);
  * non-periodic faces are left as they are (the caller's physical BC fill,
  * as incflo's fillpatch / fillphysbc). */
 MGX_API int mgx_fill_boundary(mgx_step h, double* field, int ng, int ncomp);
+/* mgx_fill_boundary without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight stepping loops, harnesses that forbid hipMalloc.  The
+ * canonical staging copy lives in a fixed 64 MiB device array that each GPU
+ * holds until the process ends, so ncomp x cells must be at most 8,388,608
+ * (MGX_ERR_ARG otherwise).  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (several streams or host threads) race. */
+MGX_API int mgx_fill_boundary_noalloc(mgx_step h, double* field, int ng, int ncomp);
 /* incflo::compute_vel_forces (src/incflo_compute_forces.cpp:50-151, the
  * non-Boussinesq branch :134-148): forces = -(gp + gp0)/rho + gravity
  * (without gp when include_pressure_gradient == 0), plus divtau when given
