@@ -84,10 +84,12 @@ class Case:
             a.free()
 
 
-def run_mode(ctx, P, case, mode, steps, warmup, rtol, timed_repeat=True):
+def run_mode(ctx, P, case, mode, steps, warmup, rtol, timed_repeat=True, tune=False):
     """warmup + steps projections; per-step HIP-event times of the projection
     call, the host-clock total of the steps (restore copy included), and an
-    instrumented repeat that times the finest-level sweeps"""
+    instrumented repeat that times the finest-level sweeps.  With tune, the
+    launch shapes are autotuned after the warm-up (then one more warm-up
+    captures the V-cycle graphs with the tuned shapes)."""
     from incflo_amd import mgx
     c = case
     use_stream_copy = c.vel.nbytes % 16 == 0
@@ -104,6 +106,11 @@ def run_mode(ctx, P, case, mode, steps, warmup, rtol, timed_repeat=True):
                                         rho_ng=0, rtol=rtol, atol=0.0, mode=mode)
 
     for _ in range(warmup):
+        restore()
+        project()
+    report = None
+    if tune:
+        report = P.autotune(mode)
         restore()
         project()
     ctx.sync()
@@ -130,7 +137,7 @@ def run_mode(ctx, P, case, mode, steps, warmup, rtol, timed_repeat=True):
         ctx.sync()
         ms, cnt = P.get_timing()
         P.set_timing(False)
-    return (t1 - t0), infos, ev, ms, cnt
+    return (t1 - t0), infos, ev, ms, cnt, report
 
 
 def measured_traffic(kernel_key, n):
@@ -240,6 +247,8 @@ def main():
                     help="FP32 V-cycles per FP64 refinement step in the mixed forms "
                          "(0: the library default)")
     ap.add_argument("--cpu-n", type=int, default=96, help="size of the CPU-baseline sample")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (no mgx_nodal_autotune)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-oracle-solve", action="store_true")
@@ -264,7 +273,7 @@ def main():
     nnodes = n ** 3
 
     modes = {"fp64": mgx.MODE_FP64, "mixed": mgx.MODE_MIXED, "learned": mgx.MODE_LEARNED}
-    results, phis, vels = {}, {}, {}
+    results, phis, vels, tuned = {}, {}, {}, {}
     net = None
     mode_names = args.modes.split(",")
     if "learned" in mode_names:
@@ -285,7 +294,10 @@ def main():
     for name in mode_names:
         mode = modes[name]
         log(f"bench: mode {name}: warmup {args.warmup}, steps {args.steps}")
-        t, infos, ev, ms, cnt = run_mode(ctx, P, case, mode, args.steps, args.warmup, args.rtol)
+        t, infos, ev, ms, cnt, rep = run_mode(ctx, P, case, mode, args.steps, args.warmup,
+                                              args.rtol, tune=not args.no_tune)
+        if rep:
+            tuned[name] = rep.splitlines()
         # stored word of the finest-level V-cycle arrays (x, b, sigma)
         word = 8 if mode == mgx.MODE_FP64 else args.vcycle_bits // 8
         avg_ms = ms / max(cnt, 1)
@@ -339,8 +351,8 @@ def main():
         case = Case(ctx, n, seed, args.max_grid_size, args.cache)
         per_seed[seed] = {}
         for name in mode_names:
-            t, infos, ev, _, _ = run_mode(ctx, P, case, modes[name], 5, 1, args.rtol,
-                                          timed_repeat=False)
+            t, infos, ev, _, _, _ = run_mode(ctx, P, case, modes[name], 5, 1, args.rtol,
+                                             timed_repeat=False)
             per_seed[seed][name] = {"ms_event_median": statistics.median(ev),
                                     "vcycles": infos[-1].vcycles,
                                     "fallback_used": infos[-1].fallback_used,
@@ -400,6 +412,7 @@ def main():
         "seeds": per_seed,
         "seed_speedup_vs_fp64": seed_ratio,
         "check": check,
+        "tuning": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

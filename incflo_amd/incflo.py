@@ -66,6 +66,7 @@ class DeviceIncflo:
         self.diff.set_eta(const_eta=mu)
         self.time = 0.0
         self.nstep = 0
+        self.dt_last = -1.0
         self.solves = dict(mac=0, nodal=0, diff=0)
         self.step.fill_boundary(self.vel, NG_STATE, 3)
 
@@ -118,6 +119,7 @@ class DeviceIncflo:
         self.predictor(dt, incremental=False)
         self.time += dt
         self.nstep += 1
+        self.dt_last = dt
 
     def run(self, stop_time, dt, initial_iterations):
         self.initial_projection()
@@ -125,6 +127,51 @@ class DeviceIncflo:
         while not (self.time >= stop_time - 1e-12 * dt):
             self.advance(dt)
         self.ctx.sync()
+
+    # ---- output and restart (src/utilities/io.cpp) ---------------------------
+    def write_checkpoint(self, path):
+        """incflo::WriteCheckPointFile (:70-111): Header, then velocity,
+        density, gradp and the pressure of the projection in use as VisMF
+        multifabs under Level_0.  The reference writes p_cc when it runs the
+        nodal projection and p_nd with the cell-centred one (:99-105), so a
+        restart loses the pressure the run used; this build writes the
+        nodal-projection pressure as p_nd."""
+        mgx.write_checkpoint_header(path, [self.lev], self.nstep, self.time, self.dt_last,
+                                    self.dt_last, self.dt_last)
+        pre = path + "/Level_0/"
+        mgx.write_multifab(self.ctx, pre + "velocity", self.lev, self.vel, NG_STATE, 3)
+        mgx.write_multifab(self.ctx, pre + "density", self.lev, self.rho, 0, 1)
+        mgx.write_multifab(self.ctx, pre + "gradp", self.lev, self.gp, 0, 3)
+        mgx.write_multifab(self.ctx, pre + "p_nd", self.lev, self.p, 0, 1, nodal=True)
+
+    def restart(self, path):
+        """incflo::ReadCheckpointFile (:113-239) on this level's box list."""
+        h = mgx.read_checkpoint_header(path)
+        mine = [(tuple(lo), tuple(hi)) for lo, hi in self.bxs]
+        if h["finest_level"] != 0 or h["boxes"][0] != mine:
+            raise mgx.MgxError(mgx.MGX_ERR_ARG, "restart: the checkpoint's boxes differ")
+        pre = path + "/Level_0/"
+        mgx.read_multifab(self.ctx, pre + "velocity", self.lev, self.vel, NG_STATE, 3)
+        mgx.read_multifab(self.ctx, pre + "density", self.lev, self.rho, 0, 1)
+        mgx.read_multifab(self.ctx, pre + "gradp", self.lev, self.gp, 0, 3)
+        mgx.read_multifab(self.ctx, pre + "p_nd", self.lev, self.p, 0, 1, nodal=True)
+        self.nstep, self.time, self.dt_last = h["nstep"], h["cur_time"], h["dt"]
+
+    def write_plotfile(self, path):
+        """incflo::WritePlotFile's default variables velx, vely, velz, gpx,
+        gpy, gpz, density (src/utilities/io.cpp:283-460), as one 7-component
+        cell multifab."""
+        n = self.n
+        shape = (n[2], n[1], n[0])
+        v = self.velocity()
+        gp = B.from_boxed(self.gp.download(), self.bxs, 0, shape, ncomp=3)
+        rho = B.from_boxed(self.rho.download(), self.bxs, 0, shape)
+        data = np.concatenate([v, gp, rho[None]])
+        d = self.ctx.array(B.to_boxed(data, self.bxs, 0))
+        mgx.write_plotfile(self.ctx, path, [self.lev], [d], 0,
+                           ["velx", "vely", "velz", "gpx", "gpy", "gpz", "density"], self.time,
+                           level_steps=[self.nstep])
+        d.free()
 
     # ---- results (host copies, for checking) -------------------------------
     def velocity(self):

@@ -166,6 +166,19 @@ def lib():
             "mgx_nodal_autotune": [vp, C.c_int, C.c_char_p, C.c_int],
             "mgx_tuning_get": [C.c_char_p, C.c_int],
             "mgx_tuning_set": [C.c_char_p],
+            "mgx_write_multifab": [vp, C.c_char_p, C.POINTER(LevelDesc), vp, C.c_int, C.c_int,
+                                   C.c_int],
+            "mgx_read_multifab": [vp, C.c_char_p, C.POINTER(LevelDesc), vp, C.c_int, C.c_int,
+                                  C.c_int],
+            "mgx_write_checkpoint_header": [C.c_char_p, C.c_int, C.POINTER(LevelDesc), C.c_int,
+                                            C.c_double, C.c_double, C.c_double, C.c_double],
+            "mgx_read_checkpoint_header": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int,
+                                           C.POINTER(C.c_int), C.c_int],
+            "mgx_write_plotfile": [vp, C.c_char_p, C.c_int, C.POINTER(LevelDesc), C.POINTER(vp),
+                                   C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_double,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "mgx_step_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
             "mgx_step_destroy": [vp],
             "mgx_fill_boundary": [vp, vp, C.c_int, C.c_int],
@@ -703,6 +716,59 @@ class Diffusion:
         if self.h:
             lib().mgx_diff_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def write_multifab(ctx, prefix, level, field, ng, ncomp, nodal=False):
+    ctx.check(lib().mgx_write_multifab(ctx.h, prefix.encode(), C.byref(level), field.ptr, ng,
+                                       ncomp, int(nodal)))
+
+
+def read_multifab(ctx, prefix, level, field, ng, ncomp, nodal=False):
+    ctx.check(lib().mgx_read_multifab(ctx.h, prefix.encode(), C.byref(level), field.ptr, ng,
+                                      ncomp, int(nodal)))
+
+
+def write_checkpoint_header(path, levels, nstep, cur_time, dt, prev_dt, prev_prev_dt):
+    arr = (LevelDesc * len(levels))(*levels)
+    st = lib().mgx_write_checkpoint_header(path.encode(), len(levels), arr, nstep, cur_time, dt,
+                                           prev_dt, prev_prev_dt)
+    if st != MGX_OK:
+        raise MgxError(st, f"mgx_write_checkpoint_header({path}) failed")
+
+
+def read_checkpoint_header(path, max_levels=8, max_boxes=1 << 16):
+    fl, ns = C.c_int(), C.c_int()
+    times = (C.c_double * 4)()
+    lo, hi = (C.c_double * 3)(), (C.c_double * 3)()
+    nb = (C.c_int * max_levels)()
+    bx = (C.c_int * (6 * max_boxes))()
+    st = lib().mgx_read_checkpoint_header(path.encode(), C.byref(fl), C.byref(ns), times, lo, hi,
+                                          nb, max_levels, bx, max_boxes)
+    if st != MGX_OK:
+        raise MgxError(st, f"mgx_read_checkpoint_header({path}) failed")
+    boxes, used = [], 0
+    for l in range(fl.value + 1):
+        lv = []
+        for b in range(nb[l]):
+            v = bx[6 * (used + b): 6 * (used + b) + 6]
+            lv.append((tuple(v[0:3]), tuple(v[3:6])))
+        used += nb[l]
+        boxes.append(lv)
+    return {"finest_level": fl.value, "nstep": ns.value, "cur_time": times[0], "dt": times[1],
+            "prev_dt": times[2], "prev_prev_dt": times[3], "prob_lo": list(lo),
+            "prob_hi": list(hi), "boxes": boxes}
+
+
+def write_plotfile(ctx, path, levels, data, ng, varnames, time, level_steps=None,
+                   ref_ratio=None):
+    nl = len(levels)
+    arr = (LevelDesc * nl)(*levels)
+    ptrs = (C.c_void_p * nl)(*[d.ptr for d in data])
+    names = (C.c_char_p * len(varnames))(*[v.encode() for v in varnames])
+    ls = None if level_steps is None else (C.c_int * nl)(*level_steps)
+    rr = None if ref_ratio is None else (C.c_int * max(nl - 1, 1))(*ref_ratio)
+    ctx.check(lib().mgx_write_plotfile(ctx.h, path.encode(), nl, arr, ptrs, ng, len(varnames),
+                                       names, time, ls, rr))
 
 
 def eb_cylinder_geometry(level, cx, cy, r):

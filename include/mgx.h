@@ -559,6 +559,39 @@ MGX_API int mgx_diff_solve(mgx_diff h, double* field, int ng, int ncomp, const d
                            int a_ng, double dt, double rtol, double atol, int mode,
                            mgx_solve_info* info);
 
+/* ---- AMReX-format output and restart ---------------------------------------
+ * incflo writes plotfiles and checkpoints through AMReX
+ * (src/utilities/io.cpp:16-111 and WritePlotFile); these write the same
+ * formats from device fields (copied to the host first).
+ * VisMF::Write of one boxed field: <prefix>_H and <prefix>_D_00000, fabs over
+ * the boxes grown by ng, cell or nodal (nodal != 0) boxes, ncomp components. */
+MGX_API int mgx_write_multifab(mgx_ctx ctx, const char* prefix, const mgx_level_desc* level,
+                               const double* field, int ng, int ncomp, int nodal);
+/* VisMF::Read into a boxed device field of the same box list (valid cells and
+ * the ghosts both sides have); MGX_ERR_ARG when the boxes differ. */
+MGX_API int mgx_read_multifab(mgx_ctx ctx, const char* prefix, const mgx_level_desc* level,
+                              double* field, int ng, int ncomp, int nodal);
+/* incflo::WriteHeader for a checkpoint (src/utilities/io.cpp:16-68):
+ * "Checkpoint version: 1", finest level, step, time, dt, prev_dt,
+ * prev_prev_dt, prob_lo, prob_hi, one BoxArray per level; creates dir. */
+MGX_API int mgx_write_checkpoint_header(const char* dir, int nlevels, const mgx_level_desc* levels,
+                                        int nstep, double cur_time, double dt, double prev_dt,
+                                        double prev_prev_dt);
+/* incflo::ReadCheckpointFile's header part (:113-205): times = {cur_time,
+ * dt, prev_dt, prev_prev_dt}; per level its box count and the boxes (6 ints
+ * each, levels back to back, at most max_boxes in all). */
+MGX_API int mgx_read_checkpoint_header(const char* dir, int* finest_level, int* nstep,
+                                       double* times, double* prob_lo, double* prob_hi,
+                                       int* nboxes_per_level, int max_levels, int* boxes,
+                                       int max_boxes);
+/* amrex::WriteMultiLevelPlotfile ("HyperCLaw-V1.1" Header, Level_<l>/Cell
+ * multifabs): data[l] is level l's boxed cell field with nvars components
+ * and ng ghosts; level_steps / ref_ratio may be NULL (0 / 2). */
+MGX_API int mgx_write_plotfile(mgx_ctx ctx, const char* dir, int nlevels,
+                               const mgx_level_desc* levels, const double* const* data, int ng,
+                               int nvars, const char* const* varnames, double time,
+                               const int* level_steps, const int* ref_ratio);
+
 /* ---- learned coarse-level predictor --------------------------------------
  * No counterpart in the reference (incflo has no learned component); these
  * entry points serve MGX_MODE_LEARNED of the nodal solve.  A 3-D conv U-net
