@@ -108,6 +108,48 @@ def summarize(dirs, copy_bytes):
     return "\n".join(out)
 
 
+def size_sweep(sizes):
+    """Finest-level Jacobi sweep time against the problem size: a working set
+    far beyond the 256 MB Infinity Cache (512^3 FP32: 4 arrays of 537 MB)
+    shows whether the 256^3 rate is a cache effect.  The sweep is timed by the
+    autotuner (2 warm-up + 5 back-to-back launches, default launch shape) on
+    a constant-sigma problem; the copy kernel over the same bytes is timed
+    alongside."""
+    from incflo_amd import boxes as B
+    from incflo_amd import mgx
+    ctx = mgx.Context(0)
+    for n in sizes:
+        bxs = B.chop_domain((n, n, n), 64)
+        for mode, word, name in ((mgx.MODE_MIXED, 4, "FP32"), (mgx.MODE_FP64, 8, "FP64")):
+            P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))
+            P.set_const_sigma(1.0)
+            mgx.tuning_set("coarse_nodes 1\n")  # default shapes for every pass
+            rep = P.autotune(mode)
+            line = next(l for l in rep.splitlines()
+                        if l.startswith(f"stencil {n} {n} {n} {word} {word} 0 1 "))
+            f_ = line.split()
+            us = float(f_[f_.index("default_us") + 1])
+            nbytes = 4 * word * n ** 3
+            print(f"size {n}^3 {name} fine Jacobi {us:.2f} us  {nbytes / us / 1e3:.1f} GB/s "
+                  f"({nbytes / 1e6:.0f} MB algorithmic)", flush=True)
+            P.close()
+        # the copy kernel moving the sweep's FP32 bytes (2 arrays read+written of 2 n^3 words)
+        nb = 2 * 4 * n ** 3
+        a, b = ctx.array(nb // 8), ctx.array(nb // 8)
+        for _ in range(2):
+            ctx.check(mgx.lib().mgx_stream_copy(ctx.h, b.ptr, a.ptr, nb))
+        ctx.record(0)
+        for _ in range(5):
+            ctx.check(mgx.lib().mgx_stream_copy(ctx.h, b.ptr, a.ptr, nb))
+        ctx.record(1)
+        ms = ctx.elapsed_ms(0, 1) / 5
+        print(f"size {n}^3 stream copy of {nb / 1e6:.0f} MB: {1e3 * ms:.2f} us  "
+              f"{2 * nb / ms / 1e6:.1f} GB/s (read + written)", flush=True)
+        a.free()
+        b.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=256)
@@ -117,7 +159,11 @@ def main():
     ap.add_argument("--copies", type=int, default=4)
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--summarize", nargs="+")
+    ap.add_argument("--size-sweep", nargs="+", type=int)
     args = ap.parse_args()
+    if args.size_sweep:
+        size_sweep(args.size_sweep)
+        return
     if args.summarize:
         print(summarize(args.summarize, args.copy_mib << 20))
     else:
