@@ -225,9 +225,56 @@ def oracle_check(case, phis, vels):
     return out
 
 
+def tgv_workload(args):
+    """The Taylor-Green vortex on 64^3 cells (unit cube, rho = 1, mu = 1e-3,
+    max_grid_size 32): K incflo steps of the device predictor (Godunov
+    extrapolation, MAC projection, convective term, implicit viscous solves,
+    nodal projection; MLMG to 1e-10) after W untimed ones, each step timed
+    with HIP events on the context's stream.  A secondary line, not the
+    headline metric."""
+    from incflo_amd import mgx
+    from incflo_amd.incflo import DeviceIncflo
+    n = args.n if args.n != 256 else 64
+    h = 1.0 / n
+    c = (np.arange(n) + 0.5) * h
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    tp = 2 * np.pi
+    vel = np.stack([np.sin(tp * x) * np.cos(tp * y) * np.cos(tp * z),
+                    -np.cos(tp * x) * np.sin(tp * y) * np.cos(tp * z), np.zeros_like(x)])
+    ctx = mgx.Context(0)
+    I = DeviceIncflo(ctx, (n, n, n), (1.0, 1.0, 1.0), 1e-3, vel, max_grid_size=32,
+                     mac_rtol=1e-10, nodal_rtol=1e-10, diff_rtol=1e-10)
+    dt = 0.5 * h  # CFL 0.5 at max |u| = 1
+    I.initial_projection()
+    for _ in range(args.warmup):
+        I.advance(dt)
+    ctx.sync()
+    ev = []
+    solves0 = dict(I.solves)
+    for _ in range(args.steps):
+        ctx.record(0)
+        I.advance(dt)
+        ctx.record(1)
+        ev.append(ctx.elapsed_ms(0, 1))
+    ctx.sync()
+    med = statistics.median(ev)
+    per = {k: (I.solves[k] - solves0[k]) / args.steps for k in I.solves}
+    print(json.dumps({"metric": f"incflo steps/s at {n}^3 (Taylor-Green, device predictor)",
+                      "value": 1e3 / med, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": med, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                      "data": "synthetic", "config": {"workload": f"taylor-green {n}^3, rho 1, "
+                                                      f"mu 1e-3, max_grid_size 32, dt {dt:g}",
+                                                      "solves_per_step": per},
+                      "ms_event_median": med, "ms_event_min": min(ev)}), flush=True)
+    I.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="projection", choices=("projection", "tgv"),
+                    help="projection: the 256^3 headline; tgv: the 64^3 Taylor-Green step")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
@@ -259,6 +306,10 @@ def main():
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design only", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "tgv":
+        tgv_workload(args)
+        return
 
     from incflo_amd import mgx
 

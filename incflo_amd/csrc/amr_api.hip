@@ -63,6 +63,9 @@ static NodalGeom refine_geom(const NodalGeom& c) {
     return f;
 }
 
+// block of the point kernels launched over a TileSet's blocks
+static dim3 tile_block() { return dim3(TileSet::kTX, TileSet::kTY, 1); }
+
 static bool is_dir_host(const NodalGeom& g, int i, int j, int k) {
     return (g.dlo[0] && i == 0) || (g.dhi[0] && i == g.nn[0] - 1) || (g.dlo[1] && j == 0) ||
            (g.dhi[1] && j == g.nn[1] - 1) || (g.dlo[2] && k == 0) || (g.dhi[2] && k == g.nn[2] - 1);
@@ -81,6 +84,7 @@ class AmrNodal {
     DevBuf xc, bc, rtmp, xf, bf, yf;
     DevBuf ef_a, ef_b, rfT, yT, rcT;    // V-cycle work (FP64-sized, reused for FP32)
     DevBuf scal, part, ibuf;
+    TileSet ftiles;                     // the fine level's nodes in or on F: every fine launch
     unsigned long long* h_pinned = nullptr;
     double ndof = 0.0;
     bool sigma_set = false;
@@ -128,6 +132,11 @@ class AmrNodal {
                     nc += cd[id] && !is_dir_host(gc, i, j, k);
                 }
         ndof = (double)(nf + nc);
+        {
+            std::vector<unsigned char> act(ft.size());
+            for (size_t q = 0; q < ft.size(); ++q) act[q] = ft[q] != AMR_OUT;
+            ftiles.build(gf, act);
+        }
         ftype.alloc(ft.size());
         cdof.alloc(cd.size());
         ccov.alloc(cov.size());
@@ -147,7 +156,13 @@ class AmrNodal {
             MGX_HIP(hipMemset(sigf.p, 0, gf.nsig() * sizeof(double)));
         }
         for (DevBuf* b : {&xc, &bc, &rtmp, &rcT}) b->alloc(nnc * sizeof(double));
-        for (DevBuf* b : {&xf, &bf, &yf, &ef_a, &ef_b, &rfT, &yT}) b->alloc(nnf * sizeof(double));
+        // the fine launches visit only the tiles that hold nodes of F's
+        // closure; what lies elsewhere in these arrays is never read but is
+        // multiplied by the zero sigma outside F, so it starts finite (0)
+        for (DevBuf* b : {&xf, &bf, &yf, &ef_a, &ef_b, &rfT, &yT}) {
+            b->alloc(nnf * sizeof(double));
+            MGX_HIP(hipMemset(b->p, 0, nnf * sizeof(double)));
+        }
         bC.alloc(nnc * sizeof(double));
         scal.alloc(64 * sizeof(double));
         part.alloc(4096 * sizeof(double));
@@ -206,18 +221,23 @@ class AmrNodal {
     double residual(hipStream_t s) {
         const KGeom kc = kgeom(gc), kf = kgeom(gf);
         const unsigned char* ft = ftype.as<unsigned char>();
-        amr_fill_kernel<double><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, xc.as<double>(), xf.as<double>());
+        int nbk = 0;
+        const int4* tl = ftiles.blocks(nbk);
+        if (nbk)
+            amr_fill_kernel<double><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, xc.as<double>(),
+                                                             xf.as<double>(), tl);
         SArgs<double, double> a{};
         a.g = kf;
         a.c = make_coef<double>(gf.dx);
         a.x = xf.as<double>();
         a.sig = sigf.as<double>();
         a.rout = yf.as<double>();
-        launch_stencil<double, double, KS_APPLY>(gf, a, s, false);
+        launch_stencil<double, double, KS_APPLY>(gf, a, s, false, &ftiles);
         zero_norm(1, s);
-        amr_fine_resid_kernel<TO><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
-            kf, ft, bf.as<double>(), yf.as<double>(), rfT.as<TO>(), scal.as<unsigned long long>() + 1);
+        if (nbk)
+            amr_fine_resid_kernel<TO><<<nbk, tile_block(), 0, s>>>(kf, ft, bf.as<double>(),
+                                                               yf.as<double>(), rfT.as<TO>(),
+                                                               scal.as<unsigned long long>() + 1, tl);
         SArgs<double, double> c{};
         c.g = kc;
         c.c = make_coef<double>(gc.dx);
@@ -247,13 +267,15 @@ class AmrNodal {
         a.sig = sfT;
         a.omega = (T)nodal_omega(gf, opts.omega);
         a.nmask = ft;
+        int nbk = 0;
+        const int4* tl = ftiles.blocks(nbk);
         if (opts.nu1 > 0) {
             a.xout = e;
-            launch_stencil<T, T, KS_JFIRST>(gf, a, s, false);
+            launch_stencil<T, T, KS_JFIRST>(gf, a, s, false, &ftiles);
             for (int q = 1; q < opts.nu1; ++q) {
                 a.x = e;
                 a.xout = e2;
-                launch_stencil<T, T, KS_JACOBI>(gf, a, s, false);
+                launch_stencil<T, T, KS_JACOBI>(gf, a, s, false, &ftiles);
                 std::swap(e, e2);
             }
         } else {
@@ -265,16 +287,15 @@ class AmrNodal {
         ap.x = e;
         ap.sig = sfT;
         ap.rout = yT.as<T>();
-        launch_stencil<T, T, KS_APPLY>(gf, ap, s, false);
+        launch_stencil<T, T, KS_APPLY>(gf, ap, s, false, &ftiles);
         amr_coarse_rhs_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
             kf, kc, ft, cdof.as<unsigned char>(), rfT.as<T>(), yT.as<T>(), rcT.as<T>(), H.L[0].b);
         H.vcycle(opts, s, ibuf.as<int>());
-        amr_prolong_kernel<T><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, H.L[0].x, e);
+        if (nbk) amr_prolong_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, H.L[0].x, e, tl);
         for (int q = 0; q < opts.nu2; ++q) {
             a.x = e;
             a.xout = e2;
-            launch_stencil<T, T, KS_JACOBI>(gf, a, s, false);
+            launch_stencil<T, T, KS_JACOBI>(gf, a, s, false, &ftiles);
             std::swap(e, e2);
         }
         {
@@ -286,7 +307,7 @@ class AmrNodal {
             af.x = e;
             af.sig = sfT;
             af.rout = yT.as<T>();
-            launch_stencil<T, T, KS_APPLY>(gf, af, s, false);
+            launch_stencil<T, T, KS_APPLY>(gf, af, s, false, &ftiles);
             SArgs<T, T> ac{};
             ac.g = kc;
             ac.c = make_coef<T>(gc.dx);
@@ -298,8 +319,7 @@ class AmrNodal {
                 kf, kc, ft, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), yT.as<T>(), H.L[0].sig,
                 make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), H.L[0].x);
         }
-        amr_add_kernel<T><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
-            kf, ft, AMR_DOF, e, xf.as<double>());
+        if (nbk) amr_add_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, ft, AMR_DOF, e, xf.as<double>(), tl);
         amr_add_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
             kc, cdof.as<unsigned char>(), 1, H.L[0].x, xc.as<double>());
     }
@@ -333,8 +353,11 @@ class AmrNodal {
         if (gc.singular()) remove_rhs_mean(s);
         // ||b||_inf over the unknowns of both levels
         zero_norm(0, s);
-        amr_fine_resid_kernel<double><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
-            kf, ft, bf.as<double>(), nullptr, yf.as<double>(), scal.as<unsigned long long>());
+        int nbk = 0;
+        const int4* tl = ftiles.blocks(nbk);
+        if (nbk)
+            amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                kf, ft, bf.as<double>(), nullptr, yf.as<double>(), scal.as<unsigned long long>(), tl);
         amr_coarse_combine_kernel<double><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
             kf, kc, ft, cdof.as<unsigned char>(), bc.as<double>(), yf.as<double>(), 0.0,
             rtmp.as<double>(), scal.as<unsigned long long>());
@@ -361,8 +384,9 @@ class AmrNodal {
             conv = rn <= tol;
         }
         // dF from the coarse level, coarse nodes under F from the fine level
-        amr_fill_kernel<double><<<grid3(gf.nn[0], gf.nn[1], gf.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, xc.as<double>(), xf.as<double>());
+        if (nbk)
+            amr_fill_kernel<double><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, xc.as<double>(),
+                                                             xf.as<double>(), tl);
         amr_inject_kernel<<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
             kf, kc, cdof.as<unsigned char>(), xf.as<double>(), xc.as<double>());
         if (gc.singular()) {
@@ -423,6 +447,7 @@ struct mgx_amr_s {
         ugc.alloc(3 * g.nsig() * sizeof(double));
         ugf.alloc(3 * solver.gf.nsig() * sizeof(double));
         MGX_HIP(hipMemset(ugc.p, 0, 3 * g.nsig() * sizeof(double)));
+        MGX_HIP(hipMemset(ugf.p, 0, 3 * solver.gf.nsig() * sizeof(double)));
         dirty = false;
     }
 };
@@ -566,8 +591,8 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
         nodal_rhs_kernel<<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
             kc, h->ugc.as<double>(), S.rtmp.as<double>(), 0.25 / S.gc.dx[0], 0.25 / S.gc.dx[1],
             0.25 / S.gc.dx[2]);
-        // fine divergence of the velocity inside F
-        MGX_HIP(hipMemsetAsync(h->ugf.p, 0, 3 * S.gf.nsig() * sizeof(double), s));
+        // fine divergence of the velocity inside F (the gather writes the
+        // same cells every call; the rest of ugf stays 0 from the set-up)
         if (two) {
             GatherVelArgs f = a;
             f.g = kf;
@@ -577,12 +602,16 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
             f.ug = h->ugf.as<double>();
             gather_vel_kernel<<<h->lay_f.ntiles[REG_CELL_GROWN], 256, 0, s>>>(f);
         }
-        nodal_rhs_kernel<<<grid3(S.gf.nn[0], S.gf.nn[1], S.gf.nn[2]), block3(), 0, s>>>(
-            kf, h->ugf.as<double>(), S.yf.as<double>(), 0.25 / S.gf.dx[0], 0.25 / S.gf.dx[1],
-            0.25 / S.gf.dx[2]);
+        int nbk = 0;
+        const int4* tl = S.ftiles.blocks(nbk);
         const unsigned char* ft = S.ftype.as<unsigned char>();
-        amr_fine_resid_kernel<double><<<grid3(S.gf.nn[0], S.gf.nn[1], S.gf.nn[2]), block3(), 0, s>>>(
-            kf, ft, S.yf.as<double>(), nullptr, S.bf.as<double>(), nullptr);
+        if (nbk) {
+            nodal_rhs_kernel<<<nbk, tile_block(), 0, s>>>(kf, h->ugf.as<double>(), S.yf.as<double>(),
+                                                      0.25 / S.gf.dx[0], 0.25 / S.gf.dx[1],
+                                                      0.25 / S.gf.dx[2], tl);
+            amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                kf, ft, S.yf.as<double>(), nullptr, S.bf.as<double>(), nullptr, tl);
+        }
         amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
             kf, kc, ft, S.cdof.as<unsigned char>(), S.rtmp.as<double>(), S.yf.as<double>(), 1.0,
             S.bc.as<double>(), nullptr);

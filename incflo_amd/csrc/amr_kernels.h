@@ -55,9 +55,10 @@ template <class T>
 static __global__ void __launch_bounds__(256) amr_fill_kernel(KGeom gf, KGeom gc,
                                                               const unsigned char* __restrict__ ft,
                                                               const T* __restrict__ xc,
-                                                              T* __restrict__ xf) {
+                                                              T* __restrict__ xf,
+                                                              const int4* tl = nullptr) {
     int i, j, k;
-    if (!grid_point(gf.nn[0], gf.nn[1], i, j, k)) return;
+    if (!grid_point_tl(tl, gf.nn[0], gf.nn[1], i, j, k)) return;
     const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
     const int t = ft[id];
     if (t == AMR_BDRY) xf[id] = amr_interp(gc, xc, i, j, k);
@@ -69,9 +70,10 @@ template <class T>
 static __global__ void __launch_bounds__(256) amr_prolong_kernel(KGeom gf, KGeom gc,
                                                                  const unsigned char* __restrict__ ft,
                                                                  const T* __restrict__ ec,
-                                                                 T* __restrict__ ef) {
+                                                                 T* __restrict__ ef,
+                                                                 const int4* tl = nullptr) {
     int i, j, k;
-    if (!grid_point(gf.nn[0], gf.nn[1], i, j, k)) return;
+    if (!grid_point_tl(tl, gf.nn[0], gf.nn[1], i, j, k)) return;
     const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
     const int t = ft[id];
     if (t == AMR_OUT) { ef[id] = 0; return; }
@@ -139,9 +141,10 @@ static __global__ void __launch_bounds__(256) amr_coarse_combine_kernel(
 template <class TO>
 static __global__ void __launch_bounds__(256) amr_fine_resid_kernel(
     KGeom gf, const unsigned char* __restrict__ ft, const double* __restrict__ b,
-    const double* __restrict__ y, TO* __restrict__ out, unsigned long long* norm) {
+    const double* __restrict__ y, TO* __restrict__ out, unsigned long long* norm,
+    const int4* tl = nullptr) {
     int i, j, k;
-    const bool valid = grid_point(gf.nn[0], gf.nn[1], i, j, k);
+    const bool valid = grid_point_tl(tl, gf.nn[0], gf.nn[1], i, j, k);
     double r = 0.0;
     if (valid) {
         const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
@@ -238,9 +241,10 @@ static __global__ void __launch_bounds__(256) amr_coarse_jacobi_kernel(
 template <class T>
 static __global__ void __launch_bounds__(256) amr_add_kernel(KGeom g, const unsigned char* __restrict__ mask,
                                                              unsigned char want, const T* __restrict__ src,
-                                                             double* __restrict__ dst) {
+                                                             double* __restrict__ dst,
+                                                             const int4* tl = nullptr) {
     int i, j, k;
-    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    if (!grid_point_tl(tl, g.nn[0], g.nn[1], i, j, k)) return;
     const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
     if (mask[id] == want && !is_dir(g, i, j, k)) dst[id] += (double)src[id];
 }

@@ -159,9 +159,10 @@ static __global__ void __launch_bounds__(256) gather_vel_kernel(GatherVelArgs a)
 // (D u)_n = sum_d 1/(4 h_d) [ sum_{+d cells} u_d - sum_{-d cells} u_d ]
 static __global__ void __launch_bounds__(256) nodal_rhs_kernel(KGeom g, const double* __restrict__ ug,
                                                         double* __restrict__ rhs, double fx,
-                                                        double fy, double fz) {
+                                                        double fy, double fz,
+                                                        const int4* tl = nullptr) {
     int i, j, k;
-    if (!grid_point(g.nn[0], g.nn[1], i, j, k)) return;
+    if (!grid_point_tl(tl, g.nn[0], g.nn[1], i, j, k)) return;
     const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
     const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
     double sx = 0.0, sy = 0.0, sz = 0.0;

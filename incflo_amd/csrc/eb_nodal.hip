@@ -324,98 +324,57 @@ __global__ void eb_subtract_masked_kernel(const unsigned char* __restrict__ mask
 
 // one-workgroup BiCGStab (van der Vorst 1992) on the coarsest EB level, FP64,
 // vectors in global memory; inactive nodes hold 0; mean removed when singular
-__global__ void __launch_bounds__(1024) eb_bottom_kernel(EBLev L, const double* __restrict__ b_in,
-                                                         double* __restrict__ x_out, double* work,
-                                                         int maxiter, double rtol, int singular,
-                                                         double nact) {
-    __shared__ double red[16];
-    const KGeom& g = L.g;
-    const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
-    double *x = work, *r = work + n, *rh = work + 2 * n, *p = work + 3 * n, *v = work + 4 * n,
-           *sv = work + 5 * n, *t = work + 6 * n;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    auto node = [&](long id, int& i, int& j, int& k) {
-        i = (int)(id % g.nn[0]);
-        j = (int)((id / g.nn[0]) % g.nn[1]);
-        k = (int)(id / ((long)g.nn[0] * g.nn[1]));
-    };
-    auto apply = [&](const double* in, double* out) {
+// The EB nodal operator in FP64 for the bottom solve: inactive nodes are held
+// at zero and the singular mean is over the active ones; its smoother is the
+// V-cycle's weighted Jacobi sweep.
+struct EBBottomOp {
+    EBLev L;
+    double nact;
+    double omega;
+    __device__ long size() const { return (long)L.g.nn[0] * L.g.nn[1] * L.g.nn[2]; }
+    __device__ bool held(long id) const { return !L.mask[id]; }
+    __device__ double nfree() const { return nact; }
+    __device__ void node(long id, int& i, int& j, int& k) const {
+        i = (int)(id % L.g.nn[0]);
+        j = (int)((id / L.g.nn[0]) % L.g.nn[1]);
+        k = (int)(id / ((long)L.g.nn[0] * L.g.nn[1]));
+    }
+    __device__ void apply(const double* in, double* out, long n) const {
         __syncthreads();
-        for (long id = tid; id < n; id += nt) {
+        for (long id = threadIdx.x; id < n; id += blockDim.x) {
             int i, j, k;
             node(id, i, j, k);
             double dg;
             out[id] = L.mask[id] ? eb_apply_node(L, in, nullptr, i, j, k, dg) : 0.0;
         }
         __syncthreads();
-    };
-    double bs = 0.0;
-    for (long id = tid; id < n; id += nt) {
-        r[id] = L.mask[id] ? b_in[id] : 0.0;
-        bs += r[id];
     }
-    bs = bsum(bs, red);
-    const double bm = (singular && nact > 0) ? bs / nact : 0.0;
-    double rn = 0.0;
-    for (long id = tid; id < n; id += nt) {
-        if (L.mask[id]) r[id] -= bm;
-        rh[id] = r[id];
-        x[id] = 0.0;
-        p[id] = 0.0;
-        v[id] = 0.0;
-        rn = fmax(rn, nabs(r[id]));
-    }
-    const double r0 = bmax(rn, red);
-    double rho = 1.0, alpha = 1.0, omega = 1.0;
-    if (r0 > 0.0) {
-        for (int it = 1; it <= maxiter; ++it) {
-            double q = 0.0;
-            for (long id = tid; id < n; id += nt) q += rh[id] * r[id];
-            const double rho_new = bsum(q, red);
-            if (rho_new == 0.0) break;
-            const double beta = (rho_new / rho) * (alpha / omega);
-            for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
-            apply(p, v);
-            q = 0.0;
-            for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
-            const double rhv = bsum(q, red);
-            if (rhv == 0.0) break;
-            alpha = rho_new / rhv;
-            double sn = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                x[id] += alpha * p[id];
-                sv[id] = r[id] - alpha * v[id];
-                sn = fmax(sn, nabs(sv[id]));
+    __device__ void smooth(const BottomVecs& V, long n) const {
+        __syncthreads();
+        for (long id = threadIdx.x; id < n; id += blockDim.x) {
+            int i, j, k;
+            node(id, i, j, k);
+            double u = 0.0;
+            if (L.mask[id]) {
+                double dg;
+                const double lx = eb_apply_node(L, V.x, nullptr, i, j, k, dg);
+                if (dg != 0.0) u = V.x[id] + omega * (V.b[id] - lx) / dg;
             }
-            sn = bmax(sn, red);
-            if (sn <= rtol * r0) break;
-            apply(sv, t);
-            double tt = 0.0, ts = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                tt += t[id] * t[id];
-                ts += t[id] * sv[id];
-            }
-            tt = bsum(tt, red);
-            ts = bsum(ts, red);
-            if (tt == 0.0) break;
-            omega = ts / tt;
-            double rr = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                x[id] += omega * sv[id];
-                r[id] = sv[id] - omega * t[id];
-                rr = fmax(rr, nabs(r[id]));
-            }
-            rr = bmax(rr, red);
-            if (rr <= rtol * r0 || omega == 0.0) break;
-            rho = rho_new;
+            V.t[id] = u;
         }
+        __syncthreads();
+        for (long id = threadIdx.x; id < n; id += blockDim.x) V.x[id] = V.t[id];
+        __syncthreads();
     }
-    __syncthreads();
-    double xs = 0.0;
-    for (long id = tid; id < n; id += nt) xs += L.mask[id] ? x[id] : 0.0;
-    xs = bsum(xs, red);
-    const double xm = (singular && nact > 0) ? xs / nact : 0.0;
-    for (long id = tid; id < n; id += nt) x_out[id] = L.mask[id] ? x[id] - xm : 0.0;
+};
+
+__global__ void __launch_bounds__(1024) eb_bottom_kernel(EBLev L, const double* __restrict__ b_in,
+                                                         double* __restrict__ x_out, double* work,
+                                                         int kind, int maxiter, double rtol,
+                                                         double omega, int singular, double nact) {
+    __shared__ double red[16];
+    const EBBottomOp op{L, nact, omega};
+    bottom_solve_body(op, b_in, x_out, work, kind, maxiter, rtol, singular, nullptr, red);
 }
 
 void node_mask(const NodalGeom& g, const std::vector<int>& cidx, std::vector<unsigned char>& m) {
@@ -601,7 +560,8 @@ void EBNodal::vcycle(const MGOpts& o, hipStream_t s) {
         const long nb = lb.g.nnode();
         const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
         eb_bottom_kernel<<<1, threads, 0, s>>>(lev(nl - 1), lb.b_buf.as<double>(), lb.x_buf.as<double>(),
-                                               bottom_work.as<double>(), o.bottom_maxiter, o.bottom_rtol,
+                                               bottom_work.as<double>(), o.bottom_solver,
+                                               o.bottom_maxiter, o.bottom_rtol, nodal_omega(lb.g, o.omega),
                                                lb.g.singular() ? 1 : 0, (double)nact);
     }
     for (int l = nl - 2; l >= 0; --l) {

@@ -272,87 +272,55 @@ __device__ void mac_bottom_apply(const MGeomK& g, const MCoef<double>& c, const 
     __syncthreads();
 }
 
+// The MAC operator in FP64 for the bottom solve; its smoother is the
+// red-black sweep of the V-cycle, each colour updated from the values before
+// it (cells of one colour meet across a periodic wrap of odd length).
+template <class T>
+struct MacBottomOp {
+    MGeomK g;
+    MCoef<double> c;
+    const T *bx, *by, *bz, *a;
+    __device__ long size() const { return (long)g.n[0] * g.n[1] * g.n[2]; }
+    __device__ bool held(long) const { return false; }
+    __device__ double nfree() const { return (double)size(); }
+    __device__ void apply(const double* x, double* y, long n) const {
+        mac_bottom_apply<T>(g, c, bx, by, bz, a, x, y, n);
+    }
+    __device__ void smooth(const BottomVecs& V, long n) const {
+        for (int color = 0; color < 2; ++color) {
+            __syncthreads();
+            for (long id = threadIdx.x; id < n; id += blockDim.x) {
+                const int q = (int)(id / g.n[0]);
+                const int i = (int)(id - (long)q * g.n[0]);
+                const int j = q % g.n[1];
+                const int k = q / g.n[1];
+                double u = V.x[id];
+                if (((i + j + k) & 1) == color) {
+                    double dg;
+                    const double acc = mac_apply_cell<double, T, double>(g, c, bx, by, bz, a, V.x,
+                                                                         i, j, k, dg);
+                    if (dg != 0.0) u += (V.b[id] - acc) / dg;
+                }
+                V.t[id] = u;
+            }
+            __syncthreads();
+            for (long id = threadIdx.x; id < n; id += blockDim.x) V.x[id] = V.t[id];
+        }
+        __syncthreads();
+    }
+};
+
 template <class T>
 static __global__ void __launch_bounds__(1024) mac_bottom_kernel(MGeomK g, MCoef<double> c,
                                                                  const T* bx, const T* by,
                                                                  const T* bz, const T* a,
                                                                  const T* b_in,
-                                                                 T* x_out, double* work,
+                                                                 T* x_out, double* work, int kind,
                                                                  int maxiter, double rtol,
                                                                  int singular, int* iters_acc) {
     __shared__ double red[16];
-    const long n = (long)g.n[0] * g.n[1] * g.n[2];
-    double* x = work;
-    double* r = work + n;
-    double* rh = work + 2 * n;
-    double* p = work + 3 * n;
-    double* v = work + 4 * n;
-    double* s = work + 5 * n;
-    double* t = work + 6 * n;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    double bs = 0.0;
-    for (long id = tid; id < n; id += nt) { r[id] = (double)b_in[id]; bs += r[id]; }
-    bs = bsum(bs, red);
-    const double bmean = singular ? bs / (double)n : 0.0;
-    for (long id = tid; id < n; id += nt) {
-        r[id] -= bmean;
-        rh[id] = r[id];
-        x[id] = 0.0; p[id] = 0.0; v[id] = 0.0;
-    }
-    double rn = 0.0;
-    for (long id = tid; id < n; id += nt) rn = fmax(rn, nabs(r[id]));
-    const double rnorm0 = bmax(rn, red);
-    double rho = 1.0, alpha = 1.0, omega = 1.0;
-    int it = 0;
-    if (rnorm0 > 0.0) {
-        for (it = 1; it <= maxiter; ++it) {
-            double q = 0.0;
-            for (long id = tid; id < n; id += nt) q += rh[id] * r[id];
-            const double rho_new = bsum(q, red);
-            if (rho_new == 0.0) break;
-            const double beta = (rho_new / rho) * (alpha / omega);
-            for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
-            mac_bottom_apply<T>(g, c, bx, by, bz, a, p, v, n);
-            q = 0.0;
-            for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
-            const double rhv = bsum(q, red);
-            if (rhv == 0.0) break;
-            alpha = rho_new / rhv;
-            double sn = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                x[id] += alpha * p[id];
-                s[id] = r[id] - alpha * v[id];
-                sn = fmax(sn, nabs(s[id]));
-            }
-            sn = bmax(sn, red);
-            if (sn <= rtol * rnorm0) break;
-            mac_bottom_apply<T>(g, c, bx, by, bz, a, s, t, n);
-            double tt = 0.0, ts = 0.0;
-            for (long id = tid; id < n; id += nt) { tt += t[id] * t[id]; ts += t[id] * s[id]; }
-            tt = bsum(tt, red);
-            ts = bsum(ts, red);
-            if (tt == 0.0) break;
-            omega = ts / tt;
-            double rr = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                x[id] += omega * s[id];
-                r[id] = s[id] - omega * t[id];
-                rr = fmax(rr, nabs(r[id]));
-            }
-            rr = bmax(rr, red);
-            if (rr <= rtol * rnorm0) break;
-            if (omega == 0.0) break;
-            rho = rho_new;
-        }
-        if (it > maxiter) it = maxiter;
-    }
-    __syncthreads();
-    double xs = 0.0;
-    for (long id = tid; id < n; id += nt) xs += x[id];
-    xs = bsum(xs, red);
-    const double xmean = singular ? xs / (double)n : 0.0;
-    for (long id = tid; id < n; id += nt) x_out[id] = (T)(x[id] - xmean);
-    if (tid == 0 && iters_acc) atomicAdd(iters_acc, it);
+    const MacBottomOp<T> op{g, c, bx, by, bz, a};
+    bottom_solve_body(op, b_in, x_out, work, kind, maxiter, rtol, singular, iters_acc, red);
 }
 
 // ---- Robin domain faces: a phi + b dphi/dn = f ---------------------------

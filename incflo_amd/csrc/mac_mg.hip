@@ -112,7 +112,8 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
         const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
         mac_bottom_kernel<T><<<1, threads, 0, s>>>(lb.kg, make_mcoef<double>(lb.g.dx), lb.beta[0],
                                                    lb.beta[1], lb.beta[2], lb.a, lb.b, lb.x,
-                                                   bottom_work.as<double>(), o.bottom_maxiter,
+                                                   bottom_work.as<double>(), o.bottom_solver,
+                                                   o.bottom_maxiter,
                                                    o.bottom_rtol, lb.g.singular() ? 1 : 0,
                                                    bottom_iters);
     }

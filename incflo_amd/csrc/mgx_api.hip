@@ -157,6 +157,7 @@ MGX_API void mgx_default_mlmg_opts(mgx_mlmg_opts* o) {
     o->omega = m.omega;
     o->stall_iters = m.stall_iters;
     o->cycles_per_update = m.cycles_per_update;
+    o->bottom_solver = m.bottom_solver;
 }
 
 MGX_API int mgx_ctx_create(int device, void* stream, mgx_ctx* out) {

@@ -18,6 +18,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include "mgx.h"
+
 #include <type_traits>
 
 namespace mgx {
@@ -103,6 +105,19 @@ __device__ __forceinline__ bool grid_point(int nx, int ny, int& i, int& j, int& 
     return i < nx && j < ny;
 }
 
+// Blocks of blockDim.x x blockDim.y nodes of one plane taken from a list of
+// (bx, by, k) entries (the blocks that hold active nodes of a partly used
+// level); tl NULL: the full 64 x 4 grid.
+__device__ __forceinline__ bool grid_point_tl(const int4* tl, int nx, int ny, int& i, int& j,
+                                              int& k) {
+    if (!tl) return grid_point(nx, ny, i, j, k);
+    const int4 t = tl[blockIdx.x];
+    i = t.x * (int)blockDim.x + threadIdx.x;
+    j = t.y * (int)blockDim.y + threadIdx.y;
+    k = t.z;
+    return i < nx && j < ny;
+}
+
 inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + 63) / 64, (ny + 3) / 4, nz); }
 inline dim3 block3() { return dim3(64, 4, 1); }
 
@@ -163,6 +178,10 @@ struct SArgs {
     // scaled reduced-precision V-cycle (0 means 1): UPDRES adds cor_scale*cor;
     // RESID/UPDRES store rscale*r and x1 from x1scale*r
     double cor_scale, rscale, x1scale;
+    // optional list of the tiles to visit (linear tile indices bx + gx (by +
+    // gy bz), one workgroup each): a level whose active nodes fill only part
+    // of its array (an AMR fine level) is swept over those tiles alone
+    const int* tiles;
 };
 
 // One 27-point pass over a 64 x 8 node tile, streaming along z.  A workgroup
@@ -265,10 +284,17 @@ __device__ __forceinline__ T apply27(const T (&P0)[NPR][3], const T (&P1)[NPR][3
     return acc;
 }
 
-template <class T, class TC, int MODE, bool ISO, bool FINE, int NPT, bool MASK = false>
-static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_kernel(SArgs<T, TC> a) {
+// TXP: tile width in x (64: one wave per node row; 16: a wave spans 4 rows of
+// 16, 16 x 16 threads, for tile-listed launches over partly used levels,
+// where narrow tiles skip more of the unused volume)
+template <class T, class TC, int MODE, bool ISO, bool FINE, int NPT, bool MASK = false,
+          int TXP = 64>
+static __global__ void __launch_bounds__(TXP == 64 && NPT == 1 ? 512 : 256)
+    nodal_stencil_kernel(SArgs<T, TC> a) {
     using TA = acc_t<T>;  // arithmetic type (T is the storage type of x, b, sigma)
-    constexpr int TX = 64, NY = (NPT == 1 ? 8 : 4 * NPT), TYT = NY / NPT;  // threads per row, tile rows, thread rows
+    static_assert(TXP == 64 || (TXP == 16 && NPT == 1), "tile shapes: 64 x 4NPT or 16 x 16");
+    // threads per row, tile rows, thread rows
+    constexpr int TX = TXP, NY = TXP == 16 ? 16 : (NPT == 1 ? 8 : 4 * NPT), TYT = NY / NPT;
     constexpr int NT = TX * TYT;
     constexpr int PR = NY + 2, PC = TX + 2;          // phi tile incl. halo
     constexpr int SR = NY + 1, SC = TX + 1;          // sigma tile
@@ -279,7 +305,7 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
     const KGeom& g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
     const int nb = a.gx * a.gy * a.gz;
-    const int lb = xcd_remap(blockIdx.x, nb);
+    const int lb = a.tiles ? a.tiles[blockIdx.x] : xcd_remap(blockIdx.x, nb);
     const int bx = lb % a.gx, by = (lb / a.gx) % a.gy, bz = lb / (a.gx * a.gy);
     const int i0 = bx * TX, j0 = by * NY, k0 = bz * a.kz;
     const int k1 = min(k0 + a.kz, g.nn[2]);
@@ -528,6 +554,197 @@ static __global__ void __launch_bounds__(NPT == 1 ? 512 : 256) nodal_stencil_ker
     if ((MODE == KS_RESID || MODE == KS_UPDRES) && a.norm) {
         const double m = block_max<NT>(rmax, red);
         if (tid == 0) atomic_max_nonneg(a.norm, m);
+    }
+}
+
+// ---- the Jacobi sweep with LDS-direct plane loads (4-byte storage) ---------
+// Same tile, z-streaming and arithmetic as nodal_stencil_kernel<..., KS_JACOBI,
+// ..., 2> (64 x 8 nodes, 64 x 4 threads, 2 nodes per thread), but every plane's
+// x, sigma and b rows go from global memory straight into a ring of R plane
+// slots in LDS (global_load_lds_dword: 64 lanes fill 64 consecutive words),
+// R - 1 planes ahead, with no staging registers and no LDS stores.  A plane
+// is 29 row loads (10 x rows, their two halo columns, 9 sigma rows and their
+// extra column, 8 b rows), dealt round-robin to the 4 waves; each wave waits
+// for its own loads of a plane with s_waitcnt vmcnt (loads return in order,
+// so the count of loads issued after it is the bound) and a barrier then
+// publishes the plane to the workgroup.  A slot is refilled only after the
+// barrier that follows the last reads of it.
+constexpr int kDmaRowLoads = 29;
+constexpr int kDmaMinLoadsPerWave = kDmaRowLoads / 4;  // 7 (wave 0 issues 8)
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt_lgkm0() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    // gfx9 encoding: vmcnt[3:0], expcnt[6:4] (7: no wait), lgkmcnt[11:8], vmcnt[15:14]
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (0 << 8) | ((N >> 4) << 14));
+}
+
+// workgroup barrier without a memory fence (a fence would wait for every
+// outstanding load, including the planes in flight); the "memory" clobber
+// keeps the compiler from moving LDS accesses across it
+__device__ __forceinline__ void barrier_nofence() { asm volatile("s_barrier" ::: "memory"); }
+
+template <class T, bool ISO, bool FINE, int R>
+static __global__ void __launch_bounds__(256) nodal_jacobi_dma_kernel(SArgs<T, T> a) {
+    static_assert(sizeof(T) == 4, "LDS-direct sweep: 4-byte storage");
+    using TA = acc_t<T>;
+    constexpr int TX = 64, NY = 8, NPT = 2, TYT = 4, D = R - 1;
+    constexpr int PR = NY + 2, SR = NY + 1;
+    // slot layout (words): x rows | x halo columns (left rows 0..9, right rows
+    // 0..9, rest unused) | sigma rows | sigma extra column | b rows
+    constexpr int O_PH = 0, O_PHH = O_PH + PR * TX, O_SG = O_PHH + TX, O_SGX = O_SG + SR * TX,
+                  O_B = O_SGX + TX, SLOT = O_B + NY * TX;
+    __shared__ T lds[R][SLOT];
+    const KGeom& g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int nb = a.gx * a.gy * a.gz;
+    const int lb = a.tiles ? a.tiles[blockIdx.x] : xcd_remap(blockIdx.x, nb);
+    const int bx = lb % a.gx, by = (lb / a.gx) % a.gy, bz = lb / (a.gx * a.gy);
+    const int i0 = bx * TX, j0 = by * NY, k0 = bz * a.kz;
+    const int k1 = min(k0 + a.kz, g.nn[2]);
+    const int i = i0 + tx, jr = j0 + NPT * ty;
+    const long sxy = (long)g.nn[0] * g.nn[1];
+    const long ssxy = (long)g.sg[0] * g.sg[1];
+
+    // this lane's in-plane byte offset for each of the wave's row loads
+    // (indices wrapped / clamped so every address is valid)
+    constexpr int NQ = (kDmaRowLoads + 3) / 4;  // 8
+    unsigned off[NQ];
+    int kind[NQ], lds_off[NQ];  // kind: 0 x, 1 sigma, 2 b, -1 none
+#pragma unroll
+    for (int s = 0; s < NQ; ++s) {
+        const int q = ty + 4 * s;
+        kind[s] = -1;
+        off[s] = 0;
+        lds_off[s] = 0;
+        if (q >= kDmaRowLoads) continue;
+        if (q < PR) {  // x row j0-1+q, interior columns
+            const int row = wrapc(j0 - 1 + q, g.nn[1], g.per[1]);
+            const int col = wrapc(i0 + tx, g.nn[0], g.per[0]);
+            kind[s] = 0;
+            off[s] = (unsigned)(col + g.nn[0] * row) * 4u;
+            lds_off[s] = O_PH + q * TX;
+        } else if (q == PR) {  // x halo columns: lanes 0..9 left, 10..19 right
+            const int r = tx < PR ? tx : (tx < 2 * PR ? tx - PR : 0);
+            const int row = wrapc(j0 - 1 + r, g.nn[1], g.per[1]);
+            const int col = wrapc(tx >= PR && tx < 2 * PR ? i0 + TX : i0 - 1, g.nn[0], g.per[0]);
+            kind[s] = 0;
+            off[s] = (unsigned)(col + g.nn[0] * row) * 4u;
+            lds_off[s] = O_PHH;
+        } else if (q < PR + 1 + SR) {  // sigma row j0+r (ghosted indexing)
+            const int r = q - PR - 1;
+            const int row = min(j0 + r, g.sg[1] - 1);
+            const int col = min(i0 + tx, g.sg[0] - 1);
+            kind[s] = 1;
+            off[s] = (unsigned)(col + g.sg[0] * row) * 4u;
+            lds_off[s] = O_SG + r * TX;
+        } else if (q == PR + 1 + SR) {  // sigma extra column i0+64, lanes 0..8
+            const int row = min(j0 + min(tx, SR - 1), g.sg[1] - 1);
+            const int col = min(i0 + TX, g.sg[0] - 1);
+            kind[s] = 1;
+            off[s] = (unsigned)(col + g.sg[0] * row) * 4u;
+            lds_off[s] = O_SGX;
+        } else {  // b row j0+r
+            const int r = q - PR - 2 - SR;
+            const int row = min(j0 + r, g.nn[1] - 1);
+            const int col = min(i0 + tx, g.nn[0] - 1);
+            kind[s] = 2;
+            off[s] = (unsigned)(col + g.nn[0] * row) * 4u;
+            lds_off[s] = O_B + r * TX;
+        }
+    }
+    // issue plane kk's row loads of this wave into its slot
+    auto issue = [&](int kk) {
+        T* slot = lds[(kk + R) % R];
+        const T* xk = a.x + (long)wrapc(kk, g.nn[2], g.per[2]) * sxy;
+        const T* sk = a.sig + (long)min(max(kk, 0), g.sg[2] - 1) * ssxy;
+        const T* bk = a.b + (long)min(max(kk - 1, 0), g.nn[2] - 1) * sxy;
+#pragma unroll
+        for (int s = 0; s < NQ; ++s) {
+            // kind and LDS offset depend on the wave only: made scalar, the
+            // choice is a scalar branch and the load takes a scalar base
+            const int kd = __builtin_amdgcn_readfirstlane(kind[s]);
+            if (kd < 0) continue;
+            const T* base = kd == 0 ? xk : (kd == 1 ? sk : bk);
+            __builtin_amdgcn_global_load_lds(byte_add(base, off[s]),
+                                             slot + __builtin_amdgcn_readfirstlane(lds_off[s]), 4,
+                                             0, 0);
+        }
+    };
+    // wait until this wave's loads of the plane `later` planes before the
+    // newest issued one have landed, then publish the slot
+    auto wait_plane = [&](int later) {
+        if (later >= 2) wait_vmcnt_lgkm0<2 * kDmaMinLoadsPerWave>();
+        else if (later == 1) wait_vmcnt_lgkm0<kDmaMinLoadsPerWave>();
+        else wait_vmcnt_lgkm0<0>();
+        barrier_nofence();
+    };
+    static_assert(D <= 3, "wait_plane covers up to 2 planes in flight behind the one waited for");
+
+    TA P[3][NPT + 2][3], S[2][NPT + 1][2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < NPT + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[p][r][c] = 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < NPT + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) S[p][r][c] = 0;
+
+    for (int p = 0; p < D; ++p)
+        if (k0 - 1 + p <= k1) issue(k0 - 1 + p);
+    for (int kk = k0 - 1; kk <= k1; ++kk) {
+        wait_plane(min(D - 1, k1 - kk));
+        if (kk + D <= k1) issue(kk + D);
+        const T* slot = lds[(kk + R) % R];
+        // rotate the register planes: below <- centre <- above, sigma below <- above
+#pragma unroll
+        for (int r = 0; r < NPT + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                P[0][r][c] = P[1][r][c];
+                P[1][r][c] = P[2][r][c];
+            }
+#pragma unroll
+        for (int r = 0; r < NPT + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) S[1][r][c] = S[0][r][c];
+#pragma unroll
+        for (int r = 0; r < NPT + 2; ++r) {
+            const int row = NPT * ty + r;
+            P[2][r][0] = tx == 0 ? slot[O_PHH + row] : slot[O_PH + row * TX + tx - 1];
+            P[2][r][1] = slot[O_PH + row * TX + tx];
+            P[2][r][2] = tx == TX - 1 ? slot[O_PHH + PR + row] : slot[O_PH + row * TX + tx + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < NPT + 1; ++r) {
+            const int row = NPT * ty + r;
+            S[0][r][0] = slot[O_SG + row * TX + tx];
+            S[0][r][1] = tx == TX - 1 ? slot[O_SGX + row] : slot[O_SG + row * TX + tx + 1];
+        }
+        if (kk >= k0 + 1) {
+            const int k = kk - 1;
+            const long kp = sxy * k;
+#pragma unroll
+            for (int n = 0; n < NPT; ++n) {
+                const int j = jr + n;
+                if (!(i < g.nn[0] && j < g.nn[1])) continue;
+                TA ssum;
+                const TA acc = (n == 0)
+                                   ? apply27<TA, ISO, 0>(P[0], P[1], P[2], S[1], S[0], a.c, ssum)
+                                   : apply27<TA, ISO, 1>(P[0], P[1], P[2], S[1], S[0], a.c, ssum);
+                const TA c = P[1][n + 1][1];
+                const TA bv = slot[O_B + (NPT * ty + n) * TX + tx];
+                TA v = c;
+                if (is_dir(g, i, j, k)) v = 0;
+                else if (ssum != (TA)0) v = c + (bv - acc) * (a.omega * diag_rcp(-a.c.diag4 * ssum));
+                a.xout[kp + i + (long)g.nn[0] * j] = st<T>(v);
+            }
+        }
     }
 }
 
@@ -1009,100 +1226,219 @@ __device__ void bottom_apply(const KGeom& g, const Coef<double>& c, const TS* si
     __syncthreads();
 }
 
-template <class T>
-__device__ void bottom_bicgstab_body(const KGeom& g, const Coef<double>& c, const T* sig,
-                                     const unsigned char* m, const T* b_in, T* x_out,
-                                     double* work, int maxiter, double rtol, int singular,
-                                     int* iters_acc, double* red) {
-    const long n = (long)g.nn[0] * g.nn[1] * g.nn[2];
-    double* x = work;
-    double* r = work + n;
-    double* rh = work + 2 * n;
-    double* p = work + 3 * n;
-    double* v = work + 4 * n;
-    double* s = work + 5 * n;
-    double* t = work + 6 * n;
+// The bottom_solver choices of MLMG (MGX_BOTTOM_*): BiCGStab (van der Vorst
+// 1992), CG (Hestenes and Stiefel 1952), BiCGStab and then CG from zero when
+// the first one fails ("bicgcg"), the reverse ("cgbicg"), or smoothing sweeps
+// only ("smoother").  A Krylov solve that does not reach rtol is followed by
+// kBottomSmooths smoothing sweeps, from zero when it broke down (a zero
+// inner product).  Both operators are negative (semi-)definite and
+// symmetric, so CG applies as it stands.
+constexpr int kBottomSmooths = 8;
+
+// the 7 FP64 work vectors of the bottom solve
+struct BottomVecs {
+    double *x, *r, *b, *p, *v, *s, *t;
+};
+
+__device__ __forceinline__ BottomVecs bottom_vecs(double* w, long n) {
+    return BottomVecs{w, w + n, w + 2 * n, w + 3 * n, w + 4 * n, w + 5 * n, w + 6 * n};
+}
+
+// 0: converged, 1: broke down, 2: iteration limit.  Starts from x = 0.
+template <class Op>
+__device__ int bottom_bicgstab(const Op& op, const BottomVecs& V, long n, int maxiter,
+                               double rtol, double rnorm0, int& iters, double* red) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    // b (mean-free when singular; zero at overset-masked nodes) into r; x = 0
-    double bs = 0.0;
+    double *x = V.x, *r = V.r, *rh = V.b, *p = V.p, *v = V.v, *s = V.s, *t = V.t;
     for (long id = tid; id < n; id += nt) {
-        r[id] = (m && !m[id]) ? 0.0 : (double)b_in[id];
-        bs += r[id];
-    }
-    bs = bsum(bs, red);
-    const double bmean = singular ? bs / (double)n : 0.0;
-    for (long id = tid; id < n; id += nt) {
-        r[id] -= bmean;
-        rh[id] = r[id];
+        r[id] = rh[id];
         x[id] = 0.0; p[id] = 0.0; v[id] = 0.0;
     }
-    double rn = 0.0;
-    for (long id = tid; id < n; id += nt) rn = fmax(rn, nabs(r[id]));
-    const double rnorm0 = bmax(rn, red);
     double rho = 1.0, alpha = 1.0, omega = 1.0;
+    for (int it = 1; it <= maxiter; ++it) {
+        iters = it;
+        double q = 0.0;
+        for (long id = tid; id < n; id += nt) q += rh[id] * r[id];
+        const double rho_new = bsum(q, red);
+        if (rho_new == 0.0) return 1;
+        const double beta = (rho_new / rho) * (alpha / omega);
+        for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
+        op.apply(p, v, n);
+        q = 0.0;
+        for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
+        const double rhv = bsum(q, red);
+        if (rhv == 0.0) return 1;
+        alpha = rho_new / rhv;
+        double sn = 0.0;
+        for (long id = tid; id < n; id += nt) {
+            x[id] += alpha * p[id];
+            s[id] = r[id] - alpha * v[id];
+            sn = fmax(sn, nabs(s[id]));
+        }
+        sn = bmax(sn, red);
+        if (sn <= rtol * rnorm0) return 0;
+        op.apply(s, t, n);
+        double tt = 0.0, ts = 0.0;
+        for (long id = tid; id < n; id += nt) { tt += t[id] * t[id]; ts += t[id] * s[id]; }
+        tt = bsum(tt, red);
+        ts = bsum(ts, red);
+        if (tt == 0.0) return 1;
+        omega = ts / tt;
+        double rr = 0.0;
+        for (long id = tid; id < n; id += nt) {
+            x[id] += omega * s[id];
+            r[id] = s[id] - omega * t[id];
+            rr = fmax(rr, nabs(r[id]));
+        }
+        rr = bmax(rr, red);
+        if (rr <= rtol * rnorm0) return 0;
+        if (omega == 0.0) return 1;
+        rho = rho_new;
+    }
+    return 2;
+}
+
+// Conjugate gradients from x = 0; same return codes.
+template <class Op>
+__device__ int bottom_cg(const Op& op, const BottomVecs& V, long n, int maxiter, double rtol,
+                         double rnorm0, int& iters, double* red) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *x = V.x, *r = V.r, *p = V.p, *v = V.v;
+    for (long id = tid; id < n; id += nt) {
+        r[id] = V.b[id];
+        x[id] = 0.0; p[id] = 0.0;
+    }
+    double rho_old = 1.0;
+    for (int it = 1; it <= maxiter; ++it) {
+        iters = it;
+        double q = 0.0;
+        for (long id = tid; id < n; id += nt) q += r[id] * r[id];
+        const double rho = bsum(q, red);
+        if (rho == 0.0) return 0;  // the residual vanished
+        const double beta = it == 1 ? 0.0 : rho / rho_old;
+        for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * p[id];
+        op.apply(p, v, n);
+        q = 0.0;
+        for (long id = tid; id < n; id += nt) q += p[id] * v[id];
+        const double pv = bsum(q, red);
+        if (pv == 0.0) return 1;
+        const double alpha = rho / pv;
+        double rn = 0.0;
+        for (long id = tid; id < n; id += nt) {
+            x[id] += alpha * p[id];
+            r[id] -= alpha * v[id];
+            rn = fmax(rn, nabs(r[id]));
+        }
+        rn = bmax(rn, red);
+        if (rn <= rtol * rnorm0) return 0;
+        rho_old = rho;
+    }
+    return 2;
+}
+
+// The bottom solve of op's level: b_in (mean-free over the free unknowns when
+// singular, zero at held ones) -> x_out (likewise; zero at held unknowns).
+// Adds the Krylov iterations (or smoothing sweeps) to *iters_acc.
+template <class Op, class T>
+__device__ void bottom_solve_body(const Op& op, const T* b_in, T* x_out, double* work, int kind,
+                                  int maxiter, double rtol, int singular, int* iters_acc,
+                                  double* red) {
+    const long n = op.size();
+    const BottomVecs V = bottom_vecs(work, n);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double bs = 0.0;
+    for (long id = tid; id < n; id += nt) {
+        V.b[id] = op.held(id) ? 0.0 : (double)b_in[id];
+        bs += V.b[id];
+    }
+    bs = bsum(bs, red);
+    const double nfree = op.nfree();
+    const double bmean = (singular && nfree > 0.0) ? bs / nfree : 0.0;
+    double rn = 0.0;
+    for (long id = tid; id < n; id += nt) {
+        if (!op.held(id)) V.b[id] -= bmean;
+        V.x[id] = 0.0;
+        rn = fmax(rn, nabs(V.b[id]));
+    }
+    const double rnorm0 = bmax(rn, red);
     int it = 0;
     if (rnorm0 > 0.0) {
-        for (it = 1; it <= maxiter; ++it) {
-            double q = 0.0;
-            for (long id = tid; id < n; id += nt) q += rh[id] * r[id];
-            const double rho_new = bsum(q, red);
-            if (rho_new == 0.0) break;
-            const double beta = (rho_new / rho) * (alpha / omega);
-            for (long id = tid; id < n; id += nt) p[id] = r[id] + beta * (p[id] - omega * v[id]);
-            bottom_apply(g, c, sig, m, p, v, n);
-            q = 0.0;
-            for (long id = tid; id < n; id += nt) q += rh[id] * v[id];
-            const double rhv = bsum(q, red);
-            if (rhv == 0.0) break;
-            alpha = rho_new / rhv;
-            double sn = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                x[id] += alpha * p[id];
-                s[id] = r[id] - alpha * v[id];
-                sn = fmax(sn, nabs(s[id]));
+        int st = 2, smooths = 0;
+        if (kind == MGX_BOTTOM_SMOOTHER) {
+            smooths = kBottomSmooths;
+        } else {
+            const bool cg_first = kind == MGX_BOTTOM_CG || kind == MGX_BOTTOM_CGBICG;
+            st = cg_first ? bottom_cg(op, V, n, maxiter, rtol, rnorm0, it, red)
+                          : bottom_bicgstab(op, V, n, maxiter, rtol, rnorm0, it, red);
+            if (st != 0 && (kind == MGX_BOTTOM_BICGCG || kind == MGX_BOTTOM_CGBICG)) {
+                int it2 = 0;
+                st = cg_first ? bottom_bicgstab(op, V, n, maxiter, rtol, rnorm0, it2, red)
+                              : bottom_cg(op, V, n, maxiter, rtol, rnorm0, it2, red);
+                it += it2;
             }
-            sn = bmax(sn, red);
-            if (sn <= rtol * rnorm0) break;
-            bottom_apply(g, c, sig, m, s, t, n);
-            double tt = 0.0, ts = 0.0;
-            for (long id = tid; id < n; id += nt) { tt += t[id] * t[id]; ts += t[id] * s[id]; }
-            tt = bsum(tt, red);
-            ts = bsum(ts, red);
-            if (tt == 0.0) break;
-            omega = ts / tt;
-            double rr = 0.0;
-            for (long id = tid; id < n; id += nt) {
-                x[id] += omega * s[id];
-                r[id] = s[id] - omega * t[id];
-                rr = fmax(rr, nabs(r[id]));
-            }
-            rr = bmax(rr, red);
-            if (rr <= rtol * rnorm0) break;
-            if (omega == 0.0) break;
-            rho = rho_new;
+            if (st != 0) smooths = kBottomSmooths;
         }
-        if (it > maxiter) it = maxiter;
+        if (st == 1)
+            for (long id = tid; id < n; id += nt) V.x[id] = 0.0;
+        for (int q = 0; q < smooths; ++q) op.smooth(V, n);
+        if (kind == MGX_BOTTOM_SMOOTHER) it = smooths;
     }
     __syncthreads();
     double xs = 0.0;
-    for (long id = tid; id < n; id += nt) xs += x[id];
+    for (long id = tid; id < n; id += nt) xs += op.held(id) ? 0.0 : V.x[id];
     xs = bsum(xs, red);
-    const double xmean = singular ? xs / (double)n : 0.0;
-    for (long id = tid; id < n; id += nt) x_out[id] = (T)(x[id] - xmean);
+    const double xmean = (singular && nfree > 0.0) ? xs / nfree : 0.0;
+    for (long id = tid; id < n; id += nt) x_out[id] = op.held(id) ? (T)0 : (T)(V.x[id] - xmean);
     if (tid == 0 && iters_acc) atomicAdd(iters_acc, it);
 }
 
+// The nodal operator in FP64 for the bottom solve; its smoother is the
+// weighted Jacobi sweep of the V-cycle.
+template <class TS>
+struct NodalBottomOp {
+    KGeom g;
+    Coef<double> c;
+    const TS* sig;
+    const unsigned char* m;  // overset mask, may be null
+    double omega;
+    __device__ long size() const { return (long)g.nn[0] * g.nn[1] * g.nn[2]; }
+    __device__ bool held(long id) const { return m && !m[id]; }
+    // a singular problem has no held nodes (a mask that holds one makes it regular)
+    __device__ double nfree() const { return (double)size(); }
+    __device__ void apply(const double* x, double* y, long n) const {
+        bottom_apply(g, c, sig, m, x, y, n);
+    }
+    __device__ void smooth(const BottomVecs& V, long n) const {
+        __syncthreads();
+        for (long id = threadIdx.x; id < n; id += blockDim.x) {
+            const int i = id % g.nn[0];
+            const int j = (id / g.nn[0]) % g.nn[1];
+            const int k = id / ((long)g.nn[0] * g.nn[1]);
+            double ssum;
+            const double acc = apply_node_nbhd<double, TS, double>(g, c, sig, V.x, i, j, k, ssum);
+            double u = V.x[id];
+            if (node_fixed(g, m, id, i, j, k)) u = 0.0;
+            else if (ssum != 0.0) u += omega * (V.b[id] - acc) / (-c.diag4 * ssum);
+            V.t[id] = u;
+        }
+        __syncthreads();
+        for (long id = threadIdx.x; id < n; id += blockDim.x) V.x[id] = V.t[id];
+        __syncthreads();
+    }
+};
+
 template <class T>
-static __global__ void __launch_bounds__(1024) bottom_bicgstab_kernel(KGeom g, Coef<double> c,
-                                                                      const T* sig,
-                                                                      const unsigned char* m,
-                                                                      const T* b_in, T* x_out,
-                                                                      double* work, int maxiter,
-                                                                      double rtol, int singular,
-                                                                      int* iters_acc) {
+static __global__ void __launch_bounds__(1024) bottom_solve_kernel(KGeom g, Coef<double> c,
+                                                                   const T* sig,
+                                                                   const unsigned char* m,
+                                                                   const T* b_in, T* x_out,
+                                                                   double* work, int kind,
+                                                                   int maxiter, double rtol,
+                                                                   double omega, int singular,
+                                                                   int* iters_acc) {
     __shared__ double red[16];
-    bottom_bicgstab_body<T>(g, c, sig, m, b_in, x_out, work, maxiter, rtol, singular, iters_acc,
-                            red);
+    const NodalBottomOp<T> op{g, c, sig, m, omega};
+    bottom_solve_body(op, b_in, x_out, work, kind, maxiter, rtol, singular, iters_acc, red);
 }
 
 // ---- coarse levels: the whole V-cycle below a size threshold in one workgroup ----
@@ -1133,6 +1469,8 @@ struct CoarseArgs {
     int nu1, nu2;
     Coef<double> cb;  // bottom operator coefficients (double)
     double* work;
+    int bottom_kind;  // MGX_BOTTOM_*
+    double bottom_omega;
     int maxiter;
     double rtol;
     int singular;
@@ -1303,8 +1641,9 @@ static __global__ void __launch_bounds__(512) coarse_vcycle_kernel(const CoarseA
     // bottom
     {
         const int lb = a.nl - 1;
-        bottom_bicgstab_body<T>(sl[lb].g, a.cb, sl[lb].sig, sl[lb].m, sl[lb].b, cur(lb), swork,
-                                a.maxiter, a.rtol, a.singular, a.iters_acc, red);
+        const NodalBottomOp<T> op{sl[lb].g, a.cb, sl[lb].sig, sl[lb].m, a.bottom_omega};
+        bottom_solve_body(op, sl[lb].b, cur(lb), swork, a.bottom_kind, a.maxiter, a.rtol,
+                          a.singular, a.iters_acc, red);
         __syncthreads();
     }
     // up

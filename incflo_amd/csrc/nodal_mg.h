@@ -1,6 +1,8 @@
 // nodal_mg.h -- nodal multigrid hierarchy and the outer MLMG iteration.
 #pragma once
 
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -15,6 +17,7 @@ struct MGOpts {
     int maxiter = 100;
     int bottom_maxiter = 100;
     double bottom_rtol = 1e-4;
+    int bottom_solver = MGX_BOTTOM_BICGCG;
     int max_coarsening_level = 100;
     int nu1 = 2, nu2 = 2;
     double omega = 8.0 / 9.0;
@@ -33,7 +36,13 @@ inline void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
     MGX_REQUIRE(o->omega > 0.0 && o->omega < 2.0, MGX_ERR_ARG, "omega must lie in (0,2)");
     MGX_REQUIRE(o->cycles_per_update >= 0 && o->cycles_per_update <= 8, MGX_ERR_ARG,
                 "cycles_per_update must lie in 0..8");
+    MGX_REQUIRE(o->bottom_solver != MGX_BOTTOM_HYPRE, MGX_ERR_UNSUPPORTED,
+                "bottom_solver hypre is not available (bicgcg, bicgstab, cg, cgbicg, smoother)");
+    MGX_REQUIRE(o->bottom_solver >= MGX_BOTTOM_DEFAULT && o->bottom_solver <= MGX_BOTTOM_SMOOTHER,
+                MGX_ERR_ARG, "unknown bottom_solver");
     m.verbose = o->verbose;
+    m.bottom_solver = o->bottom_solver == MGX_BOTTOM_DEFAULT ? MGOpts{}.bottom_solver
+                                                             : o->bottom_solver;
     m.maxiter = o->maxiter;
     m.bottom_maxiter = o->bottom_maxiter;
     m.bottom_rtol = o->bottom_rtol;
@@ -73,9 +82,37 @@ struct MGLevel {
     T* r = nullptr;
 };
 
-// Launch one 27-point stencil pass (Jacobi / residual / update+residual).
+// The nodes of a level that take part in a computation (an AMR fine level:
+// the nodes in or on the boundary of the fine region) and, built on first use
+// per tile shape, the lists of the launch tiles that hold any of them.
+// Launches over such a level visit those tiles only; the rest of its arrays
+// must hold finite values that nothing reads (zeroed once).
+class TileSet {
+   public:
+    // the tile shapes of the launches over such a level
+    static constexpr int kTX = 16, kTY = 16, kKZ = 8;
+    void build(const NodalGeom& g, const std::vector<unsigned char>& active);
+    bool built() const { return nn_[0] > 0; }
+    // stencil tiles of kTX x kTY nodes x kz planes on a gx x gy x gz tile grid
+    // (linear indices); count 0: none
+    const int* tiles(int kz, int gx, int gy, int gz, int& count);
+    // kTX x kTY blocks of one plane for the point kernels: (bx, by, k, 0)
+    const int4* blocks(int& count);
+
+   private:
+    int nn_[3] = {0, 0, 0};
+    int gx_ = 0, gy_ = 0;
+    std::vector<unsigned char> blk2_;  // any active node in (x block, y block, plane)
+    std::map<int, std::pair<DevBuf, int>> lists_;  // kz -> tiles
+    DevBuf blk_;
+    int nblk_ = -1;
+};
+
+// Launch one 27-point stencil pass (Jacobi / residual / update+residual);
+// ts (optional) restricts it to the tiles that hold active nodes.
 template <class T, class TC, int MODE>
-void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine);
+void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine,
+                    TileSet* ts = nullptr);
 
 // Optional stream-ordered timing of selected launches (event pairs).
 struct LaunchTimer {

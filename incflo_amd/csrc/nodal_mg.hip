@@ -39,6 +39,7 @@ struct StencilShape {
     int npt = 2;
     long target = 1024;
     int kzmin = 2;
+    int dma = 0;  // Jacobi with 4-byte storage and npt 2: LDS-direct plane loads
 };
 struct ShapeKey {
     int nn0, nn1, nn2, word, cword, mode, fine;
@@ -86,6 +87,20 @@ static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s
     a.gz = (g.nn[2] + kz - 1) / kz;
     const unsigned nb = (unsigned)(gxy * a.gz);
     dim3 blk(TX, NY / NPT);
+    if constexpr (MODE == KS_JACOBI && NPT == 2 && std::is_same<T, float>::value &&
+                  std::is_same<TC, float>::value) {
+        if (sh.dma && !a.nmask) {
+            if (g.iso()) {
+                if (fine)
+                    nodal_jacobi_dma_kernel<T, true, true, 4><<<nb, blk, 0, s>>>(a);
+                else
+                    nodal_jacobi_dma_kernel<T, true, false, 4><<<nb, blk, 0, s>>>(a);
+            } else {
+                nodal_jacobi_dma_kernel<T, false, false, 4><<<nb, blk, 0, s>>>(a);
+            }
+            return;
+        }
+    }
     if (a.nmask) {
         // composite-level sweeps (node types); kept out of the plain kernels
         if (g.iso())
@@ -115,10 +130,99 @@ static void launch_stencil_shape(const NodalGeom& g, SArgs<T, TC> a, hipStream_t
         launch_stencil_npt<T, TC, MODE, 2>(g, a, s, fine, sh);
 }
 
+// a pass over the active tiles of a partly used level: 16 x 16-node tiles of
+// 8 planes, one workgroup each
 template <class T, class TC, int MODE>
-void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine) {
+static void launch_stencil_tiles(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, TileSet* ts) {
+    constexpr int TX = TileSet::kTX, TY = TileSet::kTY;
+    a.gx = (g.nn[0] + TX - 1) / TX;
+    a.gy = (g.nn[1] + TY - 1) / TY;
+    a.kz = std::min(TileSet::kKZ, g.nn[2]);
+    a.gz = (g.nn[2] + a.kz - 1) / a.kz;
+    int cnt = 0;
+    a.tiles = ts->tiles(a.kz, a.gx, a.gy, a.gz, cnt);
+    if (cnt == 0) return;
+    const dim3 blk(TX, TY);
+    if (a.nmask) {
+        if (g.iso())
+            nodal_stencil_kernel<T, TC, MODE, true, false, 1, true, TX><<<cnt, blk, 0, s>>>(a);
+        else
+            nodal_stencil_kernel<T, TC, MODE, false, false, 1, true, TX><<<cnt, blk, 0, s>>>(a);
+    } else {
+        if (g.iso())
+            nodal_stencil_kernel<T, TC, MODE, true, false, 1, false, TX><<<cnt, blk, 0, s>>>(a);
+        else
+            nodal_stencil_kernel<T, TC, MODE, false, false, 1, false, TX><<<cnt, blk, 0, s>>>(a);
+    }
+}
+
+template <class T, class TC, int MODE>
+void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine, TileSet* ts) {
+    if (ts) {
+        launch_stencil_tiles<T, TC, MODE>(g, a, s, ts);
+        return;
+    }
     launch_stencil_shape<T, TC, MODE>(
         g, a, s, fine, stencil_shape(g, (int)sizeof(T), (int)sizeof(TC), MODE, fine));
+}
+
+void TileSet::build(const NodalGeom& g, const std::vector<unsigned char>& active) {
+    MGX_REQUIRE((long)active.size() == g.nnode(), MGX_ERR_ARG, "tile set: one flag per node");
+    for (int d = 0; d < 3; ++d) nn_[d] = g.nn[d];
+    gx_ = (nn_[0] + kTX - 1) / kTX;
+    gy_ = (nn_[1] + kTY - 1) / kTY;
+    blk2_.assign((size_t)gx_ * gy_ * nn_[2], 0);
+    for (int k = 0; k < nn_[2]; ++k)
+        for (int j = 0; j < nn_[1]; ++j)
+            for (int i = 0; i < nn_[0]; ++i)
+                if (active[i + (size_t)nn_[0] * (j + (size_t)nn_[1] * k)])
+                    blk2_[i / kTX + (size_t)gx_ * (j / kTY + (size_t)gy_ * k)] = 1;
+    lists_.clear();
+    nblk_ = -1;
+}
+
+const int* TileSet::tiles(int kz, int gx, int gy, int gz, int& count) {
+    MGX_REQUIRE(built() && gx == gx_ && gy == gy_, MGX_ERR_STATE,
+                "tile set: not built for this level");
+    auto it = lists_.find(kz);
+    if (it == lists_.end()) {
+        std::vector<int> t;
+        for (int bz = 0; bz < gz; ++bz)
+            for (int by = 0; by < gy; ++by)
+                for (int bx = 0; bx < gx; ++bx) {
+                    bool any = false;
+                    for (int k = bz * kz; k < std::min((bz + 1) * kz, nn_[2]) && !any; ++k)
+                        any = blk2_[bx + (size_t)gx_ * (by + (size_t)gy_ * k)] != 0;
+                    if (any) t.push_back(bx + gx * (by + gy * bz));
+                }
+        DevBuf buf;
+        if (!t.empty()) {
+            buf.alloc(t.size() * sizeof(int));
+            MGX_HIP(hipMemcpy(buf.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        it = lists_.emplace(kz, std::make_pair(std::move(buf), (int)t.size())).first;
+    }
+    count = it->second.second;
+    return it->second.first.as<int>();
+}
+
+const int4* TileSet::blocks(int& count) {
+    MGX_REQUIRE(built(), MGX_ERR_STATE, "tile set: not built");
+    if (nblk_ < 0) {
+        std::vector<int4> t;
+        for (int k = 0; k < nn_[2]; ++k)
+            for (int by = 0; by < gy_; ++by)
+                for (int bx = 0; bx < gx_; ++bx)
+                    if (blk2_[bx + (size_t)gx_ * (by + (size_t)gy_ * k)])
+                        t.push_back(make_int4(bx, by, k, 0));
+        nblk_ = (int)t.size();
+        if (!t.empty()) {
+            blk_.alloc(t.size() * sizeof(int4));
+            MGX_HIP(hipMemcpy(blk_.p, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice));
+        }
+    }
+    count = nblk_;
+    return blk_.as<int4>();
 }
 
 // Full-weighting restriction onto coarse level gc (z-streamed, LDS-staged);
@@ -277,7 +381,8 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
 
 static bool same_opts(const MGOpts& a, const MGOpts& b) {
     return a.nu1 == b.nu1 && a.nu2 == b.nu2 && a.omega == b.omega &&
-           a.bottom_maxiter == b.bottom_maxiter && a.bottom_rtol == b.bottom_rtol;
+           a.bottom_maxiter == b.bottom_maxiter && a.bottom_rtol == b.bottom_rtol &&
+           a.bottom_solver == b.bottom_solver;
 }
 
 template <class T>
@@ -422,10 +527,10 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
         MGLevel<T>& lb = L[nl - 1];
         const long nb = lb.g.nnode();
         const int threads = nb >= 1024 ? 1024 : (nb >= 256 ? 256 : 64);
-        bottom_bicgstab_kernel<T><<<1, threads, 0, s>>>(
+        bottom_solve_kernel<T><<<1, threads, 0, s>>>(
             lb.kg, make_coef<double>(lb.g.dx), lb.sig, mask(nl - 1), lb.b, lb.x,
-            bottom_work.as<double>(), o.bottom_maxiter, o.bottom_rtol, bottom_singular() ? 1 : 0,
-            bottom_iters);
+            bottom_work.as<double>(), o.bottom_solver, o.bottom_maxiter, o.bottom_rtol,
+            nodal_omega(lb.g, o.omega), bottom_singular() ? 1 : 0, bottom_iters);
     } else {
         CoarseArgs<T> ca{};
         ca.nl = nl - lc;
@@ -448,6 +553,8 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
         ca.nu2 = o.nu2;
         ca.cb = make_coef<double>(lb.g.dx);
         ca.work = bottom_work.as<double>();
+        ca.bottom_kind = o.bottom_solver;
+        ca.bottom_omega = nodal_omega(lb.g, o.omega);
         ca.maxiter = o.bottom_maxiter;
         ca.rtol = o.bottom_rtol;
         ca.singular = bottom_singular() ? 1 : 0;
@@ -465,13 +572,13 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
     }
 }
 
-template void launch_stencil<double, double, KS_APPLY>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
-template void launch_stencil<double, double, KS_RESID>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
-template void launch_stencil<double, double, KS_JACOBI>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
-template void launch_stencil<double, double, KS_JFIRST>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool);
-template void launch_stencil<float, float, KS_APPLY>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool);
-template void launch_stencil<float, float, KS_JACOBI>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool);
-template void launch_stencil<float, float, KS_JFIRST>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool);
+template void launch_stencil<double, double, KS_APPLY>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool, TileSet*);
+template void launch_stencil<double, double, KS_RESID>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool, TileSet*);
+template void launch_stencil<double, double, KS_JACOBI>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool, TileSet*);
+template void launch_stencil<double, double, KS_JFIRST>(const NodalGeom&, SArgs<double, double>, hipStream_t, bool, TileSet*);
+template void launch_stencil<float, float, KS_APPLY>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool, TileSet*);
+template void launch_stencil<float, float, KS_JACOBI>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool, TileSet*);
+template void launch_stencil<float, float, KS_JFIRST>(const NodalGeom&, SArgs<float, float>, hipStream_t, bool, TileSet*);
 
 template class NodalHierarchy<double>;
 template class NodalHierarchy<float>;
@@ -749,7 +856,8 @@ struct TuneTimer {
 };
 
 // Candidate shapes, one per distinct (NPT, z-run) they give on this level.
-std::vector<StencilShape> shape_candidates(const NodalGeom& g) {
+// dma: also the LDS-direct variants (Jacobi, 4-byte storage, npt 2)
+std::vector<StencilShape> shape_candidates(const NodalGeom& g, bool dma) {
     std::vector<StencilShape> out;
     std::vector<std::pair<int, int>> seen;
     for (int npt : {1, 2, 4}) {
@@ -767,6 +875,10 @@ std::vector<StencilShape> shape_candidates(const NodalGeom& g) {
                 sh.target = target;
                 sh.kzmin = kzmin;
                 out.push_back(sh);
+                if (dma && npt == 2) {
+                    sh.dma = 1;
+                    out.push_back(sh);
+                }
             }
     }
     return out;
@@ -779,7 +891,9 @@ void tune_pass(const NodalGeom& g, const SArgs<T, TC>& a, hipStream_t s, bool fi
     const double t_def = tt.time([&] { launch_stencil_shape<T, TC, MODE>(g, a, s, fine, def); }, s);
     StencilShape best = def;
     double t_best = t_def;
-    for (const StencilShape& c : shape_candidates(g)) {
+    constexpr bool dma = MODE == KS_JACOBI && std::is_same<T, float>::value &&
+                         std::is_same<TC, float>::value;
+    for (const StencilShape& c : shape_candidates(g, dma)) {
         const double t = tt.time([&] { launch_stencil_shape<T, TC, MODE>(g, a, s, fine, c); }, s);
         if (t < 0.97 * t_best) {  // keep the incumbent unless clearly faster
             t_best = t;
@@ -793,8 +907,8 @@ void tune_pass(const NodalGeom& g, const SArgs<T, TC>& a, hipStream_t s, bool fi
     }
     rep << "stencil " << g.nn[0] << " " << g.nn[1] << " " << g.nn[2] << " " << sizeof(T) << " "
         << sizeof(TC) << " " << MODE << " " << (fine ? 1 : 0) << " npt " << best.npt << " target "
-        << best.target << " kzmin " << best.kzmin << " us " << t_best << " default_us " << t_def
-        << "\n";
+        << best.target << " kzmin " << best.kzmin << " dma " << best.dma << " us " << t_best
+        << " default_us " << t_def << "\n";
 }
 
 template <class T>
@@ -915,7 +1029,7 @@ std::string tuning_table() {
         const ShapeKey& k = kv.first;
         o << "stencil " << k.nn0 << " " << k.nn1 << " " << k.nn2 << " " << k.word << " " << k.cword
           << " " << k.mode << " " << k.fine << " " << kv.second.npt << " " << kv.second.target << " "
-          << kv.second.kzmin << "\n";
+          << kv.second.kzmin << " " << kv.second.dma << "\n";
     }
     o << "coarse_nodes " << g_coarse_nodes << "\n";
     return o.str();
@@ -931,10 +1045,12 @@ void set_tuning_table(const std::string& text) {
             ShapeKey k{};
             StencilShape sh;
             in >> k.nn0 >> k.nn1 >> k.nn2 >> k.word >> k.cword >> k.mode >> k.fine >> sh.npt >>
-                sh.target >> sh.kzmin;
+                sh.target >> sh.kzmin >> sh.dma;
             MGX_REQUIRE(!in.fail() && (sh.npt == 1 || sh.npt == 2 || sh.npt == 4) && sh.target > 0 &&
-                            (sh.kzmin == 1 || sh.kzmin == 2),
+                            (sh.kzmin == 1 || sh.kzmin == 2) && (sh.dma == 0 || sh.dma == 1),
                         MGX_ERR_ARG, "tuning table: bad stencil line");
+            // the LDS-direct sweep exists for the 4-byte Jacobi pass with npt 2 only
+            if (!(k.mode == KS_JACOBI && k.word == 4 && k.cword == 4 && sh.npt == 2)) sh.dma = 0;
             shapes[k] = sh;
         } else if (tag == "coarse_nodes") {
             in >> coarse;

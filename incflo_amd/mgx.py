@@ -16,6 +16,11 @@ LIB_PATH = os.path.join(HERE, "libmgx.so")
 MGX_OK, MGX_ERR_ARG, MGX_ERR_HIP, MGX_ERR_NOT_CONVERGED, MGX_ERR_STATE, MGX_ERR_UNSUPPORTED = range(6)
 BC_PERIODIC, BC_DIRICHLET, BC_NEUMANN, BC_INFLOW, BC_ROBIN, BC_INOUT = range(6)
 MODE_FP64, MODE_MIXED, MODE_LEARNED = 0, 1, 2
+(BOTTOM_DEFAULT, BOTTOM_BICGSTAB, BOTTOM_CG, BOTTOM_BICGCG, BOTTOM_CGBICG, BOTTOM_SMOOTHER,
+ BOTTOM_HYPRE) = range(7)
+# the bottom_solver strings of incflo's inputs (nodal_proj.bottom_solver, ...)
+BOTTOM_SOLVERS = {"bicgstab": BOTTOM_BICGSTAB, "cg": BOTTOM_CG, "bicgcg": BOTTOM_BICGCG,
+                  "cgbicg": BOTTOM_CGBICG, "smoother": BOTTOM_SMOOTHER, "hypre": BOTTOM_HYPRE}
 
 
 class MgxError(RuntimeError):
@@ -35,7 +40,8 @@ class MLMGOpts(C.Structure):
                 ("bottom_maxiter", C.c_int), ("bottom_rtol", C.c_double),
                 ("bottom_atol", C.c_double), ("max_coarsening_level", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("omega", C.c_double),
-                ("stall_iters", C.c_int), ("cycles_per_update", C.c_int)]
+                ("stall_iters", C.c_int), ("cycles_per_update", C.c_int),
+                ("bottom_solver", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -179,6 +185,14 @@ def lib():
             "mgx_write_plotfile": [vp, C.c_char_p, C.c_int, C.POINTER(LevelDesc), C.POINTER(vp),
                                    C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_double,
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)],
+            "mgx_tag_cells": [vp, C.POINTER(LevelDesc), vp, C.c_int, C.c_double, C.c_double,
+                              vp, vp, vp],
+            "mgx_cluster_tags": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                 C.c_int, C.c_double, C.POINTER(C.c_int), C.c_int,
+                                 C.POINTER(C.c_int)],
+            "mgx_fill_fine_level": [vp, C.POINTER(LevelDesc), vp, C.c_int, C.c_int,
+                                    C.POINTER(C.c_int), C.POINTER(LevelDesc), vp, C.c_int,
+                                    C.POINTER(LevelDesc), vp, C.c_int],
             "mgx_step_create": [vp, C.POINTER(LevelDesc), C.POINTER(vp)],
             "mgx_step_destroy": [vp],
             "mgx_fill_boundary": [vp, vp, C.c_int, C.c_int],
@@ -230,6 +244,8 @@ def default_opts(**kw):
     o = MLMGOpts()
     lib().mgx_default_mlmg_opts(C.byref(o))
     for k, v in kw.items():
+        if k == "bottom_solver" and isinstance(v, str):
+            v = BOTTOM_SOLVERS[v]
         setattr(o, k, v)
     return o
 
@@ -769,6 +785,59 @@ def write_plotfile(ctx, path, levels, data, ng, varnames, time, level_steps=None
     rr = None if ref_ratio is None else (C.c_int * max(nl - 1, 1))(*ref_ratio)
     ctx.check(lib().mgx_write_plotfile(ctx.h, path.encode(), nl, arr, ptrs, ng, len(varnames),
                                        names, time, ls, rr))
+
+
+# ---- dynamic refinement (ErrorEst, grid generation, FillPatch) ---------------
+def tag_cells(ctx, level, rho=None, rho_ng=0, rhoerr=None, gradrhoerr=None, region=None,
+              tags=None):
+    """incflo::ErrorEst on one level -> host uint8 tags (nz, ny, nx); None
+    disables a criterion; tags (a device uint8 array) accumulates when given."""
+    n = [level.n_cell[d] for d in range(3)]
+    t = tags if tags is not None else ctx.array(np.zeros(n[0] * n[1] * n[2], np.uint8), np.uint8)
+    lo = hi = None
+    if region is not None:
+        lo = (C.c_double * 3)(*region[0])
+        hi = (C.c_double * 3)(*region[1])
+    inf = float("inf")
+    ctx.check(lib().mgx_tag_cells(ctx.h, C.byref(level), _ptr(rho), rho_ng,
+                                  inf if rhoerr is None else rhoerr,
+                                  inf if gradrhoerr is None else gradrhoerr,
+                                  C.cast(lo, C.c_void_p) if lo else None,
+                                  C.cast(hi, C.c_void_p) if hi else None, t.ptr))
+    out = t.download().reshape(n[2], n[1], n[0])
+    if tags is None:
+        t.free()
+    return out
+
+
+def cluster_tags(tags, periodic=(1, 1, 1), n_error_buf=1, blocking_factor=8, max_grid_size=32,
+                 grid_eff=0.7, max_boxes=1 << 16):
+    """AmrMesh::MakeNewGrids for level 1 from level-0 tags (nz, ny, nx) ->
+    fine boxes [(lo, hi)] in fine index space."""
+    t = np.ascontiguousarray(tags, dtype=np.uint8)
+    n = (C.c_int * 3)(t.shape[2], t.shape[1], t.shape[0])
+    per = (C.c_int * 3)(*periodic)
+    out = (C.c_int * (6 * max_boxes))()
+    nb = C.c_int()
+    st = lib().mgx_cluster_tags(t.ctypes.data, n, per, n_error_buf, blocking_factor,
+                                max_grid_size, grid_eff, out, max_boxes, C.byref(nb))
+    if st != MGX_OK:
+        raise MgxError(st, "mgx_cluster_tags failed")
+    return [(tuple(out[6 * b:6 * b + 3]), tuple(out[6 * b + 3:6 * b + 6]))
+            for b in range(nb.value)]
+
+
+def fill_fine_level(ctx, crse, crse_field, c_ng, ncomp, fine, fine_field, f_ng, bcrec=None,
+                    old_fine=None, old_field=None, of_ng=0):
+    """FillPatchTwoLevels / InterpFromCoarseLevel with cell_cons_interp."""
+    bc = None
+    if bcrec is not None:
+        flat = np.ascontiguousarray(np.asarray(bcrec, dtype=np.int32).ravel())
+        bc = (C.c_int * flat.size)(*flat.tolist())
+    ctx.check(lib().mgx_fill_fine_level(
+        ctx.h, C.byref(crse), crse_field.ptr, c_ng, ncomp, bc, C.byref(fine), fine_field.ptr, f_ng,
+        C.byref(old_fine) if old_fine is not None else None,
+        old_field.ptr if old_field is not None else None, of_ng))
 
 
 def eb_cylinder_geometry(level, cx, cy, r):

@@ -81,6 +81,23 @@ typedef struct {
     const int* boxes;     /* nboxes*6 ints: lo[0..2], hi[0..2] (cell index, inclusive) */
 } mgx_level_desc;
 
+/* bottom_solver (MLMG::setBottomSolver through the nodal_proj.bottom_solver /
+ * mac_proj.bottom_solver / diffusion bottom_solver ParmParse strings,
+ * Docs/sphinx_documentation/source/InputsMultigrid.rst:34-35,
+ * src/diffusion/DiffusionScalarOp.cpp:306-314).  A Krylov bottom solve that
+ * does not reach bottom_rtol within bottom_maxiter is followed by 8
+ * smoothing sweeps (from zero when it broke down); "hypre" is refused with
+ * MGX_ERR_UNSUPPORTED. */
+enum {
+    MGX_BOTTOM_DEFAULT = 0,   /* bicgcg                                        */
+    MGX_BOTTOM_BICGSTAB = 1,
+    MGX_BOTTOM_CG = 2,
+    MGX_BOTTOM_BICGCG = 3,    /* BiCGStab, then CG from zero if it fails       */
+    MGX_BOTTOM_CGBICG = 4,    /* CG, then BiCGStab from zero if it fails       */
+    MGX_BOTTOM_SMOOTHER = 5,  /* 8 smoothing sweeps of the bottom level        */
+    MGX_BOTTOM_HYPRE = 6      /* refused                                       */
+};
+
 /* MLMG knobs: the nodal_proj.* / mac_proj.* ParmParse names
  * (src/incflo.H:409-445, src/setup/init.cpp:165-177,
  *  Docs/sphinx_documentation/source/InputsMultigrid.rst:10-36). */
@@ -108,6 +125,7 @@ typedef struct {
                                   single-level nodal and MAC solvers (the
                                   two-level AMR solvers run one composite
                                   cycle per step).  0 means the default, 3.  */
+    int bottom_solver;         /* MGX_BOTTOM_*, default bicgcg                 */
 } mgx_mlmg_opts;
 
 typedef struct {
@@ -177,8 +195,9 @@ MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on);
 MGX_API int mgx_nodal_set_vcycle_bits(mgx_nodal h, int bits);
 /* Launch-shape autotuner (no reference counterpart; AMReX picks fixed GPU
  * launch sizes).  Times candidate launch shapes of the nodal stencil kernel
- * (nodes per thread 1/2/4, z-run length through a target workgroup count)
- * on every level of this handle's hierarchy for `mode` (FP64 or the FP32
+ * (nodes per thread 1/2/4, z-run length through a target workgroup count,
+ * and for the 4-byte Jacobi sweep the variant whose plane loads go straight
+ * into LDS) on every level of this handle's hierarchy for `mode` (FP64 or the FP32
  * V-cycle of the mixed forms), for the Jacobi, first-sweep and residual
  * passes and the level-0 FP64 residual / update passes, then the number of
  * coarsest levels run inside the single-workgroup bottom launch (whole
@@ -189,7 +208,7 @@ MGX_API int mgx_nodal_set_vcycle_bits(mgx_nodal h, int bits);
  * the chosen shape and its time against the default's. */
 MGX_API int mgx_nodal_autotune(mgx_nodal h, int mode, char* report, int report_len);
 /* The tuning table as text ("stencil nn0 nn1 nn2 word cword pass fine npt
- * target kzmin" lines and "coarse_nodes N"), for caching per device model;
+ * target kzmin dma" lines and "coarse_nodes N"), for caching per device model;
  * mgx_tuning_set loads such a text (replacing the table). */
 MGX_API int mgx_tuning_get(char* buf, int len);
 MGX_API int mgx_tuning_set(const char* text);
@@ -591,6 +610,49 @@ MGX_API int mgx_write_plotfile(mgx_ctx ctx, const char* dir, int nlevels,
                                const mgx_level_desc* levels, const double* const* data, int ng,
                                int nvars, const char* const* varnames, double time,
                                const int* level_steps, const int* ref_ratio);
+
+/* ---- dynamic refinement of a two-level hierarchy (ratio 2) ---------------
+ * incflo regrids through AmrCore: ErrorEst tags coarse cells, AMReX's grid
+ * generation clusters them into fine boxes, and MakeNewLevelFromCoarse /
+ * RemakeLevel fill the fields of the new fine level (src/incflo_regrid.cpp:9-113).
+ *
+ * incflo::ErrorEst (src/incflo_tagging.cpp:11-198) on one level: tags[c] = 1
+ * (canonical cell index, x fastest; device bytes, other entries left as they
+ * are) where rho > rhoerr, where the largest one-sided density difference
+ * reaches gradrhoerr (rho then needs rho_ng >= 1 filled ghost cells), or
+ * where the cell centre lies in [region_lo, region_hi] (both NULL: no
+ * region).  A criterion whose threshold is not finite (INFINITY, NAN) is off,
+ * as an absent incflo.rhoerr / incflo.gradrhoerr is. */
+MGX_API int mgx_tag_cells(mgx_ctx ctx, const mgx_level_desc* level, const double* rho, int rho_ng,
+                          double rhoerr, double gradrhoerr, const double* region_lo,
+                          const double* region_hi, unsigned char* tags);
+/* AmrMesh::MakeNewGrids for level 1 over a level-0 domain of n_cell cells
+ * (host memory): tags buffered by n_error_buf cells (periodic directions
+ * wrap), coarsened to blocks of blocking_factor/2 coarse cells, clustered by
+ * Berger-Rigoutsos to grid_eff, chopped to max_grid_size and refined: fine
+ * boxes (6 ints each, fine index space) aligned to blocking_factor; at most
+ * max_boxes (MGX_ERR_ARG beyond).  amr.n_error_buf, amr.blocking_factor,
+ * amr.max_grid_size and amr.grid_eff of the inputs. */
+MGX_API int mgx_cluster_tags(const unsigned char* tags, const int n_cell[3],
+                             const int is_periodic[3], int n_error_buf, int blocking_factor,
+                             int max_grid_size, double grid_eff, int* boxes, int max_boxes,
+                             int* nboxes);
+/* FillPatchTwoLevels / InterpFromCoarseLevel with cell_cons_interp
+ * (src/boundary_conditions/incflo_fillpatch.cpp; ng = 0 as
+ * MakeNewLevelFromCoarse / RemakeLevel call them): the valid cells of the
+ * fine boxed field (f_ng ghosts, ncomp <= 8 components) take the old fine
+ * level's values where one of its boxes covers them (old_fine / old_field
+ * NULL: none, a level made from the coarse one) and the conservative linear
+ * interpolation of the coarse field elsewhere.  The components share each
+ * direction's slope limiter (as one FillPatch of a multi-component
+ * field does); bcrec: ncomp x 6 amrex::BCType values (NULL: all int_dir)
+ * select the one-sided slope next to ext_dir / hoextrap faces.  The coarse
+ * field needs c_ng >= 1 filled ghost cells on a bounded domain. */
+MGX_API int mgx_fill_fine_level(mgx_ctx ctx, const mgx_level_desc* crse, const double* crse_field,
+                                int c_ng, int ncomp, const int* bcrec,
+                                const mgx_level_desc* fine, double* fine_field, int f_ng,
+                                const mgx_level_desc* old_fine, const double* old_field,
+                                int of_ng);
 
 /* ---- learned coarse-level predictor --------------------------------------
  * No counterpart in the reference (incflo has no learned component); these

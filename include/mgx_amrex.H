@@ -563,6 +563,12 @@ class MLMG {
     void setBottomVerbose(int v) { o_.bottom_verbose = v; }
     void setBottomTolerance(double t) { o_.bottom_rtol = t; }
     void setBottomToleranceAbs(double t) { o_.bottom_atol = t; }
+    // MLMG::BottomSolver; hypre is refused at solve time (MGX_ERR_UNSUPPORTED)
+    enum class BottomSolver { Default = MGX_BOTTOM_DEFAULT, bicgstab = MGX_BOTTOM_BICGSTAB,
+                              cg = MGX_BOTTOM_CG, bicgcg = MGX_BOTTOM_BICGCG,
+                              cgbicg = MGX_BOTTOM_CGBICG, smoother = MGX_BOTTOM_SMOOTHER,
+                              hypre = MGX_BOTTOM_HYPRE };
+    void setBottomSolver(BottomSolver b) { o_.bottom_solver = static_cast<int>(b); }
     void setSolveMode(int mode) { mode_ = mode; }
     // returns the final residual norm, as amrex::MLMG::solve does
     double solve(std::vector<MultiFab*> sol, std::vector<MultiFab const*> rhs, double rtol,

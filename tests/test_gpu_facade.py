@@ -25,6 +25,18 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(scope="module", autouse=True)
+def default_shapes():
+    """The driver process starts from the default launch-shape table; the
+    direct calls here use it too (a tuned table left by an earlier test can
+    fold more coarse levels into one launch, which rounds differently)."""
+    from incflo_amd import mgx
+    saved = mgx.tuning_get()
+    mgx.tuning_set("coarse_nodes 1\n")
+    yield
+    mgx.tuning_set(saved)
+
+
 def _periodic_faces(rng, n):
     u = []
     for d in range(3):
