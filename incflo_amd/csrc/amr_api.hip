@@ -81,6 +81,7 @@ class AmrNodal {
     DevBuf bC;                          // level-0 rhs of the coarse correction (FP64-sized)
     DevBuf sigc, sigc_out, sigc_out32, sigf, sigf32;
     DevBuf ftype, cdof, ccov;
+    DevBuf cnear;                       // coarse nodes with a node of F's closure among their 27 fine nodes
     DevBuf xc, bc, rtmp, xf, bf, yf;
     DevBuf ef_a, ef_b, rfT, yT, rcT;    // V-cycle work (FP64-sized, reused for FP32)
     DevBuf scal, part, ibuf;
@@ -136,6 +137,32 @@ class AmrNodal {
             std::vector<unsigned char> act(ft.size());
             for (size_t q = 0; q < ft.size(); ++q) act[q] = ft[q] != AMR_OUT;
             ftiles.build(gf, act);
+        }
+        {
+            // the coarse nodes whose full-weighting stencil reaches a fine node
+            // of F's closure: the only ones the restrictions of fine data touch
+            std::vector<unsigned char> nr(gc.nnode(), 0);
+            for (int K = 0; K < gc.nn[2]; ++K)
+                for (int J = 0; J < gc.nn[1]; ++J)
+                    for (int I = 0; I < gc.nn[0]; ++I) {
+                        bool any = false;
+                        for (int oz = -1; oz <= 1 && !any; ++oz)
+                            for (int oy = -1; oy <= 1 && !any; ++oy)
+                                for (int ox = -1; ox <= 1 && !any; ++ox) {
+                                    int f[3] = {2 * I + ox, 2 * J + oy, 2 * K + oz};
+                                    bool in = true;
+                                    for (int d = 0; d < 3; ++d) {
+                                        if (gf.per[d]) f[d] = (f[d] + gf.nn[d]) % gf.nn[d];
+                                        else if (f[d] < 0 || f[d] >= gf.nn[d]) in = false;
+                                    }
+                                    if (in)
+                                        any = ft[f[0] + (size_t)gf.nn[0] * (f[1] + (size_t)gf.nn[1] * f[2])] !=
+                                              AMR_OUT;
+                                }
+                        nr[I + (size_t)gc.nn[0] * (J + (size_t)gc.nn[1] * K)] = any;
+                    }
+            cnear.alloc(nr.size());
+            MGX_HIP(hipMemcpy(cnear.p, nr.data(), nr.size(), hipMemcpyHostToDevice));
         }
         ftype.alloc(ft.size());
         cdof.alloc(cd.size());
@@ -248,7 +275,7 @@ class AmrNodal {
         launch_stencil<double, double, KS_RESID>(gc, c, s, false);
         amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
             kf, kc, ft, cdof.as<unsigned char>(), rtmp.as<double>(), yf.as<double>(), -1.0,
-            rcT.as<TO>(), scal.as<unsigned long long>() + 1);
+            rcT.as<TO>(), scal.as<unsigned long long>() + 1, cnear.as<unsigned char>());
         return read_norm(1, s);
     }
 
@@ -289,7 +316,8 @@ class AmrNodal {
         ap.rout = yT.as<T>();
         launch_stencil<T, T, KS_APPLY>(gf, ap, s, false, &ftiles);
         amr_coarse_rhs_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, cdof.as<unsigned char>(), rfT.as<T>(), yT.as<T>(), rcT.as<T>(), H.L[0].b);
+            kf, kc, ft, cdof.as<unsigned char>(), rfT.as<T>(), yT.as<T>(), rcT.as<T>(), H.L[0].b,
+            cnear.as<unsigned char>());
         H.vcycle(opts, s, ibuf.as<int>());
         if (nbk) amr_prolong_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, H.L[0].x, e, tl);
         for (int q = 0; q < opts.nu2; ++q) {
@@ -317,7 +345,8 @@ class AmrNodal {
             launch_stencil<T, T, KS_APPLY>(gc, ac, s, false);
             amr_coarse_jacobi_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
                 kf, kc, ft, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), yT.as<T>(), H.L[0].sig,
-                make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), H.L[0].x);
+                make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), H.L[0].x,
+                cnear.as<unsigned char>());
         }
         if (nbk) amr_add_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, ft, AMR_DOF, e, xf.as<double>(), tl);
         amr_add_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
@@ -347,6 +376,7 @@ class AmrNodal {
         const bool mixed = mode == MGX_MODE_MIXED;
         const KGeom kc = kgeom(gc), kf = kgeom(gf);
         const unsigned char* ft = ftype.as<unsigned char>();
+        ftiles.wide = amr_tile_width() == 64;
         if (mixed) prepare(H32, h32_sigma, s);
         else prepare(H64, h64_sigma, s);
         MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
@@ -360,7 +390,7 @@ class AmrNodal {
                 kf, ft, bf.as<double>(), nullptr, yf.as<double>(), scal.as<unsigned long long>(), tl);
         amr_coarse_combine_kernel<double><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
             kf, kc, ft, cdof.as<unsigned char>(), bc.as<double>(), yf.as<double>(), 0.0,
-            rtmp.as<double>(), scal.as<unsigned long long>());
+            rtmp.as<double>(), scal.as<unsigned long long>(), cnear.as<unsigned char>());
         const double bnorm = read_norm(0, s);
         MGX_HIP(hipMemsetAsync(xc.p, 0, gc.nnode() * sizeof(double), s));
         MGX_HIP(hipMemsetAsync(xf.p, 0, gf.nnode() * sizeof(double), s));
@@ -614,7 +644,7 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
         }
         amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
             kf, kc, ft, S.cdof.as<unsigned char>(), S.rtmp.as<double>(), S.yf.as<double>(), 1.0,
-            S.bc.as<double>(), nullptr);
+            S.bc.as<double>(), nullptr, S.cnear.as<unsigned char>());
         S.solve(rtol, atol, mode, info, s);
         // u -= sigma G phi on both levels, G phi out; covered coarse cells averaged down
         UpdateArgs uc{};

@@ -124,14 +124,14 @@ template <class TO>
 static __global__ void __launch_bounds__(256) amr_coarse_combine_kernel(
     KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
     const double* __restrict__ base, const double* __restrict__ y, double sgn, TO* __restrict__ out,
-    unsigned long long* norm) {
+    unsigned long long* norm, const unsigned char* __restrict__ near = nullptr) {
     int I, J, K;
     const bool valid = grid_point(gc.nn[0], gc.nn[1], I, J, K);
     double r = 0.0;
     if (valid) {
         const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
         if (cdof[id] && !is_dir(gc, I, J, K))
-            r = base[id] + sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K);
+            r = base[id] + ((near && !near[id]) ? 0.0 : sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K));
         out[id] = (TO)r;
     }
     if (norm) wave_max_atomic(norm, nabs(r));
@@ -160,11 +160,16 @@ static __global__ void __launch_bounds__(256) amr_fine_resid_kernel(
 template <class T>
 static __global__ void __launch_bounds__(256) amr_coarse_rhs_kernel(
     KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
-    const T* __restrict__ rf, const T* __restrict__ yf, const T* __restrict__ rc, T* __restrict__ bC) {
+    const T* __restrict__ rf, const T* __restrict__ yf, const T* __restrict__ rc, T* __restrict__ bC,
+    const unsigned char* __restrict__ near = nullptr) {
     int I, J, K;
     if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (is_dir(gc, I, J, K)) { bC[id] = 0; return; }
+    if (near && !near[id]) {  // no fine node of F's closure around: the coarse residual alone
+        bC[id] = cdof[id] ? rc[id] : (T)0;
+        return;
+    }
     double acc = 0.0;
     for (int oz = -1; oz <= 1; ++oz) {
         int fk = 2 * K + oz;
@@ -204,13 +209,14 @@ template <class T>
 static __global__ void __launch_bounds__(256) amr_coarse_jacobi_kernel(
     KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
     const T* __restrict__ rc, const T* __restrict__ yc, const T* __restrict__ yf,
-    const T* __restrict__ sig, double diag4, double omega, T* __restrict__ ec) {
+    const T* __restrict__ sig, double diag4, double omega, T* __restrict__ ec,
+    const unsigned char* __restrict__ near = nullptr) {
     int I, J, K;
     if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (!cdof[id] || is_dir(gc, I, J, K)) return;
     double acc = (double)yc[id];
-    for (int oz = -1; oz <= 1; ++oz) {
+    for (int oz = -1; oz <= 1 && !(near && !near[id]); ++oz) {
         int fk = 2 * K + oz;
         if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
         else if (fk < 0 || fk >= gf.nn[2]) continue;

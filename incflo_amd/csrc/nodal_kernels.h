@@ -557,6 +557,339 @@ static __global__ void __launch_bounds__(TXP == 64 && NPT == 1 ? 512 : 256)
     }
 }
 
+// Index wrap (periodic, any i) or clamp (non-periodic).
+__device__ __forceinline__ int wrapg(int i, int n, int per) {
+    if (per) return ((i % n) + n) % n;
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+// ghosted sigma index whose real cell is c - 1, wrapped the same way
+__device__ __forceinline__ int wrapsg(int c, int n, int sgn, int per) {
+    if (per) return wrapg(c - 1, n, 1) + 1;
+    return c < 0 ? 0 : (c >= sgn ? sgn - 1 : c);
+}
+
+// ---- the Jacobi sweep in wave strips (FP32, no LDS, no barriers) -----------
+// Each wave owns a strip of 64 columns x NR node rows (lanes 0 and 63 are the
+// halo columns, lanes 1..62 write) and streams it along z on its own: every
+// lane loads its column's x, sigma and b rows, and the neighbours across
+// columns come from the adjacent lanes by cross-lane shuffles, so waves never
+// wait for each other.  The four waves of a workgroup hold four strips
+// stacked in y.  Same formula at every node as nodal_stencil_kernel's
+// Jacobi pass.
+constexpr int kStripTX = 62;  // written columns per strip
+
+template <class T, bool ISO, int NR>
+static __global__ void __launch_bounds__(256) nodal_jacobi_strip_kernel(SArgs<T, T> a) {
+    static_assert(std::is_same<T, float>::value, "strip sweep: FP32");
+    using TA = T;
+    const KGeom& g = a.g;
+    const int lane = threadIdx.x, wv = threadIdx.y;
+    const int nb = a.gx * a.gy * a.gz;
+    const int lb = xcd_remap(blockIdx.x, nb);
+    const int bx = lb % a.gx, by = (lb / a.gx) % a.gy, bz = lb / (a.gx * a.gy);
+    const int i = bx * kStripTX - 1 + lane;     // this lane's column
+    const int jw = (by * 4 + wv) * NR;           // first node row of the strip
+    const int k0 = bz * a.kz, k1 = min(k0 + a.kz, g.nn[2]);
+    const long sxy = (long)g.nn[0] * g.nn[1];
+    const long ssxy = (long)g.sg[0] * g.sg[1];
+    const bool writes = lane >= 1 && lane <= kStripTX && i < g.nn[0];
+    const int gi = wrapg(i, g.nn[0], g.per[0]);
+    const int sc = wrapsg(i, g.n[0], g.sg[0], g.per[0]);  // ghosted cell left of node i
+    unsigned ox[NR + 2], os[NR + 1], ob[NR];
+#pragma unroll
+    for (int r = 0; r < NR + 2; ++r)
+        ox[r] = (unsigned)(gi + g.nn[0] * wrapg(jw - 1 + r, g.nn[1], g.per[1])) * 4u;
+#pragma unroll
+    for (int r = 0; r < NR + 1; ++r)
+        os[r] = (unsigned)(sc + g.sg[0] * wrapsg(jw + r, g.n[1], g.sg[1], g.per[1])) * 4u;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+        ob[r] = (unsigned)(gi + g.nn[0] * wrapg(jw + r, g.nn[1], g.per[1])) * 4u;
+
+    struct LoadSet {
+        TA x[NR + 2], s[NR + 1], b[NR];
+    };
+    LoadSet L0, L1;
+    auto issue = [&](LoadSet& ld_, int kk) {
+        const T* xk = a.x + (long)wrapg(kk, g.nn[2], g.per[2]) * sxy;
+        const T* sk = a.sig + (long)wrapsg(kk, g.n[2], g.sg[2], g.per[2]) * ssxy;
+        const T* bk = a.b + (long)wrapg(kk - 1, g.nn[2], g.per[2]) * sxy;
+#pragma unroll
+        for (int r = 0; r < NR + 2; ++r) ld_.x[r] = *byte_add(xk, ox[r]);
+#pragma unroll
+        for (int r = 0; r < NR + 1; ++r) ld_.s[r] = *byte_add(sk, os[r]);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) ld_.b[r] = *byte_add(bk, ob[r]);
+    };
+    TA P[3][NR + 2][3], S[2][NR + 1][2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < NR + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[p][r][c] = 0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < NR + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) S[p][r][c] = 0;
+    auto step = [&](int kk, LoadSet& cur) {
+        TA bcur[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) bcur[r] = cur.b[r];
+#pragma unroll
+        for (int r = 0; r < NR + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                P[0][r][c] = P[1][r][c];
+                P[1][r][c] = P[2][r][c];
+            }
+#pragma unroll
+        for (int r = 0; r < NR + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) S[1][r][c] = S[0][r][c];
+        // the new plane: own column, left and right neighbours from the lanes beside
+#pragma unroll
+        for (int r = 0; r < NR + 2; ++r) {
+            const TA v = cur.x[r];
+            P[2][r][0] = __shfl_up(v, 1, 64);
+            P[2][r][1] = v;
+            P[2][r][2] = __shfl_down(v, 1, 64);
+        }
+#pragma unroll
+        for (int r = 0; r < NR + 1; ++r) {
+            const TA v = cur.s[r];
+            S[0][r][0] = v;
+            S[0][r][1] = __shfl_down(v, 1, 64);
+        }
+        if (kk + 2 <= k1) issue(cur, kk + 2);
+        if (kk >= k0 + 1) {
+            const int k = kk - 1;
+            const long kp = sxy * k;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int j = jw + r;
+                if (!(writes && j < g.nn[1])) continue;
+                TA ssum;
+                TA acc;
+                if (r == 0) acc = apply27<TA, ISO, 0>(P[0], P[1], P[2], S[1], S[0], a.c, ssum);
+                else if (r == 1) acc = apply27<TA, ISO, (NR > 1 ? 1 : 0)>(P[0], P[1], P[2], S[1], S[0], a.c, ssum);
+                else if (r == 2) acc = apply27<TA, ISO, (NR > 2 ? 2 : 0)>(P[0], P[1], P[2], S[1], S[0], a.c, ssum);
+                else acc = apply27<TA, ISO, (NR > 3 ? 3 : 0)>(P[0], P[1], P[2], S[1], S[0], a.c, ssum);
+                const TA c = P[1][r + 1][1];
+                TA v = c;
+                if (is_dir(g, i, j, k)) v = 0;
+                else if (ssum != (TA)0) v = c + (bcur[r] - acc) * (a.omega * diag_rcp(-a.c.diag4 * ssum));
+                a.xout[kp + i + (long)g.nn[0] * j] = v;
+            }
+        }
+    };
+    issue(L0, k0 - 1);
+    if (k0 <= k1) issue(L1, k0);
+    for (int kk = k0 - 1; kk <= k1; kk += 2) {
+        step(kk, L0);
+        if (kk + 1 > k1) break;
+        step(kk + 1, L1);
+    }
+}
+
+// ---- two Jacobi sweeps in one pass (FP32) ----------------------------------
+// x2 = J(J(x)): the first sweep runs on a 64 x 8-node tile (the register
+// layout of nodal_stencil_kernel<..., KS_JACOBI, ..., 2>), its values go into
+// a ring of three LDS planes, and the second sweep, one plane behind, writes
+// the 62 x 6 interior nodes.  Tiles overlap by one node on each side, so every
+// node of the second sweep sees first-sweep values computed from the same
+// inputs as the unfused sweep; the pair reads x, b and sigma once (plus the
+// overlap) instead of twice and writes x once.
+
+constexpr int kPairTX = 62, kPairTY = 6;  // output nodes per tile
+
+template <class T, bool ISO>
+static __global__ void __launch_bounds__(256) nodal_jacobi_pair_kernel(SArgs<T, T> a) {
+    static_assert(std::is_same<T, float>::value, "the fused pair runs in FP32");
+    using TA = T;
+    constexpr int TX = 64, NY = 8, NPT = 2, TYT = 4, NT = 256;
+    constexpr int PR = NY + 2, PC = TX + 2, SR = NY + 1, SC = TX + 1;
+    constexpr int NQP = (PR + TYT - 1) / TYT, NQS = (SR + TYT - 1) / TYT;
+    __shared__ TA sphi[2][PR][PC];
+    __shared__ TA ssig[2][SR][SC];
+    __shared__ TA s1[3][NY][TX];  // first-sweep planes
+    const KGeom& g = a.g;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int nb = a.gx * a.gy * a.gz;
+    const int lb = xcd_remap(blockIdx.x, nb);
+    const int bx = lb % a.gx, by = (lb / a.gx) % a.gy, bz = lb / (a.gx * a.gy);
+    const int i0 = bx * kPairTX - 1, j0 = by * kPairTY - 1, k0 = bz * a.kz;
+    const int k1 = min(k0 + a.kz, g.nn[2]);  // output planes k0 .. k1-1
+    const int i = i0 + tx, jr = j0 + NPT * ty;
+    const long sxy = (long)g.nn[0] * g.nn[1];
+    const long ssxy = (long)g.sg[0] * g.sg[1];
+    const bool pout_x = tx >= 1 && tx <= kPairTX && i < g.nn[0];
+
+    // load slots (all indices wrapped / clamped into the arrays)
+    const int gi = wrapg(i, g.nn[0], g.per[0]);
+    int prow[NQP];
+#pragma unroll
+    for (int q = 0; q < NQP; ++q)
+        prow[q] = wrapg(j0 - 1 + min(ty + TYT * q, PR - 1), g.nn[1], g.per[1]);
+    const int hr = min(tid >> 1, PR - 1);
+    const int hcol = wrapg((tid & 1) ? i0 + TX : i0 - 1, g.nn[0], g.per[0]);
+    const int hrow = wrapg(j0 - 1 + hr, g.nn[1], g.per[1]);
+    const bool hst = tid < 2 * PR;
+    const int si = wrapsg(i0 + tx, g.n[0], g.sg[0], g.per[0]);
+    int srow[NQS];
+#pragma unroll
+    for (int q = 0; q < NQS; ++q)
+        srow[q] = wrapsg(j0 + min(ty + TYT * q, SR - 1), g.n[1], g.sg[1], g.per[1]);
+    const int sxr = wrapsg(j0 + min(tid, SR - 1), g.n[1], g.sg[1], g.per[1]);
+    const int sxc = wrapsg(i0 + TX, g.n[0], g.sg[0], g.per[0]);
+    const bool sxst = tid < SR;
+    unsigned opx[NQP], ohx, osg[NQS], osx, obb[NPT];
+#pragma unroll
+    for (int q = 0; q < NQP; ++q) opx[q] = (unsigned)(gi + g.nn[0] * prow[q]) * 4u;
+    ohx = (unsigned)(hcol + g.nn[0] * hrow) * 4u;
+#pragma unroll
+    for (int q = 0; q < NQS; ++q) osg[q] = (unsigned)(si + g.sg[0] * srow[q]) * 4u;
+    osx = (unsigned)(sxc + g.sg[0] * sxr) * 4u;
+#pragma unroll
+    for (int n = 0; n < NPT; ++n)
+        obb[n] = (unsigned)(gi + g.nn[0] * wrapg(jr + n, g.nn[1], g.per[1])) * 4u;
+
+    struct LoadSet {
+        TA lp[NQP], lh, ls[NQS], lsx, lb[NPT];
+    };
+    LoadSet L0, L1;
+    auto issue = [&](LoadSet& ld_, int kk) {
+        const T* xk = a.x + (long)wrapg(kk, g.nn[2], g.per[2]) * sxy;
+#pragma unroll
+        for (int q = 0; q < NQP; ++q) ld_.lp[q] = *byte_add(xk, opx[q]);
+        ld_.lh = *byte_add(xk, ohx);
+        const T* sk = a.sig + (long)wrapsg(kk, g.n[2], g.sg[2], g.per[2]) * ssxy;
+#pragma unroll
+        for (int q = 0; q < NQS; ++q) ld_.ls[q] = *byte_add(sk, osg[q]);
+        ld_.lsx = *byte_add(sk, osx);
+        const T* bk = a.b + (long)wrapg(kk - 1, g.nn[2], g.per[2]) * sxy;
+#pragma unroll
+        for (int n = 0; n < NPT; ++n) ld_.lb[n] = *byte_add(bk, obb[n]);
+    };
+
+    TA P[3][NPT + 2][3], S[3][NPT + 1][2], bprev[NPT];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < NPT + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[p][r][c] = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int r = 0; r < NPT + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) S[p][r][c] = 0;
+#pragma unroll
+    for (int n = 0; n < NPT; ++n) bprev[n] = 0;
+    const TA wdg = a.omega;
+
+    // step kk: x / sigma plane kk and b plane kk-1 arrive; the first sweep
+    // computes node plane kk-1, the second node plane kk-2
+    auto step = [&](int kk, LoadSet& cur) {
+        const int buf = kk & 1;
+#pragma unroll
+        for (int q = 0; q < NQP; ++q)
+            if (ty + TYT * q < PR) sphi[buf][ty + TYT * q][tx + 1] = cur.lp[q];
+        if (hst) sphi[buf][hr][(tid & 1) ? PC - 1 : 0] = cur.lh;
+#pragma unroll
+        for (int q = 0; q < NQS; ++q)
+            if (ty + TYT * q < SR) ssig[buf][ty + TYT * q][tx] = cur.ls[q];
+        if (sxst) ssig[buf][min(tid, SR - 1)][SC - 1] = cur.lsx;
+        TA bcur[NPT];
+#pragma unroll
+        for (int n = 0; n < NPT; ++n) bcur[n] = cur.lb[n];
+        if (kk + 2 <= k1 + 1) issue(cur, kk + 2);
+        __syncthreads();
+        // rotate: planes below <- centre <- above; sigma: older <- ... <- newest
+#pragma unroll
+        for (int r = 0; r < NPT + 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                P[0][r][c] = P[1][r][c];
+                P[1][r][c] = P[2][r][c];
+                P[2][r][c] = sphi[buf][NPT * ty + r][tx + c];
+            }
+#pragma unroll
+        for (int r = 0; r < NPT + 1; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                S[2][r][c] = S[1][r][c];
+                S[1][r][c] = S[0][r][c];
+                S[0][r][c] = ssig[buf][NPT * ty + r][tx + c];
+            }
+        // first sweep, node plane kk-1 (the whole 64 x 8 tile)
+        const int ka = kk - 1;
+        if (ka >= k0 - 1) {
+            const int kaw = wrapg(ka, g.nn[2], g.per[2]);
+#pragma unroll
+            for (int n = 0; n < NPT; ++n) {
+                TA ssum;
+                const TA acc = (n == 0) ? apply27<TA, ISO, 0>(P[0], P[1], P[2], S[1], S[0], a.c, ssum)
+                                        : apply27<TA, ISO, 1>(P[0], P[1], P[2], S[1], S[0], a.c, ssum);
+                const TA c = P[1][n + 1][1];
+                const int jw = wrapg(jr + n, g.nn[1], g.per[1]);
+                TA v = c;
+                if (is_dir(g, gi, jw, kaw)) v = 0;
+                else if (ssum != (TA)0) v = c + (bcur[n] - acc) * (wdg * diag_rcp(-a.c.diag4 * ssum));
+                s1[(ka + 3) % 3][NPT * ty + n][tx] = v;
+            }
+        }
+        __syncthreads();
+        // second sweep, node plane kk-2 (the 62 x 6 interior)
+        const int kb = kk - 2;
+        if (kb >= k0 && kb < k1) {
+            const int pa = (kb + 2) % 3, pm = (kb + 3) % 3, pn = (kb + 4) % 3;  // below, centre, above
+#pragma unroll
+            for (int n = 0; n < NPT; ++n) {
+                const int rr = NPT * ty + n;  // tile row of this node
+                const int j = jr + n;
+                if (!(pout_x && rr >= 1 && rr <= kPairTY && j < g.nn[1])) continue;
+                TA Q0[3][3], Q1[3][3], Q2[3][3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        Q0[r][c] = s1[pa][rr - 1 + r][tx - 1 + c];
+                        Q1[r][c] = s1[pm][rr - 1 + r][tx - 1 + c];
+                        Q2[r][c] = s1[pn][rr - 1 + r][tx - 1 + c];
+                    }
+                TA S0[2][2], S1[2][2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        S0[r][c] = S[2][n + r][c];
+                        S1[r][c] = S[1][n + r][c];
+                    }
+                TA ssum;
+                const TA acc = apply27<TA, ISO, 0>(Q0, Q1, Q2, S0, S1, a.c, ssum);
+                const TA c = Q1[1][1];
+                TA v = c;
+                if (is_dir(g, i, j, kb)) v = 0;
+                else if (ssum != (TA)0) v = c + (bprev[n] - acc) * (wdg * diag_rcp(-a.c.diag4 * ssum));
+                a.xout[sxy * kb + i + (long)g.nn[0] * j] = v;
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NPT; ++n) bprev[n] = bcur[n];
+    };
+    issue(L0, k0 - 2);
+    issue(L1, k0 - 1);
+    for (int kk = k0 - 2; kk <= k1 + 1; kk += 2) {
+        step(kk, L0);
+        if (kk + 1 > k1 + 1) break;
+        step(kk + 1, L1);
+    }
+}
+
 // ---- the Jacobi sweep with LDS-direct plane loads (4-byte storage) ---------
 // Same tile, z-streaming and arithmetic as nodal_stencil_kernel<..., KS_JACOBI,
 // ..., 2> (64 x 8 nodes, 64 x 4 threads, 2 nodes per thread), but every plane's

@@ -89,21 +89,24 @@ struct MGLevel {
 // must hold finite values that nothing reads (zeroed once).
 class TileSet {
    public:
-    // the tile shapes of the launches over such a level
+    // the block of the point kernels (one plane each), and the stencil tiles:
+    // 16 x 16 nodes (narrow: skip more) or 64 x 8 (wide: the dense kernel's
+    // shape), kKZ planes
     static constexpr int kTX = 16, kTY = 16, kKZ = 8;
     void build(const NodalGeom& g, const std::vector<unsigned char>& active);
     bool built() const { return nn_[0] > 0; }
-    // stencil tiles of kTX x kTY nodes x kz planes on a gx x gy x gz tile grid
-    // (linear indices); count 0: none
-    const int* tiles(int kz, int gx, int gy, int gz, int& count);
+    bool wide = false;  // stencil tiles 64 x 8 instead of 16 x 16
+    // stencil tiles of tx x ty nodes x kz planes on a gx x gy x gz tile grid
+    // (linear indices; tx a multiple of 16); count 0: none
+    const int* tiles(int tx, int ty, int kz, int gx, int gy, int gz, int& count);
     // kTX x kTY blocks of one plane for the point kernels: (bx, by, k, 0)
     const int4* blocks(int& count);
 
    private:
     int nn_[3] = {0, 0, 0};
-    int gx_ = 0, gy_ = 0;
-    std::vector<unsigned char> blk2_;  // any active node in (x block, y block, plane)
-    std::map<int, std::pair<DevBuf, int>> lists_;  // kz -> tiles
+    int g16_ = 0;
+    std::vector<unsigned char> col_;  // any active node in (16-column block, row, plane)
+    std::map<std::array<int, 3>, std::pair<DevBuf, int>> lists_;  // (tx, ty, kz) -> tiles
     DevBuf blk_;
     int nblk_ = -1;
 };
@@ -278,6 +281,8 @@ class NodalSolver {
 
 // the process-wide launch-shape table as text, and loading one back
 std::string tuning_table();
+// stencil tile width (16 or 64 nodes) of the launches over a TileSet
+int amr_tile_width();
 void set_tuning_table(const std::string& text);
 
 // Box list of one level, its device box-offset table and tile tables.

@@ -39,7 +39,9 @@ struct StencilShape {
     int npt = 2;
     long target = 1024;
     int kzmin = 2;
-    int dma = 0;  // Jacobi with 4-byte storage and npt 2: LDS-direct plane loads
+    // Jacobi with 4-byte storage and npt 2: 0 LDS-staged registers, 1 LDS-direct
+    // plane loads, 2 / 3 wave strips of 2 / 4 rows (cross-lane shuffles, no LDS)
+    int dma = 0;
 };
 struct ShapeKey {
     int nn0, nn1, nn2, word, cword, mode, fine;
@@ -71,6 +73,47 @@ static long coarse_threshold() {
     return g_coarse_nodes;
 }
 
+// two consecutive FP32 Jacobi sweeps of a level run as one fused pass
+// (nodal_jacobi_pair_kernel) when set in the tuning table; off by default:
+// at 256^3 the fused pass measured slower than two single sweeps (mixed
+// projection 8.85 against 6.85 ms)
+static int g_jacobi_pairs = 0;
+// stencil tile width of launches over a TileSet (16 or 64 nodes)
+static int g_amr_tile_width = 16;
+
+int amr_tile_width() {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    return g_amr_tile_width;
+}
+
+static bool jacobi_pairs_on() {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    return g_jacobi_pairs != 0;
+}
+
+template <class T>
+static bool launch_jacobi_pair(const NodalGeom&, SArgs<T, T>, hipStream_t) {
+    return false;
+}
+
+template <>
+bool launch_jacobi_pair<float>(const NodalGeom& g, SArgs<float, float> a, hipStream_t s) {
+    if (a.nmask || !jacobi_pairs_on()) return false;
+    a.gx = (g.nn[0] + kPairTX - 1) / kPairTX;
+    a.gy = (g.nn[1] + kPairTY - 1) / kPairTY;
+    const long gxy = (long)a.gx * a.gy;
+    int kz = (int)std::min<long>(256, std::max<long>(2, (g.nn[2] * gxy) / 1024));
+    kz = std::min(kz, g.nn[2]);
+    a.kz = kz;
+    a.gz = (g.nn[2] + kz - 1) / kz;
+    const unsigned nb = (unsigned)(gxy * a.gz);
+    if (g.iso())
+        nodal_jacobi_pair_kernel<float, true><<<nb, dim3(64, 4), 0, s>>>(a);
+    else
+        nodal_jacobi_pair_kernel<float, false><<<nb, dim3(64, 4), 0, s>>>(a);
+    return true;
+}
+
 template <class T, class TC, int MODE, int NPT>
 static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine,
                                const StencilShape& sh) {
@@ -89,7 +132,28 @@ static void launch_stencil_npt(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s
     dim3 blk(TX, NY / NPT);
     if constexpr (MODE == KS_JACOBI && NPT == 2 && std::is_same<T, float>::value &&
                   std::is_same<TC, float>::value) {
-        if (sh.dma && !a.nmask) {
+        if ((sh.dma == 2 || sh.dma == 3) && !a.nmask) {
+            // wave strips of 64 columns x NR rows, four per workgroup
+            const int nr = sh.dma == 2 ? 2 : 4;
+            SArgs<T, TC> b = a;
+            b.gx = (g.nn[0] + kStripTX - 1) / kStripTX;
+            b.gy = (g.nn[1] + 4 * nr - 1) / (4 * nr);
+            const long sxy_t = (long)b.gx * b.gy;
+            int kzs = (int)std::min<long>(256, std::max<long>(sh.kzmin, (g.nn[2] * sxy_t) / sh.target));
+            kzs = std::min(kzs, g.nn[2]);
+            b.kz = kzs;
+            b.gz = (g.nn[2] + kzs - 1) / kzs;
+            const unsigned nbs = (unsigned)(sxy_t * b.gz);
+            if (nr == 2) {
+                if (g.iso()) nodal_jacobi_strip_kernel<T, true, 2><<<nbs, blk, 0, s>>>(b);
+                else nodal_jacobi_strip_kernel<T, false, 2><<<nbs, blk, 0, s>>>(b);
+            } else {
+                if (g.iso()) nodal_jacobi_strip_kernel<T, true, 4><<<nbs, blk, 0, s>>>(b);
+                else nodal_jacobi_strip_kernel<T, false, 4><<<nbs, blk, 0, s>>>(b);
+            }
+            return;
+        }
+        if (sh.dma == 1 && !a.nmask) {
             if (g.iso()) {
                 if (fine)
                     nodal_jacobi_dma_kernel<T, true, true, 4><<<nb, blk, 0, s>>>(a);
@@ -134,13 +198,29 @@ static void launch_stencil_shape(const NodalGeom& g, SArgs<T, TC> a, hipStream_t
 // 8 planes, one workgroup each
 template <class T, class TC, int MODE>
 static void launch_stencil_tiles(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, TileSet* ts) {
-    constexpr int TX = TileSet::kTX, TY = TileSet::kTY;
-    a.gx = (g.nn[0] + TX - 1) / TX;
-    a.gy = (g.nn[1] + TY - 1) / TY;
     a.kz = std::min(TileSet::kKZ, g.nn[2]);
     a.gz = (g.nn[2] + a.kz - 1) / a.kz;
     int cnt = 0;
-    a.tiles = ts->tiles(a.kz, a.gx, a.gy, a.gz, cnt);
+    if (ts->wide) {
+        // the dense kernel's 64 x 8 tiles (2 nodes per thread)
+        a.gx = (g.nn[0] + 63) / 64;
+        a.gy = (g.nn[1] + 7) / 8;
+        a.tiles = ts->tiles(64, 8, a.kz, a.gx, a.gy, a.gz, cnt);
+        if (cnt == 0) return;
+        const dim3 blk(64, 4);
+        if (a.nmask) {
+            if (g.iso()) nodal_stencil_kernel<T, TC, MODE, true, false, 2, true><<<cnt, blk, 0, s>>>(a);
+            else nodal_stencil_kernel<T, TC, MODE, false, false, 2, true><<<cnt, blk, 0, s>>>(a);
+        } else {
+            if (g.iso()) nodal_stencil_kernel<T, TC, MODE, true, false, 2><<<cnt, blk, 0, s>>>(a);
+            else nodal_stencil_kernel<T, TC, MODE, false, false, 2><<<cnt, blk, 0, s>>>(a);
+        }
+        return;
+    }
+    constexpr int TX = TileSet::kTX, TY = TileSet::kTY;
+    a.gx = (g.nn[0] + TX - 1) / TX;
+    a.gy = (g.nn[1] + TY - 1) / TY;
+    a.tiles = ts->tiles(TX, TY, a.kz, a.gx, a.gy, a.gz, cnt);
     if (cnt == 0) return;
     const dim3 blk(TX, TY);
     if (a.nmask) {
@@ -169,30 +249,33 @@ void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine
 void TileSet::build(const NodalGeom& g, const std::vector<unsigned char>& active) {
     MGX_REQUIRE((long)active.size() == g.nnode(), MGX_ERR_ARG, "tile set: one flag per node");
     for (int d = 0; d < 3; ++d) nn_[d] = g.nn[d];
-    gx_ = (nn_[0] + kTX - 1) / kTX;
-    gy_ = (nn_[1] + kTY - 1) / kTY;
-    blk2_.assign((size_t)gx_ * gy_ * nn_[2], 0);
+    g16_ = (nn_[0] + 15) / 16;
+    col_.assign((size_t)g16_ * nn_[1] * nn_[2], 0);
     for (int k = 0; k < nn_[2]; ++k)
         for (int j = 0; j < nn_[1]; ++j)
             for (int i = 0; i < nn_[0]; ++i)
                 if (active[i + (size_t)nn_[0] * (j + (size_t)nn_[1] * k)])
-                    blk2_[i / kTX + (size_t)gx_ * (j / kTY + (size_t)gy_ * k)] = 1;
+                    col_[i / 16 + (size_t)g16_ * (j + (size_t)nn_[1] * k)] = 1;
     lists_.clear();
     nblk_ = -1;
 }
 
-const int* TileSet::tiles(int kz, int gx, int gy, int gz, int& count) {
-    MGX_REQUIRE(built() && gx == gx_ && gy == gy_, MGX_ERR_STATE,
-                "tile set: not built for this level");
-    auto it = lists_.find(kz);
+const int* TileSet::tiles(int tx, int ty, int kz, int gx, int gy, int gz, int& count) {
+    MGX_REQUIRE(built() && tx % 16 == 0 && gx == (nn_[0] + tx - 1) / tx &&
+                    gy == (nn_[1] + ty - 1) / ty,
+                MGX_ERR_STATE, "tile set: not built for this level");
+    auto it = lists_.find({tx, ty, kz});
     if (it == lists_.end()) {
+        const int c16 = tx / 16;
         std::vector<int> t;
         for (int bz = 0; bz < gz; ++bz)
             for (int by = 0; by < gy; ++by)
                 for (int bx = 0; bx < gx; ++bx) {
                     bool any = false;
                     for (int k = bz * kz; k < std::min((bz + 1) * kz, nn_[2]) && !any; ++k)
-                        any = blk2_[bx + (size_t)gx_ * (by + (size_t)gy_ * k)] != 0;
+                        for (int j = by * ty; j < std::min((by + 1) * ty, nn_[1]) && !any; ++j)
+                            for (int c = bx * c16; c < std::min((bx + 1) * c16, g16_) && !any; ++c)
+                                any = col_[c + (size_t)g16_ * (j + (size_t)nn_[1] * k)] != 0;
                     if (any) t.push_back(bx + gx * (by + gy * bz));
                 }
         DevBuf buf;
@@ -200,7 +283,9 @@ const int* TileSet::tiles(int kz, int gx, int gy, int gz, int& count) {
             buf.alloc(t.size() * sizeof(int));
             MGX_HIP(hipMemcpy(buf.p, t.data(), t.size() * sizeof(int), hipMemcpyHostToDevice));
         }
-        it = lists_.emplace(kz, std::make_pair(std::move(buf), (int)t.size())).first;
+        it = lists_.emplace(std::array<int, 3>{tx, ty, kz},
+                            std::make_pair(std::move(buf), (int)t.size()))
+                 .first;
     }
     count = it->second.second;
     return it->second.first.as<int>();
@@ -210,11 +295,15 @@ const int4* TileSet::blocks(int& count) {
     MGX_REQUIRE(built(), MGX_ERR_STATE, "tile set: not built");
     if (nblk_ < 0) {
         std::vector<int4> t;
+        const int gy = (nn_[1] + kTY - 1) / kTY;
         for (int k = 0; k < nn_[2]; ++k)
-            for (int by = 0; by < gy_; ++by)
-                for (int bx = 0; bx < gx_; ++bx)
-                    if (blk2_[bx + (size_t)gx_ * (by + (size_t)gy_ * k)])
-                        t.push_back(make_int4(bx, by, k, 0));
+            for (int by = 0; by < gy; ++by)
+                for (int bx = 0; bx < g16_; ++bx) {
+                    bool any = false;
+                    for (int j = by * kTY; j < std::min((by + 1) * kTY, nn_[1]) && !any; ++j)
+                        any = col_[bx + (size_t)g16_ * (j + (size_t)nn_[1] * k)] != 0;
+                    if (any) t.push_back(make_int4(bx, by, k, 0));
+                }
         nblk_ = (int)t.size();
         if (!t.empty()) {
             blk_.alloc(t.size() * sizeof(int4));
@@ -362,7 +451,7 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         launch_stencil<T, T, KS_JFIRST>(lv.g, a, s, false);
         --sweeps;
     }
-    for (int q = 0; q < sweeps; ++q) {
+    for (int q = 0; q < sweeps;) {
         SArgs<T, T> a{};
         a.g = lv.kg;
         a.c = lv.coef;
@@ -372,9 +461,14 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         a.xout = lv.x2;
         a.omega = (T)nodal_omega(lv.g, o.omega);
         a.nmask = mask(l);
-        if (l == 0 && timer) timer->begin(s);
-        launch_stencil<T, T, KS_JACOBI>(lv.g, a, s, l == 0);
-        if (l == 0 && timer) timer->end(s);
+        if (sweeps - q >= 2 && launch_jacobi_pair<T>(lv.g, a, s)) {
+            q += 2;
+        } else {
+            if (l == 0 && timer) timer->begin(s);
+            launch_stencil<T, T, KS_JACOBI>(lv.g, a, s, l == 0);
+            if (l == 0 && timer) timer->end(s);
+            ++q;
+        }
         std::swap(lv.x, lv.x2);
     }
 }
@@ -875,10 +969,11 @@ std::vector<StencilShape> shape_candidates(const NodalGeom& g, bool dma) {
                 sh.target = target;
                 sh.kzmin = kzmin;
                 out.push_back(sh);
-                if (dma && npt == 2) {
-                    sh.dma = 1;
-                    out.push_back(sh);
-                }
+                if (dma && npt == 2)
+                    for (int v : {1, 2, 3}) {  // LDS-direct, wave strips of 2 / 4 rows
+                        sh.dma = v;
+                        out.push_back(sh);
+                    }
             }
     }
     return out;
@@ -1032,6 +1127,8 @@ std::string tuning_table() {
           << kv.second.kzmin << " " << kv.second.dma << "\n";
     }
     o << "coarse_nodes " << g_coarse_nodes << "\n";
+    o << "jacobi_pairs " << g_jacobi_pairs << "\n";
+    o << "amr_tile_width " << g_amr_tile_width << "\n";
     return o.str();
 }
 
@@ -1040,6 +1137,7 @@ void set_tuning_table(const std::string& text) {
     std::string tag;
     std::map<ShapeKey, StencilShape> shapes;
     long coarse = 1;
+    int pairs = 0, tilew = 16;
     while (in >> tag) {
         if (tag == "stencil") {
             ShapeKey k{};
@@ -1047,14 +1145,22 @@ void set_tuning_table(const std::string& text) {
             in >> k.nn0 >> k.nn1 >> k.nn2 >> k.word >> k.cword >> k.mode >> k.fine >> sh.npt >>
                 sh.target >> sh.kzmin >> sh.dma;
             MGX_REQUIRE(!in.fail() && (sh.npt == 1 || sh.npt == 2 || sh.npt == 4) && sh.target > 0 &&
-                            (sh.kzmin == 1 || sh.kzmin == 2) && (sh.dma == 0 || sh.dma == 1),
+                            (sh.kzmin == 1 || sh.kzmin == 2) && sh.dma >= 0 && sh.dma <= 3,
                         MGX_ERR_ARG, "tuning table: bad stencil line");
-            // the LDS-direct sweep exists for the 4-byte Jacobi pass with npt 2 only
+            // the LDS-direct and strip sweeps exist for the 4-byte Jacobi pass with npt 2 only
             if (!(k.mode == KS_JACOBI && k.word == 4 && k.cword == 4 && sh.npt == 2)) sh.dma = 0;
             shapes[k] = sh;
         } else if (tag == "coarse_nodes") {
             in >> coarse;
             MGX_REQUIRE(!in.fail() && coarse >= 1, MGX_ERR_ARG, "tuning table: bad coarse_nodes");
+        } else if (tag == "amr_tile_width") {
+            in >> tilew;
+            MGX_REQUIRE(!in.fail() && (tilew == 16 || tilew == 64), MGX_ERR_ARG,
+                        "tuning table: bad amr_tile_width");
+        } else if (tag == "jacobi_pairs") {
+            in >> pairs;
+            MGX_REQUIRE(!in.fail() && (pairs == 0 || pairs == 1), MGX_ERR_ARG,
+                        "tuning table: bad jacobi_pairs");
         } else {
             throw Error(MGX_ERR_ARG, "tuning table: unknown line '" + tag + "'");
         }
@@ -1062,6 +1168,8 @@ void set_tuning_table(const std::string& text) {
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     g_shapes = std::move(shapes);
     g_coarse_nodes = coarse;
+    g_jacobi_pairs = pairs;
+    g_amr_tile_width = tilew;
 }
 
 // ---- learned coarse-level predictor ---------------------------------------

@@ -1,7 +1,7 @@
 """The launch-shape autotuner (mgx_nodal_autotune): launch shapes do not
 change any node's arithmetic, so projections after tuning are bit-identical
 to those before (within round-off when the tuner folds more coarse levels
-into the single-workgroup bottom launch); the
+into the single-workgroup bottom launch or picks a sweep variant); the
 table round-trips through mgx_tuning_get / mgx_tuning_set; the tuned shapes
 are not slower than the defaults (measured inside the tuner)."""
 import sys
@@ -57,12 +57,14 @@ def test_autotune_keeps_results_bitwise(ctx, mode):
         assert us <= dus * 1.0001, l
     p1, v1, i1 = run()
     table = mgx.tuning_get()
-    if "coarse_nodes 1\n" in table:
+    variants = any(l.split()[11] != "0" for l in table.splitlines() if l.startswith("stencil"))
+    if "coarse_nodes 1\n" in table and not variants:
         # launch shapes only: every node is computed by the same arithmetic
         assert i1.vcycles == i0.vcycles
         assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
     else:
-        # more levels in the single-workgroup bottom launch: same cycle,
+        # more levels in the single-workgroup bottom launch, or a sweep variant
+        # (the same formula, compiled with other contractions): same cycle,
         # different rounding
         assert abs(i1.vcycles - i0.vcycles) <= 1 and i1.converged
         assert np.max(np.abs(p1 - p0)) <= 1e-8 * np.max(np.abs(p0))

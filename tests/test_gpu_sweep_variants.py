@@ -1,6 +1,8 @@
-"""The Jacobi sweep with LDS-direct plane loads (nodal_jacobi_dma_kernel,
-selected per level through the launch-shape table) computes every node with
-the same formula as the register-staged sweep (the compiler may contract the
+"""The FP32 Jacobi sweep variants selected per level through the
+launch-shape table -- LDS-direct plane loads (nodal_jacobi_dma_kernel) and
+wave strips of 2 or 4 rows with cross-lane shuffles
+(nodal_jacobi_strip_kernel) -- compute every node with the same formula as
+the register-staged sweep (the compiler may contract the
 FP32 arithmetic differently, so the two agree to FP32 rounding, not bit for
 bit).  Checked on a periodic cube and on a bounded, anisotropic box whose
 tiles are partial in x and y: after one FP64 refinement step of three FP32
@@ -34,7 +36,7 @@ def ctx():
 
 
 def _force(table, dma):
-    """every 4-byte Jacobi line of the table -> npt 2, 1024 workgroups, dma"""
+    """every 4-byte Jacobi line of the table -> npt 2, 1024 workgroups, variant dma"""
     out = []
     for l in table.splitlines():
         f = l.split()
@@ -44,14 +46,15 @@ def _force(table, dma):
     return "\n".join(out) + "\n"
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("case", ["periodic", "bounded"])
-def test_dma_sweep_is_bitwise_the_staged_sweep(ctx, orc, case):
+def test_sweep_variants_match_the_staged_sweep(ctx, orc, case, variant):
     from incflo_amd import mgx
     if case == "periodic":
         n, per, bclo, bchi, ph = (64, 64, 64), (1, 1, 1), None, None, (1.0, 1.0, 1.0)
     else:
         n, per, bclo, bchi, ph = (40, 24, 20), (0, 0, 0), (D, N, N), (N, D, N), (2.0, 1.0, 1.5)
-    _say(f"case dma sweep {case} n={n}")
+    _say(f"case sweep variant {variant} {case} n={n}")
     g = orc.Geom.make(n, periodic=per, bclo=bclo or (0, 0, 0), bchi=bchi or (0, 0, 0),
                       prob_hi=ph)
     rng = np.random.default_rng(3)
@@ -76,11 +79,11 @@ def test_dma_sweep_is_bitwise_the_staged_sweep(ctx, orc, case):
         table = mgx.tuning_get()
         assert re.search(r"^stencil( \d+){3} 4 4 0 ", table, re.M), table
         out = {}
-        for dma in (0, 1):
+        for dma in (0, variant):
             mgx.tuning_set(_force(table, dma))
             out[dma] = (solve(1e-2), solve(1e-10))
         (a0, ia0), (x0, i0) = out[0]
-        (a1, ia1), (x1, i1) = out[1]
+        (a1, ia1), (x1, i1) = out[variant]
         assert ia0.vcycles == ia1.vcycles <= 3
         assert np.max(np.abs(a1 - a0)) <= 1e-5 * np.max(np.abs(a0))
         assert i0.converged and i1.converged and i1.vcycles == i0.vcycles
