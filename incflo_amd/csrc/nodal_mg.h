@@ -2,6 +2,7 @@
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <map>
 #include <string>
 #include <vector>
@@ -24,7 +25,24 @@ struct MGOpts {
     int verbose = 0;
     int stall_iters = 3;
     int cycles_per_update = 3;  // most FP32 V-cycles per FP64 refinement step (mixed modes)
+    int smoother = MGX_SMOOTHER_DEFAULT;
 };
+
+// Chebyshev sweeps for a hierarchy of FP32 (fp32) or FP64 storage
+inline bool use_chebyshev(const MGOpts& o, bool fp32) {
+    return o.smoother == MGX_SMOOTHER_CHEBYSHEV || (o.smoother == MGX_SMOOTHER_DEFAULT && fp32);
+}
+
+// Relaxation factor of sweep q (0-based) of n smoothing sweeps: weighted
+// Jacobi's omega, or the Chebyshev roots of degree n for the interval [0.5,
+// 1.5] of the Jacobi-scaled 27-point operator (its high-frequency range and
+// top at constant sigma on a cubic grid), smallest factor first; both scaled
+// down on anisotropic grids as nodal_omega does.
+inline double sweep_omega(const NodalGeom& g, const MGOpts& o, bool cheb, int q, int n) {
+    if (!cheb || n < 1) return nodal_omega(g, o.omega);
+    const double t = std::cos(3.14159265358979323846 * (2.0 * q + 1.0) / (2.0 * n));
+    return nodal_omega(g, 1.0 / (1.0 + 0.5 * t));
+}
 
 // The C options struct -> MGOpts, validated the same way for every solver
 // handle.  cycles_per_update 0 keeps the built-in default.
@@ -53,6 +71,9 @@ inline void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
     m.stall_iters = o->stall_iters;
     m.cycles_per_update = o->cycles_per_update == 0 ? MGOpts{}.cycles_per_update
                                                      : o->cycles_per_update;
+    MGX_REQUIRE(o->smoother >= MGX_SMOOTHER_DEFAULT && o->smoother <= MGX_SMOOTHER_CHEBYSHEV,
+                MGX_ERR_ARG, "unknown smoother");
+    m.smoother = o->smoother;
 }
 
 inline KGeom kgeom(const NodalGeom& g) {
@@ -152,6 +173,8 @@ class NodalHierarchy {
     // level-0 x holds an initial guess (the correction so far): the next cycle
     // pre-smooths from it instead of from zero
     bool x0_guess = false;
+    // Chebyshev factors for the per-level sweeps (set by vcycle from the options)
+    bool cheb = false;
     // optional overset mask per level (1 solve, 0 fixed at 0; empty: none)
     std::vector<const unsigned char*> masks;
     bool bottom_fixed = false;  // the bottom level's overset mask holds a node at zero

@@ -433,8 +433,10 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
     MGLevel<T>& lv = L[l];
     const long nn = lv.g.nnode();
     int sweeps = nsweeps;
+    int q0 = 0;  // index of the next sweep among the nsweeps
     if (from_zero && x_ready && sweeps > 0) {
         --sweeps;
+        q0 = 1;
     } else if (from_zero) {
         if (sweeps == 0) {
             zero_kernel<T><<<grid1d(nn), 256, 0, s>>>(lv.x, nn);
@@ -446,10 +448,11 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         a.b = lv.b;
         a.sig = lv.sig;
         a.xout = lv.x;
-        a.omega = (T)nodal_omega(lv.g, o.omega);
+        a.omega = (T)sweep_omega(lv.g, o, cheb, 0, nsweeps);
         a.nmask = mask(l);
         launch_stencil<T, T, KS_JFIRST>(lv.g, a, s, false);
         --sweeps;
+        q0 = 1;
     }
     for (int q = 0; q < sweeps;) {
         SArgs<T, T> a{};
@@ -459,9 +462,10 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
         a.b = lv.b;
         a.sig = lv.sig;
         a.xout = lv.x2;
-        a.omega = (T)nodal_omega(lv.g, o.omega);
+        a.omega = (T)sweep_omega(lv.g, o, cheb, q0 + q, nsweeps);
         a.nmask = mask(l);
-        if (sweeps - q >= 2 && launch_jacobi_pair<T>(lv.g, a, s)) {
+        // the fused pair applies one factor to both sweeps: Jacobi only
+        if (!cheb && sweeps - q >= 2 && launch_jacobi_pair<T>(lv.g, a, s)) {
             q += 2;
         } else {
             if (l == 0 && timer) timer->begin(s);
@@ -476,11 +480,12 @@ void NodalHierarchy<T>::smooth(int l, int nsweeps, bool from_zero, const MGOpts&
 static bool same_opts(const MGOpts& a, const MGOpts& b) {
     return a.nu1 == b.nu1 && a.nu2 == b.nu2 && a.omega == b.omega &&
            a.bottom_maxiter == b.bottom_maxiter && a.bottom_rtol == b.bottom_rtol &&
-           a.bottom_solver == b.bottom_solver;
+           a.bottom_solver == b.bottom_solver && a.smoother == b.smoother;
 }
 
 template <class T>
 void NodalHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) {
+    cheb = use_chebyshev(o, sizeof(T) == 4);
     // Levels below 0 are recomputed from scratch every cycle: start each from
     // its own buffer pair, so every cycle launches the same sequence.
     for (size_t l = 1; l < L.size(); ++l) {
@@ -615,7 +620,7 @@ void NodalHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_
         MGLevel<T>& lc1 = L[l + 1];
         ready = (l + 1 < lc) && o.nu1 > 0 && masks.empty();
         launch_restrict<T>(lv.kg, lc1.g, lc1.kg, lv.r, lc1.b, lc1.sig, lc1.coef.diag4,
-                           (T)nodal_omega(lc1.g, o.omega), ready ? lc1.x : nullptr, s);
+                           (T)sweep_omega(lc1.g, o, cheb, 0, o.nu1), ready ? lc1.x : nullptr, s);
     }
     if (!bottom_in_lds) {
         MGLevel<T>& lb = L[nl - 1];
@@ -1321,7 +1326,8 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         a.nmask = mask0;
         if (opts.nu1 > 0 && H.L.size() > 1 && !mask0) {
             a.x1out = H.L[0].x2;
-            a.omega1 = nodal_omega(g, opts.omega);
+            a.omega1 = sweep_omega(g, opts, use_chebyshev(opts, sizeof(*H.L[0].x) == 4), 0,
+                                   opts.nu1);
         }
     };
     auto done_first_sweep = [&](auto& a, auto& H) {

@@ -19,6 +19,7 @@ MODE_FP64, MODE_MIXED, MODE_LEARNED = 0, 1, 2
 (BOTTOM_DEFAULT, BOTTOM_BICGSTAB, BOTTOM_CG, BOTTOM_BICGCG, BOTTOM_CGBICG, BOTTOM_SMOOTHER,
  BOTTOM_HYPRE) = range(7)
 # the bottom_solver strings of incflo's inputs (nodal_proj.bottom_solver, ...)
+SMOOTHER_DEFAULT, SMOOTHER_JACOBI, SMOOTHER_CHEBYSHEV = 0, 1, 2
 BOTTOM_SOLVERS = {"bicgstab": BOTTOM_BICGSTAB, "cg": BOTTOM_CG, "bicgcg": BOTTOM_BICGCG,
                   "cgbicg": BOTTOM_CGBICG, "smoother": BOTTOM_SMOOTHER, "hypre": BOTTOM_HYPRE}
 
@@ -41,7 +42,7 @@ class MLMGOpts(C.Structure):
                 ("bottom_atol", C.c_double), ("max_coarsening_level", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("omega", C.c_double),
                 ("stall_iters", C.c_int), ("cycles_per_update", C.c_int),
-                ("bottom_solver", C.c_int)]
+                ("bottom_solver", C.c_int), ("smoother", C.c_int)]
 
 
 class SolveInfo(C.Structure):

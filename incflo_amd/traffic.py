@@ -150,6 +150,36 @@ def size_sweep(sizes):
     ctx.close()
 
 
+def conv_throughput(sizes, channels):
+    """The U-net's BF16-MFMA 3x3x3 convolution (mgx_conv3d_forward, FP32
+    activations rounded to BF16, FP32 accumulation) on batch-1 periodic cubes:
+    algorithmic 2 * 27 * cin * cout flop per voxel over the HIP-event time of
+    10 back-to-back launches, against the 2.5 PFLOP/s dense BF16 peak."""
+    from incflo_amd import mgx
+    ctx = mgx.Context(0)
+    rng = np.random.default_rng(0)
+    for n in sizes:
+        for c in channels:
+            x = ctx.array(rng.standard_normal(n ** 3 * c).astype(np.float32), np.float32)
+            w = ctx.array((0.05 * rng.standard_normal(c * 27 * c)).astype(np.float32), np.float32)
+            out = ctx.array(np.zeros(n ** 3 * c, np.float32), np.float32)
+            shape = (1, n, n, n)
+            for _ in range(2):
+                mgx.conv3d_forward(ctx, x, w, None, out, shape, c, c)
+            ctx.record(0)
+            for _ in range(10):
+                mgx.conv3d_forward(ctx, x, w, None, out, shape, c, c)
+            ctx.record(1)
+            ms = ctx.elapsed_ms(0, 1) / 10
+            flop = 2.0 * 27 * c * c * n ** 3
+            tf = flop / ms / 1e9
+            print(f"conv3d {n}^3 cin=cout={c}: {1e3 * ms:.1f} us  {tf:.1f} TFLOP/s "
+                  f"({tf / 2500.0:.3f} of the 2.5 PFLOP/s BF16 dense peak)", flush=True)
+            for arr in (x, w, out):
+                arr.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=256)
@@ -160,7 +190,13 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--summarize", nargs="+")
     ap.add_argument("--size-sweep", nargs="+", type=int)
+    ap.add_argument("--conv", nargs="+", type=int,
+                    help="time the BF16-MFMA convolution on these cube sizes")
+    ap.add_argument("--conv-channels", nargs="+", type=int, default=[16, 32, 64, 128])
     args = ap.parse_args()
+    if args.conv:
+        conv_throughput(args.conv, args.conv_channels)
+        return
     if args.size_sweep:
         size_sweep(args.size_sweep)
         return

@@ -98,6 +98,15 @@ enum {
     MGX_BOTTOM_HYPRE = 6      /* refused                                       */
 };
 
+/* Smoother of the nodal V-cycle's per-level sweeps.  MLNodeLaplacian's GPU
+ * smoother is weighted Jacobi; the Chebyshev form runs the same sweeps with
+ * the factors of the degree-nu Chebyshev polynomial for the high-frequency
+ * interval of the Jacobi-scaled operator (a smaller smoothing factor per
+ * pair of sweeps: 0.14 against 0.31 at constant sigma).  Default: Jacobi for
+ * FP64 hierarchies (MGX_MODE_FP64, as MLMG), Chebyshev for the FP32 V-cycles
+ * of the mixed and learned modes. */
+enum { MGX_SMOOTHER_DEFAULT = 0, MGX_SMOOTHER_JACOBI = 1, MGX_SMOOTHER_CHEBYSHEV = 2 };
+
 /* MLMG knobs: the nodal_proj.* / mac_proj.* ParmParse names
  * (src/incflo.H:409-445, src/setup/init.cpp:165-177,
  *  Docs/sphinx_documentation/source/InputsMultigrid.rst:10-36). */
@@ -126,6 +135,8 @@ typedef struct {
                                   two-level AMR solvers run one composite
                                   cycle per step).  0 means the default, 3.  */
     int bottom_solver;         /* MGX_BOTTOM_*, default bicgcg                 */
+    int smoother;              /* MGX_SMOOTHER_* (no reference counterpart):
+                                  the per-level sweeps of the nodal V-cycle    */
 } mgx_mlmg_opts;
 
 typedef struct {
