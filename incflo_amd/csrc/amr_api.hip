@@ -1,22 +1,24 @@
-// amr_api.hip -- two-level composite nodal projection (AMR, refinement ratio 2)
-// and its C ABI.
+// amr_api.hip -- composite nodal projection over AMR levels (refinement ratio
+// 2 between neighbours) and its C ABI.
 //
 // The reference builds one NodalProjector over all AMR levels
 // (src/projection/incflo_apply_nodal_projection.cpp:124-215), projects the
 // velocity of every level, returns phi / G phi per level and averages gp
 // down afterwards (:258-266).  Here the composite problem is the conforming
-// Q1 discretisation described in amr_kernels.h, solved by composite V-cycles:
-//   fine pre-smoothing on the fine unknowns (dF held at 0), the composite
-//   residual restricted to the whole coarse grid, one V-cycle of the coarse
-//   multigrid hierarchy, trilinear prolongation onto the fine unknowns and dF,
-//   fine post-smoothing
-// inside an FP64 outer iteration on the composite residual (MLMG's stopping
+// Q1 discretisation described in amr_kernels.h, solved by composite V-cycles
+// from the finest level down: on each level above 0, pre-smoothing of its
+// correction over its region (its boundary held at 0), the residual restricted
+// to the level below, that level's correction (level 0: one V-cycle of its
+// multigrid hierarchy over the whole domain), trilinear prolongation onto the
+// level's nodes, post-smoothing and one composite Jacobi sweep of the level
+// below's unknowns next to it; inside an FP64 outer iteration on the composite residual (MLMG's stopping
 // rule ||r||_inf <= max(atol, rtol ||b||_inf)); MGX_MODE_MIXED runs the
 // V-cycle in FP32.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "amr_kernels.h"
@@ -71,21 +73,55 @@ static bool is_dir_host(const NodalGeom& g, int i, int j, int k) {
            (g.dhi[1] && j == g.nn[1] - 1) || (g.dlo[2] && k == 0) || (g.dhi[2] && k == g.nn[2] - 1);
 }
 
+// One level above level 0: dense canonical arrays over the whole domain at
+// its resolution; only the tiles holding nodes of its region's closure are
+// ever visited.
+struct AmrFine {
+    NodalGeom g;
+    bool finer = false;                 // a still finer level lies above
+    DevBuf ftype;                       // AMR_OUT / AMR_DOF / AMR_BDRY against the level below
+    DevBuf dof;                         // composite unknowns: AMR_DOF nodes not surrounded by covered cells
+    DevBuf ccov, cnear;                 // (finer) cells under the finer level; nodes near its closure
+    DevBuf sig, sig_out, sig32, sig_out32;  // full / covered cells 0 (sig_out only when finer)
+    DevBuf x, b, y, rT, ea, eb, yT, bC, rtmp;
+    TileSet tiles;
+    const int4* blocks(int& n) { return tiles.blocks(n); }
+    template <class T> const T* sigT() const {
+        return std::is_same<T, float>::value ? (const T*)sig32.p : (const T*)sig.p;
+    }
+    template <class T> const T* sigoutT() const {
+        if (!finer) return sigT<T>();
+        return std::is_same<T, float>::value ? (const T*)sig_out32.p : (const T*)sig_out.p;
+    }
+    const double* sigout64() const { return finer ? sig_out.as<double>() : sig.as<double>(); }
+};
+
+// host masks of one level: every node's any / all of its adjacent cells set
+static std::vector<unsigned char> cells_of_boxes(const NodalGeom& g, const std::vector<int>& boxes) {
+    std::vector<unsigned char> fm((size_t)g.n[0] * g.n[1] * g.n[2], 0);
+    for (size_t b = 0; b + 5 < boxes.size(); b += 6)
+        for (int k = boxes[b + 2]; k <= boxes[b + 5]; ++k)
+            for (int j = boxes[b + 1]; j <= boxes[b + 4]; ++j)
+                for (int i = boxes[b + 0]; i <= boxes[b + 3]; ++i)
+                    fm[i + (size_t)g.n[0] * (j + (size_t)g.n[1] * k)] = 1;
+    return fm;
+}
+
 class AmrNodal {
    public:
-    NodalGeom gc, gf;
+    NodalGeom gc;
     MGOpts opts;
-    int nlev = 1;
+    int nlev = 1;                       // multigrid levels of level 0
     NodalHierarchy<double> H64;
     NodalHierarchy<float> H32;
     DevBuf bC;                          // level-0 rhs of the coarse correction (FP64-sized)
-    DevBuf sigc, sigc_out, sigc_out32, sigf, sigf32;
-    DevBuf ftype, cdof, ccov;
-    DevBuf cnear;                       // coarse nodes with a node of F's closure among their 27 fine nodes
-    DevBuf xc, bc, rtmp, xf, bf, yf;
-    DevBuf ef_a, ef_b, rfT, yT, rcT;    // V-cycle work (FP64-sized, reused for FP32)
+    DevBuf sigc, sigc_out, sigc_out32;
+    DevBuf cdof, ccov;
+    DevBuf cnear;                       // level-0 nodes with a node of level 1's closure among their 27 level-1 nodes
+    DevBuf xc, bc, rtmp, rcT;
     DevBuf scal, part, ibuf;
-    TileSet ftiles;                     // the fine level's nodes in or on F: every fine launch
+    std::vector<std::unique_ptr<AmrFine>> F;  // F[l - 1]: level l
+    std::vector<void*> ecur;            // per level: the buffer holding its correction after a cycle
     unsigned long long* h_pinned = nullptr;
     double ndof = 0.0;
     bool sigma_set = false;
@@ -94,101 +130,146 @@ class AmrNodal {
     ~AmrNodal() {
         if (h_pinned) (void)hipHostFree(h_pinned);
     }
+    int L() const { return (int)F.size(); }
+    AmrFine& lev(int l) { return *F[l - 1]; }
 
-    void setup(const NodalGeom& g0, const std::vector<int>& fboxes, const MGOpts& o) {
+    // lboxes[l - 1]: level l's boxes (6 ints each, level-l cell indices)
+    void setup(const NodalGeom& g0, const std::vector<std::vector<int>>& lboxes, const MGOpts& o) {
         // the boxes and the periodicity are fixed per handle, so a set-up for
         // new boundary types or options keeps the coefficient arrays
-        const bool keep = sigma_set;
+        const bool keep = sigma_set && (int)F.size() == (int)lboxes.size();
         gc = g0;
-        gf = refine_geom(g0);
         opts = o;
         nlev = nodal_nlevels(gc, o.max_coarsening_level);
-        // host masks: fine cells of F, covered coarse cells, node types
-        std::vector<unsigned char> fm((size_t)gf.n[0] * gf.n[1] * gf.n[2], 0);
-        for (size_t b = 0; b + 5 < fboxes.size(); b += 6)
-            for (int k = fboxes[b + 2]; k <= fboxes[b + 5]; ++k)
-                for (int j = fboxes[b + 1]; j <= fboxes[b + 4]; ++j)
-                    for (int i = fboxes[b + 0]; i <= fboxes[b + 3]; ++i)
-                        fm[i + (size_t)gf.n[0] * (j + (size_t)gf.n[1] * k)] = 1;
-        std::vector<unsigned char> cov((size_t)gc.n[0] * gc.n[1] * gc.n[2], 0);
-        for (int k = 0; k < gc.n[2]; ++k)
-            for (int j = 0; j < gc.n[1]; ++j)
-                for (int i = 0; i < gc.n[0]; ++i)
-                    cov[i + (size_t)gc.n[0] * (j + (size_t)gc.n[1] * k)] =
-                        fm[2 * i + (size_t)gf.n[0] * (2 * j + (size_t)gf.n[1] * 2 * k)];
-        std::vector<unsigned char> anyf, allf, anyc, allc;
-        node_any_all(gf, fm, anyf, allf);
-        node_any_all(gc, cov, anyc, allc);
-        std::vector<unsigned char> ft(gf.nnode()), cd(gc.nnode());
-        long nf = 0, nc = 0;
-        for (long q = 0; q < gf.nnode(); ++q) {
-            ft[q] = anyf[q] ? (allf[q] ? AMR_DOF : AMR_BDRY) : AMR_OUT;
-            nf += ft[q] == AMR_DOF;
+        const int L = (int)lboxes.size();
+        if ((int)F.size() != L) {
+            F.clear();
+            for (int l = 1; l <= L; ++l) F.emplace_back(new AmrFine());
         }
-        for (int k = 0; k < gc.nn[2]; ++k)
-            for (int j = 0; j < gc.nn[1]; ++j)
-                for (int i = 0; i < gc.nn[0]; ++i) {
-                    const long id = i + (long)gc.nn[0] * (j + (long)gc.nn[1] * k);
-                    cd[id] = allc[id] ? 0 : 1;
-                    nc += cd[id] && !is_dir_host(gc, i, j, k);
-                }
-        ndof = (double)(nf + nc);
-        {
-            std::vector<unsigned char> act(ft.size());
-            for (size_t q = 0; q < ft.size(); ++q) act[q] = ft[q] != AMR_OUT;
-            ftiles.build(gf, act);
+        ecur.assign(L + 1, nullptr);
+        std::vector<NodalGeom> G(L + 1);
+        G[0] = gc;
+        for (int l = 1; l <= L; ++l) G[l] = refine_geom(G[l - 1]);
+        std::vector<std::vector<unsigned char>> fm(L + 1), cov(L + 1), ft(L + 1);
+        fm[0].assign((size_t)gc.n[0] * gc.n[1] * gc.n[2], 1);
+        for (int l = 1; l <= L; ++l) fm[l] = cells_of_boxes(G[l], lboxes[l - 1]);
+        for (int l = 0; l <= L; ++l) {
+            const NodalGeom& g = G[l];
+            cov[l].assign((size_t)g.n[0] * g.n[1] * g.n[2], 0);
+            if (l == L) continue;
+            const NodalGeom& gf = G[l + 1];
+            for (int k = 0; k < g.n[2]; ++k)
+                for (int j = 0; j < g.n[1]; ++j)
+                    for (int i = 0; i < g.n[0]; ++i) {
+                        const size_t c = i + (size_t)g.n[0] * (j + (size_t)g.n[1] * k);
+                        cov[l][c] = fm[l + 1][2 * i + (size_t)gf.n[0] * (2 * j + (size_t)gf.n[1] * 2 * k)];
+                        MGX_REQUIRE(!cov[l][c] || fm[l][c], MGX_ERR_ARG,
+                                    "AMR nodal projector: a level reaches outside the level below it");
+                    }
         }
-        {
-            // the coarse nodes whose full-weighting stencil reaches a fine node
-            // of F's closure: the only ones the restrictions of fine data touch
-            std::vector<unsigned char> nr(gc.nnode(), 0);
-            for (int K = 0; K < gc.nn[2]; ++K)
-                for (int J = 0; J < gc.nn[1]; ++J)
-                    for (int I = 0; I < gc.nn[0]; ++I) {
+        for (int l = 1; l <= L; ++l) {
+            std::vector<unsigned char> anyf, allf;
+            node_any_all(G[l], fm[l], anyf, allf);
+            ft[l].resize(G[l].nnode());
+            for (long q = 0; q < G[l].nnode(); ++q)
+                ft[l][q] = anyf[q] ? (allf[q] ? AMR_DOF : AMR_BDRY) : AMR_OUT;
+        }
+        long count = 0;
+        for (int l = 0; l <= L; ++l) {
+            const NodalGeom& g = G[l];
+            std::vector<unsigned char> anyc, allc;
+            node_any_all(g, cov[l], anyc, allc);
+            std::vector<unsigned char> dof(g.nnode());
+            for (int k = 0; k < g.nn[2]; ++k)
+                for (int j = 0; j < g.nn[1]; ++j)
+                    for (int i = 0; i < g.nn[0]; ++i) {
+                        const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
+                        const bool in = l == 0 || ft[l][id] == AMR_DOF;
+                        // proper nesting: a node touching cells under the
+                        // finer level lies inside this level's region
+                        MGX_REQUIRE(!(anyc[id] && !in), MGX_ERR_ARG,
+                                    "AMR nodal projector: a level is not properly nested in the level below it");
+                        dof[id] = in && !allc[id];
+                        count += dof[id] && !is_dir_host(g, i, j, k);
+                    }
+            DevBuf& d = l == 0 ? cdof : lev(l).dof;
+            d.alloc(dof.size());
+            MGX_HIP(hipMemcpy(d.p, dof.data(), dof.size(), hipMemcpyHostToDevice));
+            DevBuf& cv = l == 0 ? ccov : lev(l).ccov;
+            if (l < L || l == 0) {
+                cv.alloc(cov[l].size());
+                MGX_HIP(hipMemcpy(cv.p, cov[l].data(), cov[l].size(), hipMemcpyHostToDevice));
+            }
+            if (l == L) continue;
+            // the nodes whose full-weighting stencil reaches a node of the
+            // finer level's closure: the only ones the restrictions of its
+            // data touch
+            const NodalGeom& gf = G[l + 1];
+            std::vector<unsigned char> nr(g.nnode(), 0);
+            for (int K = 0; K < g.nn[2]; ++K)
+                for (int J = 0; J < g.nn[1]; ++J)
+                    for (int I = 0; I < g.nn[0]; ++I) {
                         bool any = false;
                         for (int oz = -1; oz <= 1 && !any; ++oz)
                             for (int oy = -1; oy <= 1 && !any; ++oy)
                                 for (int ox = -1; ox <= 1 && !any; ++ox) {
                                     int f[3] = {2 * I + ox, 2 * J + oy, 2 * K + oz};
-                                    bool in = true;
-                                    for (int d = 0; d < 3; ++d) {
-                                        if (gf.per[d]) f[d] = (f[d] + gf.nn[d]) % gf.nn[d];
-                                        else if (f[d] < 0 || f[d] >= gf.nn[d]) in = false;
+                                    bool inb = true;
+                                    for (int d2 = 0; d2 < 3; ++d2) {
+                                        if (gf.per[d2]) f[d2] = (f[d2] + gf.nn[d2]) % gf.nn[d2];
+                                        else if (f[d2] < 0 || f[d2] >= gf.nn[d2]) inb = false;
                                     }
-                                    if (in)
-                                        any = ft[f[0] + (size_t)gf.nn[0] * (f[1] + (size_t)gf.nn[1] * f[2])] !=
+                                    if (inb)
+                                        any = ft[l + 1][f[0] + (size_t)gf.nn[0] * (f[1] + (size_t)gf.nn[1] * f[2])] !=
                                               AMR_OUT;
                                 }
-                        nr[I + (size_t)gc.nn[0] * (J + (size_t)gc.nn[1] * K)] = any;
+                        nr[I + (size_t)g.nn[0] * (J + (size_t)g.nn[1] * K)] = any;
                     }
-            cnear.alloc(nr.size());
-            MGX_HIP(hipMemcpy(cnear.p, nr.data(), nr.size(), hipMemcpyHostToDevice));
+            DevBuf& nb = l == 0 ? cnear : lev(l).cnear;
+            nb.alloc(nr.size());
+            MGX_HIP(hipMemcpy(nb.p, nr.data(), nr.size(), hipMemcpyHostToDevice));
         }
-        ftype.alloc(ft.size());
-        cdof.alloc(cd.size());
-        ccov.alloc(cov.size());
-        MGX_HIP(hipMemcpy(ftype.p, ft.data(), ft.size(), hipMemcpyHostToDevice));
-        MGX_HIP(hipMemcpy(cdof.p, cd.data(), cd.size(), hipMemcpyHostToDevice));
-        MGX_HIP(hipMemcpy(ccov.p, cov.data(), cov.size(), hipMemcpyHostToDevice));
-        const long nnc = gc.nnode(), nnf = gf.nnode();
+        ndof = (double)count;
+        const long nnc = gc.nnode();
         sigc.alloc(gc.nsig() * sizeof(double));
         sigc_out.alloc(gc.nsig() * sizeof(double));
         sigc_out32.alloc(gc.nsig() * sizeof(float));
-        sigf.alloc(gf.nsig() * sizeof(double));
-        sigf32.alloc(gf.nsig() * sizeof(float));
         // ghost cells outside non-periodic faces stay 0 (no element there)
         if (!keep) {
             MGX_HIP(hipMemset(sigc.p, 0, gc.nsig() * sizeof(double)));
             MGX_HIP(hipMemset(sigc_out.p, 0, gc.nsig() * sizeof(double)));
-            MGX_HIP(hipMemset(sigf.p, 0, gf.nsig() * sizeof(double)));
         }
         for (DevBuf* b : {&xc, &bc, &rtmp, &rcT}) b->alloc(nnc * sizeof(double));
-        // the fine launches visit only the tiles that hold nodes of F's
-        // closure; what lies elsewhere in these arrays is never read but is
-        // multiplied by the zero sigma outside F, so it starts finite (0)
-        for (DevBuf* b : {&xf, &bf, &yf, &ef_a, &ef_b, &rfT, &yT}) {
-            b->alloc(nnf * sizeof(double));
-            MGX_HIP(hipMemset(b->p, 0, nnf * sizeof(double)));
+        for (int l = 1; l <= L; ++l) {
+            AmrFine& f = lev(l);
+            f.g = G[l];
+            f.finer = l < L;
+            f.ftype.alloc(ft[l].size());
+            MGX_HIP(hipMemcpy(f.ftype.p, ft[l].data(), ft[l].size(), hipMemcpyHostToDevice));
+            {
+                std::vector<unsigned char> act(ft[l].size());
+                for (size_t q = 0; q < act.size(); ++q) act[q] = ft[l][q] != AMR_OUT;
+                f.tiles = TileSet();
+                f.tiles.build(f.g, act);
+            }
+            const long nn = f.g.nnode(), ns = f.g.nsig();
+            f.sig.alloc(ns * sizeof(double));
+            f.sig32.alloc(ns * sizeof(float));
+            if (!keep) MGX_HIP(hipMemset(f.sig.p, 0, ns * sizeof(double)));
+            if (f.finer) {
+                f.sig_out.alloc(ns * sizeof(double));
+                f.sig_out32.alloc(ns * sizeof(float));
+                if (!keep) MGX_HIP(hipMemset(f.sig_out.p, 0, ns * sizeof(double)));
+            }
+            // the launches visit only the tiles that hold nodes of the level's
+            // closure; what lies elsewhere in these arrays is never read but is
+            // multiplied by the zero sigma outside, so it starts finite (0)
+            std::vector<DevBuf*> bufs = {&f.x, &f.b, &f.y, &f.rT, &f.ea, &f.eb, &f.yT};
+            if (f.finer) { bufs.push_back(&f.bC); bufs.push_back(&f.rtmp); }
+            for (DevBuf* b : bufs) {
+                b->alloc(nn * sizeof(double));
+                MGX_HIP(hipMemset(b->p, 0, nn * sizeof(double)));
+            }
         }
         bC.alloc(nnc * sizeof(double));
         scal.alloc(64 * sizeof(double));
@@ -201,15 +282,23 @@ class AmrNodal {
         h64_sigma = h32_sigma = false;
     }
 
-    // sigc / sigf (ghosted FP64) are filled by the caller
+    // sigc / level sig (ghosted FP64) are filled by the caller
     void sigma_changed(hipStream_t s) {
         const KGeom kc = kgeom(gc);
         amr_sigma_out_kernel<<<grid3(gc.n[0], gc.n[1], gc.n[2]), block3(), 0, s>>>(
             kc, ccov.as<unsigned char>(), sigc.as<double>(), sigc_out.as<double>());
-        cast_kernel<float, double><<<grid1d(gf.nsig()), 256, 0, s>>>(sigf.as<double>(),
-                                                                    sigf32.as<float>(), gf.nsig());
         cast_kernel<float, double><<<grid1d(gc.nsig()), 256, 0, s>>>(sigc_out.as<double>(),
                                                                     sigc_out32.as<float>(), gc.nsig());
+        for (int l = 1; l <= L(); ++l) {
+            AmrFine& f = lev(l);
+            cast_kernel<float, double><<<grid1d(f.g.nsig()), 256, 0, s>>>(f.sig.as<double>(),
+                                                                         f.sig32.as<float>(), f.g.nsig());
+            if (!f.finer) continue;
+            amr_sigma_out_kernel<<<grid3(f.g.n[0], f.g.n[1], f.g.n[2]), block3(), 0, s>>>(
+                kgeom(f.g), f.ccov.as<unsigned char>(), f.sig.as<double>(), f.sig_out.as<double>());
+            cast_kernel<float, double><<<grid1d(f.g.nsig()), 256, 0, s>>>(
+                f.sig_out.as<double>(), f.sig_out32.as<float>(), f.g.nsig());
+        }
         sigma_set = true;
         h64_sigma = h32_sigma = false;
     }
@@ -226,45 +315,101 @@ class AmrNodal {
         return d;
     }
 
-    // v -= f * (weighted mean over the unknowns of both levels) for singular problems
+    // the unknown mask of level l >= 1 (the finest level: its AMR_DOF nodes)
+    const unsigned char* umask(int l, unsigned char& want) {
+        AmrFine& f = lev(l);
+        want = f.finer ? 1 : AMR_DOF;
+        return f.finer ? f.dof.as<unsigned char>() : f.ftype.as<unsigned char>();
+    }
+
+    // b_l -= mu / 8^(L-l) on the unknowns, mu the mean of the right-hand side
+    // over all unknowns in the symmetric scaling (singular problems)
     void remove_rhs_mean(hipStream_t s) {
         const int np = 512;
         amr_sum_kernel<<<np, 256, 0, s>>>(bc.as<double>(), cdof.as<unsigned char>(), 1, gc.nnode(),
                                           part.as<double>());
-        amr_sum_kernel<<<np, 256, 0, s>>>(bf.as<double>(), ftype.as<unsigned char>(), AMR_DOF,
-                                          gf.nnode(), part.as<double>() + np);
-        amr_mean_kernel<<<1, 64, 0, s>>>(part.as<double>(), np, 8.0, part.as<double>() + np, np, 1.0,
-                                         ndof, scal.as<double>() + 40);
+        for (int l = 1; l <= L(); ++l) {
+            unsigned char want;
+            const unsigned char* m = umask(l, want);
+            amr_sum_kernel<<<np, 256, 0, s>>>(lev(l).b.as<double>(), m, want, lev(l).g.nnode(),
+                                              part.as<double>() + (long)l * np);
+        }
+        amr_mean_levels_kernel<<<1, 64, 0, s>>>(part.as<double>(), np, L() + 1, ndof,
+                                                scal.as<double>() + 40);
         amr_shift_kernel<<<grid1d(gc.nnode()), 256, 0, s>>>(bc.as<double>(), cdof.as<unsigned char>(),
                                                             1, 0, gc.nnode(), scal.as<double>() + 40,
-                                                            0.125);
-        amr_shift_kernel<<<grid1d(gf.nnode()), 256, 0, s>>>(bf.as<double>(), ftype.as<unsigned char>(),
-                                                            AMR_DOF, 0, gf.nnode(),
-                                                            scal.as<double>() + 40, 1.0);
+                                                            std::pow(0.125, L()));
+        for (int l = 1; l <= L(); ++l) {
+            unsigned char want;
+            const unsigned char* m = umask(l, want);
+            amr_shift_kernel<<<grid1d(lev(l).g.nnode()), 256, 0, s>>>(
+                lev(l).b.as<double>(), m, want, 0, lev(l).g.nnode(), scal.as<double>() + 40,
+                std::pow(0.125, L() - l));
+        }
     }
 
-    // composite residual of (xc, xf) into rfT / rcT (type TO); returns ||r||_inf
+    // level l >= 1's closure: dF_l from level l - 1 (the levels filled upward)
+    void fill_levels(hipStream_t s) {
+        for (int l = 1; l <= L(); ++l) {
+            AmrFine& f = lev(l);
+            int nbk = 0;
+            const int4* tl = f.blocks(nbk);
+            const KGeom kcl = kgeom(l == 1 ? gc : lev(l - 1).g);
+            const double* xcl = l == 1 ? xc.as<double>() : lev(l - 1).x.as<double>();
+            if (nbk)
+                amr_fill_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                    kgeom(f.g), kcl, f.ftype.as<unsigned char>(), xcl, f.x.as<double>(), tl);
+        }
+    }
+
+    // composite residual of the levels' x into rT (levels >= 1) / rcT (level
+    // 0), type TO; returns ||r||_inf
     template <class TO>
     double residual(hipStream_t s) {
-        const KGeom kc = kgeom(gc), kf = kgeom(gf);
-        const unsigned char* ft = ftype.as<unsigned char>();
-        int nbk = 0;
-        const int4* tl = ftiles.blocks(nbk);
-        if (nbk)
-            amr_fill_kernel<double><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, xc.as<double>(),
-                                                             xf.as<double>(), tl);
-        SArgs<double, double> a{};
-        a.g = kf;
-        a.c = make_coef<double>(gf.dx);
-        a.x = xf.as<double>();
-        a.sig = sigf.as<double>();
-        a.rout = yf.as<double>();
-        launch_stencil<double, double, KS_APPLY>(gf, a, s, false, &ftiles);
+        fill_levels(s);
         zero_norm(1, s);
-        if (nbk)
-            amr_fine_resid_kernel<TO><<<nbk, tile_block(), 0, s>>>(kf, ft, bf.as<double>(),
-                                                               yf.as<double>(), rfT.as<TO>(),
-                                                               scal.as<unsigned long long>() + 1, tl);
+        const double* yB = nullptr;  // level l+1's contribution to restrict, with sign sB
+        double sB = 0.0;
+        for (int l = L(); l >= 1; --l) {
+            AmrFine& f = lev(l);
+            const KGeom kf = kgeom(f.g);
+            int nbk = 0;
+            const int4* tl = f.blocks(nbk);
+            if (!f.finer) {
+                SArgs<double, double> a{};
+                a.g = kf;
+                a.c = make_coef<double>(f.g.dx);
+                a.x = f.x.as<double>();
+                a.sig = f.sig.as<double>();
+                a.rout = f.y.as<double>();
+                launch_stencil<double, double, KS_APPLY>(f.g, a, s, false, &f.tiles);
+                if (nbk)
+                    amr_fine_resid_kernel<TO><<<nbk, tile_block(), 0, s>>>(
+                        kf, f.ftype.as<unsigned char>(), f.b.as<double>(), f.y.as<double>(), f.rT.as<TO>(),
+                        scal.as<unsigned long long>() + 1, tl);
+                yB = f.y.as<double>();
+                sB = -1.0;
+            } else {
+                AmrFine& u = lev(l + 1);
+                SArgs<double, double> a{};
+                a.g = kf;
+                a.c = make_coef<double>(f.g.dx);
+                a.x = f.x.as<double>();
+                a.b = f.b.as<double>();
+                a.sig = f.sig_out.as<double>();
+                a.rout = f.rtmp.as<double>();
+                launch_stencil<double, double, KS_RESID>(f.g, a, s, false, &f.tiles);
+                if (nbk)
+                    amr_coarse_combine_kernel<TO><<<nbk, tile_block(), 0, s>>>(
+                        kgeom(u.g), kf, u.ftype.as<unsigned char>(), f.dof.as<unsigned char>(),
+                        f.rtmp.as<double>(), yB, sB, f.rT.as<TO>(), scal.as<unsigned long long>() + 1,
+                        f.cnear.as<unsigned char>(), tl);
+                // level l's y on dF_l is -(b - L x) there (b is 0 off the unknowns)
+                yB = f.rtmp.as<double>();
+                sB = 1.0;
+            }
+        }
+        const KGeom kc = kgeom(gc);
         SArgs<double, double> c{};
         c.g = kc;
         c.c = make_coef<double>(gc.dx);
@@ -273,84 +418,147 @@ class AmrNodal {
         c.sig = sigc_out.as<double>();
         c.rout = rtmp.as<double>();
         launch_stencil<double, double, KS_RESID>(gc, c, s, false);
-        amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, cdof.as<unsigned char>(), rtmp.as<double>(), yf.as<double>(), -1.0,
-            rcT.as<TO>(), scal.as<unsigned long long>() + 1, cnear.as<unsigned char>());
+        if (L() > 0)
+            amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+                kgeom(lev(1).g), kc, lev(1).ftype.as<unsigned char>(), cdof.as<unsigned char>(),
+                rtmp.as<double>(), yB, sB, rcT.as<TO>(), scal.as<unsigned long long>() + 1,
+                cnear.as<unsigned char>());
+        else
+            amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+                kc, kc, nullptr, cdof.as<unsigned char>(), rtmp.as<double>(), nullptr, 0.0, rcT.as<TO>(),
+                scal.as<unsigned long long>() + 1, nullptr);
         return read_norm(1, s);
     }
 
-    // one composite V-cycle on the residual in rfT / rcT; the correction is
-    // added to (xc, xf)
+    // the correction of level l >= 1 from the right-hand side rhs: pre-smooth,
+    // the level below's correction (recursively; level 0: one V-cycle of its
+    // multigrid hierarchy), prolongation, post-smooth, and one composite
+    // Jacobi sweep of the level below's unknowns next to this level
     template <class T>
-    void vcycle(NodalHierarchy<T>& H, const T* sfT, const T* scoT, hipStream_t s) {
-        const KGeom kc = kgeom(gc), kf = kgeom(gf);
-        const unsigned char* ft = ftype.as<unsigned char>();
-        T* e = ef_a.as<T>();
-        T* e2 = ef_b.as<T>();
+    void cycle_level(NodalHierarchy<T>& H, int l, const T* rhs, hipStream_t s) {
+        AmrFine& f = lev(l);
+        const KGeom kf = kgeom(f.g);
+        const unsigned char* ft = f.ftype.as<unsigned char>();
+        T* e = f.ea.as<T>();
+        T* e2 = f.eb.as<T>();
         SArgs<T, T> a{};
         a.g = kf;
-        a.c = make_coef<T>(gf.dx);
-        a.b = rfT.as<T>();
-        a.sig = sfT;
-        a.omega = (T)nodal_omega(gf, opts.omega);
+        a.c = make_coef<T>(f.g.dx);
+        a.b = rhs;
+        a.sig = f.sigT<T>();
+        a.omega = (T)nodal_omega(f.g, opts.omega);
         a.nmask = ft;
         int nbk = 0;
-        const int4* tl = ftiles.blocks(nbk);
+        const int4* tl = f.blocks(nbk);
         if (opts.nu1 > 0) {
             a.xout = e;
-            launch_stencil<T, T, KS_JFIRST>(gf, a, s, false, &ftiles);
+            launch_stencil<T, T, KS_JFIRST>(f.g, a, s, false, &f.tiles);
             for (int q = 1; q < opts.nu1; ++q) {
                 a.x = e;
                 a.xout = e2;
-                launch_stencil<T, T, KS_JACOBI>(gf, a, s, false, &ftiles);
+                launch_stencil<T, T, KS_JACOBI>(f.g, a, s, false, &f.tiles);
                 std::swap(e, e2);
             }
         } else {
-            zero_kernel<T><<<grid1d(gf.nnode()), 256, 0, s>>>(e, gf.nnode());
+            zero_kernel<T><<<grid1d(f.g.nnode()), 256, 0, s>>>(e, f.g.nnode());
         }
         SArgs<T, T> ap{};
         ap.g = kf;
         ap.c = a.c;
         ap.x = e;
-        ap.sig = sfT;
-        ap.rout = yT.as<T>();
-        launch_stencil<T, T, KS_APPLY>(gf, ap, s, false, &ftiles);
-        amr_coarse_rhs_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, cdof.as<unsigned char>(), rfT.as<T>(), yT.as<T>(), rcT.as<T>(), H.L[0].b,
-            cnear.as<unsigned char>());
-        H.vcycle(opts, s, ibuf.as<int>());
-        if (nbk) amr_prolong_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, H.L[0].x, e, tl);
+        ap.sig = f.sigT<T>();
+        ap.rout = f.yT.as<T>();
+        launch_stencil<T, T, KS_APPLY>(f.g, ap, s, false, &f.tiles);
+        T* ec;
+        KGeom kcl;
+        if (l == 1) {
+            kcl = kgeom(gc);
+            amr_coarse_rhs_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+                kf, kcl, ft, cdof.as<unsigned char>(), rhs, f.yT.as<T>(), rcT.as<T>(), H.L[0].b,
+                cnear.as<unsigned char>());
+            H.vcycle(opts, s, ibuf.as<int>());
+            ec = H.L[0].x;
+        } else {
+            AmrFine& c = lev(l - 1);
+            kcl = kgeom(c.g);
+            int nbc = 0;
+            const int4* tlc = c.blocks(nbc);
+            if (nbc)
+                amr_coarse_rhs_kernel<T><<<nbc, tile_block(), 0, s>>>(
+                    kf, kcl, ft, c.dof.as<unsigned char>(), rhs, f.yT.as<T>(), c.rT.as<T>(), c.bC.as<T>(),
+                    c.cnear.as<unsigned char>(), tlc);
+            cycle_level<T>(H, l - 1, c.bC.as<T>(), s);
+            ec = (T*)ecur[l - 1];
+        }
+        if (nbk) amr_prolong_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, kcl, ft, ec, e, tl);
         for (int q = 0; q < opts.nu2; ++q) {
             a.x = e;
             a.xout = e2;
-            launch_stencil<T, T, KS_JACOBI>(gf, a, s, false, &ftiles);
+            launch_stencil<T, T, KS_JACOBI>(f.g, a, s, false, &f.tiles);
             std::swap(e, e2);
         }
         {
-            // the coarse unknowns next to F: one Jacobi sweep with the
-            // composite operator (e on dF follows the coarse correction)
-            SArgs<T, T> af{};
-            af.g = kf;
-            af.c = a.c;
-            af.x = e;
-            af.sig = sfT;
-            af.rout = yT.as<T>();
-            launch_stencil<T, T, KS_APPLY>(gf, af, s, false, &ftiles);
-            SArgs<T, T> ac{};
-            ac.g = kc;
-            ac.c = make_coef<T>(gc.dx);
-            ac.x = H.L[0].x;
-            ac.sig = scoT;
-            ac.rout = rtmp.as<T>();
-            launch_stencil<T, T, KS_APPLY>(gc, ac, s, false);
-            amr_coarse_jacobi_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-                kf, kc, ft, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), yT.as<T>(), H.L[0].sig,
-                make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), H.L[0].x,
-                cnear.as<unsigned char>());
+            // the unknowns of the level below next to this level: one Jacobi
+            // sweep with the composite operator (e on dF follows the
+            // correction below)
+            ap.x = e;
+            launch_stencil<T, T, KS_APPLY>(f.g, ap, s, false, &f.tiles);
+            if (l == 1) {
+                SArgs<T, T> ac{};
+                ac.g = kcl;
+                ac.c = make_coef<T>(gc.dx);
+                ac.x = ec;
+                ac.sig = std::is_same<T, float>::value ? (const T*)sigc_out32.p : (const T*)sigc_out.p;
+                ac.rout = rtmp.as<T>();
+                launch_stencil<T, T, KS_APPLY>(gc, ac, s, false);
+                amr_coarse_jacobi_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+                    kf, kcl, ft, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), f.yT.as<T>(), H.L[0].sig,
+                    make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), ec,
+                    cnear.as<unsigned char>());
+            } else {
+                AmrFine& c = lev(l - 1);
+                int nbc = 0;
+                const int4* tlc = c.blocks(nbc);
+                SArgs<T, T> ac{};
+                ac.g = kcl;
+                ac.c = make_coef<T>(c.g.dx);
+                ac.x = ec;
+                ac.sig = c.sigoutT<T>();
+                ac.rout = c.yT.as<T>();
+                launch_stencil<T, T, KS_APPLY>(c.g, ac, s, false, &c.tiles);
+                if (nbc)
+                    amr_coarse_jacobi_kernel<T><<<nbc, tile_block(), 0, s>>>(
+                        kf, kcl, ft, c.dof.as<unsigned char>(), c.rT.as<T>(), c.yT.as<T>(), f.yT.as<T>(),
+                        c.sigT<T>(), make_coef<double>(c.g.dx).diag4, nodal_omega(c.g, opts.omega), ec,
+                        c.cnear.as<unsigned char>(), tlc);
+            }
         }
-        if (nbk) amr_add_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, ft, AMR_DOF, e, xf.as<double>(), tl);
+        ecur[l] = e;
+    }
+
+    // one composite V-cycle on the residual in rT / rcT; the corrections are
+    // added to the levels' x
+    template <class T>
+    void vcycle(NodalHierarchy<T>& H, hipStream_t s) {
+        if (L() == 0) {
+            H.L[0].b = rcT.as<T>();
+            H.vcycle(opts, s, ibuf.as<int>());
+            H.L[0].b = bC.as<T>();
+        } else {
+            cycle_level<T>(H, L(), lev(L()).rT.as<T>(), s);
+        }
+        for (int l = 1; l <= L(); ++l) {
+            AmrFine& f = lev(l);
+            int nbk = 0;
+            const int4* tl = f.blocks(nbk);
+            unsigned char want;
+            const unsigned char* m = umask(l, want);
+            if (nbk)
+                amr_add_kernel<T><<<nbk, tile_block(), 0, s>>>(kgeom(f.g), m, want, (const T*)ecur[l],
+                                                               f.x.as<double>(), tl);
+        }
         amr_add_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kc, cdof.as<unsigned char>(), 1, H.L[0].x, xc.as<double>());
+            kgeom(gc), cdof.as<unsigned char>(), 1, H.L[0].x, xc.as<double>());
     }
 
     template <class T>
@@ -366,34 +574,44 @@ class AmrNodal {
         H.L[0].b = bC.as<T>();
     }
 
-    // composite solve of the right-hand side in (bc, bf); result in (xc, xf)
-    // with dF filled, coarse nodes under F injected and (singular) the mean
-    // of the coarse phi removed
+    // composite solve of the right-hand side in bc / the levels' b; result
+    // in xc / the levels' x with every closure filled, nodes under a finer
+    // level injected from it and (singular) the mean of the level-0 phi
+    // removed
     void solve(double rtol, double atol, int mode, mgx_solve_info* info, hipStream_t s) {
         MGX_REQUIRE(sigma_set, MGX_ERR_STATE, "AMR nodal projector: sigma not set");
         MGX_REQUIRE(mode == MGX_MODE_FP64 || mode == MGX_MODE_MIXED, MGX_ERR_UNSUPPORTED,
                     "AMR nodal projector: FP64 and mixed modes only");
         const bool mixed = mode == MGX_MODE_MIXED;
-        const KGeom kc = kgeom(gc), kf = kgeom(gf);
-        const unsigned char* ft = ftype.as<unsigned char>();
-        ftiles.wide = amr_tile_width() == 64;
+        const KGeom kc = kgeom(gc);
+        const bool wide = amr_tile_width() == 64;
+        for (auto& f : F) f->tiles.wide = wide;
         if (mixed) prepare(H32, h32_sigma, s);
         else prepare(H64, h64_sigma, s);
         MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
         if (gc.singular()) remove_rhs_mean(s);
-        // ||b||_inf over the unknowns of both levels
+        // ||b||_inf over the unknowns of every level
         zero_norm(0, s);
-        int nbk = 0;
-        const int4* tl = ftiles.blocks(nbk);
-        if (nbk)
-            amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
-                kf, ft, bf.as<double>(), nullptr, yf.as<double>(), scal.as<unsigned long long>(), tl);
+        for (int l = L(); l >= 1; --l) {
+            AmrFine& f = lev(l);
+            int nbk = 0;
+            const int4* tl = f.blocks(nbk);
+            if (!nbk) continue;
+            if (!f.finer)
+                amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                    kgeom(f.g), f.ftype.as<unsigned char>(), f.b.as<double>(), nullptr, f.y.as<double>(),
+                    scal.as<unsigned long long>(), tl);
+            else
+                amr_coarse_combine_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                    kgeom(f.g), kgeom(f.g), nullptr, f.dof.as<unsigned char>(), f.b.as<double>(), nullptr, 0.0,
+                    f.rtmp.as<double>(), scal.as<unsigned long long>(), nullptr, tl);
+        }
         amr_coarse_combine_kernel<double><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, cdof.as<unsigned char>(), bc.as<double>(), yf.as<double>(), 0.0,
-            rtmp.as<double>(), scal.as<unsigned long long>(), cnear.as<unsigned char>());
+            kc, kc, nullptr, cdof.as<unsigned char>(), bc.as<double>(), nullptr, 0.0, rtmp.as<double>(),
+            scal.as<unsigned long long>(), nullptr);
         const double bnorm = read_norm(0, s);
         MGX_HIP(hipMemsetAsync(xc.p, 0, gc.nnode() * sizeof(double), s));
-        MGX_HIP(hipMemsetAsync(xf.p, 0, gf.nnode() * sizeof(double), s));
+        for (auto& f : F) MGX_HIP(hipMemsetAsync(f->x.p, 0, f->g.nnode() * sizeof(double), s));
         const double tol = std::max(atol, rtol * bnorm);
         const double r0 = mixed ? residual<float>(s) : residual<double>(s);
         double rn = r0;
@@ -402,10 +620,10 @@ class AmrNodal {
         while (!conv && it < opts.maxiter && std::isfinite(rn)) {
             ++it;
             if (mixed) {
-                vcycle<float>(H32, sigf32.as<float>(), sigc_out32.as<float>(), s);
+                vcycle<float>(H32, s);
                 rn = residual<float>(s);
             } else {
-                vcycle<double>(H64, sigf.as<double>(), sigc_out.as<double>(), s);
+                vcycle<double>(H64, s);
                 rn = residual<double>(s);
             }
             if (opts.verbose)
@@ -413,12 +631,23 @@ class AmrNodal {
                         bnorm > 0 ? rn / bnorm : rn);
             conv = rn <= tol;
         }
-        // dF from the coarse level, coarse nodes under F from the fine level
-        if (nbk)
-            amr_fill_kernel<double><<<nbk, tile_block(), 0, s>>>(kf, kc, ft, xc.as<double>(),
-                                                             xf.as<double>(), tl);
-        amr_inject_kernel<<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kf, kc, cdof.as<unsigned char>(), xf.as<double>(), xc.as<double>());
+        // closures from the level below, nodes under a finer level from it
+        fill_levels(s);
+        for (int l = L() - 1; l >= 0; --l) {
+            AmrFine& u = lev(l + 1);
+            if (l == 0) {
+                amr_inject_kernel<<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+                    kgeom(u.g), kc, cdof.as<unsigned char>(), u.x.as<double>(), xc.as<double>());
+            } else {
+                AmrFine& f = lev(l);
+                int nbk = 0;
+                const int4* tl = f.blocks(nbk);
+                if (nbk)
+                    amr_inject_kernel<<<nbk, tile_block(), 0, s>>>(
+                        kgeom(u.g), kgeom(f.g), f.dof.as<unsigned char>(), u.x.as<double>(), f.x.as<double>(),
+                        f.ftype.as<unsigned char>(), tl);
+            }
+        }
         if (gc.singular()) {
             const int np = 512;
             amr_sum_kernel<<<np, 256, 0, s>>>(xc.as<double>(), nullptr, 0, gc.nnode(), part.as<double>());
@@ -426,15 +655,17 @@ class AmrNodal {
                                              (double)gc.nnode(), scal.as<double>() + 41);
             amr_shift_kernel<<<grid1d(gc.nnode()), 256, 0, s>>>(xc.as<double>(), nullptr, 0, 0,
                                                                 gc.nnode(), scal.as<double>() + 41, 1.0);
-            amr_shift_kernel<<<grid1d(gf.nnode()), 256, 0, s>>>(xf.as<double>(), ft, AMR_OUT, 1,
-                                                                gf.nnode(), scal.as<double>() + 41, 1.0);
+            for (auto& f : F)
+                amr_shift_kernel<<<grid1d(f->g.nnode()), 256, 0, s>>>(
+                    f->x.as<double>(), f->ftype.as<unsigned char>(), AMR_OUT, 1, f->g.nnode(),
+                    scal.as<double>() + 41, 1.0);
         }
         MGX_HIP(hipStreamSynchronize(s));
         if (info) {
             info->iters = it;
             info->vcycles = it;
             info->converged = conv ? 1 : 0;
-            info->nlevels = nlev + 1;
+            info->nlevels = nlev + L();
             info->fallback_used = 0;
             info->bnorm = bnorm;
             info->resnorm0 = r0;
@@ -453,31 +684,55 @@ struct mgx_amr_s {
     int n_cell[3] = {0, 0, 0};
     double prob_lo[3] = {0, 0, 0}, prob_hi[3] = {1, 1, 1};
     int periodic[3] = {0, 0, 0};
-    std::vector<int> cboxes, fboxes;
+    std::vector<int> cboxes;
+    std::vector<std::vector<int>> lboxes;   // levels 1.. (6 ints per box)
     int bclo[3] = {0, 0, 0}, bchi[3] = {0, 0, 0};
     bool bc_set = false;
     MGOpts opts;
     bool dirty = true;
     AmrNodal solver;
-    BoxLayout lay_c, lay_f;
-    DevBuf ugc, ugf;
+    std::vector<std::unique_ptr<BoxLayout>> lay;   // per level (0: coarse)
+    std::vector<std::unique_ptr<DevBuf>> ug, gcan; // per level: canonical ghosted u* / u_new, G phi
     hipStream_t stream() const { return ctx->stream; }
+    int nfine() const { return (int)lboxes.size(); }
 
     void ensure_setup() {
         if (!dirty) return;
         for (int d = 0; d < 3; ++d)
             MGX_REQUIRE(periodic[d] || bc_set, MGX_ERR_STATE,
                         "AMR nodal projector: setDomainBC is required for non-periodic domains");
+        // fine levels stay off inflow faces (their ghost velocity there is not
+        // part of the composite divergence)
+        for (int l = 1; l <= nfine(); ++l) {
+            const std::vector<int>& bx = lboxes[l - 1];
+            const int nf[3] = {n_cell[0] << l, n_cell[1] << l, n_cell[2] << l};
+            for (size_t b = 0; b + 5 < bx.size(); b += 6)
+                for (int d = 0; d < 3; ++d) {
+                    if (periodic[d]) continue;
+                    MGX_REQUIRE(!(bx[b + d] == 0 && bclo[d] == MGX_BC_INFLOW) &&
+                                    !(bx[b + 3 + d] == nf[d] - 1 && bchi[d] == MGX_BC_INFLOW),
+                                MGX_ERR_UNSUPPORTED, "AMR nodal projector: fine boxes must stay off inflow faces");
+                }
+        }
         double dx[3];
         for (int d = 0; d < 3; ++d) dx[d] = (prob_hi[d] - prob_lo[d]) / n_cell[d];
         const NodalGeom g = make_nodal_geom(n_cell, periodic, bclo, bchi, dx);
-        solver.setup(g, fboxes, opts);
-        lay_c.build(cboxes.data(), (int)cboxes.size() / 6, g);
-        if (!fboxes.empty()) lay_f.build(fboxes.data(), (int)fboxes.size() / 6, solver.gf, true);
-        ugc.alloc(3 * g.nsig() * sizeof(double));
-        ugf.alloc(3 * solver.gf.nsig() * sizeof(double));
-        MGX_HIP(hipMemset(ugc.p, 0, 3 * g.nsig() * sizeof(double)));
-        MGX_HIP(hipMemset(ugf.p, 0, 3 * solver.gf.nsig() * sizeof(double)));
+        solver.setup(g, lboxes, opts);
+        const int L = nfine();
+        lay.clear();
+        ug.clear();
+        gcan.clear();
+        for (int l = 0; l <= L; ++l) {
+            const NodalGeom& gl = l == 0 ? g : solver.lev(l).g;
+            lay.emplace_back(new BoxLayout());
+            if (l == 0) lay[0]->build(cboxes.data(), (int)cboxes.size() / 6, gl);
+            else lay[l]->build(lboxes[l - 1].data(), (int)lboxes[l - 1].size() / 6, gl, true);
+            ug.emplace_back(new DevBuf());
+            ug[l]->alloc(3 * gl.nsig() * sizeof(double));
+            MGX_HIP(hipMemset(ug[l]->p, 0, 3 * gl.nsig() * sizeof(double)));
+            gcan.emplace_back(new DevBuf());
+            if (l > 0) gcan[l]->alloc(3 * gl.nsig() * sizeof(double));
+        }
         dirty = false;
     }
 };
@@ -490,8 +745,8 @@ MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int 
     *out = nullptr;
     auto h = std::make_unique<mgx_amr_s>();
     int status = guarded(ctx, [&] {
-        MGX_REQUIRE(levels != nullptr && (nlevels == 1 || nlevels == 2), MGX_ERR_UNSUPPORTED,
-                    "AMR nodal projector: one coarse level and at most one fine level (ratio 2)");
+        MGX_REQUIRE(levels != nullptr && nlevels >= 1 && nlevels <= MGX_AMR_MAX_LEVELS, MGX_ERR_UNSUPPORTED,
+                    "AMR nodal projector: 1 to MGX_AMR_MAX_LEVELS levels (ratio 2)");
         const mgx_level_desc& c = levels[0];
         for (int d = 0; d < 3; ++d) {
             MGX_REQUIRE(c.n_cell[d] >= 2 && c.n_cell[d] % 2 == 0, MGX_ERR_ARG,
@@ -503,10 +758,11 @@ MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int 
         }
         MGX_REQUIRE(c.nboxes > 0 && c.boxes, MGX_ERR_ARG, "AMR nodal projector: empty coarse box list");
         h->cboxes.assign(c.boxes, c.boxes + 6 * c.nboxes);
-        if (nlevels == 2) {
-            const mgx_level_desc& f = levels[1];
+        for (int l = 1; l < nlevels; ++l) {
+            const mgx_level_desc& p = levels[l - 1];
+            const mgx_level_desc& f = levels[l];
             for (int d = 0; d < 3; ++d) {
-                MGX_REQUIRE(f.n_cell[d] == 2 * c.n_cell[d], MGX_ERR_UNSUPPORTED,
+                MGX_REQUIRE(f.n_cell[d] == 2 * p.n_cell[d], MGX_ERR_UNSUPPORTED,
                             "AMR nodal projector: refinement ratio 2 only");
                 MGX_REQUIRE(f.prob_lo[d] == c.prob_lo[d] && f.prob_hi[d] == c.prob_hi[d] &&
                                 (f.is_periodic[d] != 0) == (c.is_periodic[d] != 0),
@@ -514,6 +770,8 @@ MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int 
             }
             MGX_REQUIRE(f.nboxes >= 0 && (f.nboxes == 0 || f.boxes), MGX_ERR_ARG,
                         "AMR nodal projector: bad fine box list");
+            MGX_REQUIRE(f.nboxes > 0 || l == nlevels - 1, MGX_ERR_ARG,
+                        "AMR nodal projector: only the finest level may be empty");
             for (int b = 0; b < f.nboxes; ++b)
                 for (int d = 0; d < 3; ++d) {
                     const int lo = f.boxes[6 * b + d], hi = f.boxes[6 * b + 3 + d];
@@ -521,11 +779,8 @@ MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int 
                                 "AMR nodal projector: fine box outside the domain");
                     MGX_REQUIRE(lo % 2 == 0 && hi % 2 == 1, MGX_ERR_ARG,
                                 "AMR nodal projector: fine boxes must align with coarse cells");
-                    MGX_REQUIRE(c.is_periodic[d] || (lo > 0 && hi < f.n_cell[d] - 1),
-                                MGX_ERR_UNSUPPORTED,
-                                "AMR nodal projector: fine boxes must stay off non-periodic domain faces");
                 }
-            h->fboxes.assign(f.boxes, f.boxes + 6 * f.nboxes);
+            if (f.nboxes > 0) h->lboxes.emplace_back(f.boxes, f.boxes + 6 * f.nboxes);
         }
         h->ctx = ctx;
     });
@@ -570,19 +825,21 @@ MGX_API int mgx_amr_nodal_set_options(mgx_amr_nodal h, const mgx_mlmg_opts* o) {
 MGX_API int mgx_amr_nodal_set_sigma(mgx_amr_nodal h, const double* const* sigma, int ng) {
     if (!h || !sigma) return MGX_ERR_ARG;
     return guarded(h->ctx, [&] {
-        MGX_REQUIRE(sigma[0] != nullptr && (h->fboxes.empty() || sigma[1] != nullptr), MGX_ERR_ARG,
-                    "AMR nodal projector: sigma needed on every level");
+        for (int l = 0; l <= h->nfine(); ++l)
+            MGX_REQUIRE(sigma[l] != nullptr, MGX_ERR_ARG, "AMR nodal projector: sigma needed on every level");
         MGX_REQUIRE(ng >= 0 && ng <= 8, MGX_ERR_ARG, "sigma ghost width out of range");
         h->ensure_setup();
         AmrNodal& S = h->solver;
-        gather_sigma_kernel<<<h->lay_c.ntiles[REG_CELL], 256, 0, h->stream()>>>(
-            kgeom(S.gc), h->lay_c.dev_boxes(), h->lay_c.tiles(REG_CELL), sigma[0], ng,
+        gather_sigma_kernel<<<h->lay[0]->ntiles[REG_CELL], 256, 0, h->stream()>>>(
+            kgeom(S.gc), h->lay[0]->dev_boxes(), h->lay[0]->tiles(REG_CELL), sigma[0], ng,
             S.sigc.as<double>());
-        MGX_HIP(hipMemsetAsync(S.sigf.p, 0, S.gf.nsig() * sizeof(double), h->stream()));
-        if (!h->fboxes.empty())
-            gather_sigma_kernel<<<h->lay_f.ntiles[REG_CELL], 256, 0, h->stream()>>>(
-                kgeom(S.gf), h->lay_f.dev_boxes(), h->lay_f.tiles(REG_CELL), sigma[1], ng,
-                S.sigf.as<double>());
+        for (int l = 1; l <= h->nfine(); ++l) {
+            AmrFine& f = S.lev(l);
+            MGX_HIP(hipMemsetAsync(f.sig.p, 0, f.g.nsig() * sizeof(double), h->stream()));
+            gather_sigma_kernel<<<h->lay[l]->ntiles[REG_CELL], 256, 0, h->stream()>>>(
+                kgeom(f.g), h->lay[l]->dev_boxes(), h->lay[l]->tiles(REG_CELL), sigma[l], ng,
+                f.sig.as<double>());
+        }
         S.sigma_changed(h->stream());
     });
 }
@@ -592,9 +849,9 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
                                   double rtol, double atol, int mode, mgx_solve_info* info) {
     if (!h || !vel) return MGX_ERR_ARG;
     int status = guarded(h->ctx, [&] {
-        const bool two = !h->fboxes.empty();
-        MGX_REQUIRE(vel[0] != nullptr && (!two || vel[1] != nullptr), MGX_ERR_ARG,
-                    "AMR nodal projector: velocity needed on every level");
+        const int L = h->nfine();
+        for (int l = 0; l <= L; ++l)
+            MGX_REQUIRE(vel[l] != nullptr, MGX_ERR_ARG, "AMR nodal projector: velocity needed on every level");
         const bool any_nonper = !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
         MGX_REQUIRE(vel_ng >= (any_nonper ? 1 : 0) && vel_ng <= 8, MGX_ERR_ARG,
                     "velocity needs one ghost cell at non-periodic domain faces");
@@ -603,101 +860,114 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
         AmrNodal& S = h->solver;
         MGX_REQUIRE(S.sigma_set, MGX_ERR_STATE, "AMR nodal projector: sigma not set");
         hipStream_t s = h->stream();
-        const KGeom kc = kgeom(S.gc), kf = kgeom(S.gf);
-        // coarse divergence of the velocity outside F
-        GatherVelArgs a{};
-        a.g = kc;
-        a.boxes = h->lay_c.dev_boxes();
-        a.tiles = h->lay_c.tiles(REG_CELL_GROWN);
-        a.vel = vel[0];
-        a.vel_ng = vel_ng;
-        a.scale = 1.0;
-        a.const_rho = 1.0;
-        a.ug = h->ugc.as<double>();
-        a.ghost_rule = 0;
-        gather_vel_kernel<<<h->lay_c.ntiles[REG_CELL_GROWN], 256, 0, s>>>(a);
-        amr_zero_covered_kernel<<<grid3(S.gc.n[0], S.gc.n[1], S.gc.n[2]), block3(), 0, s>>>(
-            kc, S.ccov.as<unsigned char>(), h->ugc.as<double>());
-        nodal_rhs_kernel<<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
-            kc, h->ugc.as<double>(), S.rtmp.as<double>(), 0.25 / S.gc.dx[0], 0.25 / S.gc.dx[1],
-            0.25 / S.gc.dx[2]);
-        // fine divergence of the velocity inside F (the gather writes the
-        // same cells every call; the rest of ugf stays 0 from the set-up)
-        if (two) {
-            GatherVelArgs f = a;
-            f.g = kf;
-            f.boxes = h->lay_f.dev_boxes();
-            f.tiles = h->lay_f.tiles(REG_CELL_GROWN);
-            f.vel = vel[1];
-            f.ug = h->ugf.as<double>();
-            gather_vel_kernel<<<h->lay_f.ntiles[REG_CELL_GROWN], 256, 0, s>>>(f);
+        auto geom = [&](int l) -> const NodalGeom& { return l == 0 ? S.gc : S.lev(l).g; };
+        // u* per level (the gather writes the same cells every call; the rest
+        // of a fine level's array stays 0 from the set-up); cells under a
+        // finer level carry no velocity of their own
+        for (int l = 0; l <= L; ++l) {
+            const KGeom kl = kgeom(geom(l));
+            GatherVelArgs a{};
+            a.g = kl;
+            a.boxes = h->lay[l]->dev_boxes();
+            a.tiles = h->lay[l]->tiles(REG_CELL_GROWN);
+            a.vel = vel[l];
+            a.vel_ng = vel_ng;
+            a.scale = 1.0;
+            a.const_rho = 1.0;
+            a.ug = h->ug[l]->as<double>();
+            a.ghost_rule = 0;
+            gather_vel_kernel<<<h->lay[l]->ntiles[REG_CELL_GROWN], 256, 0, s>>>(a);
+            if (l < L) {
+                const NodalGeom& g = geom(l);
+                amr_zero_covered_kernel<<<grid3(g.n[0], g.n[1], g.n[2]), block3(), 0, s>>>(
+                    kl, l == 0 ? S.ccov.as<unsigned char>() : S.lev(l).ccov.as<unsigned char>(),
+                    h->ug[l]->as<double>());
+            }
         }
-        int nbk = 0;
-        const int4* tl = S.ftiles.blocks(nbk);
-        const unsigned char* ft = S.ftype.as<unsigned char>();
-        if (nbk) {
-            nodal_rhs_kernel<<<nbk, tile_block(), 0, s>>>(kf, h->ugf.as<double>(), S.yf.as<double>(),
-                                                      0.25 / S.gf.dx[0], 0.25 / S.gf.dx[1],
-                                                      0.25 / S.gf.dx[2], tl);
-            amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
-                kf, ft, S.yf.as<double>(), nullptr, S.bf.as<double>(), nullptr, tl);
+        // composite divergence: each level's nodal divergence, dF_{l+1}'s
+        // restricted onto level l's unknowns
+        for (int l = L; l >= 1; --l) {
+            AmrFine& f = S.lev(l);
+            const KGeom kf = kgeom(f.g);
+            int nbk = 0;
+            const int4* tl = f.blocks(nbk);
+            if (!nbk) continue;
+            nodal_rhs_kernel<<<nbk, tile_block(), 0, s>>>(kf, h->ug[l]->as<double>(), f.y.as<double>(),
+                                                          0.25 / f.g.dx[0], 0.25 / f.g.dx[1],
+                                                          0.25 / f.g.dx[2], tl);
+            if (!f.finer) {
+                amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                    kf, f.ftype.as<unsigned char>(), f.y.as<double>(), nullptr, f.b.as<double>(), nullptr, tl);
+            } else {
+                AmrFine& u = S.lev(l + 1);
+                amr_coarse_combine_kernel<double><<<nbk, tile_block(), 0, s>>>(
+                    kgeom(u.g), kf, u.ftype.as<unsigned char>(), f.dof.as<unsigned char>(), f.y.as<double>(),
+                    u.y.as<double>(), 1.0, f.b.as<double>(), nullptr, f.cnear.as<unsigned char>(), tl);
+            }
         }
-        amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
-            kf, kc, ft, S.cdof.as<unsigned char>(), S.rtmp.as<double>(), S.yf.as<double>(), 1.0,
-            S.bc.as<double>(), nullptr, S.cnear.as<unsigned char>());
+        {
+            const KGeom kc = kgeom(S.gc);
+            nodal_rhs_kernel<<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
+                kc, h->ug[0]->as<double>(), S.rtmp.as<double>(), 0.25 / S.gc.dx[0], 0.25 / S.gc.dx[1],
+                0.25 / S.gc.dx[2]);
+            if (L > 0)
+                amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
+                    kgeom(S.lev(1).g), kc, S.lev(1).ftype.as<unsigned char>(), S.cdof.as<unsigned char>(),
+                    S.rtmp.as<double>(), S.lev(1).y.as<double>(), 1.0, S.bc.as<double>(), nullptr,
+                    S.cnear.as<unsigned char>());
+            else
+                amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
+                    kc, kc, nullptr, S.cdof.as<unsigned char>(), S.rtmp.as<double>(), nullptr, 0.0,
+                    S.bc.as<double>(), nullptr, nullptr);
+        }
         S.solve(rtol, atol, mode, info, s);
-        // u -= sigma G phi on both levels, G phi out; covered coarse cells averaged down
-        UpdateArgs uc{};
-        uc.g = kc;
-        uc.boxes = h->lay_c.dev_boxes();
-        uc.tiles = h->lay_c.tiles(REG_CELL);
-        uc.phi = S.xc.as<double>();
-        uc.ug = h->ugc.as<double>();
-        uc.sig = S.sigc.as<double>();
-        for (int d = 0; d < 3; ++d) uc.f[d] = 0.25 / S.gc.dx[d];
-        uc.vel = vel[0];
-        uc.vel_ng = vel_ng;
-        uc.gp = gphi ? gphi[0] : nullptr;
-        uc.gp_ng = gphi_ng;
-        update_cells_kernel<<<h->lay_c.ntiles[REG_CELL], 256, 0, s>>>(uc);
-        if (two) {
-            UpdateArgs uf = uc;
-            uf.g = kf;
-            uf.boxes = h->lay_f.dev_boxes();
-            uf.tiles = h->lay_f.tiles(REG_CELL);
-            uf.phi = S.xf.as<double>();
-            uf.ug = h->ugf.as<double>();
-            uf.sig = S.sigf.as<double>();
-            for (int d = 0; d < 3; ++d) uf.f[d] = 0.25 / S.gf.dx[d];
-            uf.vel = vel[1];
-            uf.gp = gphi ? gphi[1] : nullptr;
-            update_cells_kernel<<<h->lay_f.ntiles[REG_CELL], 256, 0, s>>>(uf);
-            AvgDownArgs ad{};
-            ad.gc = kc;
-            ad.gf = kf;
-            ad.boxes = h->lay_c.dev_boxes();
-            ad.tiles = h->lay_c.tiles(REG_CELL);
-            ad.cov = S.ccov.as<unsigned char>();
-            ad.phif = S.xf.as<double>();
-            ad.ugf = h->ugf.as<double>();
-            ad.sigf = S.sigf.as<double>();
-            for (int d = 0; d < 3; ++d) ad.ff[d] = 0.25 / S.gf.dx[d];
-            ad.vel = vel[0];
-            ad.vel_ng = vel_ng;
-            ad.gp = gphi ? gphi[0] : nullptr;
-            ad.gp_ng = gphi_ng;
-            amr_avgdown_kernel<<<h->lay_c.ntiles[REG_CELL], 256, 0, s>>>(ad);
+        // u -= sigma G phi on every level, G phi out; from the finest level
+        // down, cells under a finer level take the mean of their children
+        for (int l = L; l >= 0; --l) {
+            const NodalGeom& g = geom(l);
+            UpdateArgs u{};
+            u.g = kgeom(g);
+            u.boxes = h->lay[l]->dev_boxes();
+            u.tiles = h->lay[l]->tiles(REG_CELL);
+            u.phi = l == 0 ? S.xc.as<double>() : S.lev(l).x.as<double>();
+            u.ug = h->ug[l]->as<double>();
+            u.sig = l == 0 ? S.sigc.as<double>() : S.lev(l).sig.as<double>();
+            for (int d = 0; d < 3; ++d) u.f[d] = 0.25 / g.dx[d];
+            u.vel = vel[l];
+            u.vel_ng = vel_ng;
+            u.gp = gphi ? gphi[l] : nullptr;
+            u.gp_ng = gphi_ng;
+            if (l > 0) {
+                u.ucan = h->ug[l]->as<double>();
+                u.gcan = gphi && gphi[l] ? h->gcan[l]->as<double>() : nullptr;
+            }
+            update_cells_kernel<<<h->lay[l]->ntiles[REG_CELL], 256, 0, s>>>(u);
+            if (l < L) {
+                AvgDownArgs ad{};
+                ad.gc = kgeom(g);
+                ad.gf = kgeom(geom(l + 1));
+                ad.boxes = h->lay[l]->dev_boxes();
+                ad.tiles = h->lay[l]->tiles(REG_CELL);
+                ad.cov = l == 0 ? S.ccov.as<unsigned char>() : S.lev(l).ccov.as<unsigned char>();
+                ad.uf = h->ug[l + 1]->as<double>();
+                ad.gf_ = gphi && gphi[l + 1] ? h->gcan[l + 1]->as<double>() : nullptr;
+                ad.vel = vel[l];
+                ad.vel_ng = vel_ng;
+                ad.gp = gphi ? gphi[l] : nullptr;
+                ad.gp_ng = gphi_ng;
+                if (l > 0) {
+                    ad.uc = h->ug[l]->as<double>();
+                    ad.gc_ = gphi && gphi[l] ? h->gcan[l]->as<double>() : nullptr;
+                }
+                amr_avgdown_kernel<<<h->lay[l]->ntiles[REG_CELL], 256, 0, s>>>(ad);
+            }
         }
-        if (phi) {
-            if (phi[0])
-                scatter_nodes_kernel<<<h->lay_c.ntiles[REG_NODE], 256, 0, s>>>(
-                    kc, h->lay_c.dev_boxes(), h->lay_c.tiles(REG_NODE), S.xc.as<double>(), phi[0],
-                    phi_ng, 0);
-            if (two && phi[1])
-                scatter_nodes_kernel<<<h->lay_f.ntiles[REG_NODE], 256, 0, s>>>(
-                    kf, h->lay_f.dev_boxes(), h->lay_f.tiles(REG_NODE), S.xf.as<double>(), phi[1],
-                    phi_ng, 0);
-        }
+        if (phi)
+            for (int l = 0; l <= L; ++l)
+                if (phi[l])
+                    scatter_nodes_kernel<<<h->lay[l]->ntiles[REG_NODE], 256, 0, s>>>(
+                        kgeom(geom(l)), h->lay[l]->dev_boxes(), h->lay[l]->tiles(REG_NODE),
+                        l == 0 ? S.xc.as<double>() : S.lev(l).x.as<double>(), phi[l], phi_ng, 0);
         MGX_HIP(hipStreamSynchronize(s));
     });
     if (status == MGX_OK && info && !info->converged) {
@@ -711,8 +981,8 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
 
 namespace {
 
-// both levels' layouts (the fine boxes need not tile the domain), checked for
-// ratio 2 and coarse-aligned fine boxes
+// both levels' layouts (neither needs to tile the domain), checked for ratio 2
+// and coarse-aligned fine boxes
 void sync_layouts(const mgx_level_desc* c, const mgx_level_desc* f, BoxLayout& lc, BoxLayout& lf,
                   KGeom& kf) {
     MGX_REQUIRE(c && f && c->boxes && f->boxes && c->nboxes > 0 && f->nboxes > 0, MGX_ERR_ARG,
@@ -732,7 +1002,9 @@ void sync_layouts(const mgx_level_desc* c, const mgx_level_desc* f, BoxLayout& l
                         MGX_ERR_ARG, "average down: fine boxes must align with coarse cells");
     const NodalGeom gc = make_nodal_geom(c->n_cell, per, zero, zero, dxc);
     const NodalGeom gf = make_nodal_geom(f->n_cell, per, zero, zero, dxf);
-    lc.build(c->boxes, c->nboxes, gc);
+    // the coarse level may itself be a fine level (any pair of neighbouring
+    // levels); a fine cell whose parent no coarse box holds is skipped
+    lc.build(c->boxes, c->nboxes, gc, true);
     lf.build(f->boxes, f->nboxes, gf, true);
     kf = kgeom(gf);
 }

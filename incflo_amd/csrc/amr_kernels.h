@@ -1,6 +1,11 @@
-// amr_kernels.h -- kernels of the two-level composite nodal solve
-// (refinement ratio 2; the reference projects all AMR levels in one
-// NodalProjector, src/projection/incflo_apply_nodal_projection.cpp:124-215).
+// amr_kernels.h -- kernels of the composite nodal solve over AMR levels
+// (refinement ratio 2 between neighbours; the reference projects all AMR
+// levels in one NodalProjector,
+// src/projection/incflo_apply_nodal_projection.cpp:124-215).  Each kernel
+// works on one pair of neighbouring levels ("coarse" and "fine" below); a
+// level between two others is the fine level of one pair and the coarse
+// level of the next, where its unknowns are its AMR_DOF nodes not surrounded
+// by cells under the finer level.
 //
 // Composite problem: conforming Q1 elements on the composite mesh.  F is the
 // union of the fine boxes; its fine nodes are unknowns (AMR_DOF), fine nodes
@@ -119,19 +124,22 @@ __device__ __forceinline__ void wave_max_atomic(unsigned long long* norm, double
 }
 
 // coarse unknowns: out = base + sgn R(y on dF); every other coarse node 0;
-// optional max |out| (every lane reaches the shuffles)
+// optional max |out| (every lane reaches the shuffles); tl: the coarse
+// level's tile list (a level above level 0), else the whole coarse grid
 template <class TO>
 static __global__ void __launch_bounds__(256) amr_coarse_combine_kernel(
     KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
     const double* __restrict__ base, const double* __restrict__ y, double sgn, TO* __restrict__ out,
-    unsigned long long* norm, const unsigned char* __restrict__ near = nullptr) {
+    unsigned long long* norm, const unsigned char* __restrict__ near = nullptr,
+    const int4* tl = nullptr) {
     int I, J, K;
-    const bool valid = grid_point(gc.nn[0], gc.nn[1], I, J, K);
+    const bool valid = grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K);
     double r = 0.0;
     if (valid) {
         const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
         if (cdof[id] && !is_dir(gc, I, J, K))
-            r = base[id] + ((near && !near[id]) ? 0.0 : sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K));
+            r = base[id] + ((sgn == 0.0 || (near && !near[id]))
+                                ? 0.0 : sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K));
         out[id] = (TO)r;
     }
     if (norm) wave_max_atomic(norm, nabs(r));
@@ -148,7 +156,7 @@ static __global__ void __launch_bounds__(256) amr_fine_resid_kernel(
     double r = 0.0;
     if (valid) {
         const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
-        if (ft[id] == AMR_DOF) r = b[id] - (y ? y[id] : 0.0);
+        if (ft[id] == AMR_DOF && !is_dir(gf, i, j, k)) r = b[id] - (y ? y[id] : 0.0);
         out[id] = (TO)r;
     }
     if (norm) wave_max_atomic(norm, nabs(r));
@@ -161,9 +169,9 @@ template <class T>
 static __global__ void __launch_bounds__(256) amr_coarse_rhs_kernel(
     KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
     const T* __restrict__ rf, const T* __restrict__ yf, const T* __restrict__ rc, T* __restrict__ bC,
-    const unsigned char* __restrict__ near = nullptr) {
+    const unsigned char* __restrict__ near = nullptr, const int4* tl = nullptr) {
     int I, J, K;
-    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    if (!grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (is_dir(gc, I, J, K)) { bC[id] = 0; return; }
     if (near && !near[id]) {  // no fine node of F's closure around: the coarse residual alone
@@ -210,9 +218,9 @@ static __global__ void __launch_bounds__(256) amr_coarse_jacobi_kernel(
     KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
     const T* __restrict__ rc, const T* __restrict__ yc, const T* __restrict__ yf,
     const T* __restrict__ sig, double diag4, double omega, T* __restrict__ ec,
-    const unsigned char* __restrict__ near = nullptr) {
+    const unsigned char* __restrict__ near = nullptr, const int4* tl = nullptr) {
     int I, J, K;
-    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    if (!grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (!cdof[id] || is_dir(gc, I, J, K)) return;
     double acc = (double)yc[id];
@@ -294,6 +302,23 @@ static __global__ void amr_mean_kernel(const double* __restrict__ pa, int na, do
     out[0] = (wa * a + wb * b) / count;
 }
 
+// out[0] = sum_l 8^(nparts-1-l) sum(part[l np .. (l+1) np)) / count: the
+// weighted mean of a right-hand side over the unknowns of nparts levels in the
+// symmetric scaling (a level's rows times 8 per level below the finest), one
+// thread, fixed order
+static __global__ void amr_mean_levels_kernel(const double* __restrict__ part, int np, int nparts,
+                                              double count, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc = 0.0, w = 1.0;
+    for (int l = nparts - 1; l >= 0; --l) {
+        double a = 0.0;
+        for (int q = 0; q < np; ++q) a += part[(long)l * np + q];
+        acc += w * a;
+        w *= 8.0;
+    }
+    out[0] = acc / count;
+}
+
 // v -= f * mu[0] where mask == want (mask NULL: everywhere), or where mask != want when invert
 static __global__ void __launch_bounds__(256) amr_shift_kernel(double* __restrict__ v,
                                                                const unsigned char* __restrict__ mask,
@@ -305,15 +330,19 @@ static __global__ void __launch_bounds__(256) amr_shift_kernel(double* __restric
     v[q] -= f * mu[0];
 }
 
-// coarse nodes under F (not unknowns) take the coincident fine value
+// coarse nodes under F (not unknowns) take the coincident fine value; ftc:
+// the coarse level's own node types when it is itself a fine level (only its
+// AMR_DOF nodes can lie under F)
 static __global__ void __launch_bounds__(256) amr_inject_kernel(KGeom gf, KGeom gc,
                                                                 const unsigned char* __restrict__ cdof,
                                                                 const double* __restrict__ xf,
-                                                                double* __restrict__ xc) {
+                                                                double* __restrict__ xc,
+                                                                const unsigned char* __restrict__ ftc = nullptr,
+                                                                const int4* tl = nullptr) {
     int I, J, K;
-    if (!grid_point(gc.nn[0], gc.nn[1], I, J, K)) return;
+    if (!grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
-    if (cdof[id]) return;
+    if (cdof[id] || (ftc && ftc[id] != AMR_DOF)) return;
     xc[id] = xf[2 * I + (long)gf.nn[0] * (2 * J + (long)gf.nn[1] * 2 * K)];
 }
 
@@ -340,18 +369,21 @@ static __global__ void __launch_bounds__(256) amr_zero_covered_kernel(KGeom gc, 
 
 // average_down onto covered coarse cells of the caller's coarse boxes:
 // u_c = mean of the 8 projected fine children, gp_c = mean of their G phi
-// (src/projection/incflo_apply_nodal_projection.cpp:258-266)
+// (src/projection/incflo_apply_nodal_projection.cpp:258-266), read from the
+// finer level's canonical copies (written by its update, and by its own
+// average-down where a still finer level lies under it); the coarse level's
+// canonical copies are written too when a still coarser level follows.
 struct AvgDownArgs {
     KGeom gc, gf;
     const BoxInfo* boxes;   // coarse layout
     const Tile* tiles;
     const unsigned char* cov;
-    const double* phif;     // canonical fine phi (closure of F filled)
-    const double* ugf;      // canonical ghosted fine u*
-    const double* sigf;     // canonical ghosted fine sigma
-    double ff[3];           // fine 1/(4 h_d)
+    const double* uf;       // canonical ghosted fine u_new (3 comps)
+    const double* gf_;      // canonical ghosted fine G phi (3 comps; NULL: no gp)
     double* vel; int vel_ng;
     double* gp; int gp_ng;
+    double* uc;             // canonical ghosted coarse u_new / G phi (NULL: not kept)
+    double* gc_;
 };
 
 static __global__ void __launch_bounds__(256) amr_avgdown_kernel(AvgDownArgs a) {
@@ -360,27 +392,34 @@ static __global__ void __launch_bounds__(256) amr_avgdown_kernel(AvgDownArgs a) 
     if (!a.cov[i + (long)a.gc.n[0] * (j + (long)a.gc.n[1] * k)]) return;
     const BoxInfo& b = a.boxes[bi];
     const KGeom& gf = a.gf;
-    const long ug3 = (long)gf.sg[0] * gf.sg[1] * gf.sg[2];
+    const long f3 = (long)gf.sg[0] * gf.sg[1] * gf.sg[2];
+    const long c3 = (long)a.gc.sg[0] * a.gc.sg[1] * a.gc.sg[2];
+    const long ci = (i + 1) + (long)a.gc.sg[0] * ((j + 1) + (long)a.gc.sg[1] * (k + 1));
     double us[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
     for (int q = 0; q < 8; ++q) {
         const int fi = 2 * i + (q & 1), fj = 2 * j + ((q >> 1) & 1), fk = 2 * k + ((q >> 2) & 1);
-        double gr[3];
-        cell_gradient(gf, a.phif, fi, fj, fk, a.ff, gr);
         const long gi = (fi + 1) + (long)gf.sg[0] * ((fj + 1) + (long)gf.sg[1] * (fk + 1));
-        const double s = a.sigf[gi];
         for (int c = 0; c < 3; ++c) {
-            gs[c] += gr[c];
-            us[c] += a.ugf[c * ug3 + gi] - s * gr[c];
+            us[c] += a.uf[c * f3 + gi];
+            if (a.gf_) gs[c] += a.gf_[c * f3 + gi];
         }
+    }
+    for (int c = 0; c < 3; ++c) {
+        us[c] *= 0.125;
+        gs[c] *= 0.125;
     }
     if (a.vel) {
         const long off = field_offset(b, a.vel_ng, 3, false);
-        for (int c = 0; c < 3; ++c) a.vel[off + box_index(b, a.vel_ng, false, i, j, k, c)] = 0.125 * us[c];
+        for (int c = 0; c < 3; ++c) a.vel[off + box_index(b, a.vel_ng, false, i, j, k, c)] = us[c];
     }
-    if (a.gp) {
+    if (a.gp && a.gf_) {
         const long off = field_offset(b, a.gp_ng, 3, false);
-        for (int c = 0; c < 3; ++c) a.gp[off + box_index(b, a.gp_ng, false, i, j, k, c)] = 0.125 * gs[c];
+        for (int c = 0; c < 3; ++c) a.gp[off + box_index(b, a.gp_ng, false, i, j, k, c)] = gs[c];
     }
+    if (a.uc)
+        for (int c = 0; c < 3; ++c) a.uc[c * c3 + ci] = us[c];
+    if (a.gc_ && a.gf_)
+        for (int c = 0; c < 3; ++c) a.gc_[c * c3 + ci] = gs[c];
 }
 
 }  // namespace mgx

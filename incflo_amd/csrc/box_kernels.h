@@ -225,6 +225,10 @@ struct UpdateArgs {
     const int* eb_cidx;
     const double* eb_W;
     const double* eb_V;
+    // canonical ghosted copies of u_new / G phi at the valid cells (NULL:
+    // none; ucan may be ug itself: each cell reads u* before writing)
+    double* ucan;
+    double* gcan;
 };
 
 // u_new = u* - sigma G(phi) (+ u_old); gp = G(phi) or gp += G(phi)
@@ -266,7 +270,13 @@ static __global__ void __launch_bounds__(256) update_cells_kernel(UpdateArgs a) 
                                           box_index(b, a.vold_ng, false, i, j, k, c)];
             if (ci == -2) u = 0.0;  // covered cells carry no velocity
             a.vel[vi] = u;
+            if (a.ucan) a.ucan[c * ug3 + gi] = u;
         }
+    }
+    if (a.gcan) {
+        const long ug3 = (long)g.sg[0] * g.sg[1] * g.sg[2];
+        const long gi = (i + 1) + (long)g.sg[0] * ((j + 1) + (long)g.sg[1] * (k + 1));
+        for (int c = 0; c < 3; ++c) a.gcan[c * ug3 + gi] = gr[c];
     }
     if (a.gp) {
         const long off = field_offset(b, a.gp_ng, 3, false);

@@ -512,8 +512,43 @@ MGX_API int mgx_fill_fine_level(mgx_ctx ctx, const mgx_level_desc* crse, const d
         const KGeom kc = cell_kgeom(gc), kf = cell_kgeom(gf);
         hipStream_t s = ctx->stream;
         BoxLayout lc, lf, lo;
-        lc.build(crse->boxes, crse->nboxes, gc, false);
+        lc.build(crse->boxes, crse->nboxes, gc, true);
         lf.build(fine->boxes, fine->nboxes, gf, true);
+        {
+            // the coarse level may itself be a fine level: it must cover every
+            // coarse cell the interpolation reads (the parents of the fine
+            // cells and their neighbours)
+            std::vector<unsigned char> cov((size_t)gc.n[0] * gc.n[1] * gc.n[2], 0);
+            for (int b = 0; b < crse->nboxes; ++b) {
+                const int* q = crse->boxes + 6 * b;
+                for (int k = q[2]; k <= q[5]; ++k)
+                    for (int j = q[1]; j <= q[4]; ++j)
+                        for (int i = q[0]; i <= q[3]; ++i)
+                            cov[i + (size_t)gc.n[0] * (j + (size_t)gc.n[1] * k)] = 1;
+            }
+            for (int b = 0; b < fine->nboxes; ++b) {
+                const int* q = fine->boxes + 6 * b;
+                int lo3[3], hi3[3];
+                for (int d = 0; d < 3; ++d) {
+                    lo3[d] = q[d] / 2 - 1;
+                    hi3[d] = q[3 + d] / 2 + 1;
+                    if (!gc.per[d]) {
+                        lo3[d] = std::max(lo3[d], 0);
+                        hi3[d] = std::min(hi3[d], gc.n[d] - 1);
+                    }
+                }
+                for (int k = lo3[2]; k <= hi3[2]; ++k)
+                    for (int j = lo3[1]; j <= hi3[1]; ++j)
+                        for (int i = lo3[0]; i <= hi3[0]; ++i) {
+                            const int w[3] = {(i + gc.n[0]) % gc.n[0], (j + gc.n[1]) % gc.n[1],
+                                              (k + gc.n[2]) % gc.n[2]};
+                            MGX_REQUIRE(cov[w[0] + (size_t)gc.n[0] * (w[1] + (size_t)gc.n[1] * w[2])],
+                                        MGX_ERR_ARG,
+                                        "fill_fine_level: the fine level is not properly nested in "
+                                        "the coarse level (a parent cell or its neighbour is missing)");
+                        }
+            }
+        }
         DevBuf cg, dense, own;
         const long ng3 = (long)(gc.n[0] + 2) * (gc.n[1] + 2) * (gc.n[2] + 2);
         cg.alloc(ng3 * ncomp * sizeof(double));

@@ -813,8 +813,8 @@ def tag_cells(ctx, level, rho=None, rho_ng=0, rhoerr=None, gradrhoerr=None, regi
 
 def cluster_tags(tags, periodic=(1, 1, 1), n_error_buf=1, blocking_factor=8, max_grid_size=32,
                  grid_eff=0.7, max_boxes=1 << 16):
-    """AmrMesh::MakeNewGrids for level 1 from level-0 tags (nz, ny, nx) ->
-    fine boxes [(lo, hi)] in fine index space."""
+    """AmrMesh::MakeNewGrids for level l+1 from level-l tags (nz, ny, nx) ->
+    boxes [(lo, hi)] in the index space of level l+1."""
     t = np.ascontiguousarray(tags, dtype=np.uint8)
     n = (C.c_int * 3)(t.shape[2], t.shape[1], t.shape[0])
     per = (C.c_int * 3)(*periodic)
@@ -897,8 +897,9 @@ def amr_average_down(ctx, crse, fine, fine_cells, fine_ng, crse_cells, crse_ng, 
 
 
 class AmrNodalProjector:
-    """Composite two-level nodal projection (mgx_amr_nodal_*): levels[0] tiles
-    the domain, levels[1] holds the fine boxes (refinement ratio 2)."""
+    """Composite nodal projection over AMR levels (mgx_amr_nodal_*): levels[0]
+    tiles the domain, each further level holds the boxes of the next finer
+    level (refinement ratio 2, properly nested; at most MGX_AMR_MAX_LEVELS)."""
 
     def __init__(self, ctx, levels):
         self.ctx = ctx

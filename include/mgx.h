@@ -301,19 +301,24 @@ MGX_API int mgx_eb_cylinder_apertures(const mgx_level_desc* level, double cx, do
 MGX_API int mgx_eb_cylinder_geometry(const mgx_level_desc* level, double cx, double cy, double r,
                                      int* type, double* moments);
 
-/* ---- AMR: composite two-level nodal projection --------------------------
+/* ---- AMR: composite nodal projection over the levels ---------------------
  * incflo projects every AMR level with one NodalProjector
  * (src/projection/incflo_apply_nodal_projection.cpp:124-215: vel and sigma
  * per level, project(rtol, atol), getPhi / getGradPhi per level) and then
- * averages gp down (:258-266).  levels[0] tiles the domain; levels[1] (if
- * nlevels == 2) has n_cell = 2 x levels[0].n_cell, the same domain and
- * periodicity, and boxes aligned with coarse cells (even lo, odd hi) that
- * stay off non-periodic domain faces.  The composite problem is the
- * conforming Q1 discretisation on the composite mesh (fine nodes inside the
- * fine region, coarse nodes elsewhere; coarse-fine boundary nodes follow the
- * coarse level); covered coarse cells receive the average of their fine
- * children for the velocity and G phi, coarse nodes under the fine region the
- * fine value. */
+ * averages gp down (:258-266).  levels[0] tiles the domain; each further
+ * level (up to MGX_AMR_MAX_LEVELS in all) has n_cell = 2 x the level below,
+ * the same domain and periodicity, boxes aligned with the cells of the level
+ * below (even lo, odd hi) and properly nested in it (every node of the level
+ * below that touches a cell under the finer level lies inside the level
+ * below's own region: one cell of buffer away from domain faces).  Fine
+ * boxes may touch Dirichlet and Neumann domain faces, whose ghost velocity
+ * the caller sets to 0, but not inflow faces (MGX_ERR_UNSUPPORTED).  The
+ * composite problem is the conforming Q1 discretisation on the composite
+ * mesh (each level's nodes inside its region, nodes on a level's boundary
+ * following the level below); cells under a finer level receive the average
+ * of their 8 children for the velocity and G phi (applied from the finest
+ * level down), nodes under a finer level its value. */
+#define MGX_AMR_MAX_LEVELS 4
 typedef struct mgx_amr_s* mgx_amr_nodal;
 MGX_API int mgx_amr_nodal_create(mgx_ctx ctx, const mgx_level_desc* levels, int nlevels,
                                  mgx_amr_nodal* out);
@@ -622,10 +627,11 @@ MGX_API int mgx_write_plotfile(mgx_ctx ctx, const char* dir, int nlevels,
                                int nvars, const char* const* varnames, double time,
                                const int* level_steps, const int* ref_ratio);
 
-/* ---- dynamic refinement of a two-level hierarchy (ratio 2) ---------------
- * incflo regrids through AmrCore: ErrorEst tags coarse cells, AMReX's grid
- * generation clusters them into fine boxes, and MakeNewLevelFromCoarse /
- * RemakeLevel fill the fields of the new fine level (src/incflo_regrid.cpp:9-113).
+/* ---- dynamic refinement (ratio 2 between neighbouring levels) ------------
+ * incflo regrids through AmrCore: ErrorEst tags the cells of each level,
+ * AMReX's grid generation clusters them into the boxes of the next level, and
+ * MakeNewLevelFromCoarse / RemakeLevel fill the fields of every new level
+ * (src/incflo_regrid.cpp:9-113).
  *
  * incflo::ErrorEst (src/incflo_tagging.cpp:11-198) on one level: tags[c] = 1
  * (canonical cell index, x fastest; device bytes, other entries left as they
@@ -637,8 +643,8 @@ MGX_API int mgx_write_plotfile(mgx_ctx ctx, const char* dir, int nlevels,
 MGX_API int mgx_tag_cells(mgx_ctx ctx, const mgx_level_desc* level, const double* rho, int rho_ng,
                           double rhoerr, double gradrhoerr, const double* region_lo,
                           const double* region_hi, unsigned char* tags);
-/* AmrMesh::MakeNewGrids for level 1 over a level-0 domain of n_cell cells
- * (host memory): tags buffered by n_error_buf cells (periodic directions
+/* AmrMesh::MakeNewGrids for level l + 1 from the tags of level l over its
+ * domain of n_cell cells (host memory): tags buffered by n_error_buf cells (periodic directions
  * wrap), coarsened to blocks of blocking_factor/2 coarse cells, clustered by
  * Berger-Rigoutsos to grid_eff, chopped to max_grid_size and refined: fine
  * boxes (6 ints each, fine index space) aligned to blocking_factor; at most
@@ -658,7 +664,10 @@ MGX_API int mgx_cluster_tags(const unsigned char* tags, const int n_cell[3],
  * direction's slope limiter (as one FillPatch of a multi-component
  * field does); bcrec: ncomp x 6 amrex::BCType values (NULL: all int_dir)
  * select the one-sided slope next to ext_dir / hoextrap faces.  The coarse
- * field needs c_ng >= 1 filled ghost cells on a bounded domain. */
+ * field needs c_ng >= 1 filled ghost cells on a bounded domain.  The coarse
+ * level may itself be a fine level (any level below the one filled): its
+ * boxes must cover the parents of the fine cells and their neighbours
+ * (MGX_ERR_ARG otherwise). */
 MGX_API int mgx_fill_fine_level(mgx_ctx ctx, const mgx_level_desc* crse, const double* crse_field,
                                 int c_ng, int ncomp, const int* bcrec,
                                 const mgx_level_desc* fine, double* fine_field, int f_ng,

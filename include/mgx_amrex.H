@@ -153,9 +153,10 @@ struct DevField {
 };
 }  // namespace detail
 
-// Hydro::NodalProjector over one level, or two (refinement ratio 2: the
-// composite projection of mgx_amr_nodal_*, src/projection/
-// incflo_apply_nodal_projection.cpp:124-215).  project() overwrites vel in place.
+// Hydro::NodalProjector over one level, or up to MGX_AMR_MAX_LEVELS
+// (refinement ratio 2: the composite projection of mgx_amr_nodal_*,
+// src/projection/incflo_apply_nodal_projection.cpp:124-215).  project()
+// overwrites vel in place.
 class NodalProjector {
    public:
     NodalProjector(Context& ctx, std::vector<MultiFab*> vel, std::vector<MultiFab const*> sigma,
@@ -168,7 +169,7 @@ class NodalProjector {
                    std::vector<Geometry> geom, LPInfo info = LPInfo())
         : ctx_(ctx), vel_(vel), geom_(geom) {
         init(info);
-        if (ha_) Abort("NodalProjector: two levels need a sigma field per level");
+        if (ha_) Abort("NodalProjector: AMR levels need a sigma field per level");
         ctx_.check(mgx_nodal_set_const_sigma(h_, const_sigma), "set sigma");
     }
     NodalProjector(std::vector<MultiFab*> vel, std::vector<MultiFab const*> sigma,
@@ -191,7 +192,7 @@ class NodalProjector {
             r[k] = (int)hi[k];
         }
         if (ha_) {
-            // the two-level handle re-sets up its levels: hand sigma over again
+            // the AMR handle re-sets up its levels: hand sigma over again
             ctx_.check(mgx_amr_nodal_set_domain_bc(ha_, l, r), "setDomainBC");
             set_sigma();
         } else {
@@ -211,16 +212,18 @@ class NodalProjector {
     }
     void project(double rtol, double atol) {
         if (ha_) {
-            for (int l = 0; l < 2; ++l) {
+            const size_t n = vel_.size();
+            std::vector<double*> v(n), ph(n), gp(n);
+            for (size_t l = 0; l < n; ++l) {
                 phis_[l].alloc(ctx_.get(), vel_[l]->ba, 1, true);
                 gphis_[l].alloc(ctx_.get(), vel_[l]->ba, 3, false);
+                if (vel_[l]->ngrow != vel_[0]->ngrow) Abort("NodalProjector: velocity ghost widths differ");
+                v[l] = vel_[l]->p;
+                ph[l] = phis_[l].view.p;
+                gp[l] = gphis_[l].view.p;
             }
-            if (vel_[0]->ngrow != vel_[1]->ngrow) Abort("NodalProjector: velocity ghost widths differ");
-            double* v[2] = {vel_[0]->p, vel_[1]->p};
-            double* ph[2] = {phis_[0].view.p, phis_[1].view.p};
-            double* gp[2] = {gphis_[0].view.p, gphis_[1].view.p};
-            ctx_.check(mgx_amr_nodal_project(ha_, v, vel_[0]->ngrow, ph, 0, gp, 0, rtol, atol, mode_,
-                                             &info_),
+            ctx_.check(mgx_amr_nodal_project(ha_, v.data(), vel_[0]->ngrow, ph.data(), 0, gp.data(), 0,
+                                             rtol, atol, mode_, &info_),
                        "NodalProjector::project");
             projected_ = true;
             return;
@@ -231,7 +234,9 @@ class NodalProjector {
     std::vector<MultiFab*> getPhi() {
         if (ha_) {
             if (!projected_) Abort("getPhi before project");
-            return {&phis_[0].view, &phis_[1].view};
+            std::vector<MultiFab*> out;
+            for (size_t l = 0; l < vel_.size(); ++l) out.push_back(&phis_[l].view);
+            return out;
         }
         phi_.alloc(ctx_.get(), vel_[0]->ba, 1, true);
         ctx_.check(mgx_nodal_get_phi(h_, phi_.view.p, 0), "getPhi");
@@ -240,7 +245,9 @@ class NodalProjector {
     std::vector<MultiFab*> getGradPhi() {
         if (ha_) {
             if (!projected_) Abort("getGradPhi before project");
-            return {&gphis_[0].view, &gphis_[1].view};
+            std::vector<MultiFab*> out;
+            for (size_t l = 0; l < vel_.size(); ++l) out.push_back(&gphis_[l].view);
+            return out;
         }
         gphi_.alloc(ctx_.get(), vel_[0]->ba, 3, false);
         ctx_.check(mgx_nodal_get_gradphi(h_, gphi_.view.p, 0), "getGradPhi");
@@ -252,12 +259,15 @@ class NodalProjector {
 
    private:
     void init(const LPInfo& info) {
-        if (vel_.size() == 2 && geom_.size() == 2) {
-            flat_ = vel_[0]->ba->flat();
-            flat1_ = vel_[1]->ba->flat();
-            mgx_level_desc d[2] = {detail::level_desc(geom_[0], flat_),
-                                   detail::level_desc(geom_[1], flat1_)};
-            ctx_.check(mgx_amr_nodal_create(ctx_.get(), d, 2, &ha_), "NodalProjector");
+        if (vel_.size() >= 2 && vel_.size() == geom_.size() && vel_.size() <= MGX_AMR_MAX_LEVELS) {
+            const size_t n = vel_.size();
+            flats_.resize(n);
+            std::vector<mgx_level_desc> d(n);
+            for (size_t l = 0; l < n; ++l) {
+                flats_[l] = vel_[l]->ba->flat();
+                d[l] = detail::level_desc(geom_[l], flats_[l]);
+            }
+            ctx_.check(mgx_amr_nodal_create(ctx_.get(), d.data(), (int)n, &ha_), "NodalProjector");
             mgx_default_mlmg_opts(&opts_);
             opts_.max_coarsening_level = info.max_coarsening_level;
             ctx_.check(mgx_amr_nodal_set_options(ha_, &opts_), "set options");
@@ -275,10 +285,13 @@ class NodalProjector {
     void set_sigma() {
         if (sigma_.empty()) return;
         if (ha_) {
-            if (sigma_.size() != 2 || sigma_[0]->ngrow != sigma_[1]->ngrow)
-                Abort("NodalProjector: two levels need sigma fields of equal ghost width");
-            const double* sg[2] = {sigma_[0]->p, sigma_[1]->p};
-            ctx_.check(mgx_amr_nodal_set_sigma(ha_, sg, sigma_[0]->ngrow), "set sigma");
+            std::vector<const double*> sg;
+            for (const MultiFab* m : sigma_) {
+                if (m->ngrow != sigma_[0]->ngrow) Abort("NodalProjector: sigma ghost widths differ");
+                sg.push_back(m->p);
+            }
+            if (sg.size() != vel_.size()) Abort("NodalProjector: AMR levels need a sigma field per level");
+            ctx_.check(mgx_amr_nodal_set_sigma(ha_, sg.data(), sigma_[0]->ngrow), "set sigma");
         } else {
             detail::one_level(sigma_.size(), "NodalProjector");
             ctx_.check(mgx_nodal_set_sigma(h_, sigma_[0]->p, sigma_[0]->ngrow), "set sigma");
@@ -288,15 +301,16 @@ class NodalProjector {
     std::vector<MultiFab*> vel_;
     std::vector<Geometry> geom_;
     std::vector<MultiFab const*> sigma_;
-    std::vector<int> flat_, flat1_;
+    std::vector<int> flat_;
+    std::vector<std::vector<int>> flats_;
     mgx_nodal h_ = nullptr;
-    mgx_amr_nodal ha_ = nullptr;  // two levels
+    mgx_amr_nodal ha_ = nullptr;  // AMR levels
     mgx_mlmg_opts opts_{};
     mgx_solve_info info_{};
     int mode_ = MGX_MODE_FP64;
     bool projected_ = false;
     detail::DevField phi_, gphi_;
-    detail::DevField phis_[2], gphis_[2];
+    detail::DevField phis_[MGX_AMR_MAX_LEVELS], gphis_[MGX_AMR_MAX_LEVELS];
 };
 
 // Hydro::MacProjector (no EB) over one level, or two (refinement ratio 2: the

@@ -1,4 +1,6 @@
-"""Two-level composite nodal projection (AMR, refinement ratio 2): FP64 CPU oracle.
+"""Composite nodal projection over AMR levels (refinement ratio 2): FP64 CPU oracle.
+AmrGeom is the two-level form; AmrLevels (below) applies the same construction
+to any number of levels.
 
 TEST INFRASTRUCTURE ONLY (the checker for the GPU composite solve); the
 product library never loads it.
@@ -312,3 +314,230 @@ def average_down_cells(fine_cells_mask, uf, uc):
     s = sum(uf[a::2, b::2, c::2] for a in (0, 1) for b in (0, 1) for c in (0, 1))
     cov = F[0::2, 0::2, 0::2]
     return np.where(cov, 0.125 * s, uc)
+
+
+# ---- any number of levels (ratio 2 between neighbours) ---------------------------
+def dirichlet_nodes(g):
+    """boolean node array: nodes on a Dirichlet domain face (fixed at 0)"""
+    m = np.zeros(g.node_shape(), bool)
+    for d, ax in ((0, 2), (1, 1), (2, 0)):
+        if g.periodic[d]:
+            continue
+        sl = [slice(None)] * 3
+        if g.bclo[d] == 1:
+            sl[ax] = 0
+            m[tuple(sl)] = True
+        if g.bchi[d] == 1:
+            sl[ax] = -1
+            m[tuple(sl)] = True
+    return m
+
+
+class AmrLevels:
+    """Composite nodal problem over levels 0..L, each refining the one below
+    by 2 (incflo projects levels 0..finest_level in one NodalProjector,
+    src/projection/incflo_apply_nodal_projection.cpp:124-215).
+
+    The two-level construction of AmrGeom applied level by level: F_l is the
+    union of level l's boxes (level 0 covers the domain), C_l the level-l cells
+    under F_{l+1}.  Level-l unknowns are the nodes of F_l's interior (every
+    in-domain cell around them in F_l) that are not surrounded by C_l, minus
+    Dirichlet nodes.  Level-l nodes on dF_l take the trilinear interpolation
+    of the level l-1 field; level-l nodes surrounded by C_l are represented on
+    level l+1.  Elements: the cells of F_l \\ C_l on each level.  Rows, in
+    finite-difference units of their own level:
+        (A x)_l = L_l^out x~_l + R(L_{l+1}^out x~_{l+1} on dF_{l+1})
+    with L_l^out the level-l operator over the cells of F_l \\ C_l.  F_{l+1}
+    must be properly nested in F_l: every level-l node touching C_l lies in
+    F_l's interior.  Fine boxes may touch Dirichlet and Neumann domain faces
+    (ghost velocity 0 there) but not inflow faces."""
+
+    OUT, DOF, BDRY = 0, 1, 2
+
+    def __init__(self, gc, level_boxes):
+        self.g = [gc]
+        for _ in level_boxes:
+            self.g.append(refine(self.g[-1]))
+        self.L = L = len(level_boxes)
+        self.boxes = [[tuple(int(v) for v in b) for b in bl] for bl in level_boxes]
+        self.fmask = [np.ones(gc.cell_shape(), bool)]
+        for l, bl in enumerate(self.boxes, 1):
+            g = self.g[l]
+            fm = np.zeros(g.cell_shape(), bool)
+            for b in bl:
+                lo, hi = b[:3], b[3:]
+                for d in range(3):
+                    assert lo[d] % 2 == 0 and hi[d] % 2 == 1, "boxes must align with coarser cells"
+                    assert 0 <= lo[d] <= hi[d] < g.n[d]
+                    if not g.periodic[d]:
+                        assert not (lo[d] == 0 and g.bclo[d] == 3) and \
+                            not (hi[d] == g.n[d] - 1 and g.bchi[d] == 3), \
+                            "fine boxes must stay off inflow faces"
+                fm[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+            self.fmask.append(fm)
+        self.cov = []
+        for l in range(L + 1):
+            if l < L:
+                nz, ny, nx = self.g[l].cell_shape()
+                c = self.fmask[l + 1].reshape(nz, 2, ny, 2, nx, 2).any(axis=(1, 3, 5))
+                assert not (c & ~self.fmask[l]).any(), "level %d outside level %d" % (l + 1, l)
+            else:
+                c = np.zeros(self.g[l].cell_shape(), bool)
+            self.cov.append(c)
+        self.ftype = [None]
+        for l in range(1, L + 1):
+            anyf, allf = _node_any_all(self.g[l], self.fmask[l])
+            t = np.zeros(self.g[l].node_shape(), np.uint8)
+            t[anyf & allf] = self.DOF
+            t[anyf & ~allf] = self.BDRY
+            self.ftype.append(t)
+        self.allcov, self.dof, self.dmask = [], [], []
+        for l in range(L + 1):
+            anyc, allc = _node_any_all(self.g[l], self.cov[l])
+            if 1 <= l < L:
+                assert not (anyc & (self.ftype[l] != self.DOF)).any(), \
+                    "level %d is not properly nested in level %d" % (l + 1, l)
+            base = np.ones(self.g[l].node_shape(), bool) if l == 0 else self.ftype[l] == self.DOF
+            self.allcov.append(allc & base)
+            self.dmask.append(dirichlet_nodes(self.g[l]))
+            self.dof.append(base & ~allc & ~self.dmask[l])
+        self.keep = [self.fmask[l] & ~self.cov[l] for l in range(L + 1)]
+
+    def singular(self):
+        g = self.g[0]
+        return all(g.periodic[d] or (g.bclo[d] != 1 and g.bchi[d] != 1) for d in range(3))
+
+    def composite(self, xs):
+        """per level: the unknowns of xs, dF_l interpolated from level l-1, 0 elsewhere"""
+        out = [np.where(self.dof[0], xs[0], 0.0)]
+        for l in range(1, self.L + 1):
+            p = nodal_prolong_add(self.g[l], out[l - 1], np.zeros(self.g[l].node_shape()))
+            t = self.ftype[l]
+            out.append(np.where(self.dof[l], xs[l],
+                                np.where(t == self.BDRY, np.where(self.dmask[l], 0.0, p), 0.0)))
+        return out
+
+    def _combine(self, y):
+        out = []
+        for l in range(self.L + 1):
+            r = y[l]
+            if l < self.L:
+                r = r + nodal_restrict(self.g[l + 1], np.where(self.ftype[l + 1] == self.BDRY,
+                                                               y[l + 1], 0.0))
+            out.append(np.where(self.dof[l], r, 0.0))
+        return out
+
+    def apply(self, sigs, xs):
+        """composite operator (finite-difference units of each level)"""
+        xt = self.composite(xs)
+        return self._combine([nodal_apply(self.g[l], np.where(self.keep[l], sigs[l], 0.0), xt[l])
+                              for l in range(self.L + 1)])
+
+    def divergence(self, us, inflow_vel=None):
+        """composite nodal divergence of the cell velocities of every level
+        (3 comps, no ghosts; level l's cells outside F_l \\ C_l are ignored)"""
+        d = []
+        for l in range(self.L + 1):
+            g = self.g[l]
+            ug = np.zeros(g.ghost_vec_shape())
+            ug[:, 1:-1, 1:-1, 1:-1] = np.where(self.keep[l][None], us[l], 0.0)
+            fill_vel_ghosts(g, ug, inflow_vel if l == 0 else None)
+            d.append(nodal_divergence(g, ug))
+        return self._combine(d)
+
+    def average_down(self, vf):
+        nz, ny, nx = vf.shape[-3] // 2, vf.shape[-2] // 2, vf.shape[-1] // 2
+        lead = vf.shape[:-3]
+        return vf.reshape(lead + (nz, 2, ny, 2, nx, 2)).mean(axis=(-5, -3, -1))
+
+    def solve(self, sigs, bs, rtol=1e-12, maxiter=20000):
+        """Jacobi-preconditioned CG on the symmetric form (level-l rows x 8^(L-l));
+        returns (xs, iterations, 0 on success)"""
+        L = self.L
+        sc = [8.0 ** (L - l) for l in range(L + 1)]
+        dof = self.dof
+
+        def A(xs):
+            ys = self.apply(sigs, xs)
+            return [-sc[l] * np.where(dof[l], ys[l], 0.0) for l in range(L + 1)]
+
+        r = [-sc[l] * np.where(dof[l], bs[l], 0.0) for l in range(L + 1)]
+        ndof = sum(int(m.sum()) for m in dof)
+        if self.singular():
+            mu = sum(v.sum() for v in r) / ndof
+            r = [np.where(dof[l], r[l] - mu, 0.0) for l in range(L + 1)]
+        bn = max(np.abs(v).max() for v in r)
+        xs = [np.zeros_like(v) for v in r]
+        if bn == 0.0:
+            return xs, 0, 0
+        dg = self._diag(sigs, sc)
+        z = [np.where(dof[l], r[l] / dg[l], 0.0) for l in range(L + 1)]
+        p = [v.copy() for v in z]
+        rz = sum(np.sum(a * b) for a, b in zip(r, z))
+        for it in range(1, maxiter + 1):
+            q = A(p)
+            alpha = rz / sum(np.sum(a * b) for a, b in zip(p, q))
+            for l in range(L + 1):
+                xs[l] += alpha * p[l]
+                r[l] -= alpha * q[l]
+            if max(np.abs(v).max() for v in r) <= rtol * bn:
+                return xs, it, 0
+            z = [np.where(dof[l], r[l] / dg[l], 0.0) for l in range(L + 1)]
+            rz2 = sum(np.sum(a * b) for a, b in zip(r, z))
+            p = [z[l] + (rz2 / rz) * p[l] for l in range(L + 1)]
+            rz = rz2
+        return xs, maxiter, 1
+
+    def _diag(self, sigs, sc):
+        """diagonal of the symmetric form by probing every level with nodes s
+        apart (their couplings reach one node of their own level)"""
+        L = self.L
+        dg = [np.zeros(self.g[l].node_shape()) for l in range(L + 1)]
+        s = next((q for q in (3, 4, 5, 6, 7, 8)
+                  if all((not self.g[0].periodic[d]) or self.g[0].n[d] % q == 0 for d in range(3))),
+                 None)
+        if s is None:
+            return [np.ones_like(v) for v in dg]
+        zero = [np.zeros_like(v) for v in dg]
+        for l in range(L + 1):
+            for oz in range(s):
+                for oy in range(s):
+                    for ox in range(s):
+                        e = np.zeros(dg[l].shape)
+                        e[oz::s, oy::s, ox::s] = 1.0
+                        e = np.where(self.dof[l], e, 0.0)
+                        xs = list(zero)
+                        xs[l] = e
+                        y = self.apply(sigs, xs)[l]
+                        dg[l] += np.where(e > 0, -sc[l] * y, 0.0)
+        return [np.where(self.dof[l] & (dg[l] > 0), dg[l], 1.0) for l in range(L + 1)]
+
+    def project(self, sigs, us, rtol=1e-12, inflow_vel=None):
+        """composite projection; returns (us_new, phis, gps, its): phi on every
+        node of level 0 and on F_l's closure for l >= 1 (levels' nodes under
+        the next finer level hold its value), G phi and the new velocity on
+        every level's cells in F_l (cells under the next level: the average of
+        their 8 children, applied from the finest level down)"""
+        L = self.L
+        bs = self.divergence(us, inflow_vel)
+        xs, its, ret = self.solve(sigs, bs, rtol=rtol)
+        assert ret == 0, "oracle CG did not converge"
+        ph = self.composite(xs)
+        for l in range(L - 1, -1, -1):
+            f = ph[l + 1][::2, ::2, ::2][:ph[l].shape[0], :ph[l].shape[1], :ph[l].shape[2]]
+            ph[l] = np.where(self.allcov[l], f, ph[l])
+        if self.singular():
+            m = ph[0].mean()
+            ph[0] = ph[0] - m
+            for l in range(1, L + 1):
+                ph[l] = np.where(self.ftype[l] != self.OUT, ph[l] - m, 0.0)
+        gp, un = [], []
+        for l in range(L + 1):
+            gl = nodal_gradient(self.g[l], ph[l]) * self.fmask[l][None]
+            gp.append(gl)
+            un.append(np.where(self.fmask[l][None], us[l] - sigs[l][None] * gl, 0.0))
+        for l in range(L - 1, -1, -1):
+            c = self.cov[l][None]
+            gp[l] = np.where(c, self.average_down(gp[l + 1]), gp[l])
+            un[l] = np.where(c, self.average_down(un[l + 1]), un[l])
+        return un, ph, gp, its

@@ -162,10 +162,23 @@ def test_amr_rejects_bad_levels(ctx):
     with pytest.raises(mgx.MgxError) as e:
         mgx.AmrNodalProjector(ctx, [lev_c, bad])
     assert e.value.code == mgx.MGX_ERR_ARG
+    # a fine box on a non-periodic face: accepted for walls and outflow, refused
+    # once that face is an inflow face
     wall = mgx.make_level((32, 16, 16), [((0, 0, 0), (7, 7, 7))], periodic=(0, 1, 1))
     lev_w = mgx.make_level((16, 8, 8), B.chop_domain((16, 8, 8), 8), periodic=(0, 1, 1))
+    P = mgx.AmrNodalProjector(ctx, [lev_w, wall])
+    sig = [ctx.array(B.field_size(B.chop_domain((16, 8, 8), 8), 0, 1)),
+           ctx.array(B.field_size([((0, 0, 0), (7, 7, 7))], 0, 1))]
+    P.setDomainBC((D, 0, 0), (N, 0, 0))
+    P.set_sigma(sig, 0)
+    P.setDomainBC((INF, 0, 0), (N, 0, 0))
     with pytest.raises(mgx.MgxError) as e:
-        mgx.AmrNodalProjector(ctx, [lev_w, wall])
+        P.set_sigma(sig, 0)
+    assert e.value.code == mgx.MGX_ERR_UNSUPPORTED
+    P.close()
+    # more levels than the library holds
+    with pytest.raises(mgx.MgxError) as e:
+        mgx.AmrNodalProjector(ctx, [lev_c] * 5)
     assert e.value.code == mgx.MGX_ERR_UNSUPPORTED
 
 

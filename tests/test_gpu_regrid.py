@@ -200,3 +200,72 @@ def test_regrid_follows_a_moving_bubble_and_projects(ctx):
     info = P.project([vcd, vfd], 1, 1e-10, 0.0, mode=mgx.MODE_FP64)
     assert info.converged and info.rel_resid <= 1e-10 and info.iters <= 20
     P.close()
+
+
+def test_three_level_regrid_nests_and_projects(ctx):
+    """Two regrids of every level (regrid_levels, max_level 2) over a density
+    bubble: the first makes level 1, the second level 2 from level 1's tags
+    and level 1 again around it.  The hierarchy nests as the composite
+    projection requires (the oracle's construction accepts it), each level's
+    boxes cover the buffered tags of the level below, level 2 holds the
+    conservative interpolation of the new level 1, and the three-level
+    composite projection converges on it."""
+    import oracle
+    from incflo_amd import mgx
+    from incflo_amd import regrid as G
+    from oracle.amr import AmrLevels
+    nc = (32, 32, 32)
+    per = (1, 1, 1)
+    _say("case regrid three levels 32^3")
+    cb = B.chop_domain(nc, 16)
+    lev0 = mgx.make_level(nc, cb)
+    zz, yy, xx = np.meshgrid(*[(np.arange(nc[d]) + 0.5) / nc[d] for d in (2, 1, 0)], indexing="ij")
+    rho0 = (1.0 + 2.0 * (((xx - 0.45) ** 2 + (yy - 0.5) ** 2 + (zz - 0.5) ** 2) < 0.2 ** 2))[None]
+    vel = np.stack([np.sin(2 * np.pi * yy), np.sin(2 * np.pi * zz), np.sin(2 * np.pi * xx)])
+    rd = ctx.array(_boxed_ghosted(R.ghosted(rho0, per), cb, 1, 1))
+    vd = ctx.array(_boxed_ghosted(R.ghosted(vel, per), cb, 1, 3))
+    pars = [G.RegridParams(rhoerr=1.5, n_error_buf=1, blocking_factor=8, max_grid_size=16),
+            G.RegridParams(rhoerr=2.5, n_error_buf=1, blocking_factor=8, max_grid_size=16)]
+    f0 = {"density": (rd, 1, 1, None), "velocity": (vd, 1, 3, None)}
+    levs, flds = G.regrid_levels(ctx, [lev0], [(rd, 1)], pars, [f0], max_level=2)
+    assert len(levs) == 2
+    levs2, flds2 = G.regrid_levels(ctx, levs, [(rd, 1), (flds[1]["density"][0], 1)], pars, flds,
+                                   max_level=2)
+    assert len(levs2) == 3
+    b1, b2 = G.boxes_of(levs2[1]), G.boxes_of(levs2[2])
+    _say(f"  level 1: {len(b1)} boxes, level 2: {len(b2)} boxes")
+    gc = oracle.Geom.make(nc)
+    A = AmrLevels(gc, [[lo + hi for lo, hi in b1], [lo + hi for lo, hi in b2]])
+    t0 = mgx.tag_cells(ctx, lev0, rd, 1, 1.5)
+    R.check_grids(b1, t0, 1, 8, 16, per)
+    t1 = mgx.tag_cells(ctx, levs[1], flds[1]["density"][0], 1, 2.5)
+    assert t1.sum() > 0
+    R.check_grids(b2, t1, 1, 8, 16, per)
+    # level 2 from the new level 1 (no old level 2)
+    s1, s2 = (64, 64, 64), (128, 128, 128)
+    d1 = B.from_boxed(flds2[1]["density"][0].download(), b1, 1, s1)
+    d2 = B.from_boxed(flds2[2]["density"][0].download(), b2, 1, s2)
+    exp2 = R.cons_interp(R.ghosted(d1[None], per), per)[0]
+    m2 = _mask(b2, s2)
+    assert np.abs(d2[m2] - exp2[m2]).max() < 1e-14 * np.abs(exp2[m2]).max()
+    # the three-level projection, the density averaged down level by level
+    mgx.amr_average_down(ctx, levs2[1], levs2[2], flds2[2]["density"][0], 1,
+                         flds2[1]["density"][0], 1)
+    mgx.amr_average_down(ctx, lev0, levs2[1], flds2[1]["density"][0], 1, rd, 1)
+    rc = B.from_boxed(rd.download(), cb, 1, nc[::-1])
+    d1 = B.from_boxed(flds2[1]["density"][0].download(), b1, 1, s1)
+    m1 = _mask(b1, s1)
+    sig = [ctx.array(B.to_boxed(0.01 / rc, cb, 0)),
+           ctx.array(B.to_boxed(np.where(m1, 0.01 / np.where(m1, d1, 1.0), 0.0), b1, 0)),
+           ctx.array(B.to_boxed(np.where(m2, 0.01 / np.where(m2, d2, 1.0), 0.0), b2, 0))]
+    P = mgx.AmrNodalProjector(ctx, levs2)
+    P.set_sigma(sig, 0)
+    v1 = B.from_boxed(flds2[1]["velocity"][0].download(), b1, 1, s1, ncomp=3)
+    v2 = B.from_boxed(flds2[2]["velocity"][0].download(), b2, 1, s2, ncomp=3)
+    vels = [ctx.array(B.to_boxed(vel, cb, 1)), ctx.array(B.to_boxed(v1, b1, 1)),
+            ctx.array(B.to_boxed(v2, b2, 1))]
+    info = P.project(vels, 1, 1e-10, 0.0, mode=mgx.MODE_FP64)
+    _say(f"  three-level projection: {info.iters} composite V-cycles")
+    assert info.converged and info.rel_resid <= 1e-10 and info.iters <= 25
+    P.close()
+    assert A.dof[2].sum() > 0
