@@ -293,6 +293,9 @@ def main():
     ap.add_argument("--cycles-per-update", type=int, default=0,
                     help="FP32 V-cycles per FP64 refinement step in the mixed forms "
                          "(0: the library default)")
+    ap.add_argument("--learned-steps", type=int, default=0,
+                    help="learned form: refinement steps started from the network's "
+                         "correction (0: the library default, the first; -1: every step)")
     ap.add_argument("--cpu-n", type=int, default=96, help="size of the CPU-baseline sample")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (no mgx_nodal_autotune)")
@@ -319,8 +322,9 @@ def main():
     case = Case(ctx, n, seeds[0], args.max_grid_size, args.cache)
     P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), case.bxs))
     P.set_vcycle_bits(args.vcycle_bits)
-    if args.cycles_per_update:
-        P.set_options(mgx.default_opts(cycles_per_update=args.cycles_per_update))
+    if args.cycles_per_update or args.learned_steps:
+        P.set_options(mgx.default_opts(cycles_per_update=args.cycles_per_update,
+                                       learned_steps=args.learned_steps))
     nnodes = n ** 3
 
     modes = {"fp64": mgx.MODE_FP64, "mixed": mgx.MODE_MIXED, "learned": mgx.MODE_LEARNED}

@@ -159,6 +159,7 @@ MGX_API void mgx_default_mlmg_opts(mgx_mlmg_opts* o) {
     o->cycles_per_update = m.cycles_per_update;
     o->bottom_solver = m.bottom_solver;
     o->smoother = m.smoother;
+    o->learned_steps = m.learned_steps;
 }
 
 MGX_API int mgx_ctx_create(int device, void* stream, mgx_ctx* out) {

@@ -26,6 +26,7 @@ struct MGOpts {
     int stall_iters = 3;
     int cycles_per_update = 3;  // most FP32 V-cycles per FP64 refinement step (mixed modes)
     int smoother = MGX_SMOOTHER_DEFAULT;
+    int learned_steps = 1;      // refinement steps started from the network (-1: all)
 };
 
 // Chebyshev sweeps for a hierarchy of FP32 (fp32) or FP64 storage
@@ -74,6 +75,9 @@ inline void to_opts(const mgx_mlmg_opts* o, MGOpts& m) {
     MGX_REQUIRE(o->smoother >= MGX_SMOOTHER_DEFAULT && o->smoother <= MGX_SMOOTHER_CHEBYSHEV,
                 MGX_ERR_ARG, "unknown smoother");
     m.smoother = o->smoother;
+    MGX_REQUIRE(o->learned_steps >= -1 && o->learned_steps <= 100, MGX_ERR_ARG,
+                "learned_steps must lie in -1..100");
+    m.learned_steps = o->learned_steps == 0 ? MGOpts{}.learned_steps : o->learned_steps;
 }
 
 inline KGeom kgeom(const NodalGeom& g) {

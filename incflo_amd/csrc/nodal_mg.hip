@@ -1435,11 +1435,11 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
     int fallback = 0;
     // a non-finite norm (NaN mapped to +inf in the reductions) ends the solve unconverged
     bool conv = std::isfinite(bnorm) && r0 <= tol;
-    if (learned && !conv && opts.maxiter > 0 && !half) {
-        // learned initial guess / coarse correction, judged in FP32 on the
-        // correction equation: kept (as the first refinement step's initial
-        // guess, its V-cycles pre-smooth from it) only if ||r - L e|| < ||r||;
-        // the FP64 residual after that step judges the iterate as always
+    // learned correction of the current FP32 residual, judged in FP32 on the
+    // correction equation: kept (as the refinement step's initial guess, its
+    // V-cycles pre-smooth from it) only if ||r - L e|| < ||r||; the FP64
+    // residual after that step judges the iterate as always
+    auto learned_guess = [&](double rnow) {
         ++vcycles;
         learned_correction(s);
         MGLevel<float>& l0 = H32.L[0];
@@ -1455,10 +1455,15 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         launch_stencil<float, float, KS_RESID>(l0.g, a, s, false);
         const double rl = read_norm(2, s);
         if (opts.verbose)
-            fprintf(stderr, "mgx nodal MLMG: learned step |r - L e| %.6e (from %.6e)\n", rl, rn);
+            fprintf(stderr, "mgx nodal MLMG: learned step |r - L e| %.6e (from %.6e)\n", rl, rnow);
         H32.x0_ready = false;
-        if (rl < rn) H32.x0_guess = true;
-        else fallback = 1;
+        H32.x0_guess = rl < rnow;
+        return rl < rnow;
+    };
+    int lsteps = 0;  // refinement steps started from the network so far
+    if (learned && !conv && opts.maxiter > 0 && !half) {
+        ++lsteps;
+        if (!learned_guess(rn)) fallback = 1;
     } else if (learned && !conv && opts.maxiter > 0) {
         // FP16-stored cycles: the step is an FP64 update of its own
         ++it;
@@ -1485,6 +1490,12 @@ void NodalSolver::solve(double* b, double rtol, double atol, int mode, mgx_solve
         ++it;
         double* pout = other();
         int ran = 1;
+        if (learned && mixed && prec == P32 && it > 1 &&
+            (opts.learned_steps < 0 || lsteps < opts.learned_steps)) {
+            // a further step from the network's correction of its residual
+            ++lsteps;
+            learned_guess(rn);
+        }
         if (prec == P16) vcycle16(s);
         else if (prec == P32) {
             // further FP32 cycles on the same FP64 residual, each from the

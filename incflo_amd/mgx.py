@@ -42,7 +42,7 @@ class MLMGOpts(C.Structure):
                 ("bottom_atol", C.c_double), ("max_coarsening_level", C.c_int),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("omega", C.c_double),
                 ("stall_iters", C.c_int), ("cycles_per_update", C.c_int),
-                ("bottom_solver", C.c_int), ("smoother", C.c_int)]
+                ("bottom_solver", C.c_int), ("smoother", C.c_int), ("learned_steps", C.c_int)]
 
 
 class SolveInfo(C.Structure):

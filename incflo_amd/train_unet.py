@@ -5,7 +5,9 @@
 Pipeline (all GPU work goes through libmgx.so):
   1. seeded variable-density flows at n^3 (rho = 1 + 0.5 tanh(32 Fourier
      modes), k^-5/3 solenoidal velocity, u* = u - dt (u.grad)u), generated on
-     host worker processes;
+     host worker processes: generated at n^3 (the 256^3 single-level case's
+     generator) or, for a fraction --c3-fraction of the flows, generated at
+     (2n)^3 and averaged down to n^3 (the base level of the two-level case);
   2. each flow is projected by the plain FP64 MLMG to 1e-10 relative
      residual; the (rhs, phi) pair is read out at multigrid level `level` as
      the network's normalised input and target (mgx_nodal_training_pair);
@@ -36,9 +38,18 @@ DEFAULT_WEIGHTS = os.path.join(HERE, "data", "unet_b16_l4.npy")
 
 
 def _flow(args):
-    """Worker: one seeded flow, already in the one-box device layout."""
-    n, seed = args
-    rho, ustar, dt = workloads.variable_density_case(n, seed)
+    """Worker: one seeded flow, already in the one-box device layout; style
+    'c3': the (2n)^3 fields averaged down to n^3 before the advection step."""
+    n, seed, style = args
+    if style == "c3":
+        h = 1.0 / n
+        rho = workloads.average_down(workloads.density_field(2 * n, seed))
+        u = workloads.average_down(workloads.spectral_velocity(2 * n, seed))
+        umax = np.max(np.sqrt(np.sum(u * u, axis=0)))
+        dt = 0.5 * h / max(umax, 1e-30)
+        ustar = workloads.advected_velocity(u, dt, h)
+    else:
+        rho, ustar, dt = workloads.variable_density_case(n, seed)
     bxs = [((0, 0, 0), (n - 1, n - 1, n - 1))]
     return seed, B.to_boxed(rho, bxs, 0), B.to_boxed(ustar, bxs, 1), dt
 
@@ -103,11 +114,14 @@ def symmetry(x, t, perm, flips):
 PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 
 
-def make_pairs(pool, ctx, n, level, seeds):
+def make_pairs(pool, ctx, n, level, seeds, c3_fraction=0.0):
     maker = PairMaker(ctx, n, level)
     X, T = [], []
     t0 = time.time()
-    for q, (seed, rho, ustar, dt) in enumerate(pool.imap(_flow, [(n, s) for s in seeds])):
+    # every flow whose running share of c3 flows falls behind the fraction is one
+    jobs = [(n, s, "c3" if int((q + 1) * c3_fraction) > int(q * c3_fraction) else "c2")
+            for q, s in enumerate(seeds)]
+    for q, (seed, rho, ustar, dt) in enumerate(pool.imap(_flow, jobs)):
         x, t = maker.pair(rho, ustar, dt)
         X.append(x)
         T.append(t)
@@ -123,7 +137,7 @@ def train(args):
     with mp.get_context("spawn").Pool(args.workers) as pool:
         ctx = mgx.Context(0)
         _log(f"generating {len(seeds)} pairs at {args.n}^3, level {args.level}")
-        X, T = make_pairs(pool, ctx, args.n, args.level, seeds)
+        X, T = make_pairs(pool, ctx, args.n, args.level, seeds, args.c3_fraction)
     rng = np.random.default_rng(args.seed)
     Xv, Tv = X[args.flows:], T[args.flows:]
     X, T = X[:args.flows], T[:args.flows]
@@ -182,7 +196,7 @@ def train(args):
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.save(args.out, net.get_params())
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
-        json.dump({"n": args.n, "level": args.level, "flows": args.flows,
+        json.dump({"n": args.n, "level": args.level, "flows": args.flows, "c3_fraction": args.c3_fraction,
                    "val_flows": args.val_flows, "base": args.base, "levels": 4,
                    "epochs": args.epochs, "batch": bs, "lr": args.lr, "seed_base": args.seed_base,
                    "history": history}, f, indent=1)
@@ -196,6 +210,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--level", type=int, default=2)
     ap.add_argument("--flows", type=int, default=256)
+    ap.add_argument("--c3-fraction", type=float, default=0.0,
+                    help="share of the flows generated at (2n)^3 and averaged down to n^3")
     ap.add_argument("--val-flows", type=int, default=16)
     ap.add_argument("--seed-base", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=0)

@@ -137,6 +137,13 @@ typedef struct {
     int bottom_solver;         /* MGX_BOTTOM_*, default bicgcg                 */
     int smoother;              /* MGX_SMOOTHER_* (no reference counterpart):
                                   the per-level sweeps of the nodal V-cycle    */
+    int learned_steps;         /* MGX_MODE_LEARNED (no reference counterpart):
+                                  the refinement steps whose FP32 V-cycles
+                                  start from the network's correction of that
+                                  step's residual (kept only when it lowers
+                                  the FP32 residual): 1 the first step only
+                                  (the initial guess; also 0, the default),
+                                  k the first k steps, -1 every step        */
 } mgx_mlmg_opts;
 
 typedef struct {
