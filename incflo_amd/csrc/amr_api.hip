@@ -584,8 +584,6 @@ class AmrNodal {
                     "AMR nodal projector: FP64 and mixed modes only");
         const bool mixed = mode == MGX_MODE_MIXED;
         const KGeom kc = kgeom(gc);
-        const bool wide = amr_tile_width() == 64;
-        for (auto& f : F) f->tiles.wide = wide;
         if (mixed) prepare(H32, h32_sigma, s);
         else prepare(H64, h64_sigma, s);
         MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));

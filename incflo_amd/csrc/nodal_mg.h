@@ -116,11 +116,10 @@ class TileSet {
    public:
     // the block of the point kernels (one plane each), and the stencil tiles:
     // 16 x 16 nodes (narrow: skip more) or 64 x 8 (wide: the dense kernel's
-    // shape), kKZ planes
+    // shape) per the tuning table's amr_tile_width(32), kKZ planes by default
     static constexpr int kTX = 16, kTY = 16, kKZ = 8;
     void build(const NodalGeom& g, const std::vector<unsigned char>& active);
     bool built() const { return nn_[0] > 0; }
-    bool wide = false;  // stencil tiles 64 x 8 instead of 16 x 16
     // stencil tiles of tx x ty nodes x kz planes on a gx x gy x gz tile grid
     // (linear indices; tx a multiple of 16); count 0: none
     const int* tiles(int tx, int ty, int kz, int gx, int gy, int gz, int& count);
@@ -308,8 +307,6 @@ class NodalSolver {
 
 // the process-wide launch-shape table as text, and loading one back
 std::string tuning_table();
-// stencil tile width (16 or 64 nodes) of the launches over a TileSet
-int amr_tile_width();
 void set_tuning_table(const std::string& text);
 
 // Box list of one level, its device box-offset table and tile tables.

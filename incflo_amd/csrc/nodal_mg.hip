@@ -78,13 +78,11 @@ static long coarse_threshold() {
 // at 256^3 the fused pass measured slower than two single sweeps (mixed
 // projection 8.85 against 6.85 ms)
 static int g_jacobi_pairs = 0;
-// stencil tile width of launches over a TileSet (16 or 64 nodes)
+// stencil tiles of launches over a TileSet: width (16 or 64 nodes) for FP64
+// and for FP32 storage, planes per tile (z-run)
 static int g_amr_tile_width = 16;
-
-int amr_tile_width() {
-    std::lock_guard<std::mutex> lk(g_tune_mtx);
-    return g_amr_tile_width;
-}
+static int g_amr_tile_width32 = 64;
+static int g_amr_tile_kz = TileSet::kKZ;
 
 static bool jacobi_pairs_on() {
     std::lock_guard<std::mutex> lk(g_tune_mtx);
@@ -198,10 +196,16 @@ static void launch_stencil_shape(const NodalGeom& g, SArgs<T, TC> a, hipStream_t
 // 8 planes, one workgroup each
 template <class T, class TC, int MODE>
 static void launch_stencil_tiles(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, TileSet* ts) {
-    a.kz = std::min(TileSet::kKZ, g.nn[2]);
+    int width, kz;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        width = sizeof(T) == 4 ? g_amr_tile_width32 : g_amr_tile_width;
+        kz = g_amr_tile_kz;
+    }
+    a.kz = std::min(kz, g.nn[2]);
     a.gz = (g.nn[2] + a.kz - 1) / a.kz;
     int cnt = 0;
-    if (ts->wide) {
+    if (width == 64) {
         // the dense kernel's 64 x 8 tiles (2 nodes per thread)
         a.gx = (g.nn[0] + 63) / 64;
         a.gy = (g.nn[1] + 7) / 8;
@@ -1134,6 +1138,8 @@ std::string tuning_table() {
     o << "coarse_nodes " << g_coarse_nodes << "\n";
     o << "jacobi_pairs " << g_jacobi_pairs << "\n";
     o << "amr_tile_width " << g_amr_tile_width << "\n";
+    o << "amr_tile_width32 " << g_amr_tile_width32 << "\n";
+    o << "amr_tile_kz " << g_amr_tile_kz << "\n";
     return o.str();
 }
 
@@ -1142,7 +1148,7 @@ void set_tuning_table(const std::string& text) {
     std::string tag;
     std::map<ShapeKey, StencilShape> shapes;
     long coarse = 1;
-    int pairs = 0, tilew = 16;
+    int pairs = 0, tilew = 16, tilew32 = 64, tilekz = TileSet::kKZ;
     while (in >> tag) {
         if (tag == "stencil") {
             ShapeKey k{};
@@ -1162,6 +1168,14 @@ void set_tuning_table(const std::string& text) {
             in >> tilew;
             MGX_REQUIRE(!in.fail() && (tilew == 16 || tilew == 64), MGX_ERR_ARG,
                         "tuning table: bad amr_tile_width");
+        } else if (tag == "amr_tile_width32") {
+            in >> tilew32;
+            MGX_REQUIRE(!in.fail() && (tilew32 == 16 || tilew32 == 64), MGX_ERR_ARG,
+                        "tuning table: bad amr_tile_width32");
+        } else if (tag == "amr_tile_kz") {
+            in >> tilekz;
+            MGX_REQUIRE(!in.fail() && tilekz >= 2 && tilekz <= 64, MGX_ERR_ARG,
+                        "tuning table: bad amr_tile_kz");
         } else if (tag == "jacobi_pairs") {
             in >> pairs;
             MGX_REQUIRE(!in.fail() && (pairs == 0 || pairs == 1), MGX_ERR_ARG,
@@ -1175,6 +1189,8 @@ void set_tuning_table(const std::string& text) {
     g_coarse_nodes = coarse;
     g_jacobi_pairs = pairs;
     g_amr_tile_width = tilew;
+    g_amr_tile_width32 = tilew32;
+    g_amr_tile_kz = tilekz;
 }
 
 // ---- learned coarse-level predictor ---------------------------------------

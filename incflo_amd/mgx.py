@@ -935,9 +935,10 @@ class AmrNodalProjector:
 
 
 class AmrMacProjector:
-    """Composite two-level MAC projection (mgx_amr_mac_*): levels[0] tiles the
-    domain, levels[1] holds the fine boxes (refinement ratio 2).  Face fields
-    are passed per level as [x, y, z]."""
+    """Composite MAC projection over AMR levels (mgx_amr_mac_*): levels[0]
+    tiles the domain, each further level holds the boxes of the next finer
+    level (refinement ratio 2, properly nested).  Face fields are passed per
+    level as [x, y, z]."""
 
     def __init__(self, ctx, levels):
         self.ctx = ctx

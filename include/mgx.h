@@ -132,8 +132,8 @@ typedef struct {
                                   step runs only the cycles the measured
                                   contraction says are left.  Honoured by the
                                   single-level nodal and MAC solvers (the
-                                  two-level AMR solvers run one composite
-                                  cycle per step).  0 means the default, 3.  */
+                                  AMR solvers run one composite cycle per
+                                  step).  0 means the default, 3.            */
     int bottom_solver;         /* MGX_BOTTOM_*, default bicgcg                 */
     int smoother;              /* MGX_SMOOTHER_* (no reference counterpart):
                                   the per-level sweeps of the nodal V-cycle    */
@@ -226,8 +226,11 @@ MGX_API int mgx_nodal_set_vcycle_bits(mgx_nodal h, int bits);
  * the chosen shape and its time against the default's. */
 MGX_API int mgx_nodal_autotune(mgx_nodal h, int mode, char* report, int report_len);
 /* The tuning table as text ("stencil nn0 nn1 nn2 word cword pass fine npt
- * target kzmin dma" lines and "coarse_nodes N"), for caching per device model;
- * mgx_tuning_set loads such a text (replacing the table). */
+ * target kzmin dma" lines, "coarse_nodes N", "jacobi_pairs 0|1" and the AMR
+ * fine levels' stencil tiles "amr_tile_width 16|64" (FP64),
+ * "amr_tile_width32 16|64" (FP32) and "amr_tile_kz K" planes), for caching
+ * per device model; mgx_tuning_set loads such a text (replacing the table;
+ * lines left out take their defaults). */
 MGX_API int mgx_tuning_get(char* buf, int len);
 MGX_API int mgx_tuning_set(const char* text);
 /* Variable sigma, a boxed cell field with ng ghosts (:191, sigma = s/rho at :117). */
@@ -345,18 +348,21 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
                                   int gphi_ng, double rtol, double atol, int mode,
                                   mgx_solve_info* info);
 
-/* ---- AMR: composite two-level MAC projection ----------------------------
+/* ---- AMR: composite MAC projection over the levels ----------------------
  * incflo builds one MacProjector over all AMR levels
  * (src/convection/incflo_compute_MAC_projected_velocities.cpp:96-133:
  * initProjector with beta per level, setDomainBC; :272-300: setUMAC with the
  * face velocities of every level, project(rtol, atol) or project(phi, rtol,
  * atol)), and the projector averages the fine faces down (:301-302).  Level
- * rules as for mgx_amr_nodal_create.  The composite problem: fine cells in the
- * fine region and uncovered coarse cells are the unknowns; every face between
- * two of them carries beta (phi_j - phi_i) / (distance of the cell centres),
- * 1.5 h_f across a fine face on the coarse-fine boundary; a coarse cell next
- * to the fine region takes the 4 fine fluxes on each such face (reflux).
- * Boundary types: Dirichlet, Neumann or inflow (no Robin). */
+ * rules as for mgx_amr_nodal_create (up to MGX_AMR_MAX_LEVELS, properly
+ * nested: every cell around a cell under a finer level belongs to its own
+ * level), except that fine boxes may touch every kind of non-periodic face.
+ * The composite problem: each level's cells not under a finer level are the
+ * unknowns; every face between two of them carries beta (phi_j - phi_i) /
+ * (distance of the cell centres), 1.5 h across a face on a level's boundary
+ * to the cell of the level below; that cell takes the 4 finer fluxes on each
+ * such face (reflux).  Boundary types: Dirichlet, Neumann or inflow (no
+ * Robin). */
 typedef struct mgx_amr_mac_s* mgx_amr_mac;
 MGX_API int mgx_amr_mac_create(mgx_ctx ctx, const mgx_level_desc* levels, int nlevels,
                                mgx_amr_mac* out);
@@ -370,10 +376,10 @@ MGX_API int mgx_amr_mac_set_options(mgx_amr_mac h, const mgx_mlmg_opts* opts);
  * is the boxed face field of level lev, direction d (ng ghosts). */
 MGX_API int mgx_amr_mac_set_beta(mgx_amr_mac h, const double* const* beta, int ng);
 /* setUMAC(umac) + project(phi, rtol, atol) (:272-299): umac[3 lev + d] (boxed
- * faces, umac_ng ghosts) is overwritten with the projected velocity, coarse
- * faces under the fine level with the mean of the fine faces on them;
- * phi[lev] (boxed cells, may be NULL) receives phi, covered coarse cells the
- * mean of their 8 fine cells.  MGX_MODE_FP64 or MGX_MODE_MIXED. */
+ * faces, umac_ng ghosts) is overwritten with the projected velocity, faces
+ * under a finer level with the mean of its faces on them (from the finest
+ * level down); phi[lev] (boxed cells, may be NULL) receives phi, cells under
+ * a finer level the mean of its 8 cells.  MGX_MODE_FP64 or MGX_MODE_MIXED. */
 MGX_API int mgx_amr_mac_project(mgx_amr_mac h, double* const* umac, int umac_ng,
                                 double* const* phi, int phi_ng, double rtol, double atol,
                                 int mode, mgx_solve_info* info);

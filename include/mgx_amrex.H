@@ -313,14 +313,14 @@ class NodalProjector {
     detail::DevField phis_[MGX_AMR_MAX_LEVELS], gphis_[MGX_AMR_MAX_LEVELS];
 };
 
-// Hydro::MacProjector (no EB) over one level, or two (refinement ratio 2: the
-// composite projection of mgx_amr_mac_*, src/convection/
-// incflo_compute_MAC_projected_velocities.cpp:96-133, 272-302).
+// Hydro::MacProjector (no EB) over one level, or up to MGX_AMR_MAX_LEVELS
+// (refinement ratio 2: the composite projection of mgx_amr_mac_*,
+// src/convection/incflo_compute_MAC_projected_velocities.cpp:96-133, 272-302).
 class MacProjector {
    public:
     MacProjector(Context& ctx, std::vector<Geometry> geom) : ctx_(ctx), geom_(geom) {
-        if (geom_.size() != 1 && geom_.size() != 2)
-            Abort("MacProjector: one level, or two with refinement ratio 2");
+        if (geom_.empty() || geom_.size() > MGX_AMR_MAX_LEVELS)
+            Abort("MacProjector: one to MGX_AMR_MAX_LEVELS levels (refinement ratio 2)");
     }
     explicit MacProjector(std::vector<Geometry> geom) : MacProjector(Context::current(), geom) {}
     ~MacProjector() {
@@ -333,8 +333,10 @@ class MacProjector {
     bool needInitialization() const { return h_ == nullptr && ha_ == nullptr; }
     // initProjector(lp_info, beta) (:112): the BoxArrays are the coefficients'
     void initProjector(LPInfo info, std::vector<std::array<MultiFab const*, 3>> beta) {
-        if (beta.size() == 2 && geom_.size() == 2) {
-            create_amr(*beta[0][0]->ba, *beta[1][0]->ba, info);
+        if (beta.size() >= 2 && beta.size() == geom_.size()) {
+            std::vector<const BoxArray*> bas;
+            for (auto& b : beta) bas.push_back(b[0]->ba);
+            create_amr(bas, info);
             updateCoeffs(beta);
             return;
         }
@@ -359,7 +361,7 @@ class MacProjector {
             r[k] = (int)hi[k];
         }
         if (ha_) {
-            // the two-level handle re-sets up its levels: hand beta over again
+            // the AMR handle re-sets up its levels: hand beta over again
             ctx_.check(mgx_amr_mac_set_domain_bc(ha_, l, r), "setDomainBC");
             if (!beta_.empty()) updateCoeffs(beta_);
             return;
@@ -376,17 +378,21 @@ class MacProjector {
                    "setLevelBC");
     }
     void updateBeta(double beta) {
-        if (ha_) Abort("MacProjector::updateBeta: two levels need beta fields");
+        if (ha_) Abort("MacProjector::updateBeta: AMR levels need beta fields");
         ctx_.check(mgx_mac_set_const_beta(need(), beta), "updateBeta");
     }
     void updateCoeffs(std::vector<std::array<MultiFab const*, 3>> beta) {
         if (ha_) {
-            if (beta.size() != 2 || beta[0][0]->ngrow != beta[1][0]->ngrow)
-                Abort("MacProjector: two levels need beta fields of equal ghost width");
+            if (beta.size() != geom_.size()) Abort("MacProjector: a beta field per level");
             beta_ = beta;
-            const double* b[6] = {beta[0][0]->p, beta[0][1]->p, beta[0][2]->p,
-                                  beta[1][0]->p, beta[1][1]->p, beta[1][2]->p};
-            ctx_.check(mgx_amr_mac_set_beta(ha_, b, beta[0][0]->ngrow), "updateCoeffs");
+            std::vector<const double*> b;
+            for (auto& lv : beta)
+                for (int d = 0; d < 3; ++d) {
+                    if (lv[d]->ngrow != beta[0][0]->ngrow)
+                        Abort("MacProjector: beta fields of equal ghost width");
+                    b.push_back(lv[d]->p);
+                }
+            ctx_.check(mgx_amr_mac_set_beta(ha_, b.data(), beta[0][0]->ngrow), "updateCoeffs");
             return;
         }
         detail::one_level(beta.size(), "MacProjector");
@@ -396,9 +402,8 @@ class MacProjector {
     }
     void setUMAC(std::vector<std::array<MultiFab*, 3>> umac) {
         if (ha_) {
-            if (umac.size() != 2) Abort("MacProjector::setUMAC: a face velocity per level");
-            umac_ = umac[0];
-            umac1_ = umac[1];
+            if (umac.size() != geom_.size()) Abort("MacProjector::setUMAC: a face velocity per level");
+            umacs_ = umac;
             return;
         }
         detail::one_level(umac.size(), "MacProjector");
@@ -408,10 +413,13 @@ class MacProjector {
     void project(double rtol, double atol) { run(nullptr, 0, rtol, atol); }
     void project(std::vector<MultiFab*> phi, double rtol, double atol) {
         if (ha_) {
-            if (phi.size() != 2 || phi[0]->ngrow != phi[1]->ngrow)
-                Abort("MacProjector::project: phi per level, equal ghost widths");
-            double* ph[2] = {phi[0]->p, phi[1]->p};
-            run_amr(ph, phi[0]->ngrow, rtol, atol);
+            if (phi.size() != geom_.size()) Abort("MacProjector::project: phi per level");
+            std::vector<double*> ph;
+            for (MultiFab* m : phi) {
+                if (m->ngrow != phi[0]->ngrow) Abort("MacProjector::project: phi ghost widths differ");
+                ph.push_back(m->p);
+            }
+            run_amr(ph.data(), phi[0]->ngrow, rtol, atol);
             return;
         }
         detail::one_level(phi.size(), "MacProjector");
@@ -442,12 +450,15 @@ class MacProjector {
         o.max_coarsening_level = info.max_coarsening_level;
         ctx_.check(mgx_mac_set_options(h_, &o), "set options");
     }
-    void create_amr(const BoxArray& ba0, const BoxArray& ba1, const LPInfo& info) {
+    void create_amr(const std::vector<const BoxArray*>& bas, const LPInfo& info) {
         if (ha_) mgx_amr_mac_destroy(ha_);
-        flat_ = ba0.flat();
-        flat1_ = ba1.flat();
-        mgx_level_desc d[2] = {detail::level_desc(geom_[0], flat_), detail::level_desc(geom_[1], flat1_)};
-        ctx_.check(mgx_amr_mac_create(ctx_.get(), d, 2, &ha_), "MacProjector");
+        flats_.resize(bas.size());
+        std::vector<mgx_level_desc> d(bas.size());
+        for (size_t l = 0; l < bas.size(); ++l) {
+            flats_[l] = bas[l]->flat();
+            d[l] = detail::level_desc(geom_[l], flats_[l]);
+        }
+        ctx_.check(mgx_amr_mac_create(ctx_.get(), d.data(), (int)d.size(), &ha_), "MacProjector");
         mgx_mlmg_opts o;
         mgx_default_mlmg_opts(&o);
         o.maxiter = 200;
@@ -456,10 +467,16 @@ class MacProjector {
         ctx_.check(mgx_amr_mac_set_options(ha_, &o), "set options");
     }
     void run_amr(double* const* phi, int phi_ng, double rtol, double atol) {
-        if (!umac_[0] || !umac1_[0]) Abort("MacProjector::project: setUMAC has not been called");
-        if (umac_[0]->ngrow != umac1_[0]->ngrow) Abort("MacProjector: face velocity ghost widths differ");
-        double* u[6] = {umac_[0]->p, umac_[1]->p, umac_[2]->p, umac1_[0]->p, umac1_[1]->p, umac1_[2]->p};
-        ctx_.check(mgx_amr_mac_project(ha_, u, umac_[0]->ngrow, phi, phi_ng, rtol, atol, mode_, &info_),
+        if (umacs_.size() != geom_.size()) Abort("MacProjector::project: setUMAC has not been called");
+        std::vector<double*> u;
+        for (auto& lv : umacs_)
+            for (int d = 0; d < 3; ++d) {
+                if (!lv[d] || lv[d]->ngrow != umacs_[0][0]->ngrow)
+                    Abort("MacProjector: face velocities missing or of unequal ghost widths");
+                u.push_back(lv[d]->p);
+            }
+        ctx_.check(mgx_amr_mac_project(ha_, u.data(), umacs_[0][0]->ngrow, phi, phi_ng, rtol, atol, mode_,
+                                       &info_),
                    "MacProjector::project");
     }
     mgx_mac need() const {
@@ -478,12 +495,13 @@ class MacProjector {
     }
     Context& ctx_;
     std::vector<Geometry> geom_;
-    std::vector<int> flat_, flat1_;
+    std::vector<int> flat_;
+    std::vector<std::vector<int>> flats_;
     mgx_mac h_ = nullptr;
-    mgx_amr_mac ha_ = nullptr;  // two levels
+    mgx_amr_mac ha_ = nullptr;  // AMR levels
     std::vector<std::array<MultiFab const*, 3>> beta_;
     std::array<MultiFab*, 3> umac_{nullptr, nullptr, nullptr};
-    std::array<MultiFab*, 3> umac1_{nullptr, nullptr, nullptr};
+    std::vector<std::array<MultiFab*, 3>> umacs_;  // AMR levels
     mgx_solve_info info_{};
     int mode_ = MGX_MODE_FP64;
 };

@@ -250,3 +250,229 @@ class MacAmr:
         ucn = self.average_down(ufn, ucn)
         phif, phic = self.levels(x)
         return ucn, ufn, phif, phic, x, div
+
+
+class MacAmrLevels:
+    """The composite MAC problem over levels 0..L (ratio 2 between
+    neighbours), MacAmr's construction applied to each pair of neighbouring
+    levels: level l's unknowns are its cells in F_l not covered by F_{l+1}
+    (level 0 covers the domain); a face between two unknowns of one level
+    carries beta_l (phi_j - phi_i) / h_l; a level-l face on the boundary of
+    F_l links its cell to the level-(l-1) cell behind it at distance 1.5 h_l
+    (so a level-(l-1) cell next to F_l sees the 4 finer faces: reflux); a
+    domain face of any level is Dirichlet (value 0 at h_l / 2) or Neumann.
+    F_{l+1} must be properly nested in F_l (the cell behind every coarse-fine
+    face is an unknown of the level below).  Right-hand side: V div u per
+    unknown with the faces under a finer level replaced by the mean of its
+    faces (from the finest level down); the projection subtracts each link's
+    flux from its face (the finer face for coarse-fine links) and averages
+    the faces down again.  Fine levels may touch Dirichlet and Neumann faces.
+
+    g: level-0 oracle.Geom; Fs[l - 1]: level l's cell mask, shape
+    (2^l nz, 2^l ny, 2^l nx)."""
+
+    def __init__(self, g, Fs):
+        self.g = g
+        self.L = L = len(Fs)
+        self.n = [tuple(g.n[d] << l for d in range(3)) for l in range(L + 1)]
+        self.h = [[g.dx[d] / 2 ** l for d in range(3)] for l in range(L + 1)]
+        self.per = [bool(g.periodic[d]) for d in range(3)]
+        self.F = [np.ones((g.n[2], g.n[1], g.n[0]), bool)] + [np.asarray(F, bool) for F in Fs]
+        self.cov = []
+        for l in range(L + 1):
+            nx, ny, nz = self.n[l]
+            if l < L:
+                Fc = self.F[l + 1].reshape(nz, 2, ny, 2, nx, 2)
+                c = Fc.any(axis=(1, 3, 5))
+                if not np.array_equal(c, Fc.all(axis=(1, 3, 5))):
+                    raise ValueError("a fine region must consist of whole cells of the level below")
+                if (c & ~self.F[l]).any():
+                    raise ValueError("level %d reaches outside level %d" % (l + 1, l))
+            else:
+                c = np.zeros((nz, ny, nx), bool)
+            self.cov.append(c)
+        self.unk = [self.F[l] & ~self.cov[l] for l in range(L + 1)]
+        self.uid, off = [], 0
+        for l in range(L + 1):
+            ids = -np.ones(self.unk[l].shape, int)
+            cnt = int(self.unk[l].sum())
+            ids[self.unk[l]] = off + np.arange(cnt)
+            self.uid.append(ids)
+            off += cnt
+        self.N = off
+        self.vol = np.concatenate([np.full(int(self.unk[l].sum()), np.prod(self.h[l]))
+                                   for l in range(L + 1)])
+        self.singular = not any(
+            not self.per[d] and (g.bclo[d] == BC_DIRICHLET or g.bchi[d] == BC_DIRICHLET)
+            for d in range(3))
+        self.links, self.dirichlet = self._links()
+
+    @staticmethod
+    def _area(h, d):
+        return h[(d + 1) % 3] * h[(d + 2) % 3]
+
+    def _face(self, l, d, pos):
+        """(level, d, k, j, i) of a level-l face; periodic face n_d is face 0"""
+        pos = list(pos)
+        if self.per[d] and pos[d] == self.n[l][d]:
+            pos[d] = 0
+        return (l, d, pos[2], pos[1], pos[0])
+
+    def _links(self):
+        """links (a, b, area/distance, face) with a on the low side of the
+        face; Dirichlet faces (a, area/distance, face, side)"""
+        links, dirichlet = [], []
+        for l in range(self.L + 1):
+            n, h = self.n[l], self.h[l]
+            for d in range(3):
+                ar = self._area(h, d)
+                for (k, j, i) in np.argwhere(self.unk[l]):
+                    p = [i, j, k]
+                    a = self.uid[l][k, j, i]
+                    for s in (-1, 1):
+                        q = list(p)
+                        q[d] += s
+                        fpos = list(p)
+                        if s > 0:
+                            fpos[d] += 1
+                        face = self._face(l, d, fpos)
+                        if q[d] < 0 or q[d] >= n[d]:
+                            if not self.per[d]:
+                                bc = self.g.bclo[d] if s < 0 else self.g.bchi[d]
+                                if bc == BC_DIRICHLET:
+                                    dirichlet.append((a, ar / (0.5 * h[d]), face, s))
+                                continue
+                            q[d] %= n[d]
+                        if self.F[l][q[2], q[1], q[0]]:
+                            b = self.uid[l][q[2], q[1], q[0]]
+                            if b >= 0 and s > 0:
+                                links.append((a, b, ar / h[d], face))
+                            continue  # b < 0: covered, the finer level links to a
+                        c = self.uid[l - 1][q[2] // 2, q[1] // 2, q[0] // 2]
+                        if c < 0:
+                            raise ValueError("level %d is not properly nested in level %d" % (l, l - 1))
+                        lo, hi = (a, c) if s > 0 else (c, a)
+                        links.append((lo, hi, ar / (1.5 * h[d]), face))
+        return links, dirichlet
+
+    @staticmethod
+    def _beta(betas, face):
+        l, d, k, j, i = face
+        return betas[l][d][k, j, i]
+
+    def matrix(self, betas):
+        """volume-weighted composite operator A (sparse, symmetric); betas[l][d]
+        level l's face coefficients"""
+        rows, cols, vals = [], [], []
+        diag = np.zeros(self.N)
+        for (a, b, gf, face) in self.links:
+            T = self._beta(betas, face) * gf
+            rows += [a, b]
+            cols += [b, a]
+            vals += [T, T]
+            diag[a] -= T
+            diag[b] -= T
+        for (a, gf, face, s) in self.dirichlet:
+            diag[a] -= self._beta(betas, face) * gf
+        rows += list(range(self.N))
+        cols += list(range(self.N))
+        vals += list(diag)
+        return sp.csr_matrix((vals, (rows, cols)), shape=(self.N, self.N))
+
+    def _sync_periodic(self, u, n):
+        for d in range(3):
+            if self.per[d]:
+                ax = 2 - d
+                sl0 = [slice(None)] * 3
+                sln = [slice(None)] * 3
+                sl0[ax] = 0
+                sln[ax] = n[d]
+                u[d][tuple(sln)] = u[d][tuple(sl0)]
+        return u
+
+    def _under(self, l, d):
+        ax = 2 - d
+        covp = np.pad(self.cov[l], [(1, 1) if a == ax else (0, 0) for a in range(3)],
+                      mode="wrap" if self.per[d] else "constant")
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        return covp[tuple(lo)] | covp[tuple(hi)]
+
+    def average_down(self, us):
+        """faces of every level under the next finer one <- the mean of its 4
+        faces on them, from the finest level down"""
+        out = [[np.array(a, dtype=np.float64) for a in u] for u in us]
+        for l in range(self.L - 1, -1, -1):
+            for d in range(3):
+                ax = 2 - d
+                tang = [a for a in range(3) if a != ax]
+                sl = [slice(None)] * 3
+                sl[ax] = slice(0, None, 2)
+                fe = out[l + 1][d][tuple(sl)]
+                s = 0.0
+                for t0 in (0, 1):
+                    for t1 in (0, 1):
+                        idx = [slice(None)] * 3
+                        idx[tang[0]] = slice(t0, None, 2)
+                        idx[tang[1]] = slice(t1, None, 2)
+                        s = s + fe[tuple(idx)]
+                out[l][d] = np.where(self._under(l, d), 0.25 * s, out[l][d])
+        return out
+
+    def divergence(self, us):
+        """composite divergence per unknown (level 0 first)"""
+        ua = self.average_down(us)
+        parts = []
+        for l in range(self.L + 1):
+            dv = sum(np.diff(ua[l][d], axis=2 - d) / self.h[l][d] for d in range(3))
+            parts.append(dv[self.unk[l]])
+        return np.concatenate(parts)
+
+    def unknowns(self, phis):
+        return np.concatenate([phis[l][self.unk[l]] for l in range(self.L + 1)])
+
+    def levels(self, x):
+        """phi per level from the unknowns; cells under a finer level take the
+        mean of its 8 cells (from the finest level down), cells outside F 0"""
+        out = []
+        for l in range(self.L + 1):
+            p = np.zeros(self.unk[l].shape)
+            p[self.unk[l]] = x[self.uid[l][self.unk[l]]]
+            out.append(p)
+        for l in range(self.L - 1, -1, -1):
+            f = out[l + 1]
+            m = sum(f[a::2, b::2, c::2] for a in (0, 1) for b in (0, 1) for c in (0, 1)) / 8.0
+            out[l] = np.where(self.cov[l], m, out[l])
+        return out
+
+    def apply(self, betas, phis):
+        """composite Laplacian (A / V) per unknown"""
+        return self.matrix(betas) @ self.unknowns(phis) / self.vol
+
+    def project(self, betas, us):
+        """returns (us_new, phis, x, div) with div the composite divergence
+        before the projection"""
+        A = self.matrix(betas)
+        div = self.divergence(us)
+        rhs = self.vol * div
+        if self.singular:
+            rhs = rhs - self.vol * (rhs.sum() / self.vol.sum())
+            K = sp.bmat([[A, sp.csr_matrix(self.vol[:, None])],
+                         [sp.csr_matrix(self.vol[None, :]), None]]).tocsc()
+            x = spla.spsolve(K, np.concatenate([rhs, [0.0]]))[:self.N]
+        else:
+            x = spla.spsolve(A.tocsc(), rhs)
+        un = [[np.array(a, dtype=np.float64) for a in u] for u in us]
+        for (a, b, gf, face) in self.links:
+            l, d, k, j, i = face
+            un[l][d][k, j, i] -= self._beta(betas, face) * gf / self._area(self.h[l], d) * (x[b] - x[a])
+        for (a, gf, face, s) in self.dirichlet:
+            l, d, k, j, i = face
+            g_ = self._beta(betas, face) * gf / self._area(self.h[l], d)
+            un[l][d][k, j, i] -= g_ * ((0.0 - x[a]) if s > 0 else (x[a] - 0.0))
+        for l in range(self.L + 1):
+            self._sync_periodic(un[l], self.n[l])
+        un = self.average_down(un)
+        return un, self.levels(x), x, div

@@ -150,10 +150,9 @@ def test_amr_mac_errors(ctx):
     nc = (8, 8, 8)
     cboxes = B.chop_domain(nc, 8)
     lev_c = mgx.make_level(nc, cboxes, periodic=(0, 1, 1))
-    # a fine box on the non-periodic x face
-    lev_f = mgx.make_level((16, 16, 16), [((0, 0, 0), (3, 3, 3))], periodic=(0, 1, 1))
+    # more levels than the library holds
     with pytest.raises(mgx.MgxError):
-        mgx.AmrMacProjector(ctx, [lev_c, lev_f])
+        mgx.AmrMacProjector(ctx, [lev_c] * 5)
     # unaligned fine box
     lev_f = mgx.make_level((16, 16, 16), [((3, 2, 2), (8, 5, 5))], periodic=(0, 1, 1))
     with pytest.raises(mgx.MgxError):

@@ -1,6 +1,6 @@
 """The committed trained network (incflo_amd/data/unet_b16_l4.npy) in the
 learned + mixed nodal projection, on seeded variable-density flows that are
-not in its training set (training used seeds 1000-1271; these are 1-3).
+not in its training set (training used seeds 1000-3031; these are 1-3).
 
 For each case the same projector runs MODE_FP64, MODE_MIXED and MODE_LEARNED
 on the same input.  The learned form must converge to the same relative
