@@ -81,9 +81,10 @@ struct AmrFine {
     bool finer = false;                 // a still finer level lies above
     DevBuf ftype;                       // AMR_OUT / AMR_DOF / AMR_BDRY against the level below
     DevBuf dof;                         // composite unknowns: AMR_DOF nodes not surrounded by covered cells
-    DevBuf ccov, cnear;                 // (finer) cells under the finer level; nodes near its closure
+    DevBuf ccov, rw;                    // (finer) cells under the finer level; restriction of its values (FP64-sized)
     DevBuf sig, sig_out, sig32, sig_out32;  // full / covered cells 0 (sig_out only when finer)
     DevBuf x, b, y, rT, ea, eb, yT, bC, rtmp;
+    DevBuf wm;                          // its dF values for the restriction onto the level below (FP64, 0 elsewhere)
     TileSet tiles;
     const int4* blocks(int& n) { return tiles.blocks(n); }
     template <class T> const T* sigT() const {
@@ -117,7 +118,7 @@ class AmrNodal {
     DevBuf bC;                          // level-0 rhs of the coarse correction (FP64-sized)
     DevBuf sigc, sigc_out, sigc_out32;
     DevBuf cdof, ccov;
-    DevBuf cnear;                       // level-0 nodes with a node of level 1's closure among their 27 level-1 nodes
+    DevBuf rw0;                         // restriction of level 1's values onto level 0 (FP64-sized)
     DevBuf xc, bc, rtmp, rcT;
     DevBuf scal, part, ibuf;
     std::vector<std::unique_ptr<AmrFine>> F;  // F[l - 1]: level l
@@ -126,6 +127,7 @@ class AmrNodal {
     double ndof = 0.0;
     bool sigma_set = false;
     bool h64_sigma = false, h32_sigma = false;
+    int last_mixed = -1;                // precision of the last solve's V-cycle buffers
 
     ~AmrNodal() {
         if (h_pinned) (void)hipHostFree(h_pinned);
@@ -201,33 +203,8 @@ class AmrNodal {
                 MGX_HIP(hipMemcpy(cv.p, cov[l].data(), cov[l].size(), hipMemcpyHostToDevice));
             }
             if (l == L) continue;
-            // the nodes whose full-weighting stencil reaches a node of the
-            // finer level's closure: the only ones the restrictions of its
-            // data touch
-            const NodalGeom& gf = G[l + 1];
-            std::vector<unsigned char> nr(g.nnode(), 0);
-            for (int K = 0; K < g.nn[2]; ++K)
-                for (int J = 0; J < g.nn[1]; ++J)
-                    for (int I = 0; I < g.nn[0]; ++I) {
-                        bool any = false;
-                        for (int oz = -1; oz <= 1 && !any; ++oz)
-                            for (int oy = -1; oy <= 1 && !any; ++oy)
-                                for (int ox = -1; ox <= 1 && !any; ++ox) {
-                                    int f[3] = {2 * I + ox, 2 * J + oy, 2 * K + oz};
-                                    bool inb = true;
-                                    for (int d2 = 0; d2 < 3; ++d2) {
-                                        if (gf.per[d2]) f[d2] = (f[d2] + gf.nn[d2]) % gf.nn[d2];
-                                        else if (f[d2] < 0 || f[d2] >= gf.nn[d2]) inb = false;
-                                    }
-                                    if (inb)
-                                        any = ft[l + 1][f[0] + (size_t)gf.nn[0] * (f[1] + (size_t)gf.nn[1] * f[2])] !=
-                                              AMR_OUT;
-                                }
-                        nr[I + (size_t)g.nn[0] * (J + (size_t)g.nn[1] * K)] = any;
-                    }
-            DevBuf& nb = l == 0 ? cnear : lev(l).cnear;
-            nb.alloc(nr.size());
-            MGX_HIP(hipMemcpy(nb.p, nr.data(), nr.size(), hipMemcpyHostToDevice));
+            DevBuf& rb = l == 0 ? rw0 : lev(l).rw;
+            rb.alloc(g.nnode() * sizeof(double));
         }
         ndof = (double)count;
         const long nnc = gc.nnode();
@@ -264,7 +241,7 @@ class AmrNodal {
             // the launches visit only the tiles that hold nodes of the level's
             // closure; what lies elsewhere in these arrays is never read but is
             // multiplied by the zero sigma outside, so it starts finite (0)
-            std::vector<DevBuf*> bufs = {&f.x, &f.b, &f.y, &f.rT, &f.ea, &f.eb, &f.yT};
+            std::vector<DevBuf*> bufs = {&f.x, &f.b, &f.y, &f.rT, &f.ea, &f.eb, &f.yT, &f.wm};
             if (f.finer) { bufs.push_back(&f.bC); bufs.push_back(&f.rtmp); }
             for (DevBuf* b : bufs) {
                 b->alloc(nn * sizeof(double));
@@ -386,8 +363,8 @@ class AmrNodal {
                 if (nbk)
                     amr_fine_resid_kernel<TO><<<nbk, tile_block(), 0, s>>>(
                         kf, f.ftype.as<unsigned char>(), f.b.as<double>(), f.y.as<double>(), f.rT.as<TO>(),
-                        scal.as<unsigned long long>() + 1, tl);
-                yB = f.y.as<double>();
+                        scal.as<unsigned long long>() + 1, tl, f.wm.as<double>());
+                yB = f.wm.as<double>();  // L x on dF, 0 elsewhere
                 sB = -1.0;
             } else {
                 AmrFine& u = lev(l + 1);
@@ -399,13 +376,14 @@ class AmrNodal {
                 a.sig = f.sig_out.as<double>();
                 a.rout = f.rtmp.as<double>();
                 launch_stencil<double, double, KS_RESID>(f.g, a, s, false, &f.tiles);
+                restrict_full<double>(u.g, f.g, yB, f.rw.as<double>(), s);
                 if (nbk)
                     amr_coarse_combine_kernel<TO><<<nbk, tile_block(), 0, s>>>(
-                        kgeom(u.g), kf, u.ftype.as<unsigned char>(), f.dof.as<unsigned char>(),
-                        f.rtmp.as<double>(), yB, sB, f.rT.as<TO>(), scal.as<unsigned long long>() + 1,
-                        f.cnear.as<unsigned char>(), tl);
-                // level l's y on dF_l is -(b - L x) there (b is 0 off the unknowns)
-                yB = f.rtmp.as<double>();
+                        kf, f.dof.as<unsigned char>(), f.rtmp.as<double>(), f.rw.as<double>(), sB,
+                        f.rT.as<TO>(), scal.as<unsigned long long>() + 1, tl, f.ftype.as<unsigned char>(),
+                        f.wm.as<double>());
+                // level l's L x on dF_l is -(b - L x) there (b is 0 off the unknowns)
+                yB = f.wm.as<double>();
                 sB = 1.0;
             }
         }
@@ -418,15 +396,10 @@ class AmrNodal {
         c.sig = sigc_out.as<double>();
         c.rout = rtmp.as<double>();
         launch_stencil<double, double, KS_RESID>(gc, c, s, false);
-        if (L() > 0)
-            amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-                kgeom(lev(1).g), kc, lev(1).ftype.as<unsigned char>(), cdof.as<unsigned char>(),
-                rtmp.as<double>(), yB, sB, rcT.as<TO>(), scal.as<unsigned long long>() + 1,
-                cnear.as<unsigned char>());
-        else
-            amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-                kc, kc, nullptr, cdof.as<unsigned char>(), rtmp.as<double>(), nullptr, 0.0, rcT.as<TO>(),
-                scal.as<unsigned long long>() + 1, nullptr);
+        if (L() > 0) restrict_full<double>(lev(1).g, gc, yB, rw0.as<double>(), s);
+        amr_coarse_combine_kernel<TO><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
+            kc, cdof.as<unsigned char>(), rtmp.as<double>(), L() > 0 ? rw0.as<double>() : nullptr, sB,
+            rcT.as<TO>(), scal.as<unsigned long long>() + 1);
         return read_norm(1, s);
     }
 
@@ -446,6 +419,8 @@ class AmrNodal {
         a.c = make_coef<T>(f.g.dx);
         a.b = rhs;
         a.sig = f.sigT<T>();
+        // weighted Jacobi (the Chebyshev factors of the level-0 hierarchy
+        // measured one composite cycle more at the two-level 128^3 / 256^3 case)
         a.omega = (T)nodal_omega(f.g, opts.omega);
         a.nmask = ft;
         int nbk = 0;
@@ -462,20 +437,25 @@ class AmrNodal {
         } else {
             zero_kernel<T><<<grid1d(f.g.nnode()), 256, 0, s>>>(e, f.g.nnode());
         }
+        // rhs - L e on the level's closure (0 from rhs on dF, and from L e
+        // away from the closure), restricted to the level below
         SArgs<T, T> ap{};
         ap.g = kf;
         ap.c = a.c;
         ap.x = e;
+        ap.b = rhs;
         ap.sig = f.sigT<T>();
         ap.rout = f.yT.as<T>();
-        launch_stencil<T, T, KS_APPLY>(f.g, ap, s, false, &f.tiles);
+        launch_stencil<T, T, KS_RESID>(f.g, ap, s, false, &f.tiles);
         T* ec;
         KGeom kcl;
+        T* w = l == 1 ? rw0.as<T>() : lev(l - 1).rw.as<T>();
+        const NodalGeom& gcl = l == 1 ? gc : lev(l - 1).g;
+        restrict_full<T>(f.g, gcl, f.yT.as<T>(), w, s);
         if (l == 1) {
             kcl = kgeom(gc);
             amr_coarse_rhs_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-                kf, kcl, ft, cdof.as<unsigned char>(), rhs, f.yT.as<T>(), rcT.as<T>(), H.L[0].b,
-                cnear.as<unsigned char>());
+                kcl, cdof.as<unsigned char>(), rcT.as<T>(), w, H.L[0].b);
             H.vcycle(opts, s, ibuf.as<int>());
             ec = H.L[0].x;
         } else {
@@ -485,8 +465,7 @@ class AmrNodal {
             const int4* tlc = c.blocks(nbc);
             if (nbc)
                 amr_coarse_rhs_kernel<T><<<nbc, tile_block(), 0, s>>>(
-                    kf, kcl, ft, c.dof.as<unsigned char>(), rhs, f.yT.as<T>(), c.rT.as<T>(), c.bC.as<T>(),
-                    c.cnear.as<unsigned char>(), tlc);
+                    kcl, c.dof.as<unsigned char>(), c.rT.as<T>(), w, c.bC.as<T>(), tlc);
             cycle_level<T>(H, l - 1, c.bC.as<T>(), s);
             ec = (T*)ecur[l - 1];
         }
@@ -502,7 +481,11 @@ class AmrNodal {
             // sweep with the composite operator (e on dF follows the
             // correction below)
             ap.x = e;
+            ap.b = nullptr;
             launch_stencil<T, T, KS_APPLY>(f.g, ap, s, false, &f.tiles);
+            if (nbk)
+                amr_mask_bdry_kernel<T><<<nbk, tile_block(), 0, s>>>(kf, ft, f.yT.as<T>(), f.yT.as<T>(), tl);
+            restrict_full<T>(f.g, gcl, f.yT.as<T>(), w, s);
             if (l == 1) {
                 SArgs<T, T> ac{};
                 ac.g = kcl;
@@ -512,9 +495,8 @@ class AmrNodal {
                 ac.rout = rtmp.as<T>();
                 launch_stencil<T, T, KS_APPLY>(gc, ac, s, false);
                 amr_coarse_jacobi_kernel<T><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-                    kf, kcl, ft, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), f.yT.as<T>(), H.L[0].sig,
-                    make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), ec,
-                    cnear.as<unsigned char>());
+                    kcl, cdof.as<unsigned char>(), rcT.as<T>(), rtmp.as<T>(), w, H.L[0].sig,
+                    make_coef<double>(gc.dx).diag4, nodal_omega(gc, opts.omega), ec);
             } else {
                 AmrFine& c = lev(l - 1);
                 int nbc = 0;
@@ -528,9 +510,8 @@ class AmrNodal {
                 launch_stencil<T, T, KS_APPLY>(c.g, ac, s, false, &c.tiles);
                 if (nbc)
                     amr_coarse_jacobi_kernel<T><<<nbc, tile_block(), 0, s>>>(
-                        kf, kcl, ft, c.dof.as<unsigned char>(), c.rT.as<T>(), c.yT.as<T>(), f.yT.as<T>(),
-                        c.sigT<T>(), make_coef<double>(c.g.dx).diag4, nodal_omega(c.g, opts.omega), ec,
-                        c.cnear.as<unsigned char>(), tlc);
+                        kcl, c.dof.as<unsigned char>(), c.rT.as<T>(), c.yT.as<T>(), w, c.sigT<T>(),
+                        make_coef<double>(c.g.dx).diag4, nodal_omega(c.g, opts.omega), ec, tlc);
             }
         }
         ecur[l] = e;
@@ -586,6 +567,15 @@ class AmrNodal {
         const KGeom kc = kgeom(gc);
         if (mixed) prepare(H32, h32_sigma, s);
         else prepare(H64, h64_sigma, s);
+        if (last_mixed != (int)mixed) {
+            // the V-cycle's buffers change element type: parts of the levels
+            // that no tile holds must read 0 again (their coefficient is 0, but
+            // reinterpreted bytes of the other type need not be finite)
+            for (auto& f : F)
+                for (DevBuf* b : {&f->rT, &f->ea, &f->eb, &f->yT, &f->bC})
+                    if (b->p) MGX_HIP(hipMemsetAsync(b->p, 0, f->g.nnode() * sizeof(double), s));
+            last_mixed = (int)mixed;
+        }
         MGX_HIP(hipMemsetAsync(ibuf.p, 0, sizeof(int), s));
         if (gc.singular()) remove_rhs_mean(s);
         // ||b||_inf over the unknowns of every level
@@ -601,12 +591,12 @@ class AmrNodal {
                     scal.as<unsigned long long>(), tl);
             else
                 amr_coarse_combine_kernel<double><<<nbk, tile_block(), 0, s>>>(
-                    kgeom(f.g), kgeom(f.g), nullptr, f.dof.as<unsigned char>(), f.b.as<double>(), nullptr, 0.0,
-                    f.rtmp.as<double>(), scal.as<unsigned long long>(), nullptr, tl);
+                    kgeom(f.g), f.dof.as<unsigned char>(), f.b.as<double>(), nullptr, 0.0, f.rtmp.as<double>(),
+                    scal.as<unsigned long long>(), tl);
         }
         amr_coarse_combine_kernel<double><<<grid3(gc.nn[0], gc.nn[1], gc.nn[2]), block3(), 0, s>>>(
-            kc, kc, nullptr, cdof.as<unsigned char>(), bc.as<double>(), nullptr, 0.0, rtmp.as<double>(),
-            scal.as<unsigned long long>(), nullptr);
+            kc, cdof.as<unsigned char>(), bc.as<double>(), nullptr, 0.0, rtmp.as<double>(),
+            scal.as<unsigned long long>());
         const double bnorm = read_norm(0, s);
         MGX_HIP(hipMemsetAsync(xc.p, 0, gc.nnode() * sizeof(double), s));
         for (auto& f : F) MGX_HIP(hipMemsetAsync(f->x.p, 0, f->g.nnode() * sizeof(double), s));
@@ -896,11 +886,14 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
             if (!f.finer) {
                 amr_fine_resid_kernel<double><<<nbk, tile_block(), 0, s>>>(
                     kf, f.ftype.as<unsigned char>(), f.y.as<double>(), nullptr, f.b.as<double>(), nullptr, tl);
+                amr_mask_bdry_kernel<double><<<nbk, tile_block(), 0, s>>>(kf, f.ftype.as<unsigned char>(),
+                                                                          f.y.as<double>(), f.wm.as<double>(), tl);
             } else {
                 AmrFine& u = S.lev(l + 1);
+                restrict_full<double>(u.g, f.g, u.wm.as<double>(), f.rw.as<double>(), s);
                 amr_coarse_combine_kernel<double><<<nbk, tile_block(), 0, s>>>(
-                    kgeom(u.g), kf, u.ftype.as<unsigned char>(), f.dof.as<unsigned char>(), f.y.as<double>(),
-                    u.y.as<double>(), 1.0, f.b.as<double>(), nullptr, f.cnear.as<unsigned char>(), tl);
+                    kf, f.dof.as<unsigned char>(), f.y.as<double>(), f.rw.as<double>(), 1.0, f.b.as<double>(),
+                    nullptr, tl, f.ftype.as<unsigned char>(), f.wm.as<double>());
             }
         }
         {
@@ -908,15 +901,10 @@ MGX_API int mgx_amr_nodal_project(mgx_amr_nodal h, double* const* vel, int vel_n
             nodal_rhs_kernel<<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
                 kc, h->ug[0]->as<double>(), S.rtmp.as<double>(), 0.25 / S.gc.dx[0], 0.25 / S.gc.dx[1],
                 0.25 / S.gc.dx[2]);
-            if (L > 0)
-                amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
-                    kgeom(S.lev(1).g), kc, S.lev(1).ftype.as<unsigned char>(), S.cdof.as<unsigned char>(),
-                    S.rtmp.as<double>(), S.lev(1).y.as<double>(), 1.0, S.bc.as<double>(), nullptr,
-                    S.cnear.as<unsigned char>());
-            else
-                amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
-                    kc, kc, nullptr, S.cdof.as<unsigned char>(), S.rtmp.as<double>(), nullptr, 0.0,
-                    S.bc.as<double>(), nullptr, nullptr);
+            if (L > 0) restrict_full<double>(S.lev(1).g, S.gc, S.lev(1).wm.as<double>(), S.rw0.as<double>(), s);
+            amr_coarse_combine_kernel<double><<<grid3(S.gc.nn[0], S.gc.nn[1], S.gc.nn[2]), block3(), 0, s>>>(
+                kc, S.cdof.as<unsigned char>(), S.rtmp.as<double>(), L > 0 ? S.rw0.as<double>() : nullptr, 1.0,
+                S.bc.as<double>(), nullptr);
         }
         S.solve(rtol, atol, mode, info, s);
         // u -= sigma G phi on every level, G phi out; from the finest level

@@ -86,36 +86,6 @@ static __global__ void __launch_bounds__(256) amr_prolong_kernel(KGeom gf, KGeom
     ef[id] = (t == AMR_DOF) ? ef[id] + p : p;
 }
 
-// full-weighting sum over the 27 fine nodes around coarse node (I, J, K) of
-// the values selected by want (a node type): sum_o w_o v(2I + o)
-template <class TV>
-__device__ __forceinline__ double amr_restrict_sel(const KGeom& gf, const unsigned char* ft,
-                                                   const TV* v, int want, int I, int J, int K) {
-    double acc = 0.0;
-    for (int oz = -1; oz <= 1; ++oz) {
-        int fk = 2 * K + oz;
-        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
-        else if (fk < 0 || fk >= gf.nn[2]) continue;
-        const double wz = oz == 0 ? 0.5 : 0.25;
-        for (int oy = -1; oy <= 1; ++oy) {
-            int fj = 2 * J + oy;
-            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
-            else if (fj < 0 || fj >= gf.nn[1]) continue;
-            const double wy = oy == 0 ? 0.5 : 0.25;
-            for (int ox = -1; ox <= 1; ++ox) {
-                int fi = 2 * I + ox;
-                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
-                else if (fi < 0 || fi >= gf.nn[0]) continue;
-                const long m = fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk);
-                if (ft[m] != want) continue;
-                const double wx = ox == 0 ? 0.5 : 0.25;
-                acc += wx * wy * wz * (double)v[m];
-            }
-        }
-    }
-    return acc;
-}
-
 // max over the block (256 threads, every one calls this), one atomic per block
 __device__ __forceinline__ void wave_max_atomic(unsigned long long* norm, double v) {
     __shared__ double red[4];
@@ -123,126 +93,95 @@ __device__ __forceinline__ void wave_max_atomic(unsigned long long* norm, double
     if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_nonneg(norm, m);
 }
 
-// coarse unknowns: out = base + sgn R(y on dF); every other coarse node 0;
-// optional max |out| (every lane reaches the shuffles); tl: the coarse
-// level's tile list (a level above level 0), else the whole coarse grid
+// coarse unknowns: out = base + sgn w, w the restriction of the finer
+// level's dF values (NULL: none); every other coarse node 0; optional max
+// |out| (every lane reaches the shuffles); tl: the coarse level's tile list
+// (a level above level 0), else the whole coarse grid.  A coarse level that
+// is itself a fine level (ftc) also writes its own dF values of base to wb
+// (0 elsewhere), for the restriction onto the level below it.
 template <class TO>
 static __global__ void __launch_bounds__(256) amr_coarse_combine_kernel(
-    KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
-    const double* __restrict__ base, const double* __restrict__ y, double sgn, TO* __restrict__ out,
-    unsigned long long* norm, const unsigned char* __restrict__ near = nullptr,
-    const int4* tl = nullptr) {
+    KGeom gc, const unsigned char* __restrict__ cdof, const double* __restrict__ base,
+    const double* __restrict__ w, double sgn, TO* __restrict__ out, unsigned long long* norm,
+    const int4* tl = nullptr, const unsigned char* __restrict__ ftc = nullptr, double* __restrict__ wb = nullptr) {
     int I, J, K;
     const bool valid = grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K);
     double r = 0.0;
     if (valid) {
         const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
-        if (cdof[id] && !is_dir(gc, I, J, K))
-            r = base[id] + ((sgn == 0.0 || (near && !near[id]))
-                                ? 0.0 : sgn * amr_restrict_sel(gf, ft, y, AMR_BDRY, I, J, K));
+        if (cdof[id] && !is_dir(gc, I, J, K)) r = base[id] + (w ? sgn * w[id] : 0.0);
         out[id] = (TO)r;
+        if (wb) wb[id] = ftc[id] == AMR_BDRY ? base[id] : 0.0;
     }
     if (norm) wave_max_atomic(norm, nabs(r));
 }
 
-// fine unknowns: r = b - y (cast to TO), 0 elsewhere; optional max |r|
+// w = y on dF (the level's AMR_BDRY nodes), 0 elsewhere, over the level's tiles
+template <class T>
+static __global__ void __launch_bounds__(256) amr_mask_bdry_kernel(KGeom g, const unsigned char* __restrict__ ft,
+                                                                   const T* __restrict__ y, T* __restrict__ w,
+                                                                   const int4* tl) {
+    int i, j, k;
+    if (!grid_point_tl(tl, g.nn[0], g.nn[1], i, j, k)) return;
+    const long id = i + (long)g.nn[0] * (j + (long)g.nn[1] * k);
+    w[id] = ft[id] == AMR_BDRY ? y[id] : (T)0;
+}
+
+// fine unknowns: r = b - y (cast to TO), 0 elsewhere; optional max |r|;
+// wb (optional): y on dF, 0 elsewhere (for the restriction onto the level
+// below)
 template <class TO>
 static __global__ void __launch_bounds__(256) amr_fine_resid_kernel(
     KGeom gf, const unsigned char* __restrict__ ft, const double* __restrict__ b,
     const double* __restrict__ y, TO* __restrict__ out, unsigned long long* norm,
-    const int4* tl = nullptr) {
+    const int4* tl = nullptr, double* __restrict__ wb = nullptr) {
     int i, j, k;
     const bool valid = grid_point_tl(tl, gf.nn[0], gf.nn[1], i, j, k);
     double r = 0.0;
     if (valid) {
         const long id = i + (long)gf.nn[0] * (j + (long)gf.nn[1] * k);
-        if (ft[id] == AMR_DOF && !is_dir(gf, i, j, k)) r = b[id] - (y ? y[id] : 0.0);
+        const int t = ft[id];
+        if (t == AMR_DOF && !is_dir(gf, i, j, k)) r = b[id] - (y ? y[id] : 0.0);
         out[id] = (TO)r;
+        if (wb) wb[id] = t == AMR_BDRY ? y[id] : 0.0;
     }
     if (norm) wave_max_atomic(norm, nabs(r));
 }
 
 // right-hand side of the coarse-grid correction (all coarse nodes):
-//   b_C = [n unknown] (r_c - R(y on dF)) + R((r_f - y) on fine unknowns)
-// with r the composite residual and y = L_f e after the fine pre-smoothing
+//   b_C = [n unknown] r_c + w,  w = R(r_f - L_f e)
+// with r the composite residual and e the fine correction after the
+// pre-smoothing (r_f is 0 on dF, and L_f e is 0 away from F's closure);
+// nodes under F have no dF node around them
 template <class T>
 static __global__ void __launch_bounds__(256) amr_coarse_rhs_kernel(
-    KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
-    const T* __restrict__ rf, const T* __restrict__ yf, const T* __restrict__ rc, T* __restrict__ bC,
-    const unsigned char* __restrict__ near = nullptr, const int4* tl = nullptr) {
+    KGeom gc, const unsigned char* __restrict__ cdof, const T* __restrict__ rc, const T* __restrict__ w,
+    T* __restrict__ bC, const int4* tl = nullptr) {
     int I, J, K;
     if (!grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (is_dir(gc, I, J, K)) { bC[id] = 0; return; }
-    if (near && !near[id]) {  // no fine node of F's closure around: the coarse residual alone
-        bC[id] = cdof[id] ? rc[id] : (T)0;
-        return;
-    }
-    double acc = 0.0;
-    for (int oz = -1; oz <= 1; ++oz) {
-        int fk = 2 * K + oz;
-        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
-        else if (fk < 0 || fk >= gf.nn[2]) continue;
-        const double wz = oz == 0 ? 0.5 : 0.25;
-        for (int oy = -1; oy <= 1; ++oy) {
-            int fj = 2 * J + oy;
-            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
-            else if (fj < 0 || fj >= gf.nn[1]) continue;
-            const double wy = oy == 0 ? 0.5 : 0.25;
-            for (int ox = -1; ox <= 1; ++ox) {
-                int fi = 2 * I + ox;
-                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
-                else if (fi < 0 || fi >= gf.nn[0]) continue;
-                const long m = fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk);
-                const int t = ft[m];
-                const double w = (ox == 0 ? 0.5 : 0.25) * wy * wz;
-                if (t == AMR_DOF) acc += w * ((double)rf[m] - (double)yf[m]);
-                else if (t == AMR_BDRY && cdof[id]) acc -= w * (double)yf[m];
-            }
-        }
-    }
-    if (cdof[id]) acc += (double)rc[id];
-    bC[id] = (T)acc;
+    bC[id] = (T)((cdof[id] ? (double)rc[id] : 0.0) + (double)w[id]);
 }
 
 // One Jacobi sweep on the coarse unknowns of the composite correction
 // equation, after the fine post-smoothing: with yc = L_c,out ec (coarse
-// elements outside F) and yf = L_f e (the fine correction, dF following the
-// coarse one), t = r_c - (yc + R(yf on dF)) and ec += omega t / diag, diag of
-// the coarse operator with the full coarse sigma (near dF the composite
-// diagonal differs by up to ~20%, which a damped Jacobi sweep tolerates).
-// Smoothing the coarse unknowns next to F with the composite operator is what
-// the coarse correction alone leaves undone.
+// elements outside F) and w = R(yf on dF) (yf = L_f e, the fine correction,
+// dF following the coarse one), t = r_c - (yc + w) and
+// ec += omega t / diag, diag of the coarse operator with the full coarse
+// sigma (near dF the composite diagonal differs by up to ~20%, which a damped
+// Jacobi sweep tolerates).  Smoothing the coarse unknowns next to F with the
+// composite operator is what the coarse correction alone leaves undone.
 template <class T>
 static __global__ void __launch_bounds__(256) amr_coarse_jacobi_kernel(
-    KGeom gf, KGeom gc, const unsigned char* __restrict__ ft, const unsigned char* __restrict__ cdof,
-    const T* __restrict__ rc, const T* __restrict__ yc, const T* __restrict__ yf,
-    const T* __restrict__ sig, double diag4, double omega, T* __restrict__ ec,
-    const unsigned char* __restrict__ near = nullptr, const int4* tl = nullptr) {
+    KGeom gc, const unsigned char* __restrict__ cdof, const T* __restrict__ rc, const T* __restrict__ yc,
+    const T* __restrict__ w, const T* __restrict__ sig, double diag4, double omega, T* __restrict__ ec,
+    const int4* tl = nullptr) {
     int I, J, K;
     if (!grid_point_tl(tl, gc.nn[0], gc.nn[1], I, J, K)) return;
     const long id = I + (long)gc.nn[0] * (J + (long)gc.nn[1] * K);
     if (!cdof[id] || is_dir(gc, I, J, K)) return;
-    double acc = (double)yc[id];
-    for (int oz = -1; oz <= 1 && !(near && !near[id]); ++oz) {
-        int fk = 2 * K + oz;
-        if (gf.per[2]) fk = fk < 0 ? fk + gf.nn[2] : (fk >= gf.nn[2] ? fk - gf.nn[2] : fk);
-        else if (fk < 0 || fk >= gf.nn[2]) continue;
-        const double wz = oz == 0 ? 0.5 : 0.25;
-        for (int oy = -1; oy <= 1; ++oy) {
-            int fj = 2 * J + oy;
-            if (gf.per[1]) fj = fj < 0 ? fj + gf.nn[1] : (fj >= gf.nn[1] ? fj - gf.nn[1] : fj);
-            else if (fj < 0 || fj >= gf.nn[1]) continue;
-            const double wy = oy == 0 ? 0.5 : 0.25;
-            for (int ox = -1; ox <= 1; ++ox) {
-                int fi = 2 * I + ox;
-                if (gf.per[0]) fi = fi < 0 ? fi + gf.nn[0] : (fi >= gf.nn[0] ? fi - gf.nn[0] : fi);
-                else if (fi < 0 || fi >= gf.nn[0]) continue;
-                const long m = fi + (long)gf.nn[0] * (fj + (long)gf.nn[1] * fk);
-                if (ft[m] == AMR_BDRY) acc += (ox == 0 ? 0.5 : 0.25) * wy * wz * (double)yf[m];
-            }
-        }
-    }
+    const double acc = (double)yc[id] + (double)w[id];
     double ss = 0.0;
 #pragma unroll
     for (int q = 0; q < 8; ++q)

@@ -135,6 +135,11 @@ class TileSet {
     int nblk_ = -1;
 };
 
+// Full weighting of a fine level's nodal array onto every node of the level
+// below it (dense, z-streamed as the V-cycle's restriction; Dirichlet nodes 0)
+template <class T>
+void restrict_full(const NodalGeom& gf, const NodalGeom& gc, const T* rf, T* rc, hipStream_t s);
+
 // Launch one 27-point stencil pass (Jacobi / residual / update+residual);
 // ts (optional) restricts it to the tiles that hold active nodes.
 template <class T, class TC, int MODE>

@@ -332,6 +332,13 @@ static void launch_restrict(const KGeom& kf, const NodalGeom& gc, const KGeom& k
         kf, kc, rf, rc, sigc, diag4c, omegac, xc1, kz, gx, gy, gz);
 }
 
+template <class T>
+void restrict_full(const NodalGeom& gf, const NodalGeom& gc, const T* rf, T* rc, hipStream_t s) {
+    launch_restrict<T, T>(kgeom(gf), gc, kgeom(gc), rf, rc, nullptr, (T)0, (T)0, nullptr, s);
+}
+template void restrict_full<float>(const NodalGeom&, const NodalGeom&, const float*, float*, hipStream_t);
+template void restrict_full<double>(const NodalGeom&, const NodalGeom&, const double*, double*, hipStream_t);
+
 void LaunchTimer::begin(hipStream_t s) {
     if (!on) return;
     while (ev.size() < used + 2) {

@@ -124,3 +124,34 @@ def test_levels_must_nest_and_stay_off_inflow_faces(ctx):
         P.set_sigma([ctx.array(B.field_size(cb, 0, 1))] * 2, 0)
     assert e.value.code == mgx.MGX_ERR_UNSUPPORTED and "inflow" in str(e.value)
     P.close()
+
+
+def test_one_handle_switches_precision(ctx, orc):
+    """FP64, mixed and FP64 again on one handle (the V-cycle buffers change
+    element type between the solves) give the oracle's projection each time."""
+    import oracle
+    from incflo_amd import mgx
+    nc = (16, 16, 16)
+    case = PER
+    _say("case amr 3 levels, one handle, fp64 / mixed / fp64")
+    gc = oracle.Geom.make(nc)
+    A = AmrLevels(gc, case["levels"])
+    rng = np.random.default_rng(5)
+    sigs = [0.01 / (1.0 + 0.5 * rng.random(g.cell_shape())) for g in A.g]
+    us = [np.where(A.fmask[l][None], rng.standard_normal((3,) + g.cell_shape()), 0.0)
+          for l, g in enumerate(A.g)]
+    boxes = [B.chop_domain(nc, 8)] + [[(b[:3], b[3:]) for b in bl] for bl in case["levels"]]
+    levs = [mgx.make_level(tuple(A.g[l].n), boxes[l]) for l in range(3)]
+    P = mgx.AmrNodalProjector(ctx, levs)
+    P.set_sigma([ctx.array(B.to_boxed(sigs[l], boxes[l], 0)) for l in range(3)], 0)
+    un, ph, _, _ = A.project(sigs, us, rtol=1e-13)
+    for mode, rtol, tol in ((mgx.MODE_FP64, 1e-12, 1e-9), (mgx.MODE_MIXED, 1e-10, 1e-7),
+                            (mgx.MODE_FP64, 1e-12, 1e-9)):
+        vel = [ctx.array(B.to_boxed(us[l], boxes[l], 1)) for l in range(3)]
+        info = P.project(vel, 1, rtol, 0.0, mode=mode)
+        assert info.converged and info.rel_resid <= rtol
+        for l in range(3):
+            got = B.from_boxed(vel[l].download(), boxes[l], 1, A.g[l].cell_shape(), ncomp=3)
+            f = A.fmask[l]
+            assert _rel(got[:, f], un[l][:, f]) <= tol, (mode, l)
+    P.close()
