@@ -50,6 +50,7 @@ class UNet {
     std::vector<DevBuf> gact_; // gradients of the activation slots
     DevBuf loss_buf, ybuf;
     DevBuf ksws;                    // split-K partial sums of the forward convolutions
+    DevBuf gws;                     // per-chunk / per-block gradient partials (summed in a fixed order)
     static constexpr long kKsWsFloats = 4L << 20;
 
     void create(int base_channels, int levels_, uint64_t seed, hipStream_t s);

@@ -30,12 +30,13 @@ __global__ void head_fwd_kernel(const float* __restrict__ x, int c, long nvox,
     y[v] = s;
 }
 
-// dx[v][c] = dy[v] w[c]; dw[c] += sum_v dy[v] x[v][c]; db += sum_v dy[v]
+// dx[v][c] = dy[v] w[c]; per block: part[block][c] = sum_v dy[v] x[v][c],
+// part[block][c_in] = sum_v dy[v] (the head's dW and db, summed in order
+// afterwards)
 __global__ void __launch_bounds__(256) head_bwd_kernel(const float* __restrict__ x, int c, long nvox,
                                                        const float* __restrict__ w,
                                                        const float* __restrict__ dy,
-                                                       float* __restrict__ dx, float* __restrict__ dw,
-                                                       float* __restrict__ db) {
+                                                       float* __restrict__ dx, float* __restrict__ part) {
     __shared__ float red[256];
     float acc[32];
     for (int ch = 0; ch < c; ++ch) acc[ch] = 0.f;
@@ -55,7 +56,7 @@ __global__ void __launch_bounds__(256) head_bwd_kernel(const float* __restrict__
             if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
             __syncthreads();
         }
-        if (threadIdx.x == 0) atomicAdd(ch < c ? &dw[ch] : db, red[0]);
+        if (threadIdx.x == 0) part[(long)blockIdx.x * (c + 1) + ch] = red[0];
         __syncthreads();
     }
 }
@@ -70,16 +71,6 @@ static void launch_conv(const ConvGeom& cg, const float* in, const unsigned shor
     conv3d_forward_launch(cg, in, w, bias, out, relu, ws, ws_floats, s);
 }
 
-static void launch_dw(const ConvGeom& cg, const float* in, const float* dy, float* dw, int ldw,
-                      hipStream_t s) {
-    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
-    const int tiles = (cg.kp / 16) * (cg.cout / 16);
-    // voxel chunk per wave: enough waves to fill the GPU, at least 512 voxels each
-    long chunk = std::max<long>(512, (nvox * tiles) / (256L * 64));
-    chunk = (chunk + 31) / 32 * 32;
-    const long nchunk = (nvox + chunk - 1) / chunk;
-    conv3d_dw_kernel<<<(unsigned)(tiles * nchunk), 64, 0, s>>>(cg, in, dy, dw, ldw, (int)chunk);
-}
 
 static int pad32(int k) { return (k + 31) / 32 * 32; }
 
@@ -316,9 +307,10 @@ void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
         const long nv = nvox_at(sh, lvl);
         relu_bwd_kernel<<<g1(nv * L.cout), 256, 0, stream>>>(y, dy, nv * L.cout);
         ConvGeom cg = geom_at(sh, lvl, L.cin, L.cout, L.kp);
-        launch_dw(cg, x, dy, G + L.woff, 27 * L.cin, stream);
-        const int bblocks = (int)std::min<long>(1024, (nv * L.cout + 255) / 256);
-        bias_grad_kernel<<<bblocks, 256, 0, stream>>>(dy, nv, L.cout, G + L.boff);
+        gws.alloc(conv_dw_workspace(cg) * sizeof(float));
+        conv_dw_launch(cg, x, dy, G + L.woff, 27 * L.cin, gws.as<float>(), stream);
+        gws.alloc((size_t)bias_grad_blocks(nv, L.cout) * L.cout * sizeof(float));
+        bias_grad_launch(dy, nv, L.cout, G + L.boff, gws.as<float>(), stream);
         if (dx) {
             ConvGeom ct = geom_at(sh, lvl, L.cout, L.cin, L.kpt);
             launch_conv(ct, dy, wt + L.wt_off, nullptr, dx, 0, ksws.as<float>(), kKsWsFloats, stream);
@@ -328,8 +320,15 @@ void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
     // head
     const long n0 = nvox_at(sh, 0);
     const int s0 = dec_slot(0);
-    head_bwd_kernel<<<(unsigned)std::min<long>(1024, g1(n0)), 256, 0, stream>>>(
-        A(s0 + 2), base, n0, P + head_w, dout, GA(s0 + 2), G + head_w, G + head_b);
+    {
+        // the head's dW (base values) and db follow each other in the parameter vector
+        const unsigned hb = (unsigned)std::min<long>(1024, g1(n0));
+        gws.alloc((size_t)hb * (base + 1) * sizeof(float));
+        head_bwd_kernel<<<hb, 256, 0, stream>>>(A(s0 + 2), base, n0, P + head_w, dout, GA(s0 + 2),
+                                                gws.as<float>());
+        MGX_REQUIRE(head_b == head_w + base, MGX_ERR_STATE, "U-net: head parameter layout");
+        sum_parts_kernel<<<1, 256, 0, stream>>>(gws.as<float>(), (int)hb, base + 1, G + head_w);
+    }
     // decoder, finest first
     int li_dec = (int)conv.size() - 2;  // index of dec level 0 conv a
     for (int l = 0; l <= levels - 2; ++l) {
@@ -385,8 +384,10 @@ float UNet::train_step(const float* in, const float* target, const NetShape& sh,
     float* dy = y + n0;
     forward(in, sh, y);
     MGX_HIP(hipMemsetAsync(loss_buf.p, 0, sizeof(float), stream));
-    mse_kernel<<<(unsigned)std::min<long>(1024, g1(n0)), 256, 0, stream>>>(y, 1, target, n0, dy,
-                                                                          loss_buf.as<float>());
+    const unsigned lb = (unsigned)std::min<long>(1024, g1(n0));
+    gws.alloc((size_t)lb * sizeof(float));
+    mse_kernel<<<lb, 256, 0, stream>>>(y, 1, target, n0, dy, gws.as<float>());
+    sum_parts_kernel<<<1, 256, 0, stream>>>(gws.as<float>(), (int)lb, 1, loss_buf.as<float>());
     zero_grads();
     backward(in, dy, sh);
     ++step;

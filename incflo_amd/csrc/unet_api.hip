@@ -160,19 +160,16 @@ MGX_API int mgx_conv3d_backward(mgx_ctx ctx, const float* in, const float* w, co
         const ConvGeom cg = conv_geom(nx, ny, nz, batch, cin, cout, periodic);
         const long nvox = (long)nx * ny * nz * batch;
         hipStream_t s = ctx->stream;
+        DevBuf ws;
         if (dw) {
             MGX_HIP(hipMemsetAsync(dw, 0, (size_t)cout * 27 * cin * sizeof(float), s));
-            const int tiles = (cg.kp / 16) * (cg.cout / 16);
-            long chunk = std::max<long>(512, (nvox * tiles) / (256L * 64));
-            chunk = (chunk + 31) / 32 * 32;
-            const long nchunk = (nvox + chunk - 1) / chunk;
-            conv3d_dw_kernel<<<(unsigned)(tiles * nchunk), 64, 0, s>>>(cg, in, dout, dw, 27 * cin,
-                                                                      (int)chunk);
+            ws.alloc(conv_dw_workspace(cg) * sizeof(float));
+            conv_dw_launch(cg, in, dout, dw, 27 * cin, ws.as<float>(), s);
         }
         if (db) {
             MGX_HIP(hipMemsetAsync(db, 0, cout * sizeof(float), s));
-            const int bb = (int)std::min<long>(1024, (nvox * cout + 255) / 256);
-            bias_grad_kernel<<<bb, 256, 0, s>>>(dout, nvox, cout, db);
+            ws.alloc((size_t)bias_grad_blocks(nvox, cout) * cout * sizeof(float));
+            bias_grad_launch(dout, nvox, cout, db, ws.as<float>(), s);
         }
         if (din) {
             MGX_REQUIRE(cin % 16 == 0, MGX_ERR_ARG, "conv3d backward-data: cin must be a multiple of 16");

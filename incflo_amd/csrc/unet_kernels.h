@@ -202,13 +202,13 @@ inline void conv3d_forward_launch(const ConvGeom& cg, const float* in, const uns
     }
 }
 
-// Weight gradient: dW[co][k] += sum_v dY[v][co] * A[v][k] (row stride ldw).  One wave owns one
-// 16 (co) x 16 (k) tile and a range of voxels; partial sums are added
-// atomically (FP32).
+// Weight gradient: part[chunk][co][k] = sum over the chunk's voxels v of
+// dY[v][co] * A[v][k] (row stride kp); conv3d_dw_reduce_kernel adds the
+// chunks in a fixed order, so training is reproducible.  One wave owns one
+// 16 (co) x 16 (k) tile and one chunk of voxels.
 static __global__ void __launch_bounds__(64) conv3d_dw_kernel(ConvGeom g, const float* __restrict__ in,
                                                               const float* __restrict__ dy,
-                                                              float* __restrict__ dw, int ldw,
-                                                              int vchunk) {
+                                                              float* __restrict__ part, int vchunk) {
     const int lane = threadIdx.x;
     const int row = lane & 15, kq = lane >> 4;
     const int ntk = g.kp / 16;
@@ -261,21 +261,90 @@ static __global__ void __launch_bounds__(64) conv3d_dw_kernel(ConvGeom g, const 
     for (int j = 0; j < 4; ++j) {
         const int co = co0 + 4 * kq + j;
         const int kc = kk0 + row;
-        if (kc < K) atomicAdd(&dw[(long)co * ldw + kc], acc[j]);
+        if (kc < K) part[((long)chunk * g.cout + co) * g.kp + kc] = acc[j];
     }
 }
 
-// bias gradient: db[c] += sum_v dY[v][c]
-static __global__ void __launch_bounds__(256) bias_grad_kernel(const float* __restrict__ dy, long nvox,
-                                                               int c, float* __restrict__ db) {
-    const int co = threadIdx.x % c;
-    const int lanes = 256 / c * c;
-    if ((int)threadIdx.x >= lanes) return;
+// dW[co][k] += sum over chunks c (in order) of part[c][co][k], k < K (row stride ldw)
+static __global__ void __launch_bounds__(256) conv3d_dw_reduce_kernel(const float* __restrict__ part,
+                                                                     int nchunk, int cout, int kp, int K,
+                                                                     float* __restrict__ dw, int ldw) {
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= (long)cout * K) return;
+    const int co = (int)(id / K), k = (int)(id - (long)co * K);
     float s = 0.f;
-    for (long v = (long)blockIdx.x * (256 / c) + threadIdx.x / c; v < nvox;
-         v += (long)gridDim.x * (256 / c))
-        s += dy[v * c + co];
-    atomicAdd(&db[co], s);
+    for (int c = 0; c < nchunk; ++c) s += part[((long)c * cout + co) * kp + k];
+    dw[(long)co * ldw + k] += s;
+}
+
+// out[i] += sum over parts p (in order) of part[p n + i]
+static __global__ void __launch_bounds__(256) sum_parts_kernel(const float* __restrict__ part, int nparts,
+                                                              int n, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int p = 0; p < nparts; ++p) s += part[(long)p * n + i];
+    out[i] += s;
+}
+
+// voxels per weight-gradient wave: enough waves to fill the GPU, at least 512
+// voxels each, a multiple of the 32 voxels of one MFMA step
+inline long conv_dw_chunk(const ConvGeom& cg) {
+    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
+    const int tiles = (cg.kp / 16) * (cg.cout / 16);
+    const long chunk = std::max<long>(512, (nvox * tiles) / (256L * 64));
+    return (chunk + 31) / 32 * 32;
+}
+
+// floats of workspace conv_dw_launch needs
+inline size_t conv_dw_workspace(const ConvGeom& cg) {
+    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
+    const long chunk = conv_dw_chunk(cg);
+    return (size_t)((nvox + chunk - 1) / chunk) * cg.cout * cg.kp;
+}
+
+// dW[co][k] += dL/dW of the convolution (row stride ldw), through the per-chunk
+// partials in ws (conv_dw_workspace floats)
+inline void conv_dw_launch(const ConvGeom& cg, const float* in, const float* dy, float* dw, int ldw,
+                           float* ws, hipStream_t s) {
+    const long nvox = (long)cg.nx * cg.ny * cg.nz * cg.batch;
+    const int tiles = (cg.kp / 16) * (cg.cout / 16);
+    const long chunk = conv_dw_chunk(cg);
+    const long nchunk = (nvox + chunk - 1) / chunk;
+    conv3d_dw_kernel<<<(unsigned)(tiles * nchunk), 64, 0, s>>>(cg, in, dy, ws, (int)chunk);
+    const long n = (long)cg.cout * 27 * cg.cin;
+    conv3d_dw_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ws, (int)nchunk, cg.cout, cg.kp,
+                                                                       27 * cg.cin, dw, ldw);
+}
+
+// bias gradient, per block: part[block][c] = sum over the block's voxels of
+// dY[v][c], summed in a fixed order (sum_parts_kernel adds the blocks)
+static __global__ void __launch_bounds__(256) bias_grad_kernel(const float* __restrict__ dy, long nvox,
+                                                               int c, float* __restrict__ part) {
+    __shared__ float red[256];
+    const int co = threadIdx.x % c;
+    const int per = 256 / c, lanes = per * c;
+    float s = 0.f;
+    if ((int)threadIdx.x < lanes)
+        for (long v = (long)blockIdx.x * per + threadIdx.x / c; v < nvox; v += (long)gridDim.x * per)
+            s += dy[v * c + co];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < c) {
+        float t = 0.f;
+        for (int r = 0; r < per; ++r) t += red[r * c + threadIdx.x];
+        part[(long)blockIdx.x * c + threadIdx.x] = t;
+    }
+}
+
+inline int bias_grad_blocks(long nvox, int c) { return (int)std::min<long>(1024, (nvox * c + 255) / 256); }
+
+// db[c] += sum_v dY[v][c] through the per-block partials in ws
+// (bias_grad_blocks * c floats)
+inline void bias_grad_launch(const float* dy, long nvox, int c, float* db, float* ws, hipStream_t s) {
+    const int nb = bias_grad_blocks(nvox, c);
+    bias_grad_kernel<<<nb, 256, 0, s>>>(dy, nvox, c, ws);
+    sum_parts_kernel<<<(unsigned)((c + 255) / 256), 256, 0, s>>>(ws, nb, c, db);
 }
 
 // Data-gradient weights: Wt[ci][tap'*Cout + co] = W[co][(26 - tap')*Cin + ci]
@@ -422,7 +491,7 @@ static __global__ void __launch_bounds__(256) mse_kernel(const float* __restrict
         if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv);
+    if (threadIdx.x == 0) loss[blockIdx.x] = red[0] * inv;  // per block; summed in order afterwards
 }
 
 // Adam (Kingma & Ba 2015) on FP32 master parameters

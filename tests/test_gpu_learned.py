@@ -192,12 +192,43 @@ def test_unet_training_reduces_loss(ctx):
     t = 0.5 * x[..., 0] - 0.25 * x[..., 1]
     xd = ctx.array(x.astype(np.float32).ravel(), np.float32)
     td = ctx.array(t.astype(np.float32).ravel(), np.float32)
-    # (FP32 atomics in the weight gradient make the trajectory vary slightly
-    # from run to run: judge the mean of the last ten steps)
     losses = [net.train_step(xd, td, shape, 1e-3) for _ in range(120)]
     assert np.all(np.isfinite(losses))
     assert np.mean(losses[-10:]) < 0.5 * losses[0], (losses[0], losses[-10:])
     net.close()
+
+
+def test_unet_training_is_reproducible(ctx):
+    """Two networks from one seed trained on the same data follow the same
+    trajectory bit for bit: every gradient and loss reduction sums its
+    partials in a fixed order.  The second network first takes gradients of a
+    smaller batch (parameters unchanged), so its workspaces are sized for
+    that batch and then grown; the two gradients of one batch are equal too."""
+    from incflo_amd import mgx
+    shape, small = (2, 16, 16, 16), (1, 16, 16, 16)
+    _say("case unet training reproducible")
+    x = _unet_input(shape, 7)
+    t = 0.5 * x[..., 0] - 0.25 * x[..., 1]
+    xs = _unet_input(small, 8)
+    ts = xs[..., 0].copy()
+    xd = ctx.array(x.astype(np.float32).ravel(), np.float32)
+    td = ctx.array(t.astype(np.float32).ravel(), np.float32)
+    runs = []
+    xsd = ctx.array(xs.astype(np.float32).ravel(), np.float32)
+    tsd = ctx.array(ts.astype(np.float32).ravel(), np.float32)
+    grads = []
+    for warm in (False, True):
+        net = mgx.UNet(ctx, base=16, levels=4, seed=3)
+        if warm:
+            grads.append(net.gradients(xsd, tsd, small))
+        losses = [net.train_step(xd, td, shape, 1e-3) for _ in range(8)]
+        runs.append((np.array(losses), net.get_params()))
+        grads.append(net.gradients(xd, td, shape))
+        net.close()
+    assert np.array_equal(runs[0][0], runs[1][0]), (runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(grads[0], grads[2])
+    assert np.all(np.isfinite(grads[1])) and runs[0][0][-1] < runs[0][0][0]
 
 
 def _learned_case(ctx, n, seed, net, level, mode, opts=None):
