@@ -1258,12 +1258,23 @@ void NodalSolver::learned_correction(hipStream_t s) {
     sh.nz = lv.g.nn[2];
     sh.batch = 1;
     sh.per[0] = sh.per[1] = sh.per[2] = 1;
-    const hipStream_t saved = net->stream;
-    net->stream = s;
-    net->forward(net_in.as<float>(), sh, net_out.as<float>());
-    net->stream = saved;
+    if (H32.graphs) {
+        net->forward_graph(net_in.as<float>(), sh, net_out.as<float>(), s);
+    } else {
+        const hipStream_t saved = net->stream;
+        net->stream = s;
+        try {
+            net->forward(net_in.as<float>(), sh, net_out.as<float>());
+        } catch (...) {
+            net->stream = saved;
+            throw;
+        }
+        net->stream = saved;
+    }
     unscale_kernel<float><<<grid1d(n), 256, 0, s>>>(net_out.as<float>(), n, scal.as<double>() + 40, lv.x);
-    // upward half of a V-cycle from the predicted coarse correction
+    // upward half of a V-cycle from the predicted coarse correction, with the
+    // smoother the V-cycles use (set here: this can run before the first cycle)
+    H32.cheb = use_chebyshev(opts, true);
     for (int l = lc - 1; l >= 0; --l) {
         MGLevel<float>& f = H32.L[l];
         zero_kernel<float><<<grid1d(f.g.nnode()), 256, 0, s>>>(f.x, f.g.nnode());

@@ -53,8 +53,17 @@ class UNet {
     DevBuf gws;                     // per-chunk / per-block gradient partials (summed in a fixed order)
     static constexpr long kKsWsFloats = 4L << 20;
 
+    UNet() = default;
+    UNet(const UNet&) = delete;
+    UNet& operator=(const UNet&) = delete;
+    ~UNet();
     void create(int base_channels, int levels_, uint64_t seed, hipStream_t s);
     void forward(const float* in, const NetShape& sh, float* out);
+    // the same forward on stream s, replayed from a HIP graph captured on the
+    // first call for these (in, out, shape); plain launches where capture is
+    // not available.  Used inside the solver, where the forward's ~30 small
+    // launches would otherwise each pay a host launch gap.
+    void forward_graph(const float* in, const NetShape& sh, float* out, hipStream_t s);
     // backward from dL/d(out) (one channel, [batch][S]); accumulates into grads
     void backward(const float* in, const float* dout, const NetShape& sh);
     float train_step(const float* in, const float* target, const NetShape& sh, float lr);
@@ -62,10 +71,23 @@ class UNet {
     void refresh_bf16();
 
    private:
+    struct FwdGraph {
+        const float* in;
+        float* out;
+        NetShape sh;
+        hipGraphExec_t exec;
+    };
+    std::vector<FwdGraph> graphs_;
+    hipStream_t cap_stream_ = nullptr;  // capture stream (the caller's may be the null stream)
+    bool graphs_ok_ = true;
+    void drop_graphs();
+    void forward_body(const float* in, const NetShape& sh, float* out);
     void build_acts(const NetShape& sh);
     int dims_at(int lvl, int d) const;
     void conv_fwd(const ConvLayer& L, const float* in, int lvl, float* out, bool relu,
                   const unsigned short* w, const float* bias);
+    void launch_conv(const ConvGeom& cg, const float* in, const unsigned short* w, const float* bias,
+                     float* out, int relu);
 };
 
 }  // namespace mgx

@@ -65,10 +65,9 @@ inline unsigned g1(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-static void launch_conv(const ConvGeom& cg, const float* in, const unsigned short* w,
-                        const float* bias, float* out, int relu, float* ws, long ws_floats,
-                        hipStream_t s) {
-    conv3d_forward_launch(cg, in, w, bias, out, relu, ws, ws_floats, s);
+void UNet::launch_conv(const ConvGeom& cg, const float* in, const unsigned short* w, const float* bias,
+                       float* out, int relu) {
+    conv3d_forward_launch(cg, in, w, bias, out, relu, ksws.as<float>(), kKsWsFloats, stream);
 }
 
 
@@ -171,6 +170,7 @@ void UNet::build_acts(const NetShape& sh) {
     if (shape == sh && !act.empty()) return;
     MGX_REQUIRE(sh.nx % 8 == 0 && sh.ny % 8 == 0 && sh.nz % 8 == 0 && sh.batch >= 1,
                 MGX_ERR_ARG, "U-net input sizes must be multiples of 8");
+    drop_graphs();  // they hold the old activation addresses
     shape = sh;
     act.clear();
     act_c.clear();
@@ -211,7 +211,7 @@ void UNet::conv_fwd(const ConvLayer& L, const float* in, int lvl, float* out, bo
     cg.cin = L.cin;
     cg.cout = L.cout;
     cg.kp = L.kp;
-    launch_conv(cg, in, w, bias, out, relu ? 1 : 0, ksws.as<float>(), kKsWsFloats, stream);
+    launch_conv(cg, in, w, bias, out, relu ? 1 : 0);
 }
 
 static ConvGeom geom_at(const NetShape& sh, int lvl, int cin, int cout, int kp) {
@@ -235,9 +235,78 @@ void UNet::zero_grads() {
     zero_f32_kernel<<<g1(nparam), 256, 0, stream>>>(grads.as<float>(), nparam);
 }
 
+UNet::~UNet() {
+    drop_graphs();
+    if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+}
+
+void UNet::drop_graphs() {
+    for (FwdGraph& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
 void UNet::forward(const float* in, const NetShape& sh, float* out) {
     build_acts(sh);
     refresh_bf16();
+    forward_body(in, sh, out);
+}
+
+void UNet::forward_graph(const float* in, const NetShape& sh, float* out, hipStream_t s) {
+    const hipStream_t saved = stream;
+    stream = s;
+    build_acts(sh);  // drops the graphs when the activations move
+    refresh_bf16();
+    stream = saved;
+    const FwdGraph* hit = nullptr;
+    for (const FwdGraph& g : graphs_)
+        if (g.in == in && g.out == out && g.sh == sh) hit = &g;
+    if (!hit && graphs_ok_) {
+        if (graphs_.size() >= 8) drop_graphs();
+        if (!cap_stream_ && hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            cap_stream_ = nullptr;
+            graphs_ok_ = false;
+        } else if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            (void)hipGetLastError();
+            graphs_ok_ = false;
+        } else {
+            bool body_ok = true;
+            stream = cap_stream_;
+            try {
+                forward_body(in, sh, out);
+            } catch (...) {
+                body_ok = false;  // rethrown by the plain launches below
+            }
+            stream = saved;
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(cap_stream_, &g);
+            hipGraphExec_t exec = nullptr;
+            if (e == hipSuccess && body_ok) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e == hipSuccess && body_ok) {
+                graphs_.push_back(FwdGraph{in, out, sh, exec});
+                hit = &graphs_.back();
+            } else {
+                (void)hipGetLastError();
+                graphs_ok_ = false;
+            }
+        }
+    }
+    if (hit) {
+        MGX_HIP(hipGraphLaunch(hit->exec, s));
+        return;
+    }
+    stream = s;
+    try {
+        forward_body(in, sh, out);
+    } catch (...) {
+        stream = saved;
+        throw;
+    }
+    stream = saved;
+}
+
+void UNet::forward_body(const float* in, const NetShape& sh, float* out) {
     const unsigned short* wb = wbf.as<unsigned short>();
     const float* P = params.as<float>();
     int li = 0, si = 0;
@@ -313,7 +382,7 @@ void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
         bias_grad_launch(dy, nv, L.cout, G + L.boff, gws.as<float>(), stream);
         if (dx) {
             ConvGeom ct = geom_at(sh, lvl, L.cout, L.cin, L.kpt);
-            launch_conv(ct, dy, wt + L.wt_off, nullptr, dx, 0, ksws.as<float>(), kKsWsFloats, stream);
+            launch_conv(ct, dy, wt + L.wt_off, nullptr, dx, 0);
         }
     };
 

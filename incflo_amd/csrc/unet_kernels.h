@@ -50,43 +50,40 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
     return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ bf16x8 to_bf16x8(const float* p, bool valid) {
+__device__ __forceinline__ bf16x8 pack_bf16x8(const float4& a, const float4& b) {
+    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     bf16x8 r;
-    if (valid) {
-        const float4 a = *reinterpret_cast<const float4*>(p);
-        const float4 b = *reinterpret_cast<const float4*>(p + 4);
-        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const unsigned short s = bf16_bits(v[i]);
-            r[i] = __builtin_bit_cast(__bf16, s);
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) r[i] = __builtin_bit_cast(__bf16, (unsigned short)0);
-    }
+    for (int i = 0; i < 8; ++i) r[i] = __builtin_bit_cast(__bf16, bf16_bits(v[i]));
     return r;
 }
 
 // Forward 3x3x3 convolution (+bias, optional fused ReLU).  One wave computes
 // 16 voxels x all output channels (cout/16 accumulator tiles); a workgroup of
-// 4 waves covers 64 consecutive voxels of the batch.
-// Split-K: with part != nullptr, blockIdx.y selects a contiguous range of the
-// K steps and the raw partial sums go to part[split][v][co] (bias and ReLU are
-// applied by conv3d_splitk_reduce_kernel); small levels (few voxels, long K)
-// then keep many waves in flight instead of a few long dependent chains.
-template <int NT>
+// 4 waves covers 64 consecutive voxels of the batch.  The K loop issues the
+// activation loads of U consecutive K steps before their MFMAs, so U loads are
+// in flight per lane (the convolutions here are latency-bound: few voxels, few
+// waves per CU).
+// Split-K (part != nullptr): blockIdx.y selects a contiguous range of the K
+// steps and the raw partial sums go to part[split][v][co]; bias and ReLU are
+// applied by conv3d_splitk_reduce_kernel, which adds the splits in a fixed
+// order.  Small levels (few voxels, long K) then keep many waves in flight
+// instead of a few long dependent chains.  (Finishing the sum in the last
+// block of each tile instead saves that launch but costs more here: the
+// partials must be written back from one XCD's L2 and read from another's,
+// and one block then sums a whole tile alone.)
+template <int NT, int U>
 static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, const float* __restrict__ in,
                                                                const unsigned short* __restrict__ w,
                                                                const float* __restrict__ bias,
                                                                float* __restrict__ out, int relu,
-                                                               float* __restrict__ part = nullptr,
-                                                               int ksplit = 1) {
+                                                               float* __restrict__ part) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long S = (long)g.nx * g.ny * g.nz;
     const long nvox = S * g.batch;
-    const long v0 = ((long)blockIdx.x * 4 + wave) * 16;
-    if (v0 >= nvox) return;
+    const long vblk = (long)blockIdx.x * 64;
+    const long v0 = vblk + wave * 16;
+    const bool active = v0 < nvox;  // whole wave
     const int row = lane & 15, kq = lane >> 4;
     // this lane's A row: voxel v0 + row
     long va = v0 + row;
@@ -99,30 +96,48 @@ static __global__ void __launch_bounds__(256) conv3d_fwd_kernel(ConvGeom g, cons
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int K = 27 * g.cin;
+    const int ksplit = (int)gridDim.y;
     const int steps = g.kp / 32, per = (steps + ksplit - 1) / ksplit;
     const int s0 = (int)blockIdx.y * per, s1 = min(steps, s0 + per);
-    for (int k0 = 32 * s0; k0 < 32 * s1; k0 += 32) {
-        const int k = k0 + 8 * kq;
-        bf16x8 afrag;
-        if (k < K) {
-            const int tap = k / g.cin, ci = k - tap * g.cin;
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            bool valid = rowok;
-            const int xx = conv_wrap(x + dx, g.nx, g.per[0], valid);
-            const int yy = conv_wrap(y + dy, g.ny, g.per[1], valid);
-            const int zz = conv_wrap(z + dz, g.nz, g.per[2], valid);
-            const long nv = (long)b * S + xx + (long)g.nx * (yy + (long)g.ny * zz);
-            afrag = to_bf16x8(in + nv * g.cin + ci, valid);
-        } else {
-            afrag = to_bf16x8(in, false);
-        }
+    const float4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    if (active) {
+        for (int st = s0; st < s1; st += U) {
+            float4 ra[U][2];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int co = t * 16 + row;
-            const bf16x8 bfrag = *reinterpret_cast<const bf16x8*>(w + (long)co * g.kp + k);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc[t], 0, 0, 0);
+            for (int u = 0; u < U; ++u) {
+                const int k = 32 * (st + u) + 8 * kq;
+                ra[u][0] = zero4;
+                ra[u][1] = zero4;
+                if (st + u < s1 && k < K) {
+                    const int tap = k / g.cin, ci = k - tap * g.cin;
+                    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+                    bool valid = rowok;
+                    const int xx = conv_wrap(x + dx, g.nx, g.per[0], valid);
+                    const int yy = conv_wrap(y + dy, g.ny, g.per[1], valid);
+                    const int zz = conv_wrap(z + dz, g.nz, g.per[2], valid);
+                    if (valid) {
+                        const float* p = in + ((long)b * S + xx + (long)g.nx * (yy + (long)g.ny * zz)) * g.cin + ci;
+                        ra[u][0] = *reinterpret_cast<const float4*>(p);
+                        ra[u][1] = *reinterpret_cast<const float4*>(p + 4);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (st + u < s1) {  // uniform across the wave
+                    const int k = 32 * (st + u) + 8 * kq;
+                    const bf16x8 afrag = pack_bf16x8(ra[u][0], ra[u][1]);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const int co = t * 16 + row;
+                        const bf16x8 bfrag = *reinterpret_cast<const bf16x8*>(w + (long)co * g.kp + k);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag, acc[t], 0, 0, 0);
+                    }
+                }
+            }
         }
     }
+    if (!active) return;
     if (part) {
         float* dst = part + (long)blockIdx.y * nvox * g.cout;
 #pragma unroll
@@ -185,16 +200,19 @@ inline void conv3d_forward_launch(const ConvGeom& cg, const float* in, const uns
     }
     float* part = ksplit > 1 ? ws : nullptr;
     const dim3 grid(nb, ksplit);
+#define MGX_CONV_FWD(NT, U) \
+    conv3d_fwd_kernel<NT, U><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part)
     switch (cg.cout / 16) {
-        case 1: conv3d_fwd_kernel<1><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
-        case 2: conv3d_fwd_kernel<2><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
-        case 3: conv3d_fwd_kernel<3><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
-        case 4: conv3d_fwd_kernel<4><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
-        case 6: conv3d_fwd_kernel<6><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
-        case 8: conv3d_fwd_kernel<8><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
-        case 12: conv3d_fwd_kernel<12><<<grid, 256, 0, s>>>(cg, in, w, bias, out, relu, part, ksplit); break;
+        case 1: MGX_CONV_FWD(1, 8); break;
+        case 2: MGX_CONV_FWD(2, 4); break;
+        case 3: MGX_CONV_FWD(3, 2); break;
+        case 4: MGX_CONV_FWD(4, 2); break;
+        case 6: MGX_CONV_FWD(6, 1); break;
+        case 8: MGX_CONV_FWD(8, 1); break;
+        case 12: MGX_CONV_FWD(12, 1); break;
         default: throw Error(MGX_ERR_UNSUPPORTED, "conv3d: output channels must be 16,32,48,64,96,128 or 192");
     }
+#undef MGX_CONV_FWD
     if (part) {
         const long n = nvox * cg.cout;
         conv3d_splitk_reduce_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, cg.cout, ksplit, part,

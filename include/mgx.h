@@ -201,9 +201,10 @@ MGX_API int mgx_nodal_set_options(mgx_nodal h, const mgx_mlmg_opts* opts);
 MGX_API int mgx_nodal_set_timing(mgx_nodal h, int on);
 MGX_API int mgx_nodal_get_timing(mgx_nodal h, int which, double* total_ms, long long* count);
 /* Launch mode (no reference counterpart; for callers that must not use HIP
- * graphs, and for tests): on (default), each V-cycle is captured once as a
- * HIP graph and replayed; off, its kernels are launched one by one.  The
- * kernels and results are the same either way. */
+ * graphs, and for tests): on (default), each V-cycle and the learned
+ * predictor's network forward are captured once as HIP graphs and replayed;
+ * off, their kernels are launched one by one.  The kernels and results are
+ * the same either way. */
 MGX_API int mgx_nodal_set_graphs(mgx_nodal h, int on);
 /* Storage of the finest level of the mixed-mode V-cycle (no reference
  * counterpart; MLMG runs in FP64 throughout): 32 (default) stores FP32; 16

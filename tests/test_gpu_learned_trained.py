@@ -106,3 +106,36 @@ def test_trained_network_learned_mode(ctx, net, orc, n, seed):
         rel = np.max(np.abs(r)) / np.max(np.abs(b))
         _say(f"  oracle rel. residual of the learned phi {rel:.2e}")
         assert rel <= 1e-10
+
+
+def test_learned_graph_replays_match_plain_launches(ctx, net):
+    """The learned projection with its network forward and V-cycles replayed
+    from HIP graphs (captured on the first call, replayed on the next two)
+    gives bit for bit the phi and velocity of plain launches: the split-K
+    convolutions sum their splits in a fixed order and reset their arrival
+    counters themselves, so every replay starts from the same state."""
+    from incflo_amd import mgx, workloads
+    u, meta = net
+    n, seed = 128, 2
+    _say(f"case learned graphs vs plain launches n={n} seed={seed}")
+    rho, ustar, dt = workloads.variable_density_case(n, seed)
+    bxs = B.chop_domain((n, n, n), 32)
+    P = mgx.NodalProjector(ctx, mgx.make_level((n, n, n), bxs))
+    P.set_unet(u, int(round(np.log2(n / (meta["n"] >> meta["level"])))))
+    vel0 = ctx.array(B.to_boxed(ustar, bxs, 1))
+    rd = ctx.array(B.to_boxed(rho, bxs, 0))
+    runs = []
+    for graphs in (True, True, True, False):
+        P.set_graphs(graphs)
+        vel = ctx.array(vel0.n)
+        vel.copy_from(vel0)
+        gp = ctx.array(B.field_size(bxs, 0, 3))
+        pn = ctx.array(B.field_size(bxs, 0, 1, True))
+        info = P.apply_nodal_projection(vel, 1, gp, 0, pn, 0, dt, density=rd, rtol=1e-10, atol=0.0,
+                                        mode=mgx.MODE_LEARNED)
+        assert info.converged and info.fallback_used == 0
+        runs.append((info.vcycles, pn.download(), vel.download()))
+    P.close()
+    for vc, ph, ve in runs[1:]:
+        assert vc == runs[0][0]
+        assert np.array_equal(ph, runs[0][1]) and np.array_equal(ve, runs[0][2])
