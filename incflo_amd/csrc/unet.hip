@@ -337,12 +337,8 @@ void UNet::forward_body(const float* in, const NetShape& sh, float* out) {
         const ConvLayer& Lb = conv[li++];
         const int cl = base << l, ctot = cur_c + cl;
         float* cat = A(si);
-        const long nup = nvox_at(sh, l) * cur_c;
-        upsample_fwd_kernel<<<g1(nup), 256, 0, stream>>>(cur, sh.nx >> (l + 1), sh.ny >> (l + 1),
-                                                         sh.nz >> (l + 1), cur_c, sh.batch, cat, ctot, 0);
-        const long nsk = nvox_at(sh, l) * cl;
-        channel_copy_kernel<<<g1(nsk), 256, 0, stream>>>(A(3 * l + 1), cl, 0, cat, ctot, cur_c, cl,
-                                                         nvox_at(sh, l), 0);
+        upsample_concat_kernel<<<g1(nvox_at(sh, l) * ctot), 256, 0, stream>>>(
+            cur, sh.nx >> (l + 1), sh.ny >> (l + 1), sh.nz >> (l + 1), cur_c, A(3 * l + 1), cl, sh.batch, cat);
         conv_fwd(La, cat, l, A(si + 1), true, wb + La.wb_off, P + La.boff);
         conv_fwd(Lb, A(si + 1), l, A(si + 2), true, wb + Lb.wb_off, P + Lb.boff);
         cur = A(si + 2);
