@@ -61,6 +61,9 @@ enum { AM_RESID = 0, AM_GSRB = 1 };
 //   h/2) or Neumann.  x NULL reads 0.
 // RESID: r = b - L x on the unknowns (0 elsewhere), ||r||_inf into norm.
 // GSRB:  one colour of red-black Gauss-Seidel on x, in place.
+// tl: the 16 x 16-cell blocks of one plane that hold cells of fm (levels
+// above 0; no other cell is visited, and none of them is read by the passes
+// of that level), NULL: every cell on the dense 64 x 4 grid (level 0)
 // FM / CV: the level has a region mask / a finer level (compile-time, so that
 // level 0 and the finest level carry no mask tests)
 template <class T, class TX, class TO, int MODE, bool FM, bool CV>
@@ -72,11 +75,12 @@ static __global__ void amac_level_kernel(MGeomK g, MGeomK gc, MGeomK gf, AmacCoe
                                          const T* __restrict__ bfy, const T* __restrict__ bfz, TX* x,
                                          const TX* __restrict__ xc, const TX* __restrict__ xf,
                                          const T* __restrict__ b, TO* __restrict__ r,
-                                         unsigned long long* norm, int color) {
+                                         unsigned long long* norm, int color,
+                                         const int4* __restrict__ tl) {
     __shared__ double red[4];
     int i, j, k;
     double m = 0.0;
-    if (grid_point(g.n[0], g.n[1], i, j, k)) {
+    if (grid_point_tl(tl, g.n[0], g.n[1], i, j, k)) {
         const long id = cidx(g, i, j, k);
         const bool unk = (!FM || fm[id]) && !(CV && comp && cov[id]);
         const bool mine = MODE != AM_GSRB || (((i + j + k + color) & 1) == 0);
@@ -177,14 +181,34 @@ static __global__ void amac_restrict_kernel(MGeomK gc, MGeomK gf, const unsigned
     out[id] = (T)0.125 * s;
 }
 
-// e_f += e_c (parent) on F
+// e_f += e_c (parent) on F (tl: F's blocks, as for amac_level_kernel)
 template <class T>
 static __global__ void amac_prolong_kernel(MGeomK gf, MGeomK gc, const unsigned char* __restrict__ fm,
-                                           const T* __restrict__ ec, T* __restrict__ ef) {
+                                           const T* __restrict__ ec, T* __restrict__ ef,
+                                           const int4* __restrict__ tl) {
     int i, j, k;
-    if (!grid_point(gf.n[0], gf.n[1], i, j, k)) return;
+    if (!grid_point_tl(tl, gf.n[0], gf.n[1], i, j, k)) return;
     const long id = cidx(gf, i, j, k);
     if (fm[id]) ef[id] += ec[cidx(gc, i >> 1, j >> 1, k >> 1)];
+}
+
+// x += e on the unknowns of a level above 0, over its blocks (tl)
+template <class T>
+static __global__ void amac_add_tl_kernel(MGeomK g, const unsigned char* __restrict__ unk,
+                                          const T* __restrict__ e, double* __restrict__ x,
+                                          const int4* __restrict__ tl) {
+    int i, j, k;
+    if (!grid_point_tl(tl, g.n[0], g.n[1], i, j, k)) return;
+    const long id = cidx(g, i, j, k);
+    if (unk[id]) x[id] += (double)e[id];
+}
+
+// e = 0 on a level's blocks (tl); cells outside them are never written
+template <class T>
+static __global__ void amac_zero_tl_kernel(MGeomK g, T* __restrict__ e, const int4* __restrict__ tl) {
+    int i, j, k;
+    if (!grid_point_tl(tl, g.n[0], g.n[1], i, j, k)) return;
+    e[cidx(g, i, j, k)] = (T)0;
 }
 
 // x += e where mask == want
@@ -336,6 +360,7 @@ struct AmacLev {
     DevBuf beta[3], beta32[3];
     DevBuf x, b, r, r2, e;           // FP64 iterate, right-hand side, V-cycle work (FP64-sized)
     DevBuf R;                        // rhs of the level's correction (l >= 1; FP64-sized)
+    TileSet tiles;                   // blocks holding cells of the level (l >= 1)
     long nunk = 0;
     template <class T> const T* betaT(int d) const {
         return std::is_same<T, float>::value ? (const T*)beta32[d].p : (const T*)beta[d].p;
@@ -440,6 +465,7 @@ class AmrMac {
             if (l > 0) {
                 v.fm.alloc(fm[l].size());
                 MGX_HIP(hipMemcpy(v.fm.p, fm[l].data(), fm[l].size(), hipMemcpyHostToDevice));
+                v.tiles.build(g.n, fm[l]);
             }
             if (l < L) {
                 v.cov.alloc(cov[l].size());
@@ -511,13 +537,22 @@ class AmrMac {
         const T* bf[3] = {nullptr, nullptr, nullptr};
         if (l < L())
             for (int d = 0; d < 3; ++d) bf[d] = at(l + 1).betaT<T>(d);
-        const dim3 gr = grid3(v.g.n[0], v.g.n[1], v.g.n[2]);
+        dim3 gr = grid3(v.g.n[0], v.g.n[1], v.g.n[2]), blk = block3();
+        const int4* tl = nullptr;
+        if (l > 0) {
+            // only the blocks that hold cells of the level
+            int nb = 0;
+            tl = v.tiles.blocks(nb);
+            if (nb == 0) return;
+            gr = dim3(nb);
+            blk = dim3(TileSet::kTX, TileSet::kTY);
+        }
         const unsigned char* fm = l > 0 ? v.fm.as<unsigned char>() : nullptr;
         const unsigned char* cv = l < L() ? v.cov.as<unsigned char>() : nullptr;
 #define MGX_AMAC_LAUNCH(FMB, CVB)                                                                 \
-    amac_level_kernel<T, TX, TO, MODE, FMB, CVB><<<gr, block3(), 0, s>>>(                         \
+    amac_level_kernel<T, TX, TO, MODE, FMB, CVB><<<gr, blk, 0, s>>>(                              \
         v.g.k(), kcs(l), kfn(l), coef<T>(l), fm, cv, comp, v.betaT<T>(0), v.betaT<T>(1),         \
-        v.betaT<T>(2), bf[0], bf[1], bf[2], x, xc, xf, b, r, norm, color)
+        v.betaT<T>(2), bf[0], bf[1], bf[2], x, xc, xf, b, r, norm, color, tl)
         if (fm && cv) MGX_AMAC_LAUNCH(true, true);
         else if (fm) MGX_AMAC_LAUNCH(true, false);
         else if (cv) MGX_AMAC_LAUNCH(false, true);
@@ -563,10 +598,15 @@ class AmrMac {
     void cycle_level(MacHierarchy<T>& H, int l, hipStream_t s) {
         AmacLev& v = at(l);
         AmacLev& c = at(l - 1);
-        const long n = v.g.ncell();
         T* e = v.e.as<T>();
         const T* R = l == L() ? v.r.as<T>() : v.R.as<T>();
-        zero_kernel<T><<<grid1d(n), 256, 0, s>>>(e, n);
+        {
+            // e is read on the level's cells only, and nothing writes it elsewhere
+            int nb = 0;
+            const int4* tl = v.tiles.blocks(nb);
+            if (nb > 0)
+                amac_zero_tl_kernel<T><<<nb, dim3(TileSet::kTX, TileSet::kTY), 0, s>>>(v.g.k(), e, tl);
+        }
         for (int q = 0; q < opts.nu1; ++q)
             for (int color = 0; color < 2; ++color)
                 level_pass<T, T, T, AM_GSRB>(l, 0, e, nullptr, nullptr, R, nullptr, nullptr, color, s);
@@ -583,8 +623,13 @@ class AmrMac {
             cycle_level<T>(H, l - 1, s);
             ec = c.e.as<T>();
         }
-        amac_prolong_kernel<T><<<grid3(v.g.n[0], v.g.n[1], v.g.n[2]), block3(), 0, s>>>(
-            v.g.k(), c.g.k(), v.fm.as<unsigned char>(), ec, e);
+        {
+            int nb = 0;
+            const int4* tl = v.tiles.blocks(nb);
+            if (nb > 0)
+                amac_prolong_kernel<T><<<nb, dim3(TileSet::kTX, TileSet::kTY), 0, s>>>(
+                    v.g.k(), c.g.k(), v.fm.as<unsigned char>(), ec, e, tl);
+        }
         for (int q = 0; q < opts.nu2; ++q)
             for (int color = 0; color < 2; ++color)
                 level_pass<T, T, T, AM_GSRB>(l, 0, e, ec, nullptr, R, nullptr, nullptr, color, s);
@@ -605,8 +650,16 @@ class AmrMac {
         }
         for (int l = 0; l <= L(); ++l) {
             const T* e = l == 0 ? H.L[0].x : at(l).e.as<T>();
-            amac_add_kernel<T><<<grid1d(at(l).g.ncell()), 256, 0, s>>>(at(l).unk.as<unsigned char>(), 1, e,
-                                                                        at(l).x.as<double>(), at(l).g.ncell());
+            if (l == 0) {
+                amac_add_kernel<T><<<grid1d(at(l).g.ncell()), 256, 0, s>>>(at(l).unk.as<unsigned char>(), 1, e,
+                                                                            at(l).x.as<double>(), at(l).g.ncell());
+            } else {
+                int nb = 0;
+                const int4* tl = at(l).tiles.blocks(nb);
+                if (nb > 0)
+                    amac_add_tl_kernel<T><<<nb, dim3(TileSet::kTX, TileSet::kTY), 0, s>>>(
+                        at(l).g.k(), at(l).unk.as<unsigned char>(), e, at(l).x.as<double>(), tl);
+            }
         }
     }
 

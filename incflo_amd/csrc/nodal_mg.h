@@ -119,6 +119,8 @@ class TileSet {
     // shape) per the tuning table's amr_tile_width(32), kKZ planes by default
     static constexpr int kTX = 16, kTY = 16, kKZ = 8;
     void build(const NodalGeom& g, const std::vector<unsigned char>& active);
+    // the same over any dims[0] x dims[1] x dims[2] array of points (e.g. cells)
+    void build(const int dims[3], const std::vector<unsigned char>& active);
     bool built() const { return nn_[0] > 0; }
     // stencil tiles of tx x ty nodes x kz planes on a gx x gy x gz tile grid
     // (linear indices; tx a multiple of 16); count 0: none

@@ -250,9 +250,12 @@ void launch_stencil(const NodalGeom& g, SArgs<T, TC> a, hipStream_t s, bool fine
         g, a, s, fine, stencil_shape(g, (int)sizeof(T), (int)sizeof(TC), MODE, fine));
 }
 
-void TileSet::build(const NodalGeom& g, const std::vector<unsigned char>& active) {
-    MGX_REQUIRE((long)active.size() == g.nnode(), MGX_ERR_ARG, "tile set: one flag per node");
-    for (int d = 0; d < 3; ++d) nn_[d] = g.nn[d];
+void TileSet::build(const NodalGeom& g, const std::vector<unsigned char>& active) { build(g.nn, active); }
+
+void TileSet::build(const int dims[3], const std::vector<unsigned char>& active) {
+    MGX_REQUIRE((long)active.size() == (long)dims[0] * dims[1] * dims[2], MGX_ERR_ARG,
+                "tile set: one flag per point");
+    for (int d = 0; d < 3; ++d) nn_[d] = dims[d];
     g16_ = (nn_[0] + 15) / 16;
     col_.assign((size_t)g16_ * nn_[1] * nn_[2], 0);
     for (int k = 0; k < nn_[2]; ++k)
