@@ -80,7 +80,28 @@ static __global__ void amac_level_kernel(MGeomK g, MGeomK gc, MGeomK gf, AmacCoe
     __shared__ double red[4];
     int i, j, k;
     double m = 0.0;
-    if (grid_point_tl(tl, g.n[0], g.n[1], i, j, k)) {
+    bool in;
+    if (MODE == AM_GSRB) {
+        // the threads cover the cells of one colour: a block of blockDim.x
+        // threads spans 2 blockDim.x cells of a row (launched with half the x
+        // threads of the other passes)
+        int i0;
+        if (tl) {
+            const int4 t = tl[blockIdx.x];
+            i0 = t.x * 2 * (int)blockDim.x;
+            j = t.y * (int)blockDim.y + threadIdx.y;
+            k = t.z;
+        } else {
+            i0 = blockIdx.x * 2 * (int)blockDim.x;
+            j = blockIdx.y * (int)blockDim.y + threadIdx.y;
+            k = blockIdx.z;
+        }
+        i = i0 + 2 * (int)threadIdx.x + ((j + k + color) & 1);  // i0 is even
+        in = i < g.n[0] && j < g.n[1];
+    } else {
+        in = grid_point_tl(tl, g.n[0], g.n[1], i, j, k);
+    }
+    if (in) {
         const long id = cidx(g, i, j, k);
         const bool unk = (!FM || fm[id]) && !(CV && comp && cov[id]);
         const bool mine = MODE != AM_GSRB || (((i + j + k + color) & 1) == 0);
@@ -547,6 +568,7 @@ class AmrMac {
             gr = dim3(nb);
             blk = dim3(TileSet::kTX, TileSet::kTY);
         }
+        if (MODE == AM_GSRB) blk.x /= 2;  // one thread per cell of the colour
         const unsigned char* fm = l > 0 ? v.fm.as<unsigned char>() : nullptr;
         const unsigned char* cv = l < L() ? v.cov.as<unsigned char>() : nullptr;
 #define MGX_AMAC_LAUNCH(FMB, CVB)                                                                 \
