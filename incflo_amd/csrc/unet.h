@@ -81,6 +81,7 @@ class UNet {
     hipStream_t cap_stream_ = nullptr;  // capture stream (the caller's may be the null stream)
     bool graphs_ok_ = true;
     void drop_graphs();
+    void reserve_grad_ws(const NetShape& sh);
     void forward_body(const float* in, const NetShape& sh, float* out);
     void build_acts(const NetShape& sh);
     int dims_at(int lvl, int d) const;

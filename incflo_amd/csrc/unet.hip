@@ -349,8 +349,30 @@ void UNet::forward_body(const float* in, const NetShape& sh, float* out) {
     head_fwd_kernel<<<g1(n0), 256, 0, stream>>>(cur, base, n0, P + head_w, P + head_b, out);
 }
 
+// Size the gradient workspace for every reduction of one backward pass and
+// the loss before any of them is launched (growing it later would free
+// memory that launched kernels may still use).
+void UNet::reserve_grad_ws(const NetShape& sh) {
+    size_t need = 1;
+    const int nl = (int)conv.size();
+    for (int li = 0; li < nl; ++li) {
+        // encoder level li / 2, bottleneck levels - 1, decoder from the coarsest
+        const int lvl = li < 2 * (levels - 1) ? li / 2
+                                              : (li < 2 * levels ? levels - 1 : levels - 2 - (li - 2 * levels) / 2);
+        const ConvLayer& L = conv[li];
+        const ConvGeom cg = geom_at(sh, lvl, L.cin, L.cout, L.kp);
+        need = std::max(need, conv_dw_workspace(cg));
+        need = std::max(need, (size_t)bias_grad_blocks(nvox_at(sh, lvl), L.cout) * L.cout);
+    }
+    const long n0 = nvox_at(sh, 0);
+    need = std::max(need, (size_t)std::min<long>(1024, g1(n0)) * (base + 1));  // head
+    need = std::max(need, (size_t)std::min<long>(1024, g1(n0)));             // loss
+    gws.alloc(need * sizeof(float));
+}
+
 void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
     MGX_REQUIRE(shape == sh && !act.empty(), MGX_ERR_STATE, "U-net backward without forward");
+    reserve_grad_ws(sh);
     const unsigned short* wt = wtbf.as<unsigned short>();
     float* G = grads.as<float>();
     const float* P = params.as<float>();
@@ -372,9 +394,7 @@ void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
         const long nv = nvox_at(sh, lvl);
         relu_bwd_kernel<<<g1(nv * L.cout), 256, 0, stream>>>(y, dy, nv * L.cout);
         ConvGeom cg = geom_at(sh, lvl, L.cin, L.cout, L.kp);
-        gws.alloc(conv_dw_workspace(cg) * sizeof(float));
         conv_dw_launch(cg, x, dy, G + L.woff, 27 * L.cin, gws.as<float>(), stream);
-        gws.alloc((size_t)bias_grad_blocks(nv, L.cout) * L.cout * sizeof(float));
         bias_grad_launch(dy, nv, L.cout, G + L.boff, gws.as<float>(), stream);
         if (dx) {
             ConvGeom ct = geom_at(sh, lvl, L.cout, L.cin, L.kpt);
@@ -388,7 +408,6 @@ void UNet::backward(const float* in, const float* dout, const NetShape& sh) {
     {
         // the head's dW (base values) and db follow each other in the parameter vector
         const unsigned hb = (unsigned)std::min<long>(1024, g1(n0));
-        gws.alloc((size_t)hb * (base + 1) * sizeof(float));
         head_bwd_kernel<<<hb, 256, 0, stream>>>(A(s0 + 2), base, n0, P + head_w, dout, GA(s0 + 2),
                                                 gws.as<float>());
         MGX_REQUIRE(head_b == head_w + base, MGX_ERR_STATE, "U-net: head parameter layout");
@@ -450,7 +469,7 @@ float UNet::train_step(const float* in, const float* target, const NetShape& sh,
     forward(in, sh, y);
     MGX_HIP(hipMemsetAsync(loss_buf.p, 0, sizeof(float), stream));
     const unsigned lb = (unsigned)std::min<long>(1024, g1(n0));
-    gws.alloc((size_t)lb * sizeof(float));
+    reserve_grad_ws(sh);
     mse_kernel<<<lb, 256, 0, stream>>>(y, 1, target, n0, dy, gws.as<float>());
     sum_parts_kernel<<<1, 256, 0, stream>>>(gws.as<float>(), (int)lb, 1, loss_buf.as<float>());
     zero_grads();
