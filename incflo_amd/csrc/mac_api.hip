@@ -284,6 +284,18 @@ MGX_API int mgx_mac_set_options(mgx_mac h, const mgx_mlmg_opts* o) {
     });
 }
 
+MGX_API int mgx_mac_set_graphs(mgx_mac h, int on) {
+    if (!h) return MGX_ERR_ARG;
+    return guarded(h->ctx, [&] {
+        h->solver.H64.graphs = on != 0;
+        h->solver.H32.graphs = on != 0;
+        if (!on) {
+            h->solver.H64.drop_graphs();
+            h->solver.H32.drop_graphs();
+        }
+    });
+}
+
 MGX_API int mgx_mac_set_beta(mgx_mac h, const double* bx, const double* by, const double* bz,
                              int ng) {
     if (!h) return MGX_ERR_ARG;

@@ -73,6 +73,16 @@ class MacHierarchy {
     // level-0 x holds an initial guess (the correction so far): the next cycle
     // smooths from it instead of from zero
     bool x0_guess = false;
+    // on (default): each V-cycle is captured once per (level-0 x and b,
+    // initial guess, options, bottom counter) as a HIP graph and replayed,
+    // which removes the host launch gaps between the many small kernels of
+    // the coarse levels; off: the same kernels launched one by one
+    bool graphs = true;
+    MacHierarchy() = default;
+    MacHierarchy(const MacHierarchy&) = delete;
+    MacHierarchy& operator=(const MacHierarchy&) = delete;
+    ~MacHierarchy();
+    void drop_graphs();
     void build(const MacGeom& g0, int nlev);
     // level-0 beta from FP64 face arrays, then coarsen; with Robin sides the
     // boundary planes of each level are then scaled by rscale(l, d, side)
@@ -83,6 +93,19 @@ class MacHierarchy {
     // take the mean of their 8 fine cells (needs g0.has_a)
     void set_acoef(const double* a0, hipStream_t s);
     void vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters);
+
+   private:
+    struct CycleGraph {
+        T* x0;
+        const T* b0;
+        bool guess0;
+        int* bottom_iters;
+        MGOpts o;
+        hipGraphExec_t exec;
+    };
+    std::vector<CycleGraph> cache_;
+    hipStream_t cap_stream_ = nullptr;  // capture stream (the caller's may be the null stream)
+    void vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_iters);
 };
 
 class MacSolver {

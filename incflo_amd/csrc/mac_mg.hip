@@ -14,7 +14,20 @@
 namespace mgx {
 
 template <class T>
+MacHierarchy<T>::~MacHierarchy() {
+    drop_graphs();
+    if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+}
+
+template <class T>
+void MacHierarchy<T>::drop_graphs() {
+    for (CycleGraph& c : cache_) (void)hipGraphExecDestroy(c.exec);
+    cache_.clear();
+}
+
+template <class T>
 void MacHierarchy<T>::build(const MacGeom& g0, int nlev) {
+    drop_graphs();  // they hold the old levels' addresses
     L.clear();
     L.resize(nlev);
     for (int l = 0; l < nlev; ++l) {
@@ -91,8 +104,66 @@ static void gsrb(MacLevel<T>& lv, hipStream_t s) {
             lv.kg, lv.c, lv.beta[0], lv.beta[1], lv.beta[2], lv.a, lv.b, lv.x, color);
 }
 
+// the options a V-cycle's launches depend on
+static bool same_cycle_opts(const MGOpts& a, const MGOpts& b) {
+    return a.nu1 == b.nu1 && a.nu2 == b.nu2 && a.bottom_maxiter == b.bottom_maxiter &&
+           a.bottom_rtol == b.bottom_rtol && a.bottom_solver == b.bottom_solver;
+}
+
 template <class T>
 void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) {
+    if (!graphs) {
+        vcycle_body(o, s, bottom_iters);
+        return;
+    }
+    const bool guess0 = x0_guess;
+    const CycleGraph* hit = nullptr;
+    for (const CycleGraph& c : cache_)
+        if (c.x0 == L[0].x && c.b0 == L[0].b && c.guess0 == guess0 && c.bottom_iters == bottom_iters &&
+            same_cycle_opts(c.o, o))
+            hit = &c;
+    if (!hit) {
+        if (cache_.size() >= 16) drop_graphs();
+        bool ok = true;
+        if (!cap_stream_ && hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess) {
+            cap_stream_ = nullptr;
+            ok = false;
+        }
+        if (ok && hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeRelaxed) != hipSuccess) ok = false;
+        if (ok) {
+            bool body_ok = true;
+            try {
+                vcycle_body(o, cap_stream_, bottom_iters);
+            } catch (...) {
+                body_ok = false;
+            }
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(cap_stream_, &g);
+            hipGraphExec_t exec = nullptr;
+            if (e == hipSuccess && body_ok) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e == hipSuccess && body_ok) {
+                cache_.push_back(CycleGraph{L[0].x, L[0].b, guess0, bottom_iters, o, exec});
+                hit = &cache_.back();
+            } else {
+                ok = false;
+            }
+        }
+        if (!ok) {
+            // capture not available here: plain launches from now on
+            (void)hipGetLastError();
+            graphs = false;
+            x0_guess = guess0;
+            vcycle_body(o, s, bottom_iters);
+            return;
+        }
+    }
+    MGX_HIP(hipGraphLaunch(hit->exec, s));
+    x0_guess = false;
+}
+
+template <class T>
+void MacHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_iters) {
     const int nl = (int)L.size();
     for (int l = 0; l < nl - 1; ++l) {
         MacLevel<T>& lv = L[l];

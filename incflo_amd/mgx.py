@@ -102,6 +102,7 @@ def lib():
             "mgx_mac_destroy": [vp],
             "mgx_mac_set_domain_bc": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
             "mgx_mac_set_options": [vp, C.POINTER(MLMGOpts)],
+            "mgx_mac_set_graphs": [vp, C.c_int],
             "mgx_mac_set_beta": [vp, vp, vp, vp, C.c_int],
             "mgx_mac_set_const_beta": [vp, C.c_double],
             "mgx_mac_set_beta_from_density": [vp, vp, C.c_int, C.c_double],
@@ -473,6 +474,9 @@ class MacProjector:
 
     def set_options(self, opts):
         self.ctx.check(lib().mgx_mac_set_options(self.h, C.byref(opts)))
+
+    def set_graphs(self, on):
+        self.ctx.check(lib().mgx_mac_set_graphs(self.h, int(on)))
 
     def set_beta(self, bx, by, bz, ng=0):
         self.ctx.check(lib().mgx_mac_set_beta(self.h, bx.ptr, by.ptr, bz.ptr, ng))

@@ -422,6 +422,11 @@ MGX_API int mgx_mac_set_level_bc(mgx_mac h, const double* robin_a, const double*
                                  const double* robin_f, int ng);
 /* mac_proj.* knobs (maxiter / bottom_maxiter default 200). */
 MGX_API int mgx_mac_set_options(mgx_mac h, const mgx_mlmg_opts* opts);
+/* Launch mode (no reference counterpart; for callers that must not use HIP
+ * graphs, and for tests): on (default), each V-cycle of the cell-centred
+ * multigrid is captured once as a HIP graph and replayed; off, its kernels
+ * are launched one by one.  The kernels and results are the same either way. */
+MGX_API int mgx_mac_set_graphs(mgx_mac h, int on);
 /* initProjector(lp_info, beta) / updateCoeffs(beta) (:112, :131): face coefficient
  * fields (boxed face layout, ng ghosts). */
 MGX_API int mgx_mac_set_beta(mgx_mac h, const double* bx, const double* by, const double* bz,
