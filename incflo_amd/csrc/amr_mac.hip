@@ -224,6 +224,42 @@ static __global__ void amac_add_tl_kernel(MGeomK g, const unsigned char* __restr
     if (unk[id]) x[id] += (double)e[id];
 }
 
+// part[blockIdx.x] = sum of v over the unknowns (unk) of a level above 0,
+// its blocks (tl, nb of them) taken in a grid-stride loop; the partials are
+// those of amr_sum_kernel (one per workgroup, fixed order)
+static __global__ void __launch_bounds__(256) amac_sum_tl_kernel(MGeomK g, const double* __restrict__ v,
+                                                                 const unsigned char* __restrict__ unk,
+                                                                 const int4* __restrict__ tl, int nb,
+                                                                 double* __restrict__ part) {
+    __shared__ double red[256];
+    const int tid = threadIdx.x + 16 * threadIdx.y;
+    double s = 0.0;
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int4 t = tl[b];
+        const int i = t.x * 16 + threadIdx.x, j = t.y * 16 + threadIdx.y;
+        if (i < g.n[0] && j < g.n[1]) {
+            const long id = cidx(g, i, j, t.z);
+            if (unk[id]) s += v[id];
+        }
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+// v -= f mu[0] on the unknowns of a level above 0, over its blocks (tl)
+static __global__ void amac_shift_tl_kernel(MGeomK g, double* __restrict__ v, const unsigned char* __restrict__ unk,
+                                            const double* __restrict__ mu, double f, const int4* __restrict__ tl) {
+    int i, j, k;
+    if (!grid_point_tl(tl, g.n[0], g.n[1], i, j, k)) return;
+    const long id = cidx(g, i, j, k);
+    if (unk[id]) v[id] -= f * mu[0];
+}
+
 // e = 0 on a level's blocks (tl); cells outside them are never written
 template <class T>
 static __global__ void amac_zero_tl_kernel(MGeomK g, T* __restrict__ e, const int4* __restrict__ tl) {
@@ -586,14 +622,31 @@ class AmrMac {
     // the mean lands in scal); vals[l] a per-level canonical cell array
     void remove_mean(const std::vector<double*>& vals, int slot, hipStream_t s) {
         const int np = 512;
-        for (int l = 0; l <= L(); ++l)
-            amr_sum_kernel<<<np, 256, 0, s>>>(vals[l], at(l).unk.as<unsigned char>(), 1, at(l).g.ncell(),
-                                              part.as<double>() + (long)l * np);
+        // levels above 0: over their blocks only (their unknowns lie in them)
+        for (int l = 0; l <= L(); ++l) {
+            int nb = 0;
+            const int4* tl = l > 0 ? at(l).tiles.blocks(nb) : nullptr;
+            if (l > 0)
+                amac_sum_tl_kernel<<<np, dim3(TileSet::kTX, TileSet::kTY), 0, s>>>(
+                    at(l).g.k(), vals[l], at(l).unk.as<unsigned char>(), tl, nb, part.as<double>() + (long)l * np);
+            else
+                amr_sum_kernel<<<np, 256, 0, s>>>(vals[l], at(l).unk.as<unsigned char>(), 1, at(l).g.ncell(),
+                                                  part.as<double>() + (long)l * np);
+        }
         amr_mean_levels_kernel<<<1, 64, 0, s>>>(part.as<double>(), np, L() + 1, ndof, scal.as<double>() + slot);
-        for (int l = 0; l <= L(); ++l)
-            amr_shift_kernel<<<grid1d(at(l).g.ncell()), 256, 0, s>>>(vals[l], at(l).unk.as<unsigned char>(), 1, 0,
-                                                                     at(l).g.ncell(), scal.as<double>() + slot,
-                                                                     1.0);
+        for (int l = 0; l <= L(); ++l) {
+            if (l > 0) {
+                int nb = 0;
+                const int4* tl = at(l).tiles.blocks(nb);
+                if (nb > 0)
+                    amac_shift_tl_kernel<<<nb, dim3(TileSet::kTX, TileSet::kTY), 0, s>>>(
+                        at(l).g.k(), vals[l], at(l).unk.as<unsigned char>(), scal.as<double>() + slot, 1.0, tl);
+            } else {
+                amr_shift_kernel<<<grid1d(at(l).g.ncell()), 256, 0, s>>>(vals[l], at(l).unk.as<unsigned char>(), 1,
+                                                                         0, at(l).g.ncell(),
+                                                                         scal.as<double>() + slot, 1.0);
+            }
+        }
     }
 
     // FP64 composite residual of the levels' x into their r (stored as TO); ||r||_inf
