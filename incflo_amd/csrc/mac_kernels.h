@@ -168,6 +168,100 @@ static __global__ void mac_prolong_add_kernel(MGeomK gf, MGeomK gc, const T* __r
     xf[cidx(gf, i, j, k)] += xc[cidx(gc, i >> 1, j >> 1, k >> 1)];
 }
 
+// The coarse levels of a V-cycle in one workgroup (each level at most a few
+// thousand cells, where separate launches would each cost more than their
+// work): L[0..nlev-1] are smoothed, L[nlev] is the level below them (the
+// bottom).  Down (up == 0): for each level x = 0, nu1 x (red, black)
+// Gauss-Seidel, r = b - L x, then the next level's b = the mean of 8 r.  Up:
+// from the deepest, x += the parent's x, nu2 x (red, black).  The per-cell
+// arithmetic is that of mac_gsrb_kernel, mac_resid_kernel,
+// mac_restrict_kernel and mac_prolong_add_kernel, with a workgroup barrier
+// between passes, so the results are the same bit for bit.
+constexpr int kMacCoarseMax = 8;
+
+template <class T>
+struct MacCoarseLevel {
+    MGeomK g;
+    MCoef<T> c;
+    const T* beta[3];
+    const T* a;
+    T* b;
+    T* x;
+    T* r;
+};
+
+template <class T>
+struct MacCoarseArgs {
+    MacCoarseLevel<T> L[kMacCoarseMax + 1];
+    int nlev, nu1, nu2, up;
+};
+
+template <class T>
+__device__ __forceinline__ void mac_coarse_gs(const MacCoarseLevel<T>& v, int color) {
+    const int hx = (v.g.n[0] + 1) / 2;
+    const int nh = hx * v.g.n[1] * v.g.n[2];
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+        const int ih = h % hx, j = (h / hx) % v.g.n[1], k = h / (hx * v.g.n[1]);
+        const int i = 2 * ih + ((j + k + color) & 1);
+        if (i >= v.g.n[0]) continue;
+        T dg;
+        const T acc = mac_apply_cell<T, T, T>(v.g, v.c, v.beta[0], v.beta[1], v.beta[2], v.a, v.x, i, j, k, dg);
+        const long id = cidx(v.g, i, j, k);
+        if (dg != (T)0) v.x[id] += (v.b[id] - acc) / dg;
+    }
+    __syncthreads();
+}
+
+template <class T>
+static __global__ void __launch_bounds__(1024) mac_coarse_cycle_kernel(MacCoarseArgs<T> a) {
+    if (!a.up) {
+        for (int q = 0; q < a.nlev; ++q) {
+            const MacCoarseLevel<T>& v = a.L[q];
+            const int n = v.g.n[0] * v.g.n[1] * v.g.n[2];
+            for (int id = threadIdx.x; id < n; id += blockDim.x) v.x[id] = (T)0;
+            __syncthreads();
+            for (int s = 0; s < a.nu1; ++s) {
+                mac_coarse_gs(v, 0);
+                mac_coarse_gs(v, 1);
+            }
+            for (int id = threadIdx.x; id < n; id += blockDim.x) {
+                const int i = id % v.g.n[0], j = (id / v.g.n[0]) % v.g.n[1], k = id / (v.g.n[0] * v.g.n[1]);
+                T dg;
+                const T acc = mac_apply_cell<T, T, T>(v.g, v.c, v.beta[0], v.beta[1], v.beta[2], v.a, v.x, i, j, k, dg);
+                const long c = cidx(v.g, i, j, k);
+                v.r[c] = v.b[c] - acc;
+            }
+            __syncthreads();
+            const MacCoarseLevel<T>& w = a.L[q + 1];
+            const int nc = w.g.n[0] * w.g.n[1] * w.g.n[2];
+            for (int id = threadIdx.x; id < nc; id += blockDim.x) {
+                const int I = id % w.g.n[0], J = (id / w.g.n[0]) % w.g.n[1], K = id / (w.g.n[0] * w.g.n[1]);
+                T s = 0;
+#pragma unroll
+                for (int q8 = 0; q8 < 8; ++q8)
+                    s += v.r[cidx(v.g, 2 * I + (q8 & 1), 2 * J + ((q8 >> 1) & 1), 2 * K + ((q8 >> 2) & 1))];
+                w.b[cidx(w.g, I, J, K)] = (T)0.125 * s;
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int q = a.nlev - 1; q >= 0; --q) {
+            const MacCoarseLevel<T>& v = a.L[q];
+            const MacCoarseLevel<T>& w = a.L[q + 1];
+            const int n = v.g.n[0] * v.g.n[1] * v.g.n[2];
+            for (int id = threadIdx.x; id < n; id += blockDim.x) {
+                const int i = id % v.g.n[0], j = (id / v.g.n[0]) % v.g.n[1], k = id / (v.g.n[0] * v.g.n[1]);
+                v.x[cidx(v.g, i, j, k)] += w.x[cidx(w.g, i >> 1, j >> 1, k >> 1)];
+            }
+            __syncthreads();
+            for (int s = 0; s < a.nu2; ++s) {
+                mac_coarse_gs(v, 0);
+                mac_coarse_gs(v, 1);
+            }
+        }
+    }
+}
+
 // coarse face beta = mean of the 4 fine faces covering it
 template <class T, class TF>
 static __global__ void mac_beta_coarsen_kernel(MGeomK gf, MGeomK gc, int d,

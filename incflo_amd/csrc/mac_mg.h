@@ -100,6 +100,7 @@ class MacHierarchy {
         const T* b0;
         bool guess0;
         int* bottom_iters;
+        long coarse;  // mac_coarse_cells() at capture
         MGOpts o;
         hipGraphExec_t exec;
     };

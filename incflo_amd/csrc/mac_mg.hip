@@ -117,10 +117,11 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
         return;
     }
     const bool guess0 = x0_guess;
+    const long coarse = mac_coarse_cells();
     const CycleGraph* hit = nullptr;
     for (const CycleGraph& c : cache_)
         if (c.x0 == L[0].x && c.b0 == L[0].b && c.guess0 == guess0 && c.bottom_iters == bottom_iters &&
-            same_cycle_opts(c.o, o))
+            c.coarse == coarse && same_cycle_opts(c.o, o))
             hit = &c;
     if (!hit) {
         if (cache_.size() >= 16) drop_graphs();
@@ -143,7 +144,7 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
             if (e == hipSuccess && body_ok) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
             if (g) (void)hipGraphDestroy(g);
             if (e == hipSuccess && body_ok) {
-                cache_.push_back(CycleGraph{L[0].x, L[0].b, guess0, bottom_iters, o, exec});
+                cache_.push_back(CycleGraph{L[0].x, L[0].b, guess0, bottom_iters, coarse, o, exec});
                 hit = &cache_.back();
             } else {
                 ok = false;
@@ -162,10 +163,45 @@ void MacHierarchy<T>::vcycle(const MGOpts& o, hipStream_t s, int* bottom_iters) 
     x0_guess = false;
 }
 
+static_assert(sizeof(MacCoarseArgs<double>) <= 3072, "coarse-cycle kernel arguments");
+
+// the first level of the one-workgroup coarse cycle (nl - 1: none)
+template <class T>
+static int mac_coarse_first(const std::vector<MacLevel<T>>& L) {
+    const int nl = (int)L.size();
+    const long maxc = mac_coarse_cells();
+    int lc = nl - 1;
+    while (lc - 1 >= 1 && L[lc - 1].g.ncell() <= maxc && nl - 1 - (lc - 1) <= kMacCoarseMax) --lc;
+    return lc;
+}
+
+template <class T>
+static MacCoarseArgs<T> mac_coarse_args(std::vector<MacLevel<T>>& L, int lc, const MGOpts& o, int up) {
+    MacCoarseArgs<T> a{};
+    const int nl = (int)L.size();
+    a.nlev = nl - 1 - lc;
+    a.nu1 = o.nu1;
+    a.nu2 = o.nu2;
+    a.up = up;
+    for (int q = 0; q <= a.nlev; ++q) {
+        MacLevel<T>& v = L[lc + q];
+        a.L[q].g = v.kg;
+        a.L[q].c = v.c;
+        for (int d = 0; d < 3; ++d) a.L[q].beta[d] = v.beta[d];
+        a.L[q].a = v.a;
+        a.L[q].b = v.b;
+        a.L[q].x = v.x;
+        a.L[q].r = v.r;
+    }
+    return a;
+}
+
 template <class T>
 void MacHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_iters) {
     const int nl = (int)L.size();
-    for (int l = 0; l < nl - 1; ++l) {
+    // levels lc..nl-2 in one workgroup (lc == nl - 1: none)
+    const int lc = mac_coarse_first(L);
+    for (int l = 0; l < lc; ++l) {
         MacLevel<T>& lv = L[l];
         const long n = lv.g.ncell();
         if (l > 0 || !x0_guess) zero_kernel<T><<<grid1d(n), 256, 0, s>>>(lv.x, n);
@@ -177,6 +213,7 @@ void MacHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_it
         mac_restrict_kernel<T><<<grid3(cv.g.n[0], cv.g.n[1], cv.g.n[2]), block3(), 0, s>>>(
             lv.kg, cv.kg, lv.r, cv.b);
     }
+    if (lc < nl - 1) mac_coarse_cycle_kernel<T><<<1, 1024, 0, s>>>(mac_coarse_args(L, lc, o, 0));
     {
         MacLevel<T>& lb = L[nl - 1];
         const long nb = lb.g.ncell();
@@ -189,7 +226,8 @@ void MacHierarchy<T>::vcycle_body(const MGOpts& o, hipStream_t s, int* bottom_it
                                                    bottom_iters);
     }
     x0_guess = false;
-    for (int l = nl - 2; l >= 0; --l) {
+    if (lc < nl - 1) mac_coarse_cycle_kernel<T><<<1, 1024, 0, s>>>(mac_coarse_args(L, lc, o, 1));
+    for (int l = lc - 1; l >= 0; --l) {
         MacLevel<T>& lv = L[l];
         mac_prolong_add_kernel<T><<<grid3(lv.g.n[0], lv.g.n[1], lv.g.n[2]), block3(), 0, s>>>(
             lv.kg, L[l + 1].kg, L[l + 1].x, lv.x);

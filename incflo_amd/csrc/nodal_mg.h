@@ -137,6 +137,10 @@ class TileSet {
     int nblk_ = -1;
 };
 
+// cell-centred V-cycles: the most cells of a level below the first that the
+// one-workgroup coarse cycle takes (tuning table mac_coarse_cells; 0: none)
+long mac_coarse_cells();
+
 // Full weighting of a fine level's nodal array onto every node of the level
 // below it (dense, z-streamed as the V-cycle's restriction; Dirichlet nodes 0)
 template <class T>

@@ -78,11 +78,21 @@ static long coarse_threshold() {
 // at 256^3 the fused pass measured slower than two single sweeps (mixed
 // projection 8.85 against 6.85 ms)
 static int g_jacobi_pairs = 0;
+// cell-centred V-cycles: levels of at most this many cells (below the first
+// level) run in one workgroup (mac_coarse_cycle_kernel); 0: separate launches.
+// Taylor-Green 64^3 step: 6.00 ms with 0, 5.64 with 512, 5.84 with 4096,
+// 10.5 with 32768 (one workgroup is too slow for a 32^3 level)
+static long g_mac_coarse_cells = 512;
 // stencil tiles of launches over a TileSet: width (16 or 64 nodes) for FP64
 // and for FP32 storage, planes per tile (z-run)
 static int g_amr_tile_width = 16;
 static int g_amr_tile_width32 = 64;
 static int g_amr_tile_kz = TileSet::kKZ;
+
+long mac_coarse_cells() {
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    return g_mac_coarse_cells;
+}
 
 static bool jacobi_pairs_on() {
     std::lock_guard<std::mutex> lk(g_tune_mtx);
@@ -1147,6 +1157,7 @@ std::string tuning_table() {
     }
     o << "coarse_nodes " << g_coarse_nodes << "\n";
     o << "jacobi_pairs " << g_jacobi_pairs << "\n";
+    o << "mac_coarse_cells " << g_mac_coarse_cells << "\n";
     o << "amr_tile_width " << g_amr_tile_width << "\n";
     o << "amr_tile_width32 " << g_amr_tile_width32 << "\n";
     o << "amr_tile_kz " << g_amr_tile_kz << "\n";
@@ -1157,7 +1168,7 @@ void set_tuning_table(const std::string& text) {
     std::istringstream in(text);
     std::string tag;
     std::map<ShapeKey, StencilShape> shapes;
-    long coarse = 1;
+    long coarse = 1, mac_coarse = 512;
     int pairs = 0, tilew = 16, tilew32 = 64, tilekz = TileSet::kKZ;
     while (in >> tag) {
         if (tag == "stencil") {
@@ -1186,6 +1197,9 @@ void set_tuning_table(const std::string& text) {
             in >> tilekz;
             MGX_REQUIRE(!in.fail() && tilekz >= 2 && tilekz <= 64, MGX_ERR_ARG,
                         "tuning table: bad amr_tile_kz");
+        } else if (tag == "mac_coarse_cells") {
+            in >> mac_coarse;
+            MGX_REQUIRE(!in.fail() && mac_coarse >= 0, MGX_ERR_ARG, "tuning table: bad mac_coarse_cells");
         } else if (tag == "jacobi_pairs") {
             in >> pairs;
             MGX_REQUIRE(!in.fail() && (pairs == 0 || pairs == 1), MGX_ERR_ARG,
@@ -1197,6 +1211,7 @@ void set_tuning_table(const std::string& text) {
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     g_shapes = std::move(shapes);
     g_coarse_nodes = coarse;
+    g_mac_coarse_cells = mac_coarse;
     g_jacobi_pairs = pairs;
     g_amr_tile_width = tilew;
     g_amr_tile_width32 = tilew32;

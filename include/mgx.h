@@ -229,7 +229,9 @@ MGX_API int mgx_nodal_autotune(mgx_nodal h, int mode, char* report, int report_l
 /* The tuning table as text ("stencil nn0 nn1 nn2 word cword pass fine npt
  * target kzmin dma" lines, "coarse_nodes N", "jacobi_pairs 0|1" and the AMR
  * fine levels' stencil tiles "amr_tile_width 16|64" (FP64),
- * "amr_tile_width32 16|64" (FP32) and "amr_tile_kz K" planes), for caching
+ * "amr_tile_width32 16|64" (FP32) and "amr_tile_kz K" planes, and
+ * "mac_coarse_cells N": cell-centred multigrid levels below the first with
+ * at most N cells run in one workgroup per V-cycle half, 0: none), for caching
  * per device model; mgx_tuning_set loads such a text (replacing the table;
  * lines left out take their defaults). */
 MGX_API int mgx_tuning_get(char* buf, int len);

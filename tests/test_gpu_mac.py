@@ -309,3 +309,43 @@ def test_mac_graph_replay_matches_plain_launches(ctx, orc, mode):
         assert np.array_equal(phi, ref[1])
         for d in range(3):
             assert np.array_equal(us[d], ref[2][d])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mac_coarse_cycle_matches_separate_launches(ctx, orc, mode):
+    """The coarse levels of the cell-centred V-cycle run in one workgroup
+    (levels of at most mac_coarse_cells cells) give bit for bit the projection
+    of one launch per pass (mac_coarse_cells 0): same per-cell arithmetic, a
+    barrier between passes.  Bounded channel (inflow, Dirichlet) and the
+    periodic case, FP64 and mixed."""
+    from incflo_amd import mgx
+    base = mgx.tuning_get()
+    lines = [ln for ln in base.splitlines() if not ln.startswith("mac_coarse_cells")]
+    try:
+        for case in ("channel", "periodic"):
+            n, per, bclo, bchi, ph = CASES[case]
+            _say(f"case mac coarse cycle {case} mode={mode}")
+            g = orc.Geom.make(n, periodic=per, bclo=bclo, bchi=bchi, prob_hi=ph)
+            rng = np.random.default_rng(13)
+            rho = 1.0 + 0.5 * np.tanh(rng.standard_normal(g.cell_shape()))
+            u = _periodic_consistent(orc, g, [rng.standard_normal(orc.face_shape(g, d)) for d in range(3)])
+            if not per[0]:
+                u[0][:, :, 0] = 1.0
+            bxs = B.chop_domain(n, (16, 8, 12))
+            out = {}
+            for cells in (4096, 0):
+                mgx.tuning_set("\n".join(lines + [f"mac_coarse_cells {cells}"]) + "\n")
+                P = _mac(ctx, n, bxs, per, bclo, bchi, ph)
+                P.set_beta_from_density(ctx.array(B.to_boxed(rho, bxs, 1, periodic=per)), 1, 0.01)
+                ud = [ctx.array(B.faces_to_boxed(u[d], bxs, 0, d)) for d in range(3)]
+                phid = ctx.array(B.field_size(bxs, 1, 1))
+                info = P.project(ud[0], ud[1], ud[2], 0, 1e-11, 0.0, phi=phid, phi_ng=1, mode=mode)
+                assert info.converged
+                out[cells] = (info.iters, phid.download(), [a.download() for a in ud])
+                P.close()
+            assert out[4096][0] == out[0][0]
+            assert np.array_equal(out[4096][1], out[0][1])
+            for d in range(3):
+                assert np.array_equal(out[4096][2][d], out[0][2][d])
+    finally:
+        mgx.tuning_set(base)
